@@ -1,0 +1,244 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the CPU
+baseline leg of bench.py, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+HIT_DTYPE = np.dtype([("id", "<i4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
+BRUTE, KD_RESTART, SHORT_STACK = 0, 1, 2
+LOWER_BITMAP, LOWER_SAH = 0, 2
+
+
+class BuildParams(C.Structure):
+    _fields_ = [("lower_algorithm", C.c_int), ("empty_space_splitting", C.c_int),
+                ("empty_space_threshold", C.c_float), ("split_method", C.c_int),
+                ("propagate_aabbs", C.c_int), ("traversal_cost", C.c_float),
+                ("lower_size", C.c_int), ("max_upper_levels", C.c_int),
+                ("sah_leaf_max", C.c_int), ("sah_empty_bonus", C.c_float)]
+
+
+class TraceParams(C.Structure):
+    _fields_ = [("leaf_skipping", C.c_int), ("stack_size", C.c_int), ("brute_cap", C.c_int),
+                ("threads", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("num_nodes", C.c_int), ("reachable_nodes", C.c_int), ("inner_nodes", C.c_int),
+                ("leaves", C.c_int), ("empty_leaves", C.c_int), ("max_depth", C.c_int),
+                ("mean_leaf_depth", C.c_double), ("leaf_refs", C.c_longlong),
+                ("max_leaf_size", C.c_int), ("upper_levels", C.c_int), ("lower_levels", C.c_int),
+                ("num_prim_indices", C.c_int), ("num_upper_leaves", C.c_int)]
+
+
+def build_params(preset: str = "reference", **kw) -> BuildParams:
+    """'reference' = TriangleMap defaults with the SAH lower creator
+    (TriangleMap.cu:27-49, TriangleMapUpperCreator.cu:28-32);
+    'bitmap' = the reference default lower creator (BITMAP)."""
+    p = BuildParams(lower_algorithm=LOWER_SAH, empty_space_splitting=1,
+                    empty_space_threshold=0.35, split_method=0, propagate_aabbs=1,
+                    traversal_cost=24.0, lower_size=32, max_upper_levels=0,
+                    sah_leaf_max=0, sah_empty_bonus=1.0)
+    if preset == "bitmap":
+        p.lower_algorithm = LOWER_BITMAP
+    elif preset == "config":
+        p.sah_leaf_max = 8
+        p.sah_empty_bonus = 0.8
+    elif preset != "reference":
+        raise ValueError(preset)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def ensure_built() -> None:
+    if not os.path.exists(LIB_PATH):
+        subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        ensure_built()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.orc_build.argtypes = [P, P, P, C.c_int, C.POINTER(BuildParams), C.POINTER(P)]
+        L.orc_build.restype = C.c_int
+        L.orc_free.argtypes = [P]
+        L.orc_last_error.restype = C.c_char_p
+        for name in ("orc_num_nodes", "orc_num_prim_indices", "orc_num_leaf_ids",
+                     "orc_upper_levels", "orc_lower_levels"):
+            getattr(L, name).argtypes = [P]
+            getattr(L, name).restype = C.c_int
+        L.orc_upper_level_refs.argtypes = [P, C.c_int]
+        L.orc_upper_level_refs.restype = C.c_longlong
+        L.orc_lower_level_nodes.argtypes = [P, C.c_int]
+        L.orc_lower_level_nodes.restype = C.c_longlong
+        L.orc_copy_tree.argtypes = [P] * 12
+        L.orc_stats_of.argtypes = [P, C.POINTER(Stats)]
+        L.orc_stats_of.restype = C.c_int
+        L.orc_tree_from_arrays.argtypes = [C.c_int, P, P, P, P, P, P, P, C.c_int, P, C.POINTER(P)]
+        L.orc_tree_from_arrays.restype = C.c_int
+        L.orc_trace.argtypes = [P, P, P, P, C.c_int, C.c_int, P, P, C.c_int,
+                                C.POINTER(TraceParams), P, P]
+        L.orc_trace.restype = C.c_int
+        L.orc_mt32.argtypes = [P, P, P, P, P, C.c_float, P]
+        L.orc_mt32.restype = C.c_int
+        L.orc_mt64.argtypes = [P] * 6
+        L.orc_mt64_adjoint.argtypes = [P] * 5 + [C.c_double] * 3 + [P] * 5
+        L.orc_sah_node.argtypes = [C.c_int, P, P, P, C.c_longlong, C.c_float, C.c_float, C.c_int,
+                                   C.c_float, C.POINTER(C.c_int), C.POINTER(C.c_float),
+                                   C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                   C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.orc_sah_node.restype = C.c_int
+        L.orc_tri_area.argtypes = [P, P, P]
+        L.orc_tri_area.restype = C.c_float
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class Tree:
+    """Oracle tree; ``arrays()`` returns the reference SoA layout."""
+
+    def __init__(self, handle, keep=()):
+        self.h = handle
+        self._keep = keep
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_free(self.h)
+            self.h = None
+
+    @property
+    def num_nodes(self) -> int:
+        return lib().orc_num_nodes(self.h)
+
+    def arrays(self) -> dict:
+        L = lib()
+        n = L.orc_num_nodes(self.h)
+        npi = L.orc_num_prim_indices(self.h)
+        nl = L.orc_num_leaf_ids(self.h)
+        a = dict(info=np.zeros(n, np.int8), split_pos=np.zeros(n, np.float32),
+                 aabb_min=np.zeros((n, 4), np.float32), aabb_max=np.zeros((n, 4), np.float32),
+                 primitive_index=np.zeros(n, np.int32), primitive_bitmap=np.zeros(n, np.int64),
+                 children=np.zeros((n, 2), np.int32), surface_area=np.zeros(n, np.float32),
+                 parent=np.zeros(n, np.int32), prim_indices=np.zeros(npi, np.int32),
+                 leaf_ids=np.zeros(nl, np.int32))
+        L.orc_copy_tree(self.h, *[_ptr(a[k]) for k in (
+            "info", "split_pos", "aabb_min", "aabb_max", "primitive_index", "primitive_bitmap",
+            "children", "surface_area", "parent", "prim_indices", "leaf_ids")])
+        return a
+
+    def stats(self) -> dict:
+        s = Stats()
+        if lib().orc_stats_of(self.h, C.byref(s)) != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+        d = {k: getattr(s, k) for k, _ in Stats._fields_}
+        L = lib()
+        d["upper_level_refs"] = [L.orc_upper_level_refs(self.h, i) for i in range(s.upper_levels)]
+        d["lower_level_nodes"] = [L.orc_lower_level_nodes(self.h, i) for i in range(s.lower_levels)]
+        return d
+
+
+def build(tris, params: BuildParams | None = None) -> Tree:
+    p0, p1, p2 = (_c(t, np.float32) for t in tris)
+    n = p0.shape[0]
+    params = params or build_params()
+    h = C.c_void_p()
+    rc = lib().orc_build(_ptr(p0), _ptr(p1), _ptr(p2), n, C.byref(params), C.byref(h))
+    if rc != 0:
+        raise RuntimeError(f"orc_build failed ({rc}): {lib().orc_last_error().decode()}")
+    return Tree(h)
+
+
+def tree_from_arrays(a: dict) -> Tree:
+    keep = {k: np.ascontiguousarray(v) for k, v in a.items()}
+    h = C.c_void_p()
+    rc = lib().orc_tree_from_arrays(
+        keep["info"].shape[0], _ptr(_c(keep["info"], np.int8)), _ptr(_c(keep["split_pos"], np.float32)),
+        _ptr(_c(keep["aabb_min"], np.float32)), _ptr(_c(keep["aabb_max"], np.float32)),
+        _ptr(_c(keep["primitive_index"], np.int32)), _ptr(_c(keep["primitive_bitmap"], np.int64)),
+        _ptr(_c(keep["children"], np.int32)), keep["prim_indices"].shape[0],
+        _ptr(_c(keep["prim_indices"], np.int32)), C.byref(h))
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return Tree(h)
+
+
+def trace(tree: Tree | None, tris, org, dirs, algo: int, leaf_skipping=True, stack_size=4,
+          brute_cap=0, threads=0, counters=False):
+    p0, p1, p2 = (_c(t, np.float32) for t in tris)
+    org = _c(org, np.float32)
+    dirs = _c(dirs, np.float32)
+    n = org.shape[0]
+    out = np.zeros(n, HIT_DTYPE)
+    cnt = np.zeros((n, 4), np.int32) if counters else None
+    prm = TraceParams(int(bool(leaf_skipping)), stack_size, brute_cap, threads)
+    rc = lib().orc_trace(tree.h if tree is not None else None, _ptr(p0), _ptr(p1), _ptr(p2),
+                         p0.shape[0], algo, _ptr(org), _ptr(dirs), n, C.byref(prm),
+                         _ptr(out), _ptr(cnt))
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return (out, cnt) if counters else out
+
+
+def mt32(v0, v1, v2, o, d, t_max=2139095040.0):
+    a = [_c(x, np.float32) for x in (v0, v1, v2, o, d)]
+    r = np.zeros(3, np.float32)
+    hit = lib().orc_mt32(*[_ptr(x) for x in a], t_max, _ptr(r))
+    return bool(hit), r
+
+
+def mt64(v0, v1, v2, o, d):
+    a = [_c(x, np.float64) for x in (v0, v1, v2, o, d)]
+    r = np.zeros(4, np.float64)
+    lib().orc_mt64(*[_ptr(x) for x in a], _ptr(r))
+    return r
+
+
+def mt64_adjoint(v0, v1, v2, o, d, gt, gu, gv):
+    a = [_c(x, np.float64) for x in (v0, v1, v2, o, d)]
+    outs = [np.zeros(3, np.float64) for _ in range(5)]
+    lib().orc_mt64_adjoint(*[_ptr(x) for x in a], gt, gu, gv, *[_ptr(x) for x in outs])
+    return outs
+
+
+def sah_node(bmin, bmax, areas, bitmap, node_area, traversal_cost=24.0, leaf_max=0, bonus=1.0):
+    bmin = _c(bmin, np.float32)
+    bmax = _c(bmax, np.float32)
+    areas = _c(areas, np.float32)
+    n = bmin.shape[0]
+    ax, pos = C.c_int(), C.c_float()
+    l, r = C.c_longlong(), C.c_longlong()
+    la, ra, cost = C.c_float(), C.c_float(), C.c_float()
+    s = lib().orc_sah_node(n, _ptr(bmin), _ptr(bmax), _ptr(areas), int(bitmap), node_area,
+                           traversal_cost, leaf_max, bonus, C.byref(ax), C.byref(pos), C.byref(l),
+                           C.byref(r), C.byref(la), C.byref(ra), C.byref(cost))
+    return dict(split=bool(s), axis=ax.value, pos=pos.value, left=l.value & (2**64 - 1),
+                right=r.value & (2**64 - 1), left_area=la.value, right_area=ra.value,
+                cost=cost.value)
+
+
+def tri_area(v0, v1, v2) -> float:
+    a = [_c(x, np.float32) for x in (v0, v1, v2)]
+    return float(lib().orc_tri_area(*[_ptr(x) for x in a]))
