@@ -1,0 +1,267 @@
+// pm_api.hip -- the C ABI declared in include/pmkd.h.
+#include <algorithm>
+#include <vector>
+
+#include "pm_internal.h"
+
+using pmkd::Error;
+
+namespace {
+
+int fail(pm_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+template <class F>
+int guarded(pm_ctx *ctx, F &&f) {
+    if (!ctx) return PM_ERR_ARG;
+    try {
+        PM_HIP(hipSetDevice(ctx->device));
+        f();
+        ctx->err.clear();
+        return PM_OK;
+    } catch (const Error &e) {
+        return fail(ctx, e.code, e.msg);
+    } catch (const std::bad_alloc &) {
+        return fail(ctx, PM_ERR_HIP, "host allocation failed");
+    }
+}
+
+void check_build_params(const pm_build_params &p) {
+    if (p.split_method == PM_SPLIT_DIVIDE || p.split_method == PM_SPLIT_SPLIT)
+        throw Error{PM_ERR_NOT_IMPLEMENTED, "split method not implemented (only BOX)"};
+    if (p.split_method != PM_SPLIT_BOX) throw Error{PM_ERR_ARG, "unknown split method"};
+    if (p.lower_algorithm == PM_LOWER_BALANCED)
+        throw Error{PM_ERR_UNSUPPORTED, "BALANCED lower creator is not provided"};
+    if (p.lower_algorithm != PM_LOWER_BITMAP && p.lower_algorithm != PM_LOWER_SAH)
+        throw Error{PM_ERR_ARG, "unknown lower algorithm"};
+    if (p.lower_size < 1 || p.lower_size > 64) throw Error{PM_ERR_ARG, "lower_size must be in [1, 64]"};
+    if (p.max_upper_levels < 0) throw Error{PM_ERR_ARG, "max_upper_levels must be >= 0"};
+    if (p.max_ref_factor < 0) throw Error{PM_ERR_ARG, "max_ref_factor must be >= 0"};
+}
+
+}  // namespace
+
+extern "C" {
+
+void pm_build_params_default(pm_build_params *p) {
+    if (!p) return;
+    p->lower_algorithm = PM_LOWER_BITMAP;
+    p->empty_space_splitting = 1;
+    p->empty_space_threshold = 0.35f;
+    p->split_method = PM_SPLIT_BOX;
+    p->propagate_aabbs = 1;
+    p->traversal_cost = 24.0f;
+    p->lower_size = 32;
+    p->max_upper_levels = 0;
+    p->sah_leaf_max = 0;
+    p->sah_empty_bonus = 1.0f;
+    p->max_ref_factor = 0;
+}
+
+void pm_trace_params_default(pm_trace_params *p) {
+    if (!p) return;
+    p->leaf_skipping = 1;
+    p->stack_size = 4;
+    p->brute_cap = 0;
+    p->reserved = 0;
+}
+
+int pm_create(pm_ctx **out, int device) {
+    if (!out) return PM_ERR_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return PM_ERR_HIP;
+    if (device < 0 || device >= count) return PM_ERR_ARG;
+    pm_ctx *c = new (std::nothrow) pm_ctx();
+    if (!c) return PM_ERR_HIP;
+    c->device = device;
+    *out = c;
+    return PM_OK;
+}
+
+int pm_destroy(pm_ctx *ctx) {
+    if (!ctx) return PM_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    delete ctx;
+    return PM_OK;
+}
+
+const char *pm_last_error(const pm_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int pm_build(pm_ctx *ctx, const float *p0, const float *p1, const float *p2, int n,
+             const pm_build_params *params, void *stream) {
+    return guarded(ctx, [&] {
+        if (n < 0 || (n > 0 && (!p0 || !p1 || !p2))) throw Error{PM_ERR_ARG, "invalid geometry"};
+        pm_build_params prm;
+        if (params) prm = *params; else pm_build_params_default(&prm);
+        check_build_params(prm);
+        pmkd::build_tree(ctx, reinterpret_cast<const float4 *>(p0), reinterpret_cast<const float4 *>(p1),
+                         reinterpret_cast<const float4 *>(p2), n, prm, (hipStream_t)stream);
+    });
+}
+
+int pm_set_geometry(pm_ctx *ctx, const float *p0, const float *p1, const float *p2, int n) {
+    return guarded(ctx, [&] {
+        if (n < 0 || (n > 0 && (!p0 || !p1 || !p2))) throw Error{PM_ERR_ARG, "invalid geometry"};
+        ctx->p0 = reinterpret_cast<const float4 *>(p0);
+        ctx->p1 = reinterpret_cast<const float4 *>(p1);
+        ctx->p2 = reinterpret_cast<const float4 *>(p2);
+        ctx->ntris = n;
+    });
+}
+
+int pm_tree_view(pm_ctx *ctx, pm_tree *out) {
+    return guarded(ctx, [&] {
+        if (!out) throw Error{PM_ERR_ARG, "null output"};
+        if (!ctx->has_tree) throw Error{PM_ERR_NO_TREE, "no tree: call pm_build or pm_set_tree first"};
+        pm_tree &t = *out;
+        t.num_nodes = ctx->num_nodes;
+        t.info = ctx->nodes.info.ptr;
+        t.split_pos = ctx->nodes.split.ptr;
+        t.aabb_min = reinterpret_cast<const float *>(ctx->nodes.bmin.ptr);
+        t.aabb_max = reinterpret_cast<const float *>(ctx->nodes.bmax.ptr);
+        t.primitive_index = ctx->nodes.prim_index.ptr;
+        t.primitive_bitmap = ctx->nodes.bitmap.ptr;
+        t.children = reinterpret_cast<const int *>(ctx->nodes.children.ptr);
+        t.surface_area = ctx->nodes.area.ptr;
+        t.parent = ctx->nodes.parent.ptr;
+        t.num_prim_indices = ctx->num_prim_indices;
+        t.prim_indices = ctx->prim_indices.ptr;
+        t.num_leaf_ids = ctx->num_leaf_ids;
+        t.leaf_ids = ctx->leaf_ids.ptr;
+        t.p0 = reinterpret_cast<const float *>(ctx->p0);
+        t.p1 = reinterpret_cast<const float *>(ctx->p1);
+        t.p2 = reinterpret_cast<const float *>(ctx->p2);
+        t.num_tris = ctx->ntris;
+        t.upper_levels = ctx->upper_levels;
+        t.lower_levels = ctx->lower_levels;
+        t.has_boxes = ctx->has_boxes ? 1 : 0;
+    });
+}
+
+int pm_set_tree(pm_ctx *ctx, const pm_tree *h, void *stream) {
+    return guarded(ctx, [&] {
+        if (!h || h->num_nodes < 1 || !h->info || !h->split_pos || !h->children || !h->primitive_index ||
+            !h->primitive_bitmap || h->num_prim_indices < 0 || (h->num_prim_indices > 0 && !h->prim_indices))
+            throw Error{PM_ERR_ARG, "invalid host tree"};
+        if (h->num_tris < 0 || (h->num_tris > 0 && (!h->p0 || !h->p1 || !h->p2)))
+            throw Error{PM_ERR_ARG, "invalid geometry"};
+        hipStream_t s = (hipStream_t)stream;
+        const size_t n = (size_t)h->num_nodes;
+        pmkd::NodeArrays &a = ctx->nodes;
+        a.reserve(n, s, 0);
+        PM_HIP(hipMemcpyAsync(a.info.ptr, h->info, n, hipMemcpyHostToDevice, s));
+        PM_HIP(hipMemcpyAsync(a.split.ptr, h->split_pos, n * 4, hipMemcpyHostToDevice, s));
+        if (h->aabb_min && h->aabb_max) {
+            PM_HIP(hipMemcpyAsync(a.bmin.ptr, h->aabb_min, n * 16, hipMemcpyHostToDevice, s));
+            PM_HIP(hipMemcpyAsync(a.bmax.ptr, h->aabb_max, n * 16, hipMemcpyHostToDevice, s));
+        }
+        PM_HIP(hipMemcpyAsync(a.prim_index.ptr, h->primitive_index, n * 4, hipMemcpyHostToDevice, s));
+        PM_HIP(hipMemcpyAsync(a.bitmap.ptr, h->primitive_bitmap, n * 8, hipMemcpyHostToDevice, s));
+        PM_HIP(hipMemcpyAsync(a.children.ptr, h->children, n * 8, hipMemcpyHostToDevice, s));
+        ctx->prim_indices.reserve((size_t)h->num_prim_indices + 1, s);
+        if (h->num_prim_indices > 0)
+            PM_HIP(hipMemcpyAsync(ctx->prim_indices.ptr, h->prim_indices, (size_t)h->num_prim_indices * 4,
+                                  hipMemcpyHostToDevice, s));
+        PM_HIP(hipStreamSynchronize(s));
+        ctx->num_nodes = h->num_nodes;
+        ctx->num_prim_indices = h->num_prim_indices;
+        ctx->num_leaf_ids = 0;
+        ctx->upper_levels = h->upper_levels;
+        ctx->lower_levels = h->lower_levels;
+        ctx->has_boxes = h->aabb_min && h->aabb_max && h->has_boxes;
+        ctx->p0 = reinterpret_cast<const float4 *>(h->p0);
+        ctx->p1 = reinterpret_cast<const float4 *>(h->p1);
+        ctx->p2 = reinterpret_cast<const float4 *>(h->p2);
+        ctx->ntris = h->num_tris;
+        ctx->has_tree = true;
+    });
+}
+
+int pm_tree_stats(pm_ctx *ctx, pm_stats *out, void *stream) {
+    return guarded(ctx, [&] {
+        if (!out) throw Error{PM_ERR_ARG, "null output"};
+        if (!ctx->has_tree) throw Error{PM_ERR_NO_TREE, "no tree"};
+        hipStream_t s = (hipStream_t)stream;
+        const int n = ctx->num_nodes;
+        std::vector<int8_t> info(n);
+        std::vector<long long> bmp(n);
+        std::vector<int2> ch(n);
+        PM_HIP(hipMemcpyAsync(info.data(), ctx->nodes.info.ptr, n, hipMemcpyDeviceToHost, s));
+        PM_HIP(hipMemcpyAsync(bmp.data(), ctx->nodes.bitmap.ptr, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+        PM_HIP(hipMemcpyAsync(ch.data(), ctx->nodes.children.ptr, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+        PM_HIP(hipStreamSynchronize(s));
+        pm_stats &st = *out;
+        std::memset(&st, 0, sizeof(st));
+        st.num_nodes = n;
+        st.upper_levels = ctx->upper_levels;
+        st.lower_levels = ctx->lower_levels;
+        st.num_prim_indices = ctx->num_prim_indices;
+        st.num_upper_leaves = ctx->num_leaf_ids;
+        std::vector<std::pair<int, int>> stk{{0, 0}};
+        double depth_sum = 0.0;
+        long long guard = 0;
+        while (!stk.empty()) {
+            if (++guard > 4LL * n + 4) throw Error{PM_ERR_INTERNAL, "tree is not acyclic"};
+            const auto [node, depth] = stk.back();
+            stk.pop_back();
+            if (node < 0 || node >= n) throw Error{PM_ERR_INTERNAL, "child index out of range"};
+            st.reachable_nodes++;
+            st.max_depth = std::max(st.max_depth, depth);
+            if ((info[node] & 3) == 0) {
+                st.leaves++;
+                const int c = __builtin_popcountll((unsigned long long)bmp[node]);
+                if (c == 0) st.empty_leaves++;
+                else depth_sum += depth;
+                st.leaf_refs += c;
+                st.max_leaf_size = std::max(st.max_leaf_size, c);
+            } else {
+                st.inner_nodes++;
+                stk.push_back({ch[node].y, depth + 1});
+                stk.push_back({ch[node].x, depth + 1});
+            }
+        }
+        const int nonempty = st.leaves - st.empty_leaves;
+        st.mean_leaf_depth = nonempty ? depth_sum / nonempty : 0.0;
+    });
+}
+
+int pm_trace(pm_ctx *ctx, int algo, const float *org, const float *dir, int nrays, pm_hit *out,
+             const pm_trace_params *params, int *counters, void *stream) {
+    return guarded(ctx, [&] {
+        if (nrays < 0 || (nrays > 0 && (!org || !dir || !out))) throw Error{PM_ERR_ARG, "invalid rays"};
+        if (algo != PM_TRACE_BRUTE && algo != PM_TRACE_KD_RESTART && algo != PM_TRACE_SHORT_STACK)
+            throw Error{PM_ERR_ARG, "unknown trace algorithm"};
+        pm_trace_params prm;
+        if (params) prm = *params; else pm_trace_params_default(&prm);
+        if (prm.stack_size < 1 || prm.stack_size > 16) throw Error{PM_ERR_ARG, "stack_size must be in [1, 16]"};
+        if (algo == PM_TRACE_BRUTE) {
+            if (ctx->ntris > 0 && !ctx->p0) throw Error{PM_ERR_NO_TREE, "no geometry"};
+        } else if (!ctx->has_tree) {
+            throw Error{PM_ERR_NO_TREE, "no tree: call pm_build or pm_set_tree first"};
+        }
+        pmkd::trace_rays(ctx, algo, reinterpret_cast<const float4 *>(org), reinterpret_cast<const float4 *>(dir),
+                         nrays, out, prm, reinterpret_cast<int4 *>(counters), (hipStream_t)stream);
+    });
+}
+
+int pm_photon_bounce(pm_ctx *ctx, const float *org, const float *dir, const pm_hit *hits, int n,
+                     int absorb_from, unsigned long long seed, float offset, float *out_org,
+                     float *out_dir, int *out_src, int *out_count, int *n_out, void *stream) {
+    return guarded(ctx, [&] {
+        if (n < 0 || (n > 0 && (!org || !dir || !hits || !out_org || !out_dir)))
+            throw Error{PM_ERR_ARG, "invalid rays"};
+        if (!ctx->p0 && ctx->ntris > 0) throw Error{PM_ERR_NO_TREE, "no geometry"};
+        const int c = pmkd::photon_bounce(ctx, reinterpret_cast<const float4 *>(org),
+                                          reinterpret_cast<const float4 *>(dir), hits, n, absorb_from,
+                                          seed, offset, reinterpret_cast<float4 *>(out_org),
+                                          reinterpret_cast<float4 *>(out_dir), out_src, out_count,
+                                          (hipStream_t)stream);
+        if (n_out) *n_out = c;
+    });
+}
+
+}  // extern "C"

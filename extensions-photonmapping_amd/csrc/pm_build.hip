@@ -1,0 +1,924 @@
+// pm_build.hip -- kd-tree construction for gfx950.
+//
+// Upper (large node) stage, level synchronous like
+// TriangleMapUpperCreator::Create (TriangleMapUpperCreator.cu:110-183) and
+// producing the same node numbering, ref order and leaf order:
+//   - refs of an active node are cut into segments of SEG = 256 refs; one
+//     wave64 owns a segment (4 refs per lane, coalesced float4 loads);
+//   - node boxes are exact min/max reductions done with ordered-int atomics
+//     (order independent, so identical to any sequential reduction); the
+//     boxes of the next level are reduced inside the scatter kernel that
+//     writes the clipped refs, so refs are read once per level for the box;
+//   - the reference's 2R+1-element split/leaf scans
+//     (TriangleMapUpperCreator.cu:557,631) are replaced by scans over
+//     per-segment counts plus in-wave ballot ranks, which give the same
+//     addresses (stable partition: left copies of all refs, then right).
+// Lower stage: the SAH creator (TriangleMapSAHCreator.cu:53-246) with one
+// wave64 per node: candidate masks are 64-bit ballots, costs are summed by
+// one lane per candidate in ascending bit order, the winner is a wave
+// argmin with the reference's first-minimum tie rule.  The bitmap creator
+// (TriangleMapBitmapCreator.cu:32-65) is one thread per upper leaf.
+#include <algorithm>
+#include <climits>
+
+#include "pm_device.h"
+#include "pm_internal.h"
+
+namespace pmkd {
+
+namespace {
+
+constexpr int BS = 256;                 // threads per block, per-node kernels
+constexpr int WPB = 4;                  // waves per block, per-segment kernels
+constexpr int SCAN_T = 1024, SCAN_E = 4, SCAN_BLK = SCAN_T * SCAN_E;
+
+inline int blocks_for(long long n, int per) { return (int)((n + per - 1) / per); }
+
+// ------------------------------------------------------------------ scan
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(v, off, 64);
+        if (lane >= off) v += o;
+    }
+    return v;
+}
+
+// exclusive scan of in[0..n) with in[n] treated as 0, i.e. out[0..n]
+__global__ __launch_bounds__(SCAN_T) void scan_block_kernel(const int *__restrict__ in,
+                                                            int *__restrict__ out, int n,
+                                                            int *__restrict__ block_sums) {
+    __shared__ int wsum[SCAN_T / 64];
+    const long long base = (long long)blockIdx.x * SCAN_BLK + (long long)threadIdx.x * SCAN_E;
+    int v[SCAN_E];
+    int tsum = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_E; ++j) {
+        const long long k = base + j;
+        v[j] = k < n ? in[k] : 0;
+        tsum += v[j];
+    }
+    const int incl = wave_incl_scan(tsum);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    if (w == 0) {
+        int x = lane < SCAN_T / 64 ? wsum[lane] : 0;
+        x = wave_incl_scan(x);
+        if (lane < SCAN_T / 64) wsum[lane] = x;
+    }
+    __syncthreads();
+    int run = (w ? wsum[w - 1] : 0) + incl - tsum;
+#pragma unroll
+    for (int j = 0; j < SCAN_E; ++j) {
+        const long long k = base + j;
+        if (k <= n) out[k] = run;
+        run += v[j];
+    }
+    if (block_sums && threadIdx.x == SCAN_T - 1) block_sums[blockIdx.x] = wsum[SCAN_T / 64 - 1];
+}
+
+__global__ __launch_bounds__(SCAN_T) void scan_add_kernel(int *__restrict__ out, int n,
+                                                          const int *__restrict__ offs) {
+    const long long base = (long long)blockIdx.x * SCAN_BLK;
+    const int add = offs[blockIdx.x];
+    for (int j = threadIdx.x; j < SCAN_BLK; j += SCAN_T) {
+        const long long k = base + j;
+        if (k <= n) out[k] += add;
+    }
+}
+
+// copy up to 8 ints (given by device pointers) into dst, for one read-back
+struct Gather { const int *src[8]; int count; };
+__global__ void gather_kernel(Gather g, int *dst) {
+    const int i = threadIdx.x;
+    if (i < g.count) dst[i] = *g.src[i];
+}
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ unsigned long long lanemask_lt() {
+    const int lane = __lane_id();
+    return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+struct NodePtrs {
+    int8_t *info; float *split; float4 *bmin, *bmax; int *prim_index; long long *bitmap;
+    int2 *children; float *area; int *parent;
+};
+
+__device__ __forceinline__ void zero_node(const NodePtrs &N, int id) {
+    N.info[id] = LEAF; N.split[id] = 0.0f;
+    N.bmin[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    N.bmax[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    N.prim_index[id] = 0; N.bitmap[id] = 0;
+    N.children[id] = make_int2(0, 0);
+    N.area[id] = 0.0f; N.parent[id] = 0;
+}
+
+// ------------------------------------------------------------------ root
+__global__ void init_keys_kernel(int *keys, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) {
+        const int kmin = fkey(F_INF), kmax = fkey(-F_INF);   // AabbMemset, ReduceSegments.h:328-336
+        keys[6 * i + 0] = kmin; keys[6 * i + 1] = kmin; keys[6 * i + 2] = kmin;
+        keys[6 * i + 3] = kmax; keys[6 * i + 4] = kmax; keys[6 * i + 5] = kmax;
+    }
+}
+
+__global__ void init_root_kernel(NodePtrs N, int ntris) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {   // TriangleMapUpperCreator.cu:128-137
+        zero_node(N, 0);
+        N.bitmap[0] = ntris;
+    }
+}
+
+// CalcPrimitiveAabb (Kernels/TriangleUpper.h:10-28) + root box reduction
+__global__ __launch_bounds__(BS) void tri_box_kernel(const float4 *__restrict__ P0,
+                                                     const float4 *__restrict__ P1,
+                                                     const float4 *__restrict__ P2, int n,
+                                                     float4 *__restrict__ rmin,
+                                                     float4 *__restrict__ rmax,
+                                                     int *__restrict__ keys) {
+    __shared__ int red[6][BS / 64];
+    int k0 = fkey(F_INF), k1 = k0, k2 = k0, k3 = fkey(-F_INF), k4 = k3, k5 = k3;
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+        const float4 a = P0[i], b = P1[i], c = P2[i];
+        float mnx = a.x, mny = a.y, mnz = a.z, mxx = a.x, mxy = a.y, mxz = a.z;
+        mnx = omin(mnx, b.x); mny = omin(mny, b.y); mnz = omin(mnz, b.z);
+        mxx = omax(mxx, b.x); mxy = omax(mxy, b.y); mxz = omax(mxz, b.z);
+        mnx = omin(mnx, c.x); mny = omin(mny, c.y); mnz = omin(mnz, c.z);
+        mxx = omax(mxx, c.x); mxy = omax(mxy, c.y); mxz = omax(mxz, c.z);
+        rmin[i] = make_float4(mnx, mny, mnz, __int_as_float(i));
+        rmax[i] = make_float4(mxx, mxy, mxz, 0.0f);
+        k0 = min(k0, fkey(mnx)); k1 = min(k1, fkey(mny)); k2 = min(k2, fkey(mnz));
+        k3 = max(k3, fkey(mxx)); k4 = max(k4, fkey(mxy)); k5 = max(k5, fkey(mxz));
+    }
+    k0 = wave_min(k0); k1 = wave_min(k1); k2 = wave_min(k2);
+    k3 = wave_max(k3); k4 = wave_max(k4); k5 = wave_max(k5);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) { red[0][w] = k0; red[1][w] = k1; red[2][w] = k2; red[3][w] = k3; red[4][w] = k4; red[5][w] = k5; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int j = 1; j < BS / 64; ++j) {
+            k0 = min(k0, red[0][j]); k1 = min(k1, red[1][j]); k2 = min(k2, red[2][j]);
+            k3 = max(k3, red[3][j]); k4 = max(k4, red[4][j]); k5 = max(k5, red[5][j]);
+        }
+        atomicMin(keys + 0, k0); atomicMin(keys + 1, k1); atomicMin(keys + 2, k2);
+        atomicMax(keys + 3, k3); atomicMax(keys + 4, k4); atomicMax(keys + 5, k5);
+    }
+}
+
+// ------------------------------------------------------------------ level
+// ReduceAabb result -> node box, CalcEmptySpaceSplits
+// (EmptySpaceSplitting.h:10-70), CalcUpperNodeSplitInfo
+// (ReduceSegments.h:466-484), segments per node.
+__global__ __launch_bounds__(BS) void node_prepare_kernel(NodePtrs N, int active_index, int A,
+                                                          const int *__restrict__ keys,
+                                                          int empty_space, float thr,
+                                                          int *__restrict__ planes_out,
+                                                          int *__restrict__ req,
+                                                          int *__restrict__ nseg) {
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i >= A) return;
+    const int node = active_index + i;
+    const f3 rMin = {unkey(keys[6 * i + 0]), unkey(keys[6 * i + 1]), unkey(keys[6 * i + 2])};
+    const f3 rMax = {unkey(keys[6 * i + 3]), unkey(keys[6 * i + 4]), unkey(keys[6 * i + 5])};
+    int pl = 0;
+    if (empty_space && active_index != 0) {
+        const f3 pMin = mk3(N.bmin[node]), pMax = mk3(N.bmax[node]);
+        if (rMin.x - pMin.x > thr * (pMax.x - pMin.x)) pl |= 1;
+        if (rMin.y - pMin.y > thr * (pMax.y - pMin.y)) pl |= 2;
+        if (rMin.z - pMin.z > thr * (pMax.z - pMin.z)) pl |= 4;
+        if (pMax.x - rMax.x > thr * (pMax.x - pMin.x)) pl |= 8;
+        if (pMax.y - rMax.y > thr * (pMax.y - pMin.y)) pl |= 16;
+        if (pMax.z - rMax.z > thr * (pMax.z - pMin.z)) pl |= 32;
+    }
+    planes_out[i] = pl;
+    req[i] = 2 * __popc(pl);
+    N.bmin[node] = make_float4(rMin.x, rMin.y, rMin.z, 0.0f);
+    N.bmax[node] = make_float4(rMax.x, rMax.y, rMax.z, 0.0f);
+    const f3 sz = sub3(rMax, rMin);
+    const f3 med = {sz.x * 0.5f + rMin.x, sz.y * 0.5f + rMin.y, sz.z * 0.5f + rMin.z};
+    const bool yAboveX = sz.x < sz.y;
+    const float mxs = yAboveX ? sz.y : sz.x;
+    const float spl = yAboveX ? med.y : med.x;
+    const int ax = yAboveX ? AX_Y : AX_X;
+    const bool zHigher = mxs < sz.z;
+    N.split[node] = zHigher ? med.z : spl;
+    N.info[node] = (int8_t)(zHigher ? AX_Z : ax);
+    const int cnt = (int)N.bitmap[node];
+    nseg[i] = (cnt + SEG - 1) / SEG;
+}
+
+// EmptySpaceSplitting2 (EmptySpaceSplitting.h:74-184): a chain of split
+// nodes with empty leaves between the node and its parent.
+__global__ __launch_bounds__(BS) void empty_chain_kernel(NodePtrs N, int active_index, int A,
+                                                         const int *__restrict__ planes_in,
+                                                         const int *__restrict__ eaddr) {
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i >= A) return;
+    int pl = planes_in[i];
+    if (!pl) return;
+    const int leafID = active_index + i;
+    const int start = active_index + A + eaddr[i];
+    const int cnt = 2 * __popc(pl);
+    for (int j = 0; j < cnt; ++j) zero_node(N, start + j);
+    const int parentID = N.parent[leafID];
+    if (N.children[parentID].x == leafID) N.children[parentID].x = start;
+    else N.children[parentID].y = start;
+    const f3 minP = mk3(N.bmin[leafID]), maxP = mk3(N.bmax[leafID]);
+    int pid = start;
+    for (int a = 0; a < 3; ++a) {
+        if (!(pl & (1 << a))) continue;
+        pl -= 1 << a;
+        N.info[pid] = (int8_t)(AX_X + a);
+        N.split[pid] = a == 0 ? minP.x : (a == 1 ? minP.y : minP.z);
+        const int left = pid + 1, right = pl ? left + 1 : leafID;
+        N.children[pid] = make_int2(left, right);
+        N.info[left] = LEAF; N.bitmap[left] = 0;
+        pid = right;
+    }
+    for (int a = 0; a < 3; ++a) {
+        if (!(pl & (8 << a))) continue;
+        pl -= 8 << a;
+        // the max-z case is an X split at max.x (EmptySpaceSplitting.h:172-175)
+        N.info[pid] = (int8_t)(a == 1 ? AX_Y : AX_X);
+        N.split[pid] = a == 1 ? maxP.y : maxP.x;
+        const int right = pid + 1, left = pl ? right + 1 : leafID;
+        N.children[pid] = make_int2(left, right);
+        N.info[right] = LEAF; N.bitmap[right] = 0;
+        pid = left;
+    }
+}
+
+// segment table (Segment, TriangleMapUpperCreator.cu:207-239)
+__global__ __launch_bounds__(BS) void seg_fill_kernel(const NodePtrs N, int active_index, int A,
+                                                      const int *__restrict__ segbase,
+                                                      int *__restrict__ seg_node,
+                                                      int *__restrict__ seg_start,
+                                                      int *__restrict__ seg_count) {
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i >= A) return;
+    const int node = active_index + i;
+    const int pi = N.prim_index[node];
+    const int cnt = (int)N.bitmap[node];
+    const int b = segbase[i], e = segbase[i + 1];
+    for (int s = b; s < e; ++s) {
+        const int off = (s - b) * SEG;
+        seg_node[s] = i;
+        seg_start[s] = pi + off;
+        seg_count[s] = min(SEG, cnt - off);
+    }
+}
+
+// SetSplitSide (TriangleUpperChildren.h:10-62) counted per segment
+__global__ __launch_bounds__(WPB * 64) void classify_kernel(
+    const NodePtrs N, int active_index, int S, const int *__restrict__ seg_node,
+    const int *__restrict__ seg_start, const int *__restrict__ seg_count,
+    const float4 *__restrict__ rmin, const float4 *__restrict__ rmax, int *__restrict__ segL,
+    int *__restrict__ segR) {
+    const int s = blockIdx.x * WPB + (threadIdx.x >> 6);
+    if (s >= S) return;
+    const int lane = threadIdx.x & 63;
+    const int node = active_index + seg_node[s];
+    const int ax = N.info[node];
+    const float sp = N.split[node];
+    const int start = seg_start[s], count = seg_count[s];
+    int cl = 0, cr = 0;
+#pragma unroll
+    for (int it = 0; it < SEG / 64; ++it) {
+        const int j = it * 64 + lane;
+        const bool valid = j < count;
+        bool l = false, r = false;
+        if (valid) {
+            l = comp4(rmin[start + j], ax) <= sp;
+            r = sp < comp4(rmax[start + j], ax);
+        }
+        cl += __popcll(__ballot(l));
+        cr += __popcll(__ballot(r));
+    }
+    if (lane == 0) { segL[s] = cl; segR[s] = cr; }
+}
+
+// CalcNodeChildSize (TriangleUpperChildren.h:144-162) + MarkNodeLeafs (:190-201)
+__global__ __launch_bounds__(BS) void child_size_kernel(int A, int LS,
+                                                        const int *__restrict__ segbase,
+                                                        const int *__restrict__ segLoff,
+                                                        const int *__restrict__ segRoff,
+                                                        int *__restrict__ slot_leaf) {
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i >= A) return;
+    const int b = segbase[i], e = segbase[i + 1];
+    const int L = segLoff[e] - segLoff[b];
+    const int R = segRoff[e] - segRoff[b];
+    slot_leaf[i] = L < LS;
+    slot_leaf[A + i] = R < LS;
+}
+
+// SetPrimitiveLeafSide (TriangleUpperChildren.h:164-186) counted per segment
+__global__ __launch_bounds__(BS) void seg_leaf_kernel(int A, int S, const int *__restrict__ seg_node,
+                                                      const int *__restrict__ slot_leaf,
+                                                      const int *__restrict__ segL,
+                                                      const int *__restrict__ segR,
+                                                      int *__restrict__ segLL,
+                                                      int *__restrict__ segLR) {
+    const int s = blockIdx.x * BS + threadIdx.x;
+    if (s >= S) return;
+    const int i = seg_node[s];
+    segLL[s] = slot_leaf[i] ? segL[s] : 0;
+    segLR[s] = slot_leaf[A + i] ? segR[s] : 0;
+}
+
+struct LevelTotals { int L_all, LL_all, leaf_nodes, ulp, nleaf; };
+
+// CreateUpperChildren (TriangleUpperChildren.h:203-281), MarkLeafNodes
+// (:385-395), PropagateAabbToChildren (EmptySpaceSplitting.h:299-331).
+__global__ __launch_bounds__(BS) void children_kernel(
+    NodePtrs N, int active_index, int A, int child_start, int next_active, LevelTotals tot,
+    int propagate, const int *__restrict__ segbase, const int *__restrict__ segLoff,
+    const int *__restrict__ segRoff, const int *__restrict__ segLLoff,
+    const int *__restrict__ segLRoff, const int *__restrict__ slot_leaf,
+    const int *__restrict__ lna, int *__restrict__ leaf_ids, int *__restrict__ keys_next) {
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i >= A) return;
+    const int par = active_index + i;
+    const int b = segbase[i], e = segbase[i + 1];
+    const int sizes[2] = {segLoff[e] - segLoff[b], segRoff[e] - segRoff[b]};
+    const int split_addr[2] = {segLoff[b], tot.L_all + segRoff[b]};
+    const int leaf_addr[2] = {segLLoff[b], tot.LL_all + segLRoff[b]};
+    const int ax = N.info[par] & 3;
+    const float s = N.split[par];
+    const float4 pmn = N.bmin[par], pmx = N.bmax[par];
+    int ids[2];
+    const int kmin = fkey(F_INF), kmax = fkey(-F_INF);
+    for (int side = 0; side < 2; ++side) {
+        const int slot = side * A + i;
+        const bool is_leaf = slot_leaf[slot] != 0;
+        const int id = child_start + (is_leaf ? lna[slot] : slot - lna[slot] + tot.leaf_nodes);
+        ids[side] = id;
+        N.info[id] = LEAF; N.split[id] = 0.0f;
+        N.prim_index[id] = is_leaf ? leaf_addr[side] + tot.ulp : split_addr[side] - leaf_addr[side];
+        N.bitmap[id] = sizes[side];
+        N.children[id] = make_int2(0, 0);
+        N.area[id] = 0.0f;
+        N.parent[id] = par;
+        if (propagate) {
+            if (side == 0) {
+                N.bmin[id] = pmn;
+                N.bmax[id] = make_float4(ax == AX_X ? s : pmx.x, ax == AX_Y ? s : pmx.y, ax == AX_Z ? s : pmx.z, 0.0f);
+            } else {
+                N.bmin[id] = make_float4(ax == AX_X ? s : pmn.x, ax == AX_Y ? s : pmn.y, ax == AX_Z ? s : pmn.z, 0.0f);
+                N.bmax[id] = pmx;
+            }
+        } else {
+            N.bmin[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            N.bmax[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        if (is_leaf) {
+            leaf_ids[tot.nleaf + lna[slot]] = id;
+        } else {
+            int *k = keys_next + 6 * (id - next_active);
+            k[0] = kmin; k[1] = kmin; k[2] = kmin; k[3] = kmax; k[4] = kmax; k[5] = kmax;
+        }
+    }
+    N.children[par] = make_int2(ids[0], ids[1]);
+}
+
+// SplitTriangles (TriangleUpperChildren.h:283-383): stable partition of the
+// refs with clipping, plus the next level's box reduction.
+__global__ __launch_bounds__(WPB * 64) void scatter_kernel(
+    const NodePtrs N, int active_index, int A, int S, int next_active, LevelTotals tot,
+    const int *__restrict__ seg_node, const int *__restrict__ seg_start,
+    const int *__restrict__ seg_count, const int *__restrict__ segLoff,
+    const int *__restrict__ segRoff, const int *__restrict__ segLLoff,
+    const int *__restrict__ segLRoff, const int *__restrict__ slot_leaf,
+    const float4 *__restrict__ rmin, const float4 *__restrict__ rmax,
+    float4 *__restrict__ nmin, float4 *__restrict__ nmax, float4 *__restrict__ fmin,
+    float4 *__restrict__ fmax, int *__restrict__ keys_next) {
+    const int s = blockIdx.x * WPB + (threadIdx.x >> 6);
+    if (s >= S) return;
+    const int lane = threadIdx.x & 63;
+    const int i = seg_node[s];
+    const int node = active_index + i;
+    const int ax = N.info[node];
+    const float sp = N.split[node];
+    const int2 ch = N.children[node];
+    const bool leafL = slot_leaf[i] != 0, leafR = slot_leaf[A + i] != 0;
+    const int start = seg_start[s], count = seg_count[s];
+    float4 *dLmin, *dLmax, *dRmin, *dRmax;
+    long long bL, bR;
+    if (leafL) { dLmin = fmin; dLmax = fmax; bL = (long long)tot.ulp + segLLoff[s]; }
+    else { dLmin = nmin; dLmax = nmax; bL = segLoff[s] - segLLoff[s]; }
+    if (leafR) { dRmin = fmin; dRmax = fmax; bR = (long long)tot.ulp + tot.LL_all + segLRoff[s]; }
+    else { dRmin = nmin; dRmax = nmax; bR = (long long)(tot.L_all - tot.LL_all) + (segRoff[s] - segLRoff[s]); }
+    const unsigned long long lt = lanemask_lt();
+    int rl = 0, rr = 0;
+    int l0 = fkey(F_INF), l1 = l0, l2 = l0, l3 = fkey(-F_INF), l4 = l3, l5 = l3;
+    int r0 = l0, r1 = l0, r2 = l0, r3 = l3, r4 = l3, r5 = l3;
+#pragma unroll
+    for (int it = 0; it < SEG / 64; ++it) {
+        const int j = it * 64 + lane;
+        const bool valid = j < count;
+        float4 mn = make_float4(0.0f, 0.0f, 0.0f, 0.0f), mx = mn;
+        bool l = false, r = false;
+        if (valid) {
+            mn = rmin[start + j];
+            mx = rmax[start + j];
+            l = comp4(mn, ax) <= sp;
+            r = sp < comp4(mx, ax);
+        }
+        const unsigned long long blm = __ballot(l), brm = __ballot(r);
+        if (l) {
+            const float4 cmx = make_float4(ax == AX_X ? omin(sp, mx.x) : mx.x,
+                                           ax == AX_Y ? omin(sp, mx.y) : mx.y,
+                                           ax == AX_Z ? omin(sp, mx.z) : mx.z, mx.w);
+            const long long p = bL + rl + __popcll(blm & lt);
+            dLmin[p] = mn;
+            dLmax[p] = cmx;
+            if (!leafL) {
+                l0 = min(l0, fkey(mn.x)); l1 = min(l1, fkey(mn.y)); l2 = min(l2, fkey(mn.z));
+                l3 = max(l3, fkey(cmx.x)); l4 = max(l4, fkey(cmx.y)); l5 = max(l5, fkey(cmx.z));
+            }
+        }
+        if (r) {
+            const float4 cmn = make_float4(ax == AX_X ? omax(sp, mn.x) : mn.x,
+                                           ax == AX_Y ? omax(sp, mn.y) : mn.y,
+                                           ax == AX_Z ? omax(sp, mn.z) : mn.z, mn.w);
+            const long long p = bR + rr + __popcll(brm & lt);
+            dRmin[p] = cmn;
+            dRmax[p] = mx;
+            if (!leafR) {
+                r0 = min(r0, fkey(cmn.x)); r1 = min(r1, fkey(cmn.y)); r2 = min(r2, fkey(cmn.z));
+                r3 = max(r3, fkey(mx.x)); r4 = max(r4, fkey(mx.y)); r5 = max(r5, fkey(mx.z));
+            }
+        }
+        rl += __popcll(blm);
+        rr += __popcll(brm);
+    }
+    if (!leafL && rl > 0) {
+        l0 = wave_min(l0); l1 = wave_min(l1); l2 = wave_min(l2);
+        l3 = wave_max(l3); l4 = wave_max(l4); l5 = wave_max(l5);
+        if (lane == 0) {
+            int *k = keys_next + 6 * (ch.x - next_active);
+            atomicMin(k + 0, l0); atomicMin(k + 1, l1); atomicMin(k + 2, l2);
+            atomicMax(k + 3, l3); atomicMax(k + 4, l4); atomicMax(k + 5, l5);
+        }
+    }
+    if (!leafR && rr > 0) {
+        r0 = wave_min(r0); r1 = wave_min(r1); r2 = wave_min(r2);
+        r3 = wave_max(r3); r4 = wave_max(r4); r5 = wave_max(r5);
+        if (lane == 0) {
+            int *k = keys_next + 6 * (ch.y - next_active);
+            atomicMin(k + 0, r0); atomicMin(k + 1, r1); atomicMin(k + 2, r2);
+            atomicMax(k + 3, r3); atomicMax(k + 4, r4); atomicMax(k + 5, r5);
+        }
+    }
+}
+
+// ExtractIndexFromAabb (TriangleUpper.h:43-52)
+__global__ void extract_index_kernel(const float4 *__restrict__ fmin, int *__restrict__ out, int n) {
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i < n) out[i] = __float_as_int(fmin[i].w);
+}
+
+// PropagateParentAabb<true> (EmptySpaceSplitting.h:333-368)
+__global__ void propagate_parent_kernel(NodePtrs N, const int *__restrict__ leaf_ids, int n) {
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i >= n) return;
+    const int leaf = leaf_ids[i];
+    const int par = N.parent[leaf];
+    const float4 pmn = N.bmin[par], pmx = N.bmax[par];
+    const int ax = N.info[par] & 3;
+    const float s = N.split[par];
+    if (N.children[par].x == leaf) {
+        N.bmin[leaf] = make_float4(pmn.x, pmn.y, pmn.z, 0.0f);
+        N.bmax[leaf] = make_float4(ax == AX_X ? s : pmx.x, ax == AX_Y ? s : pmx.y, ax == AX_Z ? s : pmx.z, 0.0f);
+    } else {
+        N.bmin[leaf] = make_float4(ax == AX_X ? s : pmn.x, ax == AX_Y ? s : pmn.y, ax == AX_Z ? s : pmn.z, 0.0f);
+        N.bmax[leaf] = make_float4(pmx.x, pmx.y, pmx.z, 0.0f);
+    }
+}
+
+// ------------------------------------------------------------------ lower
+// PreprocessLeafNodes (TriangleMapBitmapCreator.cu:32-49)
+__global__ void bitmap_leaves_kernel(NodePtrs N, const int *__restrict__ leaf_ids, int n) {
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i >= n) return;
+    const int leaf = leaf_ids[i];
+    N.info[leaf] = LEAF;
+    const long long cnt = N.bitmap[leaf];
+    N.bitmap[leaf] = cnt >= 64 ? -1ll : (long long)((1ull << cnt) - 1ull);
+}
+
+// CalcSurfaceArea (LowerTriangleMap.h:24-38)
+__global__ void area_kernel(const float4 *__restrict__ P0, const float4 *__restrict__ P1,
+                            const float4 *__restrict__ P2, const int *__restrict__ prim_indices,
+                            int n, float *__restrict__ areas) {
+    const int k = blockIdx.x * BS + threadIdx.x;
+    if (k >= n) return;
+    const int t = prim_indices[k];
+    areas[k] = tri_area(mk3(P0[t]), mk3(P1[t]), mk3(P2[t]));
+}
+
+// PreprocesLowerNodes (LowerTriangleMap.h:56-81)
+__global__ void lower_prep_kernel(NodePtrs N, const int *__restrict__ leaf_ids, int n,
+                                  const float *__restrict__ areas) {
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i >= n) return;
+    const int leaf = leaf_ids[i];
+    const int pi = N.prim_index[leaf];
+    const int cnt = (int)N.bitmap[leaf];
+    float a = 0.0f;
+    unsigned long long bmp = 0;
+    for (int j = 0; j < cnt; ++j) {
+        const float aj = areas[pi + j];
+        bmp += aj > 0.0f ? (1ull << j) : 0ull;
+        a += aj;
+    }
+    N.area[leaf] = a;
+    N.bitmap[leaf] = (long long)bmp;
+}
+
+// CalcSAH + CalcAreaForSets (LowerTriangleMap.h:412-555), one wave per node.
+constexpr int MAXC = 6 * 64;
+__global__ __launch_bounds__(WPB * 64) void sah_kernel(
+    NodePtrs N, const int *__restrict__ list, int list_base, int A,
+    const float4 *__restrict__ pmin, const float4 *__restrict__ pmax,
+    const float *__restrict__ areas, float Ct, int leaf_max, float bonus,
+    int *__restrict__ flag, long long *__restrict__ sets, float *__restrict__ careas) {
+    __shared__ unsigned long long cL[WPB][MAXC];
+    __shared__ unsigned long long cR[WPB][MAXC];
+    __shared__ float ar[WPB][64];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int id = blockIdx.x * WPB + w;
+    if (id >= A) return;
+    const int par = list ? list[id] : list_base + id;
+    const int pi = N.prim_index[par];
+    const unsigned long long bmp = (unsigned long long)N.bitmap[par];
+    const int n = __popcll(bmp);
+    const float nodeA = N.area[par];
+    const bool forced = leaf_max > 0 && n > leaf_max;
+    if (!forced && !(((float)n - Ct) * nodeA > 0.0f)) {   // costs are >= 0: no split can win
+        if (lane == 0) { N.info[par] = LEAF; flag[id] = 0; }
+        return;
+    }
+    const bool has = (bmp >> lane) & 1ull;
+    f3 mn = {0.0f, 0.0f, 0.0f}, mx = {0.0f, 0.0f, 0.0f};
+    float a = 0.0f;
+    if (has) { mn = mk3(pmin[pi + lane]); mx = mk3(pmax[pi + lane]); a = areas[pi + lane]; }
+    ar[w][lane] = a;
+    // candidate planes (CreateSplittingPlanes, LowerTriangleMap.h:84-155):
+    // prim ascending, axis X/Y/Z, low plane then high plane
+    int c = 0;
+    for (unsigned long long b = bmp; b; b &= b - 1) {
+        const int i = __builtin_ctzll(b);
+        for (int axis = AX_X; axis <= AX_Z; ++axis) {
+            const float mnj = comp3(mn, axis), mxj = comp3(mx, axis);
+            const float lo = __shfl(mnj, i, 64);
+            const float hi = __shfl(mxj, i, 64);
+            const unsigned long long lL = __ballot(has && mnj <= lo);
+            const unsigned long long lR = __ballot(has && lo < mxj);
+            const unsigned long long hL = __ballot(has && mnj <= hi);
+            const unsigned long long hR = __ballot(has && hi < mxj);
+            if (lane == 0) {
+                cL[w][c] = lL; cR[w][c] = lR;
+                cL[w][c + 1] = hL; cR[w][c + 1] = hR;
+            }
+            c += 2;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const int nc = c;
+    float best = F_INF;
+    int bestc = INT_MAX;
+    for (int cc = lane; cc < nc; cc += 64) {
+        const unsigned long long L = cL[w][cc], R = cR[w][cc];
+        float aL = 0.0f, aR = 0.0f;
+        for (unsigned long long m = L; m; m &= m - 1) aL += ar[w][__builtin_ctzll(m)];
+        for (unsigned long long m = R; m; m &= m - 1) aR += ar[w][__builtin_ctzll(m)];
+        float cost = (float)__popcll(L) * aL + (float)__popcll(R) * aR;
+        if (bonus != 1.0f && (L == 0ull || R == 0ull)) cost = cost * bonus;
+        const bool progress = L != bmp && R != bmp;
+        if (forced && !progress) continue;
+        if (cost < best) { best = cost; bestc = cc; }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const float ob = __shfl_xor(best, off, 64);
+        const int oc = __shfl_xor(bestc, off, 64);
+        if (ob < best || (ob == best && oc < bestc)) { best = ob; bestc = oc; }
+    }
+    if (lane != 0) return;
+    bool split = false;
+    if (bestc != INT_MAX) {
+        const unsigned long long L = cL[w][bestc], R = cR[w][bestc];
+        float aL = 0.0f, aR = 0.0f;
+        for (unsigned long long m = L; m; m &= m - 1) aL += ar[w][__builtin_ctzll(m)];
+        for (unsigned long long m = R; m; m &= m - 1) aR += ar[w][__builtin_ctzll(m)];
+        const bool progress = L != bmp && R != bmp;
+        const float leaf_cost = ((float)n - Ct) * nodeA;
+        split = progress && (forced || best < leaf_cost);
+        if (split) {
+            const int rank = bestc / 6;
+            const int axis = (bestc % 6) / 2 + 1;
+            const bool high = bestc & 1;
+            unsigned long long b = bmp;
+            for (int k = 0; k < rank; ++k) b &= b - 1;
+            const int slot = __builtin_ctzll(b);
+            const float pos = high ? comp4(pmax[pi + slot], axis) : comp4(pmin[pi + slot], axis);
+            N.info[par] = (int8_t)axis;
+            N.split[par] = pos;
+            sets[2 * id] = (long long)L; sets[2 * id + 1] = (long long)R;
+            careas[2 * id] = aL; careas[2 * id + 1] = aR;
+        }
+    }
+    if (!split) N.info[par] = LEAF;
+    flag[id] = split;
+}
+
+// CreateLowerSAHChildren (LowerTriangleMap.h:557-600) + PropagateAabbToChildren
+// (LowerTriangleMap.h:602-636)
+__global__ void lower_children_kernel(NodePtrs N, const int *__restrict__ list, int list_base, int A,
+                                      int base, int splits, int propagate,
+                                      const int *__restrict__ flag, const int *__restrict__ off,
+                                      const long long *__restrict__ sets,
+                                      const float *__restrict__ careas) {
+    const int id = blockIdx.x * BS + threadIdx.x;
+    if (id >= A || !flag[id]) return;
+    const int par = list ? list[id] : list_base + id;
+    const int l = base + off[id], r = l + splits;
+    const int pi = N.prim_index[par];
+    const int ax = N.info[par] & 3;
+    const float s = N.split[par];
+    const float4 pmn = N.bmin[par], pmx = N.bmax[par];
+    const int ids[2] = {l, r};
+    for (int side = 0; side < 2; ++side) {
+        const int c = ids[side];
+        N.info[c] = LEAF; N.split[c] = 0.0f;
+        N.prim_index[c] = pi;
+        N.bitmap[c] = sets[2 * id + side];
+        N.area[c] = careas[2 * id + side];
+        N.children[c] = make_int2(0, 0);
+        N.parent[c] = par;
+        if (propagate) {
+            if (side == 0) {
+                N.bmin[c] = pmn;
+                N.bmax[c] = make_float4(ax == AX_X ? s : pmx.x, ax == AX_Y ? s : pmx.y, ax == AX_Z ? s : pmx.z, 0.0f);
+            } else {
+                N.bmin[c] = make_float4(ax == AX_X ? s : pmn.x, ax == AX_Y ? s : pmn.y, ax == AX_Z ? s : pmn.z, 0.0f);
+                N.bmax[c] = pmx;
+            }
+        } else {
+            N.bmin[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            N.bmax[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    }
+    N.children[par] = make_int2(l, r);
+}
+
+NodePtrs node_ptrs(pm_ctx *ctx) {
+    NodeArrays &a = ctx->nodes;
+    return NodePtrs{a.info.ptr, a.split.ptr, a.bmin.ptr, a.bmax.ptr, a.prim_index.ptr,
+                    a.bitmap.ptr, a.children.ptr, a.area.ptr, a.parent.ptr};
+}
+
+// read `count` ints from device locations in one round trip
+void readback(BuildScratch &sc, std::initializer_list<const int *> srcs, int *dst, hipStream_t s) {
+    Gather g{};
+    g.count = 0;
+    for (const int *p : srcs) g.src[g.count++] = p;
+    sc.totals.reserve(8, s);
+    sc.host.ensure(8);
+    gather_kernel<<<1, 64, 0, s>>>(g, sc.totals.ptr);
+    PM_LAUNCH_CHECK();
+    PM_HIP(hipMemcpyAsync(sc.host.ptr, sc.totals.ptr, g.count * sizeof(int), hipMemcpyDeviceToHost, s));
+    PM_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < g.count; ++i) dst[i] = sc.host.ptr[i];
+}
+
+}  // namespace
+
+void scan_exclusive(const int *in, int *out, int n, BuildScratch &sc, hipStream_t s) {
+    const int nb = blocks_for((long long)n + 1, SCAN_BLK);
+    if (nb == 1) {
+        scan_block_kernel<<<1, SCAN_T, 0, s>>>(in, out, n, nullptr);
+        PM_LAUNCH_CHECK();
+        return;
+    }
+    if (nb > SCAN_BLK) throw Error{PM_ERR_ARG, "scan too large"};
+    sc.scan_tmp.reserve((size_t)nb + 1, s);
+    scan_block_kernel<<<nb, SCAN_T, 0, s>>>(in, out, n, sc.scan_tmp.ptr);
+    PM_LAUNCH_CHECK();
+    scan_block_kernel<<<1, SCAN_T, 0, s>>>(sc.scan_tmp.ptr, sc.scan_tmp.ptr, nb, nullptr);
+    PM_LAUNCH_CHECK();
+    scan_add_kernel<<<nb, SCAN_T, 0, s>>>(out, n, sc.scan_tmp.ptr);
+    PM_LAUNCH_CHECK();
+}
+
+void build_tree(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2, int N,
+                const pm_build_params &prm, hipStream_t s) {
+    BuildScratch &sc = ctx->scratch;
+    NodeArrays &nodes = ctx->nodes;
+    const int LS = prm.lower_size;
+    const int max_levels = prm.max_upper_levels > 0 ? prm.max_upper_levels : 128;
+    ctx->has_tree = false;
+    ctx->p0 = P0; ctx->p1 = P1; ctx->p2 = P2; ctx->ntris = N;
+    ctx->level_refs.clear();
+
+    int nodes_size = 1, final_refs = 0, nleaf = 0;
+    nodes.reserve(1024, s, 0);
+    sc.ref_min[0].reserve((size_t)N + 1, s); sc.ref_max[0].reserve((size_t)N + 1, s);
+    sc.keys[0].reserve(6, s);
+    init_root_kernel<<<1, 64, 0, s>>>(node_ptrs(ctx), N);
+    PM_LAUNCH_CHECK();
+    init_keys_kernel<<<1, 64, 0, s>>>(sc.keys[0].ptr, 1);
+    PM_LAUNCH_CHECK();
+    if (N > 0) {
+        const int blocks = std::min(blocks_for(N, BS), 2048);
+        tri_box_kernel<<<blocks, BS, 0, s>>>(P0, P1, P2, N, sc.ref_min[0].ptr, sc.ref_max[0].ptr, sc.keys[0].ptr);
+        PM_LAUNCH_CHECK();
+    }
+    int cur = 0;          // ping-pong index of the level's ref arrays and keys
+    int active_index = 0, A = 1, R = N, level = 0;
+    int hv[8];
+    while (A > 0) {
+        if (level >= max_levels)
+            throw Error{PM_ERR_DEPTH, "upper build did not finish within " + std::to_string(max_levels) +
+                                          " levels (>= lower_size coincident triangles?)"};
+        const long long max_refs = (long long)(prm.max_ref_factor > 0 ? prm.max_ref_factor : 16) * N + 65536;
+        if (R > max_refs)
+            throw Error{PM_ERR_DEPTH, "upper build exceeded " + std::to_string(max_refs) +
+                                          " references (>= lower_size coincident triangles?)"};
+        ctx->level_refs.push_back(R);
+        NodePtrs NP = node_ptrs(ctx);
+        sc.node_req.reserve(A + 1, s); sc.node_eaddr.reserve(A + 1, s);
+        sc.node_nseg.reserve(A + 1, s); sc.node_segbase.reserve(A + 1, s);
+        sc.slot_leaf.reserve(2 * (size_t)A + 1, s); sc.slot_lna.reserve(2 * (size_t)A + 1, s);
+        // planes are kept in slot_lna's storage until the chain kernel has run
+        int *planes = sc.slot_lna.ptr;
+        node_prepare_kernel<<<blocks_for(A, BS), BS, 0, s>>>(NP, active_index, A, sc.keys[cur].ptr,
+                                                              prm.empty_space_splitting,
+                                                              prm.empty_space_threshold, planes,
+                                                              sc.node_req.ptr, sc.node_nseg.ptr);
+        PM_LAUNCH_CHECK();
+        scan_exclusive(sc.node_req.ptr, sc.node_eaddr.ptr, A, sc, s);
+        scan_exclusive(sc.node_nseg.ptr, sc.node_segbase.ptr, A, sc, s);
+        readback(sc, {sc.node_eaddr.ptr + A, sc.node_segbase.ptr + A}, hv, s);
+        const int E = hv[0], S = hv[1];
+        if (E > 0) {
+            nodes.reserve((size_t)nodes_size + E, s, nodes_size);
+            NP = node_ptrs(ctx);
+            empty_chain_kernel<<<blocks_for(A, BS), BS, 0, s>>>(NP, active_index, A, planes, sc.node_eaddr.ptr);
+            PM_LAUNCH_CHECK();
+            nodes_size += E;
+        }
+        // segments
+        sc.seg_node.reserve(S + 1, s); sc.seg_start.reserve(S + 1, s); sc.seg_count.reserve(S + 1, s);
+        sc.segL.reserve(S + 1, s); sc.segR.reserve(S + 1, s);
+        sc.segLL.reserve(S + 1, s); sc.segLR.reserve(S + 1, s);
+        sc.segLoff.reserve(S + 1, s); sc.segRoff.reserve(S + 1, s);
+        sc.segLLoff.reserve(S + 1, s); sc.segLRoff.reserve(S + 1, s);
+        seg_fill_kernel<<<blocks_for(A, BS), BS, 0, s>>>(NP, active_index, A, sc.node_segbase.ptr,
+                                                          sc.seg_node.ptr, sc.seg_start.ptr, sc.seg_count.ptr);
+        PM_LAUNCH_CHECK();
+        if (S > 0) {
+            classify_kernel<<<blocks_for(S, WPB), WPB * 64, 0, s>>>(NP, active_index, S, sc.seg_node.ptr,
+                                                                   sc.seg_start.ptr, sc.seg_count.ptr,
+                                                                   sc.ref_min[cur].ptr, sc.ref_max[cur].ptr,
+                                                                   sc.segL.ptr, sc.segR.ptr);
+            PM_LAUNCH_CHECK();
+        }
+        scan_exclusive(sc.segL.ptr, sc.segLoff.ptr, S, sc, s);
+        scan_exclusive(sc.segR.ptr, sc.segRoff.ptr, S, sc, s);
+        child_size_kernel<<<blocks_for(A, BS), BS, 0, s>>>(A, LS, sc.node_segbase.ptr, sc.segLoff.ptr,
+                                                            sc.segRoff.ptr, sc.slot_leaf.ptr);
+        PM_LAUNCH_CHECK();
+        if (S > 0) {
+            seg_leaf_kernel<<<blocks_for(S, BS), BS, 0, s>>>(A, S, sc.seg_node.ptr, sc.slot_leaf.ptr,
+                                                              sc.segL.ptr, sc.segR.ptr, sc.segLL.ptr, sc.segLR.ptr);
+            PM_LAUNCH_CHECK();
+        }
+        scan_exclusive(sc.segLL.ptr, sc.segLLoff.ptr, S, sc, s);
+        scan_exclusive(sc.segLR.ptr, sc.segLRoff.ptr, S, sc, s);
+        scan_exclusive(sc.slot_leaf.ptr, sc.slot_lna.ptr, 2 * A, sc, s);
+        readback(sc, {sc.segLoff.ptr + S, sc.segRoff.ptr + S, sc.segLLoff.ptr + S, sc.segLRoff.ptr + S,
+                      sc.slot_lna.ptr + 2 * A}, hv, s);
+        const int L_all = hv[0], R_all = hv[1], LL_all = hv[2], LR_all = hv[3], leaf_nodes = hv[4];
+        const long long newTri = (long long)L_all + R_all;
+        if (newTri < R) throw Error{PM_ERR_INTERNAL, "new triangle count below old"};
+        if (newTri > INT_MAX) throw Error{PM_ERR_ARG, "reference count overflows int"};
+        const int leafTri = LL_all + LR_all;
+        const int nextR = (int)newTri - leafTri;
+        const int child_start = nodes_size;
+        const int created = 2 * A - leaf_nodes;
+        const int next_active = child_start + leaf_nodes;
+        nodes.reserve((size_t)nodes_size + 2 * (size_t)A, s, nodes_size);
+        NP = node_ptrs(ctx);
+        nodes_size += 2 * A;
+        sc.prim_min.reserve((size_t)final_refs + leafTri + 1, s, final_refs);
+        sc.prim_max.reserve((size_t)final_refs + leafTri + 1, s, final_refs);
+        ctx->leaf_ids.reserve((size_t)nleaf + leaf_nodes + 1, s, nleaf);
+        const int nxt = cur ^ 1;
+        sc.ref_min[nxt].reserve((size_t)nextR + 1, s); sc.ref_max[nxt].reserve((size_t)nextR + 1, s);
+        sc.keys[nxt].reserve(6 * (size_t)created + 6, s);
+        LevelTotals tot{L_all, LL_all, leaf_nodes, final_refs, nleaf};
+        children_kernel<<<blocks_for(A, BS), BS, 0, s>>>(
+            NP, active_index, A, child_start, next_active, tot, prm.empty_space_splitting,
+            sc.node_segbase.ptr, sc.segLoff.ptr, sc.segRoff.ptr, sc.segLLoff.ptr, sc.segLRoff.ptr,
+            sc.slot_leaf.ptr, sc.slot_lna.ptr, ctx->leaf_ids.ptr, sc.keys[nxt].ptr);
+        PM_LAUNCH_CHECK();
+        if (S > 0) {
+            scatter_kernel<<<blocks_for(S, WPB), WPB * 64, 0, s>>>(
+                NP, active_index, A, S, next_active, tot, sc.seg_node.ptr, sc.seg_start.ptr,
+                sc.seg_count.ptr, sc.segLoff.ptr, sc.segRoff.ptr, sc.segLLoff.ptr, sc.segLRoff.ptr,
+                sc.slot_leaf.ptr, sc.ref_min[cur].ptr, sc.ref_max[cur].ptr, sc.ref_min[nxt].ptr,
+                sc.ref_max[nxt].ptr, sc.prim_min.ptr, sc.prim_max.ptr, sc.keys[nxt].ptr);
+            PM_LAUNCH_CHECK();
+        }
+        final_refs += leafTri;
+        nleaf += leaf_nodes;
+        cur = nxt;
+        active_index = nodes_size - created;
+        A = created;
+        R = nextR;
+        ++level;
+    }
+    ctx->upper_levels = level;
+    NodePtrs NP = node_ptrs(ctx);
+    ctx->prim_indices.reserve((size_t)final_refs + 1, s);
+    if (final_refs > 0) {
+        extract_index_kernel<<<blocks_for(final_refs, BS), BS, 0, s>>>(sc.prim_min.ptr, ctx->prim_indices.ptr, final_refs);
+        PM_LAUNCH_CHECK();
+    }
+    if (!prm.empty_space_splitting && prm.propagate_aabbs && nleaf > 0) {
+        propagate_parent_kernel<<<blocks_for(nleaf, BS), BS, 0, s>>>(NP, ctx->leaf_ids.ptr, nleaf);
+        PM_LAUNCH_CHECK();
+    }
+
+    // lower stage
+    int lower_levels = 0;
+    if (prm.lower_algorithm == PM_LOWER_BITMAP) {
+        if (nleaf > 0) {
+            bitmap_leaves_kernel<<<blocks_for(nleaf, BS), BS, 0, s>>>(NP, ctx->leaf_ids.ptr, nleaf);
+            PM_LAUNCH_CHECK();
+        }
+    } else {
+        sc.areas.reserve((size_t)final_refs + 1, s);
+        if (final_refs > 0) {
+            area_kernel<<<blocks_for(final_refs, BS), BS, 0, s>>>(P0, P1, P2, ctx->prim_indices.ptr, final_refs, sc.areas.ptr);
+            PM_LAUNCH_CHECK();
+        }
+        if (nleaf > 0) {
+            lower_prep_kernel<<<blocks_for(nleaf, BS), BS, 0, s>>>(NP, ctx->leaf_ids.ptr, nleaf, sc.areas.ptr);
+            PM_LAUNCH_CHECK();
+        }
+        const int *list = ctx->leaf_ids.ptr;
+        int list_base = 0, An = nleaf;
+        while (An > 0) {
+            sc.lower_flag.reserve(An + 1, s); sc.lower_off.reserve(An + 1, s);
+            sc.lower_sets.reserve(2 * (size_t)An, s); sc.lower_careas.reserve(2 * (size_t)An, s);
+            sah_kernel<<<blocks_for(An, WPB), WPB * 64, 0, s>>>(
+                NP, list, list_base, An, sc.prim_min.ptr, sc.prim_max.ptr, sc.areas.ptr,
+                prm.traversal_cost, prm.sah_leaf_max, prm.sah_empty_bonus, sc.lower_flag.ptr,
+                sc.lower_sets.ptr, sc.lower_careas.ptr);
+            PM_LAUNCH_CHECK();
+            scan_exclusive(sc.lower_flag.ptr, sc.lower_off.ptr, An, sc, s);
+            readback(sc, {sc.lower_off.ptr + An}, hv, s);
+            const int splits = hv[0];
+            ++lower_levels;
+            if (splits == 0) break;
+            const int base = nodes_size;
+            nodes.reserve((size_t)nodes_size + 2 * (size_t)splits, s, nodes_size);
+            NP = node_ptrs(ctx);
+            nodes_size += 2 * splits;
+            lower_children_kernel<<<blocks_for(An, BS), BS, 0, s>>>(NP, list, list_base, An, base, splits,
+                                                                    prm.propagate_aabbs, sc.lower_flag.ptr,
+                                                                    sc.lower_off.ptr, sc.lower_sets.ptr,
+                                                                    sc.lower_careas.ptr);
+            PM_LAUNCH_CHECK();
+            list = nullptr;
+            list_base = base;
+            An = 2 * splits;
+        }
+    }
+    ctx->lower_levels = lower_levels;
+    ctx->num_nodes = nodes_size;
+    ctx->num_prim_indices = final_refs;
+    ctx->num_leaf_ids = nleaf;
+    ctx->has_boxes = prm.propagate_aabbs != 0;
+    ctx->has_tree = true;
+}
+
+}  // namespace pmkd

@@ -1,0 +1,98 @@
+// pm_device.h -- device-side helpers shared by the build and trace kernels.
+//
+// All float arithmetic is written in the reference's operation order and the
+// library is compiled with -ffp-contract=off and correctly rounded fp32
+// division/sqrt, so every value is the IEEE binary32 result of the same
+// sequence of operations the reference's host branch performs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pmkd {
+
+// Utils.h:18: fInfinity is the integer 0x7f800000 used as a float value.
+constexpr float F_INF = 2139095040.0f;
+constexpr int LEAF = 0, AX_X = 1, AX_Y = 2, AX_Z = 3;  // KDNode.h:62-65
+constexpr int MAX_DESCENT = 4096;
+constexpr int SEG = 256;          // refs per build segment (one wave64, 4 refs per lane)
+
+struct f3 { float x, y, z; };
+
+__device__ __forceinline__ f3 mk3(const float4 &a) { return {a.x, a.y, a.z}; }
+__device__ __forceinline__ f3 sub3(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 cross3(f3 a, f3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float comp3(f3 a, int axis) {
+    return axis == AX_X ? a.x : (axis == AX_Y ? a.y : a.z);
+}
+__device__ __forceinline__ float comp4(const float4 &a, int axis) {
+    return axis == AX_X ? a.x : (axis == AX_Y ? a.y : a.z);
+}
+
+// IEEE total order key: signed integer compare of keys orders floats (-0 < +0).
+__device__ __forceinline__ int fkey(float f) {
+    int b = __float_as_int(f);
+    return b >= 0 ? b : (b ^ 0x7fffffff);
+}
+__device__ __forceinline__ float unkey(int k) {
+    return __int_as_float(k >= 0 ? k : (k ^ 0x7fffffff));
+}
+__device__ __forceinline__ float omin(float a, float b) { return fkey(b) < fkey(a) ? b : a; }
+__device__ __forceinline__ float omax(float a, float b) { return fkey(b) > fkey(a) ? b : a; }
+
+// IRayTracer.h:60-65
+__device__ __forceinline__ f3 adjust_dir(f3 d) {
+    const float e = 0x1p-80f;
+    d.x = fabsf(d.x) > e ? d.x : copysignf(e, d.x);
+    d.y = fabsf(d.y) > e ? d.y : copysignf(e, d.y);
+    d.z = fabsf(d.z) > e ? d.z : copysignf(e, d.z);
+    return d;
+}
+
+// Moller-Trumbore as IRayTracer.h:111-145 (IEEE 1/det).  Later prims win
+// equal t because the acceptance test is t <= th.
+__device__ __forceinline__ bool moller_trumbore(f3 v0, f3 v1, f3 v2, f3 o, f3 d, float &th,
+                                                float &tu, float &tv) {
+    const f3 e1 = sub3(v1, v0);
+    const f3 e2 = sub3(v2, v0);
+    const f3 p = cross3(d, e2);
+    const f3 t = sub3(o, v0);
+    const f3 q = cross3(t, e1);
+    const float inv = 1.0f / dot3(p, e1);
+    const float hx = inv * dot3(q, e2);
+    if (0.0f <= hx && hx <= th) {
+        const float hy = inv * dot3(p, t);
+        const float hz = inv * dot3(q, d);
+        if (hy >= 0.0f && hz >= 0.0f && hy + hz <= 1.0f) {
+            th = hx; tu = hy; tv = hz;
+            return true;
+        }
+    }
+    return false;
+}
+
+// IRayTracer.h:147-168, invDir = true
+__device__ __forceinline__ bool ray_box(f3 o, f3 inv, f3 bmin, f3 bmax) {
+    bmin = {(bmin.x - o.x) * inv.x, (bmin.y - o.y) * inv.y, (bmin.z - o.z) * inv.z};
+    bmax = {(bmax.x - o.x) * inv.x, (bmax.y - o.y) * inv.y, (bmax.z - o.z) * inv.z};
+    float nearA = fminf(bmin.x, bmax.x);
+    nearA = fmaxf(nearA, fminf(bmin.y, bmax.y));
+    nearA = fmaxf(nearA, fminf(bmin.z, bmax.z));
+    float farA = fmaxf(bmax.x, bmin.x);
+    farA = fminf(farA, fmaxf(bmin.y, bmax.y));
+    farA = fminf(farA, fmaxf(bmin.z, bmax.z));
+    return nearA <= farA;
+}
+
+// LowerTriangleMap.h:32-36
+__device__ __forceinline__ float tri_area(f3 v0, f3 v1, f3 v2) {
+    const f3 c = cross3(sub3(v1, v0), sub3(v2, v0));
+    return 0.5f * sqrtf(dot3(c, c));
+}
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+}  // namespace pmkd

@@ -1,0 +1,157 @@
+// pm_internal.h -- host-side internals of libpmkd: device buffers, the
+// context (tree + scratch), and the entry points of the build and trace
+// translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pmkd.h"
+
+namespace pmkd {
+
+// Error carrying a pmkd status code.
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define PM_HIP(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            throw ::pmkd::Error{PM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+#define PM_LAUNCH_CHECK() PM_HIP(hipGetLastError())
+
+// Grow-only device allocation (the CUDADataBlock<T> role: Extend never
+// shrinks, contents are kept when it grows).
+template <class T>
+struct DevBuf {
+    T *ptr = nullptr;
+    size_t cap = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { if (ptr) (void)hipFree(ptr); }
+    // ensure capacity >= n elements; keeps the first `keep` elements
+    void reserve(size_t n, hipStream_t s, size_t keep = 0) {
+        if (n <= cap) return;
+        size_t ncap = cap ? cap : 256;
+        while (ncap < n) ncap += ncap / 2 + 256;
+        T *np = nullptr;
+        PM_HIP(hipMalloc(&np, ncap * sizeof(T)));
+        if (ptr && keep) PM_HIP(hipMemcpyAsync(np, ptr, keep * sizeof(T), hipMemcpyDeviceToDevice, s));
+        if (ptr) {
+            PM_HIP(hipStreamSynchronize(s));
+            PM_HIP(hipFree(ptr));
+        }
+        ptr = np;
+        cap = ncap;
+    }
+};
+
+// Pinned host staging for small read-backs.
+struct HostPinned {
+    int *ptr = nullptr;
+    size_t n = 0;
+    ~HostPinned() { if (ptr) (void)hipHostFree(ptr); }
+    void ensure(size_t count) {
+        if (count <= n) return;
+        if (ptr) PM_HIP(hipHostFree(ptr));
+        PM_HIP(hipHostMalloc((void **)&ptr, count * sizeof(int), hipHostMallocDefault));
+        n = count;
+    }
+};
+
+// Node arrays in the reference's SoA layout.
+struct NodeArrays {
+    DevBuf<int8_t> info;
+    DevBuf<float> split;
+    DevBuf<float4> bmin, bmax;
+    DevBuf<int> prim_index;
+    DevBuf<long long> bitmap;
+    DevBuf<int2> children;
+    DevBuf<float> area;
+    DevBuf<int> parent;
+    void reserve(size_t n, hipStream_t s, size_t keep) {
+        info.reserve(n, s, keep); split.reserve(n, s, keep);
+        bmin.reserve(n, s, keep); bmax.reserve(n, s, keep);
+        prim_index.reserve(n, s, keep); bitmap.reserve(n, s, keep);
+        children.reserve(n, s, keep); area.reserve(n, s, keep); parent.reserve(n, s, keep);
+    }
+};
+
+// Device view used by the tracers.
+struct TreeDev {
+    const int8_t *info;
+    const float *split;
+    const int2 *children;
+    const int *prim_index;
+    const long long *bitmap;
+    const float4 *bmin, *bmax;
+    const int *prim_indices;
+    int num_nodes;
+};
+
+// Build scratch (kept across builds; grow-only).
+struct BuildScratch {
+    DevBuf<float4> ref_min[2], ref_max[2];     // level ref arrays (ping-pong)
+    DevBuf<float4> prim_min, prim_max;         // final (upper leaf) refs
+    DevBuf<int> keys[2];                        // 6 ordered-int box keys per active node
+    DevBuf<int> node_req, node_eaddr;           // empty-space nodes per active node
+    DevBuf<int> node_nseg, node_segbase;        // segments per active node
+    DevBuf<int> seg_node, seg_start, seg_count; // segment table
+    DevBuf<int> segL, segR, segLL, segLR;       // per-segment counts (left, right, leaf-left, leaf-right)
+    DevBuf<int> segLoff, segRoff, segLLoff, segLRoff;
+    DevBuf<int> slot_leaf, slot_lna;            // 2A leaf-slot flags and addresses
+    DevBuf<int> scan_tmp;                       // block sums for scans
+    DevBuf<int> totals;                         // small device-side counters
+    DevBuf<float> areas;                        // lower stage: triangle areas per final ref
+    DevBuf<int> lower_list;                     // lower stage: node list
+    DevBuf<int> lower_flag, lower_off;
+    DevBuf<long long> lower_sets;               // 2 per node
+    DevBuf<float> lower_careas;                 // 2 per node
+    DevBuf<int> bounce_flags, bounce_offs;      // photon driver compaction
+    HostPinned host;
+};
+
+}  // namespace pmkd
+
+struct pm_ctx {
+    int device = 0;
+    std::string err;
+    // geometry (borrowed)
+    const float4 *p0 = nullptr, *p1 = nullptr, *p2 = nullptr;
+    int ntris = 0;
+    // tree
+    bool has_tree = false;
+    bool has_boxes = false;
+    int num_nodes = 0, num_prim_indices = 0, num_leaf_ids = 0;
+    int upper_levels = 0, lower_levels = 0;
+    pmkd::NodeArrays nodes;
+    pmkd::DevBuf<int> prim_indices;
+    pmkd::DevBuf<int> leaf_ids;
+    pmkd::BuildScratch scratch;
+    std::vector<long long> level_refs;
+};
+
+namespace pmkd {
+// pm_build.hip
+void build_tree(pm_ctx *ctx, const float4 *p0, const float4 *p1, const float4 *p2, int n,
+                const pm_build_params &prm, hipStream_t s);
+// pm_trace.hip
+void trace_rays(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int nrays,
+                pm_hit *out, const pm_trace_params &prm, int4 *counters, hipStream_t s);
+// pm_photon.hip: returns the number of continuation rays
+int photon_bounce(pm_ctx *ctx, const float4 *org, const float4 *dir, const pm_hit *hits, int n,
+                  int absorb_from, unsigned long long seed, float offset, float4 *out_org,
+                  float4 *out_dir, int *out_src, int *out_count, hipStream_t s);
+// device-wide exclusive scan of n ints; out has n+1 entries (out[n] = total)
+void scan_exclusive(const int *in, int *out, int n, BuildScratch &sc, hipStream_t s);
+}  // namespace pmkd

@@ -272,8 +272,8 @@ def heightfield(res: int = 1024, seed: int = 2):
 def config2_primary(width=1920, height=1080, res=1024, seed=2):
     """C2 scene and its primary rays (the bounce rays depend on the hits)."""
     tris = heightfield(res, seed)
-    org, dirs = pinhole_rays(width, height, (0.5, 0.55, -0.35), (0.5, 0.0, 0.55),
-                             fov_deg=60.0, seed=seed)
+    org, dirs = pinhole_rays(width, height, (0.5, 0.8, 0.0), (0.5, 0.0, 0.55),
+                             fov_deg=32.0, seed=seed)
     return tris, (org, dirs)
 
 

@@ -1,0 +1,216 @@
+/*
+ * pmkd.h -- C ABI of the MI355X kd-tree build + ray traversal library
+ * (libpmkd.so).
+ *
+ * Plain pointers and sizes only.  Device pointers are HIP device memory on
+ * the context's device.  Every "stream" argument is a hipStream_t passed as
+ * void* (NULL = the null stream).  Every function returns PM_OK (0) or an
+ * error code; pm_last_error() returns the message of the last failure on
+ * that context.
+ *
+ * The reference (OpenEngineDK extensions-PhotonMapping) has no C ABI: its
+ * path is a set of C++ classes.  Each entry point below names the reference
+ * interface it stands in for (file:line, relative to the reference root).
+ * The C++ classes with the reference's names live in include/pmkd/ (one header per class) and
+ * call only these functions.
+ */
+#ifndef PMKD_H
+#define PMKD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PMKD_ABI_VERSION 1
+
+/* error codes */
+#define PM_OK                  0
+#define PM_ERR_ARG             1  /* invalid argument                       */
+#define PM_ERR_NOT_IMPLEMENTED 2  /* TriangleMapUpperCreator.cu:551-552      */
+#define PM_ERR_DEPTH           3  /* upper build exceeded max_upper_levels   */
+#define PM_ERR_INTERNAL        4  /* TriangleMapUpperCreator.cu:606-607      */
+#define PM_ERR_UNSUPPORTED     5  /* option not provided by this library     */
+#define PM_ERR_HIP             6  /* HIP runtime error (CHECK_FOR_CUDA_ERROR) */
+#define PM_ERR_NO_TREE         7  /* trace before build / set_tree           */
+
+/* TriangleMap::LowerAlgorithm, Utils/CUDA/TriangleMap.h:33 */
+#define PM_LOWER_BITMAP   0
+#define PM_LOWER_BALANCED 1       /* not provided: PM_ERR_UNSUPPORTED */
+#define PM_LOWER_SAH      2
+/* TriangleMap::SplitMethod, Utils/CUDA/TriangleMap.h:34 (only BOX) */
+#define PM_SPLIT_BOX    0
+#define PM_SPLIT_DIVIDE 1
+#define PM_SPLIT_SPLIT  2
+/* tracer kinds: BruteTracer, RayTracer (kd-restart), ShortStack */
+#define PM_TRACE_BRUTE       0
+#define PM_TRACE_KD_RESTART  1
+#define PM_TRACE_SHORT_STACK 2
+
+typedef struct pm_ctx pm_ctx;
+
+/* Hit record (16 bytes): triangle id (index into p0/p1/p2, -1 on a miss),
+ * distance t and barycentrics u (weight of p1) and v (weight of p2), the
+ * values MoellerTrumbore leaves in (primHit, tHit) (IRayTracer.h:111-145).
+ * A miss is {-1, +inf, 0, 0}. */
+typedef struct { int id; float t, u, v; } pm_hit;
+
+/* Build options: the TriangleMap / creator setters.
+ *   lower_algorithm       TriangleMap::SetLowerAlgorithm     TriangleMap.cu:68-81
+ *   empty_space_splitting TriangleMap::SplitEmptySpace       TriangleMap.cu:83-85
+ *   empty_space_threshold SetEmptySpaceThreshold             TriangleMapUpperCreator.h:86
+ *   split_method          TriangleMap::SetSplitMethod        TriangleMap.cu:90
+ *   propagate_aabbs       TriangleMap::SetPropagateBoundingBox TriangleMap.cu:96-102
+ *   traversal_cost        TriangleMap::SetTraversalCost      TriangleMap.cu:104-108
+ *   lower_size            KDNODE_MAX_LOWER_SIZE              KDNode.h:21 (<= 64)
+ *   max_upper_levels      no counterpart: depth cap (0 = 128); the reference
+ *                         loops forever on >= lower_size coincident triangles
+ *   sah_leaf_max          no counterpart: force SAH splits above this many
+ *                         triangles (0 = off)
+ *   sah_empty_bonus       no counterpart: cost factor for a split with one
+ *                         empty side (1 = off)
+ *   max_ref_factor        no counterpart: error (PM_ERR_DEPTH) when a level
+ *                         holds more than max_ref_factor * n + 65536
+ *                         references (0 = 16); the reference doubles its
+ *                         references every level on coincident triangles */
+typedef struct {
+    int   lower_algorithm;
+    int   empty_space_splitting;
+    float empty_space_threshold;
+    int   split_method;
+    int   propagate_aabbs;
+    float traversal_cost;
+    int   lower_size;
+    int   max_upper_levels;
+    int   sah_leaf_max;
+    float sah_empty_bonus;
+    int   max_ref_factor;
+} pm_build_params;
+
+/* Trace options: IRayTracer setters (IRayTracer.h:50-58) and constants.
+ *   leaf_skipping  IRayTracer::SetLeafSkipping (default on, IRayTracer.cu:33-34);
+ *                  forced off when the tree has no propagated boxes
+ *                  (Raytracer.cu:313)
+ *   stack_size     SHORT_STACK_SIZE (ShortStack.cu:25), 1..16
+ *   brute_cap      MAX_PRIMS (BruteTracer.cu:50); 0 = test every triangle */
+typedef struct {
+    int leaf_skipping;
+    int stack_size;
+    int brute_cap;
+    int reserved;
+} pm_trace_params;
+
+/* Tree view: device pointers into the context's tree (non-owning; valid
+ * until the next build/set_tree/destroy), in the reference's SoA layout
+ * (KDNode.h:65-73, TriangleNode.h:20-21):
+ *   info[n]             0 LEAF, 1 X, 2 Y, 3 Z
+ *   split_pos[n]
+ *   aabb_min/max[4n]    propagated node boxes (float4)
+ *   primitive_index[n]  first slot in prim_indices
+ *   primitive_bitmap[n] leaves: mask over prim_indices[primitive_index + i];
+ *                       inner upper nodes: the reference count
+ *   children[2n]        (left, right); left is the side below split_pos
+ *   surface_area[n], parent[n]
+ *   prim_indices[m]     triangle ids (index into p0/p1/p2)
+ *   leaf_ids[k]         upper-stage leaves in creation order */
+typedef struct {
+    int num_nodes;
+    const int8_t *info;
+    const float *split_pos;
+    const float *aabb_min;
+    const float *aabb_max;
+    const int *primitive_index;
+    const long long *primitive_bitmap;
+    const int *children;
+    const float *surface_area;
+    const int *parent;
+    int num_prim_indices;
+    const int *prim_indices;
+    int num_leaf_ids;
+    const int *leaf_ids;
+    const float *p0, *p1, *p2;
+    int num_tris;
+    int upper_levels, lower_levels;
+    int has_boxes;
+} pm_tree;
+
+/* Tree statistics (PrintTree, TriangleMap.cu:110-116, plus depth/leaf data). */
+typedef struct {
+    int num_nodes, reachable_nodes, inner_nodes, leaves, empty_leaves;
+    int max_depth;
+    double mean_leaf_depth;
+    long long leaf_refs;
+    int max_leaf_size;
+    int upper_levels, lower_levels;
+    int num_prim_indices, num_upper_leaves;
+} pm_stats;
+
+/* Defaults of the TriangleMap constructor (TriangleMap.cu:27-49) and the
+ * upper creator (TriangleMapUpperCreator.cu:28-32): BITMAP lower creator,
+ * empty-space splitting on at 0.35, BOX split, boxes propagated, traversal
+ * cost 24, lower size 32. */
+void pm_build_params_default(pm_build_params *p);
+/* IRayTracer constructor defaults (IRayTracer.cu:32-38), ShortStack size 4,
+ * no brute-force cap. */
+void pm_trace_params_default(pm_trace_params *p);
+
+/* Context = TriangleMap + the three tracers' device state.  Replaces
+ * TriangleMap::TriangleMap (TriangleMap.cu:27-49) and the tracer
+ * constructors (Raytracer.cu:38-53, ShortStack.cu:37-52, BruteTracer.cu:30-46). */
+int pm_create(pm_ctx **out, int device);
+int pm_destroy(pm_ctx *ctx);
+const char *pm_last_error(const pm_ctx *ctx);
+
+/* TriangleMap::Create (TriangleMap.cu:51-62): upper stage
+ * (TriangleMapUpperCreator::Create, .cu:110-183) then the lower creator
+ * (ITriangleMapCreator::Create, ITriangleMapCreator.h:45-46).  p0/p1/p2 are
+ * device float4[n] (the GeometryList SoA, GeometryList.h:58-68); they are
+ * borrowed and must stay valid while the tree is traced.  Stream-ordered;
+ * the call returns after the build's last host read-back. */
+int pm_build(pm_ctx *ctx, const float *p0, const float *p1, const float *p2, int n,
+             const pm_build_params *params, void *stream);
+
+/* Geometry only, for the brute-force tracer (BruteTracer(GeometryList*),
+ * BruteTracer.h:28). */
+int pm_set_geometry(pm_ctx *ctx, const float *p0, const float *p1, const float *p2, int n);
+
+/* TriangleMap::GetNodes / GetPrimitiveIndices (TriangleMap.h:77-79) and the
+ * KDNode getters (KDNode.h:85-94). */
+int pm_tree_view(pm_ctx *ctx, pm_tree *out);
+
+/* Upload a tree given in HOST memory (same layout as pm_tree, host
+ * pointers; p0/p1/p2 in the struct are DEVICE pointers).  Lets a tree built
+ * elsewhere be traced. */
+int pm_set_tree(pm_ctx *ctx, const pm_tree *host_tree, void *stream);
+
+/* TriangleMap::PrintTree (TriangleMap.cu:110-116) as numbers: walks the tree
+ * on the host (copies it back). */
+int pm_tree_stats(pm_ctx *ctx, pm_stats *out, void *stream);
+
+/* IRayTracer::Trace (IRayTracer.h:47; RayTracer::Trace Raytracer.cu:295-390,
+ * ShortStack::Trace ShortStack.cu:231-321, BruteTracer::Trace
+ * BruteTracer.cu:123-172) with the ray arrays passed in (float4 org/dir per
+ * ray, device) and the first hit of each ray written to out (device) instead
+ * of a shaded canvas.  counters (device int[4*nrays] or NULL): per ray
+ * inner-node visits, leaf visits, triangle tests, leaf-box tests. */
+int pm_trace(pm_ctx *ctx, int algo, const float *org, const float *dir, int nrays,
+             pm_hit *out, const pm_trace_params *params, int *counters, void *stream);
+
+/* Photon path continuation (no reference counterpart: the reference's only
+ * secondary rays are the shading loop's reflections, Raytracer.cu:219-256).
+ * For every ray i with hits[i].id in [0, absorb_from): one diffuse
+ * continuation ray, origin o + t d + offset * n (n the geometric normal turned
+ * against d), direction cosine-weighted about n from PCG32 stream
+ * (seed, 6 << 40 | i).  Survivors are compacted in input order into
+ * out_org/out_dir (device float4[n]); out_src[j] (device, optional) is the
+ * source index; *n_out (host) receives the count, and out_count (device,
+ * optional) too.  Uses the geometry of the last build/set_geometry. */
+int pm_photon_bounce(pm_ctx *ctx, const float *org, const float *dir, const pm_hit *hits, int n,
+                     int absorb_from, unsigned long long seed, float offset, float *out_org,
+                     float *out_dir, int *out_src, int *out_count, int *n_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PMKD_H */

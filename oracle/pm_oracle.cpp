@@ -17,8 +17,10 @@
  *    (BruteTracer.cu:50,62);
  *  - the short-stack tracer intersects with the ray direction itself, not
  *    with 1/(1/d) (ShortStack.cu:134,148);
- *  - the upper build stops with an error after max_upper_levels levels
- *    instead of looping forever on >= lower_size coincident triangles;
+ *  - the upper build stops with an error after max_upper_levels levels or
+ *    when a level holds more than max_ref_factor * N + 65536 references,
+ *    instead of looping forever (and doubling its references every level)
+ *    on >= lower_size coincident triangles;
  *  - fields the reference leaves unwritten (boxes/prim index of empty-space
  *    nodes, w of node boxes, surface area of inner nodes) are zero, and the
  *    parent of a lower SAH child is its parent (LowerTriangleMap.h:557-600
@@ -150,7 +152,7 @@ struct Tree {
 struct Params {
     int lower_algorithm, empty_space_splitting, split_method, propagate_aabbs;
     float empty_space_threshold, traversal_cost, sah_empty_bonus;
-    int lower_size, max_upper_levels, sah_leaf_max;
+    int lower_size, max_upper_levels, sah_leaf_max, max_ref_factor;
 };
 
 template <class T>
@@ -196,6 +198,12 @@ int build_upper(const f4 *P0, const f4 *P1, const f4 *P2, int N, const Params &p
             return 3;
         }
         const int R = (int)rmin.size();
+        const long long max_refs = (long long)(prm.max_ref_factor > 0 ? prm.max_ref_factor : 16) * N + 65536;
+        if (R > max_refs) {
+            g_err = "upper build exceeded " + std::to_string(max_refs) +
+                    " references (>= lower_size coincident triangles?)";
+            return 3;
+        }
         T.level_refs.push_back(R);
 
         /* ReduceAabb, TriangleMapUpperCreator.cu:241-318 */
@@ -782,6 +790,7 @@ Params to_params(const orc_build_params *p) {
     q.max_upper_levels = p->max_upper_levels;
     q.sah_leaf_max = p->sah_leaf_max;
     q.sah_empty_bonus = p->sah_empty_bonus;
+    q.max_ref_factor = p->max_ref_factor;
     return q;
 }
 

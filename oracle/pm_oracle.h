@@ -41,6 +41,7 @@ typedef struct {
     int   max_upper_levels;      /* depth cap (no counterpart); 0 = 128 */
     int   sah_leaf_max;          /* 0 = off */
     float sah_empty_bonus;       /* 1 = off */
+    int   max_ref_factor;        /* refs guard: error above f*N + 65536 (0 = 16) */
 } orc_build_params;
 
 typedef struct {

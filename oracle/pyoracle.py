@@ -24,7 +24,8 @@ class BuildParams(C.Structure):
                 ("empty_space_threshold", C.c_float), ("split_method", C.c_int),
                 ("propagate_aabbs", C.c_int), ("traversal_cost", C.c_float),
                 ("lower_size", C.c_int), ("max_upper_levels", C.c_int),
-                ("sah_leaf_max", C.c_int), ("sah_empty_bonus", C.c_float)]
+                ("sah_leaf_max", C.c_int), ("sah_empty_bonus", C.c_float),
+                ("max_ref_factor", C.c_int)]
 
 
 class TraceParams(C.Structure):
@@ -47,7 +48,7 @@ def build_params(preset: str = "reference", **kw) -> BuildParams:
     p = BuildParams(lower_algorithm=LOWER_SAH, empty_space_splitting=1,
                     empty_space_threshold=0.35, split_method=0, propagate_aabbs=1,
                     traversal_cost=24.0, lower_size=32, max_upper_levels=0,
-                    sah_leaf_max=0, sah_empty_bonus=1.0)
+                    sah_leaf_max=0, sah_empty_bonus=1.0, max_ref_factor=0)
     if preset == "bitmap":
         p.lower_algorithm = LOWER_BITMAP
     elif preset == "config":
