@@ -30,7 +30,7 @@ BRUTE, KD_RESTART, SHORT_STACK = 0, 1, 2
 # every symbol include/pmkd.h declares
 EXPORTS = ("pm_build_params_default", "pm_trace_params_default", "pm_create", "pm_destroy",
            "pm_last_error", "pm_build", "pm_set_geometry", "pm_tree_view", "pm_set_tree",
-           "pm_tree_stats", "pm_trace", "pm_photon_bounce")
+           "pm_tree_stats", "pm_trace", "pm_photon_bounce", "pm_add_mesh")
 
 
 class PmError(RuntimeError):
@@ -100,8 +100,9 @@ def load_library(path: str = LIB_PATH):
     L.pm_trace.argtypes = [P, I, P, P, I, P, C.POINTER(TraceParams), P, P]
     L.pm_photon_bounce.argtypes = [P, P, P, P, I, I, C.c_ulonglong, C.c_float, P, P, P, P,
                                    C.POINTER(C.c_int), P]
+    L.pm_add_mesh.argtypes = [P, P, I, P, I, P, P, P, P, I, P]
     for n in ("pm_create", "pm_destroy", "pm_build", "pm_set_geometry", "pm_tree_view",
-              "pm_set_tree", "pm_tree_stats", "pm_trace", "pm_photon_bounce"):
+              "pm_set_tree", "pm_tree_stats", "pm_trace", "pm_photon_bounce", "pm_add_mesh"):
         getattr(L, n).restype = C.c_int
     _lib = L
     return L
@@ -275,6 +276,18 @@ class Context:
                                      _dev_ptr(out) if n else None, C.byref(params), cptr,
                                      _stream_ptr(stream)))
         return out
+
+    def add_mesh(self, vertices, indices, p0, p1, p2, offset: int = 0, model=None, stream=None):
+        """gather an indexed mesh (device float4 vertices, device int32/uint32
+        indices) into p0/p1/p2 rows [offset, offset + len(indices)//3)"""
+        m = None
+        if model is not None:
+            m = np.ascontiguousarray(model, dtype=np.float32).reshape(16)
+        self._check(self._L.pm_add_mesh(self.h, _dev_ptr(vertices), vertices.shape[0],
+                                        _dev_ptr(indices), indices.numel(),
+                                        m.ctypes.data if m is not None else None,
+                                        _dev_ptr(p0), _dev_ptr(p1), _dev_ptr(p2), offset,
+                                        _stream_ptr(stream)))
 
     def photon_bounce(self, org, dirs, hits, absorb_from: int, seed: int, offset: float = 1e-4,
                       out_org=None, out_dir=None, out_src=None, stream=None):

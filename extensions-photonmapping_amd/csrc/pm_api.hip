@@ -248,6 +248,19 @@ int pm_trace(pm_ctx *ctx, int algo, const float *org, const float *dir, int nray
     });
 }
 
+int pm_add_mesh(pm_ctx *ctx, const float *vertices, int nv, const unsigned *indices, int nidx,
+                const float *model, float *p0, float *p1, float *p2, int offset, void *stream) {
+    return guarded(ctx, [&] {
+        if (nv < 0 || nidx < 0 || nidx % 3 || offset < 0 || (nidx > 0 && (!vertices || !indices || !p0 || !p1 || !p2)))
+            throw Error{PM_ERR_ARG, "invalid mesh"};
+        float4 *q0 = reinterpret_cast<float4 *>(p0) + offset;
+        float4 *q1 = reinterpret_cast<float4 *>(p1) + offset;
+        float4 *q2 = reinterpret_cast<float4 *>(p2) + offset;
+        pmkd::add_mesh(ctx, reinterpret_cast<const float4 *>(vertices), nv, indices, nidx, model, q0, q1, q2,
+                       (hipStream_t)stream);
+    });
+}
+
 int pm_photon_bounce(pm_ctx *ctx, const float *org, const float *dir, const pm_hit *hits, int n,
                      int absorb_from, unsigned long long seed, float offset, float *out_org,
                      float *out_dir, int *out_src, int *out_count, int *n_out, void *stream) {

@@ -1,0 +1,72 @@
+// pm_geometry.hip -- geometry staging: indexed mesh -> triangle SoA.
+//
+// AddMeshKernel (float4 vertex variant, GeometryList.cu:197-232): triangle
+// id = index triple 3*id..3*id+2, each vertex transformed by the model
+// matrix row by row, p_i = dot(row_i, v) evaluated left to right
+// (Math/CUDA/Matrix.h:44-49).  One thread per triangle (the reference
+// launches one thread per index, GeometryList.cu:331); indices outside the
+// vertex array are reported instead of read.
+#include "pm_device.h"
+#include "pm_internal.h"
+
+namespace pmkd {
+
+namespace {
+
+constexpr int TB = 256;
+
+struct Mat4 { float4 r[4]; };
+
+__device__ __forceinline__ float dot4(const float4 &a, const float4 &b) {
+    return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
+}
+
+__global__ __launch_bounds__(TB) void add_mesh_kernel(const float4 *__restrict__ verts, int nv,
+                                                      const unsigned *__restrict__ idx, int ntris,
+                                                      Mat4 m, float4 *__restrict__ p0,
+                                                      float4 *__restrict__ p1,
+                                                      float4 *__restrict__ p2, int *bad) {
+    const int id = blockIdx.x * TB + threadIdx.x;
+    if (id >= ntris) return;
+    const unsigned i0 = idx[3 * id], i1 = idx[3 * id + 1], i2 = idx[3 * id + 2];
+    if (i0 >= (unsigned)nv || i1 >= (unsigned)nv || i2 >= (unsigned)nv) {
+        atomicOr(bad, 1);
+        return;
+    }
+    const float4 v[3] = {verts[i0], verts[i1], verts[i2]};
+    float4 *out[3] = {p0, p1, p2};
+    for (int k = 0; k < 3; ++k)
+        out[k][id] = make_float4(dot4(m.r[0], v[k]), dot4(m.r[1], v[k]), dot4(m.r[2], v[k]), dot4(m.r[3], v[k]));
+}
+
+__global__ void clear_flag_kernel(int *flag) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *flag = 0;
+}
+
+}  // namespace
+
+void add_mesh(pm_ctx *ctx, const float4 *verts, int nv, const unsigned *idx, int nidx,
+              const float *model, float4 *p0, float4 *p1, float4 *p2, hipStream_t s) {
+    Mat4 m;
+    for (int i = 0; i < 4; ++i) {
+        if (model)
+            m.r[i] = make_float4(model[4 * i], model[4 * i + 1], model[4 * i + 2], model[4 * i + 3]);
+        else
+            m.r[i] = make_float4(i == 0, i == 1, i == 2, i == 3);
+    }
+    const int ntris = nidx / 3;
+    BuildScratch &sc = ctx->scratch;
+    sc.totals.reserve(8, s);
+    sc.host.ensure(8);
+    clear_flag_kernel<<<1, 64, 0, s>>>(sc.totals.ptr);
+    PM_LAUNCH_CHECK();
+    if (ntris > 0) {
+        add_mesh_kernel<<<(ntris + TB - 1) / TB, TB, 0, s>>>(verts, nv, idx, ntris, m, p0, p1, p2, sc.totals.ptr);
+        PM_LAUNCH_CHECK();
+    }
+    PM_HIP(hipMemcpyAsync(sc.host.ptr, sc.totals.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+    PM_HIP(hipStreamSynchronize(s));
+    if (sc.host.ptr[0]) throw Error{PM_ERR_ARG, "mesh index out of range of the vertex array"};
+}
+
+}  // namespace pmkd

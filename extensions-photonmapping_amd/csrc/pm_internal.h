@@ -152,6 +152,9 @@ void trace_rays(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int
 int photon_bounce(pm_ctx *ctx, const float4 *org, const float4 *dir, const pm_hit *hits, int n,
                   int absorb_from, unsigned long long seed, float offset, float4 *out_org,
                   float4 *out_dir, int *out_src, int *out_count, hipStream_t s);
+// pm_geometry.hip
+void add_mesh(pm_ctx *ctx, const float4 *verts, int nv, const unsigned *idx, int nidx,
+              const float *model, float4 *p0, float4 *p1, float4 *p2, hipStream_t s);
 // device-wide exclusive scan of n ints; out has n+1 entries (out[n] = total)
 void scan_exclusive(const int *in, int *out, int n, BuildScratch &sc, hipStream_t s);
 }  // namespace pmkd

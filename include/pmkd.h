@@ -197,6 +197,17 @@ int pm_tree_stats(pm_ctx *ctx, pm_stats *out, void *stream);
 int pm_trace(pm_ctx *ctx, int algo, const float *org, const float *dir, int nrays,
              pm_hit *out, const pm_trace_params *params, int *counters, void *stream);
 
+/* GeometryList::AddMesh(CUDAMeshNode*, modelMat) (GeometryList.cu:318-341,
+ * AddMeshKernel :197-232): gather an indexed mesh (device float4 vertices[nv],
+ * device uint indices[nidx], nidx a multiple of 3) into the triangle SoA,
+ * writing p0/p1/p2[offset .. offset + nidx/3) (device float4).  model is a
+ * HOST row-major 4x4 matrix (NULL = identity) applied as p = M v with each
+ * row dotted left to right; a caller holding an OpenEngine model matrix
+ * passes its transpose, as the reference does.  Out-of-range indices give
+ * PM_ERR_ARG.  Synchronises the stream. */
+int pm_add_mesh(pm_ctx *ctx, const float *vertices, int nv, const unsigned *indices, int nidx,
+                const float *model, float *p0, float *p1, float *p2, int offset, void *stream);
+
 /* Photon path continuation (no reference counterpart: the reference's only
  * secondary rays are the shading loop's reflections, Raytracer.cu:219-256).
  * For every ray i with hits[i].id in [0, absorb_from): one diffuse
