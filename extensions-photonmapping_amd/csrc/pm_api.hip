@@ -151,6 +151,19 @@ int pm_set_tree(pm_ctx *ctx, const pm_tree *h, void *stream) {
             throw Error{PM_ERR_ARG, "invalid geometry"};
         hipStream_t s = (hipStream_t)stream;
         const size_t n = (size_t)h->num_nodes;
+        // every index the tracers follow must stay inside its array
+        for (size_t i = 0; i < n; ++i) {
+            if (h->info[i] & 3) {
+                const int l = h->children[2 * i], r = h->children[2 * i + 1];
+                if (l < 0 || r < 0 || (size_t)l >= n || (size_t)r >= n)
+                    throw Error{PM_ERR_ARG, "child index out of range at node " + std::to_string(i)};
+            } else {
+                const unsigned long long b = (unsigned long long)h->primitive_bitmap[i];
+                if (b && (h->primitive_index[i] < 0 ||
+                          (long long)h->primitive_index[i] + 63 - __builtin_clzll(b) >= h->num_prim_indices))
+                    throw Error{PM_ERR_ARG, "leaf slots out of range at node " + std::to_string(i)};
+            }
+        }
         pmkd::NodeArrays &a = ctx->nodes;
         a.reserve(n, s, 0);
         PM_HIP(hipMemcpyAsync(a.info.ptr, h->info, n, hipMemcpyHostToDevice, s));
@@ -177,6 +190,8 @@ int pm_set_tree(pm_ctx *ctx, const pm_tree *h, void *stream) {
         ctx->p1 = reinterpret_cast<const float4 *>(h->p1);
         ctx->p2 = reinterpret_cast<const float4 *>(h->p2);
         ctx->ntris = h->num_tris;
+        pmkd::build_trace_view(ctx, s);
+        PM_HIP(hipStreamSynchronize(s));
         ctx->has_tree = true;
     });
 }

@@ -918,6 +918,7 @@ void build_tree(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P
     ctx->num_prim_indices = final_refs;
     ctx->num_leaf_ids = nleaf;
     ctx->has_boxes = prm.propagate_aabbs != 0;
+    build_trace_view(ctx, s);
     ctx->has_tree = true;
 }
 

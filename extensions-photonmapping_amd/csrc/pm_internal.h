@@ -96,6 +96,8 @@ struct TreeDev {
     const long long *bitmap;
     const float4 *bmin, *bmax;
     const int *prim_indices;
+    const int4 *packed;        // one 16-byte record per node (trace view)
+    const float4 *leaf_tris;   // 3 float4 per leaf slot: v0 (w = id bits), e1, e2
     int num_nodes;
 };
 
@@ -137,6 +139,8 @@ struct pm_ctx {
     pmkd::NodeArrays nodes;
     pmkd::DevBuf<int> prim_indices;
     pmkd::DevBuf<int> leaf_ids;
+    pmkd::DevBuf<int4> packed;      // trace view: packed nodes
+    pmkd::DevBuf<float4> leaf_tris; // trace view: leaf-ordered triangles
     pmkd::BuildScratch scratch;
     std::vector<long long> level_refs;
 };
@@ -146,6 +150,7 @@ namespace pmkd {
 void build_tree(pm_ctx *ctx, const float4 *p0, const float4 *p1, const float4 *p2, int n,
                 const pm_build_params &prm, hipStream_t s);
 // pm_trace.hip
+void build_trace_view(pm_ctx *ctx, hipStream_t s);
 void trace_rays(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int nrays,
                 pm_hit *out, const pm_trace_params &prm, int4 *counters, hipStream_t s);
 // pm_photon.hip: returns the number of continuation rays

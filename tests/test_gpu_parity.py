@@ -186,6 +186,27 @@ def test_error_codes(pm, torch, scenes):
     with pytest.raises(pm.PmError) as e:
         c.trace(pm.SHORT_STACK, dev(torch, org), dev(torch, dirs), pm.trace_params(stack_size=17))
     assert e.value.code == 1
+    # an uploaded tree whose indices leave their arrays is refused, not traced
+    import pyoracle as orc
+    tris_h = scenes.random_soup(100, seed=1)
+    good = orc.build(tris_h).arrays()
+    bad = {k: v.copy() for k, v in good.items()}
+    inner = np.nonzero(bad["info"] & 3)[0][0]
+    bad["children"][inner, 1] = bad["info"].shape[0] + 5
+    with pytest.raises(pm.PmError) as e:
+        c.set_tree(bad, *tris)
+    assert e.value.code == 1
+    bad = {k: v.copy() for k, v in good.items()}
+    leaf = np.nonzero((bad["info"] == 0) & (bad["primitive_bitmap"] != 0))[0][0]
+    bad["primitive_index"][leaf] = bad["prim_indices"].shape[0]
+    with pytest.raises(pm.PmError) as e:
+        c.set_tree(bad, *tris)
+    assert e.value.code == 1
+    bad = {k: v.copy() for k, v in good.items()}
+    bad["prim_indices"][0] = 100          # triangle id outside p0/p1/p2
+    with pytest.raises(pm.PmError) as e:
+        c.set_tree(bad, *tris)
+    assert e.value.code == 1
     c.close()
 
 
