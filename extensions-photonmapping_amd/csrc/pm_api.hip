@@ -151,6 +151,7 @@ int pm_set_tree(pm_ctx *ctx, const pm_tree *h, void *stream) {
             throw Error{PM_ERR_ARG, "invalid geometry"};
         hipStream_t s = (hipStream_t)stream;
         const size_t n = (size_t)h->num_nodes;
+        ctx->has_tree = false;
         // every index the tracers follow must stay inside its array
         for (size_t i = 0; i < n; ++i) {
             if (h->info[i] & 3) {
