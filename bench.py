@@ -46,6 +46,61 @@ def ray_bytes(cnt) -> int:
     return int(48 * n + 13 * c[0] + 8 * c[1] + 52 * c[2] + 32 * c[3])
 
 
+def bench_fwd_bwd(pm, scenes, torch, reps: int = 5) -> dict:
+    """65,536-triangle perturbed sphere, 512x512 rays, loss sum (t - t_target)^2:
+    plain forward trace vs forward + hand-written backward (same GPU)."""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    v_ref, v_pert, faces, (org, dirs) = scenes.config5()
+    ctx = pm.Context(0)
+    F = torch.from_numpy(faces.reshape(-1)).to(dev)
+    nt = faces.shape[0]
+
+    def tris(V):
+        P = [torch.empty((nt, 4), dtype=torch.float32, device=dev) for _ in range(3)]
+        V4 = torch.ones((V.shape[0], 4), dtype=torch.float32, device=dev)
+        V4[:, :3] = torch.from_numpy(V).to(dev)
+        ctx.add_mesh(V4, F, *P)
+        return P
+
+    O, D = torch.from_numpy(org).to(dev), torch.from_numpy(dirs).to(dev)
+    tp = pm.trace_params(stack_size=8)
+    ctx.build(*tris(v_ref), pm.build_params("reference"))
+    ids_ref, t_ref, _, _ = pm.hits_view(ctx.trace(pm.SHORT_STACK, O, D, tp))
+    t_ref = t_ref.clone()
+    ok_ref = ids_ref >= 0
+    P = tris(v_pert)
+    ctx.build(*P, pm.build_params("reference"))
+    H = torch.empty((O.shape[0], 4), dtype=torch.float32, device=dev)
+    gt = torch.empty(O.shape[0], dtype=torch.float32, device=dev)
+    nv = v_pert.shape[0]
+    gv = torch.empty((nv, 3), dtype=torch.float32, device=dev)
+    go = torch.empty((O.shape[0], 3), dtype=torch.float32, device=dev)
+    gd = torch.empty_like(go)
+    stream = torch.cuda.current_stream()
+    fwd, bwd = [], []
+    for i in range(reps + 1):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        e[0].record(stream)
+        ctx.trace(pm.SHORT_STACK, O, D, tp, out=H)
+        e[1].record(stream)
+        ids, t, _, _ = pm.hits_view(H)
+        m = (ids >= 0) & ok_ref
+        torch.where(m, 2.0 * (t - t_ref), torch.zeros_like(t), out=gt)
+        e[2].record(stream)
+        ctx.trace_backward(O, D, H, gt, None, None, F, nv, gv, go, gd)
+        e[3].record(stream)
+        torch.cuda.synchronize()
+        if i:
+            fwd.append(e[0].elapsed_time(e[1]))
+            bwd.append(e[2].elapsed_time(e[3]))
+    f, b = float(np.median(fwd)), float(np.median(bwd))
+    ctx.close()
+    return {"workload": f"{nt}-triangle perturbed sphere, {O.shape[0]} rays, loss sum (t - t_target)^2",
+            "fwd_trace_ms": round(f, 4), "bwd_ms": round(b, 4),
+            "fwd_bwd_over_fwd": round((f + b) / f, 3)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -57,6 +112,9 @@ def main() -> int:
     ap.add_argument("--stack", type=int, default=8)
     ap.add_argument("--cpu-sample", type=int, default=1_000_000,
                     help="rays the CPU baseline traces (0 disables the baseline)")
+    ap.add_argument("--probe", action="store_true",
+                    help="counter runs: one build, one short-stack trace of the emission batch and "
+                         "a 256 MiB device copy (the HBM-counter calibration), then exit")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -90,6 +148,17 @@ def main() -> int:
     bp = pm.build_params("reference")
     tp = pm.trace_params(stack_size=args.stack)
     stream = torch.cuda.current_stream()
+
+    if args.probe:
+        ctx.build(*P, bp)
+        ctx.trace(pm.SHORT_STACK, O0, D0, tp, out=H)
+        src_buf = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst_buf = torch.empty_like(src_buf)
+        dst_buf.copy_(src_buf)
+        torch.cuda.synchronize()
+        print(json.dumps({"probe": True, "copy_bytes": src_buf.numel() * 4, "rays": n0,
+                          "tris": ntris}), flush=True)
+        return 0
 
     def step(timing: dict | None):
         """one frame: rebuild + all photon passes; returns rays traced"""
@@ -158,6 +227,15 @@ def main() -> int:
     hits0 = H.view(torch.int32)[:, 0]
     hit_frac = float((hits0 >= 0).float().mean())
 
+    # HBM bytes per launch of the same kernel on the same workload, from the
+    # calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE runs committed in profiles/
+    traffic = None
+    pmc_path = os.path.join(ROOT, "profiles", "short_stack_c3_pmc.json")
+    if os.path.exists(pmc_path):
+        pmc = json.load(open(pmc_path))
+        if (pmc.get("rays"), pmc.get("tris"), pmc.get("stack")) == (n0, ntris, args.stack):
+            traffic = pmc["traffic_bytes"]
+
     # ---- other tracers on the same tree (same GPU): kd-restart, brute force
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     ev[0].record(stream)
@@ -173,6 +251,9 @@ def main() -> int:
     torch.cuda.synchronize()
     brute_mrays = nb / (ev[0].elapsed_time(ev[1]) * 1e-3) / 1e6
     st = ctx.stats()
+
+    fwd_bwd = bench_fwd_bwd(pm, scenes, torch)
+    log(f"[bench] fwd+bwd: {fwd_bwd}")
 
     # ---- CPU baseline: the oracle's build and trace on the host cores
     cpu = None
@@ -222,6 +303,7 @@ def main() -> int:
             "brute_force_mrays": round(brute_mrays, 4),
             "short_stack_vs_brute": round(mrays / brute_mrays, 1),
             "emission_hit_fraction": round(hit_frac, 4),
+            "fwd_bwd": fwd_bwd,
             "tree": {k: st[k] for k in ("num_nodes", "leaves", "empty_leaves", "max_depth",
                                         "leaf_refs", "upper_levels", "lower_levels")},
         },
@@ -232,7 +314,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "algorithmic_bytes": bytes_pass0,
             "launch_ms": round(pass0_ms, 4),
         },

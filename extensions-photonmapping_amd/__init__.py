@@ -30,7 +30,7 @@ BRUTE, KD_RESTART, SHORT_STACK = 0, 1, 2
 # every symbol include/pmkd.h declares
 EXPORTS = ("pm_build_params_default", "pm_trace_params_default", "pm_create", "pm_destroy",
            "pm_last_error", "pm_build", "pm_set_geometry", "pm_tree_view", "pm_set_tree",
-           "pm_tree_stats", "pm_trace", "pm_photon_bounce", "pm_add_mesh")
+           "pm_tree_stats", "pm_trace", "pm_photon_bounce", "pm_add_mesh", "pm_trace_backward")
 
 
 class PmError(RuntimeError):
@@ -101,8 +101,10 @@ def load_library(path: str = LIB_PATH):
     L.pm_photon_bounce.argtypes = [P, P, P, P, I, I, C.c_ulonglong, C.c_float, P, P, P, P,
                                    C.POINTER(C.c_int), P]
     L.pm_add_mesh.argtypes = [P, P, I, P, I, P, P, P, P, I, P]
+    L.pm_trace_backward.argtypes = [P, P, P, P, I, P, P, P, P, I, P, P, P, P]
     for n in ("pm_create", "pm_destroy", "pm_build", "pm_set_geometry", "pm_tree_view",
-              "pm_set_tree", "pm_tree_stats", "pm_trace", "pm_photon_bounce", "pm_add_mesh"):
+              "pm_set_tree", "pm_tree_stats", "pm_trace", "pm_photon_bounce", "pm_add_mesh",
+              "pm_trace_backward"):
         getattr(L, n).restype = C.c_int
     _lib = L
     return L
@@ -288,6 +290,31 @@ class Context:
                                         m.ctypes.data if m is not None else None,
                                         _dev_ptr(p0), _dev_ptr(p1), _dev_ptr(p2), offset,
                                         _stream_ptr(stream)))
+
+    def trace_backward(self, org, dirs, hits, grad_t=None, grad_u=None, grad_v=None, faces=None,
+                       num_vertices=None, grad_vertices=None, grad_org=None, grad_dir=None,
+                       stream=None):
+        """gradients of (t, u, v) w.r.t. vertices (deterministic per-vertex
+        sums), ray origins and directions, for the hits of a previous trace
+        over the current geometry; returns (grad_vertices, grad_org, grad_dir)"""
+        torch = _torch()
+        n = org.shape[0]
+        dev = org.device
+        if num_vertices is None:
+            num_vertices = (3 * self._geom[0].shape[0]) if faces is None else int(faces.max().item()) + 1
+        if grad_vertices is None:
+            grad_vertices = torch.empty((max(num_vertices, 1), 3), dtype=torch.float32, device=dev)
+        if grad_org is None:
+            grad_org = torch.empty((max(n, 1), 3), dtype=torch.float32, device=dev)
+        if grad_dir is None:
+            grad_dir = torch.empty((max(n, 1), 3), dtype=torch.float32, device=dev)
+        gp = [None if g is None else _dev_ptr(g) for g in (grad_t, grad_u, grad_v)]
+        self._check(self._L.pm_trace_backward(
+            self.h, _dev_ptr(org) if n else None, _dev_ptr(dirs) if n else None,
+            _dev_ptr(hits) if n else None, n, *gp,
+            _dev_ptr(faces) if faces is not None else None, num_vertices, _dev_ptr(grad_vertices),
+            _dev_ptr(grad_org), _dev_ptr(grad_dir), _stream_ptr(stream)))
+        return grad_vertices[:num_vertices], grad_org[:n], grad_dir[:n]
 
     def photon_bounce(self, org, dirs, hits, absorb_from: int, seed: int, offset: float = 1e-4,
                       out_org=None, out_dir=None, out_src=None, stream=None):

@@ -264,6 +264,21 @@ int pm_trace(pm_ctx *ctx, int algo, const float *org, const float *dir, int nray
     });
 }
 
+int pm_trace_backward(pm_ctx *ctx, const float *org, const float *dir, const pm_hit *hits, int nrays,
+                      const float *grad_t, const float *grad_u, const float *grad_v, const int *faces,
+                      int num_vertices, float *grad_vertices, float *grad_org, float *grad_dir,
+                      void *stream) {
+    return guarded(ctx, [&] {
+        if (nrays < 0 || num_vertices < 0 || (nrays > 0 && (!org || !dir || !hits)) ||
+            (num_vertices > 0 && !grad_vertices))
+            throw Error{PM_ERR_ARG, "invalid arguments"};
+        if (!ctx->p0 && ctx->ntris > 0) throw Error{PM_ERR_NO_TREE, "no geometry"};
+        pmkd::trace_backward(ctx, reinterpret_cast<const float4 *>(org), reinterpret_cast<const float4 *>(dir),
+                             hits, nrays, grad_t, grad_u, grad_v, faces, num_vertices, grad_vertices, grad_org,
+                             grad_dir, (hipStream_t)stream);
+    });
+}
+
 int pm_add_mesh(pm_ctx *ctx, const float *vertices, int nv, const unsigned *indices, int nidx,
                 const float *model, float *p0, float *p1, float *p2, int offset, void *stream) {
     return guarded(ctx, [&] {

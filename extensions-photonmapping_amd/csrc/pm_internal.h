@@ -120,6 +120,9 @@ struct BuildScratch {
     DevBuf<long long> lower_sets;               // 2 per node
     DevBuf<float> lower_careas;                 // 2 per node
     DevBuf<int> bounce_flags, bounce_offs;      // photon driver compaction
+    DevBuf<unsigned> bwd_keys[2], bwd_vals[2];  // backward: radix sort ping-pong
+    DevBuf<float> bwd_corner;                   // backward: 3 float3 per ray
+    DevBuf<int> bwd_hist, bwd_offs;             // backward: radix histograms
     HostPinned host;
 };
 
@@ -157,6 +160,11 @@ void trace_rays(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int
 int photon_bounce(pm_ctx *ctx, const float4 *org, const float4 *dir, const pm_hit *hits, int n,
                   int absorb_from, unsigned long long seed, float offset, float4 *out_org,
                   float4 *out_dir, int *out_src, int *out_count, hipStream_t s);
+// pm_backward.hip
+void trace_backward(pm_ctx *ctx, const float4 *org, const float4 *dir, const pm_hit *hits, int n,
+                    const float *grad_t, const float *grad_u, const float *grad_v, const int *faces,
+                    int num_vertices, float *grad_vertices, float *grad_org, float *grad_dir,
+                    hipStream_t s);
 // pm_geometry.hip
 void add_mesh(pm_ctx *ctx, const float4 *verts, int nv, const unsigned *idx, int nidx,
               const float *model, float4 *p0, float4 *p1, float4 *p2, hipStream_t s);

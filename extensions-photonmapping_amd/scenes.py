@@ -80,15 +80,9 @@ def quad(a, b, c, d):
     return [(a, b, c), (a, c, d)]
 
 
-def icosphere(subdiv: int, radius: float = 1.0, center=(0.0, 0.0, 0.0)):
-    """Icosahedron subdivided ``subdiv`` times: 20 * 4**subdiv triangles."""
-    t = (1.0 + 5.0 ** 0.5) / 2.0
-    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t),
-             (0, -1, -t), (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+def _subdivide_to_sphere(verts, faces, subdiv: int, radius: float, center):
+    """midpoint subdivision, every new vertex pushed onto the unit sphere"""
     verts = [np.array(v, dtype=np.float64) / np.linalg.norm(v) for v in verts]
-    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
-             (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
-             (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
     for _ in range(subdiv):
         cache = {}
         new_faces = []
@@ -108,6 +102,27 @@ def icosphere(subdiv: int, radius: float = 1.0, center=(0.0, 0.0, 0.0)):
     V = np.array(verts) * radius + np.asarray(center, dtype=np.float64)
     F = np.array(faces, dtype=np.int64)
     return V, F
+
+
+def icosphere(subdiv: int, radius: float = 1.0, center=(0.0, 0.0, 0.0)):
+    """Icosahedron subdivided ``subdiv`` times: 20 * 4**subdiv triangles."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t),
+             (0, -1, -t), (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+             (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+             (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    return _subdivide_to_sphere(verts, faces, subdiv, radius, center)
+
+
+def tetra_sphere(subdiv: int, radius: float = 1.0, center=(0.0, 0.0, 0.0)):
+    """Tetrahedron subdivided ``subdiv`` times onto the sphere: 4 * 4**subdiv
+    triangles, every vertex shared by at most 6 triangles (a UV sphere's
+    poles are fans of n_phi triangles through one point, which no kd-tree
+    with fewer than n_phi triangles per leaf can separate)."""
+    verts = [(1, 1, 1), (1, -1, -1), (-1, 1, -1), (-1, -1, 1)]
+    faces = [(0, 1, 2), (0, 3, 1), (0, 2, 3), (1, 3, 2)]
+    return _subdivide_to_sphere(verts, faces, subdiv, radius, center)
 
 
 def uv_sphere(n_theta: int, n_phi: int, radius=1.0, center=(0.0, 0.0, 0.0)):
@@ -370,3 +385,23 @@ def random_rays(n: int, seed: int, lo=-0.5, hi=1.5, target_lo=0.0, target_hi=1.0
     d = t - o
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     return _pack(o, 1.0), _pack(d, 0.0)
+
+
+# --------------------------------------------------------------------------
+# C5: perturbed sphere for the differentiable intersection
+# --------------------------------------------------------------------------
+def perturbed_sphere(subdiv: int = 7, sigma: float = 0.01, seed: int = 5):
+    """tetrahedron sphere (radius 1, 4 * 4**subdiv triangles; 7 gives 65,536)
+    as (vertices[V,3] float32 unperturbed, perturbed by N(0, sigma^2), faces[F,3] int32)."""
+    V, F = tetra_sphere(subdiv)
+    r = Pcg32(seed, P_PERTURB, np.arange(V.shape[0], dtype=np.uint64))
+    noise = np.stack([r.normal(), r.normal(), r.normal()], axis=1) * sigma
+    return _f32(V), _f32(V + noise), F.astype(np.int32)
+
+
+def config5(width=512, height=512, seed=5, subdiv=7):
+    """C5: sphere vertices (unperturbed, perturbed), faces and 512x512 rays
+    from (0, 0, 3) towards the origin."""
+    v_ref, v_pert, faces = perturbed_sphere(subdiv, seed=seed)
+    org, dirs = pinhole_rays(width, height, (0.0, 0.0, 3.0), (0.0, 0.0, 0.0), fov_deg=45.0, seed=seed)
+    return v_ref, v_pert, faces, (org, dirs)

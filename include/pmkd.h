@@ -197,6 +197,22 @@ int pm_tree_stats(pm_ctx *ctx, pm_stats *out, void *stream);
 int pm_trace(pm_ctx *ctx, int algo, const float *org, const float *dir, int nrays,
              pm_hit *out, const pm_trace_params *params, int *counters, void *stream);
 
+/* Backward of the intersection (no reference counterpart; the forward is
+ * MoellerTrumbore, IRayTracer.h:111-145).  For every ray with hits[r].id >= 0
+ * (the records a pm_trace call produced for these rays) and upstream
+ * gradients grad_t/grad_u/grad_v (device float[nrays] each, NULL = zero):
+ * grad_org/grad_dir (device float[3*nrays], optional) get the per-ray
+ * gradients; grad_vertices (device float[3*num_vertices]) gets the per-vertex
+ * sums over corners, where faces (device int[3*triangles]) maps triangle
+ * corners to vertex ids (NULL: vertex id = 3*triangle + corner, so
+ * num_vertices must be >= 3*triangles).  The sums are deterministic: entries
+ * are added in (ray, corner) order after a stable radix sort by vertex id.
+ * Uses the geometry of the last build/set_geometry (the forward's). */
+int pm_trace_backward(pm_ctx *ctx, const float *org, const float *dir, const pm_hit *hits, int nrays,
+                      const float *grad_t, const float *grad_u, const float *grad_v, const int *faces,
+                      int num_vertices, float *grad_vertices, float *grad_org, float *grad_dir,
+                      void *stream);
+
 /* GeometryList::AddMesh(CUDAMeshNode*, modelMat) (GeometryList.cu:318-341,
  * AddMeshKernel :197-232): gather an indexed mesh (device float4 vertices[nv],
  * device uint indices[nidx], nidx a multiple of 3) into the triangle SoA,

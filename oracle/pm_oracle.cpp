@@ -1037,4 +1037,93 @@ float orc_tri_area(const float *v0, const float *v1, const float *v2) {
     return tri_area({v0[0], v0[1], v0[2]}, {v1[0], v1[1], v1[2]}, {v2[0], v2[1], v2[2]});
 }
 
+int orc_trace_backward32(const float *p0, const float *p1, const float *p2, int ntris,
+                         const float *org, const float *dir, const orc_hit *hits, int nrays,
+                         const float *grad_t, const float *grad_u, const float *grad_v,
+                         const int *faces, int num_vertices, float *grad_vertices,
+                         float *grad_org, float *grad_dir) {
+    g_err.clear();
+    if (nrays < 0 || num_vertices < 0) { g_err = "invalid argument"; return 1; }
+    const f4 *P0 = reinterpret_cast<const f4 *>(p0), *P1 = reinterpret_cast<const f4 *>(p1),
+             *P2 = reinterpret_cast<const f4 *>(p2);
+    const f4 *O = reinterpret_cast<const f4 *>(org), *D = reinterpret_cast<const f4 *>(dir);
+    std::vector<float> corner(9 * (size_t)nrays, 0.0f);
+    std::vector<int> key(3 * (size_t)nrays, -1);
+    for (int r = 0; r < nrays; ++r) {
+        const int id = hits[r].id;
+        f3 g_o = {0, 0, 0}, g_d = {0, 0, 0}, g[3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        if (id >= 0 && id < ntris) {
+            const float gt = grad_t ? grad_t[r] : 0.0f, gu = grad_u ? grad_u[r] : 0.0f,
+                        gv = grad_v ? grad_v[r] : 0.0f;
+            const f3 v0 = mk3(P0[id]), v1 = mk3(P1[id]), v2 = mk3(P2[id]);
+            const f3 o = mk3(O[r]), d = mk3(D[r]);
+            const f3 e1 = sub(v1, v0), e2 = sub(v2, v0), T = sub(o, v0);
+            const f3 p = cross(d, e2), q = cross(T, e1);
+            const float det = dot(p, e1);
+            const float inv = 1.0f / det;
+            const float a = dot(q, e2), b = dot(p, T), c = dot(q, d);
+            const float ga = gt * inv, gb = gu * inv, gc = gv * inv;
+            const float ginv = gt * a + gu * b + gv * c;
+            const float gdet = -ginv * inv * inv;
+            auto sc = [](f3 x, float k) { return f3{x.x * k, x.y * k, x.z * k}; };
+            auto ad = [](f3 x, f3 y) { return f3{x.x + y.x, x.y + y.y, x.z + y.z}; };
+            const f3 gq = ad(sc(e2, ga), sc(d, gc));
+            const f3 gp = ad(sc(T, gb), sc(e1, gdet));
+            const f3 gT = ad(sc(p, gb), cross(e1, gq));
+            const f3 ge1 = ad(sc(p, gdet), cross(gq, T));
+            const f3 gd = ad(sc(q, gc), cross(e2, gp));
+            const f3 ge2 = ad(sc(q, ga), cross(gp, d));
+            g_o = gT; g_d = gd; g[1] = ge1; g[2] = ge2;
+            g[0] = {-gT.x - ge1.x - ge2.x, -gT.y - ge1.y - ge2.y, -gT.z - ge1.z - ge2.z};
+            for (int k = 0; k < 3; ++k) key[3 * (size_t)r + k] = faces ? faces[3 * id + k] : 3 * id + k;
+        }
+        for (int k = 0; k < 3; ++k) {
+            corner[9 * (size_t)r + 3 * k] = g[k].x;
+            corner[9 * (size_t)r + 3 * k + 1] = g[k].y;
+            corner[9 * (size_t)r + 3 * k + 2] = g[k].z;
+        }
+        if (grad_org) { grad_org[3 * r] = g_o.x; grad_org[3 * r + 1] = g_o.y; grad_org[3 * r + 2] = g_o.z; }
+        if (grad_dir) { grad_dir[3 * r] = g_d.x; grad_dir[3 * r + 1] = g_d.y; grad_dir[3 * r + 2] = g_d.z; }
+    }
+    std::fill(grad_vertices, grad_vertices + 3 * (size_t)num_vertices, 0.0f);
+    for (size_t e = 0; e < key.size(); ++e) {       /* (ray, corner) order */
+        const int v = key[e];
+        if (v < 0) continue;
+        if (v >= num_vertices) { g_err = "vertex id out of range"; return 1; }
+        for (int c = 0; c < 3; ++c) grad_vertices[3 * (size_t)v + c] += corner[3 * e + c];
+    }
+    return 0;
+}
+
+int orc_trace_backward64(const float *p0, const float *p1, const float *p2, int ntris,
+                         const float *org, const float *dir, const orc_hit *hits, int nrays,
+                         const float *grad_t, const float *grad_u, const float *grad_v,
+                         const int *faces, int num_vertices, double *grad_vertices,
+                         double *grad_org, double *grad_dir) {
+    g_err.clear();
+    if (nrays < 0 || num_vertices < 0) { g_err = "invalid argument"; return 1; }
+    std::fill(grad_vertices, grad_vertices + 3 * (size_t)num_vertices, 0.0);
+    for (int r = 0; r < nrays; ++r) {
+        const int id = hits[r].id;
+        double go[3] = {0, 0, 0}, gd[3] = {0, 0, 0};
+        if (id >= 0 && id < ntris) {
+            double v[3][3], o[3], d[3], gv3[3][3];
+            for (int c = 0; c < 3; ++c) {
+                v[0][c] = p0[4 * (size_t)id + c]; v[1][c] = p1[4 * (size_t)id + c]; v[2][c] = p2[4 * (size_t)id + c];
+                o[c] = org[4 * (size_t)r + c]; d[c] = dir[4 * (size_t)r + c];
+            }
+            orc_mt64_adjoint(v[0], v[1], v[2], o, d, grad_t ? grad_t[r] : 0.0, grad_u ? grad_u[r] : 0.0,
+                             grad_v ? grad_v[r] : 0.0, gv3[0], gv3[1], gv3[2], go, gd);
+            for (int k = 0; k < 3; ++k) {
+                const int vid = faces ? faces[3 * id + k] : 3 * id + k;
+                if (vid < 0 || vid >= num_vertices) { g_err = "vertex id out of range"; return 1; }
+                for (int c = 0; c < 3; ++c) grad_vertices[3 * (size_t)vid + c] += gv3[k][c];
+            }
+        }
+        if (grad_org) for (int c = 0; c < 3; ++c) grad_org[3 * r + c] = go[c];
+        if (grad_dir) for (int c = 0; c < 3; ++c) grad_dir[3 * r + c] = gd[c];
+    }
+    return 0;
+}
+
 }  // extern "C"

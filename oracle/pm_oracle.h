@@ -130,6 +130,23 @@ void orc_mt64_adjoint(const double *v0, const double *v1, const double *v2,
                       double gu, double gv, double *g_v0, double *g_v1,
                       double *g_v2, double *g_o, double *g_d);
 
+/* Backward of the intersection over a ray batch (see pm_trace_backward in
+ * include/pmkd.h).  FP32 restates the product's operation order exactly and
+ * sums per-vertex entries sequentially in (ray, corner) order; FP64 uses
+ * orc_mt64_adjoint and the same summation order in double.  faces NULL =
+ * vertex id 3*triangle + corner.  Outputs are overwritten (grad_vertices
+ * zeroed first); grad_org/grad_dir optional. */
+int orc_trace_backward32(const float *p0, const float *p1, const float *p2, int ntris,
+                         const float *org, const float *dir, const orc_hit *hits, int nrays,
+                         const float *grad_t, const float *grad_u, const float *grad_v,
+                         const int *faces, int num_vertices, float *grad_vertices,
+                         float *grad_org, float *grad_dir);
+int orc_trace_backward64(const float *p0, const float *p1, const float *p2, int ntris,
+                         const float *org, const float *dir, const orc_hit *hits, int nrays,
+                         const float *grad_t, const float *grad_u, const float *grad_v,
+                         const int *faces, int num_vertices, double *grad_vertices,
+                         double *grad_org, double *grad_dir);
+
 #ifdef __cplusplus
 }
 #endif

@@ -104,6 +104,10 @@ def lib():
                                    C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                    C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
         L.orc_sah_node.restype = C.c_int
+        L.orc_trace_backward32.argtypes = [P, P, P, C.c_int, P, P, P, C.c_int, P, P, P, P, C.c_int, P, P, P]
+        L.orc_trace_backward32.restype = C.c_int
+        L.orc_trace_backward64.argtypes = [P, P, P, C.c_int, P, P, P, C.c_int, P, P, P, P, C.c_int, P, P, P]
+        L.orc_trace_backward64.restype = C.c_int
         L.orc_tri_area.argtypes = [P, P, P]
         L.orc_tri_area.restype = C.c_float
         _lib = L
@@ -243,3 +247,28 @@ def sah_node(bmin, bmax, areas, bitmap, node_area, traversal_cost=24.0, leaf_max
 def tri_area(v0, v1, v2) -> float:
     a = [_c(x, np.float32) for x in (v0, v1, v2)]
     return float(lib().orc_tri_area(*[_ptr(x) for x in a]))
+
+
+def trace_backward(tris, org, dirs, hits, grad_t=None, grad_u=None, grad_v=None, faces=None,
+                   num_vertices=None, fp64=False):
+    """per-vertex (and per-ray) gradients of t/u/v for the given hit records"""
+    p0, p1, p2 = (_c(t, np.float32) for t in tris)
+    org = _c(org, np.float32)
+    dirs = _c(dirs, np.float32)
+    hits = np.ascontiguousarray(hits)
+    n = org.shape[0]
+    nt = p0.shape[0]
+    if num_vertices is None:
+        num_vertices = 3 * nt if faces is None else int(np.max(faces)) + 1
+    dt = np.float64 if fp64 else np.float32
+    gvtx = np.zeros((num_vertices, 3), dt)
+    go = np.zeros((n, 3), dt)
+    gd = np.zeros((n, 3), dt)
+    gs = [None if g is None else _c(g, np.float32) for g in (grad_t, grad_u, grad_v)]
+    fc = None if faces is None else _c(faces, np.int32)
+    fn = lib().orc_trace_backward64 if fp64 else lib().orc_trace_backward32
+    rc = fn(_ptr(p0), _ptr(p1), _ptr(p2), nt, _ptr(org), _ptr(dirs), _ptr(hits), n,
+            *[_ptr(g) for g in gs], _ptr(fc), num_vertices, _ptr(gvtx), _ptr(go), _ptr(gd))
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return gvtx, go, gd
