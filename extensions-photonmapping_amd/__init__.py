@@ -30,7 +30,8 @@ BRUTE, KD_RESTART, SHORT_STACK = 0, 1, 2
 # every symbol include/pmkd.h declares
 EXPORTS = ("pm_build_params_default", "pm_trace_params_default", "pm_create", "pm_destroy",
            "pm_last_error", "pm_build", "pm_set_geometry", "pm_tree_view", "pm_set_tree",
-           "pm_tree_stats", "pm_trace", "pm_photon_bounce", "pm_add_mesh", "pm_trace_backward")
+           "pm_tree_stats", "pm_trace", "pm_photon_bounce", "pm_add_mesh", "pm_trace_backward",
+           "pm_build_batched", "pm_trace_batched", "pm_osc_surface", "pm_deform_radial")
 
 
 class PmError(RuntimeError):
@@ -61,7 +62,8 @@ class Tree(C.Structure):
                 ("num_prim_indices", C.c_int), ("prim_indices", C.c_void_p),
                 ("num_leaf_ids", C.c_int), ("leaf_ids", C.c_void_p), ("p0", C.c_void_p),
                 ("p1", C.c_void_p), ("p2", C.c_void_p), ("num_tris", C.c_int),
-                ("upper_levels", C.c_int), ("lower_levels", C.c_int), ("has_boxes", C.c_int)]
+                ("upper_levels", C.c_int), ("lower_levels", C.c_int), ("has_boxes", C.c_int),
+                ("num_roots", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -102,9 +104,15 @@ def load_library(path: str = LIB_PATH):
                                    C.POINTER(C.c_int), P]
     L.pm_add_mesh.argtypes = [P, P, I, P, I, P, P, P, P, I, P]
     L.pm_trace_backward.argtypes = [P, P, P, P, I, P, P, P, P, I, P, P, P, P]
+    L.pm_build_batched.argtypes = [P, P, P, P, I, P, I, C.POINTER(BuildParams), P]
+    L.pm_trace_batched.argtypes = [P, I, P, P, P, I, P, C.POINTER(TraceParams), P, P]
+    F = C.c_float
+    L.pm_osc_surface.argtypes = [P, P, P, I, F, F, F, F, P]
+    L.pm_deform_radial.argtypes = [P, P, I, F, P, P, P]
     for n in ("pm_create", "pm_destroy", "pm_build", "pm_set_geometry", "pm_tree_view",
               "pm_set_tree", "pm_tree_stats", "pm_trace", "pm_photon_bounce", "pm_add_mesh",
-              "pm_trace_backward"):
+              "pm_trace_backward", "pm_build_batched", "pm_trace_batched", "pm_osc_surface",
+              "pm_deform_radial"):
         getattr(L, n).restype = C.c_int
     _lib = L
     return L
@@ -207,6 +215,32 @@ class Context:
         self._check(self._L.pm_build(self.h, _dev_ptr(p0), _dev_ptr(p1), _dev_ptr(p2), n,
                                      C.byref(params), _stream_ptr(stream)))
 
+    def build_batched(self, p0, p1, p2, mesh_offsets, params: BuildParams | None = None, stream=None):
+        """one kd-tree per mesh (triangles of mesh m are rows
+        [mesh_offsets[m], mesh_offsets[m+1]) of p0/p1/p2), built together as a
+        forest whose root m is node m"""
+        off = np.ascontiguousarray(mesh_offsets, dtype=np.int32)
+        n = p0.shape[0]
+        self._geom = (p0, p1, p2)
+        params = params or build_params()
+        self._check(self._L.pm_build_batched(self.h, _dev_ptr(p0), _dev_ptr(p1), _dev_ptr(p2), n,
+                                             off.ctypes.data, off.shape[0] - 1, C.byref(params),
+                                             _stream_ptr(stream)))
+
+    def trace_batched(self, algo: int, org, dirs, ray_mesh, params: TraceParams | None = None,
+                      out=None, counters=None, stream=None):
+        """trace ray r in the tree of mesh ray_mesh[r] (device int32)"""
+        torch = _torch()
+        n = org.shape[0]
+        if out is None:
+            out = torch.empty((n, 4), dtype=torch.float32, device=org.device)
+        params = params or trace_params()
+        cptr = _dev_ptr(counters) if counters is not None else None
+        self._check(self._L.pm_trace_batched(self.h, algo, _dev_ptr(org), _dev_ptr(dirs),
+                                             _dev_ptr(ray_mesh), n, _dev_ptr(out) if n else None,
+                                             C.byref(params), cptr, _stream_ptr(stream)))
+        return out
+
     def set_geometry(self, p0, p1, p2):
         self._geom = (p0, p1, p2)
         self._check(self._L.pm_set_geometry(self.h, _dev_ptr(p0), _dev_ptr(p1), _dev_ptr(p2),
@@ -240,7 +274,8 @@ class Context:
                     parent=fetch(t.parent, n, np.int32),
                     prim_indices=fetch(t.prim_indices, t.num_prim_indices, np.int32),
                     leaf_ids=fetch(t.leaf_ids, t.num_leaf_ids, np.int32),
-                    upper_levels=t.upper_levels, lower_levels=t.lower_levels)
+                    upper_levels=t.upper_levels, lower_levels=t.lower_levels,
+                    num_roots=t.num_roots)
 
     def set_tree(self, arrays: dict, p0, p1, p2, has_boxes: bool = True, stream=None):
         """upload a host tree (reference SoA layout) to trace it"""
@@ -257,6 +292,7 @@ class Context:
         t.p0, t.p1, t.p2 = _dev_ptr(p0), _dev_ptr(p1), _dev_ptr(p2)
         t.num_tris = p0.shape[0]
         t.has_boxes = int(has_boxes)
+        t.num_roots = int(arrays.get("num_roots", 1))
         self._geom = (p0, p1, p2)
         self._check(self._L.pm_set_tree(self.h, C.byref(t), _stream_ptr(stream)))
 
@@ -290,6 +326,25 @@ class Context:
                                         m.ctypes.data if m is not None else None,
                                         _dev_ptr(p0), _dev_ptr(p1), _dev_ptr(p2), offset,
                                         _stream_ptr(stream)))
+
+    def osc_surface(self, vertices, normals, detail: int, height: float, time: float,
+                    amplitude: float = 0.2, frequency: float = 0.3, stream=None):
+        """one OscCUDAMeshNode animation step on a (detail+1)^2 vertex grid"""
+        self._check(self._L.pm_osc_surface(self.h, _dev_ptr(vertices),
+                                           _dev_ptr(normals) if normals is not None else None,
+                                           detail, height, time, amplitude, frequency,
+                                           _stream_ptr(stream)))
+
+    def deform_radial(self, unit, frame: float, wave=(0.05, 8.0, 0.3, 6.0, 0.2), out=None,
+                      stream=None):
+        """unit-sphere vertices scaled by the radial wave at ``frame``"""
+        torch = _torch()
+        if out is None:
+            out = torch.empty_like(unit)
+        w = np.ascontiguousarray(wave, dtype=np.float32)
+        self._check(self._L.pm_deform_radial(self.h, _dev_ptr(unit), unit.shape[0], frame,
+                                             w.ctypes.data, _dev_ptr(out), _stream_ptr(stream)))
+        return out
 
     def trace_backward(self, org, dirs, hits, grad_t=None, grad_u=None, grad_v=None, faces=None,
                        num_vertices=None, grad_vertices=None, grad_org=None, grad_dir=None,

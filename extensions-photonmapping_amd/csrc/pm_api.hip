@@ -99,7 +99,22 @@ int pm_build(pm_ctx *ctx, const float *p0, const float *p1, const float *p2, int
         if (params) prm = *params; else pm_build_params_default(&prm);
         check_build_params(prm);
         pmkd::build_tree(ctx, reinterpret_cast<const float4 *>(p0), reinterpret_cast<const float4 *>(p1),
-                         reinterpret_cast<const float4 *>(p2), n, prm, (hipStream_t)stream);
+                         reinterpret_cast<const float4 *>(p2), n, nullptr, 1, prm, (hipStream_t)stream);
+    });
+}
+
+int pm_build_batched(pm_ctx *ctx, const float *p0, const float *p1, const float *p2, int n,
+                     const int *mesh_offsets, int num_meshes, const pm_build_params *params,
+                     void *stream) {
+    return guarded(ctx, [&] {
+        if (n < 0 || (n > 0 && (!p0 || !p1 || !p2))) throw Error{PM_ERR_ARG, "invalid geometry"};
+        if (num_meshes < 1 || !mesh_offsets) throw Error{PM_ERR_ARG, "need at least one mesh and its offsets"};
+        pm_build_params prm;
+        if (params) prm = *params; else pm_build_params_default(&prm);
+        check_build_params(prm);
+        pmkd::build_tree(ctx, reinterpret_cast<const float4 *>(p0), reinterpret_cast<const float4 *>(p1),
+                         reinterpret_cast<const float4 *>(p2), n, mesh_offsets, num_meshes, prm,
+                         (hipStream_t)stream);
     });
 }
 
@@ -139,6 +154,7 @@ int pm_tree_view(pm_ctx *ctx, pm_tree *out) {
         t.upper_levels = ctx->upper_levels;
         t.lower_levels = ctx->lower_levels;
         t.has_boxes = ctx->has_boxes ? 1 : 0;
+        t.num_roots = ctx->num_roots;
     });
 }
 
@@ -151,6 +167,8 @@ int pm_set_tree(pm_ctx *ctx, const pm_tree *h, void *stream) {
             throw Error{PM_ERR_ARG, "invalid geometry"};
         hipStream_t s = (hipStream_t)stream;
         const size_t n = (size_t)h->num_nodes;
+        const int roots = h->num_roots > 0 ? h->num_roots : 1;
+        if ((size_t)roots > n) throw Error{PM_ERR_ARG, "more roots than nodes"};
         ctx->has_tree = false;
         // every index the tracers follow must stay inside its array
         for (size_t i = 0; i < n; ++i) {
@@ -191,6 +209,8 @@ int pm_set_tree(pm_ctx *ctx, const pm_tree *h, void *stream) {
         ctx->p1 = reinterpret_cast<const float4 *>(h->p1);
         ctx->p2 = reinterpret_cast<const float4 *>(h->p2);
         ctx->ntris = h->num_tris;
+        ctx->num_roots = roots;
+        ctx->mesh_offsets.clear();
         pmkd::build_trace_view(ctx, s);
         PM_HIP(hipStreamSynchronize(s));
         ctx->has_tree = true;
@@ -217,7 +237,8 @@ int pm_tree_stats(pm_ctx *ctx, pm_stats *out, void *stream) {
         st.lower_levels = ctx->lower_levels;
         st.num_prim_indices = ctx->num_prim_indices;
         st.num_upper_leaves = ctx->num_leaf_ids;
-        std::vector<std::pair<int, int>> stk{{0, 0}};
+        std::vector<std::pair<int, int>> stk;
+        for (int r = ctx->num_roots - 1; r >= 0; --r) stk.push_back({r, 0});
         double depth_sum = 0.0;
         long long guard = 0;
         while (!stk.empty()) {
@@ -260,7 +281,23 @@ int pm_trace(pm_ctx *ctx, int algo, const float *org, const float *dir, int nray
             throw Error{PM_ERR_NO_TREE, "no tree: call pm_build or pm_set_tree first"};
         }
         pmkd::trace_rays(ctx, algo, reinterpret_cast<const float4 *>(org), reinterpret_cast<const float4 *>(dir),
-                         nrays, out, prm, reinterpret_cast<int4 *>(counters), (hipStream_t)stream);
+                         nrays, nullptr, out, prm, reinterpret_cast<int4 *>(counters), (hipStream_t)stream);
+    });
+}
+
+int pm_trace_batched(pm_ctx *ctx, int algo, const float *org, const float *dir, const int *ray_mesh,
+                     int nrays, pm_hit *out, const pm_trace_params *params, int *counters,
+                     void *stream) {
+    return guarded(ctx, [&] {
+        if (nrays < 0 || (nrays > 0 && (!org || !dir || !out || !ray_mesh))) throw Error{PM_ERR_ARG, "invalid rays"};
+        if (algo != PM_TRACE_KD_RESTART && algo != PM_TRACE_SHORT_STACK)
+            throw Error{PM_ERR_ARG, "batched tracing takes KD_RESTART or SHORT_STACK"};
+        pm_trace_params prm;
+        if (params) prm = *params; else pm_trace_params_default(&prm);
+        if (prm.stack_size < 1 || prm.stack_size > 16) throw Error{PM_ERR_ARG, "stack_size must be in [1, 16]"};
+        if (!ctx->has_tree) throw Error{PM_ERR_NO_TREE, "no tree: call pm_build_batched first"};
+        pmkd::trace_rays(ctx, algo, reinterpret_cast<const float4 *>(org), reinterpret_cast<const float4 *>(dir),
+                         nrays, ray_mesh, out, prm, reinterpret_cast<int4 *>(counters), (hipStream_t)stream);
     });
 }
 
@@ -289,6 +326,24 @@ int pm_add_mesh(pm_ctx *ctx, const float *vertices, int nv, const unsigned *indi
         float4 *q2 = reinterpret_cast<float4 *>(p2) + offset;
         pmkd::add_mesh(ctx, reinterpret_cast<const float4 *>(vertices), nv, indices, nidx, model, q0, q1, q2,
                        (hipStream_t)stream);
+    });
+}
+
+int pm_osc_surface(pm_ctx *ctx, float *vertices, float *normals, int detail, float height,
+                   float time, float amplitude, float frequency, void *stream) {
+    return guarded(ctx, [&] {
+        if (detail < 0 || detail >= 46340 || !vertices) throw Error{PM_ERR_ARG, "invalid grid"};
+        pmkd::osc_surface(reinterpret_cast<float4 *>(vertices), reinterpret_cast<float4 *>(normals), detail + 1,
+                          height, time, amplitude, frequency, (hipStream_t)stream);
+    });
+}
+
+int pm_deform_radial(pm_ctx *ctx, const float *unit, int nv, float frame, const float *wave,
+                     float *out, void *stream) {
+    return guarded(ctx, [&] {
+        if (nv < 0 || !wave || (nv > 0 && (!unit || !out))) throw Error{PM_ERR_ARG, "invalid arguments"};
+        pmkd::radial_wave(reinterpret_cast<const float4 *>(unit), nv, frame, wave, reinterpret_cast<float4 *>(out),
+                          (hipStream_t)stream);
     });
 }
 

@@ -136,11 +136,51 @@ __global__ void init_keys_kernel(int *keys, int count) {
     }
 }
 
-__global__ void init_root_kernel(NodePtrs N, int ntris) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {   // TriangleMapUpperCreator.cu:128-137
-        zero_node(N, 0);
-        N.bitmap[0] = ntris;
+// TriangleMapUpperCreator.cu:128-137, one root per mesh: root m covers
+// triangles [off[m], off[m+1]) and is its own parent
+__global__ void init_roots_kernel(NodePtrs N, const int *__restrict__ off, int M) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    zero_node(N, m);
+    N.prim_index[m] = off[m];
+    N.bitmap[m] = off[m + 1] - off[m];
+    N.parent[m] = m;
+}
+
+// root boxes of a forest: each triangle's box joins its mesh's root keys
+__global__ __launch_bounds__(BS) void root_reduce_kernel(const float4 *__restrict__ rmin,
+                                                         const float4 *__restrict__ rmax, int n,
+                                                         const int *__restrict__ off, int M,
+                                                         int *__restrict__ keys) {
+    const int i = blockIdx.x * BS + threadIdx.x;
+    const bool valid = i < n;
+    int m = 0;
+    if (valid) {
+        int lo = 0, hi = M - 1;                      // last mesh with off[m] <= i
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (off[mid] <= i) lo = mid; else hi = mid - 1;
+        }
+        m = lo;
     }
+    const int m0 = __shfl(m, 0, 64);
+    const unsigned long long act = __ballot(valid);
+    const bool uniform = __ballot(valid && m != m0) == 0ull && act != 0ull;
+    int k0 = fkey(F_INF), k1 = k0, k2 = k0, k3 = fkey(-F_INF), k4 = k3, k5 = k3;
+    if (valid) {
+        const float4 a = rmin[i], b = rmax[i];
+        k0 = fkey(a.x); k1 = fkey(a.y); k2 = fkey(a.z); k3 = fkey(b.x); k4 = fkey(b.y); k5 = fkey(b.z);
+    }
+    if (uniform) {
+        k0 = wave_min(k0); k1 = wave_min(k1); k2 = wave_min(k2);
+        k3 = wave_max(k3); k4 = wave_max(k4); k5 = wave_max(k5);
+        if (__lane_id() != 0) return;
+    } else if (!valid) {
+        return;
+    }
+    int *k = keys + 6 * m;
+    atomicMin(k + 0, k0); atomicMin(k + 1, k1); atomicMin(k + 2, k2);
+    atomicMax(k + 3, k3); atomicMax(k + 4, k4); atomicMax(k + 5, k5);
 }
 
 // CalcPrimitiveAabb (Kernels/TriangleUpper.h:10-28) + root box reduction
@@ -164,6 +204,7 @@ __global__ __launch_bounds__(BS) void tri_box_kernel(const float4 *__restrict__ 
         k0 = min(k0, fkey(mnx)); k1 = min(k1, fkey(mny)); k2 = min(k2, fkey(mnz));
         k3 = max(k3, fkey(mxx)); k4 = max(k4, fkey(mxy)); k5 = max(k5, fkey(mxz));
     }
+    if (!keys) return;                           // forest: roots reduced per mesh
     k0 = wave_min(k0); k1 = wave_min(k1); k2 = wave_min(k2);
     k3 = wave_max(k3); k4 = wave_max(k4); k5 = wave_max(k5);
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -728,7 +769,7 @@ void scan_exclusive(const int *in, int *out, int n, BuildScratch &sc, hipStream_
 }
 
 void build_tree(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2, int N,
-                const pm_build_params &prm, hipStream_t s) {
+                const int *mesh_offsets, int M, const pm_build_params &prm, hipStream_t s) {
     BuildScratch &sc = ctx->scratch;
     NodeArrays &nodes = ctx->nodes;
     const int LS = prm.lower_size;
@@ -737,21 +778,44 @@ void build_tree(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P
     ctx->p0 = P0; ctx->p1 = P1; ctx->p2 = P2; ctx->ntris = N;
     ctx->level_refs.clear();
 
-    int nodes_size = 1, final_refs = 0, nleaf = 0;
-    nodes.reserve(1024, s, 0);
+    // one root per mesh (a forest); a single mesh is M = 1 over [0, N)
+    std::vector<int> off_h(M + 1);
+    if (mesh_offsets) {
+        std::copy(mesh_offsets, mesh_offsets + M + 1, off_h.begin());
+    } else {
+        off_h[0] = 0;
+        off_h[1] = N;
+    }
+    for (int m = 0; m < M; ++m)
+        if (off_h[m] > off_h[m + 1]) throw Error{PM_ERR_ARG, "mesh offsets must be non-decreasing"};
+    if (off_h[0] != 0 || off_h[M] != N) throw Error{PM_ERR_ARG, "mesh offsets must span [0, n]"};
+    sc.mesh_off.reserve((size_t)M + 1, s);
+    PM_HIP(hipMemcpyAsync(sc.mesh_off.ptr, off_h.data(), (M + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+    PM_HIP(hipStreamSynchronize(s));
+    ctx->mesh_offsets = off_h;
+    ctx->num_roots = M;
+
+    int nodes_size = M, final_refs = 0, nleaf = 0;
+    nodes.reserve(std::max(1024, 2 * M), s, 0);
     sc.ref_min[0].reserve((size_t)N + 1, s); sc.ref_max[0].reserve((size_t)N + 1, s);
-    sc.keys[0].reserve(6, s);
-    init_root_kernel<<<1, 64, 0, s>>>(node_ptrs(ctx), N);
+    sc.keys[0].reserve(6 * (size_t)M, s);
+    init_roots_kernel<<<blocks_for(M, BS), BS, 0, s>>>(node_ptrs(ctx), sc.mesh_off.ptr, M);
     PM_LAUNCH_CHECK();
-    init_keys_kernel<<<1, 64, 0, s>>>(sc.keys[0].ptr, 1);
+    init_keys_kernel<<<blocks_for(M, BS), BS, 0, s>>>(sc.keys[0].ptr, M);
     PM_LAUNCH_CHECK();
     if (N > 0) {
         const int blocks = std::min(blocks_for(N, BS), 2048);
-        tri_box_kernel<<<blocks, BS, 0, s>>>(P0, P1, P2, N, sc.ref_min[0].ptr, sc.ref_max[0].ptr, sc.keys[0].ptr);
+        tri_box_kernel<<<blocks, BS, 0, s>>>(P0, P1, P2, N, sc.ref_min[0].ptr, sc.ref_max[0].ptr,
+                                              M == 1 ? sc.keys[0].ptr : nullptr);
         PM_LAUNCH_CHECK();
+        if (M > 1) {
+            root_reduce_kernel<<<blocks_for(N, BS), BS, 0, s>>>(sc.ref_min[0].ptr, sc.ref_max[0].ptr, N,
+                                                                sc.mesh_off.ptr, M, sc.keys[0].ptr);
+            PM_LAUNCH_CHECK();
+        }
     }
     int cur = 0;          // ping-pong index of the level's ref arrays and keys
-    int active_index = 0, A = 1, R = N, level = 0;
+    int active_index = 0, A = M, R = N, level = 0;
     int hv[8];
     while (A > 0) {
         if (level >= max_levels)

@@ -1,4 +1,5 @@
-// pm_geometry.hip -- geometry staging: indexed mesh -> triangle SoA.
+// pm_geometry.hip -- geometry staging: indexed mesh -> triangle SoA, and
+// the per-frame vertex deformers that feed a rebuild.
 //
 // AddMeshKernel (float4 vertex variant, GeometryList.cu:197-232): triangle
 // id = index triple 3*id..3*id+2, each vertex transformed by the model
@@ -39,6 +40,47 @@ __global__ __launch_bounds__(TB) void add_mesh_kernel(const float4 *__restrict__
         out[k][id] = make_float4(dot4(m.r[0], v[k]), dot4(m.r[1], v[k]), dot4(m.r[2], v[k]), dot4(m.r[3], v[k]));
 }
 
+// UpdateSurface (OscCUDAMeshNode.cu:72-93): vertex (x, y) of a d x d grid,
+// stored at y * d + x, gets height + A cos(f x + time) + A sin(f y + time)
+// as its y coordinate and the normal (-A f sin(f x + time),
+// 1 - nx^2 - nz^2, A f cos(f y + time), 0).
+__global__ __launch_bounds__(TB) void osc_surface_kernel(float4 *__restrict__ vert,
+                                                         float4 *__restrict__ norm, int d,
+                                                         float height, float time, float amp,
+                                                         float freq) {
+    const int id = blockIdx.x * TB + threadIdx.x;
+    if (id >= d * d) return;
+    const int x = id % d, y = id / d;
+    const float ax = freq * (float)x + time, ay = freq * (float)y + time;
+    float4 v = vert[id];
+    v.y = height + amp * cosf(ax);
+    v.y += amp * sinf(ay);
+    vert[id] = v;
+    if (norm) {
+        float4 n;
+        n.x = -amp * sinf(ax) * freq;
+        n.z = amp * cosf(ay) * freq;
+        n.y = 1.0f - n.x * n.x - n.z * n.z;
+        n.w = 0.0f;
+        norm[id] = n;
+    }
+}
+
+// radial wave on unit-sphere vertices: r = 1 + A sin(kt theta + wt frame)
+// cos(kp phi + wp frame) with theta = acos(z), phi = atan2(y, x)
+__global__ __launch_bounds__(TB) void radial_wave_kernel(const float4 *__restrict__ unit, int nv,
+                                                         float frame, float amp, float kt, float wt,
+                                                         float kp, float wp,
+                                                         float4 *__restrict__ out) {
+    const int id = blockIdx.x * TB + threadIdx.x;
+    if (id >= nv) return;
+    const float4 u = unit[id];
+    const float theta = acosf(fminf(fmaxf(u.z, -1.0f), 1.0f));
+    const float phi = atan2f(u.y, u.x);
+    const float r = 1.0f + amp * sinf(kt * theta + wt * frame) * cosf(kp * phi + wp * frame);
+    out[id] = make_float4(u.x * r, u.y * r, u.z * r, u.w);
+}
+
 __global__ void clear_flag_kernel(int *flag) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *flag = 0;
 }
@@ -67,6 +109,20 @@ void add_mesh(pm_ctx *ctx, const float4 *verts, int nv, const unsigned *idx, int
     PM_HIP(hipMemcpyAsync(sc.host.ptr, sc.totals.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
     PM_HIP(hipStreamSynchronize(s));
     if (sc.host.ptr[0]) throw Error{PM_ERR_ARG, "mesh index out of range of the vertex array"};
+}
+
+void osc_surface(float4 *vert, float4 *norm, int d, float height, float time, float amp, float freq,
+                 hipStream_t s) {
+    const long long n = (long long)d * d;
+    if (n == 0) return;
+    osc_surface_kernel<<<(int)((n + TB - 1) / TB), TB, 0, s>>>(vert, norm, d, height, time, amp, freq);
+    PM_LAUNCH_CHECK();
+}
+
+void radial_wave(const float4 *unit, int nv, float frame, const float *w, float4 *out, hipStream_t s) {
+    if (nv == 0) return;
+    radial_wave_kernel<<<(nv + TB - 1) / TB, TB, 0, s>>>(unit, nv, frame, w[0], w[1], w[2], w[3], w[4], out);
+    PM_LAUNCH_CHECK();
 }
 
 }  // namespace pmkd

@@ -99,6 +99,7 @@ struct TreeDev {
     const int4 *packed;        // one 16-byte record per node (trace view)
     const float4 *leaf_tris;   // 3 float4 per leaf slot: v0 (w = id bits), e1, e2
     int num_nodes;
+    int num_roots;             // roots are nodes 0..num_roots-1 (one per mesh)
 };
 
 // Build scratch (kept across builds; grow-only).
@@ -123,6 +124,7 @@ struct BuildScratch {
     DevBuf<unsigned> bwd_keys[2], bwd_vals[2];  // backward: radix sort ping-pong
     DevBuf<float> bwd_corner;                   // backward: 3 float3 per ray
     DevBuf<int> bwd_hist, bwd_offs;             // backward: radix histograms
+    DevBuf<int> mesh_off;                       // forest: triangle offsets per mesh (M+1)
     HostPinned host;
 };
 
@@ -146,16 +148,19 @@ struct pm_ctx {
     pmkd::DevBuf<float4> leaf_tris; // trace view: leaf-ordered triangles
     pmkd::BuildScratch scratch;
     std::vector<long long> level_refs;
+    int num_roots = 1;                 // meshes in the last build (roots 0..num_roots-1)
+    std::vector<int> mesh_offsets;     // triangle offsets per mesh (host copy)
 };
 
 namespace pmkd {
 // pm_build.hip
 void build_tree(pm_ctx *ctx, const float4 *p0, const float4 *p1, const float4 *p2, int n,
-                const pm_build_params &prm, hipStream_t s);
+                const int *mesh_offsets, int M, const pm_build_params &prm, hipStream_t s);
 // pm_trace.hip
 void build_trace_view(pm_ctx *ctx, hipStream_t s);
 void trace_rays(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int nrays,
-                pm_hit *out, const pm_trace_params &prm, int4 *counters, hipStream_t s);
+                const int *ray_mesh, pm_hit *out, const pm_trace_params &prm, int4 *counters,
+                hipStream_t s);
 // pm_photon.hip: returns the number of continuation rays
 int photon_bounce(pm_ctx *ctx, const float4 *org, const float4 *dir, const pm_hit *hits, int n,
                   int absorb_from, unsigned long long seed, float offset, float4 *out_org,
@@ -166,6 +171,9 @@ void trace_backward(pm_ctx *ctx, const float4 *org, const float4 *dir, const pm_
                     int num_vertices, float *grad_vertices, float *grad_org, float *grad_dir,
                     hipStream_t s);
 // pm_geometry.hip
+void osc_surface(float4 *vert, float4 *norm, int d, float height, float time, float amp, float freq,
+                 hipStream_t s);
+void radial_wave(const float4 *unit, int nv, float frame, const float *wave, float4 *out, hipStream_t s);
 void add_mesh(pm_ctx *ctx, const float4 *verts, int nv, const unsigned *idx, int nidx,
               const float *model, float4 *p0, float4 *p1, float4 *p2, hipStream_t s);
 // device-wide exclusive scan of n ints; out has n+1 entries (out[n] = total)

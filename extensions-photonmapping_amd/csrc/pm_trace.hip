@@ -84,23 +84,30 @@ __device__ __forceinline__ int test_leaf(const TreeDev &T, const int4 nd, int no
 
 template <bool SKIP, bool COUNT>
 __global__ __launch_bounds__(TB) void kd_restart_kernel(TreeDev T, const float4 *__restrict__ org,
-                                                        const float4 *__restrict__ dir, int n,
+                                                        const float4 *__restrict__ dir,
+                                                        const int *__restrict__ ray_root, int n,
                                                         pm_hit *__restrict__ out,
                                                         int4 *__restrict__ counters) {
     const int r = blockIdx.x * TB + threadIdx.x;
     if (r >= n) return;
     int4 cnt = make_int4(0, 0, 0, 0);
+    pm_hit res = miss_record();
+    const int root_id = ray_root ? ray_root[r] : 0;
+    if (root_id < 0 || root_id >= T.num_roots) {    // no such mesh: the ray misses
+        out[r] = res;
+        if (COUNT) counters[r] = cnt;
+        return;
+    }
     const f3 o = mk3(org[r]);
     const f3 d = adjust_dir(mk3(dir[r]));
     const f3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
     float th = 0.0f, tu = 0.0f, tv = 0.0f;
-    pm_hit res = miss_record();
-    const int4 root = T.packed[0];
+    const int4 root = T.packed[root_id];
     long long visits = 0;
     do {
         if (++visits > (long long)T.num_nodes + 1) break;
         float tNext = F_INF;
-        int node = 0;
+        int node = root_id;
         int4 nd = root;
         int steps = 0;
         bool bad = false;
@@ -132,7 +139,8 @@ __global__ __launch_bounds__(TB) void kd_restart_kernel(TreeDev T, const float4 
 
 template <int SMAX, bool SKIP, bool COUNT>
 __global__ __launch_bounds__(TB) void short_stack_kernel(TreeDev T, const float4 *__restrict__ org,
-                                                         const float4 *__restrict__ dir, int n,
+                                                         const float4 *__restrict__ dir,
+                                                         const int *__restrict__ ray_root, int n,
                                                          int S, pm_hit *__restrict__ out,
                                                          int4 *__restrict__ counters) {
     __shared__ int stk_node[SMAX][TB];
@@ -141,13 +149,19 @@ __global__ __launch_bounds__(TB) void short_stack_kernel(TreeDev T, const float4
     if (r >= n) return;
     const int lt = threadIdx.x;
     int4 cnt = make_int4(0, 0, 0, 0);
+    pm_hit res = miss_record();
+    const int root_id = ray_root ? ray_root[r] : 0;
+    if (root_id < 0 || root_id >= T.num_roots) {    // no such mesh: the ray misses
+        out[r] = res;
+        if (COUNT) counters[r] = cnt;
+        return;
+    }
     const f3 o = mk3(org[r]);
     const f3 d = adjust_dir(mk3(dir[r]));
     const f3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
     float th = 0.0f, tu = 0.0f, tv = 0.0f;
     int nxt = 0, count = 0;
-    pm_hit res = miss_record();
-    const int4 root = T.packed[0];
+    const int4 root = T.packed[root_id];
     long long visits = 0;
     do {
         if (++visits > (long long)T.num_nodes + 1) break;
@@ -155,7 +169,7 @@ __global__ __launch_bounds__(TB) void short_stack_kernel(TreeDev T, const float4
         float tNext;
         int4 nd;
         if (count == 0) {
-            node = 0; tNext = F_INF; nd = root;
+            node = root_id; tNext = F_INF; nd = root;
         } else {                                     // Pop, ShortStack.h:68-72
             nxt = (nxt == 0 ? S : nxt) - 1;
             count--;
@@ -282,22 +296,22 @@ __global__ void clear_int_kernel(int *p) {
 }
 
 template <int SMAX, bool SKIP, bool COUNT>
-void launch_ss(const TreeDev &T, const float4 *org, const float4 *dir, int n, int S, pm_hit *out,
-               int4 *cnt, hipStream_t s) {
+void launch_ss(const TreeDev &T, const float4 *org, const float4 *dir, const int *root, int n, int S,
+               pm_hit *out, int4 *cnt, hipStream_t s) {
     const int blocks = (n + TB - 1) / TB;
-    short_stack_kernel<SMAX, SKIP, COUNT><<<blocks, TB, 0, s>>>(T, org, dir, n, S, out, cnt);
+    short_stack_kernel<SMAX, SKIP, COUNT><<<blocks, TB, 0, s>>>(T, org, dir, root, n, S, out, cnt);
 }
 
 template <bool SKIP, bool COUNT>
-void dispatch(const TreeDev &T, int algo, const float4 *org, const float4 *dir, int n, int S,
-              pm_hit *out, int4 *cnt, hipStream_t s) {
+void dispatch(const TreeDev &T, int algo, const float4 *org, const float4 *dir, const int *root, int n,
+              int S, pm_hit *out, int4 *cnt, hipStream_t s) {
     const int blocks = (n + TB - 1) / TB;
     if (algo == PM_TRACE_KD_RESTART) {
-        kd_restart_kernel<SKIP, COUNT><<<blocks, TB, 0, s>>>(T, org, dir, n, out, cnt);
+        kd_restart_kernel<SKIP, COUNT><<<blocks, TB, 0, s>>>(T, org, dir, root, n, out, cnt);
     } else {
-        if (S <= 4) launch_ss<4, SKIP, COUNT>(T, org, dir, n, S, out, cnt, s);
-        else if (S <= 8) launch_ss<8, SKIP, COUNT>(T, org, dir, n, S, out, cnt, s);
-        else launch_ss<16, SKIP, COUNT>(T, org, dir, n, S, out, cnt, s);
+        if (S <= 4) launch_ss<4, SKIP, COUNT>(T, org, dir, root, n, S, out, cnt, s);
+        else if (S <= 8) launch_ss<8, SKIP, COUNT>(T, org, dir, root, n, S, out, cnt, s);
+        else launch_ss<16, SKIP, COUNT>(T, org, dir, root, n, S, out, cnt, s);
     }
 }
 
@@ -329,7 +343,8 @@ void build_trace_view(pm_ctx *ctx, hipStream_t s) {
 }
 
 void trace_rays(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int nrays,
-                pm_hit *out, const pm_trace_params &prm, int4 *counters, hipStream_t s) {
+                const int *ray_mesh, pm_hit *out, const pm_trace_params &prm, int4 *counters,
+                hipStream_t s) {
     if (nrays == 0) return;
     if (algo == PM_TRACE_BRUTE) {
         const int tris = prm.brute_cap > 0 ? std::min(ctx->ntris, prm.brute_cap) : ctx->ntris;
@@ -353,14 +368,15 @@ void trace_rays(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int
     T.packed = ctx->packed.ptr;
     T.leaf_tris = ctx->leaf_tris.ptr;
     T.num_nodes = ctx->num_nodes;
+    T.num_roots = ctx->num_roots;
     const bool skip = prm.leaf_skipping && ctx->has_boxes;
     const int S = prm.stack_size;
     if (skip) {
-        if (counters) dispatch<true, true>(T, algo, org, dir, nrays, S, out, counters, s);
-        else dispatch<true, false>(T, algo, org, dir, nrays, S, out, counters, s);
+        if (counters) dispatch<true, true>(T, algo, org, dir, ray_mesh, nrays, S, out, counters, s);
+        else dispatch<true, false>(T, algo, org, dir, ray_mesh, nrays, S, out, counters, s);
     } else {
-        if (counters) dispatch<false, true>(T, algo, org, dir, nrays, S, out, counters, s);
-        else dispatch<false, false>(T, algo, org, dir, nrays, S, out, counters, s);
+        if (counters) dispatch<false, true>(T, algo, org, dir, ray_mesh, nrays, S, out, counters, s);
+        else dispatch<false, false>(T, algo, org, dir, ray_mesh, nrays, S, out, counters, s);
     }
     PM_LAUNCH_CHECK();
 }

@@ -405,3 +405,149 @@ def config5(width=512, height=512, seed=5, subdiv=7):
     v_ref, v_pert, faces = perturbed_sphere(subdiv, seed=seed)
     org, dirs = pinhole_rays(width, height, (0.0, 0.0, 3.0), (0.0, 0.0, 0.0), fov_deg=45.0, seed=seed)
     return v_ref, v_pert, faces, (org, dirs)
+
+
+# --------------------------------------------------------------------------
+# C4: batched rebuild of many animated meshes
+# --------------------------------------------------------------------------
+def cube_sphere(k: int):
+    """Unit sphere from a cube whose six faces are k x k quad grids pushed
+    onto the sphere: 12 k^2 triangles, no vertex in more than 6 of them (no
+    pole fans).  Returns (unit vertices float64 [V, 3], faces int32 [F, 3])."""
+    k = max(1, int(k))
+    g = np.arange(k + 1)
+    u, v = np.meshgrid(g, g, indexing="ij")
+    u, v = u.reshape(-1), v.reshape(-1)
+    pts, quads = [], []
+    base = 0
+    q = (u < k) & (v < k)
+    a = np.nonzero(q)[0]
+    qi = np.stack([a, a + (k + 1), a + (k + 1) + 1, a + 1], axis=1)  # (u,v) (u+1,v) (u+1,v+1) (u,v+1)
+    for axis in range(3):
+        for side in (0, k):
+            c = np.zeros((u.shape[0], 3), dtype=np.int64)
+            c[:, axis] = side
+            c[:, (axis + 1) % 3] = u
+            c[:, (axis + 2) % 3] = v
+            pts.append(c)
+            quads.append(qi + base)
+            base += u.shape[0]
+    pts = np.concatenate(pts)
+    quads = np.concatenate(quads)
+    uniq, inv = np.unique(pts, axis=0, return_inverse=True)
+    inv = inv.reshape(-1)
+    quads = inv[quads]
+    F = np.concatenate([quads[:, [0, 1, 2]], quads[:, [0, 2, 3]]])
+    V = 2.0 * uniq.astype(np.float64) / k - 1.0
+    V /= np.linalg.norm(V, axis=1, keepdims=True)
+    # outward winding
+    n = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
+    flip = np.sum(n * (V[F[:, 0]] + V[F[:, 1]] + V[F[:, 2]]), axis=1) < 0
+    F[flip] = F[flip][:, [0, 2, 1]]
+    return V, F.astype(np.int32)
+
+
+# radial displacement r = 1 + A sin(8 theta + 0.3 f) cos(6 phi + 0.2 f)
+WAVE = dict(amplitude=0.05, k_theta=8.0, w_theta=0.3, k_phi=6.0, w_phi=0.2)
+
+
+def displace_sphere64(unit: np.ndarray, frame: float) -> np.ndarray:
+    """FP64 restatement of the per-frame sphere deformer: unit vertices
+    (float32 [V, 4] or [V, 3]) scaled by the radial wave at frame f."""
+    u = np.asarray(unit)[:, :3].astype(np.float64)
+    theta = np.arccos(np.clip(u[:, 2], -1.0, 1.0))
+    phi = np.arctan2(u[:, 1], u[:, 0])
+    w = WAVE
+    r = 1.0 + w["amplitude"] * np.sin(w["k_theta"] * theta + w["w_theta"] * frame) * \
+        np.cos(w["k_phi"] * phi + w["w_phi"] * frame)
+    return u * r[:, None]
+
+
+def c4_sizes(num_meshes: int = 256, seed: int = 4, lo: float = 1e3, hi: float = 2e5):
+    """triangle counts n_k = round(exp U(ln lo, ln hi)) and the cube-sphere
+    grid k with 12 k^2 closest to n_k"""
+    r = Pcg32(seed, P_MESH, np.arange(num_meshes, dtype=np.uint64))
+    n = np.rint(np.exp(math.log(lo) + r.uniform() * (math.log(hi) - math.log(lo))))
+    k = np.maximum(1, np.rint(np.sqrt(n / 12.0))).astype(np.int64)
+    return n.astype(np.int64), k
+
+
+def config4(num_meshes: int = 256, rays_per_mesh: int = 65536, seed: int = 4, lo: float = 1e3,
+            hi: float = 2e5):
+    """C4: ``num_meshes`` cube spheres of log-uniform size with their unit
+    vertices concatenated, and ``rays_per_mesh`` rays per mesh from radius 2
+    towards a uniform point of the unit ball.  Returns a dict with
+      unit      float32 [V, 4] unit vertices (w = 1), meshes concatenated
+      faces     int32 [F, 3] global vertex indices, meshes concatenated
+      vert_off  int64 [M+1] vertex offsets, tri_off int32 [M+1] triangle offsets
+      org, dirs float32 [M * rays_per_mesh, 4]; ray_mesh int32 mesh of each ray"""
+    _, ks = c4_sizes(num_meshes, seed, lo, hi)
+    cache = {}
+    units, faces, voff, toff = [], [], [0], [0]
+    for k in ks:
+        if k not in cache:
+            cache[k] = cube_sphere(int(k))
+        V, F = cache[k]
+        units.append(V)
+        faces.append(F + voff[-1])
+        voff.append(voff[-1] + V.shape[0])
+        toff.append(toff[-1] + F.shape[0])
+    unit = _pack(np.concatenate(units), 1.0)
+    n = num_meshes * rays_per_mesh
+    r = Pcg32(seed, P_RAYS, np.arange(n, dtype=np.uint64))
+    z = 1.0 - 2.0 * r.uniform()
+    ph = 2.0 * math.pi * r.uniform()
+    s = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    o = 2.0 * np.stack([s * np.cos(ph), s * np.sin(ph), z], axis=1)
+    rad = np.cbrt(r.uniform())
+    z2 = 1.0 - 2.0 * r.uniform()
+    ph2 = 2.0 * math.pi * r.uniform()
+    s2 = np.sqrt(np.maximum(0.0, 1.0 - z2 * z2))
+    t = rad[:, None] * np.stack([s2 * np.cos(ph2), s2 * np.sin(ph2), z2], axis=1)
+    d = t - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    ray_mesh = np.repeat(np.arange(num_meshes, dtype=np.int32), rays_per_mesh)
+    return dict(unit=unit, faces=np.concatenate(faces).astype(np.int32),
+                vert_off=np.asarray(voff, dtype=np.int64), tri_off=np.asarray(toff, dtype=np.int32),
+                org=_pack(o, 1.0), dirs=_pack(d, 0.0), ray_mesh=ray_mesh, k=ks)
+
+
+# --------------------------------------------------------------------------
+# oscillating grid (OscCUDAMeshNode)
+# --------------------------------------------------------------------------
+def osc_grid(width: float, depth: float, height: float, detail: int):
+    """OscCUDAMeshNode::Init (OscCUDAMeshNode.cu:26-68): (detail+1)^2 float4
+    vertices, vertex (i, j) at i + j*(detail+1) = (i w/detail - w/2, height,
+    j d/detail - d/2, 1) in FP32, and two triangles per grid quad."""
+    d = detail + 1
+    ux = np.float32(width) / np.float32(detail)
+    uy = np.float32(depth) / np.float32(detail)
+    j, i = np.meshgrid(np.arange(d), np.arange(d), indexing="ij")   # index = i + j*d
+    i, j = i.reshape(-1), j.reshape(-1)
+    V = np.empty((d * d, 4), np.float32)
+    V[:, 0] = i.astype(np.float32) * ux - np.float32(width) * np.float32(0.5)
+    V[:, 1] = np.float32(height)
+    V[:, 2] = j.astype(np.float32) * uy - np.float32(depth) * np.float32(0.5)
+    V[:, 3] = 1.0
+    m, n = np.meshgrid(np.arange(detail), np.arange(detail), indexing="ij")
+    m, n = m.reshape(-1), n.reshape(-1)
+    q = np.stack([m + n * d, m + (n + 1) * d, m + 1 + n * d,
+                  m + 1 + n * d, m + (n + 1) * d, m + 1 + (n + 1) * d], axis=1)
+    return V, q.reshape(-1).astype(np.uint32)
+
+
+def osc_surface64(V: np.ndarray, detail: int, height: float, time: float, amplitude: float = 0.2,
+                  frequency: float = 0.3):
+    """FP64 restatement of UpdateSurface (OscCUDAMeshNode.cu:72-93):
+    returns (vertices with the new y, normals)"""
+    d = detail + 1
+    idx = np.arange(d * d)
+    x, y = (idx % d).astype(np.float64), (idx // d).astype(np.float64)
+    ax, ay = frequency * x + time, frequency * y + time
+    out = np.asarray(V, np.float64).copy()
+    out[:, 1] = height + amplitude * np.cos(ax) + amplitude * np.sin(ay)
+    n = np.zeros((d * d, 4))
+    n[:, 0] = -amplitude * np.sin(ax) * frequency
+    n[:, 2] = amplitude * np.cos(ay) * frequency
+    n[:, 1] = 1.0 - n[:, 0] ** 2 - n[:, 2] ** 2
+    return out, n

@@ -113,7 +113,9 @@ typedef struct {
  *   children[2n]        (left, right); left is the side below split_pos
  *   surface_area[n], parent[n]
  *   prim_indices[m]     triangle ids (index into p0/p1/p2)
- *   leaf_ids[k]         upper-stage leaves in creation order */
+ *   leaf_ids[k]         upper-stage leaves in creation order
+ * num_roots: the roots are nodes 0..num_roots-1, one per mesh of a
+ * batched build (1 for pm_build; 0 is read as 1 by pm_set_tree). */
 typedef struct {
     int num_nodes;
     const int8_t *info;
@@ -133,6 +135,7 @@ typedef struct {
     int num_tris;
     int upper_levels, lower_levels;
     int has_boxes;
+    int num_roots;
 } pm_tree;
 
 /* Tree statistics (PrintTree, TriangleMap.cu:110-116, plus depth/leaf data). */
@@ -197,6 +200,27 @@ int pm_tree_stats(pm_ctx *ctx, pm_stats *out, void *stream);
 int pm_trace(pm_ctx *ctx, int algo, const float *org, const float *dir, int nrays,
              pm_hit *out, const pm_trace_params *params, int *counters, void *stream);
 
+/* Batched build: one kd-tree per mesh for M meshes whose triangles are
+ * concatenated in p0/p1/p2 (mesh m owns triangles
+ * [mesh_offsets[m], mesh_offsets[m+1]); mesh_offsets is HOST int[M+1],
+ * non-decreasing, from 0 to n).  All M trees are built together, level by
+ * level, into one node array (a forest): root m is node m, and each tree is
+ * the one pm_build makes for its mesh alone, up to a renumbering of nodes and
+ * prim slots that keeps their order (hit ids are ids into the concatenated
+ * arrays).  Replaces a loop of TriangleMap::Create (TriangleMap.cu:51-62)
+ * over the meshes of a GeometryList (GeometryList.h:40-68), which in the
+ * reference builds one tree over all of them. */
+int pm_build_batched(pm_ctx *ctx, const float *p0, const float *p1, const float *p2, int n,
+                     const int *mesh_offsets, int num_meshes, const pm_build_params *params,
+                     void *stream);
+
+/* pm_trace over a forest: ray r starts at root ray_mesh[r] (device int[nrays];
+ * a ray whose mesh index is outside [0, num_meshes) misses).  KD_RESTART and
+ * SHORT_STACK only.  Same traversal as IRayTracer::Trace (see pm_trace). */
+int pm_trace_batched(pm_ctx *ctx, int algo, const float *org, const float *dir, const int *ray_mesh,
+                     int nrays, pm_hit *out, const pm_trace_params *params, int *counters,
+                     void *stream);
+
 /* Backward of the intersection (no reference counterpart; the forward is
  * MoellerTrumbore, IRayTracer.h:111-145).  For every ray with hits[r].id >= 0
  * (the records a pm_trace call produced for these rays) and upstream
@@ -223,6 +247,24 @@ int pm_trace_backward(pm_ctx *ctx, const float *org, const float *dir, const pm_
  * PM_ERR_ARG.  Synchronises the stream. */
 int pm_add_mesh(pm_ctx *ctx, const float *vertices, int nv, const unsigned *indices, int nidx,
                 const float *model, float *p0, float *p1, float *p2, int offset, void *stream);
+
+/* OscCUDAMeshNode::Handle / UpdateSurface (OscCUDAMeshNode.cu:72-107): one
+ * animation step of the oscillating grid.  vertices (device float4[d*d],
+ * d = detail + 1, vertex (x, y) at y*d + x as OscCUDAMeshNode::Init lays it
+ * out) get y = height + amplitude cos(frequency x + time)
+ * + amplitude sin(frequency y + time); normals (device float4[d*d] or NULL)
+ * the matching unnormalised normal.  The reference uses amplitude 0.2,
+ * frequency 0.3 and time = accumulated milliseconds / 700000. */
+int pm_osc_surface(pm_ctx *ctx, float *vertices, float *normals, int detail, float height,
+                   float time, float amplitude, float frequency, void *stream);
+
+/* Per-frame deformer of unit-sphere meshes (the animated spheres a batched
+ * rebuild traces; no reference counterpart, written in the style of
+ * UpdateSurface): out[i] = unit[i] * r with
+ * r = 1 + wave[0] sin(wave[1] theta + wave[2] frame) cos(wave[3] phi + wave[4] frame),
+ * theta = acos(z), phi = atan2(y, x); w is copied.  unit/out: device float4[nv]. */
+int pm_deform_radial(pm_ctx *ctx, const float *unit, int nv, float frame, const float *wave,
+                     float *out, void *stream);
 
 /* Photon path continuation (no reference counterpart: the reference's only
  * secondary rays are the shading loop's reflections, Raytracer.cu:219-256).

@@ -139,6 +139,7 @@ struct Tree {
     std::vector<long long> level_refs;
     std::vector<long long> lower_level_nodes;
     int upper_levels = 0, lower_levels = 0;
+    int num_roots = 1;             /* roots are nodes 0..num_roots-1 */
 
     int size() const { return (int)info.size(); }
     void resize(int n) {
@@ -165,14 +166,20 @@ void exclusive_scan(const std::vector<T> &in, std::vector<T> &out) {
 /* ------------------------------------------------------------------ */
 /* Upper (large node) stage: TriangleMapUpperCreator.cu:110-759        */
 /* ------------------------------------------------------------------ */
-int build_upper(const f4 *P0, const f4 *P1, const f4 *P2, int N, const Params &prm,
-                Tree &T, std::vector<f4> &prim_min, std::vector<f4> &prim_max) {
+int build_upper(const f4 *P0, const f4 *P1, const f4 *P2, int N, const int *off, int M,
+                const Params &prm, Tree &T, std::vector<f4> &prim_min, std::vector<f4> &prim_max) {
     const int LS = prm.lower_size;
     const int max_levels = prm.max_upper_levels > 0 ? prm.max_upper_levels : 128;
 
-    /* root: TriangleMapUpperCreator.cu:128-137 */
-    T.resize(1);
-    T.prim_index[0] = 0; T.bitmap[0] = N; T.parent[0] = 0;
+    /* root: TriangleMapUpperCreator.cu:128-137.  A forest of M meshes has
+     * one root per mesh, root m (node m) over triangles [off[m], off[m+1])
+     * and its own parent; the level loop below treats them as the first
+     * level's M active nodes. */
+    T.resize(M);
+    T.num_roots = M;
+    for (int m = 0; m < M; ++m) {
+        T.prim_index[m] = off[m]; T.bitmap[m] = off[m + 1] - off[m]; T.parent[m] = m;
+    }
 
     /* CalcPrimitiveAabb, Kernels/TriangleUpper.h:10-28 */
     std::vector<f4> rmin((size_t)N), rmax((size_t)N);
@@ -188,7 +195,7 @@ int build_upper(const f4 *P0, const f4 *P1, const f4 *P2, int N, const Params &p
         rmax[i] = {mx.x, mx.y, mx.z, 0.0f};
     }
 
-    int active_index = 0, A = 1, level = 0;
+    int active_index = 0, A = M, level = 0;
     std::vector<f4> prop_min, prop_max, nmin, nmax;
     std::vector<int> side, addr, leaf_side, leaf_addr, lf, lna, owner;
     while (A > 0) {
@@ -681,7 +688,7 @@ inline int test_leaf(const View &V, const f4 *P0, const f4 *P1, const f4 *P2, in
 
 /* KDRestart / TraceNode, Raytracer.cu:57-106,165-262 (first hit only) */
 orc_hit trace_kd_restart(const View &V, const f4 *P0, const f4 *P1, const f4 *P2, f3 o, f3 draw,
-                         bool leaf_skip, int *cnt) {
+                         bool leaf_skip, int root, int *cnt) {
     const f3 d = adjust_dir(draw);
     const f3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
     float th = 0.0f, tu = 0.0f, tv = 0.0f;
@@ -689,8 +696,8 @@ orc_hit trace_kd_restart(const View &V, const f4 *P0, const f4 *P1, const f4 *P2
     do {
         if (++visits > (long long)V.num_nodes + 1) break;
         float tNext = F_INF;
-        int node = 0;
-        int8_t axis = V.info[0];
+        int node = root;
+        int8_t axis = V.info[root];
         int steps = 0;
         while ((axis & 3) != LEAF) {
             if (++steps > MAX_DESCENT) return miss_record();
@@ -717,7 +724,7 @@ orc_hit trace_kd_restart(const View &V, const f4 *P0, const f4 *P1, const f4 *P2
 
 /* ShortStackTrace + Stack<N>, ShortStack.cu:58-194, ShortStack.h:43-98 */
 orc_hit trace_short_stack(const View &V, const f4 *P0, const f4 *P1, const f4 *P2, f3 o, f3 draw,
-                          bool leaf_skip, int S, int *cnt) {
+                          bool leaf_skip, int S, int root, int *cnt) {
     const f3 d = adjust_dir(draw);
     const f3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
     int stk_node[64]; float stk_t[64];
@@ -727,7 +734,7 @@ orc_hit trace_short_stack(const View &V, const f4 *P0, const f4 *P1, const f4 *P
     do {
         if (++visits > (long long)V.num_nodes + 1) break;
         int node; float tNext;
-        if (count == 0) { node = 0; tNext = F_INF; }
+        if (count == 0) { node = root; tNext = F_INF; }
         else {
             nxt = (nxt == 0 ? S : nxt) - 1;
             count--;
@@ -802,10 +809,15 @@ extern "C" {
 
 const char *orc_last_error(void) { return g_err.c_str(); }
 
-int orc_build(const float *p0, const float *p1, const float *p2, int n,
-              const orc_build_params *params, orc_tree **out) {
+int orc_build_batched(const float *p0, const float *p1, const float *p2, int n,
+                      const int *mesh_offsets, int num_meshes, const orc_build_params *params,
+                      orc_tree **out) {
     g_err.clear();
     if (!out || !params || n < 0 || (n > 0 && (!p0 || !p1 || !p2))) { g_err = "invalid argument"; return 1; }
+    if (num_meshes < 1 || !mesh_offsets) { g_err = "need at least one mesh and its offsets"; return 1; }
+    for (int m = 0; m < num_meshes; ++m)
+        if (mesh_offsets[m] > mesh_offsets[m + 1]) { g_err = "mesh offsets must be non-decreasing"; return 1; }
+    if (mesh_offsets[0] != 0 || mesh_offsets[num_meshes] != n) { g_err = "mesh offsets must span [0, n]"; return 1; }
     const Params prm = to_params(params);
     if (prm.split_method != 0) { g_err = "split method not implemented"; return 2; }
     if (prm.lower_size < 1 || prm.lower_size > 64) { g_err = "lower_size must be in [1, 64]"; return 1; }
@@ -814,12 +826,20 @@ int orc_build(const float *p0, const float *p1, const float *p2, int n,
     const f4 *P0 = reinterpret_cast<const f4 *>(p0), *P1 = reinterpret_cast<const f4 *>(p1),
              *P2 = reinterpret_cast<const f4 *>(p2);
     std::vector<f4> pmin, pmax;
-    int rc = build_upper(P0, P1, P2, n, prm, t->T, pmin, pmax);
+    int rc = build_upper(P0, P1, P2, n, mesh_offsets, num_meshes, prm, t->T, pmin, pmax);
     if (!rc) rc = build_lower(P0, P1, P2, prm, t->T, pmin, pmax);
     if (rc) { delete t; return rc; }
     *out = t;
     return 0;
 }
+
+int orc_build(const float *p0, const float *p1, const float *p2, int n,
+              const orc_build_params *params, orc_tree **out) {
+    const int off[2] = {0, n < 0 ? 0 : n};
+    return orc_build_batched(p0, p1, p2, n, off, 1, params, out);
+}
+
+int orc_num_roots(const orc_tree *t) { return t->T.num_roots; }
 
 void orc_free(orc_tree *t) { delete t; }
 int orc_num_nodes(const orc_tree *t) { return t->T.size(); }
@@ -882,7 +902,8 @@ int orc_stats_of(const orc_tree *t, orc_stats *s) {
     s->num_prim_indices = (int)T.prim_indices.size();
     s->num_upper_leaves = (int)T.leaf_ids.size();
     if (T.size() == 0) return 0;
-    std::vector<std::pair<int, int>> stack{{0, 0}};
+    std::vector<std::pair<int, int>> stack;
+    for (int r = T.num_roots - 1; r >= 0; --r) stack.push_back({r, 0});
     double depth_sum = 0.0;
     long long guard = 0;
     while (!stack.empty()) {
@@ -910,12 +931,14 @@ int orc_stats_of(const orc_tree *t, orc_stats *s) {
     return 0;
 }
 
-int orc_trace(const orc_tree *t, const float *p0, const float *p1, const float *p2, int ntris,
-              int algo, const float *org, const float *dir, int nrays,
-              const orc_trace_params *params, orc_hit *out, int *counters) {
+int orc_trace_batched(const orc_tree *t, const float *p0, const float *p1, const float *p2, int ntris,
+                      int algo, const float *org, const float *dir, const int *ray_root, int nrays,
+                      const orc_trace_params *params, orc_hit *out, int *counters) {
     g_err.clear();
     if (nrays < 0 || !out || !params) { g_err = "invalid argument"; return 1; }
     if (algo != 0 && !t) { g_err = "tree required"; return 1; }
+    if (ray_root && algo == 0) { g_err = "brute force has no roots"; return 1; }
+    const int roots = t ? t->T.num_roots : 1;
     const f4 *P0 = reinterpret_cast<const f4 *>(p0), *P1 = reinterpret_cast<const f4 *>(p1),
              *P2 = reinterpret_cast<const f4 *>(p2);
     const f4 *O = reinterpret_cast<const f4 *>(org), *D = reinterpret_cast<const f4 *>(dir);
@@ -936,11 +959,19 @@ int orc_trace(const orc_tree *t, const float *p0, const float *p1, const float *
         int *cnt = counters ? counters + 4 * (size_t)r : nullptr;
         if (cnt) cnt[0] = cnt[1] = cnt[2] = cnt[3] = 0;
         const f3 o = mk3(O[r]), d = mk3(D[r]);
+        const int root = ray_root ? ray_root[r] : 0;
+        if (root < 0 || root >= roots) { out[r] = miss_record(); continue; }   /* no such mesh */
         if (algo == 0) out[r] = trace_brute(P0, P1, P2, ntris, params->brute_cap, o, d, cnt);
-        else if (algo == 1) out[r] = trace_kd_restart(V, P0, P1, P2, o, d, skip, cnt);
-        else out[r] = trace_short_stack(V, P0, P1, P2, o, d, skip, S, cnt);
+        else if (algo == 1) out[r] = trace_kd_restart(V, P0, P1, P2, o, d, skip, root, cnt);
+        else out[r] = trace_short_stack(V, P0, P1, P2, o, d, skip, S, root, cnt);
     }
     return 0;
+}
+
+int orc_trace(const orc_tree *t, const float *p0, const float *p1, const float *p2, int ntris,
+              int algo, const float *org, const float *dir, int nrays,
+              const orc_trace_params *params, orc_hit *out, int *counters) {
+    return orc_trace_batched(t, p0, p1, p2, ntris, algo, org, dir, nullptr, nrays, params, out, counters);
 }
 
 int orc_mt32(const float *v0, const float *v1, const float *v2, const float *o, const float *d,

@@ -68,6 +68,14 @@ int  orc_build(const float *p0, const float *p1, const float *p2, int n,
                const orc_build_params *params, orc_tree **out);
 void orc_free(orc_tree *t);
 const char *orc_last_error(void);
+/* forest of one tree per mesh (mesh m = triangles [mesh_offsets[m],
+ * mesh_offsets[m+1])): the upper stage starts from num_meshes roots, nodes
+ * 0..num_meshes-1, processed together level by level; orc_build is the
+ * one-mesh case. */
+int  orc_build_batched(const float *p0, const float *p1, const float *p2, int n,
+                       const int *mesh_offsets, int num_meshes,
+                       const orc_build_params *params, orc_tree **out);
+int  orc_num_roots(const orc_tree *t);
 
 /* tree arrays (copies are owned by the tree) */
 int  orc_num_nodes(const orc_tree *t);
@@ -101,6 +109,12 @@ int  orc_trace(const orc_tree *t, const float *p0, const float *p1,
                const float *p2, int ntris, int algo, const float *org,
                const float *dir, int nrays, const orc_trace_params *params,
                orc_hit *out, int *counters);
+/* the same over a forest: ray r starts at root ray_root[r] (a ray whose
+ * root is not one of the forest's misses); kd-restart and short-stack. */
+int  orc_trace_batched(const orc_tree *t, const float *p0, const float *p1,
+                       const float *p2, int ntris, int algo, const float *org,
+                       const float *dir, const int *ray_root, int nrays,
+                       const orc_trace_params *params, orc_hit *out, int *counters);
 
 /* single Moller-Trumbore test exactly as the tracers run it (FP32), and the
  * same formula evaluated in FP64 (no acceptance test, raw t/u/v/det). */

@@ -77,10 +77,12 @@ def lib():
         P = C.c_void_p
         L.orc_build.argtypes = [P, P, P, C.c_int, C.POINTER(BuildParams), C.POINTER(P)]
         L.orc_build.restype = C.c_int
+        L.orc_build_batched.argtypes = [P, P, P, C.c_int, P, C.c_int, C.POINTER(BuildParams), C.POINTER(P)]
+        L.orc_build_batched.restype = C.c_int
         L.orc_free.argtypes = [P]
         L.orc_last_error.restype = C.c_char_p
         for name in ("orc_num_nodes", "orc_num_prim_indices", "orc_num_leaf_ids",
-                     "orc_upper_levels", "orc_lower_levels"):
+                     "orc_upper_levels", "orc_lower_levels", "orc_num_roots"):
             getattr(L, name).argtypes = [P]
             getattr(L, name).restype = C.c_int
         L.orc_upper_level_refs.argtypes = [P, C.c_int]
@@ -94,6 +96,9 @@ def lib():
         L.orc_tree_from_arrays.restype = C.c_int
         L.orc_trace.argtypes = [P, P, P, P, C.c_int, C.c_int, P, P, C.c_int,
                                 C.POINTER(TraceParams), P, P]
+        L.orc_trace_batched.argtypes = [P, P, P, P, C.c_int, C.c_int, P, P, P, C.c_int,
+                                        C.POINTER(TraceParams), P, P]
+        L.orc_trace_batched.restype = C.c_int
         L.orc_trace.restype = C.c_int
         L.orc_mt32.argtypes = [P, P, P, P, P, C.c_float, P]
         L.orc_mt32.restype = C.c_int
@@ -152,6 +157,7 @@ class Tree:
         L.orc_copy_tree(self.h, *[_ptr(a[k]) for k in (
             "info", "split_pos", "aabb_min", "aabb_max", "primitive_index", "primitive_bitmap",
             "children", "surface_area", "parent", "prim_indices", "leaf_ids")])
+        a["num_roots"] = L.orc_num_roots(self.h)
         return a
 
     def stats(self) -> dict:
@@ -174,6 +180,39 @@ def build(tris, params: BuildParams | None = None) -> Tree:
     if rc != 0:
         raise RuntimeError(f"orc_build failed ({rc}): {lib().orc_last_error().decode()}")
     return Tree(h)
+
+
+def build_batched(tris, mesh_offsets, params: BuildParams | None = None) -> Tree:
+    """forest: one tree per mesh, root m = node m (mesh m = triangles
+    [mesh_offsets[m], mesh_offsets[m+1]))"""
+    p0, p1, p2 = (_c(t, np.float32) for t in tris)
+    off = _c(mesh_offsets, np.int32)
+    params = params or build_params()
+    h = C.c_void_p()
+    rc = lib().orc_build_batched(_ptr(p0), _ptr(p1), _ptr(p2), p0.shape[0], _ptr(off),
+                                 off.shape[0] - 1, C.byref(params), C.byref(h))
+    if rc != 0:
+        raise RuntimeError(f"orc_build_batched failed ({rc}): {lib().orc_last_error().decode()}")
+    return Tree(h)
+
+
+def trace_batched(tree: Tree, tris, org, dirs, ray_root, algo: int, leaf_skipping=True,
+                  stack_size=4, threads=0, counters=False):
+    """trace ray r from root ray_root[r] of a forest"""
+    p0, p1, p2 = (_c(t, np.float32) for t in tris)
+    org = _c(org, np.float32)
+    dirs = _c(dirs, np.float32)
+    root = _c(ray_root, np.int32)
+    n = org.shape[0]
+    out = np.zeros(n, HIT_DTYPE)
+    cnt = np.zeros((n, 4), np.int32) if counters else None
+    prm = TraceParams(int(bool(leaf_skipping)), stack_size, 0, threads)
+    rc = lib().orc_trace_batched(tree.h, _ptr(p0), _ptr(p1), _ptr(p2), p0.shape[0], algo,
+                                 _ptr(org), _ptr(dirs), _ptr(root), n, C.byref(prm), _ptr(out),
+                                 _ptr(cnt))
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return (out, cnt) if counters else out
 
 
 def tree_from_arrays(a: dict) -> Tree:

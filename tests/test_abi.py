@@ -38,9 +38,27 @@ def test_defaults_are_reference_constants(pm):
     assert r.lower_algorithm == pm.LOWER_SAH
 
 
-def test_struct_layouts_match_header(pm):
+def test_struct_layouts_match_header(pm, tmp_path):
+    """every ctypes struct field sits at the offset the C compiler gives it"""
     import ctypes as C
-    assert C.sizeof(pm.BuildParams) == 11 * 4
-    assert C.sizeof(pm.TraceParams) == 16
-    # pm_tree: int + 9 ptrs + int + ptr + int + ptr + 3 ptrs + 4 ints (with padding)
-    assert C.sizeof(pm.Tree) == 8 + 9 * 8 + 8 + 8 + 8 + 8 + 3 * 8 + 16
+    import subprocess
+    structs = {"pm_build_params": pm.BuildParams, "pm_trace_params": pm.TraceParams,
+               "pm_tree": pm.Tree, "pm_stats": pm.Stats}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "pmkd.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
+    got = {}
+    for line in subprocess.check_output([str(exe)], text=True).splitlines():
+        cname, field, val = line.split()
+        got[(cname, field)] = int(val)
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
