@@ -101,6 +101,101 @@ def bench_fwd_bwd(pm, scenes, torch, reps: int = 5) -> dict:
             "fwd_bwd_over_fwd": round((f + b) / f, 3)}
 
 
+def bench_batched(pm, scenes, torch, meshes: int = 256, rays_per_mesh: int = 65536,
+                  frames: int = 4, per_mesh_frames: int = 2, stack: int = 8) -> dict:
+    """Animated meshes rebuilt every frame: ``meshes`` cube spheres of
+    log-uniform size (1k-200k triangles), radial wave deformed on the GPU,
+    gathered into one triangle SoA, then either ONE forest build + ONE
+    batched trace (batched) or one build + one trace per mesh (per-mesh),
+    ``rays_per_mesh`` rays each; wall time per frame, same GPU, same
+    deform/gather step in both."""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    c = scenes.config4(num_meshes=meshes, rays_per_mesh=rays_per_mesh)
+    log(f"[bench] batched scene generated in {time.time() - t0:.1f}s")
+    ctx = pm.Context(0)
+    U = torch.from_numpy(c["unit"]).to(dev)
+    V = torch.empty_like(U)
+    F = torch.from_numpy(c["faces"].reshape(-1)).to(dev)
+    off = c["tri_off"]
+    n = int(off[-1])
+    P = [torch.empty((n, 4), dtype=torch.float32, device=dev) for _ in range(3)]
+    O, D = torch.from_numpy(c["org"]).to(dev), torch.from_numpy(c["dirs"]).to(dev)
+    RM = torch.from_numpy(c["ray_mesh"]).to(dev)
+    H = torch.empty((O.shape[0], 4), dtype=torch.float32, device=dev)
+    bp = pm.build_params("reference")
+    tp = pm.trace_params(stack_size=stack)
+    stream = torch.cuda.current_stream()
+    R = rays_per_mesh
+
+    def stage(f):
+        ctx.deform_radial(U, float(f), out=V)
+        ctx.add_mesh(V, F, *P)
+
+    def batched(f, ev=None, bp=bp, tp=tp):
+        stage(f)
+        if ev:
+            ev[0].record(stream)
+        ctx.build_batched(*P, off, bp)
+        if ev:
+            ev[1].record(stream)
+        ctx.trace_batched(pm.SHORT_STACK, O, D, RM, tp, out=H)
+        if ev:
+            ev[2].record(stream)
+
+    def per_mesh(f):
+        stage(f)
+        for m in range(meshes):
+            a, b = int(off[m]), int(off[m + 1])
+            ctx.build(P[0][a:b], P[1][a:b], P[2][a:b], bp)
+            ctx.trace(pm.SHORT_STACK, O[m * R:(m + 1) * R], D[m * R:(m + 1) * R], tp,
+                      out=H[m * R:(m + 1) * R])
+
+    def run_batched(bp, tp):
+        batched(0, bp=bp, tp=tp)
+        bt, bb, btr = [], [], []
+        for f in range(frames):
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            batched(f + 1, ev, bp, tp)
+            torch.cuda.synchronize()
+            bt.append((time.perf_counter() - t) * 1e3)
+            bb.append(ev[0].elapsed_time(ev[1]))
+            btr.append(ev[1].elapsed_time(ev[2]))
+        return float(np.median(bt)), float(np.median(bb)), float(np.median(btr))
+
+    per_mesh(0)
+    b, b_build, b_trace = run_batched(bp, tp)
+    hit_frac = float((H.view(torch.int32)[:, 0] >= 0).float().mean())
+    # autotuned chunk size, leaf threshold and stack depth for this workload
+    stage(0)
+    tune = ctx.autotune(*P, O, D, mesh_offsets=off, ray_mesh=RM, base=bp, reps=2)
+    tb, tb_build, tb_trace = run_batched(tune["build_params"], tune["trace_params"])
+    pt = []
+    for f in range(per_mesh_frames):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        per_mesh(f + 1)
+        torch.cuda.synchronize()
+        pt.append((time.perf_counter() - t) * 1e3)
+    ctx.close()
+    p = float(np.median(pt))
+    return {"workload": f"{meshes} animated cube spheres, {n} tris, {O.shape[0]} rays per frame "
+                        f"(radial wave deform + gather + SAH rebuild + short-stack depth {stack})",
+            "batched_frame_ms": round(b, 3), "per_mesh_frame_ms": round(p, 3),
+            "speedup": round(p / b, 2),
+            "batched_build_ms": round(b_build, 3),
+            "batched_trace_ms": round(b_trace, 3),
+            "batched_trace_mrays": round(O.shape[0] / (b_trace * 1e-3) / 1e6, 1),
+            "hit_fraction": round(hit_frac, 4),
+            "autotuned": {"chunk_size": tune["chunk_size"], "lower_size": tune["lower_size"],
+                          "stack_size": tune["stack_size"], "trials": tune["trials"],
+                          "frame_ms": round(tb, 3), "build_ms": round(tb_build, 3),
+                          "trace_ms": round(tb_trace, 3), "speedup_vs_per_mesh": round(p / tb, 2)}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +207,8 @@ def main() -> int:
     ap.add_argument("--stack", type=int, default=8)
     ap.add_argument("--cpu-sample", type=int, default=1_000_000,
                     help="rays the CPU baseline traces (0 disables the baseline)")
+    ap.add_argument("--meshes", type=int, default=256,
+                    help="meshes of the batched-rebuild comparison (0 skips it)")
     ap.add_argument("--probe", action="store_true",
                     help="counter runs: one build, one short-stack trace of the emission batch and "
                          "a 256 MiB device copy (the HBM-counter calibration), then exit")
@@ -254,6 +351,8 @@ def main() -> int:
 
     fwd_bwd = bench_fwd_bwd(pm, scenes, torch)
     log(f"[bench] fwd+bwd: {fwd_bwd}")
+    batched = bench_batched(pm, scenes, torch, meshes=args.meshes) if args.meshes > 0 else None
+    log(f"[bench] batched: {batched}")
 
     # ---- CPU baseline: the oracle's build and trace on the host cores
     cpu = None
@@ -277,7 +376,7 @@ def main() -> int:
         log(f"[bench] cpu baseline: build {cpu_build_ms:.0f} ms, {cpu['value']} Mrays/s")
 
     line = {
-        "metric": "Mrays/s + kd-build ms @1M tris",
+        "metric": "Mrays/s + kd-build ms @1M tris; batched & fwd+bwd vs plain, same GPU",
         "value": round(mrays, 2),
         "unit": "Mrays/s",
         "n_gpus": 1,
@@ -304,6 +403,7 @@ def main() -> int:
             "short_stack_vs_brute": round(mrays / brute_mrays, 1),
             "emission_hit_fraction": round(hit_frac, 4),
             "fwd_bwd": fwd_bwd,
+            "batched": batched,
             "tree": {k: st[k] for k in ("num_nodes", "leaves", "empty_leaves", "max_depth",
                                         "leaf_refs", "upper_levels", "lower_levels")},
         },

@@ -31,7 +31,8 @@ BRUTE, KD_RESTART, SHORT_STACK = 0, 1, 2
 EXPORTS = ("pm_build_params_default", "pm_trace_params_default", "pm_create", "pm_destroy",
            "pm_last_error", "pm_build", "pm_set_geometry", "pm_tree_view", "pm_set_tree",
            "pm_tree_stats", "pm_trace", "pm_photon_bounce", "pm_add_mesh", "pm_trace_backward",
-           "pm_build_batched", "pm_trace_batched", "pm_osc_surface", "pm_deform_radial")
+           "pm_build_batched", "pm_trace_batched", "pm_osc_surface", "pm_deform_radial",
+           "pm_autotune")
 
 
 class PmError(RuntimeError):
@@ -46,7 +47,7 @@ class BuildParams(C.Structure):
                 ("propagate_aabbs", C.c_int), ("traversal_cost", C.c_float),
                 ("lower_size", C.c_int), ("max_upper_levels", C.c_int),
                 ("sah_leaf_max", C.c_int), ("sah_empty_bonus", C.c_float),
-                ("max_ref_factor", C.c_int)]
+                ("max_ref_factor", C.c_int), ("chunk_size", C.c_int)]
 
 
 class TraceParams(C.Structure):
@@ -64,6 +65,11 @@ class Tree(C.Structure):
                 ("p1", C.c_void_p), ("p2", C.c_void_p), ("num_tris", C.c_int),
                 ("upper_levels", C.c_int), ("lower_levels", C.c_int), ("has_boxes", C.c_int),
                 ("num_roots", C.c_int)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("chunk_size", C.c_int), ("lower_size", C.c_int), ("stack_size", C.c_int),
+                ("build_ms", C.c_float), ("trace_ms", C.c_float), ("trials", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -109,10 +115,12 @@ def load_library(path: str = LIB_PATH):
     F = C.c_float
     L.pm_osc_surface.argtypes = [P, P, P, I, F, F, F, F, P]
     L.pm_deform_radial.argtypes = [P, P, I, F, P, P, P]
+    L.pm_autotune.argtypes = [P, P, P, P, I, P, I, P, P, P, I, C.POINTER(BuildParams), F, I,
+                              C.POINTER(Tuning), P]
     for n in ("pm_create", "pm_destroy", "pm_build", "pm_set_geometry", "pm_tree_view",
               "pm_set_tree", "pm_tree_stats", "pm_trace", "pm_photon_bounce", "pm_add_mesh",
               "pm_trace_backward", "pm_build_batched", "pm_trace_batched", "pm_osc_surface",
-              "pm_deform_radial"):
+              "pm_deform_radial", "pm_autotune"):
         getattr(L, n).restype = C.c_int
     _lib = L
     return L
@@ -240,6 +248,30 @@ class Context:
                                              _dev_ptr(ray_mesh), n, _dev_ptr(out) if n else None,
                                              C.byref(params), cptr, _stream_ptr(stream)))
         return out
+
+    def autotune(self, p0, p1, p2, org, dirs, mesh_offsets=None, ray_mesh=None,
+                 base: BuildParams | None = None, builds_per_trace: float = 1.0, reps: int = 3,
+                 stream=None) -> dict:
+        """time this workload over chunk size, leaf threshold and stack depth
+        (pm_autotune); the context keeps the tree of the chosen setting.
+        Returns the choice with its times and ready-made parameter structs."""
+        base = base or build_params()
+        off = None
+        if mesh_offsets is not None:
+            off = np.ascontiguousarray(mesh_offsets, dtype=np.int32)
+        t = Tuning()
+        self._geom = (p0, p1, p2)
+        self._check(self._L.pm_autotune(
+            self.h, _dev_ptr(p0), _dev_ptr(p1), _dev_ptr(p2), p0.shape[0],
+            off.ctypes.data if off is not None else None, (off.shape[0] - 1) if off is not None else 1,
+            _dev_ptr(org), _dev_ptr(dirs), _dev_ptr(ray_mesh) if ray_mesh is not None else None,
+            org.shape[0], C.byref(base), builds_per_trace, reps, C.byref(t), _stream_ptr(stream)))
+        res = {k: getattr(t, k) for k, _ in Tuning._fields_}
+        bp = BuildParams.from_buffer_copy(base)
+        bp.chunk_size, bp.lower_size = t.chunk_size, t.lower_size
+        res["build_params"] = bp
+        res["trace_params"] = trace_params(stack_size=t.stack_size)
+        return res
 
     def set_geometry(self, p0, p1, p2):
         self._geom = (p0, p1, p2)

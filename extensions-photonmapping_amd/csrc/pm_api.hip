@@ -39,6 +39,9 @@ void check_build_params(const pm_build_params &p) {
     if (p.lower_size < 1 || p.lower_size > 64) throw Error{PM_ERR_ARG, "lower_size must be in [1, 64]"};
     if (p.max_upper_levels < 0) throw Error{PM_ERR_ARG, "max_upper_levels must be >= 0"};
     if (p.max_ref_factor < 0) throw Error{PM_ERR_ARG, "max_ref_factor must be >= 0"};
+    if (p.chunk_size != 0 && p.chunk_size != 128 && p.chunk_size != 256 && p.chunk_size != 512 &&
+        p.chunk_size != 1024)
+        throw Error{PM_ERR_ARG, "chunk_size must be 0, 128, 256, 512 or 1024"};
 }
 
 }  // namespace
@@ -58,6 +61,7 @@ void pm_build_params_default(pm_build_params *p) {
     p->sah_leaf_max = 0;
     p->sah_empty_bonus = 1.0f;
     p->max_ref_factor = 0;
+    p->chunk_size = 0;
 }
 
 void pm_trace_params_default(pm_trace_params *p) {
@@ -326,6 +330,27 @@ int pm_add_mesh(pm_ctx *ctx, const float *vertices, int nv, const unsigned *indi
         float4 *q2 = reinterpret_cast<float4 *>(p2) + offset;
         pmkd::add_mesh(ctx, reinterpret_cast<const float4 *>(vertices), nv, indices, nidx, model, q0, q1, q2,
                        (hipStream_t)stream);
+    });
+}
+
+int pm_autotune(pm_ctx *ctx, const float *p0, const float *p1, const float *p2, int n,
+                const int *mesh_offsets, int num_meshes, const float *org, const float *dir,
+                const int *ray_mesh, int nrays, const pm_build_params *base,
+                float builds_per_trace, int reps, pm_tuning *out, void *stream) {
+    return guarded(ctx, [&] {
+        if (n < 0 || (n > 0 && (!p0 || !p1 || !p2))) throw Error{PM_ERR_ARG, "invalid geometry"};
+        if (nrays < 1 || !org || !dir) throw Error{PM_ERR_ARG, "the autotuner needs rays"};
+        if (!out || reps < 1 || reps > 100 || !(builds_per_trace >= 0.0f))
+            throw Error{PM_ERR_ARG, "invalid autotuner arguments"};
+        if (mesh_offsets && num_meshes < 1) throw Error{PM_ERR_ARG, "need at least one mesh"};
+        if (!mesh_offsets && ray_mesh) throw Error{PM_ERR_ARG, "ray_mesh needs mesh_offsets"};
+        pm_build_params prm;
+        if (base) prm = *base; else pm_build_params_default(&prm);
+        check_build_params(prm);
+        pmkd::autotune(ctx, reinterpret_cast<const float4 *>(p0), reinterpret_cast<const float4 *>(p1),
+                       reinterpret_cast<const float4 *>(p2), n, mesh_offsets, mesh_offsets ? num_meshes : 1,
+                       reinterpret_cast<const float4 *>(org), reinterpret_cast<const float4 *>(dir), ray_mesh,
+                       nrays, prm, builds_per_trace, reps, *out, (hipStream_t)stream);
     });
 }
 

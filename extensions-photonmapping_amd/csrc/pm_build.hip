@@ -3,8 +3,10 @@
 // Upper (large node) stage, level synchronous like
 // TriangleMapUpperCreator::Create (TriangleMapUpperCreator.cu:110-183) and
 // producing the same node numbering, ref order and leaf order:
-//   - refs of an active node are cut into segments of SEG = 256 refs; one
-//     wave64 owns a segment (4 refs per lane, coalesced float4 loads);
+//   - refs of an active node are cut into segments (chunks) of 256 refs by
+//     default (128..1024, a build parameter the autotuner picks); one wave64
+//     owns a segment (chunk/64 refs per lane, coalesced float4 loads); the
+//     chunk size changes how work is cut, never the tree;
 //   - node boxes are exact min/max reductions done with ordered-int atomics
 //     (order independent, so identical to any sequential reduction); the
 //     boxes of the next level are reduced inside the scatter kernel that
@@ -45,9 +47,9 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
     return v;
 }
 
-// exclusive scan of in[0..n) with in[n] treated as 0, i.e. out[0..n]
-__global__ __launch_bounds__(SCAN_T) void scan_block_kernel(const int *__restrict__ in,
-                                                            int *__restrict__ out, int n,
+// exclusive scan of in[0..n) with in[n] treated as 0, i.e. out[0..n];
+// in may equal out (each thread reads its elements before it writes them)
+__global__ __launch_bounds__(SCAN_T) void scan_block_kernel(const int *in, int *out, int n,
                                                             int *__restrict__ block_sums) {
     __shared__ int wsum[SCAN_T / 64];
     const long long base = (long long)blockIdx.x * SCAN_BLK + (long long)threadIdx.x * SCAN_E;
@@ -225,7 +227,7 @@ __global__ __launch_bounds__(BS) void tri_box_kernel(const float4 *__restrict__ 
 // (EmptySpaceSplitting.h:10-70), CalcUpperNodeSplitInfo
 // (ReduceSegments.h:466-484), segments per node.
 __global__ __launch_bounds__(BS) void node_prepare_kernel(NodePtrs N, int active_index, int A,
-                                                          const int *__restrict__ keys,
+                                                          int seg, const int *__restrict__ keys,
                                                           int empty_space, float thr,
                                                           int *__restrict__ planes_out,
                                                           int *__restrict__ req,
@@ -259,7 +261,7 @@ __global__ __launch_bounds__(BS) void node_prepare_kernel(NodePtrs N, int active
     N.split[node] = zHigher ? med.z : spl;
     N.info[node] = (int8_t)(zHigher ? AX_Z : ax);
     const int cnt = (int)N.bitmap[node];
-    nseg[i] = (cnt + SEG - 1) / SEG;
+    nseg[i] = (cnt + seg - 1) / seg;
 }
 
 // EmptySpaceSplitting2 (EmptySpaceSplitting.h:74-184): a chain of split
@@ -305,7 +307,7 @@ __global__ __launch_bounds__(BS) void empty_chain_kernel(NodePtrs N, int active_
 
 // segment table (Segment, TriangleMapUpperCreator.cu:207-239)
 __global__ __launch_bounds__(BS) void seg_fill_kernel(const NodePtrs N, int active_index, int A,
-                                                      const int *__restrict__ segbase,
+                                                      int seg, const int *__restrict__ segbase,
                                                       int *__restrict__ seg_node,
                                                       int *__restrict__ seg_start,
                                                       int *__restrict__ seg_count) {
@@ -316,14 +318,15 @@ __global__ __launch_bounds__(BS) void seg_fill_kernel(const NodePtrs N, int acti
     const int cnt = (int)N.bitmap[node];
     const int b = segbase[i], e = segbase[i + 1];
     for (int s = b; s < e; ++s) {
-        const int off = (s - b) * SEG;
+        const int off = (s - b) * seg;
         seg_node[s] = i;
         seg_start[s] = pi + off;
-        seg_count[s] = min(SEG, cnt - off);
+        seg_count[s] = min(seg, cnt - off);
     }
 }
 
 // SetSplitSide (TriangleUpperChildren.h:10-62) counted per segment
+template <int SEGT>
 __global__ __launch_bounds__(WPB * 64) void classify_kernel(
     const NodePtrs N, int active_index, int S, const int *__restrict__ seg_node,
     const int *__restrict__ seg_start, const int *__restrict__ seg_count,
@@ -338,7 +341,7 @@ __global__ __launch_bounds__(WPB * 64) void classify_kernel(
     const int start = seg_start[s], count = seg_count[s];
     int cl = 0, cr = 0;
 #pragma unroll
-    for (int it = 0; it < SEG / 64; ++it) {
+    for (int it = 0; it < SEGT / 64; ++it) {
         const int j = it * 64 + lane;
         const bool valid = j < count;
         bool l = false, r = false;
@@ -438,6 +441,7 @@ __global__ __launch_bounds__(BS) void children_kernel(
 
 // SplitTriangles (TriangleUpperChildren.h:283-383): stable partition of the
 // refs with clipping, plus the next level's box reduction.
+template <int SEGT>
 __global__ __launch_bounds__(WPB * 64) void scatter_kernel(
     const NodePtrs N, int active_index, int A, int S, int next_active, LevelTotals tot,
     const int *__restrict__ seg_node, const int *__restrict__ seg_start,
@@ -468,7 +472,7 @@ __global__ __launch_bounds__(WPB * 64) void scatter_kernel(
     int l0 = fkey(F_INF), l1 = l0, l2 = l0, l3 = fkey(-F_INF), l4 = l3, l5 = l3;
     int r0 = l0, r1 = l0, r2 = l0, r3 = l3, r4 = l3, r5 = l3;
 #pragma unroll
-    for (int it = 0; it < SEG / 64; ++it) {
+    for (int it = 0; it < SEGT / 64; ++it) {
         const int j = it * 64 + lane;
         const bool valid = j < count;
         float4 mn = make_float4(0.0f, 0.0f, 0.0f, 0.0f), mx = mn;
@@ -729,6 +733,27 @@ __global__ void lower_children_kernel(NodePtrs N, const int *__restrict__ list, 
     N.children[par] = make_int2(l, r);
 }
 
+// chunk size -> kernel instance (one wave per chunk, chunk/64 refs per lane)
+template <class... Args>
+void launch_classify(int seg, int blocks, hipStream_t s, Args... a) {
+    switch (seg) {
+    case 128: classify_kernel<128><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    case 512: classify_kernel<512><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    case 1024: classify_kernel<1024><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    default: classify_kernel<256><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    }
+}
+
+template <class... Args>
+void launch_scatter(int seg, int blocks, hipStream_t s, Args... a) {
+    switch (seg) {
+    case 128: scatter_kernel<128><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    case 512: scatter_kernel<512><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    case 1024: scatter_kernel<1024><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    default: scatter_kernel<256><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    }
+}
+
 NodePtrs node_ptrs(pm_ctx *ctx) {
     NodeArrays &a = ctx->nodes;
     return NodePtrs{a.info.ptr, a.split.ptr, a.bmin.ptr, a.bmax.ptr, a.prim_index.ptr,
@@ -751,21 +776,27 @@ void readback(BuildScratch &sc, std::initializer_list<const int *> srcs, int *ds
 
 }  // namespace
 
-void scan_exclusive(const int *in, int *out, int n, BuildScratch &sc, hipStream_t s) {
+// exclusive scan, out[n] = total: one pass when n fits a block, else
+// block scans, a scan of the block sums (recursively, in place) and an add
+static void scan_level(const int *in, int *out, int n, BuildScratch &sc, int depth, hipStream_t s) {
     const int nb = blocks_for((long long)n + 1, SCAN_BLK);
     if (nb == 1) {
         scan_block_kernel<<<1, SCAN_T, 0, s>>>(in, out, n, nullptr);
         PM_LAUNCH_CHECK();
         return;
     }
-    if (nb > SCAN_BLK) throw Error{PM_ERR_ARG, "scan too large"};
-    sc.scan_tmp.reserve((size_t)nb + 1, s);
-    scan_block_kernel<<<nb, SCAN_T, 0, s>>>(in, out, n, sc.scan_tmp.ptr);
+    if (depth >= 3) throw Error{PM_ERR_ARG, "scan too large"};
+    DevBuf<int> &tmp = sc.scan_tmp[depth];
+    tmp.reserve((size_t)nb + 1, s);
+    scan_block_kernel<<<nb, SCAN_T, 0, s>>>(in, out, n, tmp.ptr);
     PM_LAUNCH_CHECK();
-    scan_block_kernel<<<1, SCAN_T, 0, s>>>(sc.scan_tmp.ptr, sc.scan_tmp.ptr, nb, nullptr);
+    scan_level(tmp.ptr, tmp.ptr, nb, sc, depth + 1, s);
+    scan_add_kernel<<<nb, SCAN_T, 0, s>>>(out, n, tmp.ptr);
     PM_LAUNCH_CHECK();
-    scan_add_kernel<<<nb, SCAN_T, 0, s>>>(out, n, sc.scan_tmp.ptr);
-    PM_LAUNCH_CHECK();
+}
+
+void scan_exclusive(const int *in, int *out, int n, BuildScratch &sc, hipStream_t s) {
+    scan_level(in, out, n, sc, 0, s);
 }
 
 void build_tree(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2, int N,
@@ -774,6 +805,7 @@ void build_tree(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P
     NodeArrays &nodes = ctx->nodes;
     const int LS = prm.lower_size;
     const int max_levels = prm.max_upper_levels > 0 ? prm.max_upper_levels : 128;
+    const int seg = prm.chunk_size > 0 ? prm.chunk_size : 256;
     ctx->has_tree = false;
     ctx->p0 = P0; ctx->p1 = P1; ctx->p2 = P2; ctx->ntris = N;
     ctx->level_refs.clear();
@@ -832,7 +864,7 @@ void build_tree(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P
         sc.slot_leaf.reserve(2 * (size_t)A + 1, s); sc.slot_lna.reserve(2 * (size_t)A + 1, s);
         // planes are kept in slot_lna's storage until the chain kernel has run
         int *planes = sc.slot_lna.ptr;
-        node_prepare_kernel<<<blocks_for(A, BS), BS, 0, s>>>(NP, active_index, A, sc.keys[cur].ptr,
+        node_prepare_kernel<<<blocks_for(A, BS), BS, 0, s>>>(NP, active_index, A, seg, sc.keys[cur].ptr,
                                                               prm.empty_space_splitting,
                                                               prm.empty_space_threshold, planes,
                                                               sc.node_req.ptr, sc.node_nseg.ptr);
@@ -854,14 +886,13 @@ void build_tree(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P
         sc.segLL.reserve(S + 1, s); sc.segLR.reserve(S + 1, s);
         sc.segLoff.reserve(S + 1, s); sc.segRoff.reserve(S + 1, s);
         sc.segLLoff.reserve(S + 1, s); sc.segLRoff.reserve(S + 1, s);
-        seg_fill_kernel<<<blocks_for(A, BS), BS, 0, s>>>(NP, active_index, A, sc.node_segbase.ptr,
+        seg_fill_kernel<<<blocks_for(A, BS), BS, 0, s>>>(NP, active_index, A, seg, sc.node_segbase.ptr,
                                                           sc.seg_node.ptr, sc.seg_start.ptr, sc.seg_count.ptr);
         PM_LAUNCH_CHECK();
         if (S > 0) {
-            classify_kernel<<<blocks_for(S, WPB), WPB * 64, 0, s>>>(NP, active_index, S, sc.seg_node.ptr,
-                                                                   sc.seg_start.ptr, sc.seg_count.ptr,
-                                                                   sc.ref_min[cur].ptr, sc.ref_max[cur].ptr,
-                                                                   sc.segL.ptr, sc.segR.ptr);
+            launch_classify(seg, blocks_for(S, WPB), s, NP, active_index, S, sc.seg_node.ptr,
+                            sc.seg_start.ptr, sc.seg_count.ptr, sc.ref_min[cur].ptr, sc.ref_max[cur].ptr,
+                            sc.segL.ptr, sc.segR.ptr);
             PM_LAUNCH_CHECK();
         }
         scan_exclusive(sc.segL.ptr, sc.segLoff.ptr, S, sc, s);
@@ -904,7 +935,7 @@ void build_tree(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P
             sc.slot_leaf.ptr, sc.slot_lna.ptr, ctx->leaf_ids.ptr, sc.keys[nxt].ptr);
         PM_LAUNCH_CHECK();
         if (S > 0) {
-            scatter_kernel<<<blocks_for(S, WPB), WPB * 64, 0, s>>>(
+            launch_scatter(seg, blocks_for(S, WPB), s,
                 NP, active_index, A, S, next_active, tot, sc.seg_node.ptr, sc.seg_start.ptr,
                 sc.seg_count.ptr, sc.segLoff.ptr, sc.segRoff.ptr, sc.segLLoff.ptr, sc.segLRoff.ptr,
                 sc.slot_leaf.ptr, sc.ref_min[cur].ptr, sc.ref_max[cur].ptr, sc.ref_min[nxt].ptr,

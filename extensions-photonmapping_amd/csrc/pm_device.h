@@ -15,7 +15,6 @@ namespace pmkd {
 constexpr float F_INF = 2139095040.0f;
 constexpr int LEAF = 0, AX_X = 1, AX_Y = 2, AX_Z = 3;  // KDNode.h:62-65
 constexpr int MAX_DESCENT = 4096;
-constexpr int SEG = 256;          // refs per build segment (one wave64, 4 refs per lane)
 
 struct f3 { float x, y, z; };
 

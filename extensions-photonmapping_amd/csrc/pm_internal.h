@@ -113,7 +113,7 @@ struct BuildScratch {
     DevBuf<int> segL, segR, segLL, segLR;       // per-segment counts (left, right, leaf-left, leaf-right)
     DevBuf<int> segLoff, segRoff, segLLoff, segLRoff;
     DevBuf<int> slot_leaf, slot_lna;            // 2A leaf-slot flags and addresses
-    DevBuf<int> scan_tmp;                       // block sums for scans
+    DevBuf<int> scan_tmp[3];                    // block sums for scans, per level
     DevBuf<int> totals;                         // small device-side counters
     DevBuf<float> areas;                        // lower stage: triangle areas per final ref
     DevBuf<int> lower_list;                     // lower stage: node list
@@ -171,6 +171,10 @@ void trace_backward(pm_ctx *ctx, const float4 *org, const float4 *dir, const pm_
                     int num_vertices, float *grad_vertices, float *grad_org, float *grad_dir,
                     hipStream_t s);
 // pm_geometry.hip
+void autotune(pm_ctx *ctx, const float4 *p0, const float4 *p1, const float4 *p2, int n,
+              const int *mesh_offsets, int M, const float4 *org, const float4 *dir,
+              const int *ray_mesh, int nrays, const pm_build_params &base, float builds_per_trace,
+              int reps, pm_tuning &out, hipStream_t s);
 void osc_surface(float4 *vert, float4 *norm, int d, float height, float time, float amp, float freq,
                  hipStream_t s);
 void radial_wave(const float4 *unit, int nv, float frame, const float *wave, float4 *out, hipStream_t s);

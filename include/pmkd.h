@@ -73,7 +73,11 @@ typedef struct { int id; float t, u, v; } pm_hit;
  *   max_ref_factor        no counterpart: error (PM_ERR_DEPTH) when a level
  *                         holds more than max_ref_factor * n + 65536
  *                         references (0 = 16); the reference doubles its
- *                         references every level on coincident triangles */
+ *                         references every level on coincident triangles
+ *   chunk_size            refs per upper-stage segment, the reference's
+ *                         Segments::SEGMENT_SIZE (Segments.h:27, 256);
+ *                         128/256/512/1024, 0 = 256.  Changes how the work
+ *                         is cut, never the tree (pm_autotune picks it) */
 typedef struct {
     int   lower_algorithm;
     int   empty_space_splitting;
@@ -86,6 +90,7 @@ typedef struct {
     int   sah_leaf_max;
     float sah_empty_bonus;
     int   max_ref_factor;
+    int   chunk_size;
 } pm_build_params;
 
 /* Trace options: IRayTracer setters (IRayTracer.h:50-58) and constants.
@@ -220,6 +225,30 @@ int pm_build_batched(pm_ctx *ctx, const float *p0, const float *p1, const float 
 int pm_trace_batched(pm_ctx *ctx, int algo, const float *org, const float *dir, const int *ray_mesh,
                      int nrays, pm_hit *out, const pm_trace_params *params, int *counters,
                      void *stream);
+
+/* Autotuner result (pm_autotune). */
+typedef struct {
+    int chunk_size, lower_size, stack_size;   /* the chosen setting */
+    float build_ms, trace_ms;                 /* its median times */
+    int trials;                               /* settings timed */
+} pm_tuning;
+
+/* Autotuner (no reference counterpart: the reference fixes these at
+ * compile time, Segments.h:27 SEGMENT_SIZE, KDNode.h:21 lower size,
+ * ShortStack.cu:25 stack size).  Times the caller's own workload on this
+ * GPU: chunk_size over {128, 256, 512, 1024} (build time), then lower_size
+ * (the upper stage's leaf threshold) over {8, 16, 24, 32, 48, 64} (cost =
+ * builds_per_trace * build + trace), then stack_size over {4, 8, 16}
+ * (trace time); each setting is timed `reps` times (median) after one
+ * warm-up; a leaf threshold whose build stops at the reference-count
+ * guard (PM_ERR_DEPTH) is skipped.  Every other field comes from `base`.  The geometry is one mesh
+ * (mesh_offsets NULL) or a batch (as pm_build_batched); the rays are traced
+ * with SHORT_STACK (ray_mesh as pm_trace_batched, NULL for one mesh).  On
+ * return the context holds the tree of the chosen setting. */
+int pm_autotune(pm_ctx *ctx, const float *p0, const float *p1, const float *p2, int n,
+                const int *mesh_offsets, int num_meshes, const float *org, const float *dir,
+                const int *ray_mesh, int nrays, const pm_build_params *base,
+                float builds_per_trace, int reps, pm_tuning *out, void *stream);
 
 /* Backward of the intersection (no reference counterpart; the forward is
  * MoellerTrumbore, IRayTracer.h:111-145).  For every ray with hits[r].id >= 0

@@ -80,7 +80,8 @@ def test_fp64_adjoint_matches_central_differences(orc, small_sphere):
     hits = s["hits"].copy()
     hits["id"][~s["mask"]] = -1
     m = s["mask"]
-    gt = np.where(m, 2.0 * (s["hits"]["t"].astype(np.float64) - s["target"]), 0.0).astype(np.float32)
+    gt = np.zeros(m.shape[0], np.float32)
+    gt[m] = 2.0 * (s["hits"]["t"][m].astype(np.float64) - s["target"][m])
     g64, _, _ = orc.trace_backward(s["tris"], s["org"], s["dirs"], hits, grad_t=gt, faces=s["faces"],
                                    num_vertices=s["v_pert"].shape[0], fp64=True)
     rng = np.random.default_rng(0)

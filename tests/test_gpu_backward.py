@@ -81,7 +81,8 @@ def test_c5_gradient_matches_fp64_central_differences(pm, orc, scenes, torch):
     m = s["mask"]
     t = s["h"]["t"]
     # loss = sum over rays hitting both spheres of (t - t_target)^2
-    gt = np.where(m, 2.0 * (t.astype(np.float32) - s["h_ref"]["t"].astype(np.float32)), 0.0).astype(np.float32)
+    gt = np.zeros(t.shape[0], np.float32)
+    gt[m] = 2.0 * (t[m].astype(np.float32) - s["h_ref"]["t"][m].astype(np.float32))
     hits = s["H"].clone()
     hits.view(torch.int32)[:, 0][torch.from_numpy(~m).to(dev)] = -1
     F = torch.from_numpy(s["faces"].reshape(-1)).to(dev)

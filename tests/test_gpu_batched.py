@@ -164,3 +164,45 @@ def test_c4_frame_pipeline_bit_exact(pm, orc, scenes, torch, ctx):
     r = orc.trace_batched(ref, tris, c["org"], c["dirs"], c["ray_mesh"], orc.SHORT_STACK, stack_size=8)
     assert_same_hits(h, r, "C4 frame short-stack")
     assert (h["id"] >= 0).mean() > 0.95
+
+
+@pytest.mark.parametrize("chunk", [128, 512, 1024])
+def test_chunk_size_changes_the_work_not_the_tree(pm, orc, scenes, torch, ctx, chunk):
+    tris = scenes.random_soup(60000, seed=11)
+    params = pm.build_params("reference", chunk_size=chunk)
+    P = [dev(torch, t) for t in tris]
+    ctx.build(*P, params)
+    assert_same_tree(ctx.tree_arrays(), orc.build(tris, orc_params(orc, params)).arrays(), f"chunk {chunk}")
+    btris, off, *_ = c4_small(scenes, num_meshes=6, empty=(1,))
+    P2, got = gpu_forest(pm, ctx, torch, btris, off, params)
+    assert_same_tree(got, orc.build_batched(btris, off, orc_params(orc, params)).arrays(), f"forest chunk {chunk}")
+    with pytest.raises(pm.PmError):
+        ctx.build(*P, pm.build_params("reference", chunk_size=300))
+
+
+def test_autotune_picks_a_valid_setting(pm, orc, scenes, torch, ctx):
+    tris, off, org, dirs, ray_mesh = c4_small(scenes, num_meshes=8)
+    P = [dev(torch, t) for t in tris]
+    O, D, RM = dev(torch, org), dev(torch, dirs), dev(torch, ray_mesh)
+    res = ctx.autotune(*P, O, D, mesh_offsets=off, ray_mesh=RM, base=pm.build_params("reference"),
+                       reps=1)
+    assert res["chunk_size"] in (128, 256, 512, 1024)
+    assert res["lower_size"] in (8, 16, 24, 32, 48, 64)
+    assert res["stack_size"] in (4, 8, 16)
+    # leaf thresholds the meshes cannot reach are skipped, the rest are timed
+    assert 4 + 1 + 3 <= res["trials"] <= 4 + 6 + 3 and res["build_ms"] > 0 and res["trace_ms"] > 0
+    # the context holds the chosen tree; tracing it with the chosen depth is
+    # the oracle's trace of the oracle's tree with the same parameters
+    bp, tp = res["build_params"], res["trace_params"]
+    ref = orc.build_batched(tris, off, orc_params(orc, bp))
+    assert_same_tree(ctx.tree_arrays(), ref.arrays(), "autotuned forest")
+    h = pm.hits_to_numpy(ctx.trace_batched(pm.SHORT_STACK, O, D, RM, tp))
+    r = orc.trace_batched(ref, tris, org, dirs, ray_mesh, orc.SHORT_STACK, stack_size=tp.stack_size)
+    assert_same_hits(h, r, "autotuned trace")
+    # one mesh, no mesh indices
+    stris = scenes.random_soup(20000, seed=12)
+    sorg, sdirs = scenes.random_rays(20000, seed=13)
+    SP = [dev(torch, t) for t in stris]
+    res1 = ctx.autotune(*SP, dev(torch, sorg), dev(torch, sdirs), reps=1)
+    assert_same_tree(ctx.tree_arrays(), orc.build(stris, orc_params(orc, res1["build_params"])).arrays(),
+                     "autotuned single mesh")

@@ -190,19 +190,19 @@ def test_error_codes(pm, torch, scenes):
     import pyoracle as orc
     tris_h = scenes.random_soup(100, seed=1)
     good = orc.build(tris_h).arrays()
-    bad = {k: v.copy() for k, v in good.items()}
+    bad = {k: np.copy(v) for k, v in good.items()}
     inner = np.nonzero(bad["info"] & 3)[0][0]
     bad["children"][inner, 1] = bad["info"].shape[0] + 5
     with pytest.raises(pm.PmError) as e:
         c.set_tree(bad, *tris)
     assert e.value.code == 1
-    bad = {k: v.copy() for k, v in good.items()}
+    bad = {k: np.copy(v) for k, v in good.items()}
     leaf = np.nonzero((bad["info"] == 0) & (bad["primitive_bitmap"] != 0))[0][0]
     bad["primitive_index"][leaf] = bad["prim_indices"].shape[0]
     with pytest.raises(pm.PmError) as e:
         c.set_tree(bad, *tris)
     assert e.value.code == 1
-    bad = {k: v.copy() for k, v in good.items()}
+    bad = {k: np.copy(v) for k, v in good.items()}
     bad["prim_indices"][0] = 100          # triangle id outside p0/p1/p2
     with pytest.raises(pm.PmError) as e:
         c.set_tree(bad, *tris)
