@@ -47,47 +47,70 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
     return v;
 }
 
-// exclusive scan of in[0..n) with in[n] treated as 0, i.e. out[0..n];
-// in may equal out (each thread reads its elements before it writes them)
-__global__ __launch_bounds__(SCAN_T) void scan_block_kernel(const int *in, int *out, int n,
-                                                            int *__restrict__ block_sums) {
-    __shared__ int wsum[SCAN_T / 64];
+// K independent exclusive scans of the same length in one pass (the level
+// loop scans its counts in pairs): in[k][0..n) with in[k][n] treated as 0,
+// i.e. out[k][0..n].  in may equal out (each thread reads its elements
+// before it writes them).
+template <int K>
+struct ScanIO {
+    const int *in[K];
+    int *out[K];
+};
+
+template <int K>
+__global__ __launch_bounds__(SCAN_T) void scan_block_kernel(ScanIO<K> io, int n, int *__restrict__ block_sums,
+                                                            int sums_stride) {
+    __shared__ int wsum[K][SCAN_T / 64];
     const long long base = (long long)blockIdx.x * SCAN_BLK + (long long)threadIdx.x * SCAN_E;
-    int v[SCAN_E];
-    int tsum = 0;
-#pragma unroll
-    for (int j = 0; j < SCAN_E; ++j) {
-        const long long k = base + j;
-        v[j] = k < n ? in[k] : 0;
-        tsum += v[j];
-    }
-    const int incl = wave_incl_scan(tsum);
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (lane == 63) wsum[w] = incl;
+    int v[K][SCAN_E], tsum[K], incl[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        tsum[q] = 0;
+#pragma unroll
+        for (int j = 0; j < SCAN_E; ++j) {
+            const long long k = base + j;
+            v[q][j] = k < n ? io.in[q][k] : 0;
+            tsum[q] += v[q][j];
+        }
+        incl[q] = wave_incl_scan(tsum[q]);
+        if (lane == 63) wsum[q][w] = incl[q];
+    }
     __syncthreads();
     if (w == 0) {
-        int x = lane < SCAN_T / 64 ? wsum[lane] : 0;
-        x = wave_incl_scan(x);
-        if (lane < SCAN_T / 64) wsum[lane] = x;
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            int x = lane < SCAN_T / 64 ? wsum[q][lane] : 0;
+            x = wave_incl_scan(x);
+            if (lane < SCAN_T / 64) wsum[q][lane] = x;
+        }
     }
     __syncthreads();
-    int run = (w ? wsum[w - 1] : 0) + incl - tsum;
 #pragma unroll
-    for (int j = 0; j < SCAN_E; ++j) {
-        const long long k = base + j;
-        if (k <= n) out[k] = run;
-        run += v[j];
+    for (int q = 0; q < K; ++q) {
+        int run = (w ? wsum[q][w - 1] : 0) + incl[q] - tsum[q];
+#pragma unroll
+        for (int j = 0; j < SCAN_E; ++j) {
+            const long long k = base + j;
+            if (k <= n) io.out[q][k] = run;
+            run += v[q][j];
+        }
+        if (block_sums && threadIdx.x == SCAN_T - 1)
+            block_sums[q * sums_stride + blockIdx.x] = wsum[q][SCAN_T / 64 - 1];
     }
-    if (block_sums && threadIdx.x == SCAN_T - 1) block_sums[blockIdx.x] = wsum[SCAN_T / 64 - 1];
 }
 
-__global__ __launch_bounds__(SCAN_T) void scan_add_kernel(int *__restrict__ out, int n,
-                                                          const int *__restrict__ offs) {
+template <int K>
+__global__ __launch_bounds__(SCAN_T) void scan_add_kernel(ScanIO<K> io, int n,
+                                                          const int *__restrict__ offs, int offs_stride) {
     const long long base = (long long)blockIdx.x * SCAN_BLK;
-    const int add = offs[blockIdx.x];
-    for (int j = threadIdx.x; j < SCAN_BLK; j += SCAN_T) {
-        const long long k = base + j;
-        if (k <= n) out[k] += add;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        const int add = offs[q * offs_stride + blockIdx.x];
+        for (int j = threadIdx.x; j < SCAN_BLK; j += SCAN_T) {
+            const long long k = base + j;
+            if (k <= n) io.out[q][k] += add;
+        }
     }
 }
 
@@ -149,67 +172,79 @@ __global__ void init_roots_kernel(NodePtrs N, const int *__restrict__ off, int M
     N.parent[m] = m;
 }
 
-// root boxes of a forest: each triangle's box joins its mesh's root keys
-__global__ __launch_bounds__(BS) void root_reduce_kernel(const float4 *__restrict__ rmin,
-                                                         const float4 *__restrict__ rmax, int n,
-                                                         const int *__restrict__ off, int M,
-                                                         int *__restrict__ keys) {
-    const int i = blockIdx.x * BS + threadIdx.x;
-    const bool valid = i < n;
-    int m = 0;
-    if (valid) {
-        int lo = 0, hi = M - 1;                      // last mesh with off[m] <= i
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (off[mid] <= i) lo = mid; else hi = mid - 1;
-        }
-        m = lo;
+// mesh of triangle i: the last m with off[m] <= i, searched in [lo, hi]
+__device__ __forceinline__ int mesh_of(const int *__restrict__ off, int i, int lo, int hi) {
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= i) lo = mid; else hi = mid - 1;
     }
-    const int m0 = __shfl(m, 0, 64);
-    const unsigned long long act = __ballot(valid);
-    const bool uniform = __ballot(valid && m != m0) == 0ull && act != 0ull;
-    int k0 = fkey(F_INF), k1 = k0, k2 = k0, k3 = fkey(-F_INF), k4 = k3, k5 = k3;
-    if (valid) {
-        const float4 a = rmin[i], b = rmax[i];
-        k0 = fkey(a.x); k1 = fkey(a.y); k2 = fkey(a.z); k3 = fkey(b.x); k4 = fkey(b.y); k5 = fkey(b.z);
-    }
-    if (uniform) {
-        k0 = wave_min(k0); k1 = wave_min(k1); k2 = wave_min(k2);
-        k3 = wave_max(k3); k4 = wave_max(k4); k5 = wave_max(k5);
-        if (__lane_id() != 0) return;
-    } else if (!valid) {
-        return;
-    }
-    int *k = keys + 6 * m;
+    return lo;
+}
+
+__device__ __forceinline__ void key_atomics(int *k, int k0, int k1, int k2, int k3, int k4, int k5) {
     atomicMin(k + 0, k0); atomicMin(k + 1, k1); atomicMin(k + 2, k2);
     atomicMax(k + 3, k3); atomicMax(k + 4, k4); atomicMax(k + 5, k5);
 }
 
-// CalcPrimitiveAabb (Kernels/TriangleUpper.h:10-28) + root box reduction
+constexpr int BOX_TILE = BS * 8;   // consecutive triangles per block
+
+// CalcPrimitiveAabb (Kernels/TriangleUpper.h:10-28) + the root boxes.  A
+// block takes BOX_TILE consecutive triangles; a tile inside one mesh is
+// reduced in the block and joins that mesh's root keys with 6 atomics, a
+// tile that crosses mesh boundaries reduces wave by wave (per lane where a
+// wave itself straddles a boundary).
 __global__ __launch_bounds__(BS) void tri_box_kernel(const float4 *__restrict__ P0,
                                                      const float4 *__restrict__ P1,
                                                      const float4 *__restrict__ P2, int n,
                                                      float4 *__restrict__ rmin,
                                                      float4 *__restrict__ rmax,
+                                                     const int *__restrict__ off, int M,
                                                      int *__restrict__ keys) {
     __shared__ int red[6][BS / 64];
+    const int t0 = blockIdx.x * BOX_TILE;
+    const int t1 = min(t0 + BOX_TILE, n) - 1;
+    const int m0 = M > 1 ? mesh_of(off, t0, 0, M - 1) : 0;
+    const int m1 = M > 1 ? mesh_of(off, t1, m0, M - 1) : 0;
+    const bool tile_uniform = m0 == m1;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int k0 = fkey(F_INF), k1 = k0, k2 = k0, k3 = fkey(-F_INF), k4 = k3, k5 = k3;
-    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
-        const float4 a = P0[i], b = P1[i], c = P2[i];
-        float mnx = a.x, mny = a.y, mnz = a.z, mxx = a.x, mxy = a.y, mxz = a.z;
-        mnx = omin(mnx, b.x); mny = omin(mny, b.y); mnz = omin(mnz, b.z);
-        mxx = omax(mxx, b.x); mxy = omax(mxy, b.y); mxz = omax(mxz, b.z);
-        mnx = omin(mnx, c.x); mny = omin(mny, c.y); mnz = omin(mnz, c.z);
-        mxx = omax(mxx, c.x); mxy = omax(mxy, c.y); mxz = omax(mxz, c.z);
-        rmin[i] = make_float4(mnx, mny, mnz, __int_as_float(i));
-        rmax[i] = make_float4(mxx, mxy, mxz, 0.0f);
-        k0 = min(k0, fkey(mnx)); k1 = min(k1, fkey(mny)); k2 = min(k2, fkey(mnz));
-        k3 = max(k3, fkey(mxx)); k4 = max(k4, fkey(mxy)); k5 = max(k5, fkey(mxz));
+    for (int j = 0; j < BOX_TILE / BS; ++j) {
+        const int i = t0 + j * BS + threadIdx.x;
+        const bool valid = i < n;
+        int e0 = fkey(F_INF), e1 = e0, e2 = e0, e3 = fkey(-F_INF), e4 = e3, e5 = e3;
+        if (valid) {
+            const float4 a = P0[i], b = P1[i], c = P2[i];
+            float mnx = a.x, mny = a.y, mnz = a.z, mxx = a.x, mxy = a.y, mxz = a.z;
+            mnx = omin(mnx, b.x); mny = omin(mny, b.y); mnz = omin(mnz, b.z);
+            mxx = omax(mxx, b.x); mxy = omax(mxy, b.y); mxz = omax(mxz, b.z);
+            mnx = omin(mnx, c.x); mny = omin(mny, c.y); mnz = omin(mnz, c.z);
+            mxx = omax(mxx, c.x); mxy = omax(mxy, c.y); mxz = omax(mxz, c.z);
+            rmin[i] = make_float4(mnx, mny, mnz, __int_as_float(i));
+            rmax[i] = make_float4(mxx, mxy, mxz, 0.0f);
+            e0 = fkey(mnx); e1 = fkey(mny); e2 = fkey(mnz); e3 = fkey(mxx); e4 = fkey(mxy); e5 = fkey(mxz);
+        }
+        if (tile_uniform) {
+            k0 = min(k0, e0); k1 = min(k1, e1); k2 = min(k2, e2);
+            k3 = max(k3, e3); k4 = max(k4, e4); k5 = max(k5, e5);
+            continue;
+        }
+        // the tile crosses a mesh boundary: reduce per wave where the wave
+        // is inside one mesh, per lane where it is not
+        const int m = valid ? mesh_of(off, i, m0, m1) : -1;
+        const int wm = __shfl(m, 0, 64);
+        const bool wave_uniform = __ballot(valid && m != wm) == 0ull;
+        if (wave_uniform) {
+            if (wm < 0) continue;              // no valid lane (lane 0 invalid => all invalid)
+            e0 = wave_min(e0); e1 = wave_min(e1); e2 = wave_min(e2);
+            e3 = wave_max(e3); e4 = wave_max(e4); e5 = wave_max(e5);
+            if (lane == 0) key_atomics(keys + 6 * wm, e0, e1, e2, e3, e4, e5);
+        } else if (valid) {
+            key_atomics(keys + 6 * m, e0, e1, e2, e3, e4, e5);
+        }
     }
-    if (!keys) return;                           // forest: roots reduced per mesh
+    if (!tile_uniform) return;                 // block-uniform: no thread passes the barrier
     k0 = wave_min(k0); k1 = wave_min(k1); k2 = wave_min(k2);
     k3 = wave_max(k3); k4 = wave_max(k4); k5 = wave_max(k5);
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (lane == 0) { red[0][w] = k0; red[1][w] = k1; red[2][w] = k2; red[3][w] = k3; red[4][w] = k4; red[5][w] = k5; }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -217,8 +252,7 @@ __global__ __launch_bounds__(BS) void tri_box_kernel(const float4 *__restrict__ 
             k0 = min(k0, red[0][j]); k1 = min(k1, red[1][j]); k2 = min(k2, red[2][j]);
             k3 = max(k3, red[3][j]); k4 = max(k4, red[4][j]); k5 = max(k5, red[5][j]);
         }
-        atomicMin(keys + 0, k0); atomicMin(keys + 1, k1); atomicMin(keys + 2, k2);
-        atomicMax(keys + 3, k3); atomicMax(keys + 4, k4); atomicMax(keys + 5, k5);
+        key_atomics(keys + 6 * m0, k0, k1, k2, k3, k4, k5);
     }
 }
 
@@ -566,16 +600,6 @@ __global__ void bitmap_leaves_kernel(NodePtrs N, const int *__restrict__ leaf_id
     N.bitmap[leaf] = cnt >= 64 ? -1ll : (long long)((1ull << cnt) - 1ull);
 }
 
-// CalcSurfaceArea (LowerTriangleMap.h:24-38)
-__global__ void area_kernel(const float4 *__restrict__ P0, const float4 *__restrict__ P1,
-                            const float4 *__restrict__ P2, const int *__restrict__ prim_indices,
-                            int n, float *__restrict__ areas) {
-    const int k = blockIdx.x * BS + threadIdx.x;
-    if (k >= n) return;
-    const int t = prim_indices[k];
-    areas[k] = tri_area(mk3(P0[t]), mk3(P1[t]), mk3(P2[t]));
-}
-
 // PreprocesLowerNodes (LowerTriangleMap.h:56-81)
 __global__ void lower_prep_kernel(NodePtrs N, const int *__restrict__ leaf_ids, int n,
                                   const float *__restrict__ areas) {
@@ -594,6 +618,51 @@ __global__ void lower_prep_kernel(NodePtrs N, const int *__restrict__ leaf_ids, 
     N.area[leaf] = a;
     N.bitmap[leaf] = (long long)bmp;
 }
+
+// Exact early rejection for the SAH decision.  Every candidate plane puts
+// each prim on at least one side, so a candidate with k prims on the left
+// and m >= n - k on the right costs at least k s_k + (n - k) s_(n-k), where
+// s_j is the sum of the j smallest areas (the empty-side factor only ever
+// scales a cost by min(bonus, 1)).  When the smallest such bound, less a
+// margin far above the FP32 rounding of any candidate's cost (sums of <= 64
+// positive terms: relative error < 1e-5), is still >= the leaf cost, no
+// candidate can win and the node stays a leaf, exactly as the full search
+// would decide.  Non-finite areas disable the shortcut.
+__device__ bool sah_cannot_split(bool has, float a, int n, float bonus, float leaf_cost) {
+    const int lane = __lane_id();
+    if (__ballot(has && !(a <= 3.0e38f)) != 0ull) return false;
+    float v = has ? a : __int_as_float(0x7f800000);
+    // bitonic sort of the 64 lanes, ascending
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const float o = __shfl_xor(v, j, 64);
+            const bool up = (lane & k) == 0 || k == 64;
+            const bool low = (lane & j) == 0;
+            v = (low == up) ? fminf(v, o) : fmaxf(v, o);
+        }
+    }
+    // inclusive prefix sums: lane j holds s_(j+1)
+    float pre = lane < n ? v : 0.0f;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const float o = __shfl_up(pre, off, 64);
+        if (lane >= off) pre += o;
+    }
+    const int k = lane, m = n - lane;
+    const float sk = __shfl(pre, k > 0 ? k - 1 : 0, 64);
+    const float sm = __shfl(pre, m > 0 ? m - 1 : 0, 64);
+    float f = __int_as_float(0x7f800000);
+    if (k <= n) {
+        f = (float)k * (k > 0 ? sk : 0.0f) + (float)m * (m > 0 ? sm : 0.0f);
+        if (k == 0 || m == 0) f *= fminf(bonus, 1.0f);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) f = fminf(f, __shfl_xor(f, off, 64));
+    return f * (1.0f - 1.0f / 1024.0f) >= leaf_cost;
+}
+
 
 // CalcSAH + CalcAreaForSets (LowerTriangleMap.h:412-555), one wave per node.
 constexpr int MAXC = 6 * 64;
@@ -619,9 +688,14 @@ __global__ __launch_bounds__(WPB * 64) void sah_kernel(
         return;
     }
     const bool has = (bmp >> lane) & 1ull;
+    float a = has ? areas[pi + lane] : 0.0f;
+    const float leaf_cost = ((float)n - Ct) * nodeA;
+    if (!forced && sah_cannot_split(has, a, n, bonus, leaf_cost)) {
+        if (lane == 0) { N.info[par] = LEAF; flag[id] = 0; }
+        return;
+    }
     f3 mn = {0.0f, 0.0f, 0.0f}, mx = {0.0f, 0.0f, 0.0f};
-    float a = 0.0f;
-    if (has) { mn = mk3(pmin[pi + lane]); mx = mk3(pmax[pi + lane]); a = areas[pi + lane]; }
+    if (has) { mn = mk3(pmin[pi + lane]); mx = mk3(pmax[pi + lane]); }
     ar[w][lane] = a;
     // candidate planes (CreateSplittingPlanes, LowerTriangleMap.h:84-155):
     // prim ascending, axis X/Y/Z, low plane then high plane
@@ -673,7 +747,6 @@ __global__ __launch_bounds__(WPB * 64) void sah_kernel(
         for (unsigned long long m = L; m; m &= m - 1) aL += ar[w][__builtin_ctzll(m)];
         for (unsigned long long m = R; m; m &= m - 1) aR += ar[w][__builtin_ctzll(m)];
         const bool progress = L != bmp && R != bmp;
-        const float leaf_cost = ((float)n - Ct) * nodeA;
         split = progress && (forced || best < leaf_cost);
         if (split) {
             const int rank = bestc / 6;
@@ -776,27 +849,44 @@ void readback(BuildScratch &sc, std::initializer_list<const int *> srcs, int *ds
 
 }  // namespace
 
-// exclusive scan, out[n] = total: one pass when n fits a block, else
+// exclusive scans, out[n] = total: one pass when n fits a block, else
 // block scans, a scan of the block sums (recursively, in place) and an add
-static void scan_level(const int *in, int *out, int n, BuildScratch &sc, int depth, hipStream_t s) {
+template <int K>
+static void scan_level(ScanIO<K> io, int n, BuildScratch &sc, int depth, hipStream_t s) {
     const int nb = blocks_for((long long)n + 1, SCAN_BLK);
     if (nb == 1) {
-        scan_block_kernel<<<1, SCAN_T, 0, s>>>(in, out, n, nullptr);
+        scan_block_kernel<K><<<1, SCAN_T, 0, s>>>(io, n, nullptr, 0);
         PM_LAUNCH_CHECK();
         return;
     }
     if (depth >= 3) throw Error{PM_ERR_ARG, "scan too large"};
     DevBuf<int> &tmp = sc.scan_tmp[depth];
-    tmp.reserve((size_t)nb + 1, s);
-    scan_block_kernel<<<nb, SCAN_T, 0, s>>>(in, out, n, tmp.ptr);
+    const int stride = nb + 1;
+    tmp.reserve((size_t)K * stride, s);
+    scan_block_kernel<K><<<nb, SCAN_T, 0, s>>>(io, n, tmp.ptr, stride);
     PM_LAUNCH_CHECK();
-    scan_level(tmp.ptr, tmp.ptr, nb, sc, depth + 1, s);
-    scan_add_kernel<<<nb, SCAN_T, 0, s>>>(out, n, tmp.ptr);
+    ScanIO<K> sums;
+    for (int q = 0; q < K; ++q) {
+        sums.in[q] = tmp.ptr + q * stride;
+        sums.out[q] = tmp.ptr + q * stride;
+    }
+    scan_level<K>(sums, nb, sc, depth + 1, s);
+    scan_add_kernel<K><<<nb, SCAN_T, 0, s>>>(io, n, tmp.ptr, stride);
     PM_LAUNCH_CHECK();
 }
 
 void scan_exclusive(const int *in, int *out, int n, BuildScratch &sc, hipStream_t s) {
-    scan_level(in, out, n, sc, 0, s);
+    ScanIO<1> io;
+    io.in[0] = in; io.out[0] = out;
+    scan_level<1>(io, n, sc, 0, s);
+}
+
+void scan_exclusive2(const int *in0, int *out0, const int *in1, int *out1, int n, BuildScratch &sc,
+                     hipStream_t s) {
+    ScanIO<2> io;
+    io.in[0] = in0; io.out[0] = out0;
+    io.in[1] = in1; io.out[1] = out1;
+    scan_level<2>(io, n, sc, 0, s);
 }
 
 void build_tree(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2, int N,
@@ -836,15 +926,10 @@ void build_tree(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P
     init_keys_kernel<<<blocks_for(M, BS), BS, 0, s>>>(sc.keys[0].ptr, M);
     PM_LAUNCH_CHECK();
     if (N > 0) {
-        const int blocks = std::min(blocks_for(N, BS), 2048);
-        tri_box_kernel<<<blocks, BS, 0, s>>>(P0, P1, P2, N, sc.ref_min[0].ptr, sc.ref_max[0].ptr,
-                                              M == 1 ? sc.keys[0].ptr : nullptr);
+        tri_box_kernel<<<blocks_for(N, BOX_TILE), BS, 0, s>>>(P0, P1, P2, N, sc.ref_min[0].ptr,
+                                                              sc.ref_max[0].ptr, sc.mesh_off.ptr, M,
+                                                              sc.keys[0].ptr);
         PM_LAUNCH_CHECK();
-        if (M > 1) {
-            root_reduce_kernel<<<blocks_for(N, BS), BS, 0, s>>>(sc.ref_min[0].ptr, sc.ref_max[0].ptr, N,
-                                                                sc.mesh_off.ptr, M, sc.keys[0].ptr);
-            PM_LAUNCH_CHECK();
-        }
     }
     int cur = 0;          // ping-pong index of the level's ref arrays and keys
     int active_index = 0, A = M, R = N, level = 0;
@@ -869,8 +954,7 @@ void build_tree(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P
                                                               prm.empty_space_threshold, planes,
                                                               sc.node_req.ptr, sc.node_nseg.ptr);
         PM_LAUNCH_CHECK();
-        scan_exclusive(sc.node_req.ptr, sc.node_eaddr.ptr, A, sc, s);
-        scan_exclusive(sc.node_nseg.ptr, sc.node_segbase.ptr, A, sc, s);
+        scan_exclusive2(sc.node_req.ptr, sc.node_eaddr.ptr, sc.node_nseg.ptr, sc.node_segbase.ptr, A, sc, s);
         readback(sc, {sc.node_eaddr.ptr + A, sc.node_segbase.ptr + A}, hv, s);
         const int E = hv[0], S = hv[1];
         if (E > 0) {
@@ -895,8 +979,7 @@ void build_tree(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P
                             sc.segL.ptr, sc.segR.ptr);
             PM_LAUNCH_CHECK();
         }
-        scan_exclusive(sc.segL.ptr, sc.segLoff.ptr, S, sc, s);
-        scan_exclusive(sc.segR.ptr, sc.segRoff.ptr, S, sc, s);
+        scan_exclusive2(sc.segL.ptr, sc.segLoff.ptr, sc.segR.ptr, sc.segRoff.ptr, S, sc, s);
         child_size_kernel<<<blocks_for(A, BS), BS, 0, s>>>(A, LS, sc.node_segbase.ptr, sc.segLoff.ptr,
                                                             sc.segRoff.ptr, sc.slot_leaf.ptr);
         PM_LAUNCH_CHECK();
@@ -905,8 +988,7 @@ void build_tree(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P
                                                               sc.segL.ptr, sc.segR.ptr, sc.segLL.ptr, sc.segLR.ptr);
             PM_LAUNCH_CHECK();
         }
-        scan_exclusive(sc.segLL.ptr, sc.segLLoff.ptr, S, sc, s);
-        scan_exclusive(sc.segLR.ptr, sc.segLRoff.ptr, S, sc, s);
+        scan_exclusive2(sc.segLL.ptr, sc.segLLoff.ptr, sc.segLR.ptr, sc.segLRoff.ptr, S, sc, s);
         scan_exclusive(sc.slot_leaf.ptr, sc.slot_lna.ptr, 2 * A, sc, s);
         readback(sc, {sc.segLoff.ptr + S, sc.segRoff.ptr + S, sc.segLLoff.ptr + S, sc.segLRoff.ptr + S,
                       sc.slot_lna.ptr + 2 * A}, hv, s);
@@ -961,6 +1043,12 @@ void build_tree(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P
         propagate_parent_kernel<<<blocks_for(nleaf, BS), BS, 0, s>>>(NP, ctx->leaf_ids.ptr, nleaf);
         PM_LAUNCH_CHECK();
     }
+    // the leaf slots are final here (the lower stage only splits nodes):
+    // write the tracers' leaf triangles, and the SAH's areas in the same pass
+    ctx->num_prim_indices = final_refs;
+    const bool sah = prm.lower_algorithm != PM_LOWER_BITMAP;
+    if (sah) sc.areas.reserve((size_t)final_refs + 1, s);
+    build_leaf_view(ctx, sah ? sc.areas.ptr : nullptr, false, s);
 
     // lower stage
     int lower_levels = 0;
@@ -970,11 +1058,6 @@ void build_tree(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P
             PM_LAUNCH_CHECK();
         }
     } else {
-        sc.areas.reserve((size_t)final_refs + 1, s);
-        if (final_refs > 0) {
-            area_kernel<<<blocks_for(final_refs, BS), BS, 0, s>>>(P0, P1, P2, ctx->prim_indices.ptr, final_refs, sc.areas.ptr);
-            PM_LAUNCH_CHECK();
-        }
         if (nleaf > 0) {
             lower_prep_kernel<<<blocks_for(nleaf, BS), BS, 0, s>>>(NP, ctx->leaf_ids.ptr, nleaf, sc.areas.ptr);
             PM_LAUNCH_CHECK();
@@ -1013,7 +1096,7 @@ void build_tree(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P
     ctx->num_prim_indices = final_refs;
     ctx->num_leaf_ids = nleaf;
     ctx->has_boxes = prm.propagate_aabbs != 0;
-    build_trace_view(ctx, s);
+    build_node_view(ctx, s);
     ctx->has_tree = true;
 }
 

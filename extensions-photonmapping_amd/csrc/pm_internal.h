@@ -158,6 +158,11 @@ void build_tree(pm_ctx *ctx, const float4 *p0, const float4 *p1, const float4 *p
                 const int *mesh_offsets, int M, const pm_build_params &prm, hipStream_t s);
 // pm_trace.hip
 void build_trace_view(pm_ctx *ctx, hipStream_t s);
+// the two halves of the trace view: packed nodes, and the leaf slots'
+// triangles (plus their areas when `areas` is given; `check` verifies the
+// prim indices, needed for uploaded trees only)
+void build_node_view(pm_ctx *ctx, hipStream_t s);
+void build_leaf_view(pm_ctx *ctx, float *areas, bool check, hipStream_t s);
 void trace_rays(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int nrays,
                 const int *ray_mesh, pm_hit *out, const pm_trace_params &prm, int4 *counters,
                 hipStream_t s);
@@ -182,4 +187,6 @@ void add_mesh(pm_ctx *ctx, const float4 *verts, int nv, const unsigned *idx, int
               const float *model, float4 *p0, float4 *p1, float4 *p2, hipStream_t s);
 // device-wide exclusive scan of n ints; out has n+1 entries (out[n] = total)
 void scan_exclusive(const int *in, int *out, int n, BuildScratch &sc, hipStream_t s);
+void scan_exclusive2(const int *in0, int *out0, const int *in1, int *out1, int n, BuildScratch &sc,
+                     hipStream_t s);
 }  // namespace pmkd

@@ -268,16 +268,20 @@ __global__ __launch_bounds__(TB) void pack_nodes_kernel(const int8_t *__restrict
     }
 }
 
+// leaf slot k -> v0 (w = id bits), e1, e2; with `areas`, also the slot's
+// triangle area 0.5 |e1 x e2| (CalcSurfaceArea, LowerTriangleMap.h:24-38,
+// the same operations tri_area performs)
 __global__ __launch_bounds__(TB) void leaf_tris_kernel(const int *__restrict__ prim_indices, int m,
                                                        const float4 *__restrict__ P0,
                                                        const float4 *__restrict__ P1,
                                                        const float4 *__restrict__ P2, int ntris,
-                                                       float4 *__restrict__ out, int *bad) {
+                                                       float4 *__restrict__ out, int *bad,
+                                                       float *__restrict__ areas) {
     const int k = blockIdx.x * TB + threadIdx.x;
     if (k >= m) return;
     const int t = prim_indices[k];
     if (t < 0 || t >= ntris) {
-        atomicOr(bad, 1);
+        if (bad) atomicOr(bad, 1);
         const float nan = __int_as_float(0x7fc00000);
         out[3ll * k] = make_float4(nan, nan, nan, __int_as_float(-1));
         out[3ll * k + 1] = make_float4(nan, nan, nan, 0.0f);
@@ -289,6 +293,10 @@ __global__ __launch_bounds__(TB) void leaf_tris_kernel(const int *__restrict__ p
     out[3ll * k] = make_float4(v0.x, v0.y, v0.z, __int_as_float(t));
     out[3ll * k + 1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
     out[3ll * k + 2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
+    if (areas) {
+        const f3 c = cross3(e1, e2);
+        areas[k] = 0.5f * sqrtf(dot3(c, c));
+    }
 }
 
 __global__ void clear_int_kernel(int *p) {
@@ -317,29 +325,43 @@ void dispatch(const TreeDev &T, int algo, const float4 *org, const float4 *dir, 
 
 }  // namespace
 
-void build_trace_view(pm_ctx *ctx, hipStream_t s) {
-    const int n = ctx->num_nodes, m = ctx->num_prim_indices;
+void build_node_view(pm_ctx *ctx, hipStream_t s) {
+    const int n = ctx->num_nodes;
     ctx->packed.reserve((size_t)n + 1, s);
-    ctx->leaf_tris.reserve(3 * (size_t)m + 3, s);
     if (n > 0) {
         pack_nodes_kernel<<<(n + TB - 1) / TB, TB, 0, s>>>(ctx->nodes.info.ptr, ctx->nodes.split.ptr,
                                                            ctx->nodes.children.ptr, ctx->nodes.prim_index.ptr,
                                                            ctx->nodes.bitmap.ptr, n, ctx->packed.ptr);
         PM_LAUNCH_CHECK();
     }
-    if (m > 0) {
-        BuildScratch &sc = ctx->scratch;
+}
+
+void build_leaf_view(pm_ctx *ctx, float *areas, bool check, hipStream_t s) {
+    const int m = ctx->num_prim_indices;
+    ctx->leaf_tris.reserve(3 * (size_t)m + 3, s);
+    if (m == 0) return;
+    BuildScratch &sc = ctx->scratch;
+    int *bad = nullptr;
+    if (check) {
         sc.totals.reserve(8, s);
         sc.host.ensure(8);
         clear_int_kernel<<<1, 64, 0, s>>>(sc.totals.ptr);
         PM_LAUNCH_CHECK();
-        leaf_tris_kernel<<<(m + TB - 1) / TB, TB, 0, s>>>(ctx->prim_indices.ptr, m, ctx->p0, ctx->p1, ctx->p2,
-                                                          ctx->ntris, ctx->leaf_tris.ptr, sc.totals.ptr);
-        PM_LAUNCH_CHECK();
+        bad = sc.totals.ptr;
+    }
+    leaf_tris_kernel<<<(m + TB - 1) / TB, TB, 0, s>>>(ctx->prim_indices.ptr, m, ctx->p0, ctx->p1, ctx->p2,
+                                                      ctx->ntris, ctx->leaf_tris.ptr, bad, areas);
+    PM_LAUNCH_CHECK();
+    if (check) {
         PM_HIP(hipMemcpyAsync(sc.host.ptr, sc.totals.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
         PM_HIP(hipStreamSynchronize(s));
         if (sc.host.ptr[0]) throw Error{PM_ERR_ARG, "prim index outside the triangle array"};
     }
+}
+
+void build_trace_view(pm_ctx *ctx, hipStream_t s) {
+    build_node_view(ctx, s);
+    build_leaf_view(ctx, nullptr, true, s);
 }
 
 void trace_rays(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int nrays,
