@@ -22,7 +22,7 @@ LIB_PATH = os.path.join(HERE, "libpmkd.so")
 
 PM_OK = 0
 ERRORS = {1: "PM_ERR_ARG", 2: "PM_ERR_NOT_IMPLEMENTED", 3: "PM_ERR_DEPTH", 4: "PM_ERR_INTERNAL",
-          5: "PM_ERR_UNSUPPORTED", 6: "PM_ERR_HIP", 7: "PM_ERR_NO_TREE"}
+          5: "PM_ERR_UNSUPPORTED", 6: "PM_ERR_HIP", 7: "PM_ERR_NO_TREE", 8: "PM_ERR_CAPACITY"}
 LOWER_BITMAP, LOWER_BALANCED, LOWER_SAH = 0, 1, 2
 SPLIT_BOX, SPLIT_DIVIDE, SPLIT_SPLIT = 0, 1, 2
 BRUTE, KD_RESTART, SHORT_STACK = 0, 1, 2
@@ -32,7 +32,7 @@ EXPORTS = ("pm_build_params_default", "pm_trace_params_default", "pm_create", "p
            "pm_last_error", "pm_build", "pm_set_geometry", "pm_tree_view", "pm_set_tree",
            "pm_tree_stats", "pm_trace", "pm_photon_bounce", "pm_add_mesh", "pm_trace_backward",
            "pm_build_batched", "pm_trace_batched", "pm_osc_surface", "pm_deform_radial",
-           "pm_autotune")
+           "pm_autotune", "pm_batch_graph_capture", "pm_batch_graph_launch", "pm_batch_graph_status")
 
 
 class PmError(RuntimeError):
@@ -65,6 +65,12 @@ class Tree(C.Structure):
                 ("p1", C.c_void_p), ("p2", C.c_void_p), ("num_tris", C.c_int),
                 ("upper_levels", C.c_int), ("lower_levels", C.c_int), ("has_boxes", C.c_int),
                 ("num_roots", C.c_int)]
+
+
+class GraphCaps(C.Structure):
+    _fields_ = [("nodes", C.c_longlong), ("active", C.c_longlong), ("segments", C.c_longlong),
+                ("level_refs", C.c_longlong), ("leaf_refs", C.c_longlong), ("leaf_ids", C.c_longlong),
+                ("lower_nodes", C.c_longlong), ("upper_levels", C.c_int), ("lower_levels", C.c_int)]
 
 
 class Tuning(C.Structure):
@@ -117,10 +123,15 @@ def load_library(path: str = LIB_PATH):
     L.pm_deform_radial.argtypes = [P, P, I, F, P, P, P]
     L.pm_autotune.argtypes = [P, P, P, P, I, P, I, P, P, P, I, C.POINTER(BuildParams), F, I,
                               C.POINTER(Tuning), P]
+    L.pm_batch_graph_capture.argtypes = [P, P, P, P, I, P, I, P, P, P, I, P, I, C.POINTER(BuildParams),
+                                         C.POINTER(TraceParams), C.POINTER(GraphCaps), P]
+    L.pm_batch_graph_launch.argtypes = [P, P]
+    L.pm_batch_graph_status.argtypes = [P, C.POINTER(C.c_int), P]
     for n in ("pm_create", "pm_destroy", "pm_build", "pm_set_geometry", "pm_tree_view",
               "pm_set_tree", "pm_tree_stats", "pm_trace", "pm_photon_bounce", "pm_add_mesh",
               "pm_trace_backward", "pm_build_batched", "pm_trace_batched", "pm_osc_surface",
-              "pm_deform_radial", "pm_autotune"):
+              "pm_deform_radial", "pm_autotune", "pm_batch_graph_capture", "pm_batch_graph_launch",
+              "pm_batch_graph_status"):
         getattr(L, n).restype = C.c_int
     _lib = L
     return L
@@ -272,6 +283,35 @@ class Context:
         res["build_params"] = bp
         res["trace_params"] = trace_params(stack_size=t.stack_size)
         return res
+
+    # -- one graph launch per frame
+    def batch_graph_capture(self, p0, p1, p2, mesh_offsets, org, dirs, ray_mesh, out, algo=SHORT_STACK,
+                            build: BuildParams | None = None, trace: TraceParams | None = None,
+                            caps: dict | None = None, stream=None) -> dict:
+        """record build + trace of this batch as one hipGraph (pm_batch_graph_capture);
+        returns the capacities captured.  The tensors must stay alive while the
+        graph is replayed."""
+        off = np.ascontiguousarray(mesh_offsets, dtype=np.int32)
+        build = build or build_params()
+        trace = trace or trace_params()
+        c = GraphCaps()
+        for k, v in (caps or {}).items():
+            setattr(c, k, v)
+        self._geom = (p0, p1, p2)
+        self._graph_keep = (p0, p1, p2, org, dirs, ray_mesh, out, off)
+        self._check(self._L.pm_batch_graph_capture(
+            self.h, _dev_ptr(p0), _dev_ptr(p1), _dev_ptr(p2), p0.shape[0], off.ctypes.data, off.shape[0] - 1,
+            _dev_ptr(org), _dev_ptr(dirs), _dev_ptr(ray_mesh), org.shape[0], _dev_ptr(out), algo,
+            C.byref(build), C.byref(trace), C.byref(c), _stream_ptr(stream)))
+        return {k: getattr(c, k) for k, _ in GraphCaps._fields_}
+
+    def batch_graph_launch(self, stream=None):
+        self._check(self._L.pm_batch_graph_launch(self.h, _stream_ptr(stream)))
+
+    def batch_graph_status(self, stream=None) -> int:
+        st = C.c_int(0)
+        self._check(self._L.pm_batch_graph_status(self.h, C.byref(st), _stream_ptr(stream)))
+        return st.value
 
     def set_geometry(self, p0, p1, p2):
         self._geom = (p0, p1, p2)

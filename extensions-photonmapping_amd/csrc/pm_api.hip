@@ -6,6 +6,11 @@
 
 using pmkd::Error;
 
+pm_ctx::~pm_ctx() {
+    pmkd::destroy_graph(this);
+    if (capture_stream) (void)hipStreamDestroy(capture_stream);
+}
+
 namespace {
 
 int fail(pm_ctx *ctx, int code, const std::string &msg) {
@@ -95,6 +100,52 @@ int pm_destroy(pm_ctx *ctx) {
 
 const char *pm_last_error(const pm_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+int pm_batch_graph_capture(pm_ctx *ctx, const float *p0, const float *p1, const float *p2, int n,
+                           const int *mesh_offsets, int num_meshes, const float *org, const float *dir,
+                           const int *ray_mesh, int nrays, pm_hit *out, int algo,
+                           const pm_build_params *build_params, const pm_trace_params *trace_params,
+                           pm_graph_caps *caps, void *stream) {
+    return guarded(ctx, [&] {
+        if (n < 0 || (n > 0 && (!p0 || !p1 || !p2))) throw Error{PM_ERR_ARG, "invalid geometry"};
+        if (num_meshes < 1 || !mesh_offsets) throw Error{PM_ERR_ARG, "need at least one mesh and its offsets"};
+        if (nrays < 0 || (nrays > 0 && (!org || !dir || !out || !ray_mesh))) throw Error{PM_ERR_ARG, "invalid rays"};
+        if (algo != PM_TRACE_KD_RESTART && algo != PM_TRACE_SHORT_STACK)
+            throw Error{PM_ERR_ARG, "a captured batch traces with KD_RESTART or SHORT_STACK"};
+        pm_build_params bp;
+        if (build_params) bp = *build_params; else pm_build_params_default(&bp);
+        check_build_params(bp);
+        pm_trace_params tp;
+        if (trace_params) tp = *trace_params; else pm_trace_params_default(&tp);
+        if (tp.stack_size < 1 || tp.stack_size > 16) throw Error{PM_ERR_ARG, "stack_size must be in [1, 16]"};
+        pmkd::BuildSizes c;
+        if (caps) {
+            c.nodes = caps->nodes; c.active = caps->active; c.segs = caps->segments; c.refs = caps->level_refs;
+            c.leaf_refs = caps->leaf_refs; c.leaf_ids = caps->leaf_ids; c.lower = caps->lower_nodes;
+            c.upper_levels = caps->upper_levels; c.lower_levels = caps->lower_levels;
+        }
+        pmkd::capture_batch(ctx, reinterpret_cast<const float4 *>(p0), reinterpret_cast<const float4 *>(p1),
+                            reinterpret_cast<const float4 *>(p2), n, mesh_offsets, num_meshes,
+                            reinterpret_cast<const float4 *>(org), reinterpret_cast<const float4 *>(dir), ray_mesh,
+                            nrays, out, algo, bp, tp, c, (hipStream_t)stream);
+        if (caps) {
+            caps->nodes = c.nodes; caps->active = c.active; caps->segments = c.segs; caps->level_refs = c.refs;
+            caps->leaf_refs = c.leaf_refs; caps->leaf_ids = c.leaf_ids; caps->lower_nodes = c.lower;
+            caps->upper_levels = c.upper_levels; caps->lower_levels = c.lower_levels;
+        }
+    });
+}
+
+int pm_batch_graph_launch(pm_ctx *ctx, void *stream) {
+    return guarded(ctx, [&] { pmkd::launch_batch(ctx, (hipStream_t)stream); });
+}
+
+int pm_batch_graph_status(pm_ctx *ctx, int *build_status, void *stream) {
+    return guarded(ctx, [&] {
+        const int st = pmkd::sync_graph_state(ctx, (hipStream_t)stream);
+        if (build_status) *build_status = st;
+    });
+}
+
 int pm_build(pm_ctx *ctx, const float *p0, const float *p1, const float *p2, int n,
              const pm_build_params *params, void *stream) {
     return guarded(ctx, [&] {
@@ -135,6 +186,7 @@ int pm_set_geometry(pm_ctx *ctx, const float *p0, const float *p1, const float *
 int pm_tree_view(pm_ctx *ctx, pm_tree *out) {
     return guarded(ctx, [&] {
         if (!out) throw Error{PM_ERR_ARG, "null output"};
+        pmkd::sync_graph_state(ctx, nullptr);
         if (!ctx->has_tree) throw Error{PM_ERR_NO_TREE, "no tree: call pm_build or pm_set_tree first"};
         pm_tree &t = *out;
         t.num_nodes = ctx->num_nodes;
@@ -173,6 +225,8 @@ int pm_set_tree(pm_ctx *ctx, const pm_tree *h, void *stream) {
         const size_t n = (size_t)h->num_nodes;
         const int roots = h->num_roots > 0 ? h->num_roots : 1;
         if ((size_t)roots > n) throw Error{PM_ERR_ARG, "more roots than nodes"};
+        if (ctx->graph_pending) PM_HIP(hipStreamSynchronize(ctx->graph_last_stream));
+        ctx->graph_pending = false;
         ctx->has_tree = false;
         // every index the tracers follow must stay inside its array
         for (size_t i = 0; i < n; ++i) {
@@ -224,8 +278,9 @@ int pm_set_tree(pm_ctx *ctx, const pm_tree *h, void *stream) {
 int pm_tree_stats(pm_ctx *ctx, pm_stats *out, void *stream) {
     return guarded(ctx, [&] {
         if (!out) throw Error{PM_ERR_ARG, "null output"};
-        if (!ctx->has_tree) throw Error{PM_ERR_NO_TREE, "no tree"};
         hipStream_t s = (hipStream_t)stream;
+        pmkd::sync_graph_state(ctx, s);
+        if (!ctx->has_tree) throw Error{PM_ERR_NO_TREE, "no tree"};
         const int n = ctx->num_nodes;
         std::vector<int8_t> info(n);
         std::vector<long long> bmp(n);

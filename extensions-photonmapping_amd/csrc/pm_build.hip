@@ -57,10 +57,14 @@ struct ScanIO {
     int *out[K];
 };
 
+// n = n_dev ? n_mult * *n_dev : n_host (a captured build scans lengths it
+// never reads back; the grid then covers the capacity, and blocks past n
+// write zero block sums)
 template <int K>
-__global__ __launch_bounds__(SCAN_T) void scan_block_kernel(ScanIO<K> io, int n, int *__restrict__ block_sums,
-                                                            int sums_stride) {
+__global__ __launch_bounds__(SCAN_T) void scan_block_kernel(ScanIO<K> io, int n_host, const int *n_dev, int n_mult,
+                                                            int *__restrict__ block_sums, int sums_stride) {
     __shared__ int wsum[K][SCAN_T / 64];
+    const int n = n_dev ? n_mult * *n_dev : n_host;
     const long long base = (long long)blockIdx.x * SCAN_BLK + (long long)threadIdx.x * SCAN_E;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     int v[K][SCAN_E], tsum[K], incl[K];
@@ -101,8 +105,9 @@ __global__ __launch_bounds__(SCAN_T) void scan_block_kernel(ScanIO<K> io, int n,
 }
 
 template <int K>
-__global__ __launch_bounds__(SCAN_T) void scan_add_kernel(ScanIO<K> io, int n,
+__global__ __launch_bounds__(SCAN_T) void scan_add_kernel(ScanIO<K> io, int n_host, const int *n_dev, int n_mult,
                                                           const int *__restrict__ offs, int offs_stride) {
+    const int n = n_dev ? n_mult * *n_dev : n_host;
     const long long base = (long long)blockIdx.x * SCAN_BLK;
 #pragma unroll
     for (int q = 0; q < K; ++q) {
@@ -112,13 +117,6 @@ __global__ __launch_bounds__(SCAN_T) void scan_add_kernel(ScanIO<K> io, int n,
             if (k <= n) io.out[q][k] += add;
         }
     }
-}
-
-// copy up to 8 ints (given by device pointers) into dst, for one read-back
-struct Gather { const int *src[8]; int count; };
-__global__ void gather_kernel(Gather g, int *dst) {
-    const int i = threadIdx.x;
-    if (i < g.count) dst[i] = *g.src[i];
 }
 
 // ------------------------------------------------------------------ helpers
@@ -257,227 +255,243 @@ __global__ __launch_bounds__(BS) void tri_box_kernel(const float4 *__restrict__ 
 }
 
 // ------------------------------------------------------------------ level
+// Every level kernel reads its sizes from the device-side level state
+// (Lvl), written by the plan kernels between phases, and loops over its
+// items grid-stride.  The eager build launches grids sized from the state it
+// read back; a captured build launches fixed grids and reads nothing back.
+#define GRID_STRIDE(i, n) for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
+
+__device__ __forceinline__ int wave_index() { return (blockIdx.x * blockDim.x + threadIdx.x) >> 6; }
+__device__ __forceinline__ int wave_count() { return (gridDim.x * blockDim.x) >> 6; }
+
 // ReduceAabb result -> node box, CalcEmptySpaceSplits
 // (EmptySpaceSplitting.h:10-70), CalcUpperNodeSplitInfo
 // (ReduceSegments.h:466-484), segments per node.
-__global__ __launch_bounds__(BS) void node_prepare_kernel(NodePtrs N, int active_index, int A,
-                                                          int seg, const int *__restrict__ keys,
+__global__ __launch_bounds__(BS) void node_prepare_kernel(NodePtrs N, const Lvl *__restrict__ L, int seg,
+                                                          const int *__restrict__ keys,
                                                           int empty_space, float thr,
                                                           int *__restrict__ planes_out,
                                                           int *__restrict__ req,
                                                           int *__restrict__ nseg) {
-    const int i = blockIdx.x * BS + threadIdx.x;
-    if (i >= A) return;
-    const int node = active_index + i;
-    const f3 rMin = {unkey(keys[6 * i + 0]), unkey(keys[6 * i + 1]), unkey(keys[6 * i + 2])};
-    const f3 rMax = {unkey(keys[6 * i + 3]), unkey(keys[6 * i + 4]), unkey(keys[6 * i + 5])};
-    int pl = 0;
-    if (empty_space && active_index != 0) {
-        const f3 pMin = mk3(N.bmin[node]), pMax = mk3(N.bmax[node]);
-        if (rMin.x - pMin.x > thr * (pMax.x - pMin.x)) pl |= 1;
-        if (rMin.y - pMin.y > thr * (pMax.y - pMin.y)) pl |= 2;
-        if (rMin.z - pMin.z > thr * (pMax.z - pMin.z)) pl |= 4;
-        if (pMax.x - rMax.x > thr * (pMax.x - pMin.x)) pl |= 8;
-        if (pMax.y - rMax.y > thr * (pMax.y - pMin.y)) pl |= 16;
-        if (pMax.z - rMax.z > thr * (pMax.z - pMin.z)) pl |= 32;
+    const int A = L->A, active_index = L->active_index;
+    GRID_STRIDE(i, A) {
+        const int node = active_index + i;
+        const f3 rMin = {unkey(keys[6 * i + 0]), unkey(keys[6 * i + 1]), unkey(keys[6 * i + 2])};
+        const f3 rMax = {unkey(keys[6 * i + 3]), unkey(keys[6 * i + 4]), unkey(keys[6 * i + 5])};
+        int pl = 0;
+        if (empty_space && active_index != 0) {
+            const f3 pMin = mk3(N.bmin[node]), pMax = mk3(N.bmax[node]);
+            if (rMin.x - pMin.x > thr * (pMax.x - pMin.x)) pl |= 1;
+            if (rMin.y - pMin.y > thr * (pMax.y - pMin.y)) pl |= 2;
+            if (rMin.z - pMin.z > thr * (pMax.z - pMin.z)) pl |= 4;
+            if (pMax.x - rMax.x > thr * (pMax.x - pMin.x)) pl |= 8;
+            if (pMax.y - rMax.y > thr * (pMax.y - pMin.y)) pl |= 16;
+            if (pMax.z - rMax.z > thr * (pMax.z - pMin.z)) pl |= 32;
+        }
+        planes_out[i] = pl;
+        req[i] = 2 * __popc(pl);
+        N.bmin[node] = make_float4(rMin.x, rMin.y, rMin.z, 0.0f);
+        N.bmax[node] = make_float4(rMax.x, rMax.y, rMax.z, 0.0f);
+        const f3 sz = sub3(rMax, rMin);
+        const f3 med = {sz.x * 0.5f + rMin.x, sz.y * 0.5f + rMin.y, sz.z * 0.5f + rMin.z};
+        const bool yAboveX = sz.x < sz.y;
+        const float mxs = yAboveX ? sz.y : sz.x;
+        const float spl = yAboveX ? med.y : med.x;
+        const int ax = yAboveX ? AX_Y : AX_X;
+        const bool zHigher = mxs < sz.z;
+        N.split[node] = zHigher ? med.z : spl;
+        N.info[node] = (int8_t)(zHigher ? AX_Z : ax);
+        const int cnt = (int)N.bitmap[node];
+        nseg[i] = (cnt + seg - 1) / seg;
     }
-    planes_out[i] = pl;
-    req[i] = 2 * __popc(pl);
-    N.bmin[node] = make_float4(rMin.x, rMin.y, rMin.z, 0.0f);
-    N.bmax[node] = make_float4(rMax.x, rMax.y, rMax.z, 0.0f);
-    const f3 sz = sub3(rMax, rMin);
-    const f3 med = {sz.x * 0.5f + rMin.x, sz.y * 0.5f + rMin.y, sz.z * 0.5f + rMin.z};
-    const bool yAboveX = sz.x < sz.y;
-    const float mxs = yAboveX ? sz.y : sz.x;
-    const float spl = yAboveX ? med.y : med.x;
-    const int ax = yAboveX ? AX_Y : AX_X;
-    const bool zHigher = mxs < sz.z;
-    N.split[node] = zHigher ? med.z : spl;
-    N.info[node] = (int8_t)(zHigher ? AX_Z : ax);
-    const int cnt = (int)N.bitmap[node];
-    nseg[i] = (cnt + seg - 1) / seg;
 }
 
 // EmptySpaceSplitting2 (EmptySpaceSplitting.h:74-184): a chain of split
 // nodes with empty leaves between the node and its parent.
-__global__ __launch_bounds__(BS) void empty_chain_kernel(NodePtrs N, int active_index, int A,
+__global__ __launch_bounds__(BS) void empty_chain_kernel(NodePtrs N, const Lvl *__restrict__ L,
                                                          const int *__restrict__ planes_in,
                                                          const int *__restrict__ eaddr) {
-    const int i = blockIdx.x * BS + threadIdx.x;
-    if (i >= A) return;
-    int pl = planes_in[i];
-    if (!pl) return;
-    const int leafID = active_index + i;
-    const int start = active_index + A + eaddr[i];
-    const int cnt = 2 * __popc(pl);
-    for (int j = 0; j < cnt; ++j) zero_node(N, start + j);
-    const int parentID = N.parent[leafID];
-    if (N.children[parentID].x == leafID) N.children[parentID].x = start;
-    else N.children[parentID].y = start;
-    const f3 minP = mk3(N.bmin[leafID]), maxP = mk3(N.bmax[leafID]);
-    int pid = start;
-    for (int a = 0; a < 3; ++a) {
-        if (!(pl & (1 << a))) continue;
-        pl -= 1 << a;
-        N.info[pid] = (int8_t)(AX_X + a);
-        N.split[pid] = a == 0 ? minP.x : (a == 1 ? minP.y : minP.z);
-        const int left = pid + 1, right = pl ? left + 1 : leafID;
-        N.children[pid] = make_int2(left, right);
-        N.info[left] = LEAF; N.bitmap[left] = 0;
-        pid = right;
-    }
-    for (int a = 0; a < 3; ++a) {
-        if (!(pl & (8 << a))) continue;
-        pl -= 8 << a;
-        // the max-z case is an X split at max.x (EmptySpaceSplitting.h:172-175)
-        N.info[pid] = (int8_t)(a == 1 ? AX_Y : AX_X);
-        N.split[pid] = a == 1 ? maxP.y : maxP.x;
-        const int right = pid + 1, left = pl ? right + 1 : leafID;
-        N.children[pid] = make_int2(left, right);
-        N.info[right] = LEAF; N.bitmap[right] = 0;
-        pid = left;
+    const int A = L->E > 0 ? L->A : 0, active_index = L->active_index;
+    GRID_STRIDE(i, A) {
+        int pl = planes_in[i];
+        if (!pl) continue;
+        const int leafID = active_index + i;
+        const int start = active_index + A + eaddr[i];
+        const int cnt = 2 * __popc(pl);
+        for (int j = 0; j < cnt; ++j) zero_node(N, start + j);
+        const int parentID = N.parent[leafID];
+        if (N.children[parentID].x == leafID) N.children[parentID].x = start;
+        else N.children[parentID].y = start;
+        const f3 minP = mk3(N.bmin[leafID]), maxP = mk3(N.bmax[leafID]);
+        int pid = start;
+        for (int a = 0; a < 3; ++a) {
+            if (!(pl & (1 << a))) continue;
+            pl -= 1 << a;
+            N.info[pid] = (int8_t)(AX_X + a);
+            N.split[pid] = a == 0 ? minP.x : (a == 1 ? minP.y : minP.z);
+            const int left = pid + 1, right = pl ? left + 1 : leafID;
+            N.children[pid] = make_int2(left, right);
+            N.info[left] = LEAF; N.bitmap[left] = 0;
+            pid = right;
+        }
+        for (int a = 0; a < 3; ++a) {
+            if (!(pl & (8 << a))) continue;
+            pl -= 8 << a;
+            // the max-z case is an X split at max.x (EmptySpaceSplitting.h:172-175)
+            N.info[pid] = (int8_t)(a == 1 ? AX_Y : AX_X);
+            N.split[pid] = a == 1 ? maxP.y : maxP.x;
+            const int right = pid + 1, left = pl ? right + 1 : leafID;
+            N.children[pid] = make_int2(left, right);
+            N.info[right] = LEAF; N.bitmap[right] = 0;
+            pid = left;
+        }
     }
 }
 
 // segment table (Segment, TriangleMapUpperCreator.cu:207-239)
-__global__ __launch_bounds__(BS) void seg_fill_kernel(const NodePtrs N, int active_index, int A,
+__global__ __launch_bounds__(BS) void seg_fill_kernel(const NodePtrs N, const Lvl *__restrict__ L,
                                                       int seg, const int *__restrict__ segbase,
                                                       int *__restrict__ seg_node,
                                                       int *__restrict__ seg_start,
                                                       int *__restrict__ seg_count) {
-    const int i = blockIdx.x * BS + threadIdx.x;
-    if (i >= A) return;
-    const int node = active_index + i;
-    const int pi = N.prim_index[node];
-    const int cnt = (int)N.bitmap[node];
-    const int b = segbase[i], e = segbase[i + 1];
-    for (int s = b; s < e; ++s) {
-        const int off = (s - b) * seg;
-        seg_node[s] = i;
-        seg_start[s] = pi + off;
-        seg_count[s] = min(seg, cnt - off);
+    const int A = L->A, active_index = L->active_index;
+    GRID_STRIDE(i, A) {
+        const int node = active_index + i;
+        const int pi = N.prim_index[node];
+        const int cnt = (int)N.bitmap[node];
+        const int b = segbase[i], e = segbase[i + 1];
+        for (int s = b; s < e; ++s) {
+            const int off = (s - b) * seg;
+            seg_node[s] = i;
+            seg_start[s] = pi + off;
+            seg_count[s] = min(seg, cnt - off);
+        }
     }
 }
 
 // SetSplitSide (TriangleUpperChildren.h:10-62) counted per segment
 template <int SEGT>
 __global__ __launch_bounds__(WPB * 64) void classify_kernel(
-    const NodePtrs N, int active_index, int S, const int *__restrict__ seg_node,
+    const NodePtrs N, const Lvl *__restrict__ L, const int *__restrict__ seg_node,
     const int *__restrict__ seg_start, const int *__restrict__ seg_count,
     const float4 *__restrict__ rmin, const float4 *__restrict__ rmax, int *__restrict__ segL,
     int *__restrict__ segR) {
-    const int s = blockIdx.x * WPB + (threadIdx.x >> 6);
-    if (s >= S) return;
+    const int S = L->S, active_index = L->active_index;
     const int lane = threadIdx.x & 63;
-    const int node = active_index + seg_node[s];
-    const int ax = N.info[node];
-    const float sp = N.split[node];
-    const int start = seg_start[s], count = seg_count[s];
-    int cl = 0, cr = 0;
+    for (int s = wave_index(); s < S; s += wave_count()) {
+        const int node = active_index + seg_node[s];
+        const int ax = N.info[node];
+        const float sp = N.split[node];
+        const int start = seg_start[s], count = seg_count[s];
+        int cl = 0, cr = 0;
 #pragma unroll
-    for (int it = 0; it < SEGT / 64; ++it) {
-        const int j = it * 64 + lane;
-        const bool valid = j < count;
-        bool l = false, r = false;
-        if (valid) {
-            l = comp4(rmin[start + j], ax) <= sp;
-            r = sp < comp4(rmax[start + j], ax);
+        for (int it = 0; it < SEGT / 64; ++it) {
+            const int j = it * 64 + lane;
+            const bool valid = j < count;
+            bool l = false, r = false;
+            if (valid) {
+                l = comp4(rmin[start + j], ax) <= sp;
+                r = sp < comp4(rmax[start + j], ax);
+            }
+            cl += __popcll(__ballot(l));
+            cr += __popcll(__ballot(r));
         }
-        cl += __popcll(__ballot(l));
-        cr += __popcll(__ballot(r));
+        if (lane == 0) { segL[s] = cl; segR[s] = cr; }
     }
-    if (lane == 0) { segL[s] = cl; segR[s] = cr; }
 }
 
 // CalcNodeChildSize (TriangleUpperChildren.h:144-162) + MarkNodeLeafs (:190-201)
-__global__ __launch_bounds__(BS) void child_size_kernel(int A, int LS,
+__global__ __launch_bounds__(BS) void child_size_kernel(const Lvl *__restrict__ L, int LS,
                                                         const int *__restrict__ segbase,
                                                         const int *__restrict__ segLoff,
                                                         const int *__restrict__ segRoff,
                                                         int *__restrict__ slot_leaf) {
-    const int i = blockIdx.x * BS + threadIdx.x;
-    if (i >= A) return;
-    const int b = segbase[i], e = segbase[i + 1];
-    const int L = segLoff[e] - segLoff[b];
-    const int R = segRoff[e] - segRoff[b];
-    slot_leaf[i] = L < LS;
-    slot_leaf[A + i] = R < LS;
+    const int A = L->A;
+    GRID_STRIDE(i, A) {
+        const int b = segbase[i], e = segbase[i + 1];
+        const int nl = segLoff[e] - segLoff[b];
+        const int nr = segRoff[e] - segRoff[b];
+        slot_leaf[i] = nl < LS;
+        slot_leaf[A + i] = nr < LS;
+    }
 }
 
 // SetPrimitiveLeafSide (TriangleUpperChildren.h:164-186) counted per segment
-__global__ __launch_bounds__(BS) void seg_leaf_kernel(int A, int S, const int *__restrict__ seg_node,
+__global__ __launch_bounds__(BS) void seg_leaf_kernel(const Lvl *__restrict__ L, const int *__restrict__ seg_node,
                                                       const int *__restrict__ slot_leaf,
                                                       const int *__restrict__ segL,
                                                       const int *__restrict__ segR,
                                                       int *__restrict__ segLL,
                                                       int *__restrict__ segLR) {
-    const int s = blockIdx.x * BS + threadIdx.x;
-    if (s >= S) return;
-    const int i = seg_node[s];
-    segLL[s] = slot_leaf[i] ? segL[s] : 0;
-    segLR[s] = slot_leaf[A + i] ? segR[s] : 0;
+    const int A = L->A, S = L->S;
+    GRID_STRIDE(s, S) {
+        const int i = seg_node[s];
+        segLL[s] = slot_leaf[i] ? segL[s] : 0;
+        segLR[s] = slot_leaf[A + i] ? segR[s] : 0;
+    }
 }
-
-struct LevelTotals { int L_all, LL_all, leaf_nodes, ulp, nleaf; };
 
 // CreateUpperChildren (TriangleUpperChildren.h:203-281), MarkLeafNodes
 // (:385-395), PropagateAabbToChildren (EmptySpaceSplitting.h:299-331).
 __global__ __launch_bounds__(BS) void children_kernel(
-    NodePtrs N, int active_index, int A, int child_start, int next_active, LevelTotals tot,
-    int propagate, const int *__restrict__ segbase, const int *__restrict__ segLoff,
-    const int *__restrict__ segRoff, const int *__restrict__ segLLoff,
-    const int *__restrict__ segLRoff, const int *__restrict__ slot_leaf,
-    const int *__restrict__ lna, int *__restrict__ leaf_ids, int *__restrict__ keys_next) {
-    const int i = blockIdx.x * BS + threadIdx.x;
-    if (i >= A) return;
-    const int par = active_index + i;
-    const int b = segbase[i], e = segbase[i + 1];
-    const int sizes[2] = {segLoff[e] - segLoff[b], segRoff[e] - segRoff[b]};
-    const int split_addr[2] = {segLoff[b], tot.L_all + segRoff[b]};
-    const int leaf_addr[2] = {segLLoff[b], tot.LL_all + segLRoff[b]};
-    const int ax = N.info[par] & 3;
-    const float s = N.split[par];
-    const float4 pmn = N.bmin[par], pmx = N.bmax[par];
-    int ids[2];
-    const int kmin = fkey(F_INF), kmax = fkey(-F_INF);
-    for (int side = 0; side < 2; ++side) {
-        const int slot = side * A + i;
-        const bool is_leaf = slot_leaf[slot] != 0;
-        const int id = child_start + (is_leaf ? lna[slot] : slot - lna[slot] + tot.leaf_nodes);
-        ids[side] = id;
-        N.info[id] = LEAF; N.split[id] = 0.0f;
-        N.prim_index[id] = is_leaf ? leaf_addr[side] + tot.ulp : split_addr[side] - leaf_addr[side];
-        N.bitmap[id] = sizes[side];
-        N.children[id] = make_int2(0, 0);
-        N.area[id] = 0.0f;
-        N.parent[id] = par;
-        if (propagate) {
-            if (side == 0) {
-                N.bmin[id] = pmn;
-                N.bmax[id] = make_float4(ax == AX_X ? s : pmx.x, ax == AX_Y ? s : pmx.y, ax == AX_Z ? s : pmx.z, 0.0f);
+    NodePtrs N, const Lvl *__restrict__ L, int propagate, const int *__restrict__ segbase,
+    const int *__restrict__ segLoff, const int *__restrict__ segRoff,
+    const int *__restrict__ segLLoff, const int *__restrict__ segLRoff,
+    const int *__restrict__ slot_leaf, const int *__restrict__ lna, int *__restrict__ leaf_ids,
+    int *__restrict__ keys_next) {
+    const int A = L->A, active_index = L->active_index, child_start = L->child_start;
+    const int next_active = L->next_active, L_all = L->L_all, LL_all = L->LL_all;
+    const int leaf_nodes = L->leaf_nodes, ulp = L->final_refs, nleaf = L->nleaf;
+    GRID_STRIDE(i, A) {
+        const int par = active_index + i;
+        const int b = segbase[i], e = segbase[i + 1];
+        const int sizes[2] = {segLoff[e] - segLoff[b], segRoff[e] - segRoff[b]};
+        const int split_addr[2] = {segLoff[b], L_all + segRoff[b]};
+        const int leaf_addr[2] = {segLLoff[b], LL_all + segLRoff[b]};
+        const int ax = N.info[par] & 3;
+        const float s = N.split[par];
+        const float4 pmn = N.bmin[par], pmx = N.bmax[par];
+        int ids[2];
+        const int kmin = fkey(F_INF), kmax = fkey(-F_INF);
+        for (int side = 0; side < 2; ++side) {
+            const int slot = side * A + i;
+            const bool is_leaf = slot_leaf[slot] != 0;
+            const int id = child_start + (is_leaf ? lna[slot] : slot - lna[slot] + leaf_nodes);
+            ids[side] = id;
+            N.info[id] = LEAF; N.split[id] = 0.0f;
+            N.prim_index[id] = is_leaf ? leaf_addr[side] + ulp : split_addr[side] - leaf_addr[side];
+            N.bitmap[id] = sizes[side];
+            N.children[id] = make_int2(0, 0);
+            N.area[id] = 0.0f;
+            N.parent[id] = par;
+            if (propagate) {
+                if (side == 0) {
+                    N.bmin[id] = pmn;
+                    N.bmax[id] = make_float4(ax == AX_X ? s : pmx.x, ax == AX_Y ? s : pmx.y, ax == AX_Z ? s : pmx.z, 0.0f);
+                } else {
+                    N.bmin[id] = make_float4(ax == AX_X ? s : pmn.x, ax == AX_Y ? s : pmn.y, ax == AX_Z ? s : pmn.z, 0.0f);
+                    N.bmax[id] = pmx;
+                }
             } else {
-                N.bmin[id] = make_float4(ax == AX_X ? s : pmn.x, ax == AX_Y ? s : pmn.y, ax == AX_Z ? s : pmn.z, 0.0f);
-                N.bmax[id] = pmx;
+                N.bmin[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                N.bmax[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             }
-        } else {
-            N.bmin[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            N.bmax[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (is_leaf) {
+                leaf_ids[nleaf + lna[slot]] = id;
+            } else {
+                int *k = keys_next + 6 * (id - next_active);
+                k[0] = kmin; k[1] = kmin; k[2] = kmin; k[3] = kmax; k[4] = kmax; k[5] = kmax;
+            }
         }
-        if (is_leaf) {
-            leaf_ids[tot.nleaf + lna[slot]] = id;
-        } else {
-            int *k = keys_next + 6 * (id - next_active);
-            k[0] = kmin; k[1] = kmin; k[2] = kmin; k[3] = kmax; k[4] = kmax; k[5] = kmax;
-        }
+        N.children[par] = make_int2(ids[0], ids[1]);
     }
-    N.children[par] = make_int2(ids[0], ids[1]);
 }
 
 // SplitTriangles (TriangleUpperChildren.h:283-383): stable partition of the
 // refs with clipping, plus the next level's box reduction.
 template <int SEGT>
-__global__ __launch_bounds__(WPB * 64) void scatter_kernel(
-    const NodePtrs N, int active_index, int A, int S, int next_active, LevelTotals tot,
+__device__ __forceinline__ void scatter_segment(
+    const NodePtrs &N, int s, int A, int active_index, int next_active, int L_all, int LL_all, int ulp,
     const int *__restrict__ seg_node, const int *__restrict__ seg_start,
     const int *__restrict__ seg_count, const int *__restrict__ segLoff,
     const int *__restrict__ segRoff, const int *__restrict__ segLLoff,
@@ -485,8 +499,6 @@ __global__ __launch_bounds__(WPB * 64) void scatter_kernel(
     const float4 *__restrict__ rmin, const float4 *__restrict__ rmax,
     float4 *__restrict__ nmin, float4 *__restrict__ nmax, float4 *__restrict__ fmin,
     float4 *__restrict__ fmax, int *__restrict__ keys_next) {
-    const int s = blockIdx.x * WPB + (threadIdx.x >> 6);
-    if (s >= S) return;
     const int lane = threadIdx.x & 63;
     const int i = seg_node[s];
     const int node = active_index + i;
@@ -497,10 +509,10 @@ __global__ __launch_bounds__(WPB * 64) void scatter_kernel(
     const int start = seg_start[s], count = seg_count[s];
     float4 *dLmin, *dLmax, *dRmin, *dRmax;
     long long bL, bR;
-    if (leafL) { dLmin = fmin; dLmax = fmax; bL = (long long)tot.ulp + segLLoff[s]; }
+    if (leafL) { dLmin = fmin; dLmax = fmax; bL = (long long)ulp + segLLoff[s]; }
     else { dLmin = nmin; dLmax = nmax; bL = segLoff[s] - segLLoff[s]; }
-    if (leafR) { dRmin = fmin; dRmax = fmax; bR = (long long)tot.ulp + tot.LL_all + segLRoff[s]; }
-    else { dRmin = nmin; dRmax = nmax; bR = (long long)(tot.L_all - tot.LL_all) + (segRoff[s] - segLRoff[s]); }
+    if (leafR) { dRmin = fmin; dRmax = fmax; bR = (long long)ulp + LL_all + segLRoff[s]; }
+    else { dRmin = nmin; dRmax = nmax; bR = (long long)(L_all - LL_all) + (segRoff[s] - segLRoff[s]); }
     const unsigned long long lt = lanemask_lt();
     int rl = 0, rr = 0;
     int l0 = fkey(F_INF), l1 = l0, l2 = l0, l3 = fkey(-F_INF), l4 = l3, l5 = l3;
@@ -548,75 +560,156 @@ __global__ __launch_bounds__(WPB * 64) void scatter_kernel(
     if (!leafL && rl > 0) {
         l0 = wave_min(l0); l1 = wave_min(l1); l2 = wave_min(l2);
         l3 = wave_max(l3); l4 = wave_max(l4); l5 = wave_max(l5);
-        if (lane == 0) {
-            int *k = keys_next + 6 * (ch.x - next_active);
-            atomicMin(k + 0, l0); atomicMin(k + 1, l1); atomicMin(k + 2, l2);
-            atomicMax(k + 3, l3); atomicMax(k + 4, l4); atomicMax(k + 5, l5);
-        }
+        if (lane == 0) key_atomics(keys_next + 6 * (ch.x - next_active), l0, l1, l2, l3, l4, l5);
     }
     if (!leafR && rr > 0) {
         r0 = wave_min(r0); r1 = wave_min(r1); r2 = wave_min(r2);
         r3 = wave_max(r3); r4 = wave_max(r4); r5 = wave_max(r5);
-        if (lane == 0) {
-            int *k = keys_next + 6 * (ch.y - next_active);
-            atomicMin(k + 0, r0); atomicMin(k + 1, r1); atomicMin(k + 2, r2);
-            atomicMax(k + 3, r3); atomicMax(k + 4, r4); atomicMax(k + 5, r5);
-        }
+        if (lane == 0) key_atomics(keys_next + 6 * (ch.y - next_active), r0, r1, r2, r3, r4, r5);
     }
 }
 
+template <int SEGT>
+__global__ __launch_bounds__(WPB * 64) void scatter_kernel(
+    const NodePtrs N, const Lvl *__restrict__ L,
+    const int *__restrict__ seg_node, const int *__restrict__ seg_start,
+    const int *__restrict__ seg_count, const int *__restrict__ segLoff,
+    const int *__restrict__ segRoff, const int *__restrict__ segLLoff,
+    const int *__restrict__ segLRoff, const int *__restrict__ slot_leaf,
+    const float4 *__restrict__ rmin, const float4 *__restrict__ rmax,
+    float4 *__restrict__ nmin, float4 *__restrict__ nmax, float4 *__restrict__ fmin,
+    float4 *__restrict__ fmax, int *__restrict__ keys_next) {
+    const int S = L->S, A = L->A, active_index = L->active_index, next_active = L->next_active;
+    const int L_all = L->L_all, LL_all = L->LL_all, ulp = L->final_refs;
+    for (int s = wave_index(); s < S; s += wave_count())
+        scatter_segment<SEGT>(N, s, A, active_index, next_active, L_all, LL_all, ulp, seg_node, seg_start,
+                              seg_count, segLoff, segRoff, segLLoff, segLRoff, slot_leaf, rmin, rmax, nmin,
+                              nmax, fmin, fmax, keys_next);
+}
+
+// ---- plan kernels: the host arithmetic of the level loop
+// (TriangleMapUpperCreator.cu:140-180), on the device
+__device__ void lvl_fail(Lvl *L, int code) {
+    if (!L->status) L->status = code;
+    L->A = 0; L->S = 0; L->E = 0; L->An = 0;
+}
+
+// after the node pass: empty-space nodes and segments of the level
+__global__ void plan_nodes_kernel(Lvl *L, const int *__restrict__ eaddr, const int *__restrict__ segbase) {
+    if (threadIdx.x | blockIdx.x) return;
+    const int A = L->A;
+    const int E = eaddr[A], S = segbase[A];
+    if ((long long)L->nodes_size + E + 2ll * A > L->cap_nodes) { lvl_fail(L, PM_ERR_CAPACITY); return; }
+    if (S > L->cap_segs) { lvl_fail(L, PM_ERR_CAPACITY); return; }
+    L->E = E;
+    L->S = S;
+    L->nodes_size += E;
+}
+
+// after the counts: children layout and the next level's sizes
+__global__ void plan_children_kernel(Lvl *L, const int *__restrict__ segLoff, const int *__restrict__ segRoff,
+                                     const int *__restrict__ segLLoff, const int *__restrict__ segLRoff,
+                                     const int *__restrict__ lna) {
+    if (threadIdx.x | blockIdx.x) return;
+    const int A = L->A, S = L->S;
+    const int L_all = segLoff[S], R_all = segRoff[S], LL_all = segLLoff[S], LR_all = segLRoff[S];
+    const int leaf_nodes = lna[2 * A];
+    const long long newTri = (long long)L_all + R_all;
+    if (newTri < L->R) { lvl_fail(L, PM_ERR_INTERNAL); return; }
+    if (newTri > INT_MAX) { lvl_fail(L, PM_ERR_ARG); return; }
+    const int leafTri = LL_all + LR_all;
+    const int nextR = (int)newTri - leafTri;
+    const int created = 2 * A - leaf_nodes;
+    if (created > 0 && nextR > L->max_refs) { lvl_fail(L, PM_ERR_DEPTH); return; }
+    if (nextR > L->cap_refs || (long long)L->final_refs + leafTri > L->cap_leaf_refs ||
+        (long long)L->nleaf + leaf_nodes > L->cap_leaf_ids || created > L->cap_active) {
+        lvl_fail(L, PM_ERR_CAPACITY);
+        return;
+    }
+    L->L_all = L_all; L->R_all = R_all; L->LL_all = LL_all; L->LR_all = LR_all;
+    L->leaf_nodes = leaf_nodes; L->leafTri = leafTri; L->nextR = nextR; L->created = created;
+    L->child_start = L->nodes_size;
+    L->next_active = L->nodes_size + leaf_nodes;
+}
+
+// after children and scatter: advance to the next level
+__global__ void plan_advance_kernel(Lvl *L) {
+    if (threadIdx.x | blockIdx.x) return;
+    if (L->status || L->A == 0) return;
+    L->final_refs += L->leafTri;
+    L->nleaf += L->leaf_nodes;
+    L->nodes_size += 2 * L->A;
+    L->active_index = L->nodes_size - L->created;
+    L->A = L->created;
+    L->R = L->nextR;
+    L->level += 1;
+    if (L->A > 0 && L->level >= L->max_levels) lvl_fail(L, PM_ERR_DEPTH);
+}
+
 // ExtractIndexFromAabb (TriangleUpper.h:43-52)
-__global__ void extract_index_kernel(const float4 *__restrict__ fmin, int *__restrict__ out, int n) {
-    const int i = blockIdx.x * BS + threadIdx.x;
-    if (i < n) out[i] = __float_as_int(fmin[i].w);
+__global__ void extract_index_kernel(const float4 *__restrict__ fmin, int *__restrict__ out, const Lvl *__restrict__ L) {
+    const int n = L->status ? 0 : L->final_refs;
+    GRID_STRIDE(i, n) out[i] = __float_as_int(fmin[i].w);
 }
 
 // PropagateParentAabb<true> (EmptySpaceSplitting.h:333-368)
-__global__ void propagate_parent_kernel(NodePtrs N, const int *__restrict__ leaf_ids, int n) {
-    const int i = blockIdx.x * BS + threadIdx.x;
-    if (i >= n) return;
-    const int leaf = leaf_ids[i];
-    const int par = N.parent[leaf];
-    const float4 pmn = N.bmin[par], pmx = N.bmax[par];
-    const int ax = N.info[par] & 3;
-    const float s = N.split[par];
-    if (N.children[par].x == leaf) {
-        N.bmin[leaf] = make_float4(pmn.x, pmn.y, pmn.z, 0.0f);
-        N.bmax[leaf] = make_float4(ax == AX_X ? s : pmx.x, ax == AX_Y ? s : pmx.y, ax == AX_Z ? s : pmx.z, 0.0f);
-    } else {
-        N.bmin[leaf] = make_float4(ax == AX_X ? s : pmn.x, ax == AX_Y ? s : pmn.y, ax == AX_Z ? s : pmn.z, 0.0f);
-        N.bmax[leaf] = make_float4(pmx.x, pmx.y, pmx.z, 0.0f);
+__global__ void propagate_parent_kernel(NodePtrs N, const int *__restrict__ leaf_ids, const Lvl *__restrict__ L) {
+    const int n = L->status ? 0 : L->nleaf;
+    GRID_STRIDE(i, n) {
+        const int leaf = leaf_ids[i];
+        const int par = N.parent[leaf];
+        const float4 pmn = N.bmin[par], pmx = N.bmax[par];
+        const int ax = N.info[par] & 3;
+        const float s = N.split[par];
+        if (N.children[par].x == leaf) {
+            N.bmin[leaf] = make_float4(pmn.x, pmn.y, pmn.z, 0.0f);
+            N.bmax[leaf] = make_float4(ax == AX_X ? s : pmx.x, ax == AX_Y ? s : pmx.y, ax == AX_Z ? s : pmx.z, 0.0f);
+        } else {
+            N.bmin[leaf] = make_float4(ax == AX_X ? s : pmn.x, ax == AX_Y ? s : pmn.y, ax == AX_Z ? s : pmn.z, 0.0f);
+            N.bmax[leaf] = make_float4(pmx.x, pmx.y, pmx.z, 0.0f);
+        }
     }
 }
 
 // ------------------------------------------------------------------ lower
 // PreprocessLeafNodes (TriangleMapBitmapCreator.cu:32-49)
-__global__ void bitmap_leaves_kernel(NodePtrs N, const int *__restrict__ leaf_ids, int n) {
-    const int i = blockIdx.x * BS + threadIdx.x;
-    if (i >= n) return;
-    const int leaf = leaf_ids[i];
-    N.info[leaf] = LEAF;
-    const long long cnt = N.bitmap[leaf];
-    N.bitmap[leaf] = cnt >= 64 ? -1ll : (long long)((1ull << cnt) - 1ull);
+__global__ void bitmap_leaves_kernel(NodePtrs N, const int *__restrict__ leaf_ids, const Lvl *__restrict__ L) {
+    const int n = L->status ? 0 : L->nleaf;
+    GRID_STRIDE(i, n) {
+        const int leaf = leaf_ids[i];
+        N.info[leaf] = LEAF;
+        const long long cnt = N.bitmap[leaf];
+        N.bitmap[leaf] = cnt >= 64 ? -1ll : (long long)((1ull << cnt) - 1ull);
+    }
 }
 
-// PreprocesLowerNodes (LowerTriangleMap.h:56-81)
-__global__ void lower_prep_kernel(NodePtrs N, const int *__restrict__ leaf_ids, int n,
+// PreprocesLowerNodes (LowerTriangleMap.h:56-81); starts the lower levels
+// with the upper leaves as the node list
+__global__ void lower_prep_kernel(NodePtrs N, const int *__restrict__ leaf_ids, const Lvl *__restrict__ L,
                                   const float *__restrict__ areas) {
-    const int i = blockIdx.x * BS + threadIdx.x;
-    if (i >= n) return;
-    const int leaf = leaf_ids[i];
-    const int pi = N.prim_index[leaf];
-    const int cnt = (int)N.bitmap[leaf];
-    float a = 0.0f;
-    unsigned long long bmp = 0;
-    for (int j = 0; j < cnt; ++j) {
-        const float aj = areas[pi + j];
-        bmp += aj > 0.0f ? (1ull << j) : 0ull;
-        a += aj;
+    const int n = L->status ? 0 : L->nleaf;
+    GRID_STRIDE(i, n) {
+        const int leaf = leaf_ids[i];
+        const int pi = N.prim_index[leaf];
+        const int cnt = (int)N.bitmap[leaf];
+        float a = 0.0f;
+        unsigned long long bmp = 0;
+        for (int j = 0; j < cnt; ++j) {
+            const float aj = areas[pi + j];
+            bmp += aj > 0.0f ? (1ull << j) : 0ull;
+            a += aj;
+        }
+        N.area[leaf] = a;
+        N.bitmap[leaf] = (long long)bmp;
     }
-    N.area[leaf] = a;
-    N.bitmap[leaf] = (long long)bmp;
+}
+
+__global__ void plan_lower_start_kernel(Lvl *L) {
+    if (threadIdx.x | blockIdx.x) return;
+    L->An = L->status ? 0 : L->nleaf;
+    L->use_list = 1;
+    L->list_base = 0;
+    L->lower_levels = 0;
 }
 
 // Exact early rejection for the SAH decision.  Every candidate plane puts
@@ -666,18 +759,19 @@ __device__ bool sah_cannot_split(bool has, float a, int n, float bonus, float le
 
 // CalcSAH + CalcAreaForSets (LowerTriangleMap.h:412-555), one wave per node.
 constexpr int MAXC = 6 * 64;
-__global__ __launch_bounds__(WPB * 64) void sah_kernel(
-    NodePtrs N, const int *__restrict__ list, int list_base, int A,
-    const float4 *__restrict__ pmin, const float4 *__restrict__ pmax,
-    const float *__restrict__ areas, float Ct, int leaf_max, float bonus,
+struct SahLds {
+    unsigned long long cL[WPB][MAXC];
+    unsigned long long cR[WPB][MAXC];
+    float ar[WPB][64];
+};
+
+__device__ __forceinline__ void sah_node(
+    const NodePtrs &N, int id, int par, SahLds &lds, const float4 *__restrict__ pmin,
+    const float4 *__restrict__ pmax, const float *__restrict__ areas, float Ct, int leaf_max, float bonus,
     int *__restrict__ flag, long long *__restrict__ sets, float *__restrict__ careas) {
-    __shared__ unsigned long long cL[WPB][MAXC];
-    __shared__ unsigned long long cR[WPB][MAXC];
-    __shared__ float ar[WPB][64];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int id = blockIdx.x * WPB + w;
-    if (id >= A) return;
-    const int par = list ? list[id] : list_base + id;
+    unsigned long long *cL = lds.cL[w], *cR = lds.cR[w];
+    float *ar = lds.ar[w];
     const int pi = N.prim_index[par];
     const unsigned long long bmp = (unsigned long long)N.bitmap[par];
     const int n = __popcll(bmp);
@@ -696,7 +790,7 @@ __global__ __launch_bounds__(WPB * 64) void sah_kernel(
     }
     f3 mn = {0.0f, 0.0f, 0.0f}, mx = {0.0f, 0.0f, 0.0f};
     if (has) { mn = mk3(pmin[pi + lane]); mx = mk3(pmax[pi + lane]); }
-    ar[w][lane] = a;
+    ar[lane] = a;
     // candidate planes (CreateSplittingPlanes, LowerTriangleMap.h:84-155):
     // prim ascending, axis X/Y/Z, low plane then high plane
     int c = 0;
@@ -711,8 +805,8 @@ __global__ __launch_bounds__(WPB * 64) void sah_kernel(
             const unsigned long long hL = __ballot(has && mnj <= hi);
             const unsigned long long hR = __ballot(has && hi < mxj);
             if (lane == 0) {
-                cL[w][c] = lL; cR[w][c] = lR;
-                cL[w][c + 1] = hL; cR[w][c + 1] = hR;
+                cL[c] = lL; cR[c] = lR;
+                cL[c + 1] = hL; cR[c + 1] = hR;
             }
             c += 2;
         }
@@ -723,10 +817,10 @@ __global__ __launch_bounds__(WPB * 64) void sah_kernel(
     float best = F_INF;
     int bestc = INT_MAX;
     for (int cc = lane; cc < nc; cc += 64) {
-        const unsigned long long L = cL[w][cc], R = cR[w][cc];
+        const unsigned long long L = cL[cc], R = cR[cc];
         float aL = 0.0f, aR = 0.0f;
-        for (unsigned long long m = L; m; m &= m - 1) aL += ar[w][__builtin_ctzll(m)];
-        for (unsigned long long m = R; m; m &= m - 1) aR += ar[w][__builtin_ctzll(m)];
+        for (unsigned long long m = L; m; m &= m - 1) aL += ar[__builtin_ctzll(m)];
+        for (unsigned long long m = R; m; m &= m - 1) aR += ar[__builtin_ctzll(m)];
         float cost = (float)__popcll(L) * aL + (float)__popcll(R) * aR;
         if (bonus != 1.0f && (L == 0ull || R == 0ull)) cost = cost * bonus;
         const bool progress = L != bmp && R != bmp;
@@ -742,10 +836,10 @@ __global__ __launch_bounds__(WPB * 64) void sah_kernel(
     if (lane != 0) return;
     bool split = false;
     if (bestc != INT_MAX) {
-        const unsigned long long L = cL[w][bestc], R = cR[w][bestc];
+        const unsigned long long L = cL[bestc], R = cR[bestc];
         float aL = 0.0f, aR = 0.0f;
-        for (unsigned long long m = L; m; m &= m - 1) aL += ar[w][__builtin_ctzll(m)];
-        for (unsigned long long m = R; m; m &= m - 1) aR += ar[w][__builtin_ctzll(m)];
+        for (unsigned long long m = L; m; m &= m - 1) aL += ar[__builtin_ctzll(m)];
+        for (unsigned long long m = R; m; m &= m - 1) aR += ar[__builtin_ctzll(m)];
         const bool progress = L != bmp && R != bmp;
         split = progress && (forced || best < leaf_cost);
         if (split) {
@@ -766,16 +860,51 @@ __global__ __launch_bounds__(WPB * 64) void sah_kernel(
     flag[id] = split;
 }
 
+__global__ __launch_bounds__(WPB * 64) void sah_kernel(
+    NodePtrs N, const int *__restrict__ leaf_ids, const Lvl *__restrict__ L,
+    const float4 *__restrict__ pmin, const float4 *__restrict__ pmax,
+    const float *__restrict__ areas, float Ct, int leaf_max, float bonus,
+    int *__restrict__ flag, long long *__restrict__ sets, float *__restrict__ careas) {
+    __shared__ SahLds lds;
+    const int An = L->An, list_base = L->list_base;
+    const int *list = L->use_list ? leaf_ids : nullptr;
+    for (int id = wave_index(); id < An; id += wave_count()) {
+        const int par = list ? list[id] : list_base + id;
+        sah_node(N, id, par, lds, pmin, pmax, areas, Ct, leaf_max, bonus, flag, sets, careas);
+    }
+}
+
+// after the SAH pass: place the split nodes' children
+__global__ void plan_lower_kernel(Lvl *L, const int *__restrict__ off) {
+    if (threadIdx.x | blockIdx.x) return;
+    const int An = L->An;
+    L->splits = 0;
+    if (An == 0) return;
+    L->lower_levels += 1;
+    const int splits = off[An];
+    if ((long long)L->nodes_size + 2ll * splits > L->cap_nodes || 2 * splits > L->cap_lower) {
+        lvl_fail(L, PM_ERR_CAPACITY);
+        return;
+    }
+    L->splits = splits;
+    L->lower_base = L->nodes_size;
+    L->nodes_size += 2 * splits;
+}
+
+// after the children: the next lower level is the new children
+__global__ void plan_lower_advance_kernel(Lvl *L) {
+    if (threadIdx.x | blockIdx.x) return;
+    L->An = 2 * L->splits;
+    L->use_list = 0;
+    L->list_base = L->lower_base;
+}
+
 // CreateLowerSAHChildren (LowerTriangleMap.h:557-600) + PropagateAabbToChildren
 // (LowerTriangleMap.h:602-636)
-__global__ void lower_children_kernel(NodePtrs N, const int *__restrict__ list, int list_base, int A,
-                                      int base, int splits, int propagate,
-                                      const int *__restrict__ flag, const int *__restrict__ off,
-                                      const long long *__restrict__ sets,
-                                      const float *__restrict__ careas) {
-    const int id = blockIdx.x * BS + threadIdx.x;
-    if (id >= A || !flag[id]) return;
-    const int par = list ? list[id] : list_base + id;
+__device__ __forceinline__ void lower_children_node(const NodePtrs &N, int id, int par, int base, int splits,
+                                                    int propagate, const int *__restrict__ off,
+                                                    const long long *__restrict__ sets,
+                                                    const float *__restrict__ careas) {
     const int l = base + off[id], r = l + splits;
     const int pi = N.prim_index[par];
     const int ax = N.info[par] & 3;
@@ -806,6 +935,19 @@ __global__ void lower_children_kernel(NodePtrs N, const int *__restrict__ list, 
     N.children[par] = make_int2(l, r);
 }
 
+__global__ void lower_children_kernel(NodePtrs N, const int *__restrict__ leaf_ids, const Lvl *__restrict__ L,
+                                      int propagate, const int *__restrict__ flag, const int *__restrict__ off,
+                                      const long long *__restrict__ sets, const float *__restrict__ careas) {
+    const int An = L->splits > 0 ? L->An : 0, list_base = L->list_base;
+    const int base = L->lower_base, splits = L->splits;
+    const int *list = L->use_list ? leaf_ids : nullptr;
+    GRID_STRIDE(id, An) {
+        if (!flag[id]) continue;
+        const int par = list ? list[id] : list_base + id;
+        lower_children_node(N, id, par, base, splits, propagate, off, sets, careas);
+    }
+}
+
 // chunk size -> kernel instance (one wave per chunk, chunk/64 refs per lane)
 template <class... Args>
 void launch_classify(int seg, int blocks, hipStream_t s, Args... a) {
@@ -833,29 +975,17 @@ NodePtrs node_ptrs(pm_ctx *ctx) {
                     a.bitmap.ptr, a.children.ptr, a.area.ptr, a.parent.ptr};
 }
 
-// read `count` ints from device locations in one round trip
-void readback(BuildScratch &sc, std::initializer_list<const int *> srcs, int *dst, hipStream_t s) {
-    Gather g{};
-    g.count = 0;
-    for (const int *p : srcs) g.src[g.count++] = p;
-    sc.totals.reserve(8, s);
-    sc.host.ensure(8);
-    gather_kernel<<<1, 64, 0, s>>>(g, sc.totals.ptr);
-    PM_LAUNCH_CHECK();
-    PM_HIP(hipMemcpyAsync(sc.host.ptr, sc.totals.ptr, g.count * sizeof(int), hipMemcpyDeviceToHost, s));
-    PM_HIP(hipStreamSynchronize(s));
-    for (int i = 0; i < g.count; ++i) dst[i] = sc.host.ptr[i];
-}
-
 }  // namespace
 
 // exclusive scans, out[n] = total: one pass when n fits a block, else
-// block scans, a scan of the block sums (recursively, in place) and an add
+// block scans, a scan of the block sums (recursively, in place) and an add.
+// With n_dev the length is read on the device and n is its upper bound.
 template <int K>
-static void scan_level(ScanIO<K> io, int n, BuildScratch &sc, int depth, hipStream_t s) {
+static void scan_level(ScanIO<K> io, int n, const int *n_dev, int n_mult, BuildScratch &sc, int depth,
+                       hipStream_t s) {
     const int nb = blocks_for((long long)n + 1, SCAN_BLK);
     if (nb == 1) {
-        scan_block_kernel<K><<<1, SCAN_T, 0, s>>>(io, n, nullptr, 0);
+        scan_block_kernel<K><<<1, SCAN_T, 0, s>>>(io, n, n_dev, n_mult, nullptr, 0);
         PM_LAUNCH_CHECK();
         return;
     }
@@ -863,44 +993,308 @@ static void scan_level(ScanIO<K> io, int n, BuildScratch &sc, int depth, hipStre
     DevBuf<int> &tmp = sc.scan_tmp[depth];
     const int stride = nb + 1;
     tmp.reserve((size_t)K * stride, s);
-    scan_block_kernel<K><<<nb, SCAN_T, 0, s>>>(io, n, tmp.ptr, stride);
+    scan_block_kernel<K><<<nb, SCAN_T, 0, s>>>(io, n, n_dev, n_mult, tmp.ptr, stride);
     PM_LAUNCH_CHECK();
     ScanIO<K> sums;
     for (int q = 0; q < K; ++q) {
         sums.in[q] = tmp.ptr + q * stride;
         sums.out[q] = tmp.ptr + q * stride;
     }
-    scan_level<K>(sums, nb, sc, depth + 1, s);
-    scan_add_kernel<K><<<nb, SCAN_T, 0, s>>>(io, n, tmp.ptr, stride);
+    scan_level<K>(sums, nb, nullptr, 1, sc, depth + 1, s);
+    scan_add_kernel<K><<<nb, SCAN_T, 0, s>>>(io, n, n_dev, n_mult, tmp.ptr, stride);
     PM_LAUNCH_CHECK();
 }
 
-void scan_exclusive(const int *in, int *out, int n, BuildScratch &sc, hipStream_t s) {
+void scan_exclusive(const int *in, int *out, int n, BuildScratch &sc, hipStream_t s, const int *n_dev,
+                    int n_mult) {
     ScanIO<1> io;
     io.in[0] = in; io.out[0] = out;
-    scan_level<1>(io, n, sc, 0, s);
+    scan_level<1>(io, n, n_dev, n_mult, sc, 0, s);
 }
 
 void scan_exclusive2(const int *in0, int *out0, const int *in1, int *out1, int n, BuildScratch &sc,
-                     hipStream_t s) {
+                     hipStream_t s, const int *n_dev, int n_mult) {
     ScanIO<2> io;
     io.in[0] = in0; io.out[0] = out0;
     io.in[1] = in1; io.out[1] = out1;
-    scan_level<2>(io, n, sc, 0, s);
+    scan_level<2>(io, n, n_dev, n_mult, sc, 0, s);
 }
 
-void build_tree(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2, int N,
-                const int *mesh_offsets, int M, const pm_build_params &prm, hipStream_t s) {
+namespace {
+
+__global__ void init_lvl_kernel(Lvl *L, Lvl v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *L = v;
+}
+
+// a captured build that needed more levels than it recorded
+__global__ void plan_finish_kernel(Lvl *L, int lower) {
+    if (threadIdx.x | blockIdx.x) return;
+    if (!L->status && (lower ? L->An > 0 : L->A > 0)) L->status = PM_ERR_CAPACITY;
+}
+
+void read_state(BuildScratch &sc, Lvl &h, hipStream_t s) {
+    sc.host.ensure(sizeof(Lvl) / sizeof(int) + 1);
+    PM_HIP(hipMemcpyAsync(sc.host.ptr, sc.lvl.ptr, sizeof(Lvl), hipMemcpyDeviceToHost, s));
+    PM_HIP(hipStreamSynchronize(s));
+    std::memcpy(&h, sc.host.ptr, sizeof(Lvl));
+}
+
+[[noreturn]] void throw_status(const Lvl &h) {
+    switch (h.status) {
+    case PM_ERR_DEPTH:
+        if (h.level >= h.max_levels)
+            throw Error{PM_ERR_DEPTH, "upper build did not finish within " + std::to_string(h.max_levels) +
+                                          " levels (>= lower_size coincident triangles?)"};
+        throw Error{PM_ERR_DEPTH, "upper build exceeded " + std::to_string(h.max_refs) +
+                                      " references (>= lower_size coincident triangles?)"};
+    case PM_ERR_INTERNAL: throw Error{PM_ERR_INTERNAL, "new triangle count below old"};
+    case PM_ERR_ARG: throw Error{PM_ERR_ARG, "reference count overflows int"};
+    case PM_ERR_CAPACITY: throw Error{PM_ERR_CAPACITY, "captured build outgrew its capacities"};
+    default: throw Error{h.status, "build failed"};
+    }
+}
+
+inline int grid_for(long long n, int per, int cap) {
+    return std::max(1, (int)std::min<long long>(blocks_for(std::max(n, 1ll), per), cap));
+}
+
+// The build's kernel sequence.  caps == nullptr: eager (the level state is
+// read back twice per upper level and once per lower level to size buffers
+// and grids; ends when no node is active).  caps != nullptr: captured (every
+// buffer is already at its capacity, grids are fixed, caps->upper_levels /
+// lower_levels levels are recorded, nothing is read back or synchronised).
+void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2, int N, int M,
+               const int *off_dev, const pm_build_params &prm, hipStream_t s, const BuildSizes *caps) {
     BuildScratch &sc = ctx->scratch;
     NodeArrays &nodes = ctx->nodes;
+    const bool graph = caps != nullptr;
     const int LS = prm.lower_size;
-    const int max_levels = prm.max_upper_levels > 0 ? prm.max_upper_levels : 128;
     const int seg = prm.chunk_size > 0 ? prm.chunk_size : 256;
-    ctx->has_tree = false;
-    ctx->p0 = P0; ctx->p1 = P1; ctx->p2 = P2; ctx->ntris = N;
-    ctx->level_refs.clear();
+    const long long BIG = LLONG_MAX / 4;
 
-    // one root per mesh (a forest); a single mesh is M = 1 over [0, N)
+    Lvl h{};
+    h.A = M; h.active_index = 0; h.nodes_size = M; h.R = N;
+    h.max_levels = prm.max_upper_levels > 0 ? prm.max_upper_levels : 128;
+    h.max_refs = (long long)(prm.max_ref_factor > 0 ? prm.max_ref_factor : 16) * N + 65536;
+    h.cap_nodes = graph ? caps->nodes : BIG;
+    h.cap_active = graph ? caps->active : BIG;
+    h.cap_segs = graph ? caps->segs : BIG;
+    h.cap_refs = graph ? caps->refs : BIG;
+    h.cap_leaf_refs = graph ? caps->leaf_refs : BIG;
+    h.cap_leaf_ids = graph ? caps->leaf_ids : BIG;
+    h.cap_lower = graph ? caps->lower : BIG;
+    BuildSizes seen;
+    seen.active = M; seen.refs = N;
+
+    sc.lvl.reserve(1, s);
+    Lvl *L = sc.lvl.ptr;
+    init_lvl_kernel<<<1, 64, 0, s>>>(L, h);
+    PM_LAUNCH_CHECK();
+    if (!graph) {
+        nodes.reserve(std::max(1024, 2 * M), s, 0);
+        sc.ref_min[0].reserve((size_t)N + 1, s); sc.ref_max[0].reserve((size_t)N + 1, s);
+        sc.keys[0].reserve(6 * (size_t)M, s);
+    }
+    init_roots_kernel<<<blocks_for(M, BS), BS, 0, s>>>(node_ptrs(ctx), off_dev, M);
+    PM_LAUNCH_CHECK();
+    init_keys_kernel<<<blocks_for(M, BS), BS, 0, s>>>(sc.keys[0].ptr, M);
+    PM_LAUNCH_CHECK();
+    if (N > 0) {
+        tri_box_kernel<<<blocks_for(N, BOX_TILE), BS, 0, s>>>(P0, P1, P2, N, sc.ref_min[0].ptr,
+                                                              sc.ref_max[0].ptr, off_dev, M, sc.keys[0].ptr);
+        PM_LAUNCH_CHECK();
+    }
+
+    // grids: eager from the sizes just read back, captured from the caps
+    auto g_nodes = [&](long long n) { return grid_for(graph ? caps->active : n, BS, graph ? 2048 : INT_MAX); };
+    auto g_waves = [&](long long n) { return grid_for(graph ? caps->segs : n, WPB, graph ? 2048 : INT_MAX); };
+    auto g_segs = [&](long long n) { return grid_for(graph ? caps->segs : n, BS, graph ? 2048 : INT_MAX); };
+    const int *dA = graph ? &L->A : nullptr;
+    const int *dS = graph ? &L->S : nullptr;
+    const int nA = graph ? (int)caps->active : 0;
+    const int nS = graph ? (int)caps->segs : 0;
+
+    int cur = 0;
+    for (int lvl = 0; graph ? lvl < caps->upper_levels : h.A > 0; ++lvl) {
+        if (!graph) {
+            ctx->level_refs.push_back(h.R);
+            sc.node_req.reserve(h.A + 1, s); sc.node_eaddr.reserve(h.A + 1, s);
+            sc.node_nseg.reserve(h.A + 1, s); sc.node_segbase.reserve(h.A + 1, s);
+            sc.slot_leaf.reserve(2 * (size_t)h.A + 1, s); sc.slot_lna.reserve(2 * (size_t)h.A + 1, s);
+        }
+        NodePtrs NP = node_ptrs(ctx);
+        // planes are kept in slot_lna's storage until the chain kernel has run
+        int *planes = sc.slot_lna.ptr;
+        node_prepare_kernel<<<g_nodes(h.A), BS, 0, s>>>(NP, L, seg, sc.keys[cur].ptr, prm.empty_space_splitting,
+                                                          prm.empty_space_threshold, planes, sc.node_req.ptr,
+                                                          sc.node_nseg.ptr);
+        PM_LAUNCH_CHECK();
+        scan_exclusive2(sc.node_req.ptr, sc.node_eaddr.ptr, sc.node_nseg.ptr, sc.node_segbase.ptr,
+                        graph ? nA : h.A, sc, s, dA);
+        plan_nodes_kernel<<<1, 64, 0, s>>>(L, sc.node_eaddr.ptr, sc.node_segbase.ptr);
+        PM_LAUNCH_CHECK();
+        if (!graph) {
+            read_state(sc, h, s);
+            if (h.status) throw_status(h);
+            nodes.reserve((size_t)h.nodes_size + 2 * (size_t)h.A, s, h.nodes_size - h.E);
+            const size_t S1 = (size_t)h.S + 1;
+            sc.seg_node.reserve(S1, s); sc.seg_start.reserve(S1, s); sc.seg_count.reserve(S1, s);
+            sc.segL.reserve(S1, s); sc.segR.reserve(S1, s); sc.segLL.reserve(S1, s); sc.segLR.reserve(S1, s);
+            sc.segLoff.reserve(S1, s); sc.segRoff.reserve(S1, s);
+            sc.segLLoff.reserve(S1, s); sc.segLRoff.reserve(S1, s);
+            seen.segs = std::max<long long>(seen.segs, h.S);
+        }
+        NP = node_ptrs(ctx);
+        empty_chain_kernel<<<g_nodes(h.A), BS, 0, s>>>(NP, L, planes, sc.node_eaddr.ptr);
+        PM_LAUNCH_CHECK();
+        seg_fill_kernel<<<g_nodes(h.A), BS, 0, s>>>(NP, L, seg, sc.node_segbase.ptr, sc.seg_node.ptr,
+                                                      sc.seg_start.ptr, sc.seg_count.ptr);
+        PM_LAUNCH_CHECK();
+        launch_classify(seg, g_waves(h.S), s, NP, (const Lvl *)L, (const int *)sc.seg_node.ptr,
+                        (const int *)sc.seg_start.ptr, (const int *)sc.seg_count.ptr,
+                        (const float4 *)sc.ref_min[cur].ptr, (const float4 *)sc.ref_max[cur].ptr, sc.segL.ptr,
+                        sc.segR.ptr);
+        PM_LAUNCH_CHECK();
+        scan_exclusive2(sc.segL.ptr, sc.segLoff.ptr, sc.segR.ptr, sc.segRoff.ptr, graph ? nS : h.S, sc, s, dS);
+        child_size_kernel<<<g_nodes(h.A), BS, 0, s>>>(L, LS, sc.node_segbase.ptr, sc.segLoff.ptr, sc.segRoff.ptr,
+                                                        sc.slot_leaf.ptr);
+        PM_LAUNCH_CHECK();
+        seg_leaf_kernel<<<g_segs(h.S), BS, 0, s>>>(L, sc.seg_node.ptr, sc.slot_leaf.ptr, sc.segL.ptr, sc.segR.ptr,
+                                                     sc.segLL.ptr, sc.segLR.ptr);
+        PM_LAUNCH_CHECK();
+        scan_exclusive2(sc.segLL.ptr, sc.segLLoff.ptr, sc.segLR.ptr, sc.segLRoff.ptr, graph ? nS : h.S, sc, s, dS);
+        scan_exclusive(sc.slot_leaf.ptr, sc.slot_lna.ptr, graph ? 2 * nA : 2 * h.A, sc, s, dA, 2);
+        plan_children_kernel<<<1, 64, 0, s>>>(L, sc.segLoff.ptr, sc.segRoff.ptr, sc.segLLoff.ptr, sc.segLRoff.ptr,
+                                              sc.slot_lna.ptr);
+        PM_LAUNCH_CHECK();
+        const int nxt = cur ^ 1;
+        if (!graph) {
+            read_state(sc, h, s);
+            if (h.status) throw_status(h);
+            sc.prim_min.reserve((size_t)h.final_refs + h.leafTri + 1, s, h.final_refs);
+            sc.prim_max.reserve((size_t)h.final_refs + h.leafTri + 1, s, h.final_refs);
+            ctx->leaf_ids.reserve((size_t)h.nleaf + h.leaf_nodes + 1, s, h.nleaf);
+            sc.ref_min[nxt].reserve((size_t)h.nextR + 1, s); sc.ref_max[nxt].reserve((size_t)h.nextR + 1, s);
+            sc.keys[nxt].reserve(6 * (size_t)h.created + 6, s);
+            seen.refs = std::max<long long>(seen.refs, h.nextR);
+            seen.active = std::max<long long>(seen.active, h.created);
+        }
+        NP = node_ptrs(ctx);
+        children_kernel<<<g_nodes(h.A), BS, 0, s>>>(NP, L, prm.empty_space_splitting, sc.node_segbase.ptr,
+                                                      sc.segLoff.ptr, sc.segRoff.ptr, sc.segLLoff.ptr,
+                                                      sc.segLRoff.ptr, sc.slot_leaf.ptr, sc.slot_lna.ptr,
+                                                      ctx->leaf_ids.ptr, sc.keys[nxt].ptr);
+        PM_LAUNCH_CHECK();
+        launch_scatter(seg, g_waves(h.S), s, NP, (const Lvl *)L, (const int *)sc.seg_node.ptr,
+                       (const int *)sc.seg_start.ptr, (const int *)sc.seg_count.ptr, (const int *)sc.segLoff.ptr,
+                       (const int *)sc.segRoff.ptr, (const int *)sc.segLLoff.ptr, (const int *)sc.segLRoff.ptr,
+                       (const int *)sc.slot_leaf.ptr, (const float4 *)sc.ref_min[cur].ptr,
+                       (const float4 *)sc.ref_max[cur].ptr, sc.ref_min[nxt].ptr, sc.ref_max[nxt].ptr,
+                       sc.prim_min.ptr, sc.prim_max.ptr, sc.keys[nxt].ptr);
+        PM_LAUNCH_CHECK();
+        plan_advance_kernel<<<1, 64, 0, s>>>(L);
+        PM_LAUNCH_CHECK();
+        if (!graph) {   // the same arithmetic as plan_advance_kernel
+            h.final_refs += h.leafTri;
+            h.nleaf += h.leaf_nodes;
+            h.nodes_size += 2 * h.A;
+            h.active_index = h.nodes_size - h.created;
+            h.A = h.created;
+            h.R = h.nextR;
+            h.level += 1;
+            if (h.A > 0 && h.level >= h.max_levels) { h.status = PM_ERR_DEPTH; throw_status(h); }
+        }
+        cur = nxt;
+    }
+    if (graph) {
+        plan_finish_kernel<<<1, 64, 0, s>>>(L, 0);
+        PM_LAUNCH_CHECK();
+    }
+
+    // leaf slots: prim indices, the tracers' leaf triangles and (SAH) areas
+    NodePtrs NP = node_ptrs(ctx);
+    const long long nref = graph ? caps->leaf_refs : h.final_refs;
+    const long long nlf = graph ? caps->leaf_ids : h.nleaf;
+    if (!graph) ctx->prim_indices.reserve((size_t)h.final_refs + 1, s);
+    extract_index_kernel<<<grid_for(nref, BS, graph ? 8192 : INT_MAX), BS, 0, s>>>(sc.prim_min.ptr,
+                                                                                    ctx->prim_indices.ptr, L);
+    PM_LAUNCH_CHECK();
+    if (!prm.empty_space_splitting && prm.propagate_aabbs) {
+        propagate_parent_kernel<<<grid_for(nlf, BS, graph ? 4096 : INT_MAX), BS, 0, s>>>(NP, ctx->leaf_ids.ptr, L);
+        PM_LAUNCH_CHECK();
+    }
+    ctx->num_prim_indices = (int)nref;
+    const bool sah = prm.lower_algorithm != PM_LOWER_BITMAP;
+    if (sah && !graph) sc.areas.reserve((size_t)h.final_refs + 1, s);
+    build_leaf_view(ctx, sah ? sc.areas.ptr : nullptr, false, s, graph ? &L->final_refs : nullptr);
+
+    // lower stage
+    if (!sah) {
+        bitmap_leaves_kernel<<<grid_for(nlf, BS, graph ? 4096 : INT_MAX), BS, 0, s>>>(NP, ctx->leaf_ids.ptr, L);
+        PM_LAUNCH_CHECK();
+    } else {
+        lower_prep_kernel<<<grid_for(nlf, BS, graph ? 4096 : INT_MAX), BS, 0, s>>>(NP, ctx->leaf_ids.ptr, L,
+                                                                                   sc.areas.ptr);
+        PM_LAUNCH_CHECK();
+        plan_lower_start_kernel<<<1, 64, 0, s>>>(L);
+        PM_LAUNCH_CHECK();
+        long long An = h.nleaf;
+        for (int ll = 0; graph ? ll < caps->lower_levels : An > 0; ++ll) {
+            const long long nAn = graph ? caps->lower : An;
+            if (!graph) {
+                sc.lower_flag.reserve(An + 1, s); sc.lower_off.reserve(An + 1, s);
+                sc.lower_sets.reserve(2 * (size_t)An, s); sc.lower_careas.reserve(2 * (size_t)An, s);
+                seen.lower = std::max(seen.lower, An);
+            }
+            NP = node_ptrs(ctx);
+            sah_kernel<<<grid_for(nAn, WPB, graph ? 2048 : INT_MAX), WPB * 64, 0, s>>>(
+                NP, ctx->leaf_ids.ptr, L, sc.prim_min.ptr, sc.prim_max.ptr, sc.areas.ptr, prm.traversal_cost,
+                prm.sah_leaf_max, prm.sah_empty_bonus, sc.lower_flag.ptr, sc.lower_sets.ptr, sc.lower_careas.ptr);
+            PM_LAUNCH_CHECK();
+            scan_exclusive(sc.lower_flag.ptr, sc.lower_off.ptr, (int)nAn, sc, s, graph ? &L->An : nullptr);
+            plan_lower_kernel<<<1, 64, 0, s>>>(L, sc.lower_off.ptr);
+            PM_LAUNCH_CHECK();
+            if (!graph) {
+                read_state(sc, h, s);
+                if (h.status) throw_status(h);
+                if (h.splits == 0) break;
+                nodes.reserve((size_t)h.nodes_size, s, h.nodes_size - 2 * h.splits);
+            }
+            NP = node_ptrs(ctx);
+            lower_children_kernel<<<grid_for(nAn, BS, graph ? 4096 : INT_MAX), BS, 0, s>>>(
+                NP, ctx->leaf_ids.ptr, L, prm.propagate_aabbs, sc.lower_flag.ptr, sc.lower_off.ptr,
+                sc.lower_sets.ptr, sc.lower_careas.ptr);
+            PM_LAUNCH_CHECK();
+            plan_lower_advance_kernel<<<1, 64, 0, s>>>(L);
+            PM_LAUNCH_CHECK();
+            if (!graph) An = 2ll * h.splits;
+        }
+        if (graph) {
+            plan_finish_kernel<<<1, 64, 0, s>>>(L, 1);
+            PM_LAUNCH_CHECK();
+        }
+    }
+    ctx->has_boxes = prm.propagate_aabbs != 0;
+    if (!graph) {
+        ctx->upper_levels = h.level;
+        ctx->lower_levels = h.lower_levels;
+        ctx->num_nodes = h.nodes_size;
+        ctx->num_prim_indices = h.final_refs;
+        ctx->num_leaf_ids = h.nleaf;
+        seen.nodes = h.nodes_size;
+        seen.leaf_refs = h.final_refs;
+        seen.leaf_ids = h.nleaf;
+        seen.upper_levels = h.level;
+        seen.lower_levels = h.lower_levels;
+        ctx->last_sizes = seen;
+    } else {
+        ctx->num_nodes = (int)caps->nodes;
+    }
+    build_node_view(ctx, s, graph ? &L->nodes_size : nullptr);
+}
+
+void upload_offsets(pm_ctx *ctx, const int *mesh_offsets, int N, int M, hipStream_t s) {
+    BuildScratch &sc = ctx->scratch;
     std::vector<int> off_h(M + 1);
     if (mesh_offsets) {
         std::copy(mesh_offsets, mesh_offsets + M + 1, off_h.begin());
@@ -916,188 +1310,198 @@ void build_tree(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P
     PM_HIP(hipStreamSynchronize(s));
     ctx->mesh_offsets = off_h;
     ctx->num_roots = M;
+}
 
-    int nodes_size = M, final_refs = 0, nleaf = 0;
-    nodes.reserve(std::max(1024, 2 * M), s, 0);
-    sc.ref_min[0].reserve((size_t)N + 1, s); sc.ref_max[0].reserve((size_t)N + 1, s);
-    sc.keys[0].reserve(6 * (size_t)M, s);
-    init_roots_kernel<<<blocks_for(M, BS), BS, 0, s>>>(node_ptrs(ctx), sc.mesh_off.ptr, M);
-    PM_LAUNCH_CHECK();
-    init_keys_kernel<<<blocks_for(M, BS), BS, 0, s>>>(sc.keys[0].ptr, M);
-    PM_LAUNCH_CHECK();
-    if (N > 0) {
-        tri_box_kernel<<<blocks_for(N, BOX_TILE), BS, 0, s>>>(P0, P1, P2, N, sc.ref_min[0].ptr,
-                                                              sc.ref_max[0].ptr, sc.mesh_off.ptr, M,
-                                                              sc.keys[0].ptr);
-        PM_LAUNCH_CHECK();
-    }
-    int cur = 0;          // ping-pong index of the level's ref arrays and keys
-    int active_index = 0, A = M, R = N, level = 0;
-    int hv[8];
-    while (A > 0) {
-        if (level >= max_levels)
-            throw Error{PM_ERR_DEPTH, "upper build did not finish within " + std::to_string(max_levels) +
-                                          " levels (>= lower_size coincident triangles?)"};
-        const long long max_refs = (long long)(prm.max_ref_factor > 0 ? prm.max_ref_factor : 16) * N + 65536;
-        if (R > max_refs)
-            throw Error{PM_ERR_DEPTH, "upper build exceeded " + std::to_string(max_refs) +
-                                          " references (>= lower_size coincident triangles?)"};
-        ctx->level_refs.push_back(R);
-        NodePtrs NP = node_ptrs(ctx);
-        sc.node_req.reserve(A + 1, s); sc.node_eaddr.reserve(A + 1, s);
-        sc.node_nseg.reserve(A + 1, s); sc.node_segbase.reserve(A + 1, s);
-        sc.slot_leaf.reserve(2 * (size_t)A + 1, s); sc.slot_lna.reserve(2 * (size_t)A + 1, s);
-        // planes are kept in slot_lna's storage until the chain kernel has run
-        int *planes = sc.slot_lna.ptr;
-        node_prepare_kernel<<<blocks_for(A, BS), BS, 0, s>>>(NP, active_index, A, seg, sc.keys[cur].ptr,
-                                                              prm.empty_space_splitting,
-                                                              prm.empty_space_threshold, planes,
-                                                              sc.node_req.ptr, sc.node_nseg.ptr);
-        PM_LAUNCH_CHECK();
-        scan_exclusive2(sc.node_req.ptr, sc.node_eaddr.ptr, sc.node_nseg.ptr, sc.node_segbase.ptr, A, sc, s);
-        readback(sc, {sc.node_eaddr.ptr + A, sc.node_segbase.ptr + A}, hv, s);
-        const int E = hv[0], S = hv[1];
-        if (E > 0) {
-            nodes.reserve((size_t)nodes_size + E, s, nodes_size);
-            NP = node_ptrs(ctx);
-            empty_chain_kernel<<<blocks_for(A, BS), BS, 0, s>>>(NP, active_index, A, planes, sc.node_eaddr.ptr);
-            PM_LAUNCH_CHECK();
-            nodes_size += E;
-        }
-        // segments
-        sc.seg_node.reserve(S + 1, s); sc.seg_start.reserve(S + 1, s); sc.seg_count.reserve(S + 1, s);
-        sc.segL.reserve(S + 1, s); sc.segR.reserve(S + 1, s);
-        sc.segLL.reserve(S + 1, s); sc.segLR.reserve(S + 1, s);
-        sc.segLoff.reserve(S + 1, s); sc.segRoff.reserve(S + 1, s);
-        sc.segLLoff.reserve(S + 1, s); sc.segLRoff.reserve(S + 1, s);
-        seg_fill_kernel<<<blocks_for(A, BS), BS, 0, s>>>(NP, active_index, A, seg, sc.node_segbase.ptr,
-                                                          sc.seg_node.ptr, sc.seg_start.ptr, sc.seg_count.ptr);
-        PM_LAUNCH_CHECK();
-        if (S > 0) {
-            launch_classify(seg, blocks_for(S, WPB), s, NP, active_index, S, sc.seg_node.ptr,
-                            sc.seg_start.ptr, sc.seg_count.ptr, sc.ref_min[cur].ptr, sc.ref_max[cur].ptr,
-                            sc.segL.ptr, sc.segR.ptr);
-            PM_LAUNCH_CHECK();
-        }
-        scan_exclusive2(sc.segL.ptr, sc.segLoff.ptr, sc.segR.ptr, sc.segRoff.ptr, S, sc, s);
-        child_size_kernel<<<blocks_for(A, BS), BS, 0, s>>>(A, LS, sc.node_segbase.ptr, sc.segLoff.ptr,
-                                                            sc.segRoff.ptr, sc.slot_leaf.ptr);
-        PM_LAUNCH_CHECK();
-        if (S > 0) {
-            seg_leaf_kernel<<<blocks_for(S, BS), BS, 0, s>>>(A, S, sc.seg_node.ptr, sc.slot_leaf.ptr,
-                                                              sc.segL.ptr, sc.segR.ptr, sc.segLL.ptr, sc.segLR.ptr);
-            PM_LAUNCH_CHECK();
-        }
-        scan_exclusive2(sc.segLL.ptr, sc.segLLoff.ptr, sc.segLR.ptr, sc.segLRoff.ptr, S, sc, s);
-        scan_exclusive(sc.slot_leaf.ptr, sc.slot_lna.ptr, 2 * A, sc, s);
-        readback(sc, {sc.segLoff.ptr + S, sc.segRoff.ptr + S, sc.segLLoff.ptr + S, sc.segLRoff.ptr + S,
-                      sc.slot_lna.ptr + 2 * A}, hv, s);
-        const int L_all = hv[0], R_all = hv[1], LL_all = hv[2], LR_all = hv[3], leaf_nodes = hv[4];
-        const long long newTri = (long long)L_all + R_all;
-        if (newTri < R) throw Error{PM_ERR_INTERNAL, "new triangle count below old"};
-        if (newTri > INT_MAX) throw Error{PM_ERR_ARG, "reference count overflows int"};
-        const int leafTri = LL_all + LR_all;
-        const int nextR = (int)newTri - leafTri;
-        const int child_start = nodes_size;
-        const int created = 2 * A - leaf_nodes;
-        const int next_active = child_start + leaf_nodes;
-        nodes.reserve((size_t)nodes_size + 2 * (size_t)A, s, nodes_size);
-        NP = node_ptrs(ctx);
-        nodes_size += 2 * A;
-        sc.prim_min.reserve((size_t)final_refs + leafTri + 1, s, final_refs);
-        sc.prim_max.reserve((size_t)final_refs + leafTri + 1, s, final_refs);
-        ctx->leaf_ids.reserve((size_t)nleaf + leaf_nodes + 1, s, nleaf);
-        const int nxt = cur ^ 1;
-        sc.ref_min[nxt].reserve((size_t)nextR + 1, s); sc.ref_max[nxt].reserve((size_t)nextR + 1, s);
-        sc.keys[nxt].reserve(6 * (size_t)created + 6, s);
-        LevelTotals tot{L_all, LL_all, leaf_nodes, final_refs, nleaf};
-        children_kernel<<<blocks_for(A, BS), BS, 0, s>>>(
-            NP, active_index, A, child_start, next_active, tot, prm.empty_space_splitting,
-            sc.node_segbase.ptr, sc.segLoff.ptr, sc.segRoff.ptr, sc.segLLoff.ptr, sc.segLRoff.ptr,
-            sc.slot_leaf.ptr, sc.slot_lna.ptr, ctx->leaf_ids.ptr, sc.keys[nxt].ptr);
-        PM_LAUNCH_CHECK();
-        if (S > 0) {
-            launch_scatter(seg, blocks_for(S, WPB), s,
-                NP, active_index, A, S, next_active, tot, sc.seg_node.ptr, sc.seg_start.ptr,
-                sc.seg_count.ptr, sc.segLoff.ptr, sc.segRoff.ptr, sc.segLLoff.ptr, sc.segLRoff.ptr,
-                sc.slot_leaf.ptr, sc.ref_min[cur].ptr, sc.ref_max[cur].ptr, sc.ref_min[nxt].ptr,
-                sc.ref_max[nxt].ptr, sc.prim_min.ptr, sc.prim_max.ptr, sc.keys[nxt].ptr);
-            PM_LAUNCH_CHECK();
-        }
-        final_refs += leafTri;
-        nleaf += leaf_nodes;
-        cur = nxt;
-        active_index = nodes_size - created;
-        A = created;
-        R = nextR;
-        ++level;
-    }
-    ctx->upper_levels = level;
-    NodePtrs NP = node_ptrs(ctx);
-    ctx->prim_indices.reserve((size_t)final_refs + 1, s);
-    if (final_refs > 0) {
-        extract_index_kernel<<<blocks_for(final_refs, BS), BS, 0, s>>>(sc.prim_min.ptr, ctx->prim_indices.ptr, final_refs);
-        PM_LAUNCH_CHECK();
-    }
-    if (!prm.empty_space_splitting && prm.propagate_aabbs && nleaf > 0) {
-        propagate_parent_kernel<<<blocks_for(nleaf, BS), BS, 0, s>>>(NP, ctx->leaf_ids.ptr, nleaf);
-        PM_LAUNCH_CHECK();
-    }
-    // the leaf slots are final here (the lower stage only splits nodes):
-    // write the tracers' leaf triangles, and the SAH's areas in the same pass
-    ctx->num_prim_indices = final_refs;
-    const bool sah = prm.lower_algorithm != PM_LOWER_BITMAP;
-    if (sah) sc.areas.reserve((size_t)final_refs + 1, s);
-    build_leaf_view(ctx, sah ? sc.areas.ptr : nullptr, false, s);
+}  // namespace
 
-    // lower stage
-    int lower_levels = 0;
-    if (prm.lower_algorithm == PM_LOWER_BITMAP) {
-        if (nleaf > 0) {
-            bitmap_leaves_kernel<<<blocks_for(nleaf, BS), BS, 0, s>>>(NP, ctx->leaf_ids.ptr, nleaf);
-            PM_LAUNCH_CHECK();
-        }
-    } else {
-        if (nleaf > 0) {
-            lower_prep_kernel<<<blocks_for(nleaf, BS), BS, 0, s>>>(NP, ctx->leaf_ids.ptr, nleaf, sc.areas.ptr);
-            PM_LAUNCH_CHECK();
-        }
-        const int *list = ctx->leaf_ids.ptr;
-        int list_base = 0, An = nleaf;
-        while (An > 0) {
-            sc.lower_flag.reserve(An + 1, s); sc.lower_off.reserve(An + 1, s);
-            sc.lower_sets.reserve(2 * (size_t)An, s); sc.lower_careas.reserve(2 * (size_t)An, s);
-            sah_kernel<<<blocks_for(An, WPB), WPB * 64, 0, s>>>(
-                NP, list, list_base, An, sc.prim_min.ptr, sc.prim_max.ptr, sc.areas.ptr,
-                prm.traversal_cost, prm.sah_leaf_max, prm.sah_empty_bonus, sc.lower_flag.ptr,
-                sc.lower_sets.ptr, sc.lower_careas.ptr);
-            PM_LAUNCH_CHECK();
-            scan_exclusive(sc.lower_flag.ptr, sc.lower_off.ptr, An, sc, s);
-            readback(sc, {sc.lower_off.ptr + An}, hv, s);
-            const int splits = hv[0];
-            ++lower_levels;
-            if (splits == 0) break;
-            const int base = nodes_size;
-            nodes.reserve((size_t)nodes_size + 2 * (size_t)splits, s, nodes_size);
-            NP = node_ptrs(ctx);
-            nodes_size += 2 * splits;
-            lower_children_kernel<<<blocks_for(An, BS), BS, 0, s>>>(NP, list, list_base, An, base, splits,
-                                                                    prm.propagate_aabbs, sc.lower_flag.ptr,
-                                                                    sc.lower_off.ptr, sc.lower_sets.ptr,
-                                                                    sc.lower_careas.ptr);
-            PM_LAUNCH_CHECK();
-            list = nullptr;
-            list_base = base;
-            An = 2 * splits;
-        }
-    }
-    ctx->lower_levels = lower_levels;
-    ctx->num_nodes = nodes_size;
-    ctx->num_prim_indices = final_refs;
-    ctx->num_leaf_ids = nleaf;
-    ctx->has_boxes = prm.propagate_aabbs != 0;
-    build_node_view(ctx, s);
+void build_tree(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2, int N,
+                const int *mesh_offsets, int M, const pm_build_params &prm, hipStream_t s) {
+    ctx->has_tree = false;
+    if (ctx->graph_pending) PM_HIP(hipStreamSynchronize(ctx->graph_last_stream));
+    ctx->graph_pending = false;
+    ctx->p0 = P0; ctx->p1 = P1; ctx->p2 = P2; ctx->ntris = N;
+    ctx->level_refs.clear();
+    upload_offsets(ctx, mesh_offsets, N, M, s);
+    run_build(ctx, P0, P1, P2, N, M, ctx->scratch.mesh_off.ptr, prm, s, nullptr);
     ctx->has_tree = true;
+}
+
+namespace {
+
+// the device buffers a captured batch bakes into its graph
+std::vector<void **> graph_buffers(pm_ctx *ctx) {
+    BuildScratch &sc = ctx->scratch;
+    NodeArrays &a = ctx->nodes;
+    std::vector<void **> v = {
+        (void **)&a.info.ptr, (void **)&a.split.ptr, (void **)&a.bmin.ptr, (void **)&a.bmax.ptr,
+        (void **)&a.prim_index.ptr, (void **)&a.bitmap.ptr, (void **)&a.children.ptr, (void **)&a.area.ptr,
+        (void **)&a.parent.ptr, (void **)&ctx->prim_indices.ptr, (void **)&ctx->leaf_ids.ptr,
+        (void **)&ctx->packed.ptr, (void **)&ctx->leaf_tris.ptr, (void **)&sc.prim_min.ptr,
+        (void **)&sc.prim_max.ptr, (void **)&sc.node_req.ptr, (void **)&sc.node_eaddr.ptr,
+        (void **)&sc.node_nseg.ptr, (void **)&sc.node_segbase.ptr, (void **)&sc.seg_node.ptr,
+        (void **)&sc.seg_start.ptr, (void **)&sc.seg_count.ptr, (void **)&sc.segL.ptr, (void **)&sc.segR.ptr,
+        (void **)&sc.segLL.ptr, (void **)&sc.segLR.ptr, (void **)&sc.segLoff.ptr, (void **)&sc.segRoff.ptr,
+        (void **)&sc.segLLoff.ptr, (void **)&sc.segLRoff.ptr, (void **)&sc.slot_leaf.ptr,
+        (void **)&sc.slot_lna.ptr, (void **)&sc.areas.ptr, (void **)&sc.lower_flag.ptr,
+        (void **)&sc.lower_off.ptr, (void **)&sc.lower_sets.ptr, (void **)&sc.lower_careas.ptr,
+        (void **)&ctx->graph_off.ptr, (void **)&sc.lvl.ptr};
+    for (int i = 0; i < 2; ++i) {
+        v.push_back((void **)&sc.ref_min[i].ptr);
+        v.push_back((void **)&sc.ref_max[i].ptr);
+        v.push_back((void **)&sc.keys[i].ptr);
+    }
+    for (int i = 0; i < 3; ++i) v.push_back((void **)&sc.scan_tmp[i].ptr);
+    return v;
+}
+
+// block-sum buffers the scans of length up to n need, per recursion depth
+void reserve_scan_tmp(BuildScratch &sc, long long n, hipStream_t s) {
+    for (int depth = 0; depth < 3; ++depth) {
+        const long long nb = blocks_for(n + 1, SCAN_BLK);
+        if (nb <= 1) return;
+        sc.scan_tmp[depth].reserve(2 * (size_t)(nb + 1), s);
+        n = nb;
+    }
+}
+
+}  // namespace
+
+void destroy_graph(pm_ctx *ctx) {
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    ctx->graph_exec = nullptr;
+    ctx->graph_ptrs.clear();
+}
+
+void capture_batch(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2, int N,
+                   const int *mesh_offsets, int M, const float4 *org, const float4 *dir, const int *ray_mesh,
+                   int nrays, pm_hit *out, int algo, const pm_build_params &bp, const pm_trace_params &tp,
+                   BuildSizes &caps, hipStream_t s) {
+    destroy_graph(ctx);
+    // 1. an eager build of the same batch gives the sizes a frame reaches
+    build_tree(ctx, P0, P1, P2, N, mesh_offsets, M, bp, s);
+    const BuildSizes seen = ctx->last_sizes;
+    auto grow = [](long long v, long long given) { return given > 0 ? given : v + v / 2 + 1024; };
+    caps.nodes = std::max<long long>(grow(seen.nodes, caps.nodes), M);
+    caps.active = std::max<long long>(grow(seen.active, caps.active), M);
+    caps.segs = grow(seen.segs, caps.segs);
+    caps.refs = std::max<long long>(grow(seen.refs, caps.refs), N);
+    caps.leaf_refs = grow(seen.leaf_refs, caps.leaf_refs);
+    caps.leaf_ids = grow(seen.leaf_ids, caps.leaf_ids);
+    caps.lower = std::max(grow(seen.lower, caps.lower), caps.leaf_ids);
+    if (caps.upper_levels <= 0) caps.upper_levels = seen.upper_levels + 4;
+    if (caps.lower_levels <= 0) caps.lower_levels = seen.lower_levels + 2;
+    for (long long c : {caps.nodes, 2 * caps.active, caps.segs, caps.refs, 3 * caps.leaf_refs, 2 * caps.lower})
+        if (c >= INT_MAX) throw Error{PM_ERR_ARG, "capacities exceed the 32-bit index range"};
+
+    // 2. every buffer at its capacity: nothing may allocate while recording
+    BuildScratch &sc = ctx->scratch;
+    ctx->nodes.reserve((size_t)caps.nodes + 1, s, 0);
+    for (int i = 0; i < 2; ++i) {
+        sc.ref_min[i].reserve((size_t)caps.refs + 1, s);
+        sc.ref_max[i].reserve((size_t)caps.refs + 1, s);
+        sc.keys[i].reserve(6 * (size_t)caps.active + 6, s);
+    }
+    for (DevBuf<int> *b : {&sc.node_req, &sc.node_eaddr, &sc.node_nseg, &sc.node_segbase})
+        b->reserve((size_t)caps.active + 1, s);
+    sc.slot_leaf.reserve(2 * (size_t)caps.active + 1, s);
+    sc.slot_lna.reserve(2 * (size_t)caps.active + 1, s);
+    for (DevBuf<int> *b : {&sc.seg_node, &sc.seg_start, &sc.seg_count, &sc.segL, &sc.segR, &sc.segLL, &sc.segLR,
+                           &sc.segLoff, &sc.segRoff, &sc.segLLoff, &sc.segLRoff})
+        b->reserve((size_t)caps.segs + 1, s);
+    sc.prim_min.reserve((size_t)caps.leaf_refs + 1, s);
+    sc.prim_max.reserve((size_t)caps.leaf_refs + 1, s);
+    ctx->prim_indices.reserve((size_t)caps.leaf_refs + 1, s);
+    sc.areas.reserve((size_t)caps.leaf_refs + 1, s);
+    ctx->leaf_tris.reserve(3 * (size_t)caps.leaf_refs + 3, s);
+    ctx->leaf_ids.reserve((size_t)caps.leaf_ids + 1, s);
+    sc.lower_flag.reserve((size_t)caps.lower + 1, s);
+    sc.lower_off.reserve((size_t)caps.lower + 1, s);
+    sc.lower_sets.reserve(2 * (size_t)caps.lower, s);
+    sc.lower_careas.reserve(2 * (size_t)caps.lower, s);
+    ctx->packed.reserve((size_t)caps.nodes + 1, s);
+    reserve_scan_tmp(sc, std::max({2 * caps.active, caps.segs, caps.lower}), s);
+    sc.lvl.reserve(1, s);
+    // the batch's own copy of its mesh offsets (eager builds rewrite the scratch copy)
+    ctx->graph_off.reserve((size_t)M + 1, s);
+    PM_HIP(hipMemcpyAsync(ctx->graph_off.ptr, ctx->mesh_offsets.data(), (M + 1) * sizeof(int),
+                          hipMemcpyHostToDevice, s));
+    PM_HIP(hipStreamSynchronize(s));
+    ctx->graph_p0 = P0; ctx->graph_p1 = P1; ctx->graph_p2 = P2;
+    ctx->graph_n = N; ctx->graph_M = M;
+    ctx->graph_boxes = bp.propagate_aabbs != 0;
+    ctx->graph_offsets = ctx->mesh_offsets;
+
+    // 3. record build + trace on the context's own stream (recording runs nothing)
+    if (!ctx->capture_stream) PM_HIP(hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
+    hipStream_t cs = ctx->capture_stream;
+    ctx->has_tree = false;
+    PM_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    hipGraph_t graph = nullptr;
+    try {
+        run_build(ctx, P0, P1, P2, N, M, ctx->graph_off.ptr, bp, cs, &caps);
+        ctx->trace_fail = &sc.lvl.ptr->status;
+        trace_rays(ctx, algo, org, dir, nrays, ray_mesh, out, tp, nullptr, cs);
+        ctx->trace_fail = nullptr;
+    } catch (...) {
+        ctx->trace_fail = nullptr;
+        (void)hipStreamEndCapture(cs, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    PM_HIP(hipStreamEndCapture(cs, &graph));
+    hipError_t e = hipGraphInstantiate(&ctx->graph_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        ctx->graph_exec = nullptr;
+        throw Error{PM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)};
+    }
+    for (void **p : graph_buffers(ctx)) ctx->graph_ptrs.push_back({p, *p});
+    ctx->graph_caps = caps;
+
+    // 4. one replay, checked
+    launch_batch(ctx, s);
+    const int st = sync_graph_state(ctx, s);
+    if (st) {
+        Lvl h;
+        read_state(sc, h, s);
+        destroy_graph(ctx);
+        throw_status(h);
+    }
+}
+
+void launch_batch(pm_ctx *ctx, hipStream_t s) {
+    if (!ctx->graph_exec) throw Error{PM_ERR_NO_TREE, "no captured batch: call pm_batch_graph_capture first"};
+    for (const auto &pv : ctx->graph_ptrs)
+        if (*pv.first != pv.second) {
+            destroy_graph(ctx);
+            throw Error{PM_ERR_CAPACITY, "a later call regrew the buffers of the captured batch: capture it again"};
+        }
+    if (ctx->graph_pending && ctx->graph_last_stream != s) PM_HIP(hipStreamSynchronize(ctx->graph_last_stream));
+    PM_HIP(hipGraphLaunch(ctx->graph_exec, s));
+    ctx->graph_pending = true;
+    ctx->graph_last_stream = s;
+    // the context's tree is now the batch's
+    ctx->p0 = ctx->graph_p0; ctx->p1 = ctx->graph_p1; ctx->p2 = ctx->graph_p2;
+    ctx->ntris = ctx->graph_n;
+    ctx->num_roots = ctx->graph_M;
+    ctx->mesh_offsets = ctx->graph_offsets;
+    ctx->has_boxes = ctx->graph_boxes;
+    ctx->has_tree = true;
+}
+
+int sync_graph_state(pm_ctx *ctx, hipStream_t s) {
+    if (!ctx->graph_pending) return 0;
+    PM_HIP(hipStreamSynchronize(ctx->graph_last_stream));
+    Lvl h;
+    read_state(ctx->scratch, h, s);
+    ctx->graph_pending = false;
+    ctx->num_nodes = h.nodes_size;
+    ctx->num_prim_indices = h.final_refs;
+    ctx->num_leaf_ids = h.nleaf;
+    ctx->upper_levels = h.level;
+    ctx->lower_levels = h.lower_levels;
+    ctx->has_tree = h.status == 0;
+    return h.status;
 }
 
 }  // namespace pmkd

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/pmkd.h"
@@ -100,6 +101,30 @@ struct TreeDev {
     const float4 *leaf_tris;   // 3 float4 per leaf slot: v0 (w = id bits), e1, e2
     int num_nodes;
     int num_roots;             // roots are nodes 0..num_roots-1 (one per mesh)
+    const int *fail;           // device flag: nonzero = the tree is not valid, every ray misses
+};
+
+// Level state of a build, in device memory: the plan kernels write it
+// between the phases of a level, the level kernels read their sizes from
+// it.  The eager build reads it back to size buffers and grids; a captured
+// build never does (caps bound every write, a failed check sets status).
+struct Lvl {
+    int A, active_index, nodes_size, R, S, E;          // upper level
+    int L_all, R_all, LL_all, LR_all, leaf_nodes;      // this level's counts
+    int leafTri, nextR, created, child_start, next_active;
+    int final_refs, nleaf, level;
+    int An, list_base, use_list, splits, lower_base, lower_levels;   // lower levels
+    int status;                                        // 0, or the PM_ERR_* that stopped the build
+    int max_levels;
+    long long max_refs;
+    long long cap_nodes, cap_active, cap_segs, cap_refs, cap_leaf_refs, cap_leaf_ids, cap_lower;
+};
+
+// Sizes an eager build reached (the capacities a captured build derives
+// from) and the fixed capacities / level counts of a captured build.
+struct BuildSizes {
+    long long nodes = 0, active = 0, segs = 0, refs = 0, leaf_refs = 0, leaf_ids = 0, lower = 0;
+    int upper_levels = 0, lower_levels = 0;
 };
 
 // Build scratch (kept across builds; grow-only).
@@ -125,6 +150,7 @@ struct BuildScratch {
     DevBuf<float> bwd_corner;                   // backward: 3 float3 per ray
     DevBuf<int> bwd_hist, bwd_offs;             // backward: radix histograms
     DevBuf<int> mesh_off;                       // forest: triangle offsets per mesh (M+1)
+    DevBuf<Lvl> lvl;                            // level state (one record)
     HostPinned host;
 };
 
@@ -150,19 +176,46 @@ struct pm_ctx {
     std::vector<long long> level_refs;
     int num_roots = 1;                 // meshes in the last build (roots 0..num_roots-1)
     std::vector<int> mesh_offsets;     // triangle offsets per mesh (host copy)
+    pmkd::BuildSizes last_sizes;       // what the last eager build reached
+    // captured batch (build + trace replayed as one graph launch)
+    hipGraphExec_t graph_exec = nullptr;
+    hipStream_t capture_stream = nullptr;
+    hipStream_t graph_last_stream = nullptr;
+    pmkd::BuildSizes graph_caps;
+    std::vector<std::pair<void **, void *>> graph_ptrs;   // buffers baked into the graph
+    pmkd::DevBuf<int> graph_off;       // the captured batch's mesh offsets (its own copy)
+    const float4 *graph_p0 = nullptr, *graph_p1 = nullptr, *graph_p2 = nullptr;
+    int graph_n = 0, graph_M = 0;
+    bool graph_boxes = false;
+    std::vector<int> graph_offsets;
+    bool graph_pending = false;        // a replay ran; host-side tree sizes not read back yet
+    const int *trace_fail = nullptr;   // while recording: the build status the trace checks
+    ~pm_ctx();
 };
 
 namespace pmkd {
 // pm_build.hip
 void build_tree(pm_ctx *ctx, const float4 *p0, const float4 *p1, const float4 *p2, int n,
                 const int *mesh_offsets, int M, const pm_build_params &prm, hipStream_t s);
+// record build + trace of a batch into a graph owned by ctx (see pm_batch_graph_capture)
+void capture_batch(pm_ctx *ctx, const float4 *p0, const float4 *p1, const float4 *p2, int n,
+                   const int *mesh_offsets, int M, const float4 *org, const float4 *dir, const int *ray_mesh,
+                   int nrays, pm_hit *out, int algo, const pm_build_params &bp, const pm_trace_params &tp,
+                   BuildSizes &caps, hipStream_t s);
+// replay the captured batch on stream s (checks its buffers were not regrown)
+void launch_batch(pm_ctx *ctx, hipStream_t s);
+// after a replay: read the level state back and update ctx's host-side sizes;
+// returns the build status (0 or a PM_ERR_*)
+int sync_graph_state(pm_ctx *ctx, hipStream_t s);
+void destroy_graph(pm_ctx *ctx);
 // pm_trace.hip
 void build_trace_view(pm_ctx *ctx, hipStream_t s);
 // the two halves of the trace view: packed nodes, and the leaf slots'
 // triangles (plus their areas when `areas` is given; `check` verifies the
-// prim indices, needed for uploaded trees only)
-void build_node_view(pm_ctx *ctx, hipStream_t s);
-void build_leaf_view(pm_ctx *ctx, float *areas, bool check, hipStream_t s);
+// prim indices, needed for uploaded trees only).  With n_dev / m_dev the
+// count is read on the device and ctx's count is its upper bound.
+void build_node_view(pm_ctx *ctx, hipStream_t s, const int *n_dev = nullptr);
+void build_leaf_view(pm_ctx *ctx, float *areas, bool check, hipStream_t s, const int *m_dev = nullptr);
 void trace_rays(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int nrays,
                 const int *ray_mesh, pm_hit *out, const pm_trace_params &prm, int4 *counters,
                 hipStream_t s);
@@ -186,7 +239,10 @@ void radial_wave(const float4 *unit, int nv, float frame, const float *wave, flo
 void add_mesh(pm_ctx *ctx, const float4 *verts, int nv, const unsigned *idx, int nidx,
               const float *model, float4 *p0, float4 *p1, float4 *p2, hipStream_t s);
 // device-wide exclusive scan of n ints; out has n+1 entries (out[n] = total)
-void scan_exclusive(const int *in, int *out, int n, BuildScratch &sc, hipStream_t s);
+// exclusive scans with out[n] = total; with n_dev the length is
+// n_mult * *n_dev (read on the device) and n is its upper bound
+void scan_exclusive(const int *in, int *out, int n, BuildScratch &sc, hipStream_t s,
+                    const int *n_dev = nullptr, int n_mult = 1);
 void scan_exclusive2(const int *in0, int *out0, const int *in1, int *out1, int n, BuildScratch &sc,
-                     hipStream_t s);
+                     hipStream_t s, const int *n_dev = nullptr, int n_mult = 1);
 }  // namespace pmkd

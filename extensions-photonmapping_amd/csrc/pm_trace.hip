@@ -15,6 +15,9 @@
 //                  contiguous and need no prim-index indirection.
 // e1/e2 are the same IEEE subtractions Moller-Trumbore performs, so hits are
 // bit-identical to testing the original vertices.
+#include <algorithm>
+#include <climits>
+
 #include "pm_device.h"
 #include "pm_internal.h"
 
@@ -93,7 +96,8 @@ __global__ __launch_bounds__(TB) void kd_restart_kernel(TreeDev T, const float4 
     int4 cnt = make_int4(0, 0, 0, 0);
     pm_hit res = miss_record();
     const int root_id = ray_root ? ray_root[r] : 0;
-    if (root_id < 0 || root_id >= T.num_roots) {    // no such mesh: the ray misses
+    // no such mesh, or (captured batch) the build before this trace failed: a miss
+    if (root_id < 0 || root_id >= T.num_roots || (T.fail && *T.fail)) {
         out[r] = res;
         if (COUNT) counters[r] = cnt;
         return;
@@ -151,7 +155,8 @@ __global__ __launch_bounds__(TB) void short_stack_kernel(TreeDev T, const float4
     int4 cnt = make_int4(0, 0, 0, 0);
     pm_hit res = miss_record();
     const int root_id = ray_root ? ray_root[r] : 0;
-    if (root_id < 0 || root_id >= T.num_roots) {    // no such mesh: the ray misses
+    // no such mesh, or (captured batch) the build before this trace failed: a miss
+    if (root_id < 0 || root_id >= T.num_roots || (T.fail && *T.fail)) {
         out[r] = res;
         if (COUNT) counters[r] = cnt;
         return;
@@ -250,35 +255,33 @@ __global__ __launch_bounds__(TB) void brute_kernel(const float4 *__restrict__ P0
 }
 
 // ---- trace view
+// n = n_dev ? *n_dev : n_host (a captured build passes its device node count)
 __global__ __launch_bounds__(TB) void pack_nodes_kernel(const int8_t *__restrict__ info,
                                                         const float *__restrict__ split,
                                                         const int2 *__restrict__ children,
                                                         const int *__restrict__ prim_index,
-                                                        const long long *__restrict__ bitmap, int n,
-                                                        int4 *__restrict__ out) {
-    const int i = blockIdx.x * TB + threadIdx.x;
-    if (i >= n) return;
-    const int ax = info[i] & 3;
-    if (ax) {
-        const int2 c = children[i];
-        out[i] = make_int4(__float_as_int(split[i]), c.x, c.y, ax);
-    } else {
-        const unsigned long long b = (unsigned long long)bitmap[i];
-        out[i] = make_int4(prim_index[i], (int)(unsigned)(b & 0xffffffffull), (int)(unsigned)(b >> 32), LEAF);
+                                                        const long long *__restrict__ bitmap, int n_host,
+                                                        const int *n_dev, int4 *__restrict__ out) {
+    const int n = n_dev ? *n_dev : n_host;
+    for (int i = blockIdx.x * TB + threadIdx.x; i < n; i += gridDim.x * TB) {
+        const int ax = info[i] & 3;
+        if (ax) {
+            const int2 c = children[i];
+            out[i] = make_int4(__float_as_int(split[i]), c.x, c.y, ax);
+        } else {
+            const unsigned long long b = (unsigned long long)bitmap[i];
+            out[i] = make_int4(prim_index[i], (int)(unsigned)(b & 0xffffffffull), (int)(unsigned)(b >> 32), LEAF);
+        }
     }
 }
 
 // leaf slot k -> v0 (w = id bits), e1, e2; with `areas`, also the slot's
 // triangle area 0.5 |e1 x e2| (CalcSurfaceArea, LowerTriangleMap.h:24-38,
 // the same operations tri_area performs)
-__global__ __launch_bounds__(TB) void leaf_tris_kernel(const int *__restrict__ prim_indices, int m,
-                                                       const float4 *__restrict__ P0,
-                                                       const float4 *__restrict__ P1,
-                                                       const float4 *__restrict__ P2, int ntris,
-                                                       float4 *__restrict__ out, int *bad,
-                                                       float *__restrict__ areas) {
-    const int k = blockIdx.x * TB + threadIdx.x;
-    if (k >= m) return;
+__device__ __forceinline__ void leaf_slot(const int *__restrict__ prim_indices, int k,
+                                          const float4 *__restrict__ P0, const float4 *__restrict__ P1,
+                                          const float4 *__restrict__ P2, int ntris, float4 *__restrict__ out,
+                                          int *bad, float *__restrict__ areas) {
     const int t = prim_indices[k];
     if (t < 0 || t >= ntris) {
         if (bad) atomicOr(bad, 1);
@@ -297,6 +300,18 @@ __global__ __launch_bounds__(TB) void leaf_tris_kernel(const int *__restrict__ p
         const f3 c = cross3(e1, e2);
         areas[k] = 0.5f * sqrtf(dot3(c, c));
     }
+}
+
+// m = m_dev ? *m_dev : m_host
+__global__ __launch_bounds__(TB) void leaf_tris_kernel(const int *__restrict__ prim_indices, int m_host,
+                                                       const int *m_dev, const float4 *__restrict__ P0,
+                                                       const float4 *__restrict__ P1,
+                                                       const float4 *__restrict__ P2, int ntris,
+                                                       float4 *__restrict__ out, int *bad,
+                                                       float *__restrict__ areas) {
+    const int m = m_dev ? *m_dev : m_host;
+    for (int k = blockIdx.x * TB + threadIdx.x; k < m; k += gridDim.x * TB)
+        leaf_slot(prim_indices, k, P0, P1, P2, ntris, out, bad, areas);
 }
 
 __global__ void clear_int_kernel(int *p) {
@@ -325,18 +340,19 @@ void dispatch(const TreeDev &T, int algo, const float4 *org, const float4 *dir, 
 
 }  // namespace
 
-void build_node_view(pm_ctx *ctx, hipStream_t s) {
+void build_node_view(pm_ctx *ctx, hipStream_t s, const int *n_dev) {
     const int n = ctx->num_nodes;
     ctx->packed.reserve((size_t)n + 1, s);
     if (n > 0) {
-        pack_nodes_kernel<<<(n + TB - 1) / TB, TB, 0, s>>>(ctx->nodes.info.ptr, ctx->nodes.split.ptr,
-                                                           ctx->nodes.children.ptr, ctx->nodes.prim_index.ptr,
-                                                           ctx->nodes.bitmap.ptr, n, ctx->packed.ptr);
+        const int blocks = std::min((n + TB - 1) / TB, n_dev ? 4096 : INT_MAX);
+        pack_nodes_kernel<<<blocks, TB, 0, s>>>(ctx->nodes.info.ptr, ctx->nodes.split.ptr, ctx->nodes.children.ptr,
+                                                ctx->nodes.prim_index.ptr, ctx->nodes.bitmap.ptr, n, n_dev,
+                                                ctx->packed.ptr);
         PM_LAUNCH_CHECK();
     }
 }
 
-void build_leaf_view(pm_ctx *ctx, float *areas, bool check, hipStream_t s) {
+void build_leaf_view(pm_ctx *ctx, float *areas, bool check, hipStream_t s, const int *m_dev) {
     const int m = ctx->num_prim_indices;
     ctx->leaf_tris.reserve(3 * (size_t)m + 3, s);
     if (m == 0) return;
@@ -349,8 +365,9 @@ void build_leaf_view(pm_ctx *ctx, float *areas, bool check, hipStream_t s) {
         PM_LAUNCH_CHECK();
         bad = sc.totals.ptr;
     }
-    leaf_tris_kernel<<<(m + TB - 1) / TB, TB, 0, s>>>(ctx->prim_indices.ptr, m, ctx->p0, ctx->p1, ctx->p2,
-                                                      ctx->ntris, ctx->leaf_tris.ptr, bad, areas);
+    const int blocks = std::min((m + TB - 1) / TB, m_dev ? 8192 : INT_MAX);
+    leaf_tris_kernel<<<blocks, TB, 0, s>>>(ctx->prim_indices.ptr, m, m_dev, ctx->p0, ctx->p1, ctx->p2,
+                                           ctx->ntris, ctx->leaf_tris.ptr, bad, areas);
     PM_LAUNCH_CHECK();
     if (check) {
         PM_HIP(hipMemcpyAsync(sc.host.ptr, sc.totals.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -391,6 +408,7 @@ void trace_rays(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int
     T.leaf_tris = ctx->leaf_tris.ptr;
     T.num_nodes = ctx->num_nodes;
     T.num_roots = ctx->num_roots;
+    T.fail = ctx->trace_fail;
     const bool skip = prm.leaf_skipping && ctx->has_boxes;
     const int S = prm.stack_size;
     if (skip) {

@@ -34,6 +34,7 @@ extern "C" {
 #define PM_ERR_UNSUPPORTED     5  /* option not provided by this library     */
 #define PM_ERR_HIP             6  /* HIP runtime error (CHECK_FOR_CUDA_ERROR) */
 #define PM_ERR_NO_TREE         7  /* trace before build / set_tree           */
+#define PM_ERR_CAPACITY        8  /* a captured build outgrew its capacities */
 
 /* TriangleMap::LowerAlgorithm, Utils/CUDA/TriangleMap.h:33 */
 #define PM_LOWER_BITMAP   0
@@ -225,6 +226,45 @@ int pm_build_batched(pm_ctx *ctx, const float *p0, const float *p1, const float 
 int pm_trace_batched(pm_ctx *ctx, int algo, const float *org, const float *dir, const int *ray_mesh,
                      int nrays, pm_hit *out, const pm_trace_params *params, int *counters,
                      void *stream);
+
+/* Capacities of a captured batch (pm_batch_graph_capture).  In: 0 fields
+ * are derived from an eager build of the same batch (1.5x what it reached
+ * + 1024; levels + 4 upper, + 2 lower); out: what was captured. */
+typedef struct {
+    long long nodes;          /* tree nodes */
+    long long active;         /* active nodes of one upper level */
+    long long segments;       /* chunks of one upper level */
+    long long level_refs;     /* references of one upper level */
+    long long leaf_refs;      /* leaf slots (prim indices) */
+    long long leaf_ids;       /* upper-stage leaves */
+    long long lower_nodes;    /* nodes of one lower (SAH) level */
+    int upper_levels, lower_levels;
+} pm_graph_caps;
+
+/* One graph launch per frame (no reference counterpart; the reference
+ * rebuilds and traces with host-driven launches, PhotonRenderingView.cpp:
+ * 34-85).  Records the batched build (as pm_build_batched) followed by the
+ * batched trace (as pm_trace_batched, algo KD_RESTART or SHORT_STACK) into
+ * a hipGraph owned by the context: every level's sizes live in device
+ * memory, buffers are allocated at the capacities beforehand, and no host
+ * read-back or synchronisation happens inside.  All pointers are baked in:
+ * each replay rebuilds the trees from the CURRENT contents of p0/p1/p2
+ * (e.g. after pm_deform_radial + pm_add_mesh) and traces org/dir into out;
+ * the mesh offsets, sizes and parameters are fixed.  Capture runs one eager
+ * build first (for the capacities) and one checked replay.  A replay whose
+ * geometry outgrows the capacities stops early with PM_ERR_CAPACITY
+ * reported by pm_batch_graph_status; capture again then. */
+int pm_batch_graph_capture(pm_ctx *ctx, const float *p0, const float *p1, const float *p2, int n,
+                           const int *mesh_offsets, int num_meshes, const float *org, const float *dir,
+                           const int *ray_mesh, int nrays, pm_hit *out, int algo,
+                           const pm_build_params *build_params, const pm_trace_params *trace_params,
+                           pm_graph_caps *caps, void *stream);
+/* Replay: one hipGraphLaunch on `stream`, asynchronous. */
+int pm_batch_graph_launch(pm_ctx *ctx, void *stream);
+/* Wait for the last replay and report its build status (PM_OK or the error
+ * that stopped it); also makes the tree's sizes visible to pm_tree_view /
+ * pm_tree_stats (which call it themselves). */
+int pm_batch_graph_status(pm_ctx *ctx, int *build_status, void *stream);
 
 /* Autotuner result (pm_autotune). */
 typedef struct {

@@ -1,0 +1,119 @@
+"""A batch's rebuild + trace recorded once and replayed as one hipGraph launch
+per frame (pm_batch_graph_capture / _launch / _status).
+
+Bar: every replay gives the tree and hits an eager batched build + trace of
+the same geometry gives, bit for bit (and so the oracle forest's); replays
+follow in-place geometry updates; replays interleaved with eager work on
+the same context stay exact; a replay that outgrows its capacities reports
+PM_ERR_CAPACITY and traces nothing.
+"""
+import numpy as np
+import pytest
+
+from forest_util import c4_small
+from test_gpu_parity import assert_same_hits, assert_same_tree, dev, orc_params
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as T
+    assert T.cuda.is_available(), "these tests need the GPU"
+    return T
+
+
+def frame_tris(scenes, c, frame):
+    """triangles of the C4-style batch ``c`` at ``frame`` (FP64 deformer)"""
+    V = scenes.displace_sphere64(c["unit"], frame)
+    F = c["faces"]
+    return scenes.tris_from(V[F[:, 0]], V[F[:, 1]], V[F[:, 2]])
+
+
+@pytest.mark.parametrize("preset", ["reference", "default"])
+def test_replays_equal_eager_and_oracle(pm, orc, scenes, torch, preset):
+    c = scenes.config4(num_meshes=10, rays_per_mesh=1000, hi=1.5e4)
+    off = c["tri_off"]
+    tris = frame_tris(scenes, c, 0.0)
+    P = [dev(torch, t) for t in tris]
+    O, D, RM = dev(torch, c["org"]), dev(torch, c["dirs"]), dev(torch, c["ray_mesh"])
+    H = torch.empty((O.shape[0], 4), dtype=torch.float32, device="cuda:0")
+    bp = pm.build_params(preset)
+    tp = pm.trace_params(stack_size=8)
+    g = pm.Context(0)
+    caps = g.batch_graph_capture(*P, off, O, D, RM, H, pm.SHORT_STACK, bp, tp)
+    assert caps["upper_levels"] > 0 and caps["nodes"] > 0
+    eager = pm.Context(0)
+    for frame in (0.0, 1.0, 5.0, 9.0):
+        t = frame_tris(scenes, c, frame)
+        for p, x in zip(P, t):
+            p.copy_(torch.from_numpy(x))
+        g.batch_graph_launch()
+        assert g.batch_graph_status() == 0
+        got_tree = g.tree_arrays()
+        got_hits = pm.hits_to_numpy(H)
+        # the same frame, eager
+        eager.build_batched(*P, off, bp)
+        ref_tree = eager.tree_arrays()
+        assert_same_tree(got_tree, ref_tree, f"{preset} frame {frame}")
+        h2 = pm.hits_to_numpy(eager.trace_batched(pm.SHORT_STACK, O, D, RM, tp))
+        assert_same_hits(got_hits, h2, f"{preset} frame {frame}")
+        # and the oracle forest of the same triangles
+        ref = orc.build_batched(t, off, orc_params(orc, bp))
+        assert_same_tree(got_tree, ref.arrays(), f"{preset} frame {frame} oracle")
+        r = orc.trace_batched(ref, t, c["org"], c["dirs"], c["ray_mesh"], orc.SHORT_STACK, stack_size=8)
+        assert_same_hits(got_hits, r, f"{preset} frame {frame} oracle hits")
+        st = g.stats()
+        assert st["num_nodes"] == ref_tree["info"].shape[0]
+
+
+def test_replays_with_eager_work_in_between(pm, orc, scenes, torch):
+    """several replays on one context, with eager builds/traces of other
+    geometry on the same context between them (same stream); every replay
+    is checked against an eager result"""
+    tris, off, org, dirs, ray_mesh = c4_small(scenes, num_meshes=7, empty=(2,))
+    P = [dev(torch, t) for t in tris]
+    O, D, RM = dev(torch, org), dev(torch, dirs), dev(torch, ray_mesh)
+    H = torch.empty((O.shape[0], 4), dtype=torch.float32, device="cuda:0")
+    bp = pm.build_params("reference")
+    ctx = pm.Context(0)
+    ctx.batch_graph_capture(*P, off, O, D, RM, H, pm.KD_RESTART, bp)
+    ref = orc.build_batched(tris, off, orc_params(orc, bp))
+    r = orc.trace_batched(ref, tris, org, dirs, ray_mesh, orc.KD_RESTART)
+    small = [dev(torch, t) for t in scenes.random_soup(3000, seed=21)]
+    sorg, sdirs = scenes.random_rays(4000, seed=22)
+    for i in range(4):
+        H.fill_(0.0)
+        ctx.batch_graph_launch()
+        torch.cuda.synchronize()
+        assert ctx.batch_graph_status() == 0
+        assert_same_hits(pm.hits_to_numpy(H), r, f"replay {i}")
+        assert_same_tree(ctx.tree_arrays(), ref.arrays(), f"replay {i} tree")
+        # eager work on the same context (a smaller tree: no buffer regrows)
+        ctx.build(*small, bp)
+        ctx.trace(pm.SHORT_STACK, dev(torch, sorg), dev(torch, sdirs))
+        torch.cuda.synchronize()
+
+
+def test_capacity_errors(pm, scenes, torch):
+    tris, off, org, dirs, ray_mesh = c4_small(scenes, num_meshes=5)
+    P = [dev(torch, t) for t in tris]
+    O, D, RM = dev(torch, org), dev(torch, dirs), dev(torch, ray_mesh)
+    H = torch.empty((O.shape[0], 4), dtype=torch.float32, device="cuda:0")
+    ctx = pm.Context(0)
+    # too few levels recorded, or too few nodes: the checked first replay fails
+    for caps in ({"upper_levels": 2}, {"nodes": 1}):
+        with pytest.raises(pm.PmError) as e:
+            ctx.batch_graph_capture(*P, off, O, D, RM, H, pm.SHORT_STACK, caps=caps)
+        assert e.value.code == 8, caps
+        with pytest.raises(pm.PmError):
+            ctx.batch_graph_launch()                  # nothing captured
+    # a later call that regrows the captured buffers invalidates the graph
+    ctx.batch_graph_capture(*P, off, O, D, RM, H, pm.SHORT_STACK)
+    big = [dev(torch, t) for t in scenes.random_soup(200000, seed=23)]
+    ctx.build(*big, pm.build_params("reference"))
+    with pytest.raises(pm.PmError) as e:
+        ctx.batch_graph_launch()
+    assert e.value.code == 8
+    with pytest.raises(pm.PmError):                   # BRUTE is not a batched tracer
+        ctx.batch_graph_capture(*P, off, O, D, RM, H, pm.BRUTE)
