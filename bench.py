@@ -173,6 +173,24 @@ def bench_batched(pm, scenes, torch, meshes: int = 256, rays_per_mesh: int = 655
     stage(0)
     tune = ctx.autotune(*P, O, D, mesh_offsets=off, ray_mesh=RM, base=bp, reps=2)
     tb, tb_build, tb_trace = run_batched(tune["build_params"], tune["trace_params"])
+    # the whole rebuild + trace as one captured graph, replayed once per frame
+    gctx = pm.Context(0)
+    stage(0)
+    caps = gctx.batch_graph_capture(*P, off, O, D, RM, H, pm.SHORT_STACK, bp, tp)
+    gt, gk = [], []
+    for f in range(frames):
+        stage(f + 1)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        ev[0].record(stream)
+        gctx.batch_graph_launch()
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        gt.append((time.perf_counter() - t) * 1e3)
+        gk.append(ev[0].elapsed_time(ev[1]))
+    graph_status = gctx.batch_graph_status()
+    gctx.close()
     pt = []
     for f in range(per_mesh_frames):
         torch.cuda.synchronize()
@@ -190,6 +208,11 @@ def bench_batched(pm, scenes, torch, meshes: int = 256, rays_per_mesh: int = 655
             "batched_trace_ms": round(b_trace, 3),
             "batched_trace_mrays": round(O.shape[0] / (b_trace * 1e-3) / 1e6, 1),
             "hit_fraction": round(hit_frac, 4),
+            "graph": {"launch_ms": round(float(np.median(gt)), 3),
+                      "launch_gpu_ms": round(float(np.median(gk)), 3),
+                      "speedup_vs_per_mesh": round(p / float(np.median(gt)), 2),
+                      "status": graph_status, "upper_levels": caps["upper_levels"],
+                      "lower_levels": caps["lower_levels"]},
             "autotuned": {"chunk_size": tune["chunk_size"], "lower_size": tune["lower_size"],
                           "stack_size": tune["stack_size"], "trials": tune["trials"],
                           "frame_ms": round(tb, 3), "build_ms": round(tb_build, 3),

@@ -1388,8 +1388,8 @@ void capture_batch(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4
     caps.leaf_refs = grow(seen.leaf_refs, caps.leaf_refs);
     caps.leaf_ids = grow(seen.leaf_ids, caps.leaf_ids);
     caps.lower = std::max(grow(seen.lower, caps.lower), caps.leaf_ids);
-    if (caps.upper_levels <= 0) caps.upper_levels = seen.upper_levels + 4;
-    if (caps.lower_levels <= 0) caps.lower_levels = seen.lower_levels + 2;
+    if (caps.upper_levels <= 0) caps.upper_levels = seen.upper_levels + 2;
+    if (caps.lower_levels <= 0) caps.lower_levels = seen.lower_levels + 1;
     for (long long c : {caps.nodes, 2 * caps.active, caps.segs, caps.refs, 3 * caps.leaf_refs, 2 * caps.lower})
         if (c >= INT_MAX) throw Error{PM_ERR_ARG, "capacities exceed the 32-bit index range"};
 

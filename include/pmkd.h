@@ -229,7 +229,7 @@ int pm_trace_batched(pm_ctx *ctx, int algo, const float *org, const float *dir, 
 
 /* Capacities of a captured batch (pm_batch_graph_capture).  In: 0 fields
  * are derived from an eager build of the same batch (1.5x what it reached
- * + 1024; levels + 4 upper, + 2 lower); out: what was captured. */
+ * + 1024; levels + 2 upper, + 1 lower); out: what was captured. */
 typedef struct {
     long long nodes;          /* tree nodes */
     long long active;         /* active nodes of one upper level */

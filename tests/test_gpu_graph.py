@@ -110,7 +110,7 @@ def test_capacity_errors(pm, scenes, torch):
             ctx.batch_graph_launch()                  # nothing captured
     # a later call that regrows the captured buffers invalidates the graph
     ctx.batch_graph_capture(*P, off, O, D, RM, H, pm.SHORT_STACK)
-    big = [dev(torch, t) for t in scenes.random_soup(200000, seed=23)]
+    big = [dev(torch, t) for t in scenes.random_soup(200000, seed=23, size=0.01)]
     ctx.build(*big, pm.build_params("reference"))
     with pytest.raises(pm.PmError) as e:
         ctx.batch_graph_launch()
