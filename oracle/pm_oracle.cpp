@@ -119,6 +119,72 @@ inline bool ray_box(f3 o, f3 inv, f3 bmin, f3 bmax) {
     return nearA <= farA;
 }
 
+/* Akenine-Moller triangle/box step 3 (the 9 edge x axis separating axes),
+ * IntersectionTests.h:20-160,378-395.  The three a0X/a1X/a2X groups are
+ * the same 2-D test on the (y,z), (z,x) and (x,y) components; GAMMA and
+ * EPSILON widen each interval. */
+constexpr float SAT_GAMMA = 0.00001f, SAT_EPS = 1.0001f;
+
+inline bool sat_edge(float au, float aw, float bu, float bw, float cu, float cw, float hu, float hw) {
+    const float p0 = aw * bu - au * bw;
+    const float p1 = (bu - au) * cw - (bw - aw) * cu;
+    float r = hw * std::fabs(bu - au) + hu * std::fabs(bw - aw) + SAT_GAMMA;
+    r *= SAT_EPS;
+    return !((p0 > r && p1 > r) || (p0 < -r && p1 < -r));
+}
+
+inline bool sat_plane(float u0, float w0, float u1, float w1, float u2, float w2, float hu, float hw) {
+    bool ret = true;
+    ret &= sat_edge(u0, w0, u1, w1, u2, w2, hu, hw);
+    ret &= sat_edge(u1, w1, u2, w2, u0, w0, hu, hw);
+    ret &= sat_edge(u2, w2, u0, w0, u1, w1, hu, hw);
+    return ret;
+}
+
+inline bool tri_box_step3(f3 v0, f3 v1, f3 v2, f3 bmin, f3 bmax) {
+    const f3 h = {(bmax.x - bmin.x) * 0.5f, (bmax.y - bmin.y) * 0.5f, (bmax.z - bmin.z) * 0.5f};
+    const f3 c = {bmin.x + h.x, bmin.y + h.y, bmin.z + h.z};
+    v0 = sub(v0, c); v1 = sub(v1, c); v2 = sub(v2, c);
+    bool ret = true;
+    ret &= sat_plane(v0.y, v0.z, v1.y, v1.z, v2.y, v2.z, h.y, h.z);   /* a00..a02 */
+    ret &= sat_plane(v0.z, v0.x, v1.z, v1.x, v2.z, v2.x, h.z, h.x);   /* a10..a12 */
+    ret &= sat_plane(v0.x, v0.y, v1.x, v1.y, v2.x, v2.y, h.x, h.y);   /* a20..a22 */
+    return ret;
+}
+
+/* WoopTransformationMatrix, IntersectionTests.h:437-467 */
+inline void woop_matrix(f3 a, f3 b, f3 c, f4 &m0, f4 &m1, f4 &m2) {
+    const f3 i0 = sub(a, c), i1 = sub(b, c), i2 = sub(cross(i0, i1), c);
+    const float det = i0.x * (i1.y * i2.z - i1.z * i2.y) + i1.x * (i2.y * i0.z - i2.z * i0.y) +
+                      i2.x * (i0.y * i1.z - i0.z * i1.y);
+    const float inv = 1.0f / det;
+    const f3 t0 = {(i1.y * i2.z - i1.z * i2.y) * inv, (i2.x * i1.z - i2.z * i1.x) * inv,
+                   (i1.x * i2.y - i1.y * i2.x) * inv};
+    const f3 t1 = {(i2.y * i0.z - i2.z * i0.y) * inv, (i0.x * i2.z - i0.z * i2.x) * inv,
+                   (i2.x * i0.y - i2.y * i0.x) * inv};
+    const f3 t2 = {(i0.y * i1.z - i0.z * i1.y) * inv, (i1.x * i0.z - i1.z * i0.x) * inv,
+                   (i0.x * i1.y - i0.y * i1.x) * inv};
+    m0 = {t0.x, t0.y, t0.z, dot(t0, c)};
+    m1 = {t1.x, t1.y, t1.z, dot(t1, c)};
+    m2 = {t2.x, t2.y, t2.z, dot(t2, c)};
+}
+
+/* IRayTracer::Woop with WoopLambda / WoopUV, IRayTracer.h:78-109 */
+inline bool woop_test(const f4 &m0, const f4 &m1, const f4 &m2, f3 o, f3 d, float &th, float &tu, float &tv) {
+    const f3 n = mk3(m2);
+    const float lambda = -(dot(n, o) - m2.w) / dot(n, d);
+    if (0.0f <= lambda && lambda <= th) {
+        const f3 a = mk3(m0), b = mk3(m1);
+        const float u = lambda * dot(a, d) + dot(a, o) - m0.w;
+        const float v = lambda * dot(b, d) + dot(b, o) - m1.w;
+        if (u >= -0.0f && v >= -0.0f && u + v <= 1.0f) {
+            th = lambda; tu = u; tv = v;
+            return true;
+        }
+    }
+    return false;
+}
+
 /* LowerTriangleMap.h:32-36: 0.5f * length(cross(v1-v0, v2-v0)) */
 inline float tri_area(f3 v0, f3 v1, f3 v2) {
     const f3 c = cross(sub(v1, v0), sub(v2, v0));
@@ -324,8 +390,20 @@ int build_upper(const f4 *P0, const f4 *P1, const f4 *P2, int N, const int *off,
             const int node = active_index + owner[k];
             const int ax = T.info[node];
             const float s = T.split[node];
-            side[k] = comp4(rmin[k], ax) <= s;
-            side[(size_t)R + k] = s < comp4(rmax[k], ax);
+            bool l = comp4(rmin[k], ax) <= s, r = s < comp4(rmax[k], ax);
+            if (prm.split_method == 1 && l && r) {
+                /* SetDivideSide<false>, TriangleUpperChildren.h:64-142: a
+                 * straddling triangle goes to the child boxes it meets */
+                const int t = fbits(rmin[k].w);
+                const f3 v0 = mk3(P0[t]), v1 = mk3(P1[t]), v2 = mk3(P2[t]);
+                const f3 nmn = mk3(T.bmin[node]), nmx = mk3(T.bmax[node]);
+                const f3 lmax = {ax == AX_X ? s : nmx.x, ax == AX_Y ? s : nmx.y, ax == AX_Z ? s : nmx.z};
+                const f3 rmn = {ax == AX_X ? s : nmn.x, ax == AX_Y ? s : nmn.y, ax == AX_Z ? s : nmn.z};
+                l = tri_box_step3(v0, v1, v2, nmn, lmax);
+                r = tri_box_step3(v0, v1, v2, rmn, nmx);
+            }
+            side[k] = l;
+            side[(size_t)R + k] = r;
         }
         exclusive_scan(side, addr);
         const int newTri = addr[2 * (size_t)R];
@@ -547,6 +625,119 @@ SahResult sah_node(const f3 *mins, const f3 *maxs, const float *areas, uint64_t 
     return r;
 }
 
+/* The balanced ("SSAH") lower creator, TriangleMapBalancedCreator.cu:62-244
+ * with CreateSplittingPlanes / CalcRelationForSets / CalcSplit /
+ * CreateChildren (LowerTriangleMap.h:84-155,266-410): the candidate planes
+ * are SAH's (6 per prim, prim ascending, X/Y/Z, low then high), the cost of
+ * a candidate is small^2 + large^2 of its two child counts, the first
+ * minimum below lower_size^2 wins, and the node splits iff
+ * (C_t + cost) / n < n. */
+struct BalancedResult { bool split; int axis; float pos; uint64_t left, right; int cost; };
+
+/* the decision for one node; mins/maxs indexed by bit (slot - prim index) */
+BalancedResult balanced_node(const f3 *mins, const f3 *maxs, uint64_t bmp, float Ct, int lower_size) {
+    int best = lower_size * lower_size, bax = 0, bslot = 0;
+    bool bhigh = false;
+    uint64_t bl = 0, br = 0;
+    for (uint64_t b = bmp; b; b &= b - 1) {
+        const int i = ctz64(b);
+        for (int ax = AX_X; ax <= AX_Z; ++ax) {
+            const float lo = comp(mins[i], ax), hi = comp(maxs[i], ax);
+            uint64_t ll = 0, lr = 0, hl = 0, hr = 0;
+            for (uint64_t c = bmp; c; c &= c - 1) {
+                const int j = ctz64(c);
+                const float mnj = comp(mins[j], ax), mxj = comp(maxs[j], ax);
+                const uint64_t bit = (uint64_t)1 << j;
+                if (mnj <= lo) ll |= bit;
+                if (lo < mxj) lr |= bit;
+                if (mnj <= hi) hl |= bit;
+                if (hi < mxj) hr |= bit;
+            }
+            for (int h = 0; h < 2; ++h) {
+                const uint64_t l = h ? hl : ll, r = h ? hr : lr;
+                const int small = std::min(popc64(l), popc64(r)), large = std::max(popc64(l), popc64(r));
+                const int area = small * small + large * large;
+                if (area < best) { best = area; bl = l; br = r; bax = ax; bslot = i; bhigh = h; }
+            }
+        }
+    }
+    BalancedResult res{};
+    const int n = popc64(bmp);
+    const float ssah = (Ct + (float)best) / (float)n;
+    res.split = n > 0 && ssah < (float)n;
+    res.cost = best;
+    if (res.split) {
+        res.axis = bax; res.left = bl; res.right = br;
+        res.pos = bhigh ? comp(maxs[bslot], bax) : comp(mins[bslot], bax);
+    }
+    return res;
+}
+
+int build_lower_balanced(const Params &prm, Tree &T, const std::vector<f4> &prim_min,
+                         const std::vector<f4> &prim_max) {
+    const int LS = prm.lower_size;
+    for (int leaf : T.leaf_ids) {   /* PreprocessLeafNodes, LowerTriangleMap.h:40-54 */
+        const int64_t cnt = T.bitmap[leaf];
+        T.bitmap[leaf] = cnt >= 64 ? -1 : (int64_t)(((uint64_t)1 << cnt) - 1);
+    }
+    std::vector<int> list(T.leaf_ids);
+    int level = 0;
+    while (!list.empty()) {
+        const int A = (int)list.size();
+        T.lower_level_nodes.push_back(A);
+        std::vector<int> flags(A, 0), axis(A, 0);
+        std::vector<uint64_t> Ls(A), Rs(A);
+        std::vector<float> pos(A);
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int id = 0; id < A; ++id) {
+            const int par = list[id];
+            const int pi = T.prim_index[par];
+            const uint64_t bmp = (uint64_t)T.bitmap[par];
+            f3 mins[64], maxs[64];
+            for (uint64_t b = bmp; b; b &= b - 1) {
+                const int i = ctz64(b);
+                mins[i] = mk3(prim_min[pi + i]); maxs[i] = mk3(prim_max[pi + i]);
+            }
+            const BalancedResult r = balanced_node(mins, maxs, bmp, prm.traversal_cost, LS);
+            flags[id] = r.split;
+            if (r.split) { axis[id] = r.axis; Ls[id] = r.left; Rs[id] = r.right; pos[id] = r.pos; }
+        }
+        std::vector<int> off;
+        for (int id = 0; id < A; ++id) {
+            const int par = list[id];
+            T.info[par] = flags[id] ? (int8_t)axis[id] : LEAF;
+            if (flags[id]) T.split[par] = pos[id];
+        }
+        exclusive_scan(flags, off);
+        const int splits = A ? off[A - 1] + flags[A - 1] : 0;
+        const int base = T.size();
+        T.resize(base + 2 * splits);
+        for (int id = 0; id < A; ++id) {   /* CreateChildren, LowerTriangleMap.h:371-410 */
+            if (!flags[id]) continue;
+            const int par = list[id];
+            const int l = base + off[id], r = l + splits;
+            T.prim_index[l] = T.prim_index[par]; T.bitmap[l] = (int64_t)Ls[id];
+            T.prim_index[r] = T.prim_index[par]; T.bitmap[r] = (int64_t)Rs[id];
+            T.parent[l] = T.parent[r] = par;
+            T.children[2 * par] = l; T.children[2 * par + 1] = r;
+            if (prm.propagate_aabbs) {   /* PropagateAabbToChildren, LowerTriangleMap.h:602-636 */
+                const int ax = axis[id];
+                const float s = pos[id];
+                const f4 mn = T.bmin[par], mx = T.bmax[par];
+                T.bmin[l] = mn;
+                T.bmin[r] = {ax == AX_X ? s : mn.x, ax == AX_Y ? s : mn.y, ax == AX_Z ? s : mn.z, 0.0f};
+                T.bmax[l] = {ax == AX_X ? s : mx.x, ax == AX_Y ? s : mx.y, ax == AX_Z ? s : mx.z, 0.0f};
+                T.bmax[r] = mx;
+            }
+        }
+        list.clear();
+        for (int i = 0; i < 2 * splits; ++i) list.push_back(base + i);
+        ++level;
+    }
+    T.lower_levels = level;
+    return 0;
+}
+
 int build_lower(const f4 *P0, const f4 *P1, const f4 *P2, const Params &prm, Tree &T,
                 const std::vector<f4> &prim_min, const std::vector<f4> &prim_max) {
     const int LS = prm.lower_size;
@@ -559,6 +750,7 @@ int build_lower(const f4 *P0, const f4 *P1, const f4 *P2, const Params &prm, Tre
         }
         return 0;
     }
+    if (prm.lower_algorithm == 1) return build_lower_balanced(prm, T, prim_min, prim_max);
     if (prm.lower_algorithm != 2) { g_err = "lower algorithm not supported"; return 5; }
     (void)LS;
     const size_t RL = prim_min.size();
@@ -661,7 +853,15 @@ struct View {
     const int8_t *info; const float *split; const int *children;
     const int *prim_index; const int64_t *bitmap; const f4 *bmin, *bmax;
     const int *prim_indices; int num_nodes;
+    const f4 *w0, *w1, *w2;   /* Woop matrices per triangle (GeometryList::GetWoopValues), or null: MT */
 };
+
+/* one triangle: Moller-Trumbore on the vertices or Woop on its matrices */
+inline bool intersect(const View &V, const f4 *P0, const f4 *P1, const f4 *P2, int prim, f3 o, f3 d,
+                      float &th, float &tu, float &tv) {
+    if (V.w0) return woop_test(V.w0[prim], V.w1[prim], V.w2[prim], o, d, th, tu, tv);
+    return moller_trumbore(mk3(P0[prim]), mk3(P1[prim]), mk3(P2[prim]), o, d, th, tu, tv);
+}
 
 inline orc_hit miss_record() { orc_hit h; h.id = -1; h.t = INFINITY; h.u = 0.0f; h.v = 0.0f; return h; }
 
@@ -680,7 +880,7 @@ inline int test_leaf(const View &V, const f4 *P0, const f4 *P1, const f4 *P2, in
         const int i = ctz64(bm);
         const int prim = V.prim_indices[pi + i];
         if (cnt) cnt[2]++;
-        if (moller_trumbore(mk3(P0[prim]), mk3(P1[prim]), mk3(P2[prim]), o, d, th, tu, tv)) hit = prim;
+        if (intersect(V, P0, P1, P2, prim, o, d, th, tu, tv)) hit = prim;
         bm &= bm - 1;
     }
     return hit;
@@ -773,12 +973,13 @@ orc_hit trace_short_stack(const View &V, const f4 *P0, const f4 *P1, const f4 *P
 }
 
 /* Exhaustive, BruteTracer.cu:54-96 (first hit; no direction adjustment) */
-orc_hit trace_brute(const f4 *P0, const f4 *P1, const f4 *P2, int ntris, int cap, f3 o, f3 d, int *cnt) {
+orc_hit trace_brute(const View &V, const f4 *P0, const f4 *P1, const f4 *P2, int ntris, int cap, f3 o, f3 d,
+                    int *cnt) {
     const int prims = cap > 0 ? std::min(ntris, cap) : ntris;
     float th = F_INF, tu = 0.0f, tv = 0.0f;
     int hit = -1;
     for (int prim = 0; prim < prims; ++prim) {
-        if (moller_trumbore(mk3(P0[prim]), mk3(P1[prim]), mk3(P2[prim]), o, d, th, tu, tv)) hit = prim;
+        if (intersect(V, P0, P1, P2, prim, o, d, th, tu, tv)) hit = prim;
     }
     if (cnt) cnt[2] += prims;
     if (th < F_INF && hit != -1) { orc_hit h; h.id = hit; h.t = th; h.u = tu; h.v = tv; return h; }
@@ -803,6 +1004,66 @@ Params to_params(const orc_build_params *p) {
 
 }  // namespace
 
+/* ------------------------------------------------------------------ */
+/* Canvas shading: ColorKernels.h:15-85 and the tracers' shading loops */
+/* (Raytracer.cu:165-262, ShortStack.cu:96-194, BruteTracer.cu:54-96). */
+/* Host-branch arithmetic; the specular power is the correctly rounded */
+/* float of the double-precision power.                               */
+/* ------------------------------------------------------------------ */
+struct Light { f3 pos, ambient, diffuse, specular; };
+
+inline f3 scale(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline f3 add(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline f3 mul(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+inline f3 normalize3(f3 v) { return scale(v, 1.0f / std::sqrt(dot(v, v))); }
+inline float saturate(float x) { return x > 1.0f ? 1.0f : (x > 0.0f ? x : 0.0f); }
+inline float clamp01(float x) { return std::fmax(0.0f, std::fmin(x, 1.0f)); }
+inline float spec_pow(float x, float y) { return (float)std::pow((double)x, (double)y); }
+
+struct Color { float r, g, b, a; };
+
+/* PhongLighting, ColorKernels.h:15-45 (shadow = 1: the reference's shadow
+ * rays are disabled, Raytracer.cu:219-226) */
+inline Color phong(const Color c, f3 normal, f3 point, f3 origin, const Light &L) {
+    const f3 lightDir = normalize3(sub(L.pos, point));
+    const float diffuse = saturate(dot(lightDir, normal) * 1.0f);
+    f3 refl = sub(scale(normal, 2.0f * dot(normal, lightDir)), lightDir);
+    refl = normalize3(refl);
+    const float stemp = saturate(dot(normalize3(sub(origin, point)), refl));
+    const float specProp = 1.0f - c.a;
+    const float specular = specProp * spec_pow(stemp, 128.0f * specProp);
+    const f3 base = {c.r, c.g, c.b};
+    const f3 light = add(mul(base, add(L.ambient, scale(L.diffuse, diffuse))),
+                         scale(scale(scale(L.specular, specular), 10.0f), 1.0f));
+    const float alpha = c.a < specular ? specular : c.a;
+    return {clamp01(light.x), clamp01(light.y), clamp01(light.z), alpha};
+}
+
+/* Lighting, ColorKernels.h:47-77: shade the hit and turn the ray into its
+ * mirror reflection from the hit point */
+inline Color lighting(const orc_hit &h, f3 &origin, f3 &dir, const f4 *n0, const f4 *n1, const f4 *n2,
+                      const uint8_t *c0, const Light &L) {
+    const f3 point = add(origin, scale(dir, h.t));
+    const float w = 1.0f - h.u - h.v;
+    f3 normal = add(add(scale(mk3(n0[h.id]), w), scale(mk3(n1[h.id]), h.u)), scale(mk3(n2[h.id]), h.v));
+    normal = normalize3(normal);
+    const uint8_t *cc = c0 + 4 * (size_t)h.id;
+    const Color col = {cc[0] / 255.0f, cc[1] / 255.0f, cc[2] / 255.0f, cc[3] / 255.0f};
+    dir = add(scale(normal, 2.0f * dot(normal, scale(dir, -1.0f))), dir);
+    dir = normalize3(dir);
+    const Color out = phong(col, normal, point, origin, L);
+    origin = add(point, scale(dir, 0.00001f));
+    return out;
+}
+
+/* BlendColor, ColorKernels.h:79-84 */
+inline Color blend(const Color o, Color n) {
+    const float a = n.a;
+    n.a = 1.0f;
+    const float k = (1.0f - o.a) * a;
+    return {o.r + k * n.r, o.g + k * n.g, o.b + k * n.b, o.a + k * n.a};
+}
+
 struct orc_tree { Tree T; };
 
 extern "C" {
@@ -819,9 +1080,10 @@ int orc_build_batched(const float *p0, const float *p1, const float *p2, int n,
         if (mesh_offsets[m] > mesh_offsets[m + 1]) { g_err = "mesh offsets must be non-decreasing"; return 1; }
     if (mesh_offsets[0] != 0 || mesh_offsets[num_meshes] != n) { g_err = "mesh offsets must span [0, n]"; return 1; }
     const Params prm = to_params(params);
-    if (prm.split_method != 0) { g_err = "split method not implemented"; return 2; }
+    if (prm.split_method == 2) { g_err = "split method not implemented"; return 2; }
+    if (prm.split_method != 0 && prm.split_method != 1) { g_err = "unknown split method"; return 1; }
     if (prm.lower_size < 1 || prm.lower_size > 64) { g_err = "lower_size must be in [1, 64]"; return 1; }
-    if (prm.lower_algorithm != 0 && prm.lower_algorithm != 2) { g_err = "lower algorithm not supported"; return 5; }
+    if (prm.lower_algorithm < 0 || prm.lower_algorithm > 2) { g_err = "unknown lower algorithm"; return 1; }
     orc_tree *t = new orc_tree();
     const f4 *P0 = reinterpret_cast<const f4 *>(p0), *P1 = reinterpret_cast<const f4 *>(p1),
              *P2 = reinterpret_cast<const f4 *>(p2);
@@ -840,6 +1102,33 @@ int orc_build(const float *p0, const float *p1, const float *p2, int n,
 }
 
 int orc_num_roots(const orc_tree *t) { return t->T.num_roots; }
+
+void orc_woop_matrix(const float *a, const float *b, const float *c, float *m) {
+    f4 m0, m1, m2;
+    woop_matrix({a[0], a[1], a[2]}, {b[0], b[1], b[2]}, {c[0], c[1], c[2]}, m0, m1, m2);
+    const f4 ms[3] = {m0, m1, m2};
+    for (int i = 0; i < 3; ++i) {
+        m[4 * i] = ms[i].x; m[4 * i + 1] = ms[i].y; m[4 * i + 2] = ms[i].z; m[4 * i + 3] = ms[i].w;
+    }
+}
+
+int orc_balanced_node(int n, const float *bmin, const float *bmax, long long bitmap, float traversal_cost,
+                      int lower_size, int *axis, float *pos, long long *left, long long *right, int *cost) {
+    if (n < 0 || n > 64) { g_err = "n must be in [0, 64]"; return -1; }
+    f3 mins[64], maxs[64];
+    for (int i = 0; i < n; ++i) {
+        mins[i] = {bmin[3 * i], bmin[3 * i + 1], bmin[3 * i + 2]};
+        maxs[i] = {bmax[3 * i], bmax[3 * i + 1], bmax[3 * i + 2]};
+    }
+    const BalancedResult r = balanced_node(mins, maxs, (uint64_t)bitmap, traversal_cost, lower_size);
+    *axis = r.axis; *pos = r.pos; *left = (long long)r.left; *right = (long long)r.right; *cost = r.cost;
+    return r.split;
+}
+
+int orc_tri_box_step3(const float *v0, const float *v1, const float *v2, const float *bmin, const float *bmax) {
+    return tri_box_step3({v0[0], v0[1], v0[2]}, {v1[0], v1[1], v1[2]}, {v2[0], v2[1], v2[2]},
+                         {bmin[0], bmin[1], bmin[2]}, {bmax[0], bmax[1], bmax[2]});
+}
 
 void orc_free(orc_tree *t) { delete t; }
 int orc_num_nodes(const orc_tree *t) { return t->T.size(); }
@@ -946,7 +1235,19 @@ int orc_trace_batched(const orc_tree *t, const float *p0, const float *p1, const
     if (t) {
         const Tree &T = t->T;
         V = View{T.info.data(), T.split.data(), T.children.data(), T.prim_index.data(),
-                 T.bitmap.data(), T.bmin.data(), T.bmax.data(), T.prim_indices.data(), T.size()};
+                 T.bitmap.data(), T.bmin.data(), T.bmax.data(), T.prim_indices.data(), T.size(),
+                 nullptr, nullptr, nullptr};
+    }
+    /* CreateWoopValues (GeometryList.cu:80-101): matrices of (p1, p2, p0) */
+    std::vector<f4> w0, w1, w2;
+    if (params->intersection == 1) {
+        w0.resize((size_t)std::max(ntris, 0)); w1.resize(w0.size()); w2.resize(w0.size());
+#pragma omp parallel for schedule(static)
+        for (int k = 0; k < ntris; ++k) woop_matrix(mk3(P1[k]), mk3(P2[k]), mk3(P0[k]), w0[k], w1[k], w2[k]);
+        V.w0 = w0.data(); V.w1 = w1.data(); V.w2 = w2.data();
+    } else if (params->intersection != 0) {
+        g_err = "unknown intersection algorithm";
+        return 1;
     }
     const int S = params->stack_size > 0 ? params->stack_size : 4;
     if (S > 64) { g_err = "stack_size must be <= 64"; return 1; }
@@ -961,9 +1262,66 @@ int orc_trace_batched(const orc_tree *t, const float *p0, const float *p1, const
         const f3 o = mk3(O[r]), d = mk3(D[r]);
         const int root = ray_root ? ray_root[r] : 0;
         if (root < 0 || root >= roots) { out[r] = miss_record(); continue; }   /* no such mesh */
-        if (algo == 0) out[r] = trace_brute(P0, P1, P2, ntris, params->brute_cap, o, d, cnt);
+        if (algo == 0) out[r] = trace_brute(V, P0, P1, P2, ntris, params->brute_cap, o, d, cnt);
         else if (algo == 1) out[r] = trace_kd_restart(V, P0, P1, P2, o, d, skip, root, cnt);
         else out[r] = trace_short_stack(V, P0, P1, P2, o, d, skip, S, root, cnt);
+    }
+    return 0;
+}
+
+int orc_trace_canvas(const orc_tree *t, const float *p0, const float *p1, const float *p2, int ntris,
+                     int algo, const float *org, const float *dir, int nrays, const float *n0,
+                     const float *n1, const float *n2, const unsigned char *c0, const float *light,
+                     int max_bounces, const orc_trace_params *params, unsigned char *canvas) {
+    g_err.clear();
+    if (nrays < 0 || !canvas || !params || !light || (nrays > 0 && (!n0 || !n1 || !n2 || !c0))) {
+        g_err = "invalid argument";
+        return 1;
+    }
+    if (algo != 0 && !t) { g_err = "tree required"; return 1; }
+    const f4 *P0 = reinterpret_cast<const f4 *>(p0), *P1 = reinterpret_cast<const f4 *>(p1),
+             *P2 = reinterpret_cast<const f4 *>(p2);
+    const f4 *O = reinterpret_cast<const f4 *>(org), *D = reinterpret_cast<const f4 *>(dir);
+    const f4 *N0 = reinterpret_cast<const f4 *>(n0), *N1 = reinterpret_cast<const f4 *>(n1),
+             *N2 = reinterpret_cast<const f4 *>(n2);
+    const Light L = {{light[0], light[1], light[2]}, {light[3], light[4], light[5]},
+                     {light[6], light[7], light[8]}, {light[9], light[10], light[11]}};
+    View V{};
+    if (t) {
+        const Tree &T = t->T;
+        V = View{T.info.data(), T.split.data(), T.children.data(), T.prim_index.data(),
+                 T.bitmap.data(), T.bmin.data(), T.bmax.data(), T.prim_indices.data(), T.size(),
+                 nullptr, nullptr, nullptr};
+    }
+    std::vector<f4> w0, w1, w2;
+    if (params->intersection == 1) {
+        w0.resize((size_t)std::max(ntris, 0)); w1.resize(w0.size()); w2.resize(w0.size());
+        for (int k = 0; k < ntris; ++k) woop_matrix(mk3(P1[k]), mk3(P2[k]), mk3(P0[k]), w0[k], w1[k], w2[k]);
+        V.w0 = w0.data(); V.w1 = w1.data(); V.w2 = w2.data();
+    }
+    const int S = params->stack_size > 0 ? params->stack_size : 4;
+    const bool skip = params->leaf_skipping != 0;
+#ifdef _OPENMP
+    const int nt = params->threads > 0 ? params->threads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nt)
+#endif
+    for (int r = 0; r < nrays; ++r) {
+        f3 o = mk3(O[r]), d = mk3(D[r]);
+        if (algo != 0) d = adjust_dir(d);
+        Color col = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int b = 0; b < max_bounces; ++b) {
+            orc_hit h;
+            if (algo == 0) h = trace_brute(V, P0, P1, P2, ntris, params->brute_cap, o, d, nullptr);
+            else if (algo == 1) h = trace_kd_restart(V, P0, P1, P2, o, d, skip, 0, nullptr);
+            else h = trace_short_stack(V, P0, P1, P2, o, d, skip, S, 0, nullptr);
+            if (h.id < 0) break;
+            col = blend(col, lighting(h, o, d, N0, N1, N2, c0, L));
+            if (algo != 0) d = adjust_dir(d);
+            if (!(col.a < 0.97f)) break;
+        }
+        unsigned char *px = canvas + 4 * (size_t)r;
+        px[0] = (unsigned char)(col.r * 255.0f); px[1] = (unsigned char)(col.g * 255.0f);
+        px[2] = (unsigned char)(col.b * 255.0f); px[3] = (unsigned char)(col.a * 255.0f);
     }
     return 0;
 }

@@ -49,6 +49,7 @@ typedef struct {
     int stack_size;     /* ShortStack.cu:25 default 4 */
     int brute_cap;      /* BruteTracer.cu:50 4096 in the reference; 0 = all */
     int threads;        /* OpenMP threads, 0 = runtime default */
+    int intersection;   /* IRayTracer::IntersectionAlgorithm (IRayTracer.h:30): 0 MOELLER, 1 WOOP */
 } orc_trace_params;
 
 typedef struct { int id; float t, u, v; } orc_hit;
@@ -76,6 +77,17 @@ int  orc_build_batched(const float *p0, const float *p1, const float *p2, int n,
                        const int *mesh_offsets, int num_meshes,
                        const orc_build_params *params, orc_tree **out);
 int  orc_num_roots(const orc_tree *t);
+/* WoopTransformationMatrix(a, b, c) -> m[12] = m0, m1, m2 (float4 each) */
+void orc_woop_matrix(const float *a, const float *b, const float *c, float *m);
+/* the balanced lower creator's decision for one node (CalcSplit,
+ * LowerTriangleMap.h:299-369): returns split (0/1), outputs axis, position,
+ * child masks and the winning small^2 + large^2 */
+int  orc_balanced_node(int n, const float *bmin, const float *bmax, long long bitmap,
+                       float traversal_cost, int lower_size, int *axis, float *pos,
+                       long long *left, long long *right, int *cost);
+/* TriangleAabbIntersectionStep3 (the DIVIDE split's triangle/box test) */
+int  orc_tri_box_step3(const float *v0, const float *v1, const float *v2,
+                       const float *bmin, const float *bmax);
 
 /* tree arrays (copies are owned by the tree) */
 int  orc_num_nodes(const orc_tree *t);
@@ -109,6 +121,17 @@ int  orc_trace(const orc_tree *t, const float *p0, const float *p1,
                const float *p2, int ntris, int algo, const float *org,
                const float *dir, int nrays, const orc_trace_params *params,
                orc_hit *out, int *counters);
+/* The tracers' canvas output (IRayTracer::Trace(canvas), IRayTracer.h:47):
+ * per ray, first hit -> PhongLighting + mirror reflection -> BlendColor,
+ * until a miss, alpha >= 0.97 or max_bounces; canvas = uchar4 per ray.
+ * n0/n1/n2 float4 corner normals and c0 uchar4 colour per triangle
+ * (GeometryList, GeometryList.cu:197-232); light = position, ambient,
+ * diffuse, specular (3 floats each). */
+int  orc_trace_canvas(const orc_tree *t, const float *p0, const float *p1, const float *p2,
+                      int ntris, int algo, const float *org, const float *dir, int nrays,
+                      const float *n0, const float *n1, const float *n2,
+                      const unsigned char *c0, const float *light, int max_bounces,
+                      const orc_trace_params *params, unsigned char *canvas);
 /* the same over a forest: ray r starts at root ray_root[r] (a ray whose
  * root is not one of the forest's misses); kd-restart and short-stack. */
 int  orc_trace_batched(const orc_tree *t, const float *p0, const float *p1,

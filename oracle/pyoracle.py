@@ -16,7 +16,9 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 HIT_DTYPE = np.dtype([("id", "<i4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
 BRUTE, KD_RESTART, SHORT_STACK = 0, 1, 2
-LOWER_BITMAP, LOWER_SAH = 0, 2
+LOWER_BITMAP, LOWER_BALANCED, LOWER_SAH = 0, 1, 2
+SPLIT_BOX, SPLIT_DIVIDE = 0, 1
+MOELLER, WOOP = 0, 1
 
 
 class BuildParams(C.Structure):
@@ -30,7 +32,7 @@ class BuildParams(C.Structure):
 
 class TraceParams(C.Structure):
     _fields_ = [("leaf_skipping", C.c_int), ("stack_size", C.c_int), ("brute_cap", C.c_int),
-                ("threads", C.c_int)]
+                ("threads", C.c_int), ("intersection", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -51,6 +53,8 @@ def build_params(preset: str = "reference", **kw) -> BuildParams:
                     sah_leaf_max=0, sah_empty_bonus=1.0, max_ref_factor=0)
     if preset == "bitmap":
         p.lower_algorithm = LOWER_BITMAP
+    elif preset == "balanced":
+        p.lower_algorithm = LOWER_BALANCED
     elif preset == "config":
         p.sah_leaf_max = 8
         p.sah_empty_bonus = 0.8
@@ -79,6 +83,15 @@ def lib():
         L.orc_build.restype = C.c_int
         L.orc_build_batched.argtypes = [P, P, P, C.c_int, P, C.c_int, C.POINTER(BuildParams), C.POINTER(P)]
         L.orc_build_batched.restype = C.c_int
+        L.orc_woop_matrix.argtypes = [P, P, P, P]
+        L.orc_woop_matrix.restype = None
+        L.orc_tri_box_step3.argtypes = [P, P, P, P, P]
+        L.orc_tri_box_step3.restype = C.c_int
+        L.orc_balanced_node.argtypes = [C.c_int, P, P, C.c_longlong, C.c_float, C.c_int, P, P, P, P, P]
+        L.orc_balanced_node.restype = C.c_int
+        L.orc_trace_canvas.argtypes = [P, P, P, P, C.c_int, C.c_int, P, P, C.c_int, P, P, P, P, P, C.c_int,
+                                       C.POINTER(TraceParams), P]
+        L.orc_trace_canvas.restype = C.c_int
         L.orc_free.argtypes = [P]
         L.orc_last_error.restype = C.c_char_p
         for name in ("orc_num_nodes", "orc_num_prim_indices", "orc_num_leaf_ids",
@@ -197,7 +210,7 @@ def build_batched(tris, mesh_offsets, params: BuildParams | None = None) -> Tree
 
 
 def trace_batched(tree: Tree, tris, org, dirs, ray_root, algo: int, leaf_skipping=True,
-                  stack_size=4, threads=0, counters=False):
+                  stack_size=4, threads=0, counters=False, intersection=MOELLER):
     """trace ray r from root ray_root[r] of a forest"""
     p0, p1, p2 = (_c(t, np.float32) for t in tris)
     org = _c(org, np.float32)
@@ -206,7 +219,7 @@ def trace_batched(tree: Tree, tris, org, dirs, ray_root, algo: int, leaf_skippin
     n = org.shape[0]
     out = np.zeros(n, HIT_DTYPE)
     cnt = np.zeros((n, 4), np.int32) if counters else None
-    prm = TraceParams(int(bool(leaf_skipping)), stack_size, 0, threads)
+    prm = TraceParams(int(bool(leaf_skipping)), stack_size, 0, threads, intersection)
     rc = lib().orc_trace_batched(tree.h, _ptr(p0), _ptr(p1), _ptr(p2), p0.shape[0], algo,
                                  _ptr(org), _ptr(dirs), _ptr(root), n, C.byref(prm), _ptr(out),
                                  _ptr(cnt))
@@ -230,14 +243,14 @@ def tree_from_arrays(a: dict) -> Tree:
 
 
 def trace(tree: Tree | None, tris, org, dirs, algo: int, leaf_skipping=True, stack_size=4,
-          brute_cap=0, threads=0, counters=False):
+          brute_cap=0, threads=0, counters=False, intersection=MOELLER):
     p0, p1, p2 = (_c(t, np.float32) for t in tris)
     org = _c(org, np.float32)
     dirs = _c(dirs, np.float32)
     n = org.shape[0]
     out = np.zeros(n, HIT_DTYPE)
     cnt = np.zeros((n, 4), np.int32) if counters else None
-    prm = TraceParams(int(bool(leaf_skipping)), stack_size, brute_cap, threads)
+    prm = TraceParams(int(bool(leaf_skipping)), stack_size, brute_cap, threads, intersection)
     rc = lib().orc_trace(tree.h if tree is not None else None, _ptr(p0), _ptr(p1), _ptr(p2),
                          p0.shape[0], algo, _ptr(org), _ptr(dirs), n, C.byref(prm),
                          _ptr(out), _ptr(cnt))
@@ -311,3 +324,47 @@ def trace_backward(tris, org, dirs, hits, grad_t=None, grad_u=None, grad_v=None,
     if rc != 0:
         raise RuntimeError(lib().orc_last_error().decode())
     return gvtx, go, gd
+
+
+def woop_matrix(a, b, c) -> np.ndarray:
+    """WoopTransformationMatrix(a, b, c) -> float32 [3, 4] (m0, m1, m2)"""
+    x = [_c(v, np.float32) for v in (a, b, c)]
+    m = np.zeros((3, 4), np.float32)
+    lib().orc_woop_matrix(*[_ptr(v) for v in x], _ptr(m))
+    return m
+
+
+def tri_box_step3(v0, v1, v2, bmin, bmax) -> bool:
+    x = [_c(v, np.float32) for v in (v0, v1, v2, bmin, bmax)]
+    return bool(lib().orc_tri_box_step3(*[_ptr(v) for v in x]))
+
+
+def balanced_node(bmin, bmax, bitmap, traversal_cost=24.0, lower_size=32):
+    """balanced creator's decision for one node -> dict(split, axis, pos, left, right, cost)"""
+    bmin, bmax = _c(bmin, np.float32), _c(bmax, np.float32)
+    ax, pos, l, r, cost = C.c_int(), C.c_float(), C.c_longlong(), C.c_longlong(), C.c_int()
+    sp = lib().orc_balanced_node(bmin.shape[0], _ptr(bmin), _ptr(bmax), int(np.int64(np.uint64(bitmap))),
+                                 traversal_cost, lower_size, C.byref(ax), C.byref(pos), C.byref(l),
+                                 C.byref(r), C.byref(cost))
+    return dict(split=bool(sp), axis=ax.value, pos=pos.value, left=l.value & (2 ** 64 - 1),
+                right=r.value & (2 ** 64 - 1), cost=cost.value)
+
+
+def trace_canvas(tree, tris, org, dirs, algo, normals, colors, light, max_bounces=64,
+                 leaf_skipping=True, stack_size=4, brute_cap=0, threads=0, intersection=MOELLER):
+    """shaded canvas (uchar4 per ray); normals = (n0, n1, n2) float32 [N, 4],
+    colors = uint8 [N, 4], light = 12 floats (position, ambient, diffuse, specular)"""
+    p0, p1, p2 = (_c(t, np.float32) for t in tris)
+    org, dirs = _c(org, np.float32), _c(dirs, np.float32)
+    n0, n1, n2 = (_c(x, np.float32) for x in normals)
+    c0 = _c(colors, np.uint8)
+    lt = _c(light, np.float32)
+    n = org.shape[0]
+    out = np.zeros((n, 4), np.uint8)
+    prm = TraceParams(int(bool(leaf_skipping)), stack_size, brute_cap, threads, intersection)
+    rc = lib().orc_trace_canvas(tree.h if tree is not None else None, _ptr(p0), _ptr(p1), _ptr(p2),
+                                p0.shape[0], algo, _ptr(org), _ptr(dirs), n, _ptr(n0), _ptr(n1), _ptr(n2),
+                                _ptr(c0), _ptr(lt), max_bounces, C.byref(prm), _ptr(out))
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return out
