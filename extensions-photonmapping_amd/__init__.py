@@ -26,13 +26,15 @@ ERRORS = {1: "PM_ERR_ARG", 2: "PM_ERR_NOT_IMPLEMENTED", 3: "PM_ERR_DEPTH", 4: "P
 LOWER_BITMAP, LOWER_BALANCED, LOWER_SAH = 0, 1, 2
 SPLIT_BOX, SPLIT_DIVIDE, SPLIT_SPLIT = 0, 1, 2
 BRUTE, KD_RESTART, SHORT_STACK = 0, 1, 2
+MOELLER, WOOP = 0, 1
 
 # every symbol include/pmkd.h declares
 EXPORTS = ("pm_build_params_default", "pm_trace_params_default", "pm_create", "pm_destroy",
            "pm_last_error", "pm_build", "pm_set_geometry", "pm_tree_view", "pm_set_tree",
            "pm_tree_stats", "pm_trace", "pm_photon_bounce", "pm_add_mesh", "pm_trace_backward",
            "pm_build_batched", "pm_trace_batched", "pm_osc_surface", "pm_deform_radial",
-           "pm_autotune", "pm_batch_graph_capture", "pm_batch_graph_launch", "pm_batch_graph_status")
+           "pm_autotune", "pm_batch_graph_capture", "pm_batch_graph_launch", "pm_batch_graph_status",
+           "pm_trace_canvas", "pm_add_mesh_attributes")
 
 
 class PmError(RuntimeError):
@@ -52,7 +54,7 @@ class BuildParams(C.Structure):
 
 class TraceParams(C.Structure):
     _fields_ = [("leaf_skipping", C.c_int), ("stack_size", C.c_int), ("brute_cap", C.c_int),
-                ("reserved", C.c_int)]
+                ("intersection", C.c_int)]
 
 
 class Tree(C.Structure):
@@ -65,6 +67,11 @@ class Tree(C.Structure):
                 ("p1", C.c_void_p), ("p2", C.c_void_p), ("num_tris", C.c_int),
                 ("upper_levels", C.c_int), ("lower_levels", C.c_int), ("has_boxes", C.c_int),
                 ("num_roots", C.c_int)]
+
+
+class Light(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("ambient", C.c_float * 3), ("diffuse", C.c_float * 3),
+                ("specular", C.c_float * 3)]
 
 
 class GraphCaps(C.Structure):
@@ -127,11 +134,13 @@ def load_library(path: str = LIB_PATH):
                                          C.POINTER(TraceParams), C.POINTER(GraphCaps), P]
     L.pm_batch_graph_launch.argtypes = [P, P]
     L.pm_batch_graph_status.argtypes = [P, C.POINTER(C.c_int), P]
+    L.pm_trace_canvas.argtypes = [P, I, P, P, I, P, P, P, P, C.POINTER(Light), I, P, C.POINTER(TraceParams), P]
+    L.pm_add_mesh_attributes.argtypes = [P, P, P, I, P, I, P, P, P, P, P, P, P, I, P]
     for n in ("pm_create", "pm_destroy", "pm_build", "pm_set_geometry", "pm_tree_view",
               "pm_set_tree", "pm_tree_stats", "pm_trace", "pm_photon_bounce", "pm_add_mesh",
               "pm_trace_backward", "pm_build_batched", "pm_trace_batched", "pm_osc_surface",
               "pm_deform_radial", "pm_autotune", "pm_batch_graph_capture", "pm_batch_graph_launch",
-              "pm_batch_graph_status"):
+              "pm_batch_graph_status", "pm_trace_canvas", "pm_add_mesh_attributes"):
         getattr(L, n).restype = C.c_int
     _lib = L
     return L
@@ -141,7 +150,8 @@ def build_params(preset: str = "reference", **kw) -> BuildParams:
     """'default' = the reference TriangleMap defaults (BITMAP lower creator,
     TriangleMap.cu:27-49); 'reference' = the same with the SAH lower creator
     (TriangleMapSAHCreator, traversal cost 24); 'config' = 'reference' plus a
-    forced split above 8 triangles and a 0.8 empty-side cost factor."""
+    forced split above 8 triangles and a 0.8 empty-side cost factor;
+    'balanced' = the defaults with the balanced lower creator."""
     p = BuildParams()
     load_library().pm_build_params_default(C.byref(p))
     if preset in ("reference", "config"):
@@ -149,6 +159,8 @@ def build_params(preset: str = "reference", **kw) -> BuildParams:
     if preset == "config":
         p.sah_leaf_max = 8
         p.sah_empty_bonus = 0.8
+    elif preset == "balanced":
+        p.lower_algorithm = LOWER_BALANCED
     elif preset not in ("default", "reference", "bitmap"):
         raise ValueError(preset)
     for k, v in kw.items():
@@ -386,6 +398,40 @@ class Context:
                                      _dev_ptr(out) if n else None, C.byref(params), cptr,
                                      _stream_ptr(stream)))
         return out
+
+    def trace_canvas(self, algo: int, org, dirs, normals, colors, light, max_bounces: int = 64,
+                     params: TraceParams | None = None, out=None, stream=None):
+        """IRayTracer::Trace into a canvas: shaded uchar4 per ray.  normals =
+        (n0, n1, n2) device float32 [N, 4]; colors device uint8 [N, 4]; light =
+        12 floats (position, ambient, diffuse, specular)"""
+        torch = _torch()
+        n = org.shape[0]
+        if out is None:
+            out = torch.empty((max(n, 1), 4), dtype=torch.uint8, device=org.device)
+        lt = Light()
+        v = [float(x) for x in np.asarray(light, dtype=np.float32).reshape(12)]
+        for i, f in enumerate(("position", "ambient", "diffuse", "specular")):
+            getattr(lt, f)[:] = v[3 * i:3 * i + 3]
+        params = params or trace_params()
+        self._check(self._L.pm_trace_canvas(self.h, algo, _dev_ptr(org), _dev_ptr(dirs), n,
+                                            *[_dev_ptr(x) for x in normals], _dev_ptr(colors), C.byref(lt),
+                                            max_bounces, _dev_ptr(out), C.byref(params), _stream_ptr(stream)))
+        return out[:n]
+
+    def add_mesh_attributes(self, normals, indices, n0, n1, n2, colors=None, c0=None, c1=None, c2=None,
+                            offset: int = 0, model=None, stream=None):
+        """gather per-vertex normals (device float4) and colours (device
+        uint8 [V, 4]) into per-corner triangle arrays rows [offset, ...)"""
+        m = None
+        if model is not None:
+            m = np.ascontiguousarray(model, dtype=np.float32).reshape(16)
+        cp = [None, None, None, None]
+        if colors is not None:
+            cp = [_dev_ptr(colors), _dev_ptr(c0), _dev_ptr(c1), _dev_ptr(c2)]
+        self._check(self._L.pm_add_mesh_attributes(
+            self.h, _dev_ptr(normals), cp[0], normals.shape[0], _dev_ptr(indices), indices.numel(),
+            m.ctypes.data if m is not None else None, _dev_ptr(n0), _dev_ptr(n1), _dev_ptr(n2), cp[1], cp[2],
+            cp[3], offset, _stream_ptr(stream)))
 
     def add_mesh(self, vertices, indices, p0, p1, p2, offset: int = 0, model=None, stream=None):
         """gather an indexed mesh (device float4 vertices, device int32/uint32

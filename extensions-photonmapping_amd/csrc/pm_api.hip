@@ -34,12 +34,11 @@ int guarded(pm_ctx *ctx, F &&f) {
 }
 
 void check_build_params(const pm_build_params &p) {
-    if (p.split_method == PM_SPLIT_DIVIDE || p.split_method == PM_SPLIT_SPLIT)
-        throw Error{PM_ERR_NOT_IMPLEMENTED, "split method not implemented (only BOX)"};
-    if (p.split_method != PM_SPLIT_BOX) throw Error{PM_ERR_ARG, "unknown split method"};
-    if (p.lower_algorithm == PM_LOWER_BALANCED)
-        throw Error{PM_ERR_UNSUPPORTED, "BALANCED lower creator is not provided"};
-    if (p.lower_algorithm != PM_LOWER_BITMAP && p.lower_algorithm != PM_LOWER_SAH)
+    if (p.split_method == PM_SPLIT_SPLIT)   // TriangleMapUpperCreator.cu:551-552
+        throw Error{PM_ERR_NOT_IMPLEMENTED, "split method SPLIT is not implemented (as in the reference)"};
+    if (p.split_method != PM_SPLIT_BOX && p.split_method != PM_SPLIT_DIVIDE) throw Error{PM_ERR_ARG, "unknown split method"};
+    if (p.lower_algorithm != PM_LOWER_BITMAP && p.lower_algorithm != PM_LOWER_SAH &&
+        p.lower_algorithm != PM_LOWER_BALANCED)
         throw Error{PM_ERR_ARG, "unknown lower algorithm"};
     if (p.lower_size < 1 || p.lower_size > 64) throw Error{PM_ERR_ARG, "lower_size must be in [1, 64]"};
     if (p.max_upper_levels < 0) throw Error{PM_ERR_ARG, "max_upper_levels must be >= 0"};
@@ -47,6 +46,12 @@ void check_build_params(const pm_build_params &p) {
     if (p.chunk_size != 0 && p.chunk_size != 128 && p.chunk_size != 256 && p.chunk_size != 512 &&
         p.chunk_size != 1024)
         throw Error{PM_ERR_ARG, "chunk_size must be 0, 128, 256, 512 or 1024"};
+}
+
+void check_trace_params(const pm_trace_params &p) {
+    if (p.stack_size < 1 || p.stack_size > 16) throw Error{PM_ERR_ARG, "stack_size must be in [1, 16]"};
+    if (p.intersection != PM_INTERSECT_MOELLER && p.intersection != PM_INTERSECT_WOOP)
+        throw Error{PM_ERR_ARG, "unknown intersection algorithm"};
 }
 
 }  // namespace
@@ -74,7 +79,7 @@ void pm_trace_params_default(pm_trace_params *p) {
     p->leaf_skipping = 1;
     p->stack_size = 4;
     p->brute_cap = 0;
-    p->reserved = 0;
+    p->intersection = PM_INTERSECT_MOELLER;
 }
 
 int pm_create(pm_ctx **out, int device) {
@@ -116,7 +121,9 @@ int pm_batch_graph_capture(pm_ctx *ctx, const float *p0, const float *p1, const 
         check_build_params(bp);
         pm_trace_params tp;
         if (trace_params) tp = *trace_params; else pm_trace_params_default(&tp);
-        if (tp.stack_size < 1 || tp.stack_size > 16) throw Error{PM_ERR_ARG, "stack_size must be in [1, 16]"};
+        check_trace_params(tp);
+        if (tp.intersection != PM_INTERSECT_MOELLER)
+            throw Error{PM_ERR_ARG, "a captured batch traces with MOELLER (Woop matrices are made per call)"};
         pmkd::BuildSizes c;
         if (caps) {
             c.nodes = caps->nodes; c.active = caps->active; c.segs = caps->segments; c.refs = caps->level_refs;
@@ -333,7 +340,7 @@ int pm_trace(pm_ctx *ctx, int algo, const float *org, const float *dir, int nray
             throw Error{PM_ERR_ARG, "unknown trace algorithm"};
         pm_trace_params prm;
         if (params) prm = *params; else pm_trace_params_default(&prm);
-        if (prm.stack_size < 1 || prm.stack_size > 16) throw Error{PM_ERR_ARG, "stack_size must be in [1, 16]"};
+        check_trace_params(prm);
         if (algo == PM_TRACE_BRUTE) {
             if (ctx->ntris > 0 && !ctx->p0) throw Error{PM_ERR_NO_TREE, "no geometry"};
         } else if (!ctx->has_tree) {
@@ -353,7 +360,7 @@ int pm_trace_batched(pm_ctx *ctx, int algo, const float *org, const float *dir, 
             throw Error{PM_ERR_ARG, "batched tracing takes KD_RESTART or SHORT_STACK"};
         pm_trace_params prm;
         if (params) prm = *params; else pm_trace_params_default(&prm);
-        if (prm.stack_size < 1 || prm.stack_size > 16) throw Error{PM_ERR_ARG, "stack_size must be in [1, 16]"};
+        check_trace_params(prm);
         if (!ctx->has_tree) throw Error{PM_ERR_NO_TREE, "no tree: call pm_build_batched first"};
         pmkd::trace_rays(ctx, algo, reinterpret_cast<const float4 *>(org), reinterpret_cast<const float4 *>(dir),
                          nrays, ray_mesh, out, prm, reinterpret_cast<int4 *>(counters), (hipStream_t)stream);
@@ -406,6 +413,50 @@ int pm_autotune(pm_ctx *ctx, const float *p0, const float *p1, const float *p2, 
                        reinterpret_cast<const float4 *>(p2), n, mesh_offsets, mesh_offsets ? num_meshes : 1,
                        reinterpret_cast<const float4 *>(org), reinterpret_cast<const float4 *>(dir), ray_mesh,
                        nrays, prm, builds_per_trace, reps, *out, (hipStream_t)stream);
+    });
+}
+
+int pm_trace_canvas(pm_ctx *ctx, int algo, const float *org, const float *dir, int nrays,
+                    const float *n0, const float *n1, const float *n2, const unsigned char *c0,
+                    const pm_light *light, int max_bounces, unsigned char *canvas,
+                    const pm_trace_params *params, void *stream) {
+    return guarded(ctx, [&] {
+        if (nrays < 0 || (nrays > 0 && (!org || !dir || !canvas || !n0 || !n1 || !n2 || !c0)) || !light)
+            throw Error{PM_ERR_ARG, "invalid arguments"};
+        if (max_bounces < 1) throw Error{PM_ERR_ARG, "max_bounces must be >= 1"};
+        if (algo != PM_TRACE_BRUTE && algo != PM_TRACE_KD_RESTART && algo != PM_TRACE_SHORT_STACK)
+            throw Error{PM_ERR_ARG, "unknown trace algorithm"};
+        pm_trace_params prm;
+        if (params) prm = *params; else pm_trace_params_default(&prm);
+        check_trace_params(prm);
+        pmkd::sync_graph_state(ctx, (hipStream_t)stream);
+        if (algo == PM_TRACE_BRUTE) {
+            if (ctx->ntris > 0 && !ctx->p0) throw Error{PM_ERR_NO_TREE, "no geometry"};
+        } else if (!ctx->has_tree) {
+            throw Error{PM_ERR_NO_TREE, "no tree: call pm_build or pm_set_tree first"};
+        }
+        pmkd::trace_canvas(ctx, algo, reinterpret_cast<const float4 *>(org), reinterpret_cast<const float4 *>(dir),
+                           nrays, reinterpret_cast<const float4 *>(n0), reinterpret_cast<const float4 *>(n1),
+                           reinterpret_cast<const float4 *>(n2), reinterpret_cast<const uchar4 *>(c0), *light,
+                           max_bounces, reinterpret_cast<uchar4 *>(canvas), prm, (hipStream_t)stream);
+    });
+}
+
+int pm_add_mesh_attributes(pm_ctx *ctx, const float *normals, const unsigned char *colors, int nv,
+                           const unsigned *indices, int nidx, const float *model, float *n0, float *n1,
+                           float *n2, unsigned char *c0, unsigned char *c1, unsigned char *c2, int offset,
+                           void *stream) {
+    return guarded(ctx, [&] {
+        if (nv < 0 || nidx < 0 || nidx % 3 || offset < 0 ||
+            (nidx > 0 && (!normals || !indices || !n0 || !n1 || !n2 || (colors && (!c0 || !c1 || !c2)))))
+            throw Error{PM_ERR_ARG, "invalid mesh attributes"};
+        uchar4 *q0 = colors ? reinterpret_cast<uchar4 *>(c0) + offset : nullptr;
+        uchar4 *q1 = colors ? reinterpret_cast<uchar4 *>(c1) + offset : nullptr;
+        uchar4 *q2 = colors ? reinterpret_cast<uchar4 *>(c2) + offset : nullptr;
+        pmkd::add_mesh_attributes(ctx, reinterpret_cast<const float4 *>(normals),
+                                  reinterpret_cast<const uchar4 *>(colors), nv, indices, nidx, model,
+                                  reinterpret_cast<float4 *>(n0) + offset, reinterpret_cast<float4 *>(n1) + offset,
+                                  reinterpret_cast<float4 *>(n2) + offset, q0, q1, q2, (hipStream_t)stream);
     });
 }
 

@@ -369,13 +369,31 @@ __global__ __launch_bounds__(BS) void seg_fill_kernel(const NodePtrs N, const Lv
     }
 }
 
+// The DIVIDE split method (SetDivideSide<false>, TriangleUpperChildren.h:
+// 64-142): a ref that straddles the plane goes to the child boxes its
+// triangle meets (triangle/box step 3 against the node box cut at the
+// plane).  p0 == nullptr: the BOX method (SetSplitSide).
+struct DivideGeom {
+    const float4 *p0, *p1, *p2;
+};
+
+__device__ __forceinline__ void divide_sides(const DivideGeom &g, float id_bits, int ax, float sp, f3 nmn,
+                                             f3 nmx, bool &l, bool &r) {
+    const int t = __float_as_int(id_bits);
+    const f3 v0 = mk3(g.p0[t]), v1 = mk3(g.p1[t]), v2 = mk3(g.p2[t]);
+    const f3 lmax = {ax == AX_X ? sp : nmx.x, ax == AX_Y ? sp : nmx.y, ax == AX_Z ? sp : nmx.z};
+    const f3 rmin = {ax == AX_X ? sp : nmn.x, ax == AX_Y ? sp : nmn.y, ax == AX_Z ? sp : nmn.z};
+    l = tri_box_step3(v0, v1, v2, nmn, lmax);
+    r = tri_box_step3(v0, v1, v2, rmin, nmx);
+}
+
 // SetSplitSide (TriangleUpperChildren.h:10-62) counted per segment
 template <int SEGT>
 __global__ __launch_bounds__(WPB * 64) void classify_kernel(
     const NodePtrs N, const Lvl *__restrict__ L, const int *__restrict__ seg_node,
     const int *__restrict__ seg_start, const int *__restrict__ seg_count,
     const float4 *__restrict__ rmin, const float4 *__restrict__ rmax, int *__restrict__ segL,
-    int *__restrict__ segR) {
+    int *__restrict__ segR, DivideGeom dg) {
     const int S = L->S, active_index = L->active_index;
     const int lane = threadIdx.x & 63;
     for (int s = wave_index(); s < S; s += wave_count()) {
@@ -383,6 +401,8 @@ __global__ __launch_bounds__(WPB * 64) void classify_kernel(
         const int ax = N.info[node];
         const float sp = N.split[node];
         const int start = seg_start[s], count = seg_count[s];
+        f3 nmn = {0.0f, 0.0f, 0.0f}, nmx = nmn;
+        if (dg.p0) { nmn = mk3(N.bmin[node]); nmx = mk3(N.bmax[node]); }
         int cl = 0, cr = 0;
 #pragma unroll
         for (int it = 0; it < SEGT / 64; ++it) {
@@ -390,8 +410,10 @@ __global__ __launch_bounds__(WPB * 64) void classify_kernel(
             const bool valid = j < count;
             bool l = false, r = false;
             if (valid) {
-                l = comp4(rmin[start + j], ax) <= sp;
+                const float4 mn = rmin[start + j];
+                l = comp4(mn, ax) <= sp;
                 r = sp < comp4(rmax[start + j], ax);
+                if (dg.p0 && l && r) divide_sides(dg, mn.w, ax, sp, nmn, nmx, l, r);
             }
             cl += __popcll(__ballot(l));
             cr += __popcll(__ballot(r));
@@ -498,12 +520,14 @@ __device__ __forceinline__ void scatter_segment(
     const int *__restrict__ segLRoff, const int *__restrict__ slot_leaf,
     const float4 *__restrict__ rmin, const float4 *__restrict__ rmax,
     float4 *__restrict__ nmin, float4 *__restrict__ nmax, float4 *__restrict__ fmin,
-    float4 *__restrict__ fmax, int *__restrict__ keys_next) {
+    float4 *__restrict__ fmax, int *__restrict__ keys_next, const DivideGeom &dg) {
     const int lane = threadIdx.x & 63;
     const int i = seg_node[s];
     const int node = active_index + i;
     const int ax = N.info[node];
     const float sp = N.split[node];
+    f3 nmn = {0.0f, 0.0f, 0.0f}, nmx = nmn;
+    if (dg.p0) { nmn = mk3(N.bmin[node]); nmx = mk3(N.bmax[node]); }
     const int2 ch = N.children[node];
     const bool leafL = slot_leaf[i] != 0, leafR = slot_leaf[A + i] != 0;
     const int start = seg_start[s], count = seg_count[s];
@@ -528,6 +552,7 @@ __device__ __forceinline__ void scatter_segment(
             mx = rmax[start + j];
             l = comp4(mn, ax) <= sp;
             r = sp < comp4(mx, ax);
+            if (dg.p0 && l && r) divide_sides(dg, mn.w, ax, sp, nmn, nmx, l, r);
         }
         const unsigned long long blm = __ballot(l), brm = __ballot(r);
         if (l) {
@@ -578,13 +603,13 @@ __global__ __launch_bounds__(WPB * 64) void scatter_kernel(
     const int *__restrict__ segLRoff, const int *__restrict__ slot_leaf,
     const float4 *__restrict__ rmin, const float4 *__restrict__ rmax,
     float4 *__restrict__ nmin, float4 *__restrict__ nmax, float4 *__restrict__ fmin,
-    float4 *__restrict__ fmax, int *__restrict__ keys_next) {
+    float4 *__restrict__ fmax, int *__restrict__ keys_next, DivideGeom dg) {
     const int S = L->S, A = L->A, active_index = L->active_index, next_active = L->next_active;
     const int L_all = L->L_all, LL_all = L->LL_all, ulp = L->final_refs;
     for (int s = wave_index(); s < S; s += wave_count())
         scatter_segment<SEGT>(N, s, A, active_index, next_active, L_all, LL_all, ulp, seg_node, seg_start,
                               seg_count, segLoff, segRoff, segLLoff, segLRoff, slot_leaf, rmin, rmax, nmin,
-                              nmax, fmin, fmax, keys_next);
+                              nmax, fmin, fmax, keys_next, dg);
 }
 
 // ---- plan kernels: the host arithmetic of the level loop
@@ -874,6 +899,69 @@ __global__ __launch_bounds__(WPB * 64) void sah_kernel(
     }
 }
 
+// The balanced ("SSAH") creator's decision (CalcSplit with
+// CalcRelationForSets, LowerTriangleMap.h:266-369), one wave per node: the
+// SAH's candidate planes and masks (ballots), cost small^2 + large^2 of the
+// two child counts, first minimum below lower_size^2; split iff
+// (C_t + cost) / n < n.  Children get no area (the reference writes none).
+__device__ __forceinline__ void balanced_node(const NodePtrs &N, int id, int par,
+                                              const float4 *__restrict__ pmin,
+                                              const float4 *__restrict__ pmax, float Ct, int LS,
+                                              int *__restrict__ flag, long long *__restrict__ sets,
+                                              float *__restrict__ careas) {
+    const int lane = threadIdx.x & 63;
+    const int pi = N.prim_index[par];
+    const unsigned long long bmp = (unsigned long long)N.bitmap[par];
+    const int n = __popcll(bmp);
+    const bool has = (bmp >> lane) & 1ull;
+    f3 mn = {0.0f, 0.0f, 0.0f}, mx = {0.0f, 0.0f, 0.0f};
+    if (has) { mn = mk3(pmin[pi + lane]); mx = mk3(pmax[pi + lane]); }
+    int best = LS * LS, bax = 0, bslot = 0;
+    bool bhigh = false;
+    unsigned long long bl = 0, br = 0;
+    for (unsigned long long b = bmp; b; b &= b - 1) {
+        const int i = __builtin_ctzll(b);
+        for (int axis = AX_X; axis <= AX_Z; ++axis) {
+            const float mnj = comp3(mn, axis), mxj = comp3(mx, axis);
+            const float lo = __shfl(mnj, i, 64), hi = __shfl(mxj, i, 64);
+            const unsigned long long lL = __ballot(has && mnj <= lo);
+            const unsigned long long lR = __ballot(has && lo < mxj);
+            const unsigned long long hL = __ballot(has && mnj <= hi);
+            const unsigned long long hR = __ballot(has && hi < mxj);
+            for (int h = 0; h < 2; ++h) {
+                const unsigned long long l = h ? hL : lL, r = h ? hR : lR;
+                const int cl = __popcll(l), cr = __popcll(r);
+                const int small = min(cl, cr), large = max(cl, cr);
+                const int cost = small * small + large * large;
+                if (cost < best) { best = cost; bl = l; br = r; bax = axis; bslot = i; bhigh = h; }
+            }
+        }
+    }
+    if (lane != 0) return;
+    const bool split = n > 0 && (Ct + (float)best) / (float)n < (float)n;
+    if (split) {
+        N.info[par] = (int8_t)bax;
+        N.split[par] = bhigh ? comp4(pmax[pi + bslot], bax) : comp4(pmin[pi + bslot], bax);
+        sets[2 * id] = (long long)bl; sets[2 * id + 1] = (long long)br;
+        careas[2 * id] = 0.0f; careas[2 * id + 1] = 0.0f;
+    } else {
+        N.info[par] = LEAF;
+    }
+    flag[id] = split;
+}
+
+__global__ __launch_bounds__(WPB * 64) void balanced_kernel(
+    NodePtrs N, const int *__restrict__ leaf_ids, const Lvl *__restrict__ L,
+    const float4 *__restrict__ pmin, const float4 *__restrict__ pmax, float Ct, int LS,
+    int *__restrict__ flag, long long *__restrict__ sets, float *__restrict__ careas) {
+    const int An = L->An, list_base = L->list_base;
+    const int *list = L->use_list ? leaf_ids : nullptr;
+    for (int id = wave_index(); id < An; id += wave_count()) {
+        const int par = list ? list[id] : list_base + id;
+        balanced_node(N, id, par, pmin, pmax, Ct, LS, flag, sets, careas);
+    }
+}
+
 // after the SAH pass: place the split nodes' children
 __global__ void plan_lower_kernel(Lvl *L, const int *__restrict__ off) {
     if (threadIdx.x | blockIdx.x) return;
@@ -1109,6 +1197,7 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
     auto g_nodes = [&](long long n) { return grid_for(graph ? caps->active : n, BS, graph ? 2048 : INT_MAX); };
     auto g_waves = [&](long long n) { return grid_for(graph ? caps->segs : n, WPB, graph ? 2048 : INT_MAX); };
     auto g_segs = [&](long long n) { return grid_for(graph ? caps->segs : n, BS, graph ? 2048 : INT_MAX); };
+    const DivideGeom dg = {prm.split_method == PM_SPLIT_DIVIDE ? P0 : nullptr, P1, P2};
     const int *dA = graph ? &L->A : nullptr;
     const int *dS = graph ? &L->S : nullptr;
     const int nA = graph ? (int)caps->active : 0;
@@ -1153,7 +1242,7 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
         launch_classify(seg, g_waves(h.S), s, NP, (const Lvl *)L, (const int *)sc.seg_node.ptr,
                         (const int *)sc.seg_start.ptr, (const int *)sc.seg_count.ptr,
                         (const float4 *)sc.ref_min[cur].ptr, (const float4 *)sc.ref_max[cur].ptr, sc.segL.ptr,
-                        sc.segR.ptr);
+                        sc.segR.ptr, dg);
         PM_LAUNCH_CHECK();
         scan_exclusive2(sc.segL.ptr, sc.segLoff.ptr, sc.segR.ptr, sc.segRoff.ptr, graph ? nS : h.S, sc, s, dS);
         child_size_kernel<<<g_nodes(h.A), BS, 0, s>>>(L, LS, sc.node_segbase.ptr, sc.segLoff.ptr, sc.segRoff.ptr,
@@ -1190,7 +1279,7 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
                        (const int *)sc.segRoff.ptr, (const int *)sc.segLLoff.ptr, (const int *)sc.segLRoff.ptr,
                        (const int *)sc.slot_leaf.ptr, (const float4 *)sc.ref_min[cur].ptr,
                        (const float4 *)sc.ref_max[cur].ptr, sc.ref_min[nxt].ptr, sc.ref_max[nxt].ptr,
-                       sc.prim_min.ptr, sc.prim_max.ptr, sc.keys[nxt].ptr);
+                       sc.prim_min.ptr, sc.prim_max.ptr, sc.keys[nxt].ptr, dg);
         PM_LAUNCH_CHECK();
         plan_advance_kernel<<<1, 64, 0, s>>>(L);
         PM_LAUNCH_CHECK();
@@ -1224,18 +1313,23 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
         PM_LAUNCH_CHECK();
     }
     ctx->num_prim_indices = (int)nref;
-    const bool sah = prm.lower_algorithm != PM_LOWER_BITMAP;
-    if (sah && !graph) sc.areas.reserve((size_t)h.final_refs + 1, s);
-    build_leaf_view(ctx, sah ? sc.areas.ptr : nullptr, false, s, graph ? &L->final_refs : nullptr);
+    const bool sah = prm.lower_algorithm != PM_LOWER_BITMAP;   // SAH or balanced: lower levels
+    const bool need_areas = prm.lower_algorithm == PM_LOWER_SAH;
+    if (need_areas && !graph) sc.areas.reserve((size_t)h.final_refs + 1, s);
+    build_leaf_view(ctx, need_areas ? sc.areas.ptr : nullptr, false, s, graph ? &L->final_refs : nullptr);
 
     // lower stage
-    if (!sah) {
+    const bool balanced = prm.lower_algorithm == PM_LOWER_BALANCED;
+    if (!sah || balanced) {   // PreprocessLeafNodes: counts -> masks
         bitmap_leaves_kernel<<<grid_for(nlf, BS, graph ? 4096 : INT_MAX), BS, 0, s>>>(NP, ctx->leaf_ids.ptr, L);
         PM_LAUNCH_CHECK();
-    } else {
-        lower_prep_kernel<<<grid_for(nlf, BS, graph ? 4096 : INT_MAX), BS, 0, s>>>(NP, ctx->leaf_ids.ptr, L,
-                                                                                   sc.areas.ptr);
-        PM_LAUNCH_CHECK();
+    }
+    if (sah) {
+        if (!balanced) {
+            lower_prep_kernel<<<grid_for(nlf, BS, graph ? 4096 : INT_MAX), BS, 0, s>>>(NP, ctx->leaf_ids.ptr, L,
+                                                                                       sc.areas.ptr);
+            PM_LAUNCH_CHECK();
+        }
         plan_lower_start_kernel<<<1, 64, 0, s>>>(L);
         PM_LAUNCH_CHECK();
         long long An = h.nleaf;
@@ -1247,9 +1341,15 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
                 seen.lower = std::max(seen.lower, An);
             }
             NP = node_ptrs(ctx);
-            sah_kernel<<<grid_for(nAn, WPB, graph ? 2048 : INT_MAX), WPB * 64, 0, s>>>(
-                NP, ctx->leaf_ids.ptr, L, sc.prim_min.ptr, sc.prim_max.ptr, sc.areas.ptr, prm.traversal_cost,
-                prm.sah_leaf_max, prm.sah_empty_bonus, sc.lower_flag.ptr, sc.lower_sets.ptr, sc.lower_careas.ptr);
+            if (balanced)
+                balanced_kernel<<<grid_for(nAn, WPB, graph ? 2048 : INT_MAX), WPB * 64, 0, s>>>(
+                    NP, ctx->leaf_ids.ptr, L, sc.prim_min.ptr, sc.prim_max.ptr, prm.traversal_cost, LS,
+                    sc.lower_flag.ptr, sc.lower_sets.ptr, sc.lower_careas.ptr);
+            else
+                sah_kernel<<<grid_for(nAn, WPB, graph ? 2048 : INT_MAX), WPB * 64, 0, s>>>(
+                    NP, ctx->leaf_ids.ptr, L, sc.prim_min.ptr, sc.prim_max.ptr, sc.areas.ptr, prm.traversal_cost,
+                    prm.sah_leaf_max, prm.sah_empty_bonus, sc.lower_flag.ptr, sc.lower_sets.ptr,
+                    sc.lower_careas.ptr);
             PM_LAUNCH_CHECK();
             scan_exclusive(sc.lower_flag.ptr, sc.lower_off.ptr, (int)nAn, sc, s, graph ? &L->An : nullptr);
             plan_lower_kernel<<<1, 64, 0, s>>>(L, sc.lower_off.ptr);

@@ -92,6 +92,73 @@ __device__ __forceinline__ float tri_area(f3 v0, f3 v1, f3 v2) {
     return 0.5f * sqrtf(dot3(c, c));
 }
 
+// Triangle/box step 3 of Akenine-Moller (IntersectionTests.h:20-160,378-395):
+// the nine edge x axis separating axes, each interval widened by GAMMA and
+// EPSILON.  The a0X/a1X/a2X groups are one 2-D test on the (y,z), (z,x) and
+// (x,y) components.
+__device__ __forceinline__ bool sat_edge(float au, float aw, float bu, float bw, float cu, float cw,
+                                         float hu, float hw) {
+    const float p0 = aw * bu - au * bw;
+    const float p1 = (bu - au) * cw - (bw - aw) * cu;
+    float r = hw * fabsf(bu - au) + hu * fabsf(bw - aw) + 0.00001f;
+    r *= 1.0001f;
+    return !((p0 > r && p1 > r) || (p0 < -r && p1 < -r));
+}
+
+__device__ __forceinline__ bool sat_plane(float u0, float w0, float u1, float w1, float u2, float w2,
+                                          float hu, float hw) {
+    bool ret = true;
+    ret &= sat_edge(u0, w0, u1, w1, u2, w2, hu, hw);
+    ret &= sat_edge(u1, w1, u2, w2, u0, w0, hu, hw);
+    ret &= sat_edge(u2, w2, u0, w0, u1, w1, hu, hw);
+    return ret;
+}
+
+__device__ __forceinline__ bool tri_box_step3(f3 v0, f3 v1, f3 v2, f3 bmin, f3 bmax) {
+    const f3 h = {(bmax.x - bmin.x) * 0.5f, (bmax.y - bmin.y) * 0.5f, (bmax.z - bmin.z) * 0.5f};
+    const f3 c = {bmin.x + h.x, bmin.y + h.y, bmin.z + h.z};
+    v0 = sub3(v0, c); v1 = sub3(v1, c); v2 = sub3(v2, c);
+    bool ret = true;
+    ret &= sat_plane(v0.y, v0.z, v1.y, v1.z, v2.y, v2.z, h.y, h.z);
+    ret &= sat_plane(v0.z, v0.x, v1.z, v1.x, v2.z, v2.x, h.z, h.x);
+    ret &= sat_plane(v0.x, v0.y, v1.x, v1.y, v2.x, v2.y, h.x, h.y);
+    return ret;
+}
+
+// WoopTransformationMatrix (IntersectionTests.h:437-467)
+__device__ __forceinline__ void woop_matrix(f3 a, f3 b, f3 c, float4 &m0, float4 &m1, float4 &m2) {
+    const f3 i0 = sub3(a, c), i1 = sub3(b, c), i2 = sub3(cross3(i0, i1), c);
+    const float det = i0.x * (i1.y * i2.z - i1.z * i2.y) + i1.x * (i2.y * i0.z - i2.z * i0.y) +
+                      i2.x * (i0.y * i1.z - i0.z * i1.y);
+    const float inv = 1.0f / det;
+    const f3 t0 = {(i1.y * i2.z - i1.z * i2.y) * inv, (i2.x * i1.z - i2.z * i1.x) * inv,
+                   (i1.x * i2.y - i1.y * i2.x) * inv};
+    const f3 t1 = {(i2.y * i0.z - i2.z * i0.y) * inv, (i0.x * i2.z - i0.z * i2.x) * inv,
+                   (i2.x * i0.y - i2.y * i0.x) * inv};
+    const f3 t2 = {(i0.y * i1.z - i0.z * i1.y) * inv, (i1.x * i0.z - i1.z * i0.x) * inv,
+                   (i0.x * i1.y - i0.y * i1.x) * inv};
+    m0 = make_float4(t0.x, t0.y, t0.z, dot3(t0, c));
+    m1 = make_float4(t1.x, t1.y, t1.z, dot3(t1, c));
+    m2 = make_float4(t2.x, t2.y, t2.z, dot3(t2, c));
+}
+
+// IRayTracer::Woop with WoopLambda / WoopUV (IRayTracer.h:78-109)
+__device__ __forceinline__ bool woop_test(const float4 &m0, const float4 &m1, const float4 &m2, f3 o, f3 d,
+                                          float &th, float &tu, float &tv) {
+    const f3 n = mk3(m2);
+    const float lambda = -(dot3(n, o) - m2.w) / dot3(n, d);
+    if (0.0f <= lambda && lambda <= th) {
+        const f3 a = mk3(m0), b = mk3(m1);
+        const float u = lambda * dot3(a, d) + dot3(a, o) - m0.w;
+        const float v = lambda * dot3(b, d) + dot3(b, o) - m1.w;
+        if (u >= -0.0f && v >= -0.0f && u + v <= 1.0f) {
+            th = lambda; tu = u; tv = v;
+            return true;
+        }
+    }
+    return false;
+}
+
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
 }  // namespace pmkd

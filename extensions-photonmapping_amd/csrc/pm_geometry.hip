@@ -81,11 +81,65 @@ __global__ __launch_bounds__(TB) void radial_wave_kernel(const float4 *__restric
     out[id] = make_float4(u.x * r, u.y * r, u.z * r, u.w);
 }
 
+// AddMeshKernel's normals and colours (GeometryList.cu:215-230):
+// n_k = (normalMat * normal.xyz, 0), normalMat rows = model rows' xyz
+struct Mat3 { float4 r[3]; };
+
+__global__ __launch_bounds__(TB) void add_attributes_kernel(const float4 *__restrict__ normals,
+                                                            const uchar4 *__restrict__ colors, int nv,
+                                                            const unsigned *__restrict__ idx, int ntris,
+                                                            Mat3 m, float4 *__restrict__ n0,
+                                                            float4 *__restrict__ n1, float4 *__restrict__ n2,
+                                                            uchar4 *__restrict__ c0, uchar4 *__restrict__ c1,
+                                                            uchar4 *__restrict__ c2, int *bad) {
+    const int id = blockIdx.x * TB + threadIdx.x;
+    if (id >= ntris) return;
+    const unsigned i[3] = {idx[3 * id], idx[3 * id + 1], idx[3 * id + 2]};
+    if (i[0] >= (unsigned)nv || i[1] >= (unsigned)nv || i[2] >= (unsigned)nv) {
+        atomicOr(bad, 1);
+        return;
+    }
+    float4 *nout[3] = {n0, n1, n2};
+    uchar4 *cout[3] = {c0, c1, c2};
+    for (int k = 0; k < 3; ++k) {
+        const float4 v = normals[i[k]];
+        const f3 n = {v.x, v.y, v.z};
+        nout[k][id] = make_float4(dot3(mk3(m.r[0]), n), dot3(mk3(m.r[1]), n), dot3(mk3(m.r[2]), n), 0.0f);
+        if (colors) cout[k][id] = colors[i[k]];
+    }
+}
+
 __global__ void clear_flag_kernel(int *flag) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *flag = 0;
 }
 
 }  // namespace
+
+void add_mesh_attributes(pm_ctx *ctx, const float4 *normals, const uchar4 *colors, int nv, const unsigned *idx,
+                         int nidx, const float *model, float4 *n0, float4 *n1, float4 *n2, uchar4 *c0,
+                         uchar4 *c1, uchar4 *c2, hipStream_t s) {
+    Mat3 m;
+    for (int i = 0; i < 3; ++i) {
+        if (model)
+            m.r[i] = make_float4(model[4 * i], model[4 * i + 1], model[4 * i + 2], 0.0f);
+        else
+            m.r[i] = make_float4(i == 0, i == 1, i == 2, 0.0f);
+    }
+    const int ntris = nidx / 3;
+    BuildScratch &sc = ctx->scratch;
+    sc.totals.reserve(8, s);
+    sc.host.ensure(8);
+    clear_flag_kernel<<<1, 64, 0, s>>>(sc.totals.ptr);
+    PM_LAUNCH_CHECK();
+    if (ntris > 0) {
+        add_attributes_kernel<<<(ntris + TB - 1) / TB, TB, 0, s>>>(normals, colors, nv, idx, ntris, m, n0, n1, n2,
+                                                                   c0, c1, c2, sc.totals.ptr);
+        PM_LAUNCH_CHECK();
+    }
+    PM_HIP(hipMemcpyAsync(sc.host.ptr, sc.totals.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+    PM_HIP(hipStreamSynchronize(s));
+    if (sc.host.ptr[0]) throw Error{PM_ERR_ARG, "mesh index out of range of the vertex array"};
+}
 
 void add_mesh(pm_ctx *ctx, const float4 *verts, int nv, const unsigned *idx, int nidx,
               const float *model, float4 *p0, float4 *p1, float4 *p2, hipStream_t s) {

@@ -99,6 +99,7 @@ struct TreeDev {
     const int *prim_indices;
     const int4 *packed;        // one 16-byte record per node (trace view)
     const float4 *leaf_tris;   // 3 float4 per leaf slot: v0 (w = id bits), e1, e2
+    const float4 *leaf_woop;   // Woop intersection: 3 float4 per leaf slot (m0, m1, m2)
     int num_nodes;
     int num_roots;             // roots are nodes 0..num_roots-1 (one per mesh)
     const int *fail;           // device flag: nonzero = the tree is not valid, every ray misses
@@ -172,6 +173,7 @@ struct pm_ctx {
     pmkd::DevBuf<int> leaf_ids;
     pmkd::DevBuf<int4> packed;      // trace view: packed nodes
     pmkd::DevBuf<float4> leaf_tris; // trace view: leaf-ordered triangles
+    pmkd::DevBuf<float4> woop;      // Woop matrices (per leaf slot or per triangle), per call
     pmkd::BuildScratch scratch;
     std::vector<long long> level_refs;
     int num_roots = 1;                 // meshes in the last build (roots 0..num_roots-1)
@@ -216,6 +218,9 @@ void build_trace_view(pm_ctx *ctx, hipStream_t s);
 // count is read on the device and ctx's count is its upper bound.
 void build_node_view(pm_ctx *ctx, hipStream_t s, const int *n_dev = nullptr);
 void build_leaf_view(pm_ctx *ctx, float *areas, bool check, hipStream_t s, const int *m_dev = nullptr);
+void trace_canvas(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int nrays, const float4 *n0,
+                  const float4 *n1, const float4 *n2, const uchar4 *c0, const pm_light &light, int max_bounces,
+                  uchar4 *canvas, const pm_trace_params &prm, hipStream_t s);
 void trace_rays(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int nrays,
                 const int *ray_mesh, pm_hit *out, const pm_trace_params &prm, int4 *counters,
                 hipStream_t s);
@@ -236,6 +241,9 @@ void autotune(pm_ctx *ctx, const float4 *p0, const float4 *p1, const float4 *p2,
 void osc_surface(float4 *vert, float4 *norm, int d, float height, float time, float amp, float freq,
                  hipStream_t s);
 void radial_wave(const float4 *unit, int nv, float frame, const float *wave, float4 *out, hipStream_t s);
+void add_mesh_attributes(pm_ctx *ctx, const float4 *normals, const uchar4 *colors, int nv, const unsigned *idx,
+                         int nidx, const float *model, float4 *n0, float4 *n1, float4 *n2, uchar4 *c0,
+                         uchar4 *c1, uchar4 *c2, hipStream_t s);
 void add_mesh(pm_ctx *ctx, const float4 *verts, int nv, const unsigned *idx, int nidx,
               const float *model, float4 *p0, float4 *p1, float4 *p2, hipStream_t s);
 // device-wide exclusive scan of n ints; out has n+1 entries (out[n] = total)

@@ -55,7 +55,7 @@ __device__ __forceinline__ bool mt_edges(f3 v0, f3 e1, f3 e2, f3 o, f3 d, float 
 // Leaf test (Raytracer.cu:194-216): optional box test, then Moller-Trumbore
 // over the leaf bitmap in ascending bit order; the next triangle's data is
 // loaded before the current one is tested.
-template <bool SKIP, bool COUNT>
+template <bool SKIP, bool COUNT, bool WOOP>
 __device__ __forceinline__ int test_leaf(const TreeDev &T, const int4 nd, int node, f3 o, f3 d,
                                          f3 inv, float &th, float &tu, float &tv, int4 &cnt) {
     if (COUNT) cnt.y++;
@@ -65,6 +65,16 @@ __device__ __forceinline__ int test_leaf(const TreeDev &T, const int4 nd, int no
     }
     unsigned long long bm = ((unsigned long long)(unsigned)nd.z << 32) | (unsigned)nd.y;
     if (!bm) return -1;
+    if (WOOP) {   // the slot's Woop matrices; the id comes from the prim indices
+        const float4 *W = T.leaf_woop + 3ll * nd.x;
+        int slot = -1;
+        for (; bm; bm &= bm - 1) {
+            const int i = __builtin_ctzll(bm);
+            if (COUNT) cnt.z++;
+            if (woop_test(W[3 * i], W[3 * i + 1], W[3 * i + 2], o, d, th, tu, tv)) slot = i;
+        }
+        return slot < 0 ? -1 : T.prim_indices[nd.x + slot];
+    }
     const float4 *L = T.leaf_tris + 3ll * nd.x;
     int i = __builtin_ctzll(bm);
     float4 a = L[3 * i], b = L[3 * i + 1], c = L[3 * i + 2];
@@ -85,26 +95,12 @@ __device__ __forceinline__ int test_leaf(const TreeDev &T, const int4 nd, int no
     return hit;
 }
 
-template <bool SKIP, bool COUNT>
-__global__ __launch_bounds__(TB) void kd_restart_kernel(TreeDev T, const float4 *__restrict__ org,
-                                                        const float4 *__restrict__ dir,
-                                                        const int *__restrict__ ray_root, int n,
-                                                        pm_hit *__restrict__ out,
-                                                        int4 *__restrict__ counters) {
-    const int r = blockIdx.x * TB + threadIdx.x;
-    if (r >= n) return;
-    int4 cnt = make_int4(0, 0, 0, 0);
+// kd-restart for one ray from node root_id (Raytracer.cu:57-106,165-262):
+// descend to the leaf the ray is in at t = th, test it, restart behind it
+template <bool SKIP, bool COUNT, bool WOOP>
+__device__ __forceinline__ pm_hit kd_restart_ray(const TreeDev &T, int root_id, f3 o, f3 d, f3 inv,
+                                                 int4 &cnt) {
     pm_hit res = miss_record();
-    const int root_id = ray_root ? ray_root[r] : 0;
-    // no such mesh, or (captured batch) the build before this trace failed: a miss
-    if (root_id < 0 || root_id >= T.num_roots || (T.fail && *T.fail)) {
-        out[r] = res;
-        if (COUNT) counters[r] = cnt;
-        return;
-    }
-    const f3 o = mk3(org[r]);
-    const f3 d = adjust_dir(mk3(dir[r]));
-    const f3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
     float th = 0.0f, tu = 0.0f, tv = 0.0f;
     const int4 root = T.packed[root_id];
     long long visits = 0;
@@ -131,39 +127,24 @@ __global__ __launch_bounds__(TB) void kd_restart_kernel(TreeDev T, const float4 
         }
         if (bad) break;
         th = tNext * 1.00001f;
-        const int hit = test_leaf<SKIP, COUNT>(T, nd, node, o, d, inv, th, tu, tv, cnt);
+        const int hit = test_leaf<SKIP, COUNT, WOOP>(T, nd, node, o, d, inv, th, tu, tv, cnt);
         if (hit != -1) {
             res.id = hit; res.t = th; res.u = tu; res.v = tv;
             break;
         }
     } while (th < F_INF);
-    out[r] = res;
-    if (COUNT) counters[r] = cnt;
+    return res;
 }
 
-template <int SMAX, bool SKIP, bool COUNT>
-__global__ __launch_bounds__(TB) void short_stack_kernel(TreeDev T, const float4 *__restrict__ org,
-                                                         const float4 *__restrict__ dir,
-                                                         const int *__restrict__ ray_root, int n,
-                                                         int S, pm_hit *__restrict__ out,
-                                                         int4 *__restrict__ counters) {
-    __shared__ int stk_node[SMAX][TB];
-    __shared__ float stk_t[SMAX][TB];
-    const int r = blockIdx.x * TB + threadIdx.x;
-    if (r >= n) return;
+// short stack for one ray from node root_id (ShortStack.cu:58-194,
+// ShortStack.h:43-98): a ring of the S most recent far children in LDS,
+// restart from the root when it runs empty
+template <int SMAX, bool SKIP, bool COUNT, bool WOOP>
+__device__ __forceinline__ pm_hit short_stack_ray(const TreeDev &T, int root_id, f3 o, f3 d, f3 inv, int S,
+                                                  int (&stk_node)[SMAX][TB], float (&stk_t)[SMAX][TB],
+                                                  int4 &cnt) {
     const int lt = threadIdx.x;
-    int4 cnt = make_int4(0, 0, 0, 0);
     pm_hit res = miss_record();
-    const int root_id = ray_root ? ray_root[r] : 0;
-    // no such mesh, or (captured batch) the build before this trace failed: a miss
-    if (root_id < 0 || root_id >= T.num_roots || (T.fail && *T.fail)) {
-        out[r] = res;
-        if (COUNT) counters[r] = cnt;
-        return;
-    }
-    const f3 o = mk3(org[r]);
-    const f3 d = adjust_dir(mk3(dir[r]));
-    const f3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
     float th = 0.0f, tu = 0.0f, tv = 0.0f;
     int nxt = 0, count = 0;
     const int4 root = T.packed[root_id];
@@ -208,19 +189,70 @@ __global__ __launch_bounds__(TB) void short_stack_kernel(TreeDev T, const float4
         }
         if (bad) break;
         th = tNext * 1.00001f;
-        const int hit = test_leaf<SKIP, COUNT>(T, nd, node, o, d, inv, th, tu, tv, cnt);
+        const int hit = test_leaf<SKIP, COUNT, WOOP>(T, nd, node, o, d, inv, th, tu, tv, cnt);
         if (hit != -1) {
             res.id = hit; res.t = th; res.u = tu; res.v = tv;
             break;
         }
     } while (th < F_INF);
+    return res;
+}
+
+// the first-hit kernels: one lane per ray
+__device__ __forceinline__ bool ray_root_valid(const TreeDev &T, const int *ray_root, int r, int &root_id) {
+    root_id = ray_root ? ray_root[r] : 0;
+    // no such mesh, or (captured batch) the build before this trace failed: a miss
+    return !(root_id < 0 || root_id >= T.num_roots || (T.fail && *T.fail));
+}
+
+template <bool SKIP, bool COUNT, bool WOOP>
+__global__ __launch_bounds__(TB) void kd_restart_kernel(TreeDev T, const float4 *__restrict__ org,
+                                                        const float4 *__restrict__ dir,
+                                                        const int *__restrict__ ray_root, int n,
+                                                        pm_hit *__restrict__ out,
+                                                        int4 *__restrict__ counters) {
+    const int r = blockIdx.x * TB + threadIdx.x;
+    if (r >= n) return;
+    int4 cnt = make_int4(0, 0, 0, 0);
+    int root_id;
+    pm_hit res = miss_record();
+    if (ray_root_valid(T, ray_root, r, root_id)) {
+        const f3 o = mk3(org[r]);
+        const f3 d = adjust_dir(mk3(dir[r]));
+        const f3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+        res = kd_restart_ray<SKIP, COUNT, WOOP>(T, root_id, o, d, inv, cnt);
+    }
+    out[r] = res;
+    if (COUNT) counters[r] = cnt;
+}
+
+template <int SMAX, bool SKIP, bool COUNT, bool WOOP>
+__global__ __launch_bounds__(TB) void short_stack_kernel(TreeDev T, const float4 *__restrict__ org,
+                                                         const float4 *__restrict__ dir,
+                                                         const int *__restrict__ ray_root, int n,
+                                                         int S, pm_hit *__restrict__ out,
+                                                         int4 *__restrict__ counters) {
+    __shared__ int stk_node[SMAX][TB];
+    __shared__ float stk_t[SMAX][TB];
+    const int r = blockIdx.x * TB + threadIdx.x;
+    if (r >= n) return;
+    int4 cnt = make_int4(0, 0, 0, 0);
+    int root_id;
+    pm_hit res = miss_record();
+    if (ray_root_valid(T, ray_root, r, root_id)) {
+        const f3 o = mk3(org[r]);
+        const f3 d = adjust_dir(mk3(dir[r]));
+        const f3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+        res = short_stack_ray<SMAX, SKIP, COUNT, WOOP>(T, root_id, o, d, inv, S, stk_node, stk_t, cnt);
+    }
     out[r] = res;
     if (COUNT) counters[r] = cnt;
 }
 
 // Brute force: every lane tests every triangle in index order; triangles are
 // staged through LDS one tile at a time (one tile = TB triangles, 12 KiB).
-template <bool COUNT>
+// WOOP: P0/P1/P2 are the triangles' Woop matrices m0/m1/m2 instead
+template <bool COUNT, bool WOOP>
 __global__ __launch_bounds__(TB) void brute_kernel(const float4 *__restrict__ P0,
                                                    const float4 *__restrict__ P1,
                                                    const float4 *__restrict__ P2, int ntris,
@@ -243,7 +275,9 @@ __global__ __launch_bounds__(TB) void brute_kernel(const float4 *__restrict__ P0
         const int m = min(TB, ntris - base);
         if (live) {
             for (int j = 0; j < m; ++j) {
-                if (moller_trumbore(mk3(s0[j]), mk3(s1[j]), mk3(s2[j]), o, d, th, tu, tv)) hit = base + j;
+                const bool h = WOOP ? woop_test(s0[j], s1[j], s2[j], o, d, th, tu, tv)
+                                    : moller_trumbore(mk3(s0[j]), mk3(s1[j]), mk3(s2[j]), o, d, th, tu, tv);
+                if (h) hit = base + j;
             }
         }
     }
@@ -314,27 +348,179 @@ __global__ __launch_bounds__(TB) void leaf_tris_kernel(const int *__restrict__ p
         leaf_slot(prim_indices, k, P0, P1, P2, ntris, out, bad, areas);
 }
 
+// ---- canvas shading: ColorKernels.h:15-85 in the tracers' shading loops
+// (Raytracer.cu:165-262, ShortStack.cu:96-194, BruteTracer.cu:54-96); the
+// specular power is the correctly rounded float of the double power
+struct Light { f3 pos, ambient, diffuse, specular; };
+
+__device__ __forceinline__ f3 scale3(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 add3(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 mul3(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ f3 normalize3(f3 v) { return scale3(v, 1.0f / sqrtf(dot3(v, v))); }
+__device__ __forceinline__ float saturate(float x) { return x > 1.0f ? 1.0f : (x > 0.0f ? x : 0.0f); }
+__device__ __forceinline__ float clamp01(float x) { return fmaxf(0.0f, fminf(x, 1.0f)); }
+
+// PhongLighting, ColorKernels.h:15-45, shadow = 1 (the reference's shadow
+// rays are disabled, Raytracer.cu:219-226)
+__device__ __forceinline__ float4 phong(const float4 c, f3 normal, f3 point, f3 origin, const Light &L) {
+    const f3 lightDir = normalize3(sub3(L.pos, point));
+    const float diffuse = saturate(dot3(lightDir, normal) * 1.0f);
+    f3 refl = sub3(scale3(normal, 2.0f * dot3(normal, lightDir)), lightDir);
+    refl = normalize3(refl);
+    const float stemp = saturate(dot3(normalize3(sub3(origin, point)), refl));
+    const float specProp = 1.0f - c.w;
+    const float specular = specProp * (float)pow((double)stemp, (double)(128.0f * specProp));
+    const f3 base = {c.x, c.y, c.z};
+    const f3 light = add3(mul3(base, add3(L.ambient, scale3(L.diffuse, diffuse))),
+                          scale3(scale3(scale3(L.specular, specular), 10.0f), 1.0f));
+    const float alpha = c.w < specular ? specular : c.w;
+    return make_float4(clamp01(light.x), clamp01(light.y), clamp01(light.z), alpha);
+}
+
+// Lighting, ColorKernels.h:47-77: shade the hit, turn the ray into its
+// mirror reflection from the hit point
+__device__ __forceinline__ float4 lighting(const pm_hit &h, f3 &origin, f3 &dir, const float4 *__restrict__ n0,
+                                           const float4 *__restrict__ n1, const float4 *__restrict__ n2,
+                                           const uchar4 *__restrict__ c0, const Light &L) {
+    const f3 point = add3(origin, scale3(dir, h.t));
+    const float w = 1.0f - h.u - h.v;
+    f3 normal = add3(add3(scale3(mk3(n0[h.id]), w), scale3(mk3(n1[h.id]), h.u)), scale3(mk3(n2[h.id]), h.v));
+    normal = normalize3(normal);
+    const uchar4 cc = c0[h.id];
+    const float4 col = make_float4(cc.x / 255.0f, cc.y / 255.0f, cc.z / 255.0f, cc.w / 255.0f);
+    dir = add3(scale3(normal, 2.0f * dot3(normal, scale3(dir, -1.0f))), dir);
+    dir = normalize3(dir);
+    const float4 out = phong(col, normal, point, origin, L);
+    origin = add3(point, scale3(dir, 0.00001f));
+    return out;
+}
+
+// BlendColor, ColorKernels.h:79-84
+__device__ __forceinline__ float4 blend(const float4 o, float4 n) {
+    const float a = n.w;
+    n.w = 1.0f;
+    const float k = (1.0f - o.w) * a;
+    return make_float4(o.x + k * n.x, o.y + k * n.y, o.z + k * n.z, o.w + k * n.w);
+}
+
+// brute-force first hit for one ray (Exhaustive, BruteTracer.cu:54-96),
+// straight from global memory
+template <bool WOOP>
+__device__ __forceinline__ pm_hit brute_ray(const float4 *__restrict__ A, const float4 *__restrict__ B,
+                                            const float4 *__restrict__ C, int ntris, f3 o, f3 d) {
+    float th = F_INF, tu = 0.0f, tv = 0.0f;
+    int hit = -1;
+    for (int k = 0; k < ntris; ++k) {
+        const bool h = WOOP ? woop_test(A[k], B[k], C[k], o, d, th, tu, tv)
+                            : moller_trumbore(mk3(A[k]), mk3(B[k]), mk3(C[k]), o, d, th, tu, tv);
+        if (h) hit = k;
+    }
+    pm_hit res = miss_record();
+    if (th < F_INF && hit != -1) { res.id = hit; res.t = th; res.u = tu; res.v = tv; }
+    return res;
+}
+
+// IRayTracer::Trace(canvas): per ray, first hit -> Lighting -> BlendColor
+// until a miss, alpha >= 0.97 or max_bounces; one uchar4 per ray
+template <int ALGO, bool SKIP, bool WOOP>
+__global__ __launch_bounds__(TB) void canvas_kernel(TreeDev T, const float4 *__restrict__ A,
+                                                    const float4 *__restrict__ B,
+                                                    const float4 *__restrict__ C, int ntris,
+                                                    const float4 *__restrict__ org,
+                                                    const float4 *__restrict__ dir, int n, int S,
+                                                    const float4 *__restrict__ n0,
+                                                    const float4 *__restrict__ n1,
+                                                    const float4 *__restrict__ n2,
+                                                    const uchar4 *__restrict__ c0, Light L, int max_bounces,
+                                                    uchar4 *__restrict__ canvas) {
+    __shared__ int stk_node[ALGO == PM_TRACE_SHORT_STACK ? 16 : 1][TB];
+    __shared__ float stk_t[ALGO == PM_TRACE_SHORT_STACK ? 16 : 1][TB];
+    const int r = blockIdx.x * TB + threadIdx.x;
+    if (r >= n) return;
+    f3 o = mk3(org[r]), d = mk3(dir[r]);
+    if (ALGO != PM_TRACE_BRUTE) d = adjust_dir(d);
+    float4 col = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    int4 cnt = make_int4(0, 0, 0, 0);
+    for (int b = 0; b < max_bounces; ++b) {
+        pm_hit h;
+        if constexpr (ALGO == PM_TRACE_BRUTE) {
+            h = brute_ray<WOOP>(A, B, C, ntris, o, d);
+        } else if constexpr (ALGO == PM_TRACE_KD_RESTART) {
+            const f3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+            h = kd_restart_ray<SKIP, false, WOOP>(T, 0, o, d, inv, cnt);
+        } else {
+            const f3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+            h = short_stack_ray<16, SKIP, false, WOOP>(T, 0, o, d, inv, S, stk_node, stk_t, cnt);
+        }
+        if (h.id < 0) break;
+        col = blend(col, lighting(h, o, d, n0, n1, n2, c0, L));
+        if (ALGO != PM_TRACE_BRUTE) d = adjust_dir(d);
+        if (!(col.w < 0.97f)) break;
+    }
+    canvas[r] = make_uchar4((unsigned char)(col.x * 255.0f), (unsigned char)(col.y * 255.0f),
+                            (unsigned char)(col.z * 255.0f), (unsigned char)(col.w * 255.0f));
+}
+
+// CreateWoopValues (GeometryList.cu:80-101): matrices of (p1, p2, p0) for
+// every triangle (W0/W1/W2 arrays) or for every leaf slot (3 per slot)
+__global__ __launch_bounds__(TB) void woop_tris_kernel(const float4 *__restrict__ P0,
+                                                       const float4 *__restrict__ P1,
+                                                       const float4 *__restrict__ P2, int n,
+                                                       float4 *__restrict__ W0, float4 *__restrict__ W1,
+                                                       float4 *__restrict__ W2) {
+    const int k = blockIdx.x * TB + threadIdx.x;
+    if (k >= n) return;
+    float4 m0, m1, m2;
+    woop_matrix(mk3(P1[k]), mk3(P2[k]), mk3(P0[k]), m0, m1, m2);
+    W0[k] = m0; W1[k] = m1; W2[k] = m2;
+}
+
+__global__ __launch_bounds__(TB) void woop_slots_kernel(const int *__restrict__ prim_indices, int m,
+                                                        const float4 *__restrict__ P0,
+                                                        const float4 *__restrict__ P1,
+                                                        const float4 *__restrict__ P2, int ntris,
+                                                        float4 *__restrict__ W) {
+    const int k = blockIdx.x * TB + threadIdx.x;
+    if (k >= m) return;
+    const int t = prim_indices[k];
+    float4 m0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), m1 = m0, m2 = m0;
+    if (t >= 0 && t < ntris) woop_matrix(mk3(P1[t]), mk3(P2[t]), mk3(P0[t]), m0, m1, m2);
+    W[3ll * k] = m0; W[3ll * k + 1] = m1; W[3ll * k + 2] = m2;
+}
+
 __global__ void clear_int_kernel(int *p) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
 }
 
-template <int SMAX, bool SKIP, bool COUNT>
+template <int SMAX, bool SKIP, bool COUNT, bool WOOP>
 void launch_ss(const TreeDev &T, const float4 *org, const float4 *dir, const int *root, int n, int S,
                pm_hit *out, int4 *cnt, hipStream_t s) {
     const int blocks = (n + TB - 1) / TB;
-    short_stack_kernel<SMAX, SKIP, COUNT><<<blocks, TB, 0, s>>>(T, org, dir, root, n, S, out, cnt);
+    short_stack_kernel<SMAX, SKIP, COUNT, WOOP><<<blocks, TB, 0, s>>>(T, org, dir, root, n, S, out, cnt);
 }
 
-template <bool SKIP, bool COUNT>
+template <bool SKIP, bool COUNT, bool WOOP>
 void dispatch(const TreeDev &T, int algo, const float4 *org, const float4 *dir, const int *root, int n,
               int S, pm_hit *out, int4 *cnt, hipStream_t s) {
     const int blocks = (n + TB - 1) / TB;
     if (algo == PM_TRACE_KD_RESTART) {
-        kd_restart_kernel<SKIP, COUNT><<<blocks, TB, 0, s>>>(T, org, dir, root, n, out, cnt);
+        kd_restart_kernel<SKIP, COUNT, WOOP><<<blocks, TB, 0, s>>>(T, org, dir, root, n, out, cnt);
     } else {
-        if (S <= 4) launch_ss<4, SKIP, COUNT>(T, org, dir, root, n, S, out, cnt, s);
-        else if (S <= 8) launch_ss<8, SKIP, COUNT>(T, org, dir, root, n, S, out, cnt, s);
-        else launch_ss<16, SKIP, COUNT>(T, org, dir, root, n, S, out, cnt, s);
+        if (S <= 4) launch_ss<4, SKIP, COUNT, WOOP>(T, org, dir, root, n, S, out, cnt, s);
+        else if (S <= 8) launch_ss<8, SKIP, COUNT, WOOP>(T, org, dir, root, n, S, out, cnt, s);
+        else launch_ss<16, SKIP, COUNT, WOOP>(T, org, dir, root, n, S, out, cnt, s);
+    }
+}
+
+template <bool WOOP>
+void dispatch_flags(const TreeDev &T, bool skip, bool count, int algo, const float4 *org, const float4 *dir,
+                    const int *root, int n, int S, pm_hit *out, int4 *cnt, hipStream_t s) {
+    if (skip) {
+        if (count) dispatch<true, true, WOOP>(T, algo, org, dir, root, n, S, out, cnt, s);
+        else dispatch<true, false, WOOP>(T, algo, org, dir, root, n, S, out, cnt, s);
+    } else {
+        if (count) dispatch<false, true, WOOP>(T, algo, org, dir, root, n, S, out, cnt, s);
+        else dispatch<false, false, WOOP>(T, algo, org, dir, root, n, S, out, cnt, s);
     }
 }
 
@@ -381,20 +567,9 @@ void build_trace_view(pm_ctx *ctx, hipStream_t s) {
     build_leaf_view(ctx, nullptr, true, s);
 }
 
-void trace_rays(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int nrays,
-                const int *ray_mesh, pm_hit *out, const pm_trace_params &prm, int4 *counters,
-                hipStream_t s) {
-    if (nrays == 0) return;
-    if (algo == PM_TRACE_BRUTE) {
-        const int tris = prm.brute_cap > 0 ? std::min(ctx->ntris, prm.brute_cap) : ctx->ntris;
-        const int blocks = (nrays + TB - 1) / TB;
-        if (counters)
-            brute_kernel<true><<<blocks, TB, 0, s>>>(ctx->p0, ctx->p1, ctx->p2, tris, org, dir, nrays, out, counters);
-        else
-            brute_kernel<false><<<blocks, TB, 0, s>>>(ctx->p0, ctx->p1, ctx->p2, tris, org, dir, nrays, out, counters);
-        PM_LAUNCH_CHECK();
-        return;
-    }
+namespace {
+
+TreeDev tree_dev(pm_ctx *ctx) {
     TreeDev T;
     T.info = ctx->nodes.info.ptr;
     T.split = ctx->nodes.split.ptr;
@@ -406,17 +581,114 @@ void trace_rays(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int
     T.prim_indices = ctx->prim_indices.ptr;
     T.packed = ctx->packed.ptr;
     T.leaf_tris = ctx->leaf_tris.ptr;
+    T.leaf_woop = nullptr;
     T.num_nodes = ctx->num_nodes;
     T.num_roots = ctx->num_roots;
     T.fail = ctx->trace_fail;
+    return T;
+}
+
+// GetWoopValues, computed per call: per leaf slot for the tree tracers
+const float4 *woop_slots(pm_ctx *ctx, hipStream_t s) {
+    const int m = ctx->num_prim_indices;
+    ctx->woop.reserve(3 * (size_t)m + 3, s);
+    if (m > 0) {
+        woop_slots_kernel<<<(m + TB - 1) / TB, TB, 0, s>>>(ctx->prim_indices.ptr, m, ctx->p0, ctx->p1, ctx->p2,
+                                                           ctx->ntris, ctx->woop.ptr);
+        PM_LAUNCH_CHECK();
+    }
+    return ctx->woop.ptr;
+}
+
+// ... and per triangle (W0 | W1 | W2) for brute force
+float4 *woop_tris(pm_ctx *ctx, int tris, hipStream_t s) {
+    ctx->woop.reserve(3 * (size_t)tris + 3, s);
+    float4 *W = ctx->woop.ptr;
+    if (tris > 0) {
+        woop_tris_kernel<<<(tris + TB - 1) / TB, TB, 0, s>>>(ctx->p0, ctx->p1, ctx->p2, tris, W, W + tris,
+                                                             W + 2ll * tris);
+        PM_LAUNCH_CHECK();
+    }
+    return W;
+}
+
+template <int ALGO>
+void launch_canvas(bool skip, bool woop, int blocks, hipStream_t s, const TreeDev &T, const float4 *A,
+                   const float4 *B, const float4 *C, int ntris, const float4 *org, const float4 *dir, int n,
+                   int S, const float4 *n0, const float4 *n1, const float4 *n2, const uchar4 *c0,
+                   const Light &L, int maxb, uchar4 *canvas) {
+    if (skip && woop)
+        canvas_kernel<ALGO, true, true><<<blocks, TB, 0, s>>>(T, A, B, C, ntris, org, dir, n, S, n0, n1, n2, c0, L, maxb, canvas);
+    else if (skip)
+        canvas_kernel<ALGO, true, false><<<blocks, TB, 0, s>>>(T, A, B, C, ntris, org, dir, n, S, n0, n1, n2, c0, L, maxb, canvas);
+    else if (woop)
+        canvas_kernel<ALGO, false, true><<<blocks, TB, 0, s>>>(T, A, B, C, ntris, org, dir, n, S, n0, n1, n2, c0, L, maxb, canvas);
+    else
+        canvas_kernel<ALGO, false, false><<<blocks, TB, 0, s>>>(T, A, B, C, ntris, org, dir, n, S, n0, n1, n2, c0, L, maxb, canvas);
+}
+
+}  // namespace
+
+void trace_rays(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int nrays,
+                const int *ray_mesh, pm_hit *out, const pm_trace_params &prm, int4 *counters,
+                hipStream_t s) {
+    if (nrays == 0) return;
+    const bool woop = prm.intersection == PM_INTERSECT_WOOP;
+    if (algo == PM_TRACE_BRUTE) {
+        const int tris = prm.brute_cap > 0 ? std::min(ctx->ntris, prm.brute_cap) : ctx->ntris;
+        const int blocks = (nrays + TB - 1) / TB;
+        const float4 *A = ctx->p0, *B = ctx->p1, *C = ctx->p2;
+        if (woop) {
+            const float4 *W = woop_tris(ctx, tris, s);
+            A = W; B = W + tris; C = W + 2ll * tris;
+            if (counters) brute_kernel<true, true><<<blocks, TB, 0, s>>>(A, B, C, tris, org, dir, nrays, out, counters);
+            else brute_kernel<false, true><<<blocks, TB, 0, s>>>(A, B, C, tris, org, dir, nrays, out, counters);
+        } else {
+            if (counters) brute_kernel<true, false><<<blocks, TB, 0, s>>>(A, B, C, tris, org, dir, nrays, out, counters);
+            else brute_kernel<false, false><<<blocks, TB, 0, s>>>(A, B, C, tris, org, dir, nrays, out, counters);
+        }
+        PM_LAUNCH_CHECK();
+        return;
+    }
+    TreeDev T = tree_dev(ctx);
+    if (woop) T.leaf_woop = woop_slots(ctx, s);
     const bool skip = prm.leaf_skipping && ctx->has_boxes;
     const int S = prm.stack_size;
-    if (skip) {
-        if (counters) dispatch<true, true>(T, algo, org, dir, ray_mesh, nrays, S, out, counters, s);
-        else dispatch<true, false>(T, algo, org, dir, ray_mesh, nrays, S, out, counters, s);
+    if (woop) dispatch_flags<true>(T, skip, counters != nullptr, algo, org, dir, ray_mesh, nrays, S, out, counters, s);
+    else dispatch_flags<false>(T, skip, counters != nullptr, algo, org, dir, ray_mesh, nrays, S, out, counters, s);
+    PM_LAUNCH_CHECK();
+}
+
+void trace_canvas(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int nrays, const float4 *n0,
+                  const float4 *n1, const float4 *n2, const uchar4 *c0, const pm_light &light, int max_bounces,
+                  uchar4 *canvas, const pm_trace_params &prm, hipStream_t s) {
+    if (nrays == 0) return;
+    const bool woop = prm.intersection == PM_INTERSECT_WOOP;
+    const Light L = {{light.position[0], light.position[1], light.position[2]},
+                     {light.ambient[0], light.ambient[1], light.ambient[2]},
+                     {light.diffuse[0], light.diffuse[1], light.diffuse[2]},
+                     {light.specular[0], light.specular[1], light.specular[2]}};
+    const int blocks = (nrays + TB - 1) / TB;
+    TreeDev T = tree_dev(ctx);
+    const float4 *A = ctx->p0, *B = ctx->p1, *C = ctx->p2;
+    int tris = 0;
+    if (algo == PM_TRACE_BRUTE) {
+        tris = prm.brute_cap > 0 ? std::min(ctx->ntris, prm.brute_cap) : ctx->ntris;
+        if (woop) {
+            const float4 *W = woop_tris(ctx, tris, s);
+            A = W; B = W + tris; C = W + 2ll * tris;
+        }
+        launch_canvas<PM_TRACE_BRUTE>(false, woop, blocks, s, T, A, B, C, tris, org, dir, nrays, 1, n0, n1, n2, c0,
+                                      L, max_bounces, canvas);
     } else {
-        if (counters) dispatch<false, true>(T, algo, org, dir, ray_mesh, nrays, S, out, counters, s);
-        else dispatch<false, false>(T, algo, org, dir, ray_mesh, nrays, S, out, counters, s);
+        if (woop) T.leaf_woop = woop_slots(ctx, s);
+        const bool skip = prm.leaf_skipping && ctx->has_boxes;
+        if (algo == PM_TRACE_KD_RESTART)
+            launch_canvas<PM_TRACE_KD_RESTART>(skip, woop, blocks, s, T, A, B, C, 0, org, dir, nrays, 1, n0, n1, n2,
+                                               c0, L, max_bounces, canvas);
+        else
+            launch_canvas<PM_TRACE_SHORT_STACK>(skip, woop, blocks, s, T, A, B, C, 0, org, dir, nrays,
+                                                prm.stack_size, n0, n1, n2, c0, L, max_bounces, canvas);
     }
     PM_LAUNCH_CHECK();
 }

@@ -38,12 +38,16 @@ extern "C" {
 
 /* TriangleMap::LowerAlgorithm, Utils/CUDA/TriangleMap.h:33 */
 #define PM_LOWER_BITMAP   0
-#define PM_LOWER_BALANCED 1       /* not provided: PM_ERR_UNSUPPORTED */
+#define PM_LOWER_BALANCED 1
 #define PM_LOWER_SAH      2
-/* TriangleMap::SplitMethod, Utils/CUDA/TriangleMap.h:34 (only BOX) */
+/* TriangleMap::SplitMethod, Utils/CUDA/TriangleMap.h:34; SPLIT is not
+ * implemented (PM_ERR_NOT_IMPLEMENTED, as TriangleMapUpperCreator.cu:551) */
 #define PM_SPLIT_BOX    0
 #define PM_SPLIT_DIVIDE 1
 #define PM_SPLIT_SPLIT  2
+/* IRayTracer::IntersectionAlgorithm, IRayTracer.h:30 */
+#define PM_INTERSECT_MOELLER 0
+#define PM_INTERSECT_WOOP    1
 /* tracer kinds: BruteTracer, RayTracer (kd-restart), ShortStack */
 #define PM_TRACE_BRUTE       0
 #define PM_TRACE_KD_RESTART  1
@@ -99,12 +103,16 @@ typedef struct {
  *                  forced off when the tree has no propagated boxes
  *                  (Raytracer.cu:313)
  *   stack_size     SHORT_STACK_SIZE (ShortStack.cu:25), 1..16
- *   brute_cap      MAX_PRIMS (BruteTracer.cu:50); 0 = test every triangle */
+ *   brute_cap      MAX_PRIMS (BruteTracer.cu:50); 0 = test every triangle
+ *   intersection   IRayTracer::SetIntersectionAlgorithm (IRayTracer.h:52):
+ *                  PM_INTERSECT_MOELLER (default) or PM_INTERSECT_WOOP, the
+ *                  latter on per-triangle Woop matrices computed at each
+ *                  call (GeometryList::GetWoopValues, GeometryList.cu:103-117) */
 typedef struct {
     int leaf_skipping;
     int stack_size;
     int brute_cap;
-    int reserved;
+    int intersection;
 } pm_trace_params;
 
 /* Tree view: device pointers into the context's tree (non-owning; valid
@@ -289,6 +297,39 @@ int pm_autotune(pm_ctx *ctx, const float *p0, const float *p1, const float *p2, 
                 const int *mesh_offsets, int num_meshes, const float *org, const float *dir,
                 const int *ray_mesh, int nrays, const pm_build_params *base,
                 float builds_per_trace, int reps, pm_tuning *out, void *stream);
+
+/* Light of the canvas shading: ColorKernels.h:10-13 (__constant__
+ * d_lightPosition / Ambient / Diffuse / Specular). */
+typedef struct {
+    float position[3], ambient[3], diffuse[3], specular[3];
+} pm_light;
+
+/* IRayTracer::Trace(IRenderCanvas*, uchar4*) (IRayTracer.h:47) with the
+ * rays passed in: per ray, the tracer's first hit is shaded with
+ * PhongLighting and the ray turned into its mirror reflection (Lighting,
+ * ColorKernels.h:15-77), colours are blended (BlendColor, :79-84) and the
+ * reflected ray is traced again, until a miss, alpha >= 0.97 or max_bounces
+ * (the reference has no cap); canvas gets one uchar4 per ray (device).
+ * n0/n1/n2: device float4 corner normals per triangle, c0: device uchar4
+ * colour per triangle (GeometryList's n0..n2/c0, see pm_add_mesh_attributes).
+ * Shadows are off, as in the reference (Raytracer.cu:219-226).  The
+ * specular power is the correctly rounded float of the double-precision
+ * power (the reference's device __powf is not reproduced). */
+int pm_trace_canvas(pm_ctx *ctx, int algo, const float *org, const float *dir, int nrays,
+                    const float *n0, const float *n1, const float *n2, const unsigned char *c0,
+                    const pm_light *light, int max_bounces, unsigned char *canvas,
+                    const pm_trace_params *params, void *stream);
+
+/* The normal and colour half of AddMeshKernel (float4/uchar4 variant,
+ * GeometryList.cu:197-232): corner normals (normalMat * n.xyz, 0) with
+ * normalMat the upper-left 3x3 of the row-major model matrix (identity when
+ * NULL), and corner colours, for triangles [offset, offset + nidx/3).
+ * normals: device float4[nv]; colors: device uchar4[nv] or NULL (then c0..c2
+ * are left alone). */
+int pm_add_mesh_attributes(pm_ctx *ctx, const float *normals, const unsigned char *colors, int nv,
+                           const unsigned *indices, int nidx, const float *model, float *n0, float *n1,
+                           float *n2, unsigned char *c0, unsigned char *c1, unsigned char *c2, int offset,
+                           void *stream);
 
 /* Backward of the intersection (no reference counterpart; the forward is
  * MoellerTrumbore, IRayTracer.h:111-145).  For every ray with hits[r].id >= 0

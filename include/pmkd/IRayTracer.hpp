@@ -1,7 +1,8 @@
 // IRayTracer.hpp -- tracer interface (Utils/CUDA/IRayTracer.h:30-60).
 // Trace() takes device ray arrays (float4 origin/direction) and writes the
-// first hit of each ray; the reference builds camera rays itself and shades
-// into a canvas (IRayTracer.cu:72-100, Raytracer.cu:295-390).
+// first hit of each ray, or (with a Shading) the shaded canvas the reference
+// renders (Raytracer.cu:165-262, ShortStack.cu:96-194); the reference
+// builds camera rays itself (IRayTracer.cu:72-100).
 #pragma once
 #include "TriangleMap.hpp"
 
@@ -10,7 +11,15 @@ namespace Utils {
 
 class IRayTracer {
 public:
-    enum IntersectionAlgorithm { MOELLER, WOOP };
+    enum IntersectionAlgorithm { MOELLER = PM_INTERSECT_MOELLER, WOOP = PM_INTERSECT_WOOP };
+
+    // what the canvas shading reads (GeometryList n0..n2/c0, ColorKernels.h:10-13)
+    struct Shading {
+        const float4 *n0, *n1, *n2;
+        const uchar4 *c0;
+        pm_light light;
+        int max_bounces = 64;
+    };
 
     IRayTracer() {
         pm_trace_params_default(&params_);
@@ -26,10 +35,25 @@ public:
 
     void Trace(const float4 *origins, const float4 *directions, int rays, pm_hit *hits,
                hipStream_t stream = nullptr, int *counters = nullptr) {
-        if (algorithm_ == WOOP) throw NotImplemented("Woop intersection is not provided");
+        params_.intersection = algorithm_;
         hip_ok(hipEventRecord(e0_, stream), "hipEventRecord");
         check(Ctx(), pm_trace(Ctx(), Kind(), (const float *)origins, (const float *)directions, rays, hits,
                               &params_, counters, stream));
+        hip_ok(hipEventRecord(e1_, stream), "hipEventRecord");
+        if (printTiming_) {
+            hip_ok(hipEventSynchronize(e1_), "hipEventSynchronize");
+            hip_ok(hipEventElapsedTime(&renderTime_, e0_, e1_), "hipEventElapsedTime");
+        }
+    }
+    // Trace(canvas) (IRayTracer.h:47): one shaded uchar4 per ray
+    void Trace(const float4 *origins, const float4 *directions, int rays, const Shading &sh, uchar4 *canvas,
+               hipStream_t stream = nullptr) {
+        params_.intersection = algorithm_;
+        hip_ok(hipEventRecord(e0_, stream), "hipEventRecord");
+        check(Ctx(), pm_trace_canvas(Ctx(), Kind(), (const float *)origins, (const float *)directions, rays,
+                                     (const float *)sh.n0, (const float *)sh.n1, (const float *)sh.n2,
+                                     (const unsigned char *)sh.c0, &sh.light, sh.max_bounces,
+                                     (unsigned char *)canvas, &params_, stream));
         hip_ok(hipEventRecord(e1_, stream), "hipEventRecord");
         if (printTiming_) {
             hip_ok(hipEventSynchronize(e1_), "hipEventSynchronize");

@@ -110,10 +110,27 @@ int main() {
     int diff2 = 0;
     for (int i = 0; i < R; ++i) diff2 += b2[i].id != c[i].id;
     EXPECT(diff2 <= R / 1000);
+    // Woop intersection (IRayTracer.h:78-109): the same first hits up to rounding
     ss.SetIntersectionAlgorithm(Utils::IRayTracer::WOOP);
-    threw = false;
-    try { ss.Trace(O.GetDeviceData(), D.GetDeviceData(), R, hs.GetDeviceData()); } catch (const NotImplemented &) { threw = true; }
-    EXPECT(threw);
+    ss.Trace(O.GetDeviceData(), D.GetDeviceData(), R, hs.GetDeviceData());
+    hip_ok(hipDeviceSynchronize(), "sync");
+    const auto w = hs.Download();
+    int diff_woop = 0;
+    for (int i = 0; i < R; ++i) diff_woop += w[i].id != b2[i].id;
+    EXPECT(diff_woop <= R / 100);
+    ss.SetIntersectionAlgorithm(Utils::IRayTracer::MOELLER);
+
+    // the shaded canvas (IRayTracer::Trace(canvas)): every hit pixel is lit
+    Utils::IRayTracer::Shading sh{geom.GetNormal0Data(), geom.GetNormal1Data(), geom.GetNormal2Data(),
+                                  geom.GetColor0Data(), pm_light{{0.0f, 0.9f, 0.0f}, {0.2f, 0.2f, 0.2f},
+                                                                 {0.8f, 0.8f, 0.8f}, {0.0f, 0.0f, 0.0f}}, 8};
+    DeviceBuffer<uchar4> canvas(R);
+    ss.Trace(O.GetDeviceData(), D.GetDeviceData(), R, sh, canvas.GetDeviceData());
+    hip_ok(hipDeviceSynchronize(), "sync");
+    const auto px = canvas.Download();
+    int lit = 0;
+    for (int i = 0; i < R; ++i) lit += (w[i].id >= 0) == (px[i].w > 0);
+    EXPECT(lit >= R - R / 100);
 
     std::printf("%s: %d tris, %d nodes, %d/%d hits, kd diff %d, brute diff %d\n",
                 failures ? "FAILED" : "C++ API OK", geom.GetSize(), st.num_nodes, hits, R, diff_kd, diff_brute);

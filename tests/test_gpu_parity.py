@@ -172,11 +172,11 @@ def test_error_codes(pm, torch, scenes):
     assert e.value.code == 7                                   # PM_ERR_NO_TREE
     tris = [dev(torch, t) for t in scenes.random_soup(100, seed=1)]
     with pytest.raises(pm.PmError) as e:
-        c.build(*tris, pm.build_params("reference", split_method=pm.SPLIT_DIVIDE))
+        c.build(*tris, pm.build_params("reference", split_method=pm.SPLIT_SPLIT))
     assert e.value.code == 2                                   # PM_ERR_NOT_IMPLEMENTED
     with pytest.raises(pm.PmError) as e:
-        c.build(*tris, pm.build_params("reference", lower_algorithm=pm.LOWER_BALANCED))
-    assert e.value.code == 5                                   # PM_ERR_UNSUPPORTED
+        c.build(*tris, pm.build_params("reference", lower_algorithm=7))
+    assert e.value.code == 1                                   # PM_ERR_ARG
     p0 = np.repeat(np.array([[0, 0, 0, 1]], np.float32), 40, axis=0)
     p1 = np.repeat(np.array([[1, 0, 0, 1]], np.float32), 40, axis=0)
     p2 = np.repeat(np.array([[0, 1, 0, 1]], np.float32), 40, axis=0)
