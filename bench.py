@@ -215,6 +215,7 @@ def bench_batched(pm, scenes, torch, meshes: int = 256, rays_per_mesh: int = 655
                       "lower_levels": caps["lower_levels"]},
             "autotuned": {"chunk_size": tune["chunk_size"], "lower_size": tune["lower_size"],
                           "stack_size": tune["stack_size"], "trials": tune["trials"],
+                          "ray_order": ["input", "coherent"][tune["ray_order"]],
                           "frame_ms": round(tb, 3), "build_ms": round(tb_build, 3),
                           "trace_ms": round(tb_trace, 3), "speedup_vs_per_mesh": round(p / tb, 2)}}
 
@@ -266,7 +267,9 @@ def main() -> int:
     src = torch.empty(n0, dtype=torch.int32, device=dev)
     ctx = pm.Context(0)
     bp = pm.build_params("reference")
-    tp = pm.trace_params(stack_size=args.stack)
+    # photons and bounce rays are incoherent: the tracer sorts them by a
+    # spatial key inside each trace call (its time is included)
+    tp = pm.trace_params(stack_size=args.stack, ray_order=pm.RAY_ORDER_COHERENT)
     stream = torch.cuda.current_stream()
 
     if args.probe:
@@ -280,7 +283,7 @@ def main() -> int:
                           "tris": ntris}), flush=True)
         return 0
 
-    def step(timing: dict | None):
+    def step(timing: dict | None, tp=tp):
         """one frame: rebuild + all photon passes; returns rays traced"""
         O[0].copy_(O0)
         D[0].copy_(D0)
@@ -335,6 +338,14 @@ def main() -> int:
     trace_ms_total = sum(timing["trace_ms"])
     mrays = rays_total / (trace_ms_total * 1e-3) / 1e6
     build_ms = float(np.median(timing["build_ms"]))
+    # the same passes with the rays traced in input order (no ray sort)
+    t_in = {k: [] for k in ("build_ms", "build_wall_ms", "trace_ms", "pass0_ms", "rays")}
+    tp_input = pm.trace_params(stack_size=args.stack, ray_order=pm.RAY_ORDER_INPUT)
+    step(None, tp_input)
+    for i in range(2):
+        step(t_in, tp_input)
+    input_mrays = sum(t_in["rays"]) / (sum(t_in["trace_ms"]) * 1e-3) / 1e6
+    log(f"[bench] input ray order: {input_mrays:.1f} Mrays/s")
 
     # ---- roofline of the dominant kernel: short-stack trace of the emission batch
     ctx.build(*P, bp)
@@ -421,6 +432,9 @@ def main() -> int:
             "kd_build_ms": round(build_ms, 3),
             "kd_build_wall_ms": round(float(np.median(timing["build_wall_ms"])), 3),
             "trace_ms_per_step": round(trace_ms_total / args.steps, 3),
+            "ray_order": "coherent (spatial-key radix sort inside each trace call, included in the times)",
+            "input_order_mrays": round(input_mrays, 2),
+            "input_order_trace_ms_per_step": round(sum(t_in["trace_ms"]) / len(t_in["trace_ms"]), 3),
             "kd_restart_mrays": round(kd_mrays, 2),
             "brute_force_mrays": round(brute_mrays, 4),
             "short_stack_vs_brute": round(mrays / brute_mrays, 1),
@@ -432,7 +446,7 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "short_stack_kernel (emission batch)",
+            "kernel": "short_stack_kernel (emission batch; events bracket the trace call, ray sort included)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

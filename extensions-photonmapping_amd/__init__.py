@@ -27,6 +27,7 @@ LOWER_BITMAP, LOWER_BALANCED, LOWER_SAH = 0, 1, 2
 SPLIT_BOX, SPLIT_DIVIDE, SPLIT_SPLIT = 0, 1, 2
 BRUTE, KD_RESTART, SHORT_STACK = 0, 1, 2
 MOELLER, WOOP = 0, 1
+RAY_ORDER_INPUT, RAY_ORDER_COHERENT = 0, 1
 
 # every symbol include/pmkd.h declares
 EXPORTS = ("pm_build_params_default", "pm_trace_params_default", "pm_create", "pm_destroy",
@@ -54,7 +55,7 @@ class BuildParams(C.Structure):
 
 class TraceParams(C.Structure):
     _fields_ = [("leaf_skipping", C.c_int), ("stack_size", C.c_int), ("brute_cap", C.c_int),
-                ("intersection", C.c_int)]
+                ("intersection", C.c_int), ("ray_order", C.c_int)]
 
 
 class Tree(C.Structure):
@@ -82,7 +83,8 @@ class GraphCaps(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("chunk_size", C.c_int), ("lower_size", C.c_int), ("stack_size", C.c_int),
-                ("build_ms", C.c_float), ("trace_ms", C.c_float), ("trials", C.c_int)]
+                ("build_ms", C.c_float), ("trace_ms", C.c_float), ("trials", C.c_int),
+                ("ray_order", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -275,8 +277,8 @@ class Context:
     def autotune(self, p0, p1, p2, org, dirs, mesh_offsets=None, ray_mesh=None,
                  base: BuildParams | None = None, builds_per_trace: float = 1.0, reps: int = 3,
                  stream=None) -> dict:
-        """time this workload over chunk size, leaf threshold and stack depth
-        (pm_autotune); the context keeps the tree of the chosen setting.
+        """time this workload over chunk size, leaf threshold, stack depth and
+        ray order (pm_autotune); the context keeps the tree of the chosen setting.
         Returns the choice with its times and ready-made parameter structs."""
         base = base or build_params()
         off = None
@@ -293,7 +295,7 @@ class Context:
         bp = BuildParams.from_buffer_copy(base)
         bp.chunk_size, bp.lower_size = t.chunk_size, t.lower_size
         res["build_params"] = bp
-        res["trace_params"] = trace_params(stack_size=t.stack_size)
+        res["trace_params"] = trace_params(stack_size=t.stack_size, ray_order=t.ray_order)
         return res
 
     # -- one graph launch per frame

@@ -52,6 +52,8 @@ void check_trace_params(const pm_trace_params &p) {
     if (p.stack_size < 1 || p.stack_size > 16) throw Error{PM_ERR_ARG, "stack_size must be in [1, 16]"};
     if (p.intersection != PM_INTERSECT_MOELLER && p.intersection != PM_INTERSECT_WOOP)
         throw Error{PM_ERR_ARG, "unknown intersection algorithm"};
+    if (p.ray_order != PM_RAY_ORDER_INPUT && p.ray_order != PM_RAY_ORDER_COHERENT)
+        throw Error{PM_ERR_ARG, "unknown ray order"};
 }
 
 }  // namespace
@@ -80,6 +82,7 @@ void pm_trace_params_default(pm_trace_params *p) {
     p->stack_size = 4;
     p->brute_cap = 0;
     p->intersection = PM_INTERSECT_MOELLER;
+    p->ray_order = PM_RAY_ORDER_INPUT;
 }
 
 int pm_create(pm_ctx **out, int device) {

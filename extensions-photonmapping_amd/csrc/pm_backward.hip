@@ -25,8 +25,6 @@ namespace pmkd {
 namespace {
 
 constexpr int TB = 256;
-constexpr int SORT_ITEMS = 16;
-constexpr int TILE = TB * SORT_ITEMS;     // keys per sort block
 
 __device__ __forceinline__ f3 scale3(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
 __device__ __forceinline__ f3 add3(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
@@ -37,7 +35,7 @@ __global__ __launch_bounds__(TB) void mt_backward_kernel(
     const float4 *__restrict__ org, const float4 *__restrict__ dir, const pm_hit *__restrict__ hits,
     int n, const float *__restrict__ grad_t, const float *__restrict__ grad_u,
     const float *__restrict__ grad_v, const int *__restrict__ faces, unsigned miss_key,
-    unsigned *__restrict__ keys, unsigned *__restrict__ vals, float *__restrict__ corner,
+    unsigned *__restrict__ keys, float *__restrict__ corner,
     float *__restrict__ grad_org, float *__restrict__ grad_dir) {
     const int r = blockIdx.x * TB + threadIdx.x;
     if (r >= n) return;
@@ -70,67 +68,10 @@ __global__ __launch_bounds__(TB) void mt_backward_kernel(
     for (int k = 0; k < 3; ++k) {
         const long long e = 3ll * r + k;
         keys[e] = id >= 0 ? (faces ? (unsigned)faces[3 * id + k] : (unsigned)(3 * id + k)) : miss_key;
-        vals[e] = (unsigned)e;
         corner[3 * e] = gc3[k].x; corner[3 * e + 1] = gc3[k].y; corner[3 * e + 2] = gc3[k].z;
     }
     if (grad_org) { grad_org[3 * r] = g_o.x; grad_org[3 * r + 1] = g_o.y; grad_org[3 * r + 2] = g_o.z; }
     if (grad_dir) { grad_dir[3 * r] = g_d.x; grad_dir[3 * r + 1] = g_d.y; grad_dir[3 * r + 2] = g_d.z; }
-}
-
-// ---- stable LSD radix sort, 8-bit digits
-__global__ __launch_bounds__(TB) void radix_hist_kernel(const unsigned *__restrict__ keys, int n, int shift,
-                                                        int nb, int *__restrict__ hist) {
-    __shared__ int cnt[256];
-    cnt[threadIdx.x] = 0;
-    __syncthreads();
-    const long long base = (long long)blockIdx.x * TILE;
-    for (int it = 0; it < SORT_ITEMS; ++it) {
-        const long long i = base + (long long)it * TB + threadIdx.x;
-        if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1);
-    }
-    __syncthreads();
-    hist[threadIdx.x * nb + blockIdx.x] = cnt[threadIdx.x];   // digit-major
-}
-
-__global__ __launch_bounds__(TB) void radix_scatter_kernel(const unsigned *__restrict__ kin,
-                                                           const unsigned *__restrict__ vin, int n,
-                                                           int shift, int nb, const int *__restrict__ offs,
-                                                           unsigned *__restrict__ kout,
-                                                           unsigned *__restrict__ vout) {
-    __shared__ int base[256];
-    __shared__ int wcount[TB / 64][256];
-    base[threadIdx.x] = offs[threadIdx.x * nb + blockIdx.x];
-    for (int w = 0; w < TB / 64; ++w) wcount[w][threadIdx.x] = 0;
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-    const long long tbase = (long long)blockIdx.x * TILE;
-    for (int it = 0; it < SORT_ITEMS; ++it) {
-        const long long i = tbase + (long long)it * TB + threadIdx.x;    // element order = (it, thread)
-        const bool valid = i < n;
-        unsigned k = 0, v = 0, dg = 0;
-        if (valid) { k = kin[i]; v = vin[i]; dg = (k >> shift) & 255u; }
-        unsigned long long peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            const unsigned long long bb = __ballot(valid && ((dg >> b) & 1u));
-            peers &= ((dg >> b) & 1u) ? bb : ~bb;
-        }
-        const int wrank = __popcll(peers & lt);
-        if (valid && wrank == 0) wcount[wave][dg] = __popcll(peers);
-        __syncthreads();
-        if (valid) {
-            int pos = base[dg] + wrank;
-            for (int w = 0; w < wave; ++w) pos += wcount[w][dg];
-            kout[pos] = k;
-            vout[pos] = v;
-        }
-        __syncthreads();
-        int tot = 0;
-        for (int w = 0; w < TB / 64; ++w) { tot += wcount[w][threadIdx.x]; wcount[w][threadIdx.x] = 0; }
-        base[threadIdx.x] += tot;
-        __syncthreads();
-    }
 }
 
 // per vertex: sequential sum of its entries in sorted (= ray, corner) order
@@ -203,24 +144,18 @@ void trace_backward(pm_ctx *ctx, const float4 *org, const float4 *dir, const pm_
     sc.bwd_corner.reserve(3 * (size_t)m, s);
     mt_backward_kernel<<<(n + TB - 1) / TB, TB, 0, s>>>(ctx->p0, ctx->p1, ctx->p2, org, dir, hits, n, grad_t, grad_u,
                                                        grad_v, faces, miss_key, sc.bwd_keys[0].ptr,
-                                                       sc.bwd_vals[0].ptr, sc.bwd_corner.ptr, grad_org, grad_dir);
+                                                       sc.bwd_corner.ptr, grad_org, grad_dir);
     PM_LAUNCH_CHECK();
     int bits = 1;
     while (bits < 32 && ((unsigned long long)miss_key >> bits)) ++bits;
-    const int nb = (m + TILE - 1) / TILE;
-    sc.bwd_hist.reserve(256 * (size_t)nb + 1, s);
-    sc.bwd_offs.reserve(256 * (size_t)nb + 1, s);
-    int cur = 0;
-    for (int shift = 0; shift < bits; shift += 8) {
-        radix_hist_kernel<<<nb, TB, 0, s>>>(sc.bwd_keys[cur].ptr, m, shift, nb, sc.bwd_hist.ptr);
-        PM_LAUNCH_CHECK();
-        scan_exclusive(sc.bwd_hist.ptr, sc.bwd_offs.ptr, 256 * nb, sc, s);
-        radix_scatter_kernel<<<nb, TB, 0, s>>>(sc.bwd_keys[cur].ptr, sc.bwd_vals[cur].ptr, m, shift, nb,
-                                               sc.bwd_offs.ptr, sc.bwd_keys[cur ^ 1].ptr, sc.bwd_vals[cur ^ 1].ptr);
-        PM_LAUNCH_CHECK();
-        cur ^= 1;
-    }
-    segment_sum_kernel<<<(m + TB - 1) / TB, TB, 0, s>>>(sc.bwd_keys[cur].ptr, sc.bwd_vals[cur].ptr, m, miss_key,
+    sc.bwd_hist.reserve(radix_hist_size(m), s);
+    sc.bwd_offs.reserve(radix_hist_size(m), s);
+    reserve_scan_tmp(sc, (long long)radix_hist_size(m), s);
+    unsigned *const K[2] = {sc.bwd_keys[0].ptr, sc.bwd_keys[1].ptr};
+    unsigned *const V[2] = {sc.bwd_vals[0].ptr, sc.bwd_vals[1].ptr};
+    // values = entry indices (3 r + corner): the sorted order within a vertex is (ray, corner)
+    const int cur = radix_sort_pairs(K, V, m, bits, sc.bwd_hist.ptr, sc.bwd_offs.ptr, false, true, sc, s);
+    segment_sum_kernel<<<(m + TB - 1) / TB, TB, 0, s>>>(K[cur], V[cur], m, miss_key,
                                                         sc.bwd_corner.ptr, grad_vertices);
     PM_LAUNCH_CHECK();
 }

@@ -1451,8 +1451,19 @@ std::vector<void **> graph_buffers(pm_ctx *ctx) {
         v.push_back((void **)&sc.keys[i].ptr);
     }
     for (int i = 0; i < 3; ++i) v.push_back((void **)&sc.scan_tmp[i].ptr);
+    for (int i = 0; i < 2; ++i) {
+        v.push_back((void **)&sc.order_keys[i].ptr);
+        v.push_back((void **)&sc.order_vals[i].ptr);
+    }
+    v.push_back((void **)&sc.order_hist.ptr);
+    v.push_back((void **)&sc.order_offs.ptr);
+    v.push_back((void **)&sc.order_org.ptr);
+    v.push_back((void **)&sc.order_dir.ptr);
+    v.push_back((void **)&sc.order_root.ptr);
     return v;
 }
+
+}  // namespace
 
 // block-sum buffers the scans of length up to n need, per recursion depth
 void reserve_scan_tmp(BuildScratch &sc, long long n, hipStream_t s) {
@@ -1463,8 +1474,6 @@ void reserve_scan_tmp(BuildScratch &sc, long long n, hipStream_t s) {
         n = nb;
     }
 }
-
-}  // namespace
 
 void destroy_graph(pm_ctx *ctx) {
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
@@ -1520,6 +1529,7 @@ void capture_batch(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4
     sc.lower_careas.reserve(2 * (size_t)caps.lower, s);
     ctx->packed.reserve((size_t)caps.nodes + 1, s);
     reserve_scan_tmp(sc, std::max({2 * caps.active, caps.segs, caps.lower}), s);
+    if (ray_order_applies(tp, nrays)) reserve_ray_order(ctx, nrays, ray_mesh != nullptr, s);
     sc.lvl.reserve(1, s);
     // the batch's own copy of its mesh offsets (eager builds rewrite the scratch copy)
     ctx->graph_off.reserve((size_t)M + 1, s);

@@ -150,6 +150,10 @@ struct BuildScratch {
     DevBuf<unsigned> bwd_keys[2], bwd_vals[2];  // backward: radix sort ping-pong
     DevBuf<float> bwd_corner;                   // backward: 3 float3 per ray
     DevBuf<int> bwd_hist, bwd_offs;             // backward: radix histograms
+    DevBuf<unsigned> order_keys[2], order_vals[2];   // coherent ray order: radix sort ping-pong
+    DevBuf<int> order_hist, order_offs;         // coherent ray order: radix histograms
+    DevBuf<float4> order_org, order_dir;        // coherent ray order: the rays in that order
+    DevBuf<int> order_root;                     // coherent ray order: their roots (forest)
     DevBuf<int> mesh_off;                       // forest: triangle offsets per mesh (M+1)
     DevBuf<Lvl> lvl;                            // level state (one record)
     HostPinned host;
@@ -253,4 +257,27 @@ void scan_exclusive(const int *in, int *out, int n, BuildScratch &sc, hipStream_
                     const int *n_dev = nullptr, int n_mult = 1);
 void scan_exclusive2(const int *in0, int *out0, const int *in1, int *out1, int n, BuildScratch &sc,
                      hipStream_t s, const int *n_dev = nullptr, int n_mult = 1);
+// block-sum buffers of every scan level for scans of up to n elements
+void reserve_scan_tmp(BuildScratch &sc, long long n, hipStream_t s);
+// pm_order.hip: stable LSD radix sort of n (key, value) pairs on key bits
+// [0, bits), 8 bits per pass; keys[0]/vals[0] hold the input (vals unread
+// with iota: the values are then the input indices; hist already holds the
+// first digit's tile histograms with first_hist_ready); hist/offs need
+// radix_hist_size(n) ints.  Returns which buffer holds the result.
+int radix_sort_pairs(unsigned *const keys[2], unsigned *const vals[2], int n, int bits, int *hist, int *offs,
+                     bool first_hist_ready, bool iota, BuildScratch &sc, hipStream_t s);
+size_t radix_hist_size(long long n);
+// coherent ray order: whether a trace of nrays rays uses it, its buffers
+// (reserved ahead of a graph capture), and the rays in that order: lane r
+// traces org[r], dir[r] (root[r]) and writes to index[r] (device arrays,
+// valid until the next call on ctx)
+struct OrderedRays {
+    const float4 *org, *dir;
+    const int *root;
+    const unsigned *index;
+};
+bool ray_order_applies(const pm_trace_params &prm, int nrays);
+void reserve_ray_order(pm_ctx *ctx, int nrays, bool roots, hipStream_t s);
+OrderedRays ray_order(pm_ctx *ctx, const float4 *org, const float4 *dir, const int *ray_root, int nrays,
+                      hipStream_t s);
 }  // namespace pmkd

@@ -198,7 +198,8 @@ __device__ __forceinline__ pm_hit short_stack_ray(const TreeDev &T, int root_id,
     return res;
 }
 
-// the first-hit kernels: one lane per ray
+// the first-hit kernels: lane r traces org[r], dir[r] (root ray_root[r])
+// and writes the record to out[order[r]] (out[r] without an order)
 __device__ __forceinline__ bool ray_root_valid(const TreeDev &T, const int *ray_root, int r, int &root_id) {
     root_id = ray_root ? ray_root[r] : 0;
     // no such mesh, or (captured batch) the build before this trace failed: a miss
@@ -208,17 +209,20 @@ __device__ __forceinline__ bool ray_root_valid(const TreeDev &T, const int *ray_
 template <bool SKIP, bool COUNT, bool WOOP>
 __global__ __launch_bounds__(TB) void kd_restart_kernel(TreeDev T, const float4 *__restrict__ org,
                                                         const float4 *__restrict__ dir,
-                                                        const int *__restrict__ ray_root, int n,
+                                                        const int *__restrict__ ray_root,
+                                                        const unsigned *__restrict__ order, int n,
                                                         pm_hit *__restrict__ out,
                                                         int4 *__restrict__ counters) {
-    const int r = blockIdx.x * TB + threadIdx.x;
-    if (r >= n) return;
+    const int lane_ray = blockIdx.x * TB + threadIdx.x;
+    if (lane_ray >= n) return;
+    const int r = order ? (int)order[lane_ray] : lane_ray;
+    if ((unsigned)r >= (unsigned)n) return;
     int4 cnt = make_int4(0, 0, 0, 0);
     int root_id;
     pm_hit res = miss_record();
-    if (ray_root_valid(T, ray_root, r, root_id)) {
-        const f3 o = mk3(org[r]);
-        const f3 d = adjust_dir(mk3(dir[r]));
+    if (ray_root_valid(T, ray_root, lane_ray, root_id)) {
+        const f3 o = mk3(org[lane_ray]);
+        const f3 d = adjust_dir(mk3(dir[lane_ray]));
         const f3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
         res = kd_restart_ray<SKIP, COUNT, WOOP>(T, root_id, o, d, inv, cnt);
     }
@@ -229,19 +233,22 @@ __global__ __launch_bounds__(TB) void kd_restart_kernel(TreeDev T, const float4 
 template <int SMAX, bool SKIP, bool COUNT, bool WOOP>
 __global__ __launch_bounds__(TB) void short_stack_kernel(TreeDev T, const float4 *__restrict__ org,
                                                          const float4 *__restrict__ dir,
-                                                         const int *__restrict__ ray_root, int n,
+                                                         const int *__restrict__ ray_root,
+                                                         const unsigned *__restrict__ order, int n,
                                                          int S, pm_hit *__restrict__ out,
                                                          int4 *__restrict__ counters) {
     __shared__ int stk_node[SMAX][TB];
     __shared__ float stk_t[SMAX][TB];
-    const int r = blockIdx.x * TB + threadIdx.x;
-    if (r >= n) return;
+    const int lane_ray = blockIdx.x * TB + threadIdx.x;
+    if (lane_ray >= n) return;
+    const int r = order ? (int)order[lane_ray] : lane_ray;
+    if ((unsigned)r >= (unsigned)n) return;
     int4 cnt = make_int4(0, 0, 0, 0);
     int root_id;
     pm_hit res = miss_record();
-    if (ray_root_valid(T, ray_root, r, root_id)) {
-        const f3 o = mk3(org[r]);
-        const f3 d = adjust_dir(mk3(dir[r]));
+    if (ray_root_valid(T, ray_root, lane_ray, root_id)) {
+        const f3 o = mk3(org[lane_ray]);
+        const f3 d = adjust_dir(mk3(dir[lane_ray]));
         const f3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
         res = short_stack_ray<SMAX, SKIP, COUNT, WOOP>(T, root_id, o, d, inv, S, stk_node, stk_t, cnt);
     }
@@ -492,35 +499,42 @@ __global__ void clear_int_kernel(int *p) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
 }
 
+// the rays of one trace call
+struct RayBatch {
+    const float4 *org, *dir;
+    const int *root;            // per-ray root (forest) or null
+    const unsigned *order;      // lane -> index of its record, or null
+    int n;
+};
+
 template <int SMAX, bool SKIP, bool COUNT, bool WOOP>
-void launch_ss(const TreeDev &T, const float4 *org, const float4 *dir, const int *root, int n, int S,
-               pm_hit *out, int4 *cnt, hipStream_t s) {
-    const int blocks = (n + TB - 1) / TB;
-    short_stack_kernel<SMAX, SKIP, COUNT, WOOP><<<blocks, TB, 0, s>>>(T, org, dir, root, n, S, out, cnt);
+void launch_ss(const TreeDev &T, const RayBatch &R, int S, pm_hit *out, int4 *cnt, hipStream_t s) {
+    const int blocks = (R.n + TB - 1) / TB;
+    short_stack_kernel<SMAX, SKIP, COUNT, WOOP><<<blocks, TB, 0, s>>>(T, R.org, R.dir, R.root, R.order, R.n, S,
+                                                                      out, cnt);
 }
 
 template <bool SKIP, bool COUNT, bool WOOP>
-void dispatch(const TreeDev &T, int algo, const float4 *org, const float4 *dir, const int *root, int n,
-              int S, pm_hit *out, int4 *cnt, hipStream_t s) {
-    const int blocks = (n + TB - 1) / TB;
+void dispatch(const TreeDev &T, int algo, const RayBatch &R, int S, pm_hit *out, int4 *cnt, hipStream_t s) {
+    const int blocks = (R.n + TB - 1) / TB;
     if (algo == PM_TRACE_KD_RESTART) {
-        kd_restart_kernel<SKIP, COUNT, WOOP><<<blocks, TB, 0, s>>>(T, org, dir, root, n, out, cnt);
+        kd_restart_kernel<SKIP, COUNT, WOOP><<<blocks, TB, 0, s>>>(T, R.org, R.dir, R.root, R.order, R.n, out, cnt);
     } else {
-        if (S <= 4) launch_ss<4, SKIP, COUNT, WOOP>(T, org, dir, root, n, S, out, cnt, s);
-        else if (S <= 8) launch_ss<8, SKIP, COUNT, WOOP>(T, org, dir, root, n, S, out, cnt, s);
-        else launch_ss<16, SKIP, COUNT, WOOP>(T, org, dir, root, n, S, out, cnt, s);
+        if (S <= 4) launch_ss<4, SKIP, COUNT, WOOP>(T, R, S, out, cnt, s);
+        else if (S <= 8) launch_ss<8, SKIP, COUNT, WOOP>(T, R, S, out, cnt, s);
+        else launch_ss<16, SKIP, COUNT, WOOP>(T, R, S, out, cnt, s);
     }
 }
 
 template <bool WOOP>
-void dispatch_flags(const TreeDev &T, bool skip, bool count, int algo, const float4 *org, const float4 *dir,
-                    const int *root, int n, int S, pm_hit *out, int4 *cnt, hipStream_t s) {
+void dispatch_flags(const TreeDev &T, bool skip, bool count, int algo, const RayBatch &R, int S, pm_hit *out,
+                    int4 *cnt, hipStream_t s) {
     if (skip) {
-        if (count) dispatch<true, true, WOOP>(T, algo, org, dir, root, n, S, out, cnt, s);
-        else dispatch<true, false, WOOP>(T, algo, org, dir, root, n, S, out, cnt, s);
+        if (count) dispatch<true, true, WOOP>(T, algo, R, S, out, cnt, s);
+        else dispatch<true, false, WOOP>(T, algo, R, S, out, cnt, s);
     } else {
-        if (count) dispatch<false, true, WOOP>(T, algo, org, dir, root, n, S, out, cnt, s);
-        else dispatch<false, false, WOOP>(T, algo, org, dir, root, n, S, out, cnt, s);
+        if (count) dispatch<false, true, WOOP>(T, algo, R, S, out, cnt, s);
+        else dispatch<false, false, WOOP>(T, algo, R, S, out, cnt, s);
     }
 }
 
@@ -654,8 +668,13 @@ void trace_rays(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int
     if (woop) T.leaf_woop = woop_slots(ctx, s);
     const bool skip = prm.leaf_skipping && ctx->has_boxes;
     const int S = prm.stack_size;
-    if (woop) dispatch_flags<true>(T, skip, counters != nullptr, algo, org, dir, ray_mesh, nrays, S, out, counters, s);
-    else dispatch_flags<false>(T, skip, counters != nullptr, algo, org, dir, ray_mesh, nrays, S, out, counters, s);
+    RayBatch R{org, dir, ray_mesh, nullptr, nrays};
+    if (ray_order_applies(prm, nrays)) {
+        const OrderedRays o = ray_order(ctx, org, dir, ray_mesh, nrays, s);
+        R = {o.org, o.dir, o.root, o.index, nrays};
+    }
+    if (woop) dispatch_flags<true>(T, skip, counters != nullptr, algo, R, S, out, counters, s);
+    else dispatch_flags<false>(T, skip, counters != nullptr, algo, R, S, out, counters, s);
     PM_LAUNCH_CHECK();
 }
 

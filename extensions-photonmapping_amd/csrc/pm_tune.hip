@@ -1,6 +1,6 @@
 // pm_tune.hip -- autotuner: times the caller's workload over the build's
-// chunk size, the upper stage's leaf threshold and the short-stack depth,
-// and keeps the fastest setting.  Coordinate descent, one parameter at a
+// chunk size, the upper stage's leaf threshold, the short-stack depth and
+// the ray order, and keeps the fastest setting.  Coordinate descent, one parameter at a
 // time, each setting timed with HIP events on the caller's stream (median of
 // `reps` runs after one warm-up run).  The build's own host read-backs are
 // inside the timed span, so build times are what a caller waits for.
@@ -64,9 +64,10 @@ void autotune(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2,
         }
         return median(t);
     };
-    auto time_trace = [&](int stack) {
+    auto time_trace = [&](int stack, int order) {
         pm_trace_params q = tp;
         q.stack_size = stack;
+        q.ray_order = order;
         std::vector<float> t;
         for (int r = 0; r <= reps; ++r) {
             PM_HIP(hipEventRecord(ev.a, s));
@@ -105,21 +106,28 @@ void autotune(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2,
             if (e.code != PM_ERR_DEPTH) throw;
             continue;
         }
-        const float t = time_trace(8);
+        const float t = time_trace(8, PM_RAY_ORDER_INPUT);
         ++trials;
         const float cost = builds_per_trace * b + t;
         if (first || cost < best_cost) { best_cost = cost; best_leaf = ls; best_b = b; first = false; }
     }
     if (first) throw Error{PM_ERR_DEPTH, "no leaf threshold builds this geometry"};
     p.lower_size = best_leaf;
-    // 3. stack depth on the chosen tree
+    // 3. stack depth and ray order on the chosen tree (the order only
+    // matters from the size at which the tracers apply it)
     build_tree(ctx, P0, P1, P2, n, mesh_offsets, M, p, s);
-    int best_stack = 8;
+    int best_stack = 8, best_order = PM_RAY_ORDER_INPUT;
     float best_t = 0.0f;
-    for (int S : kStack) {
-        const float t = time_trace(S);
-        ++trials;
-        if (S == kStack[0] || t < best_t) { best_t = t; best_stack = S; }
+    first = true;
+    for (int order : {PM_RAY_ORDER_INPUT, PM_RAY_ORDER_COHERENT}) {
+        pm_trace_params q = tp;
+        q.ray_order = order;
+        if (order != PM_RAY_ORDER_INPUT && !ray_order_applies(q, nrays)) continue;
+        for (int S : kStack) {
+            const float t = time_trace(S, order);
+            ++trials;
+            if (first || t < best_t) { best_t = t; best_stack = S; best_order = order; first = false; }
+        }
     }
     PM_HIP(hipStreamSynchronize(s));
     out.chunk_size = best_chunk;
@@ -128,6 +136,7 @@ void autotune(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2,
     out.build_ms = best_b;
     out.trace_ms = best_t;
     out.trials = trials;
+    out.ray_order = best_order;
 }
 
 }  // namespace pmkd

@@ -48,6 +48,8 @@ extern "C" {
 /* IRayTracer::IntersectionAlgorithm, IRayTracer.h:30 */
 #define PM_INTERSECT_MOELLER 0
 #define PM_INTERSECT_WOOP    1
+#define PM_RAY_ORDER_INPUT    0
+#define PM_RAY_ORDER_COHERENT 1
 /* tracer kinds: BruteTracer, RayTracer (kd-restart), ShortStack */
 #define PM_TRACE_BRUTE       0
 #define PM_TRACE_KD_RESTART  1
@@ -107,12 +109,24 @@ typedef struct {
  *   intersection   IRayTracer::SetIntersectionAlgorithm (IRayTracer.h:52):
  *                  PM_INTERSECT_MOELLER (default) or PM_INTERSECT_WOOP, the
  *                  latter on per-triangle Woop matrices computed at each
- *                  call (GeometryList::GetWoopValues, GeometryList.cu:103-117) */
+ *                  call (GeometryList::GetWoopValues, GeometryList.cu:103-117)
+ *   ray_order      no reference counterpart: PM_RAY_ORDER_INPUT (default)
+ *                  traces the rays in input order; PM_RAY_ORDER_COHERENT
+ *                  lets the tree tracers run a trace of >= 131072 rays in
+ *                  the order of a spatial key (origin cell, direction cell;
+ *                  mesh first in a forest) so a wave's rays walk similar
+ *                  paths -- for incoherent rays (photons, diffuse bounces,
+ *                  random probes); rays that are coherent already (camera
+ *                  rays in pixel order) only pay for the sort.  Each ray's
+ *                  record (and counters) is the same either way and goes to
+ *                  the ray's own index; brute force and the canvas keep the
+ *                  input order. */
 typedef struct {
     int leaf_skipping;
     int stack_size;
     int brute_cap;
     int intersection;
+    int ray_order;
 } pm_trace_params;
 
 /* Tree view: device pointers into the context's tree (non-owning; valid
@@ -279,6 +293,7 @@ typedef struct {
     int chunk_size, lower_size, stack_size;   /* the chosen setting */
     float build_ms, trace_ms;                 /* its median times */
     int trials;                               /* settings timed */
+    int ray_order;                            /* the chosen pm_trace_params.ray_order */
 } pm_tuning;
 
 /* Autotuner (no reference counterpart: the reference fixes these at
@@ -287,7 +302,7 @@ typedef struct {
  * GPU: chunk_size over {128, 256, 512, 1024} (build time), then lower_size
  * (the upper stage's leaf threshold) over {8, 16, 24, 32, 48, 64} (cost =
  * builds_per_trace * build + trace), then stack_size over {4, 8, 16}
- * (trace time); each setting is timed `reps` times (median) after one
+ * together with ray_order over {INPUT, COHERENT} (trace time); each setting is timed `reps` times (median) after one
  * warm-up; a leaf threshold whose build stops at the reference-count
  * guard (PM_ERR_DEPTH) is skipped.  Every other field comes from `base`.  The geometry is one mesh
  * (mesh_offsets NULL) or a batch (as pm_build_batched); the rays are traced

@@ -74,6 +74,9 @@ public:
     IntersectionAlgorithm GetIntersectionAlgorithm() const { return algorithm_; }
     void SetLeafSkipping(bool s) { params_.leaf_skipping = s; }
     bool GetLeafSkipping() const { return params_.leaf_skipping != 0; }
+    // no reference counterpart: large traces of incoherent rays may run in
+    // a coherent ray order (off by default); the hit records are the same
+    void SetCoherentRayOrder(bool on) { params_.ray_order = on ? PM_RAY_ORDER_COHERENT : PM_RAY_ORDER_INPUT; }
     void PrintTiming(bool p) { printTiming_ = p; }
     bool GetPrintTiming() const { return printTiming_; }
     // trace time of the last Trace() in ms (with PrintTiming(true))

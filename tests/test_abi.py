@@ -43,7 +43,8 @@ def test_struct_layouts_match_header(pm, tmp_path):
     import ctypes as C
     import subprocess
     structs = {"pm_build_params": pm.BuildParams, "pm_trace_params": pm.TraceParams,
-               "pm_tree": pm.Tree, "pm_stats": pm.Stats}
+               "pm_tree": pm.Tree, "pm_stats": pm.Stats, "pm_tuning": pm.Tuning,
+               "pm_graph_caps": pm.GraphCaps, "pm_light": pm.Light}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "pmkd.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')

@@ -189,6 +189,7 @@ def test_autotune_picks_a_valid_setting(pm, orc, scenes, torch, ctx):
     assert res["chunk_size"] in (128, 256, 512, 1024)
     assert res["lower_size"] in (8, 16, 24, 32, 48, 64)
     assert res["stack_size"] in (4, 8, 16)
+    assert res["ray_order"] == pm.RAY_ORDER_INPUT          # too few rays to reorder
     # leaf thresholds the meshes cannot reach are skipped, the rest are timed
     assert 4 + 1 + 3 <= res["trials"] <= 4 + 6 + 3 and res["build_ms"] > 0 and res["trace_ms"] > 0
     # the context holds the chosen tree; tracing it with the chosen depth is
