@@ -154,6 +154,7 @@ struct BuildScratch {
     DevBuf<int> order_hist, order_offs;         // coherent ray order: radix histograms
     DevBuf<float4> order_org, order_dir;        // coherent ray order: the rays in that order
     DevBuf<int> order_root;                     // coherent ray order: their roots (forest)
+    DevBuf<int> trace_work;                     // tracers: rays handed out so far
     DevBuf<int> mesh_off;                       // forest: triangle offsets per mesh (M+1)
     DevBuf<Lvl> lvl;                            // level state (one record)
     HostPinned host;
@@ -195,6 +196,7 @@ struct pm_ctx {
     bool graph_boxes = false;
     std::vector<int> graph_offsets;
     bool graph_pending = false;        // a replay ran; host-side tree sizes not read back yet
+    int num_cus = 0;                   // compute units of the device (0 = not queried yet)
     const int *trace_fail = nullptr;   // while recording: the build status the trace checks
     ~pm_ctx();
 };

@@ -21,6 +21,7 @@
 #include "pm_device.h"
 #include "pm_internal.h"
 
+
 namespace pmkd {
 
 namespace {
@@ -39,7 +40,7 @@ __device__ __forceinline__ bool mt_edges(f3 v0, f3 e1, f3 e2, f3 o, f3 d, float 
     const f3 p = cross3(d, e2);
     const f3 t = sub3(o, v0);
     const f3 q = cross3(t, e1);
-    const float inv = 1.0f / dot3(p, e1);
+    const float inv = rcp_rn(dot3(p, e1));
     const float hx = inv * dot3(q, e2);
     if (0.0f <= hx && hx <= th) {
         const float hy = inv * dot3(p, t);
@@ -75,185 +76,230 @@ __device__ __forceinline__ int test_leaf(const TreeDev &T, const int4 nd, int no
         }
         return slot < 0 ? -1 : T.prim_indices[nd.x + slot];
     }
+    // two register sets, A and B, alternate: the next triangle loads while
+    // the current one is tested.  The loads are unconditional (past the last
+    // triangle they re-read the first one) so that a test waits for its own
+    // loads only.
     const float4 *L = T.leaf_tris + 3ll * nd.x;
-    int i = __builtin_ctzll(bm);
-    float4 a = L[3 * i], b = L[3 * i + 1], c = L[3 * i + 2];
+    const int first = __builtin_ctzll(bm);
+    bm &= bm - 1;
+    float4 a0 = L[3 * first], a1 = L[3 * first + 1], a2 = L[3 * first + 2];
     int hit = -1;
     for (;;) {
+        const bool more_b = bm != 0;
+        const int j = more_b ? __builtin_ctzll(bm) : first;
         bm &= bm - 1;
-        const bool more = bm != 0;
-        float4 na, nb, nc;
-        if (more) {
-            const int j = __builtin_ctzll(bm);
-            na = L[3 * j]; nb = L[3 * j + 1]; nc = L[3 * j + 2];
-        }
+        const float4 b0 = L[3 * j], b1 = L[3 * j + 1], b2 = L[3 * j + 2];
+        __builtin_amdgcn_sched_barrier(0);         // issue the loads before the test
         if (COUNT) cnt.z++;
-        if (mt_edges(mk3(a), mk3(b), mk3(c), o, d, th, tu, tv)) hit = __float_as_int(a.w);
-        if (!more) break;
-        a = na; b = nb; c = nc;
+        if (mt_edges(mk3(a0), mk3(a1), mk3(a2), o, d, th, tu, tv)) hit = __float_as_int(a0.w);
+        if (!more_b) break;
+        const bool more_a = bm != 0;
+        const int k = more_a ? __builtin_ctzll(bm) : first;
+        bm &= bm - 1;
+        a0 = L[3 * k]; a1 = L[3 * k + 1]; a2 = L[3 * k + 2];
+        __builtin_amdgcn_sched_barrier(0);
+        if (COUNT) cnt.z++;
+        if (mt_edges(mk3(b0), mk3(b1), mk3(b2), o, d, th, tu, tv)) hit = __float_as_int(b0.w);
+        if (!more_a) break;
     }
     return hit;
 }
 
-// kd-restart for one ray from node root_id (Raytracer.cu:57-106,165-262):
-// descend to the leaf the ray is in at t = th, test it, restart behind it
-template <bool SKIP, bool COUNT, bool WOOP>
-__device__ __forceinline__ pm_hit kd_restart_ray(const TreeDev &T, int root_id, f3 o, f3 d, f3 inv,
-                                                 int4 &cnt) {
-    pm_hit res = miss_record();
-    float th = 0.0f, tu = 0.0f, tv = 0.0f;
-    const int4 root = T.packed[root_id];
-    long long visits = 0;
-    do {
-        if (++visits > (long long)T.num_nodes + 1) break;
-        float tNext = F_INF;
-        int node = root_id;
-        int4 nd = root;
-        int steps = 0;
-        bool bad = false;
-        while (nd.w != LEAF) {
-            if (++steps > MAX_DESCENT) { bad = true; break; }
-            if (COUNT) cnt.x++;
-            const float s = __int_as_float(nd.x);
-            const float ori = comp3(o, nd.w), dd = comp3(inv, nd.w);
-            const float tSplit = (s - ori) * dd;
-            if (th < tSplit) {
-                node = 0 < dd ? nd.y : nd.z;
-                tNext = fminf(tSplit, tNext);
-            } else {
-                node = 0 < dd ? nd.z : nd.y;
-            }
-            nd = T.packed[node];
-        }
-        if (bad) break;
-        th = tNext * 1.00001f;
-        const int hit = test_leaf<SKIP, COUNT, WOOP>(T, nd, node, o, d, inv, th, tu, tv, cnt);
-        if (hit != -1) {
-            res.id = hit; res.t = th; res.u = tu; res.v = tv;
-            break;
-        }
-    } while (th < F_INF);
-    return res;
+// Traversal state of one ray between passes of the outer loop.
+struct RayState {
+    f3 o, d, inv;
+    float th, tu, tv;
+    int root_id;
+    int nxt, count;            // short stack: ring position and fill
+    int visits;
+};
+
+__device__ __forceinline__ RayState ray_start(const TreeDev &T, int root_id, f3 o, f3 d) {
+    RayState R;
+    R.o = o;
+    R.d = d;
+    R.inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    R.th = 0.0f; R.tu = 0.0f; R.tv = 0.0f;
+    R.root_id = root_id;
+    R.nxt = 0; R.count = 0;
+    R.visits = 0;
+    return R;
 }
 
-// short stack for one ray from node root_id (ShortStack.cu:58-194,
-// ShortStack.h:43-98): a ring of the S most recent far children in LDS,
-// restart from the root when it runs empty
+__device__ __forceinline__ pm_hit ray_record(const RayState &R, int hit) {
+    if (hit == -1) return miss_record();
+    pm_hit h;
+    h.id = hit; h.t = R.th; h.u = R.tu; h.v = R.tv;
+    return h;
+}
+
+// One pass of the tracers' outer loop: start at the root (kd-restart, or
+// short stack with an empty stack) or at the popped far child, descend to
+// the leaf the ray is in at t = th, test the leaf, move th to the leaf's
+// exit.  Returns true when the ray is done; `hit` is its triangle or -1.
+//   kd-restart   Raytracer.cu:57-106 (TraceNode), :165-262 (KDRestart)
+//   short stack  ShortStack.cu:58-194, ShortStack.h:43-98: a ring of the S
+//                most recent far children in LDS (interleaved by lane),
+//                restart from the root when it runs empty
+template <int SMAX, bool SHORT, bool SKIP, bool COUNT, bool WOOP>
+__device__ __forceinline__ bool trace_step(const TreeDev &T, RayState &R, int S, int (*stk_node)[TB],
+                                           float (*stk_t)[TB], int4 &cnt, int &hit) {
+    hit = -1;
+    if (R.visits++ > T.num_nodes) return true;
+    const int lt = threadIdx.x;
+    int node;
+    float tNext;
+    int4 nd;
+    if (!SHORT || R.count == 0) {
+        node = R.root_id; tNext = F_INF; nd = T.packed[R.root_id];
+    } else {                                     // Pop, ShortStack.h:68-72
+        R.nxt = (R.nxt == 0 ? S : R.nxt) - 1;
+        R.count--;
+        node = stk_node[R.nxt][lt];
+        tNext = stk_t[R.nxt][lt];
+        nd = T.packed[node];
+    }
+    int steps = 0;
+    while (nd.w != LEAF) {
+        if (++steps > MAX_DESCENT) return true;
+        if (COUNT) cnt.x++;
+        const float s = __int_as_float(nd.x);
+        const float ori = comp3(R.o, nd.w), dd = comp3(R.inv, nd.w);
+        const float tSplit = (s - ori) * dd;
+        const int lower = 0 < dd ? nd.y : nd.z;
+        const int upper = 0 < dd ? nd.z : nd.y;
+        if (R.th < tSplit) {
+            node = lower;
+            if (SHORT && tSplit < tNext) {           // Push, ShortStack.h:57-62
+                stk_node[R.nxt][lt] = upper;
+                stk_t[R.nxt][lt] = tNext;
+                R.nxt = R.nxt == S - 1 ? 0 : R.nxt + 1;
+                R.count = R.count == S ? S : R.count + 1;
+            }
+            tNext = fminf(tSplit, tNext);
+        } else {
+            node = upper;
+        }
+        nd = T.packed[node];
+    }
+    R.th = tNext * 1.00001f;
+    hit = test_leaf<SKIP, COUNT, WOOP>(T, nd, node, R.o, R.d, R.inv, R.th, R.tu, R.tv, cnt);
+    return hit != -1 || !(R.th < F_INF);
+}
+
+// the whole traversal of one ray (the canvas shading's per-bounce trace)
+template <bool SKIP, bool COUNT, bool WOOP>
+__device__ __forceinline__ pm_hit kd_restart_ray(const TreeDev &T, int root_id, f3 o, f3 d, int4 &cnt) {
+    RayState R = ray_start(T, root_id, o, d);
+    int hit;
+    while (!trace_step<1, false, SKIP, COUNT, WOOP>(T, R, 1, nullptr, nullptr, cnt, hit)) {
+    }
+    return ray_record(R, hit);
+}
+
 template <int SMAX, bool SKIP, bool COUNT, bool WOOP>
-__device__ __forceinline__ pm_hit short_stack_ray(const TreeDev &T, int root_id, f3 o, f3 d, f3 inv, int S,
+__device__ __forceinline__ pm_hit short_stack_ray(const TreeDev &T, int root_id, f3 o, f3 d, int S,
                                                   int (&stk_node)[SMAX][TB], float (&stk_t)[SMAX][TB],
                                                   int4 &cnt) {
-    const int lt = threadIdx.x;
-    pm_hit res = miss_record();
-    float th = 0.0f, tu = 0.0f, tv = 0.0f;
-    int nxt = 0, count = 0;
-    const int4 root = T.packed[root_id];
-    long long visits = 0;
-    do {
-        if (++visits > (long long)T.num_nodes + 1) break;
-        int node;
-        float tNext;
-        int4 nd;
-        if (count == 0) {
-            node = root_id; tNext = F_INF; nd = root;
-        } else {                                     // Pop, ShortStack.h:68-72
-            nxt = (nxt == 0 ? S : nxt) - 1;
-            count--;
-            node = stk_node[nxt][lt];
-            tNext = stk_t[nxt][lt];
-            nd = T.packed[node];
-        }
-        int steps = 0;
-        bool bad = false;
-        while (nd.w != LEAF) {
-            if (++steps > MAX_DESCENT) { bad = true; break; }
-            if (COUNT) cnt.x++;
-            const float s = __int_as_float(nd.x);
-            const float ori = comp3(o, nd.w), dd = comp3(inv, nd.w);
-            const float tSplit = (s - ori) * dd;
-            const int lower = 0 < dd ? nd.y : nd.z;
-            const int upper = 0 < dd ? nd.z : nd.y;
-            if (th < tSplit) {
-                node = lower;
-                if (tSplit < tNext) {                // Push, ShortStack.h:57-62
-                    stk_node[nxt][lt] = upper;
-                    stk_t[nxt][lt] = tNext;
-                    nxt = nxt == S - 1 ? 0 : nxt + 1;
-                    count = count == S ? S : count + 1;
-                }
-                tNext = fminf(tSplit, tNext);
-            } else {
-                node = upper;
-            }
-            nd = T.packed[node];
-        }
-        if (bad) break;
-        th = tNext * 1.00001f;
-        const int hit = test_leaf<SKIP, COUNT, WOOP>(T, nd, node, o, d, inv, th, tu, tv, cnt);
-        if (hit != -1) {
-            res.id = hit; res.t = th; res.u = tu; res.v = tv;
-            break;
-        }
-    } while (th < F_INF);
-    return res;
+    RayState R = ray_start(T, root_id, o, d);
+    int hit;
+    while (!trace_step<SMAX, true, SKIP, COUNT, WOOP>(T, R, S, stk_node, stk_t, cnt, hit)) {
+    }
+    return ray_record(R, hit);
 }
 
-// the first-hit kernels: lane r traces org[r], dir[r] (root ray_root[r])
-// and writes the record to out[order[r]] (out[r] without an order)
+// The first-hit kernels (kd-restart and short stack): persistent waves.
+// A wave takes CHUNK consecutive rays at a time from a global counter
+// (*work, zero at launch); once REFILL_MIN of its lanes are done, they take
+// the next rays of the wave's chunk, so the wave's lanes stay busy while its
+// longest rays finish.  Lane state: the ray's index q and its traversal state; one loop
+// iteration = one pass of trace_step for every busy lane.  The ray read at
+// index q (org, dir, ray_root) writes its record to out[order[q]] (out[q]
+// without an order).  Every wave leaves the loop once the counter has
+// passed n and its lanes are idle; every ray ends (trace_step's visit and
+// descent bounds), so every wave does.
+constexpr int CHUNK = 64;        // rays a wave takes from the counter at a time
+constexpr int REFILL_MIN = 32;   // idle lanes that make a wave take new rays
+
 __device__ __forceinline__ bool ray_root_valid(const TreeDev &T, const int *ray_root, int r, int &root_id) {
     root_id = ray_root ? ray_root[r] : 0;
     // no such mesh, or (captured batch) the build before this trace failed: a miss
     return !(root_id < 0 || root_id >= T.num_roots || (T.fail && *T.fail));
 }
 
-template <bool SKIP, bool COUNT, bool WOOP>
-__global__ __launch_bounds__(TB) void kd_restart_kernel(TreeDev T, const float4 *__restrict__ org,
-                                                        const float4 *__restrict__ dir,
-                                                        const int *__restrict__ ray_root,
-                                                        const unsigned *__restrict__ order, int n,
-                                                        pm_hit *__restrict__ out,
-                                                        int4 *__restrict__ counters) {
-    const int lane_ray = blockIdx.x * TB + threadIdx.x;
-    if (lane_ray >= n) return;
-    const int r = order ? (int)order[lane_ray] : lane_ray;
-    if ((unsigned)r >= (unsigned)n) return;
+template <int SMAX, bool SHORT, bool SKIP, bool COUNT, bool WOOP>
+__global__ __launch_bounds__(TB) void trace_kernel(TreeDev T, const float4 *__restrict__ org,
+                                                   const float4 *__restrict__ dir,
+                                                   const int *__restrict__ ray_root,
+                                                   const unsigned *__restrict__ order, int n, int S,
+                                                   pm_hit *__restrict__ out, int4 *__restrict__ counters,
+                                                   int *__restrict__ work) {
+    __shared__ int stk_node[SHORT ? SMAX : 1][TB];
+    __shared__ float stk_t[SHORT ? SMAX : 1][TB];
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    int next = 0, end = 0;          // the wave's chunk (same in every lane)
+    bool exhausted = false;
+    int q = -1;                     // this lane's ray, -1 = idle
+    RayState R;
     int4 cnt = make_int4(0, 0, 0, 0);
-    int root_id;
-    pm_hit res = miss_record();
-    if (ray_root_valid(T, ray_root, lane_ray, root_id)) {
-        const f3 o = mk3(org[lane_ray]);
-        const f3 d = adjust_dir(mk3(dir[lane_ray]));
-        const f3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
-        res = kd_restart_ray<SKIP, COUNT, WOOP>(T, root_id, o, d, inv, cnt);
+    for (;;) {
+        const unsigned long long idle = __ballot(q < 0);
+        // refill once half the lanes are idle (or all, at the end of the
+        // rays): new rays start at the root while the others are deep in
+        // the tree, so refilling single lanes costs more than it gains
+        if (__popcll(idle) >= REFILL_MIN && !exhausted) {
+            if (next >= end) {
+                int base = 0;
+                if (lane == 0) base = atomicAdd(work, CHUNK);
+                base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
+                if (base >= n || base < 0) {
+                    exhausted = true;
+                } else {
+                    next = base;
+                    end = n - base < CHUNK ? n : base + CHUNK;
+                }
+            }
+            if (!exhausted) {
+                const int avail = end - next;
+                const int rank = __popcll(idle & below);
+                if (q < 0 && rank < avail) {
+                    q = next + rank;
+                    int root_id;
+                    if (ray_root_valid(T, ray_root, q, root_id)) {
+                        R = ray_start(T, root_id, mk3(org[q]), adjust_dir(mk3(dir[q])));
+                        cnt = make_int4(0, 0, 0, 0);
+                    } else {
+                        const int idx = order ? (int)order[q] : q;
+                        if ((unsigned)idx < (unsigned)n) {
+                            out[idx] = miss_record();
+                            if (COUNT) counters[idx] = make_int4(0, 0, 0, 0);
+                        }
+                        q = -1;
+                    }
+                }
+                const int took = __popcll(idle);
+                next = __builtin_amdgcn_readfirstlane(next + (took < avail ? took : avail));
+            }
+        }
+        const bool busy = q >= 0;
+        if (__ballot(busy) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        if (busy) {
+            int hit;
+            if (trace_step<SMAX, SHORT, SKIP, COUNT, WOOP>(T, R, S, stk_node, stk_t, cnt, hit)) {
+                const int idx = order ? (int)order[q] : q;
+                if ((unsigned)idx < (unsigned)n) {
+                    out[idx] = ray_record(R, hit);
+                    if (COUNT) counters[idx] = cnt;
+                }
+                q = -1;
+            }
+        }
     }
-    out[r] = res;
-    if (COUNT) counters[r] = cnt;
-}
-
-template <int SMAX, bool SKIP, bool COUNT, bool WOOP>
-__global__ __launch_bounds__(TB) void short_stack_kernel(TreeDev T, const float4 *__restrict__ org,
-                                                         const float4 *__restrict__ dir,
-                                                         const int *__restrict__ ray_root,
-                                                         const unsigned *__restrict__ order, int n,
-                                                         int S, pm_hit *__restrict__ out,
-                                                         int4 *__restrict__ counters) {
-    __shared__ int stk_node[SMAX][TB];
-    __shared__ float stk_t[SMAX][TB];
-    const int lane_ray = blockIdx.x * TB + threadIdx.x;
-    if (lane_ray >= n) return;
-    const int r = order ? (int)order[lane_ray] : lane_ray;
-    if ((unsigned)r >= (unsigned)n) return;
-    int4 cnt = make_int4(0, 0, 0, 0);
-    int root_id;
-    pm_hit res = miss_record();
-    if (ray_root_valid(T, ray_root, lane_ray, root_id)) {
-        const f3 o = mk3(org[lane_ray]);
-        const f3 d = adjust_dir(mk3(dir[lane_ray]));
-        const f3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
-        res = short_stack_ray<SMAX, SKIP, COUNT, WOOP>(T, root_id, o, d, inv, S, stk_node, stk_t, cnt);
-    }
-    out[r] = res;
-    if (COUNT) counters[r] = cnt;
 }
 
 // Brute force: every lane tests every triangle in index order; triangles are
@@ -453,11 +499,9 @@ __global__ __launch_bounds__(TB) void canvas_kernel(TreeDev T, const float4 *__r
         if constexpr (ALGO == PM_TRACE_BRUTE) {
             h = brute_ray<WOOP>(A, B, C, ntris, o, d);
         } else if constexpr (ALGO == PM_TRACE_KD_RESTART) {
-            const f3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
-            h = kd_restart_ray<SKIP, false, WOOP>(T, 0, o, d, inv, cnt);
+            h = kd_restart_ray<SKIP, false, WOOP>(T, 0, o, d, cnt);
         } else {
-            const f3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
-            h = short_stack_ray<16, SKIP, false, WOOP>(T, 0, o, d, inv, S, stk_node, stk_t, cnt);
+            h = short_stack_ray<16, SKIP, false, WOOP>(T, 0, o, d, S, stk_node, stk_t, cnt);
         }
         if (h.id < 0) break;
         col = blend(col, lighting(h, o, d, n0, n1, n2, c0, L));
@@ -507,34 +551,49 @@ struct RayBatch {
     int n;
 };
 
-template <int SMAX, bool SKIP, bool COUNT, bool WOOP>
-void launch_ss(const TreeDev &T, const RayBatch &R, int S, pm_hit *out, int4 *cnt, hipStream_t s) {
-    const int blocks = (R.n + TB - 1) / TB;
-    short_stack_kernel<SMAX, SKIP, COUNT, WOOP><<<blocks, TB, 0, s>>>(T, R.org, R.dir, R.root, R.order, R.n, S,
-                                                                      out, cnt);
+// persistent grid: as many blocks as the device holds at once (fewer for a
+// small trace)
+struct Launch {
+    int num_cus;
+    int *work;
+};
+
+template <int SMAX, bool SHORT, bool SKIP, bool COUNT, bool WOOP>
+void launch_trace(const TreeDev &T, const RayBatch &R, int S, pm_hit *out, int4 *cnt, const Launch &L,
+                  hipStream_t s) {
+    static int per_cu = 0;     // resident blocks per CU of this instance (same on every gfx950)
+    if (!per_cu) {
+        int b = 0;
+        PM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, trace_kernel<SMAX, SHORT, SKIP, COUNT, WOOP>,
+                                                            TB, 0));
+        per_cu = b > 0 ? b : 1;
+    }
+    const int blocks = (int)std::min<long long>(((long long)R.n + TB - 1) / TB, (long long)per_cu * L.num_cus);
+    trace_kernel<SMAX, SHORT, SKIP, COUNT, WOOP><<<blocks, TB, 0, s>>>(T, R.org, R.dir, R.root, R.order, R.n, S,
+                                                                       out, cnt, L.work);
 }
 
 template <bool SKIP, bool COUNT, bool WOOP>
-void dispatch(const TreeDev &T, int algo, const RayBatch &R, int S, pm_hit *out, int4 *cnt, hipStream_t s) {
-    const int blocks = (R.n + TB - 1) / TB;
+void dispatch(const TreeDev &T, int algo, const RayBatch &R, int S, pm_hit *out, int4 *cnt, const Launch &L,
+              hipStream_t s) {
     if (algo == PM_TRACE_KD_RESTART) {
-        kd_restart_kernel<SKIP, COUNT, WOOP><<<blocks, TB, 0, s>>>(T, R.org, R.dir, R.root, R.order, R.n, out, cnt);
+        launch_trace<1, false, SKIP, COUNT, WOOP>(T, R, 1, out, cnt, L, s);
     } else {
-        if (S <= 4) launch_ss<4, SKIP, COUNT, WOOP>(T, R, S, out, cnt, s);
-        else if (S <= 8) launch_ss<8, SKIP, COUNT, WOOP>(T, R, S, out, cnt, s);
-        else launch_ss<16, SKIP, COUNT, WOOP>(T, R, S, out, cnt, s);
+        if (S <= 4) launch_trace<4, true, SKIP, COUNT, WOOP>(T, R, S, out, cnt, L, s);
+        else if (S <= 8) launch_trace<8, true, SKIP, COUNT, WOOP>(T, R, S, out, cnt, L, s);
+        else launch_trace<16, true, SKIP, COUNT, WOOP>(T, R, S, out, cnt, L, s);
     }
 }
 
 template <bool WOOP>
 void dispatch_flags(const TreeDev &T, bool skip, bool count, int algo, const RayBatch &R, int S, pm_hit *out,
-                    int4 *cnt, hipStream_t s) {
+                    int4 *cnt, const Launch &L, hipStream_t s) {
     if (skip) {
-        if (count) dispatch<true, true, WOOP>(T, algo, R, S, out, cnt, s);
-        else dispatch<true, false, WOOP>(T, algo, R, S, out, cnt, s);
+        if (count) dispatch<true, true, WOOP>(T, algo, R, S, out, cnt, L, s);
+        else dispatch<true, false, WOOP>(T, algo, R, S, out, cnt, L, s);
     } else {
-        if (count) dispatch<false, true, WOOP>(T, algo, R, S, out, cnt, s);
-        else dispatch<false, false, WOOP>(T, algo, R, S, out, cnt, s);
+        if (count) dispatch<false, true, WOOP>(T, algo, R, S, out, cnt, L, s);
+        else dispatch<false, false, WOOP>(T, algo, R, S, out, cnt, L, s);
     }
 }
 
@@ -673,8 +732,18 @@ void trace_rays(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int
         const OrderedRays o = ray_order(ctx, org, dir, ray_mesh, nrays, s);
         R = {o.org, o.dir, o.root, o.index, nrays};
     }
-    if (woop) dispatch_flags<true>(T, skip, counters != nullptr, algo, R, S, out, counters, s);
-    else dispatch_flags<false>(T, skip, counters != nullptr, algo, R, S, out, counters, s);
+    if (!ctx->num_cus) {
+        int v = 0;
+        PM_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, ctx->device));
+        ctx->num_cus = v > 0 ? v : 1;
+    }
+    BuildScratch &sc = ctx->scratch;
+    sc.trace_work.reserve(1, s);
+    clear_int_kernel<<<1, 64, 0, s>>>(sc.trace_work.ptr);
+    PM_LAUNCH_CHECK();
+    const Launch L{ctx->num_cus, sc.trace_work.ptr};
+    if (woop) dispatch_flags<true>(T, skip, counters != nullptr, algo, R, S, out, counters, L, s);
+    else dispatch_flags<false>(T, skip, counters != nullptr, algo, R, S, out, counters, L, s);
     PM_LAUNCH_CHECK();
 }
 
