@@ -34,23 +34,26 @@ __device__ __forceinline__ pm_hit miss_record() {
     return h;
 }
 
-// Moller-Trumbore (IRayTracer.h:111-145) from precomputed edges.
+// Moller-Trumbore (IRayTracer.h:111-145) from precomputed edges.  The same
+// values (t = inv (q.e2), u = inv (p.T), v = inv (q.d), inv = 1/det) and the
+// same acceptance (0 <= t <= th, u >= 0, v >= 0, u + v <= 1), tested in the
+// order that rejects soonest: u first (u <= 1 is implied by v >= 0 and
+// u + v <= 1, since rounding is monotonic), then v, then t.  Rays of a wave
+// that test the same triangle mostly agree, so the wave skips the rest.
 __device__ __forceinline__ bool mt_edges(f3 v0, f3 e1, f3 e2, f3 o, f3 d, float &th, float &tu,
                                          float &tv) {
     const f3 p = cross3(d, e2);
     const f3 t = sub3(o, v0);
-    const f3 q = cross3(t, e1);
     const float inv = rcp_rn(dot3(p, e1));
+    const float hy = inv * dot3(p, t);
+    if (!(hy >= 0.0f && hy <= 1.0f)) return false;
+    const f3 q = cross3(t, e1);
+    const float hz = inv * dot3(q, d);
+    if (!(hz >= 0.0f && hy + hz <= 1.0f)) return false;
     const float hx = inv * dot3(q, e2);
-    if (0.0f <= hx && hx <= th) {
-        const float hy = inv * dot3(p, t);
-        const float hz = inv * dot3(q, d);
-        if (hy >= 0.0f && hz >= 0.0f && hy + hz <= 1.0f) {
-            th = hx; tu = hy; tv = hz;
-            return true;
-        }
-    }
-    return false;
+    if (!(0.0f <= hx && hx <= th)) return false;
+    th = hx; tu = hy; tv = hz;
+    return true;
 }
 
 // Leaf test (Raytracer.cu:194-216): optional box test, then Moller-Trumbore
