@@ -139,21 +139,22 @@ __device__ __forceinline__ pm_hit ray_record(const RayState &R, int hit) {
 
 // One pass of the tracers' outer loop: start at the root (kd-restart, or
 // short stack with an empty stack) or at the popped far child, descend to
-// the leaf the ray is in at t = th, test the leaf, move th to the leaf's
-// exit.  Returns true when the ray is done; `hit` is its triangle or -1.
+// the leaf the ray is in at t = th, move th to the leaf's exit, test the
+// leaf.
 //   kd-restart   Raytracer.cu:57-106 (TraceNode), :165-262 (KDRestart)
 //   short stack  ShortStack.cu:58-194, ShortStack.h:43-98: a ring of the S
 //                most recent far children in LDS (interleaved by lane),
 //                restart from the root when it runs empty
-template <int SMAX, bool SHORT, bool SKIP, bool COUNT, bool WOOP>
-__device__ __forceinline__ bool trace_step(const TreeDev &T, RayState &R, int S, int (*stk_node)[TB],
-                                           float (*stk_t)[TB], int4 &cnt, int &hit) {
-    hit = -1;
-    if (R.visits++ > T.num_nodes) return true;
+// descend() is the inner-node half: it returns false when the ray is done
+// (a miss) before it reaches a leaf; else node/nd are the leaf and R.th its
+// exit.  trace_step() is the whole pass: it returns true when the ray is
+// done, `hit` being its triangle or -1.
+template <int SMAX, bool SHORT, bool COUNT>
+__device__ __forceinline__ bool descend(const TreeDev &T, RayState &R, int S, int (*stk_node)[TB],
+                                        float (*stk_t)[TB], int4 &cnt, int &node, int4 &nd) {
+    if (R.visits++ > T.num_nodes) return false;
     const int lt = threadIdx.x;
-    int node;
     float tNext;
-    int4 nd;
     if (!SHORT || R.count == 0) {
         node = R.root_id; tNext = F_INF; nd = T.packed[R.root_id];
     } else {                                     // Pop, ShortStack.h:68-72
@@ -165,7 +166,7 @@ __device__ __forceinline__ bool trace_step(const TreeDev &T, RayState &R, int S,
     }
     int steps = 0;
     while (nd.w != LEAF) {
-        if (++steps > MAX_DESCENT) return true;
+        if (++steps > MAX_DESCENT) return false;
         if (COUNT) cnt.x++;
         const float s = __int_as_float(nd.x);
         const float ori = comp3(R.o, nd.w), dd = comp3(R.inv, nd.w);
@@ -187,6 +188,16 @@ __device__ __forceinline__ bool trace_step(const TreeDev &T, RayState &R, int S,
         nd = T.packed[node];
     }
     R.th = tNext * 1.00001f;
+    return true;
+}
+
+template <int SMAX, bool SHORT, bool SKIP, bool COUNT, bool WOOP>
+__device__ __forceinline__ bool trace_step(const TreeDev &T, RayState &R, int S, int (*stk_node)[TB],
+                                           float (*stk_t)[TB], int4 &cnt, int &hit) {
+    hit = -1;
+    int node;
+    int4 nd;
+    if (!descend<SMAX, SHORT, COUNT>(T, R, S, stk_node, stk_t, cnt, node, nd)) return true;
     hit = test_leaf<SKIP, COUNT, WOOP>(T, nd, node, R.o, R.d, R.inv, R.th, R.tu, R.tv, cnt);
     return hit != -1 || !(R.th < F_INF);
 }
@@ -301,6 +312,198 @@ __global__ __launch_bounds__(TB) void trace_kernel(TreeDev T, const float4 *__re
                 }
                 q = -1;
             }
+        }
+    }
+}
+
+// ---- cooperative leaf test (Moller-Trumbore tracers)
+//
+// After a wave's lanes have descended to their leaves, lane i holds a leaf
+// with k_i triangles; tested lane by lane the wave would loop max k_i times
+// with most lanes idle.  Instead the wave lists all (lane, triangle) pairs
+// in LDS (COOP_WIN at a time) and every lane tests the next pair, with the
+// owner's ray (read from the owner's registers) and the owner's leaf exit
+// th.  An owner's first hit is the pair with the smallest t, and among
+// equal t the triangle latest in bitmap order -- exactly what the
+// sequential leaf loop (accept 0 <= t <= th, then th = t) ends with -- so
+// each accepted pair offers the key (t bits, ~slot) to its owner's 64-bit
+// LDS minimum, and the owner recomputes its winning triangle itself for
+// t, u, v and the id.  Slots of one leaf grow with the bit index, so
+// ~slot orders ties like the bitmap does; t >= 0 here (-0 keyed as +0).
+constexpr int COOP_WIN = 256;
+
+struct CoopLds {
+    int slot[COOP_WIN];
+    unsigned char owner[COOP_WIN];
+    unsigned long long key[64];
+};
+
+// orders this lane's LDS accesses with the other lanes' of the wave (a
+// wave's LDS instructions execute in order; this keeps the compiler from
+// moving them across)
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// every lane of the wave calls it; returns the lane's winning slot or -1
+__device__ __forceinline__ int coop_leaf(const float4 *__restrict__ L, CoopLds &C, unsigned long long bm,
+                                         int base, f3 o, f3 d, float th) {
+    const int lane = threadIdx.x & 63;
+    C.key[lane] = ~0ull;
+    const int k = __popcll(bm);
+    int incl = k;
+#pragma unroll
+    for (int dl = 1; dl < 64; dl <<= 1) {
+        const int y = __shfl_up(incl, dl, 64);
+        if (lane >= dl) incl += y;
+    }
+    const int W = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
+    int pos = incl - k;                   // this lane's first pair
+    for (int wbase = 0; wbase < W; wbase += COOP_WIN) {
+        const int wn = W - wbase < COOP_WIN ? W - wbase : COOP_WIN;
+        wave_sync();
+        while (bm && pos < wbase + wn) {  // list this lane's pairs of the window
+            const int b = __builtin_ctzll(bm);
+            bm &= bm - 1;
+            C.slot[pos - wbase] = base + b;
+            C.owner[pos - wbase] = (unsigned char)lane;
+            ++pos;
+        }
+        wave_sync();
+        for (int r = 0; r < wn; r += 64) {
+            const int e = r + lane;
+            const bool valid = e < wn;
+            const int sl = valid ? C.slot[e] : 0;
+            const int ow = valid ? (int)C.owner[e] : lane;
+            const f3 oo = {__shfl(o.x, ow, 64), __shfl(o.y, ow, 64), __shfl(o.z, ow, 64)};
+            const f3 dd = {__shfl(d.x, ow, 64), __shfl(d.y, ow, 64), __shfl(d.z, ow, 64)};
+            float tt = __shfl(th, ow, 64), uu, vv;
+            if (valid) {
+                const float4 a0 = L[3ll * sl], a1 = L[3ll * sl + 1], a2 = L[3ll * sl + 2];
+                if (mt_edges(mk3(a0), mk3(a1), mk3(a2), oo, dd, tt, uu, vv)) {
+                    const unsigned long long key =
+                        ((unsigned long long)__float_as_uint(tt + 0.0f) << 32) | (0xffffffffu - (unsigned)sl);
+                    atomicMin(&C.key[ow], key);
+                }
+            }
+        }
+    }
+    wave_sync();
+    const unsigned long long best = C.key[lane];
+    return best == ~0ull ? -1 : (int)(0xffffffffu - (unsigned)(best & 0xffffffffu));
+}
+
+// the persistent kernel of trace_kernel with the leaf test done by the wave
+template <int SMAX, bool SHORT, bool SKIP, bool COUNT>
+__global__ __launch_bounds__(TB) void trace_coop_kernel(TreeDev T, const float4 *__restrict__ org,
+                                                        const float4 *__restrict__ dir,
+                                                        const int *__restrict__ ray_root,
+                                                        const unsigned *__restrict__ order, int n, int S,
+                                                        pm_hit *__restrict__ out,
+                                                        int4 *__restrict__ counters, int *__restrict__ work) {
+    __shared__ int stk_node[SHORT ? SMAX : 1][TB];
+    __shared__ float stk_t[SHORT ? SMAX : 1][TB];
+    __shared__ CoopLds coop[TB / 64];
+    const int lane = threadIdx.x & 63;
+    CoopLds &C = coop[threadIdx.x >> 6];
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    int next = 0, end = 0;          // the wave's chunk (same in every lane)
+    bool exhausted = false;
+    int q = -1;                     // this lane's ray, -1 = idle
+    RayState R;
+    R.o = {0.0f, 0.0f, 0.0f}; R.d = R.o; R.th = 0.0f;
+    int4 cnt = make_int4(0, 0, 0, 0);
+    for (;;) {
+        const unsigned long long idle = __ballot(q < 0);
+        if (__popcll(idle) >= REFILL_MIN && !exhausted) {
+            if (next >= end) {
+                int b = 0;
+                if (lane == 0) b = atomicAdd(work, CHUNK);
+                b = __builtin_amdgcn_readfirstlane(__shfl(b, 0, 64));
+                if (b >= n || b < 0) {
+                    exhausted = true;
+                } else {
+                    next = b;
+                    end = n - b < CHUNK ? n : b + CHUNK;
+                }
+            }
+            if (!exhausted) {
+                const int avail = end - next;
+                const int rank = __popcll(idle & below);
+                if (q < 0 && rank < avail) {
+                    q = next + rank;
+                    int root_id;
+                    if (ray_root_valid(T, ray_root, q, root_id)) {
+                        R = ray_start(T, root_id, mk3(org[q]), adjust_dir(mk3(dir[q])));
+                        cnt = make_int4(0, 0, 0, 0);
+                    } else {
+                        const int idx = order ? (int)order[q] : q;
+                        if ((unsigned)idx < (unsigned)n) {
+                            out[idx] = miss_record();
+                            if (COUNT) counters[idx] = make_int4(0, 0, 0, 0);
+                        }
+                        q = -1;
+                    }
+                }
+                const int took = __popcll(idle);
+                next = __builtin_amdgcn_readfirstlane(next + (took < avail ? took : avail));
+            }
+        }
+        if (__ballot(q >= 0) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        // inner nodes, lane by lane
+        bool done = false;
+        unsigned long long bm = 0;
+        int base = 0;
+        if (q >= 0) {
+            int node;
+            int4 nd;
+            if (!descend<SMAX, SHORT, COUNT>(T, R, S, stk_node, stk_t, cnt, node, nd)) {
+                done = true;
+            } else {
+                if (COUNT) cnt.y++;
+                bool enter = true;
+                if (SKIP) {
+                    if (COUNT) cnt.w++;
+                    enter = ray_box(R.o, R.inv, mk3(T.bmin[node]), mk3(T.bmax[node]));
+                }
+                if (enter) {
+                    bm = ((unsigned long long)(unsigned)nd.z << 32) | (unsigned)nd.y;
+                    base = nd.x;
+                    if (COUNT) cnt.z += __popcll(bm);
+                }
+            }
+        }
+        // the leaves, by the whole wave
+        const int win = coop_leaf(T.leaf_tris, C, bm, base, R.o, R.d, R.th);
+        if (q >= 0 && !done) {
+            int hit = -1;
+            if (win >= 0) {             // the owner's own test of its winner
+                const float4 *W = T.leaf_tris + 3ll * win;
+                const float4 a0 = W[0];
+                if (mt_edges(mk3(a0), mk3(W[1]), mk3(W[2]), R.o, R.d, R.th, R.tu, R.tv))
+                    hit = __float_as_int(a0.w);
+            }
+            done = hit != -1 || !(R.th < F_INF);
+            if (done) {
+                const int idx = order ? (int)order[q] : q;
+                if ((unsigned)idx < (unsigned)n) {
+                    out[idx] = ray_record(R, hit);
+                    if (COUNT) counters[idx] = cnt;
+                }
+                q = -1;
+            }
+        } else if (q >= 0) {            // done before a leaf: a miss
+            const int idx = order ? (int)order[q] : q;
+            if ((unsigned)idx < (unsigned)n) {
+                out[idx] = miss_record();
+                if (COUNT) counters[idx] = cnt;
+            }
+            q = -1;
         }
     }
 }
@@ -561,19 +764,27 @@ struct Launch {
     int *work;
 };
 
+template <class K>
+int resident_blocks(K kernel, const RayBatch &R, const Launch &L) {
+    int b = 0;
+    PM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kernel, TB, 0));
+    return (int)std::min<long long>(((long long)R.n + TB - 1) / TB, (long long)(b > 0 ? b : 1) * L.num_cus);
+}
+
+#ifndef PM_COOP
+#define PM_COOP 1
+#endif
+
 template <int SMAX, bool SHORT, bool SKIP, bool COUNT, bool WOOP>
 void launch_trace(const TreeDev &T, const RayBatch &R, int S, pm_hit *out, int4 *cnt, const Launch &L,
                   hipStream_t s) {
-    static int per_cu = 0;     // resident blocks per CU of this instance (same on every gfx950)
-    if (!per_cu) {
-        int b = 0;
-        PM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, trace_kernel<SMAX, SHORT, SKIP, COUNT, WOOP>,
-                                                            TB, 0));
-        per_cu = b > 0 ? b : 1;
+    if (!WOOP && PM_COOP) {
+        auto k = trace_coop_kernel<SMAX, SHORT, SKIP, COUNT>;
+        k<<<resident_blocks(k, R, L), TB, 0, s>>>(T, R.org, R.dir, R.root, R.order, R.n, S, out, cnt, L.work);
+    } else {
+        auto k = trace_kernel<SMAX, SHORT, SKIP, COUNT, WOOP>;
+        k<<<resident_blocks(k, R, L), TB, 0, s>>>(T, R.org, R.dir, R.root, R.order, R.n, S, out, cnt, L.work);
     }
-    const int blocks = (int)std::min<long long>(((long long)R.n + TB - 1) / TB, (long long)per_cu * L.num_cus);
-    trace_kernel<SMAX, SHORT, SKIP, COUNT, WOOP><<<blocks, TB, 0, s>>>(T, R.org, R.dir, R.root, R.order, R.n, S,
-                                                                       out, cnt, L.work);
 }
 
 template <bool SKIP, bool COUNT, bool WOOP>
