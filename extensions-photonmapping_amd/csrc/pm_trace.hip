@@ -330,11 +330,10 @@ __global__ __launch_bounds__(TB) void trace_kernel(TreeDev T, const float4 *__re
 // LDS minimum, and the owner recomputes its winning triangle itself for
 // t, u, v and the id.  Slots of one leaf grow with the bit index, so
 // ~slot orders ties like the bitmap does; t >= 0 here (-0 keyed as +0).
-constexpr int COOP_WIN = 256;
+constexpr int COOP_WIN = 1024;
 
 struct CoopLds {
-    int slot[COOP_WIN];
-    unsigned char owner[COOP_WIN];
+    unsigned short pair[COOP_WIN];       // bit << 6 | owner lane
     unsigned long long key[64];
 };
 
@@ -361,25 +360,31 @@ __device__ __forceinline__ int coop_leaf(const float4 *__restrict__ L, CoopLds &
     }
     const int W = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
     int pos = incl - k;                   // this lane's first pair
+    unsigned lo = (unsigned)bm, hi = (unsigned)(bm >> 32);
     for (int wbase = 0; wbase < W; wbase += COOP_WIN) {
         const int wn = W - wbase < COOP_WIN ? W - wbase : COOP_WIN;
         wave_sync();
-        while (bm && pos < wbase + wn) {  // list this lane's pairs of the window
-            const int b = __builtin_ctzll(bm);
-            bm &= bm - 1;
-            C.slot[pos - wbase] = base + b;
-            C.owner[pos - wbase] = (unsigned char)lane;
-            ++pos;
+        // list this lane's pairs of the window, bits in ascending order
+        while (lo && pos < wbase + wn) {
+            const int b = __builtin_ctz(lo);
+            lo &= lo - 1;
+            C.pair[pos++ - wbase] = (unsigned short)(b << 6 | lane);
+        }
+        while (hi && pos < wbase + wn) {
+            const int b = __builtin_ctz(hi) + 32;
+            hi &= hi - 1;
+            C.pair[pos++ - wbase] = (unsigned short)(b << 6 | lane);
         }
         wave_sync();
         for (int r = 0; r < wn; r += 64) {
             const int e = r + lane;
             const bool valid = e < wn;
-            const int sl = valid ? C.slot[e] : 0;
-            const int ow = valid ? (int)C.owner[e] : lane;
+            const int pr = valid ? (int)C.pair[e] : lane;
+            const int ow = pr & 63;
             const f3 oo = {__shfl(o.x, ow, 64), __shfl(o.y, ow, 64), __shfl(o.z, ow, 64)};
             const f3 dd = {__shfl(d.x, ow, 64), __shfl(d.y, ow, 64), __shfl(d.z, ow, 64)};
             float tt = __shfl(th, ow, 64), uu, vv;
+            const int sl = __shfl(base, ow, 64) + (pr >> 6);
             if (valid) {
                 const float4 a0 = L[3ll * sl], a1 = L[3ll * sl + 1], a2 = L[3ll * sl + 2];
                 if (mt_edges(mk3(a0), mk3(a1), mk3(a2), oo, dd, tt, uu, vv)) {
