@@ -233,6 +233,8 @@ def main() -> int:
                     help="rays the CPU baseline traces (0 disables the baseline)")
     ap.add_argument("--meshes", type=int, default=256,
                     help="meshes of the batched-rebuild comparison (0 skips it)")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the batched-rebuild comparison and print its record (profiling runs)")
     ap.add_argument("--probe", action="store_true",
                     help="counter runs: one build, one short-stack trace of the emission batch and "
                          "a 256 MiB device copy (the HBM-counter calibration), then exit")
@@ -251,6 +253,9 @@ def main() -> int:
     scenes = importlib.import_module("extensions-photonmapping_amd.scenes")
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.batched_only:
+        print(json.dumps({"batched": bench_batched(pm, scenes, torch, meshes=args.meshes)}), flush=True)
+        return 0
 
     t0 = time.time()
     tris = scenes.cluster_soup(args.tris)

@@ -1457,9 +1457,6 @@ std::vector<void **> graph_buffers(pm_ctx *ctx) {
     }
     v.push_back((void **)&sc.order_hist.ptr);
     v.push_back((void **)&sc.order_offs.ptr);
-    v.push_back((void **)&sc.order_org.ptr);
-    v.push_back((void **)&sc.order_dir.ptr);
-    v.push_back((void **)&sc.order_root.ptr);
     v.push_back((void **)&sc.trace_work.ptr);
     return v;
 }
@@ -1530,7 +1527,7 @@ void capture_batch(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4
     sc.lower_careas.reserve(2 * (size_t)caps.lower, s);
     ctx->packed.reserve((size_t)caps.nodes + 1, s);
     reserve_scan_tmp(sc, std::max({2 * caps.active, caps.segs, caps.lower}), s);
-    if (ray_order_applies(tp, nrays)) reserve_ray_order(ctx, nrays, ray_mesh != nullptr, s);
+    if (ray_order_applies(tp, nrays)) reserve_ray_order(ctx, nrays, s);
     sc.trace_work.reserve(1, s);
     sc.lvl.reserve(1, s);
     // the batch's own copy of its mesh offsets (eager builds rewrite the scratch copy)

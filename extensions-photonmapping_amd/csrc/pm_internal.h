@@ -152,8 +152,6 @@ struct BuildScratch {
     DevBuf<int> bwd_hist, bwd_offs;             // backward: radix histograms
     DevBuf<unsigned> order_keys[2], order_vals[2];   // coherent ray order: radix sort ping-pong
     DevBuf<int> order_hist, order_offs;         // coherent ray order: radix histograms
-    DevBuf<float4> order_org, order_dir;        // coherent ray order: the rays in that order
-    DevBuf<int> order_root;                     // coherent ray order: their roots (forest)
     DevBuf<int> trace_work;                     // tracers: rays handed out so far
     DevBuf<int> mesh_off;                       // forest: triangle offsets per mesh (M+1)
     DevBuf<Lvl> lvl;                            // level state (one record)
@@ -270,16 +268,11 @@ int radix_sort_pairs(unsigned *const keys[2], unsigned *const vals[2], int n, in
                      bool first_hist_ready, bool iota, BuildScratch &sc, hipStream_t s);
 size_t radix_hist_size(long long n);
 // coherent ray order: whether a trace of nrays rays uses it, its buffers
-// (reserved ahead of a graph capture), and the rays in that order: lane r
-// traces org[r], dir[r] (root[r]) and writes to index[r] (device arrays,
-// valid until the next call on ctx)
-struct OrderedRays {
-    const float4 *org, *dir;
-    const int *root;
-    const unsigned *index;
-};
+// (reserved ahead of a graph capture), and the order itself (device array:
+// the ray indices in the order to trace them, valid until the next call on
+// ctx)
 bool ray_order_applies(const pm_trace_params &prm, int nrays);
-void reserve_ray_order(pm_ctx *ctx, int nrays, bool roots, hipStream_t s);
-OrderedRays ray_order(pm_ctx *ctx, const float4 *org, const float4 *dir, const int *ray_root, int nrays,
-                      hipStream_t s);
+void reserve_ray_order(pm_ctx *ctx, int nrays, hipStream_t s);
+const unsigned *ray_order(pm_ctx *ctx, const float4 *org, const float4 *dir, const int *ray_root, int nrays,
+                          hipStream_t s);
 }  // namespace pmkd

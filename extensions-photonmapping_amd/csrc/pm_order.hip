@@ -6,11 +6,10 @@
 // and triangle reads over the whole tree.  Before a large trace the rays are
 // put in the order of a 32-bit spatial key -- the Morton code of the origin
 // cell (6 bits per axis, relative to the ray's root box) above the Morton
-// code of the octahedral direction cell (7 bits per axis) -- the last sort
-// pass also gathers the rays (and their roots) into that order, and lane r
-// traces the r-th ray of it, writing to the ray's index order[r].  Every
-// ray is still traced on its own, from the same
-// inputs, by the same code, and its record is written to its own index, so
+// code of the octahedral direction cell (7 bits per axis) -- and the
+// tracers hand the rays to their lanes in that order.  Every ray is still
+// traced on its own, from the same inputs, by the same code, and its record
+// is written to its own index, so
 // hits and counters do not depend on the order (the tracers' records are
 // bit-identical either way).  In a forest the root index sits above the
 // spatial bits.  Not in the reference (its tracers run rays in pixel order,
@@ -165,22 +164,14 @@ __global__ __launch_bounds__(TB) void radix_digit_scan_kernel(const int *__restr
 
 // one stable pass: tile b's keys with digit d go to
 // (sum of the totals of digits < d) + offs[d * nb + b] onwards, in tile
-// order.  IOTA: the values are the keys' input indices.  GATHER (last
-// pass): the rays the values name are copied into the output order too.
-struct RayGather {
-    const float4 *org, *dir;
-    const int *root;
-    float4 *org_out, *dir_out;
-    int *root_out;
-};
-
-template <bool IOTA, bool GATHER>
+// order.  IOTA: the values are the keys' input indices.
+template <bool IOTA>
 __global__ __launch_bounds__(TB) void radix_scatter_kernel(const unsigned *__restrict__ kin,
                                                            const unsigned *__restrict__ vin, int n, int shift,
                                                            int nb, const int *__restrict__ offs,
                                                            const int *__restrict__ dtot,
                                                            unsigned *__restrict__ kout,
-                                                           unsigned *__restrict__ vout, RayGather G) {
+                                                           unsigned *__restrict__ vout) {
     __shared__ int wcnt[TB / 64][256];     // per wave: keys of each digit so far, then the wave's start
     __shared__ int dstart[256];            // digit start inside the staged tile
     __shared__ int gbase[256];             // digit start in the output
@@ -270,14 +261,8 @@ __global__ __launch_bounds__(TB) void radix_scatter_kernel(const unsigned *__res
         const unsigned key = skey[i];
         const unsigned dg = (key >> shift) & 255u;
         const int g = gbase[dg] + (i - dstart[dg]);
-        const unsigned val = sval[i];
         kout[g] = key;
-        vout[g] = val;
-        if (GATHER) {
-            G.org_out[g] = G.org[val];
-            G.dir_out[g] = G.dir[val];
-            if (G.root) G.root_out[g] = G.root[val];
-        }
+        vout[g] = sval[i];
     }
 }
 
@@ -285,15 +270,12 @@ int sort_tiles(long long n) { return (int)((n + TILE - 1) / TILE); }
 
 }  // namespace
 
-namespace {
-
-int sort_passes(unsigned *const keys[2], unsigned *const vals[2], int n, int bits, int *hist, int *offs,
-                bool first_hist_ready, bool iota, const RayGather *gather, BuildScratch &sc, hipStream_t s) {
+int radix_sort_pairs(unsigned *const keys[2], unsigned *const vals[2], int n, int bits, int *hist, int *offs,
+                     bool first_hist_ready, bool iota, BuildScratch &sc, hipStream_t s) {
     if (n <= 0) return 0;
     const int nb = sort_tiles(n);
     int *dtot = offs + 256 * (size_t)nb;        // 256 digit totals after the offsets
     int cur = 0;
-    RayGather none{};
     for (int shift = 0; shift < bits; shift += 8) {
         if (shift > 0 || !first_hist_ready) {
             radix_hist_kernel<<<nb, TB, 0, s>>>(keys[cur], n, shift, nb, hist);
@@ -301,33 +283,16 @@ int sort_passes(unsigned *const keys[2], unsigned *const vals[2], int n, int bit
         }
         radix_digit_scan_kernel<<<256, TB, 0, s>>>(hist, nb, offs, dtot);
         PM_LAUNCH_CHECK();
-        const bool first = shift == 0, last = shift + 8 >= bits;
-        unsigned *ko = keys[cur ^ 1], *vo = vals[cur ^ 1];
-        if (last && gather) {
-            if (iota && first)
-                radix_scatter_kernel<true, true><<<nb, TB, 0, s>>>(keys[cur], vals[cur], n, shift, nb, offs, dtot,
-                                                                   ko, vo, *gather);
-            else
-                radix_scatter_kernel<false, true><<<nb, TB, 0, s>>>(keys[cur], vals[cur], n, shift, nb, offs,
-                                                                    dtot, ko, vo, *gather);
-        } else if (iota && first) {
-            radix_scatter_kernel<true, false><<<nb, TB, 0, s>>>(keys[cur], vals[cur], n, shift, nb, offs, dtot,
-                                                                ko, vo, none);
-        } else {
-            radix_scatter_kernel<false, false><<<nb, TB, 0, s>>>(keys[cur], vals[cur], n, shift, nb, offs, dtot,
-                                                                 ko, vo, none);
-        }
+        if (iota && shift == 0)
+            radix_scatter_kernel<true><<<nb, TB, 0, s>>>(keys[cur], vals[cur], n, shift, nb, offs, dtot,
+                                                         keys[cur ^ 1], vals[cur ^ 1]);
+        else
+            radix_scatter_kernel<false><<<nb, TB, 0, s>>>(keys[cur], vals[cur], n, shift, nb, offs, dtot,
+                                                          keys[cur ^ 1], vals[cur ^ 1]);
         PM_LAUNCH_CHECK();
         cur ^= 1;
     }
     return cur;
-}
-
-}  // namespace
-
-int radix_sort_pairs(unsigned *const keys[2], unsigned *const vals[2], int n, int bits, int *hist, int *offs,
-                     bool first_hist_ready, bool iota, BuildScratch &sc, hipStream_t s) {
-    return sort_passes(keys, vals, n, bits, hist, offs, first_hist_ready, iota, nullptr, sc, s);
 }
 
 size_t radix_hist_size(long long n) { return 256 * (size_t)sort_tiles(n) + 256; }
@@ -336,7 +301,7 @@ bool ray_order_applies(const pm_trace_params &prm, int nrays) {
     return prm.ray_order == PM_RAY_ORDER_COHERENT && nrays >= ORDER_MIN_RAYS;
 }
 
-void reserve_ray_order(pm_ctx *ctx, int nrays, bool roots, hipStream_t s) {
+void reserve_ray_order(pm_ctx *ctx, int nrays, hipStream_t s) {
     BuildScratch &sc = ctx->scratch;
     for (int i = 0; i < 2; ++i) {
         sc.order_keys[i].reserve((size_t)nrays + 1, s);
@@ -344,15 +309,12 @@ void reserve_ray_order(pm_ctx *ctx, int nrays, bool roots, hipStream_t s) {
     }
     sc.order_hist.reserve(radix_hist_size(nrays), s);
     sc.order_offs.reserve(radix_hist_size(nrays), s);
-    sc.order_org.reserve((size_t)nrays + 1, s);
-    sc.order_dir.reserve((size_t)nrays + 1, s);
-    if (roots) sc.order_root.reserve((size_t)nrays + 1, s);
 }
 
-OrderedRays ray_order(pm_ctx *ctx, const float4 *org, const float4 *dir, const int *ray_root, int nrays,
-                      hipStream_t s) {
+const unsigned *ray_order(pm_ctx *ctx, const float4 *org, const float4 *dir, const int *ray_root, int nrays,
+                          hipStream_t s) {
     BuildScratch &sc = ctx->scratch;
-    reserve_ray_order(ctx, nrays, ray_root != nullptr, s);
+    reserve_ray_order(ctx, nrays, s);
     const int nb = sort_tiles(nrays);
     int root_bits = 0;
     while (root_bits < 31 && ((unsigned)(ctx->num_roots - 1) >> root_bits)) ++root_bits;
@@ -361,11 +323,7 @@ OrderedRays ray_order(pm_ctx *ctx, const float4 *org, const float4 *dir, const i
     PM_LAUNCH_CHECK();
     unsigned *const K[2] = {sc.order_keys[0].ptr, sc.order_keys[1].ptr};
     unsigned *const V[2] = {sc.order_vals[0].ptr, sc.order_vals[1].ptr};
-    const RayGather G{org, dir, ray_root, sc.order_org.ptr, sc.order_dir.ptr,
-                      ray_root ? sc.order_root.ptr : nullptr};
-    const int cur = sort_passes(K, V, nrays, ORDER_BITS, sc.order_hist.ptr, sc.order_offs.ptr, true, true, &G,
-                                sc, s);
-    return {sc.order_org.ptr, sc.order_dir.ptr, ray_root ? sc.order_root.ptr : nullptr, V[cur]};
+    return V[radix_sort_pairs(K, V, nrays, ORDER_BITS, sc.order_hist.ptr, sc.order_offs.ptr, true, true, sc, s)];
 }
 
 }  // namespace pmkd

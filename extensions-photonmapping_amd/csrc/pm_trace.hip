@@ -227,10 +227,11 @@ __device__ __forceinline__ pm_hit short_stack_ray(const TreeDev &T, int root_id,
 // A wave takes CHUNK consecutive rays at a time from a global counter
 // (*work, zero at launch); once REFILL_MIN of its lanes are done, they take
 // the next rays of the wave's chunk, so the wave's lanes stay busy while its
-// longest rays finish.  Lane state: the ray's index q and its traversal state; one loop
-// iteration = one pass of trace_step for every busy lane.  The ray read at
-// index q (org, dir, ray_root) writes its record to out[order[q]] (out[q]
-// without an order).  Every wave leaves the loop once the counter has
+// longest rays finish.  Lane state: the ray's index q and its traversal
+// state; one loop iteration = one pass of trace_step for every busy lane.
+// The rays are handed out in the order `order` lists them (input order
+// without one); ray q reads org[q], dir[q], ray_root[q] and writes out[q].
+// Every wave leaves the loop once the counter has
 // passed n and its lanes are idle; every ray ends (trace_step's visit and
 // descent bounds), so every wave does.
 constexpr int CHUNK = 64;        // rays a wave takes from the counter at a time
@@ -279,17 +280,16 @@ __global__ __launch_bounds__(TB) void trace_kernel(TreeDev T, const float4 *__re
                 const int avail = end - next;
                 const int rank = __popcll(idle & below);
                 if (q < 0 && rank < avail) {
-                    q = next + rank;
+                    q = order ? (int)order[next + rank] : next + rank;
                     int root_id;
-                    if (ray_root_valid(T, ray_root, q, root_id)) {
+                    if ((unsigned)q >= (unsigned)n) {
+                        q = -1;                  // not a ray index (never for a valid order)
+                    } else if (ray_root_valid(T, ray_root, q, root_id)) {
                         R = ray_start(T, root_id, mk3(org[q]), adjust_dir(mk3(dir[q])));
                         cnt = make_int4(0, 0, 0, 0);
                     } else {
-                        const int idx = order ? (int)order[q] : q;
-                        if ((unsigned)idx < (unsigned)n) {
-                            out[idx] = miss_record();
-                            if (COUNT) counters[idx] = make_int4(0, 0, 0, 0);
-                        }
+                        out[q] = miss_record();
+                        if (COUNT) counters[q] = make_int4(0, 0, 0, 0);
                         q = -1;
                     }
                 }
@@ -305,11 +305,8 @@ __global__ __launch_bounds__(TB) void trace_kernel(TreeDev T, const float4 *__re
         if (busy) {
             int hit;
             if (trace_step<SMAX, SHORT, SKIP, COUNT, WOOP>(T, R, S, stk_node, stk_t, cnt, hit)) {
-                const int idx = order ? (int)order[q] : q;
-                if ((unsigned)idx < (unsigned)n) {
-                    out[idx] = ray_record(R, hit);
-                    if (COUNT) counters[idx] = cnt;
-                }
+                out[q] = ray_record(R, hit);
+                if (COUNT) counters[q] = cnt;
                 q = -1;
             }
         }
@@ -438,17 +435,16 @@ __global__ __launch_bounds__(TB) void trace_coop_kernel(TreeDev T, const float4 
                 const int avail = end - next;
                 const int rank = __popcll(idle & below);
                 if (q < 0 && rank < avail) {
-                    q = next + rank;
+                    q = order ? (int)order[next + rank] : next + rank;
                     int root_id;
-                    if (ray_root_valid(T, ray_root, q, root_id)) {
+                    if ((unsigned)q >= (unsigned)n) {
+                        q = -1;                  // not a ray index (never for a valid order)
+                    } else if (ray_root_valid(T, ray_root, q, root_id)) {
                         R = ray_start(T, root_id, mk3(org[q]), adjust_dir(mk3(dir[q])));
                         cnt = make_int4(0, 0, 0, 0);
                     } else {
-                        const int idx = order ? (int)order[q] : q;
-                        if ((unsigned)idx < (unsigned)n) {
-                            out[idx] = miss_record();
-                            if (COUNT) counters[idx] = make_int4(0, 0, 0, 0);
-                        }
+                        out[q] = miss_record();
+                        if (COUNT) counters[q] = make_int4(0, 0, 0, 0);
                         q = -1;
                     }
                 }
@@ -495,19 +491,13 @@ __global__ __launch_bounds__(TB) void trace_coop_kernel(TreeDev T, const float4 
             }
             done = hit != -1 || !(R.th < F_INF);
             if (done) {
-                const int idx = order ? (int)order[q] : q;
-                if ((unsigned)idx < (unsigned)n) {
-                    out[idx] = ray_record(R, hit);
-                    if (COUNT) counters[idx] = cnt;
-                }
+                out[q] = ray_record(R, hit);
+                if (COUNT) counters[q] = cnt;
                 q = -1;
             }
         } else if (q >= 0) {            // done before a leaf: a miss
-            const int idx = order ? (int)order[q] : q;
-            if ((unsigned)idx < (unsigned)n) {
-                out[idx] = miss_record();
-                if (COUNT) counters[idx] = cnt;
-            }
+            out[q] = miss_record();
+            if (COUNT) counters[q] = cnt;
             q = -1;
         }
     }
@@ -758,7 +748,7 @@ __global__ void clear_int_kernel(int *p) {
 struct RayBatch {
     const float4 *org, *dir;
     const int *root;            // per-ray root (forest) or null
-    const unsigned *order;      // lane -> index of its record, or null
+    const unsigned *order;      // the order the rays are traced in, or null
     int n;
 };
 
@@ -947,10 +937,7 @@ void trace_rays(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int
     const bool skip = prm.leaf_skipping && ctx->has_boxes;
     const int S = prm.stack_size;
     RayBatch R{org, dir, ray_mesh, nullptr, nrays};
-    if (ray_order_applies(prm, nrays)) {
-        const OrderedRays o = ray_order(ctx, org, dir, ray_mesh, nrays, s);
-        R = {o.org, o.dir, o.root, o.index, nrays};
-    }
+    if (ray_order_applies(prm, nrays)) R.order = ray_order(ctx, org, dir, ray_mesh, nrays, s);
     if (!ctx->num_cus) {
         int v = 0;
         PM_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, ctx->device));
