@@ -366,10 +366,11 @@ def main() -> int:
     # HBM bytes per launch of the same kernel on the same workload, from the
     # calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE runs committed in profiles/
     traffic = None
-    pmc_path = os.path.join(ROOT, "profiles", "short_stack_c3_pmc.json")
+    pmc_path = os.path.join(ROOT, "profiles", "trace_c3_pmc.json")
     if os.path.exists(pmc_path):
         pmc = json.load(open(pmc_path))
-        if (pmc.get("rays"), pmc.get("tris"), pmc.get("stack")) == (n0, ntris, args.stack):
+        if (pmc.get("rays"), pmc.get("tris"), pmc.get("stack"), pmc.get("ray_order")) == \
+                (n0, ntris, args.stack, "coherent"):
             traffic = pmc["traffic_bytes"]
 
     # ---- other tracers on the same tree (same GPU): kd-restart, brute force
