@@ -352,6 +352,23 @@ def main() -> int:
     input_mrays = sum(t_in["rays"]) / (sum(t_in["trace_ms"]) * 1e-3) / 1e6
     log(f"[bench] input ray order: {input_mrays:.1f} Mrays/s")
 
+    # the same rebuild recorded once as a hipGraph (no host read-backs) and replayed
+    gctx = pm.Context(0)
+    off1 = np.array([0, ntris], np.int32)
+    gctx.batch_graph_capture(*P, off1, None, None, None, None, pm.SHORT_STACK, bp)
+    gb = []
+    for i in range(args.steps + 1):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record(stream)
+        gctx.batch_graph_launch()
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        if i:
+            gb.append(ev[0].elapsed_time(ev[1]))
+    assert gctx.batch_graph_status() == 0
+    build_graph_ms = float(np.median(gb))
+    gctx.close()
+
     # ---- roofline of the dominant kernel: short-stack trace of the emission batch
     ctx.build(*P, bp)
     cnt = torch.zeros((n0, 4), dtype=torch.int32, device=dev)
@@ -437,6 +454,7 @@ def main() -> int:
             "rays_per_pass": timing["pass_rays"],
             "kd_build_ms": round(build_ms, 3),
             "kd_build_wall_ms": round(float(np.median(timing["build_wall_ms"])), 3),
+            "kd_build_graph_ms": round(build_graph_ms, 3),
             "trace_ms_per_step": round(trace_ms_total / args.steps, 3),
             "ray_order": "coherent (spatial-key radix sort inside each trace call, included in the times)",
             "input_order_mrays": round(input_mrays, 2),

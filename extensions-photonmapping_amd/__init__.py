@@ -304,8 +304,10 @@ class Context:
                             caps: dict | None = None, stream=None) -> dict:
         """record build + trace of this batch as one hipGraph (pm_batch_graph_capture);
         returns the capacities captured.  The tensors must stay alive while the
-        graph is replayed."""
+        graph is replayed.  org=None records the build alone (no rays)."""
         off = np.ascontiguousarray(mesh_offsets, dtype=np.int32)
+        nrays = 0 if org is None else org.shape[0]
+        ptr = (lambda t: _dev_ptr(t) if nrays else None)
         build = build or build_params()
         trace = trace or trace_params()
         c = GraphCaps()
@@ -315,7 +317,7 @@ class Context:
         self._graph_keep = (p0, p1, p2, org, dirs, ray_mesh, out, off)
         self._check(self._L.pm_batch_graph_capture(
             self.h, _dev_ptr(p0), _dev_ptr(p1), _dev_ptr(p2), p0.shape[0], off.ctypes.data, off.shape[0] - 1,
-            _dev_ptr(org), _dev_ptr(dirs), _dev_ptr(ray_mesh), org.shape[0], _dev_ptr(out), algo,
+            ptr(org), ptr(dirs), ptr(ray_mesh), nrays, ptr(out), algo,
             C.byref(build), C.byref(trace), C.byref(c), _stream_ptr(stream)))
         return {k: getattr(c, k) for k, _ in GraphCaps._fields_}
 

@@ -344,6 +344,7 @@ int pm_trace(pm_ctx *ctx, int algo, const float *org, const float *dir, int nray
         pm_trace_params prm;
         if (params) prm = *params; else pm_trace_params_default(&prm);
         check_trace_params(prm);
+        pmkd::sync_graph_state(ctx, (hipStream_t)stream);   // after a replay: its sizes and status
         if (algo == PM_TRACE_BRUTE) {
             if (ctx->ntris > 0 && !ctx->p0) throw Error{PM_ERR_NO_TREE, "no geometry"};
         } else if (!ctx->has_tree) {
@@ -364,6 +365,7 @@ int pm_trace_batched(pm_ctx *ctx, int algo, const float *org, const float *dir, 
         pm_trace_params prm;
         if (params) prm = *params; else pm_trace_params_default(&prm);
         check_trace_params(prm);
+        pmkd::sync_graph_state(ctx, (hipStream_t)stream);   // after a replay: its sizes and status
         if (!ctx->has_tree) throw Error{PM_ERR_NO_TREE, "no tree: call pm_build_batched first"};
         pmkd::trace_rays(ctx, algo, reinterpret_cast<const float4 *>(org), reinterpret_cast<const float4 *>(dir),
                          nrays, ray_mesh, out, prm, reinterpret_cast<int4 *>(counters), (hipStream_t)stream);

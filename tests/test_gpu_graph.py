@@ -117,3 +117,33 @@ def test_capacity_errors(pm, scenes, torch):
     assert e.value.code == 8
     with pytest.raises(pm.PmError):                   # BRUTE is not a batched tracer
         ctx.batch_graph_capture(*P, off, O, D, RM, H, pm.BRUTE)
+
+
+def test_build_only_graph_then_eager_traces(pm, orc, scenes, torch):
+    """a graph of the build alone (no rays): each replay rebuilds from the
+    current triangles; eager traces after it see the replayed tree; a
+    replay that outgrows its capacities makes the next trace fail cleanly"""
+    tris = scenes.random_soup(20000, seed=41, size=0.02)
+    org, dirs = scenes.random_rays(30000, seed=42)
+    P = [dev(torch, t) for t in tris]
+    O, D = dev(torch, org), dev(torch, dirs)
+    bp = pm.build_params("reference")
+    off = np.array([0, tris[0].shape[0]], np.int32)
+    ctx = pm.Context(0)
+    ctx.batch_graph_capture(*P, off, None, None, None, None, pm.SHORT_STACK, bp)
+    for seed in (41, 43):
+        t = scenes.random_soup(20000, seed=seed, size=0.02)
+        for p, x in zip(P, t):
+            p.copy_(torch.from_numpy(x))
+        ctx.batch_graph_launch()
+        h = pm.hits_to_numpy(ctx.trace(pm.SHORT_STACK, O, D, pm.trace_params(stack_size=8)))
+        ref = orc.build(t, orc_params(orc, bp))
+        assert_same_tree(ctx.tree_arrays(), ref.arrays(), f"replay seed {seed}")
+        assert_same_hits(h, orc.trace(ref, t, org, dirs, orc.SHORT_STACK, stack_size=8), f"replay seed {seed}")
+    # much larger triangles: far more references than the capacities allow
+    big = scenes.random_soup(20000, seed=44, size=0.5)
+    for p, x in zip(P, big):
+        p.copy_(torch.from_numpy(x))
+    ctx.batch_graph_launch()
+    with pytest.raises(pm.PmError):
+        ctx.trace(pm.SHORT_STACK, O, D)
