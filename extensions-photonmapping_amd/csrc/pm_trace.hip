@@ -56,6 +56,21 @@ __device__ __forceinline__ bool mt_edges(f3 v0, f3 e1, f3 e2, f3 o, f3 d, float 
     return true;
 }
 
+// The same test without early exits, for the cooperative leaf test (its
+// lanes test different triangles, so a wave never skips a part): same
+// values, same acceptance.
+__device__ __forceinline__ bool mt_edges_flat(f3 v0, f3 e1, f3 e2, f3 o, f3 d, float th, float &hx, float &hy,
+                                              float &hz) {
+    const f3 p = cross3(d, e2);
+    const f3 t = sub3(o, v0);
+    const f3 q = cross3(t, e1);
+    const float inv = rcp_rn(dot3(p, e1));
+    hx = inv * dot3(q, e2);
+    hy = inv * dot3(p, t);
+    hz = inv * dot3(q, d);
+    return 0.0f <= hx && hx <= th && hy >= 0.0f && hz >= 0.0f && hy + hz <= 1.0f;
+}
+
 // Leaf test (Raytracer.cu:194-216): optional box test, then Moller-Trumbore
 // over the leaf bitmap in ascending bit order; the next triangle's data is
 // loaded before the current one is tested.
@@ -319,19 +334,21 @@ __global__ __launch_bounds__(TB) void trace_kernel(TreeDev T, const float4 *__re
 // with k_i triangles; tested lane by lane the wave would loop max k_i times
 // with most lanes idle.  Instead the wave lists all (lane, triangle) pairs
 // in LDS (COOP_WIN at a time) and every lane tests the next pair, with the
-// owner's ray (read from the owner's registers) and the owner's leaf exit
-// th.  An owner's first hit is the pair with the smallest t, and among
-// equal t the triangle latest in bitmap order -- exactly what the
-// sequential leaf loop (accept 0 <= t <= th, then th = t) ends with -- so
-// each accepted pair offers the key (t bits, ~slot) to its owner's 64-bit
-// LDS minimum, and the owner recomputes its winning triangle itself for
-// t, u, v and the id.  Slots of one leaf grow with the bit index, so
-// ~slot orders ties like the bitmap does; t >= 0 here (-0 keyed as +0).
-constexpr int COOP_WIN = 1024;
+// owner's ray and leaf exit th (each owner puts them in LDS first).  An
+// owner's first hit is the pair with the smallest t, and among equal t the
+// triangle latest in bitmap order -- exactly what the sequential leaf loop
+// (accept 0 <= t <= th, then th = t) ends with -- so each accepted pair
+// offers the key (t bits, ~slot) to its owner's 64-bit LDS minimum, and the
+// owner recomputes its winning triangle itself for t, u, v and the id.
+// Slots of one leaf grow with the bit index, so ~slot orders ties like the
+// bitmap does; t >= 0 here (-0 keyed as +0).
+constexpr int COOP_WIN = 512;
 
 struct CoopLds {
     unsigned short pair[COOP_WIN];       // bit << 6 | owner lane
     unsigned long long key[64];
+    float4 org[64];                      // owner's origin, w = its leaf's first slot (int bits)
+    float4 dir[64];                      // owner's direction, w = its leaf exit th
 };
 
 // orders this lane's LDS accesses with the other lanes' of the wave (a
@@ -348,6 +365,8 @@ __device__ __forceinline__ int coop_leaf(const float4 *__restrict__ L, CoopLds &
                                          int base, f3 o, f3 d, float th) {
     const int lane = threadIdx.x & 63;
     C.key[lane] = ~0ull;
+    C.org[lane] = make_float4(o.x, o.y, o.z, __int_as_float(base));
+    C.dir[lane] = make_float4(d.x, d.y, d.z, th);
     const int k = __popcll(bm);
     int incl = k;
 #pragma unroll
@@ -375,18 +394,16 @@ __device__ __forceinline__ int coop_leaf(const float4 *__restrict__ L, CoopLds &
         wave_sync();
         for (int r = 0; r < wn; r += 64) {
             const int e = r + lane;
-            const bool valid = e < wn;
-            const int pr = valid ? (int)C.pair[e] : lane;
-            const int ow = pr & 63;
-            const f3 oo = {__shfl(o.x, ow, 64), __shfl(o.y, ow, 64), __shfl(o.z, ow, 64)};
-            const f3 dd = {__shfl(d.x, ow, 64), __shfl(d.y, ow, 64), __shfl(d.z, ow, 64)};
-            float tt = __shfl(th, ow, 64), uu, vv;
-            const int sl = __shfl(base, ow, 64) + (pr >> 6);
-            if (valid) {
+            if (e < wn) {
+                const int pr = C.pair[e];
+                const int ow = pr & 63;
+                const float4 ro = C.org[ow], rd = C.dir[ow];
+                const int sl = __float_as_int(ro.w) + (pr >> 6);
                 const float4 a0 = L[3ll * sl], a1 = L[3ll * sl + 1], a2 = L[3ll * sl + 2];
-                if (mt_edges(mk3(a0), mk3(a1), mk3(a2), oo, dd, tt, uu, vv)) {
+                float hx, hy, hz;
+                if (mt_edges_flat(mk3(a0), mk3(a1), mk3(a2), mk3(ro), mk3(rd), rd.w, hx, hy, hz)) {
                     const unsigned long long key =
-                        ((unsigned long long)__float_as_uint(tt + 0.0f) << 32) | (0xffffffffu - (unsigned)sl);
+                        ((unsigned long long)__float_as_uint(hx + 0.0f) << 32) | (0xffffffffu - (unsigned)sl);
                     atomicMin(&C.key[ow], key);
                 }
             }
