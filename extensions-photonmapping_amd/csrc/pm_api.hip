@@ -44,8 +44,8 @@ void check_build_params(const pm_build_params &p) {
     if (p.max_upper_levels < 0) throw Error{PM_ERR_ARG, "max_upper_levels must be >= 0"};
     if (p.max_ref_factor < 0) throw Error{PM_ERR_ARG, "max_ref_factor must be >= 0"};
     if (p.chunk_size != 0 && p.chunk_size != 128 && p.chunk_size != 256 && p.chunk_size != 512 &&
-        p.chunk_size != 1024)
-        throw Error{PM_ERR_ARG, "chunk_size must be 0, 128, 256, 512 or 1024"};
+        p.chunk_size != 1024 && p.chunk_size != 2048 && p.chunk_size != 4096)
+        throw Error{PM_ERR_ARG, "chunk_size must be 0, 128, 256, 512, 1024, 2048 or 4096"};
 }
 
 void check_trace_params(const pm_trace_params &p) {

@@ -354,18 +354,20 @@ __global__ __launch_bounds__(BS) void seg_fill_kernel(const NodePtrs N, const Lv
                                                       int *__restrict__ seg_node,
                                                       int *__restrict__ seg_start,
                                                       int *__restrict__ seg_count) {
-    const int A = L->A, active_index = L->active_index;
-    GRID_STRIDE(i, A) {
-        const int node = active_index + i;
-        const int pi = N.prim_index[node];
-        const int cnt = (int)N.bitmap[node];
-        const int b = segbase[i], e = segbase[i + 1];
-        for (int s = b; s < e; ++s) {
-            const int off = (s - b) * seg;
-            seg_node[s] = i;
-            seg_start[s] = pi + off;
-            seg_count[s] = min(seg, cnt - off);
+    // one thread per segment; its node is the last i with segbase[i] <= s
+    // (segbase is the exclusive scan of the nodes' segment counts)
+    const int A = L->A, S = L->S, active_index = L->active_index;
+    GRID_STRIDE(s, S) {
+        int lo = 0, hi = A - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (segbase[mid] <= s) lo = mid; else hi = mid - 1;
         }
+        const int node = active_index + lo;
+        const int off = (s - segbase[lo]) * seg;
+        seg_node[s] = lo;
+        seg_start[s] = N.prim_index[node] + off;
+        seg_count[s] = min(seg, (int)N.bitmap[node] - off);
     }
 }
 
@@ -404,7 +406,7 @@ __global__ __launch_bounds__(WPB * 64) void classify_kernel(
         f3 nmn = {0.0f, 0.0f, 0.0f}, nmx = nmn;
         if (dg.p0) { nmn = mk3(N.bmin[node]); nmx = mk3(N.bmax[node]); }
         int cl = 0, cr = 0;
-#pragma unroll
+#pragma unroll 16
         for (int it = 0; it < SEGT / 64; ++it) {
             const int j = it * 64 + lane;
             const bool valid = j < count;
@@ -541,7 +543,7 @@ __device__ __forceinline__ void scatter_segment(
     int rl = 0, rr = 0;
     int l0 = fkey(F_INF), l1 = l0, l2 = l0, l3 = fkey(-F_INF), l4 = l3, l5 = l3;
     int r0 = l0, r1 = l0, r2 = l0, r3 = l3, r4 = l3, r5 = l3;
-#pragma unroll
+#pragma unroll 16
     for (int it = 0; it < SEGT / 64; ++it) {
         const int j = it * 64 + lane;
         const bool valid = j < count;
@@ -1043,6 +1045,8 @@ void launch_classify(int seg, int blocks, hipStream_t s, Args... a) {
     case 128: classify_kernel<128><<<blocks, WPB * 64, 0, s>>>(a...); break;
     case 512: classify_kernel<512><<<blocks, WPB * 64, 0, s>>>(a...); break;
     case 1024: classify_kernel<1024><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    case 2048: classify_kernel<2048><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    case 4096: classify_kernel<4096><<<blocks, WPB * 64, 0, s>>>(a...); break;
     default: classify_kernel<256><<<blocks, WPB * 64, 0, s>>>(a...); break;
     }
 }
@@ -1053,6 +1057,8 @@ void launch_scatter(int seg, int blocks, hipStream_t s, Args... a) {
     case 128: scatter_kernel<128><<<blocks, WPB * 64, 0, s>>>(a...); break;
     case 512: scatter_kernel<512><<<blocks, WPB * 64, 0, s>>>(a...); break;
     case 1024: scatter_kernel<1024><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    case 2048: scatter_kernel<2048><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    case 4096: scatter_kernel<4096><<<blocks, WPB * 64, 0, s>>>(a...); break;
     default: scatter_kernel<256><<<blocks, WPB * 64, 0, s>>>(a...); break;
     }
 }
@@ -1157,7 +1163,7 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
     NodeArrays &nodes = ctx->nodes;
     const bool graph = caps != nullptr;
     const int LS = prm.lower_size;
-    const int seg = prm.chunk_size > 0 ? prm.chunk_size : 256;
+    const int seg = prm.chunk_size > 0 ? prm.chunk_size : 1024;
     const long long BIG = LLONG_MAX / 4;
 
     Lvl h{};
@@ -1236,8 +1242,8 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
         NP = node_ptrs(ctx);
         empty_chain_kernel<<<g_nodes(h.A), BS, 0, s>>>(NP, L, planes, sc.node_eaddr.ptr);
         PM_LAUNCH_CHECK();
-        seg_fill_kernel<<<g_nodes(h.A), BS, 0, s>>>(NP, L, seg, sc.node_segbase.ptr, sc.seg_node.ptr,
-                                                      sc.seg_start.ptr, sc.seg_count.ptr);
+        seg_fill_kernel<<<g_segs(h.S), BS, 0, s>>>(NP, L, seg, sc.node_segbase.ptr, sc.seg_node.ptr,
+                                                     sc.seg_start.ptr, sc.seg_count.ptr);
         PM_LAUNCH_CHECK();
         launch_classify(seg, g_waves(h.S), s, NP, (const Lvl *)L, (const int *)sc.seg_node.ptr,
                         (const int *)sc.seg_start.ptr, (const int *)sc.seg_count.ptr,

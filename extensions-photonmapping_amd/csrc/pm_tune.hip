@@ -43,7 +43,7 @@ void autotune(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2,
               const int *mesh_offsets, int M, const float4 *org, const float4 *dir,
               const int *ray_mesh, int nrays, const pm_build_params &base, float builds_per_trace,
               int reps, pm_tuning &out, hipStream_t s) {
-    static const int kChunks[] = {128, 256, 512, 1024};
+    static const int kChunks[] = {256, 512, 1024, 2048};
     static const int kLeaf[] = {8, 16, 24, 32, 48, 64};
     static const int kStack[] = {4, 8, 16};
     DevBuf<pm_hit> hits;
@@ -83,7 +83,7 @@ void autotune(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2,
     // 1. chunk size: build time only (the tree does not depend on it; a build
     // that fails here fails for every chunk size and is reported)
     float best = 0.0f;
-    int best_chunk = 256;
+    int best_chunk = 1024;
     for (int c : kChunks) {
         p.chunk_size = c;
         const float t = time_build(p);

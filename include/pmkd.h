@@ -83,8 +83,10 @@ typedef struct { int id; float t, u, v; } pm_hit;
  *                         references every level on coincident triangles
  *   chunk_size            refs per upper-stage segment, the reference's
  *                         Segments::SEGMENT_SIZE (Segments.h:27, 256);
- *                         128/256/512/1024, 0 = 256.  Changes how the work
- *                         is cut, never the tree (pm_autotune picks it) */
+ *                         128/256/512/1024/2048/4096, 0 = 1024 (the
+ *                         fastest on MI355X for a 1M-triangle build).
+ *                         Changes how the work is cut, never the tree
+ *                         (pm_autotune picks it) */
 typedef struct {
     int   lower_algorithm;
     int   empty_space_splitting;
@@ -299,7 +301,7 @@ typedef struct {
 /* Autotuner (no reference counterpart: the reference fixes these at
  * compile time, Segments.h:27 SEGMENT_SIZE, KDNode.h:21 lower size,
  * ShortStack.cu:25 stack size).  Times the caller's own workload on this
- * GPU: chunk_size over {128, 256, 512, 1024} (build time), then lower_size
+ * GPU: chunk_size over {256, 512, 1024, 2048} (build time), then lower_size
  * (the upper stage's leaf threshold) over {8, 16, 24, 32, 48, 64} (cost =
  * builds_per_trace * build + trace), then stack_size over {4, 8, 16}
  * together with ray_order over {INPUT, COHERENT} (trace time); each setting is timed `reps` times (median) after one

@@ -166,7 +166,7 @@ def test_c4_frame_pipeline_bit_exact(pm, orc, scenes, torch, ctx):
     assert (h["id"] >= 0).mean() > 0.95
 
 
-@pytest.mark.parametrize("chunk", [128, 512, 1024])
+@pytest.mark.parametrize("chunk", [128, 256, 512, 2048, 4096])
 def test_chunk_size_changes_the_work_not_the_tree(pm, orc, scenes, torch, ctx, chunk):
     tris = scenes.random_soup(60000, seed=11)
     params = pm.build_params("reference", chunk_size=chunk)
@@ -186,7 +186,7 @@ def test_autotune_picks_a_valid_setting(pm, orc, scenes, torch, ctx):
     O, D, RM = dev(torch, org), dev(torch, dirs), dev(torch, ray_mesh)
     res = ctx.autotune(*P, O, D, mesh_offsets=off, ray_mesh=RM, base=pm.build_params("reference"),
                        reps=1)
-    assert res["chunk_size"] in (128, 256, 512, 1024)
+    assert res["chunk_size"] in (256, 512, 1024, 2048)
     assert res["lower_size"] in (8, 16, 24, 32, 48, 64)
     assert res["stack_size"] in (4, 8, 16)
     assert res["ray_order"] == pm.RAY_ORDER_INPUT          # too few rays to reorder
