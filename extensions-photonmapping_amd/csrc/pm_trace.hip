@@ -250,7 +250,7 @@ __device__ __forceinline__ pm_hit short_stack_ray(const TreeDev &T, int root_id,
 // passed n and its lanes are idle; every ray ends (trace_step's visit and
 // descent bounds), so every wave does.
 constexpr int CHUNK = 64;        // rays a wave takes from the counter at a time
-constexpr int REFILL_MIN = 32;   // idle lanes that make a wave take new rays
+constexpr int REFILL_MIN = 16;   // idle lanes that make a wave take new rays
 
 __device__ __forceinline__ bool ray_root_valid(const TreeDev &T, const int *ray_root, int r, int &root_id) {
     root_id = ray_root ? ray_root[r] : 0;
@@ -276,9 +276,9 @@ __global__ __launch_bounds__(TB) void trace_kernel(TreeDev T, const float4 *__re
     int4 cnt = make_int4(0, 0, 0, 0);
     for (;;) {
         const unsigned long long idle = __ballot(q < 0);
-        // refill once half the lanes are idle (or all, at the end of the
-        // rays): new rays start at the root while the others are deep in
-        // the tree, so refilling single lanes costs more than it gains
+        // refill once a quarter of the lanes are idle: new rays start at
+        // the root while the others are deep in the tree, so refilling
+        // single lanes costs more than it gains
         if (__popcll(idle) >= REFILL_MIN && !exhausted) {
             if (next >= end) {
                 int base = 0;
