@@ -389,32 +389,27 @@ __device__ __forceinline__ void divide_sides(const DivideGeom &g, float id_bits,
     r = tri_box_step3(v0, v1, v2, rmin, nmx);
 }
 
-// SetSplitSide (TriangleUpperChildren.h:10-62) counted per segment.  One
-// block per segment: wave w counts the w-th quarter of its refs; the block
-// writes the segment's counts and, per wave, the counts of the quarters
-// before it (the scatter's offsets inside the segment).
+// SetSplitSide (TriangleUpperChildren.h:10-62) counted per segment
 template <int SEGT>
 __global__ __launch_bounds__(WPB * 64) void classify_kernel(
     const NodePtrs N, const Lvl *__restrict__ L, const int *__restrict__ seg_node,
     const int *__restrict__ seg_start, const int *__restrict__ seg_count,
     const float4 *__restrict__ rmin, const float4 *__restrict__ rmax, int *__restrict__ segL,
-    int *__restrict__ segR, int2 *__restrict__ sub_pre, DivideGeom dg) {
-    constexpr int Q = SEGT / WPB;               // refs per wave
-    __shared__ int wl[WPB], wr[WPB];
+    int *__restrict__ segR, DivideGeom dg) {
     const int S = L->S, active_index = L->active_index;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int s = blockIdx.x; s < S; s += gridDim.x) {
+    const int lane = threadIdx.x & 63;
+    for (int s = wave_index(); s < S; s += wave_count()) {
         const int node = active_index + seg_node[s];
         const int ax = N.info[node];
         const float sp = N.split[node];
-        const int start = seg_start[s] + w * Q, count = seg_count[s] - w * Q;
+        const int start = seg_start[s], count = seg_count[s];
         f3 nmn = {0.0f, 0.0f, 0.0f}, nmx = nmn;
         if (dg.p0) { nmn = mk3(N.bmin[node]); nmx = mk3(N.bmax[node]); }
         int cl = 0, cr = 0;
-#pragma unroll
-        for (int it = 0; it < (Q + 63) / 64; ++it) {
+#pragma unroll 16
+        for (int it = 0; it < SEGT / 64; ++it) {
             const int j = it * 64 + lane;
-            const bool valid = j < Q && j < count;
+            const bool valid = j < count;
             bool l = false, r = false;
             if (valid) {
                 const float4 mn = rmin[start + j];
@@ -425,17 +420,7 @@ __global__ __launch_bounds__(WPB * 64) void classify_kernel(
             cl += __popcll(__ballot(l));
             cr += __popcll(__ballot(r));
         }
-        if (lane == 0) { wl[w] = cl; wr[w] = cr; }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int pl = 0, pr = 0;
-            for (int q = 0; q < WPB; ++q) {
-                sub_pre[WPB * (long long)s + q] = make_int2(pl, pr);
-                pl += wl[q]; pr += wr[q];
-            }
-            segL[s] = pl; segR[s] = pr;
-        }
-        __syncthreads();
+        if (lane == 0) { segL[s] = cl; segR[s] = cr; }
     }
 }
 
@@ -530,14 +515,14 @@ __global__ __launch_bounds__(BS) void children_kernel(
 // refs with clipping, plus the next level's box reduction.
 template <int SEGT>
 __device__ __forceinline__ void scatter_segment(
-    const NodePtrs &N, int s, int w, int2 pre, int A, int active_index, int next_active, int L_all, int LL_all, int ulp,
+    const NodePtrs &N, int s, int A, int active_index, int next_active, int L_all, int LL_all, int ulp,
     const int *__restrict__ seg_node, const int *__restrict__ seg_start,
     const int *__restrict__ seg_count, const int *__restrict__ segLoff,
     const int *__restrict__ segRoff, const int *__restrict__ segLLoff,
     const int *__restrict__ segLRoff, const int *__restrict__ slot_leaf,
     const float4 *__restrict__ rmin, const float4 *__restrict__ rmax,
     float4 *__restrict__ nmin, float4 *__restrict__ nmax, float4 *__restrict__ fmin,
-    float4 *__restrict__ fmax, int *__restrict__ keys_next, const DivideGeom &dg, int *box_lds, int *cnt_lds) {
+    float4 *__restrict__ fmax, int *__restrict__ keys_next, const DivideGeom &dg) {
     const int lane = threadIdx.x & 63;
     const int i = seg_node[s];
     const int node = active_index + i;
@@ -547,22 +532,21 @@ __device__ __forceinline__ void scatter_segment(
     if (dg.p0) { nmn = mk3(N.bmin[node]); nmx = mk3(N.bmax[node]); }
     const int2 ch = N.children[node];
     const bool leafL = slot_leaf[i] != 0, leafR = slot_leaf[A + i] != 0;
-    constexpr int Q = SEGT / WPB;               // this wave's quarter of the segment
-    const int start = seg_start[s] + w * Q, count = seg_count[s] - w * Q;
+    const int start = seg_start[s], count = seg_count[s];
     float4 *dLmin, *dLmax, *dRmin, *dRmax;
     long long bL, bR;
-    if (leafL) { dLmin = fmin; dLmax = fmax; bL = (long long)ulp + segLLoff[s] + pre.x; }
-    else { dLmin = nmin; dLmax = nmax; bL = segLoff[s] - segLLoff[s] + pre.x; }
-    if (leafR) { dRmin = fmin; dRmax = fmax; bR = (long long)ulp + LL_all + segLRoff[s] + pre.y; }
-    else { dRmin = nmin; dRmax = nmax; bR = (long long)(L_all - LL_all) + (segRoff[s] - segLRoff[s]) + pre.y; }
+    if (leafL) { dLmin = fmin; dLmax = fmax; bL = (long long)ulp + segLLoff[s]; }
+    else { dLmin = nmin; dLmax = nmax; bL = segLoff[s] - segLLoff[s]; }
+    if (leafR) { dRmin = fmin; dRmax = fmax; bR = (long long)ulp + LL_all + segLRoff[s]; }
+    else { dRmin = nmin; dRmax = nmax; bR = (long long)(L_all - LL_all) + (segRoff[s] - segLRoff[s]); }
     const unsigned long long lt = lanemask_lt();
     int rl = 0, rr = 0;
     int l0 = fkey(F_INF), l1 = l0, l2 = l0, l3 = fkey(-F_INF), l4 = l3, l5 = l3;
     int r0 = l0, r1 = l0, r2 = l0, r3 = l3, r4 = l3, r5 = l3;
-#pragma unroll
-    for (int it = 0; it < (Q + 63) / 64; ++it) {
+#pragma unroll 16
+    for (int it = 0; it < SEGT / 64; ++it) {
         const int j = it * 64 + lane;
-        const bool valid = j < Q && j < count;
+        const bool valid = j < count;
         float4 mn = make_float4(0.0f, 0.0f, 0.0f, 0.0f), mx = mn;
         bool l = false, r = false;
         if (valid) {
@@ -600,37 +584,16 @@ __device__ __forceinline__ void scatter_segment(
         rl += __popcll(blm);
         rr += __popcll(brm);
     }
-    // the children's boxes: wave reduction, then the block's waves combine
-    // in LDS and one lane per child joins the next level's keys (12 atomics
-    // per segment, not per wave: all segments of a large node hit the same
-    // keys).  Empty sides reduce to the neutral keys.
-    l0 = wave_min(l0); l1 = wave_min(l1); l2 = wave_min(l2);
-    l3 = wave_max(l3); l4 = wave_max(l4); l5 = wave_max(l5);
-    r0 = wave_min(r0); r1 = wave_min(r1); r2 = wave_min(r2);
-    r3 = wave_max(r3); r4 = wave_max(r4); r5 = wave_max(r5);
-    if (lane == 0) {
-        int *b = box_lds + 12 * w;
-        b[0] = l0; b[1] = l1; b[2] = l2; b[3] = l3; b[4] = l4; b[5] = l5;
-        b[6] = r0; b[7] = r1; b[8] = r2; b[9] = r3; b[10] = r4; b[11] = r5;
-        cnt_lds[2 * w] = rl; cnt_lds[2 * w + 1] = rr;
+    if (!leafL && rl > 0) {
+        l0 = wave_min(l0); l1 = wave_min(l1); l2 = wave_min(l2);
+        l3 = wave_max(l3); l4 = wave_max(l4); l5 = wave_max(l5);
+        if (lane == 0) key_atomics(keys_next + 6 * (ch.x - next_active), l0, l1, l2, l3, l4, l5);
     }
-    __syncthreads();
-    if (threadIdx.x < 12) {
-        const int c = threadIdx.x;
-        const bool left = c < 6, is_min = (c % 6) < 3;
-        int v = box_lds[c], any = cnt_lds[left ? 0 : 1];
-        for (int q = 1; q < WPB; ++q) {
-            const int x = box_lds[12 * q + c];
-            v = is_min ? min(v, x) : max(v, x);
-            any += cnt_lds[2 * q + (left ? 0 : 1)];
-        }
-        const bool leaf = left ? leafL : leafR;
-        if (!leaf && any > 0) {
-            int *k = keys_next + 6 * ((left ? ch.x : ch.y) - next_active) + (c % 6);
-            if (is_min) atomicMin(k, v); else atomicMax(k, v);
-        }
+    if (!leafR && rr > 0) {
+        r0 = wave_min(r0); r1 = wave_min(r1); r2 = wave_min(r2);
+        r3 = wave_max(r3); r4 = wave_max(r4); r5 = wave_max(r5);
+        if (lane == 0) key_atomics(keys_next + 6 * (ch.y - next_active), r0, r1, r2, r3, r4, r5);
     }
-    __syncthreads();
 }
 
 template <int SEGT>
@@ -642,15 +605,13 @@ __global__ __launch_bounds__(WPB * 64) void scatter_kernel(
     const int *__restrict__ segLRoff, const int *__restrict__ slot_leaf,
     const float4 *__restrict__ rmin, const float4 *__restrict__ rmax,
     float4 *__restrict__ nmin, float4 *__restrict__ nmax, float4 *__restrict__ fmin,
-    float4 *__restrict__ fmax, int *__restrict__ keys_next, const int2 *__restrict__ sub_pre, DivideGeom dg) {
+    float4 *__restrict__ fmax, int *__restrict__ keys_next, DivideGeom dg) {
     const int S = L->S, A = L->A, active_index = L->active_index, next_active = L->next_active;
     const int L_all = L->L_all, LL_all = L->LL_all, ulp = L->final_refs;
-    const int w = threadIdx.x >> 6;
-    __shared__ int box_lds[12 * WPB], cnt_lds[2 * WPB];
-    for (int s = blockIdx.x; s < S; s += gridDim.x)
-        scatter_segment<SEGT>(N, s, w, sub_pre[WPB * (long long)s + w], A, active_index, next_active, L_all,
-                              LL_all, ulp, seg_node, seg_start, seg_count, segLoff, segRoff, segLLoff, segLRoff,
-                              slot_leaf, rmin, rmax, nmin, nmax, fmin, fmax, keys_next, dg, box_lds, cnt_lds);
+    for (int s = wave_index(); s < S; s += wave_count())
+        scatter_segment<SEGT>(N, s, A, active_index, next_active, L_all, LL_all, ulp, seg_node, seg_start,
+                              seg_count, segLoff, segRoff, segLLoff, segLRoff, slot_leaf, rmin, rmax, nmin,
+                              nmax, fmin, fmax, keys_next, dg);
 }
 
 // ---- plan kernels: the host arithmetic of the level loop
@@ -1240,7 +1201,7 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
 
     // grids: eager from the sizes just read back, captured from the caps
     auto g_nodes = [&](long long n) { return grid_for(graph ? caps->active : n, BS, graph ? 2048 : INT_MAX); };
-    auto g_chunks = [&](long long n) { return grid_for(graph ? caps->segs : n, 1, graph ? 8192 : INT_MAX); };
+    auto g_waves = [&](long long n) { return grid_for(graph ? caps->segs : n, WPB, graph ? 2048 : INT_MAX); };
     auto g_segs = [&](long long n) { return grid_for(graph ? caps->segs : n, BS, graph ? 2048 : INT_MAX); };
     const DivideGeom dg = {prm.split_method == PM_SPLIT_DIVIDE ? P0 : nullptr, P1, P2};
     const int *dA = graph ? &L->A : nullptr;
@@ -1276,7 +1237,6 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
             sc.segL.reserve(S1, s); sc.segR.reserve(S1, s); sc.segLL.reserve(S1, s); sc.segLR.reserve(S1, s);
             sc.segLoff.reserve(S1, s); sc.segRoff.reserve(S1, s);
             sc.segLLoff.reserve(S1, s); sc.segLRoff.reserve(S1, s);
-            sc.sub_pre.reserve(WPB * S1, s);
             seen.segs = std::max<long long>(seen.segs, h.S);
         }
         NP = node_ptrs(ctx);
@@ -1285,10 +1245,10 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
         seg_fill_kernel<<<g_segs(h.S), BS, 0, s>>>(NP, L, seg, sc.node_segbase.ptr, sc.seg_node.ptr,
                                                      sc.seg_start.ptr, sc.seg_count.ptr);
         PM_LAUNCH_CHECK();
-        launch_classify(seg, g_chunks(h.S), s, NP, (const Lvl *)L, (const int *)sc.seg_node.ptr,
+        launch_classify(seg, g_waves(h.S), s, NP, (const Lvl *)L, (const int *)sc.seg_node.ptr,
                         (const int *)sc.seg_start.ptr, (const int *)sc.seg_count.ptr,
                         (const float4 *)sc.ref_min[cur].ptr, (const float4 *)sc.ref_max[cur].ptr, sc.segL.ptr,
-                        sc.segR.ptr, sc.sub_pre.ptr, dg);
+                        sc.segR.ptr, dg);
         PM_LAUNCH_CHECK();
         scan_exclusive2(sc.segL.ptr, sc.segLoff.ptr, sc.segR.ptr, sc.segRoff.ptr, graph ? nS : h.S, sc, s, dS);
         child_size_kernel<<<g_nodes(h.A), BS, 0, s>>>(L, LS, sc.node_segbase.ptr, sc.segLoff.ptr, sc.segRoff.ptr,
@@ -1320,12 +1280,12 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
                                                       sc.segLRoff.ptr, sc.slot_leaf.ptr, sc.slot_lna.ptr,
                                                       ctx->leaf_ids.ptr, sc.keys[nxt].ptr);
         PM_LAUNCH_CHECK();
-        launch_scatter(seg, g_chunks(h.S), s, NP, (const Lvl *)L, (const int *)sc.seg_node.ptr,
+        launch_scatter(seg, g_waves(h.S), s, NP, (const Lvl *)L, (const int *)sc.seg_node.ptr,
                        (const int *)sc.seg_start.ptr, (const int *)sc.seg_count.ptr, (const int *)sc.segLoff.ptr,
                        (const int *)sc.segRoff.ptr, (const int *)sc.segLLoff.ptr, (const int *)sc.segLRoff.ptr,
                        (const int *)sc.slot_leaf.ptr, (const float4 *)sc.ref_min[cur].ptr,
                        (const float4 *)sc.ref_max[cur].ptr, sc.ref_min[nxt].ptr, sc.ref_max[nxt].ptr,
-                       sc.prim_min.ptr, sc.prim_max.ptr, sc.keys[nxt].ptr, (const int2 *)sc.sub_pre.ptr, dg);
+                       sc.prim_min.ptr, sc.prim_max.ptr, sc.keys[nxt].ptr, dg);
         PM_LAUNCH_CHECK();
         plan_advance_kernel<<<1, 64, 0, s>>>(L);
         PM_LAUNCH_CHECK();
@@ -1504,7 +1464,6 @@ std::vector<void **> graph_buffers(pm_ctx *ctx) {
     v.push_back((void **)&sc.order_hist.ptr);
     v.push_back((void **)&sc.order_offs.ptr);
     v.push_back((void **)&sc.trace_work.ptr);
-    v.push_back((void **)&sc.sub_pre.ptr);
     return v;
 }
 
@@ -1562,7 +1521,6 @@ void capture_batch(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4
     for (DevBuf<int> *b : {&sc.seg_node, &sc.seg_start, &sc.seg_count, &sc.segL, &sc.segR, &sc.segLL, &sc.segLR,
                            &sc.segLoff, &sc.segRoff, &sc.segLLoff, &sc.segLRoff})
         b->reserve((size_t)caps.segs + 1, s);
-    sc.sub_pre.reserve(WPB * ((size_t)caps.segs + 1), s);
     sc.prim_min.reserve((size_t)caps.leaf_refs + 1, s);
     sc.prim_max.reserve((size_t)caps.leaf_refs + 1, s);
     ctx->prim_indices.reserve((size_t)caps.leaf_refs + 1, s);

@@ -138,7 +138,6 @@ struct BuildScratch {
     DevBuf<int> seg_node, seg_start, seg_count; // segment table
     DevBuf<int> segL, segR, segLL, segLR;       // per-segment counts (left, right, leaf-left, leaf-right)
     DevBuf<int> segLoff, segRoff, segLLoff, segLRoff;
-    DevBuf<int2> sub_pre;                       // per segment and wave: left/right refs of the quarters before
     DevBuf<int> slot_leaf, slot_lna;            // 2A leaf-slot flags and addresses
     DevBuf<int> scan_tmp[3];                    // block sums for scans, per level
     DevBuf<int> totals;                         // small device-side counters
