@@ -16,9 +16,10 @@
 //     per-segment counts plus in-wave ballot ranks, which give the same
 //     addresses (stable partition: left copies of all refs, then right).
 // Lower stage: the SAH creator (TriangleMapSAHCreator.cu:53-246) with one
-// wave64 per node: candidate masks are 64-bit ballots, costs are summed by
-// one lane per candidate in ascending bit order, the winner is a wave
-// argmin with the reference's first-minimum tie rule.  The bitmap creator
+// wave64 per node: each lane evaluates the six planes of one prim, summing
+// side areas in ascending prim order, and the winner is a wave argmin with
+// the reference's first-minimum tie rule; an exact lower bound settles most
+// nodes as leaves before any plane is evaluated.  The bitmap creator
 // (TriangleMapBitmapCreator.cu:32-65) is one thread per upper leaf.
 #include <algorithm>
 #include <climits>
@@ -737,6 +738,7 @@ __global__ void plan_lower_start_kernel(Lvl *L) {
     L->use_list = 1;
     L->list_base = 0;
     L->lower_levels = 0;
+    L->cand_n = 0;
 }
 
 // Exact early rejection for the SAH decision.  Every candidate plane puts
@@ -748,17 +750,21 @@ __global__ void plan_lower_start_kernel(Lvl *L) {
 // positive terms: relative error < 1e-5), is still >= the leaf cost, no
 // candidate can win and the node stays a leaf, exactly as the full search
 // would decide.  Non-finite areas disable the shortcut.
-__device__ bool sah_cannot_split(bool has, float a, int n, float bonus, float leaf_cost) {
+//
+// The n areas are first packed into lanes 0..n-1, so the sort, the prefix
+// sums and the minimum run over the P = 4..64 lanes of the smallest power of
+// two >= n only.  The bound is symmetric in k <-> n - k, so the k = 0..P-1
+// the group holds cover every distinct value (k <= n/2 < P).
+template <int P>
+__device__ __forceinline__ bool sah_bound_exceeds(float v, int n, float bonus, float leaf_cost) {
     const int lane = __lane_id();
-    if (__ballot(has && !(a <= 3.0e38f)) != 0ull) return false;
-    float v = has ? a : __int_as_float(0x7f800000);
-    // bitonic sort of the 64 lanes, ascending
+    // bitonic sort of lanes 0..P-1, ascending
 #pragma unroll
-    for (int k = 2; k <= 64; k <<= 1) {
+    for (int k = 2; k <= P; k <<= 1) {
 #pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
             const float o = __shfl_xor(v, j, 64);
-            const bool up = (lane & k) == 0 || k == 64;
+            const bool up = (lane & k) == 0 || k == P;
             const bool low = (lane & j) == 0;
             v = (low == up) ? fminf(v, o) : fmaxf(v, o);
         }
@@ -766,7 +772,7 @@ __device__ bool sah_cannot_split(bool has, float a, int n, float bonus, float le
     // inclusive prefix sums: lane j holds s_(j+1)
     float pre = lane < n ? v : 0.0f;
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
+    for (int off = 1; off < P; off <<= 1) {
         const float o = __shfl_up(pre, off, 64);
         if (lane >= off) pre += o;
     }
@@ -774,22 +780,44 @@ __device__ bool sah_cannot_split(bool has, float a, int n, float bonus, float le
     const float sk = __shfl(pre, k > 0 ? k - 1 : 0, 64);
     const float sm = __shfl(pre, m > 0 ? m - 1 : 0, 64);
     float f = __int_as_float(0x7f800000);
-    if (k <= n) {
+    if (k <= n && k < P) {
         f = (float)k * (k > 0 ? sk : 0.0f) + (float)m * (m > 0 ? sm : 0.0f);
         if (k == 0 || m == 0) f *= fminf(bonus, 1.0f);
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) f = fminf(f, __shfl_xor(f, off, 64));
-    return f * (1.0f - 1.0f / 1024.0f) >= leaf_cost;
+    for (int off = P >> 1; off > 0; off >>= 1) f = fminf(f, __shfl_xor(f, off, 64));
+    return __shfl(f, 0, 64) * (1.0f - 1.0f / 1024.0f) >= leaf_cost;
+}
+
+__device__ bool sah_cannot_split(unsigned long long bmp, float a, int n, float bonus, float leaf_cost) {
+    const int lane = __lane_id();
+    const bool has = (bmp >> lane) & 1ull;
+    if (__ballot(has && !(a <= 3.0e38f)) != 0ull) return false;
+    // present lane -> its rank among the present lanes, absent lane -> n + its
+    // rank among the absent ones (a permutation of the 64 lanes)
+    const unsigned long long below = lane ? bmp << (64 - lane) : 0ull;
+    const int rank = __popcll(below);
+    const int dst = has ? rank : n + (lane - rank);
+    float v = __int_as_float(__builtin_amdgcn_ds_permute(dst << 2, __float_as_int(a)));
+    if (lane >= n) v = __int_as_float(0x7f800000);
+    if (n <= 4) return sah_bound_exceeds<4>(v, n, bonus, leaf_cost);
+    if (n <= 8) return sah_bound_exceeds<8>(v, n, bonus, leaf_cost);
+    if (n <= 16) return sah_bound_exceeds<16>(v, n, bonus, leaf_cost);
+    if (n <= 32) return sah_bound_exceeds<32>(v, n, bonus, leaf_cost);
+    return sah_bound_exceeds<64>(v, n, bonus, leaf_cost);
 }
 
 
 // CalcSAH + CalcAreaForSets (LowerTriangleMap.h:412-555), one wave per node.
-constexpr int MAXC = 6 * 64;
+// Candidate c = 6 r + 2 axis + high is the low (high) plane of the r-th
+// present prim on that axis (CreateSplittingPlanes, LowerTriangleMap.h:84-155).
+// Lane i evaluates the six planes of its own prim: one pass over the present
+// prims in ascending order counts and sums the areas on each side, so every
+// side area is the same in-order FP32 sum the sequential loop forms.  The
+// winner is the lowest cost, ties to the lowest candidate index.
 struct SahLds {
-    unsigned long long cL[WPB][MAXC];
-    unsigned long long cR[WPB][MAXC];
-    float ar[WPB][64];
+    float4 lo[WPB][64];   // prim min corner, w = area
+    float4 hi[WPB][64];   // prim max corner
 };
 
 __device__ __forceinline__ void sah_node(
@@ -797,62 +825,49 @@ __device__ __forceinline__ void sah_node(
     const float4 *__restrict__ pmax, const float *__restrict__ areas, float Ct, int leaf_max, float bonus,
     int *__restrict__ flag, long long *__restrict__ sets, float *__restrict__ careas) {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    unsigned long long *cL = lds.cL[w], *cR = lds.cR[w];
-    float *ar = lds.ar[w];
     const int pi = N.prim_index[par];
     const unsigned long long bmp = (unsigned long long)N.bitmap[par];
     const int n = __popcll(bmp);
     const float nodeA = N.area[par];
     const bool forced = leaf_max > 0 && n > leaf_max;
-    if (!forced && !(((float)n - Ct) * nodeA > 0.0f)) {   // costs are >= 0: no split can win
-        if (lane == 0) { N.info[par] = LEAF; flag[id] = 0; }
-        return;
-    }
-    const bool has = (bmp >> lane) & 1ull;
-    float a = has ? areas[pi + lane] : 0.0f;
     const float leaf_cost = ((float)n - Ct) * nodeA;
-    if (!forced && sah_cannot_split(has, a, n, bonus, leaf_cost)) {
-        if (lane == 0) { N.info[par] = LEAF; flag[id] = 0; }
-        return;
-    }
-    f3 mn = {0.0f, 0.0f, 0.0f}, mx = {0.0f, 0.0f, 0.0f};
-    if (has) { mn = mk3(pmin[pi + lane]); mx = mk3(pmax[pi + lane]); }
-    ar[lane] = a;
-    // candidate planes (CreateSplittingPlanes, LowerTriangleMap.h:84-155):
-    // prim ascending, axis X/Y/Z, low plane then high plane
-    int c = 0;
-    for (unsigned long long b = bmp; b; b &= b - 1) {
-        const int i = __builtin_ctzll(b);
-        for (int axis = AX_X; axis <= AX_Z; ++axis) {
-            const float mnj = comp3(mn, axis), mxj = comp3(mx, axis);
-            const float lo = __shfl(mnj, i, 64);
-            const float hi = __shfl(mxj, i, 64);
-            const unsigned long long lL = __ballot(has && mnj <= lo);
-            const unsigned long long lR = __ballot(has && lo < mxj);
-            const unsigned long long hL = __ballot(has && mnj <= hi);
-            const unsigned long long hR = __ballot(has && hi < mxj);
-            if (lane == 0) {
-                cL[c] = lL; cR[c] = lR;
-                cL[c + 1] = hL; cR[c + 1] = hR;
-            }
-            c += 2;
-        }
-    }
+    const bool has = (bmp >> lane) & 1ull;
+    const float a = has ? areas[pi + lane] : 0.0f;
+    float4 mn = make_float4(0.0f, 0.0f, 0.0f, 0.0f), mx = mn;
+    if (has) { mn = pmin[pi + lane]; mx = pmax[pi + lane]; }
+    float4 *lo = lds.lo[w], *hi = lds.hi[w];
+    lo[lane] = make_float4(mn.x, mn.y, mn.z, a);
+    hi[lane] = mx;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    const int nc = c;
+    const float p[6] = {mn.x, mx.x, mn.y, mx.y, mn.z, mx.z};
+    float sL[6], sR[6];
+    int nL[6], nR[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) { sL[k] = 0.0f; sR[k] = 0.0f; nL[k] = 0; nR[k] = 0; }
+    for (unsigned long long b = bmp; b; b &= b - 1) {
+        const int j = __builtin_ctzll(b);
+        const float4 q = lo[j], r = hi[j];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const float qm = k < 2 ? q.x : k < 4 ? q.y : q.z;
+            const float rm = k < 2 ? r.x : k < 4 ? r.y : r.z;
+            if (qm <= p[k]) { sL[k] += q.w; ++nL[k]; }
+            if (p[k] < rm) { sR[k] += q.w; ++nR[k]; }
+        }
+    }
     float best = F_INF;
     int bestc = INT_MAX;
-    for (int cc = lane; cc < nc; cc += 64) {
-        const unsigned long long L = cL[cc], R = cR[cc];
-        float aL = 0.0f, aR = 0.0f;
-        for (unsigned long long m = L; m; m &= m - 1) aL += ar[__builtin_ctzll(m)];
-        for (unsigned long long m = R; m; m &= m - 1) aR += ar[__builtin_ctzll(m)];
-        float cost = (float)__popcll(L) * aL + (float)__popcll(R) * aR;
-        if (bonus != 1.0f && (L == 0ull || R == 0ull)) cost = cost * bonus;
-        const bool progress = L != bmp && R != bmp;
-        if (forced && !progress) continue;
-        if (cost < best) { best = cost; bestc = cc; }
+    const int rank = __popcll(lane ? bmp << (64 - lane) : 0ull);
+    if (has) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            float cost = (float)nL[k] * sL[k] + (float)nR[k] * sR[k];
+            if (bonus != 1.0f && (nL[k] == 0 || nR[k] == 0)) cost = cost * bonus;
+            const bool progress = nL[k] != n && nR[k] != n;
+            if (forced && !progress) continue;
+            if (cost < best) { best = cost; bestc = 6 * rank + k; }
+        }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -860,42 +875,125 @@ __device__ __forceinline__ void sah_node(
         const int oc = __shfl_xor(bestc, off, 64);
         if (ob < best || (ob == best && oc < bestc)) { best = ob; bestc = oc; }
     }
-    if (lane != 0) return;
     bool split = false;
-    if (bestc != INT_MAX) {
-        const unsigned long long L = cL[bestc], R = cR[bestc];
-        float aL = 0.0f, aR = 0.0f;
-        for (unsigned long long m = L; m; m &= m - 1) aL += ar[__builtin_ctzll(m)];
-        for (unsigned long long m = R; m; m &= m - 1) aR += ar[__builtin_ctzll(m)];
+    if (bestc != INT_MAX) {   // wave-uniform: every lane holds the same winner
+        const int kk = bestc % 6;
+        const int axis = kk / 2 + 1;
+        const int owner = __builtin_ctzll(__ballot(has && rank == bestc / 6));
+        float mp = p[0], ml = sL[0], mr = sR[0];
+#pragma unroll
+        for (int k = 1; k < 6; ++k)
+            if (kk == k) { mp = p[k]; ml = sL[k]; mr = sR[k]; }
+        const float pos = __shfl(mp, owner, 64);
+        const float aL = __shfl(ml, owner, 64), aR = __shfl(mr, owner, 64);
+        const float lo_ax = axis == AX_X ? mn.x : axis == AX_Y ? mn.y : mn.z;
+        const float hi_ax = axis == AX_X ? mx.x : axis == AX_Y ? mx.y : mx.z;
+        const unsigned long long L = __ballot(has && lo_ax <= pos);
+        const unsigned long long R = __ballot(has && pos < hi_ax);
         const bool progress = L != bmp && R != bmp;
         split = progress && (forced || best < leaf_cost);
-        if (split) {
-            const int rank = bestc / 6;
-            const int axis = (bestc % 6) / 2 + 1;
-            const bool high = bestc & 1;
-            unsigned long long b = bmp;
-            for (int k = 0; k < rank; ++k) b &= b - 1;
-            const int slot = __builtin_ctzll(b);
-            const float pos = high ? comp4(pmax[pi + slot], axis) : comp4(pmin[pi + slot], axis);
+        if (split && lane == 0) {
             N.info[par] = (int8_t)axis;
             N.split[par] = pos;
             sets[2 * id] = (long long)L; sets[2 * id + 1] = (long long)R;
             careas[2 * id] = aL; careas[2 * id + 1] = aR;
         }
     }
-    if (!split) N.info[par] = LEAF;
-    flag[id] = split;
+    if (lane == 0) {
+        if (!split) N.info[par] = LEAF;
+        flag[id] = split;
+    }
 }
 
+__device__ __forceinline__ int lane_int(int v, int k) { return __builtin_amdgcn_readlane(v, k); }
+__device__ __forceinline__ float lane_float(float v, int k) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+__device__ __forceinline__ unsigned long long lane_u64(unsigned long long v, int k) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, k);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), k);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// First pass of a lower level: a wave takes 64 nodes at a time, one per lane.
+// Nodes with leaf cost <= 0 (n <= C_t: costs are >= 0, no split can win)
+// become leaves lane by lane; the others get the exact bound of
+// sah_cannot_split one node at a time, with the next node's areas already in
+// flight.  Nodes the bound cannot settle (and nodes forced to split) are
+// listed for sah_kernel, one list append per wave and batch.  Each node's
+// decision is its own, so the list order does not matter.
+__global__ __launch_bounds__(WPB * 64) void sah_screen_kernel(
+    NodePtrs N, const int *__restrict__ leaf_ids, Lvl *__restrict__ L, const float *__restrict__ areas,
+    float Ct, int leaf_max, float bonus, int *__restrict__ flag, int *__restrict__ cand) {
+    const int An = L->An, list_base = L->list_base;
+    const int *list = L->use_list ? leaf_ids : nullptr;
+    const int lane = threadIdx.x & 63;
+    for (int base = wave_index() * 64; base < An; base += wave_count() * 64) {
+        const int id = base + lane;
+        const bool valid = id < An;
+        int par = 0, pi = 0;
+        unsigned long long bmp = 0ull;
+        float nodeA = 0.0f;
+        if (valid) {
+            par = list ? list[id] : list_base + id;
+            bmp = (unsigned long long)N.bitmap[par];
+            nodeA = N.area[par];
+            pi = N.prim_index[par];
+        }
+        const int n = __popcll(bmp);
+        const bool forced = leaf_max > 0 && n > leaf_max;
+        const float leaf_cost = ((float)n - Ct) * nodeA;
+        unsigned long long m = __ballot(valid && (forced || leaf_cost > 0.0f));
+        unsigned long long search = 0ull;
+        int k = 0;
+        float a = 0.0f;
+        if (m) {
+            k = __builtin_ctzll(m);
+            const unsigned long long kb = lane_u64(bmp, k);
+            if ((kb >> lane) & 1ull) a = areas[lane_int(pi, k) + lane];
+        }
+        while (m) {
+            const unsigned long long m2 = m & (m - 1);
+            int k2 = 0;
+            float a2 = 0.0f;
+            if (m2) {   // the next node's areas, loaded while this node is decided
+                k2 = __builtin_ctzll(m2);
+                const unsigned long long kb2 = lane_u64(bmp, k2);
+                if ((kb2 >> lane) & 1ull) a2 = areas[lane_int(pi, k2) + lane];
+            }
+            const unsigned long long kb = lane_u64(bmp, k);
+            const int kn = __popcll(kb);
+            const bool kforced = leaf_max > 0 && kn > leaf_max;
+            if (kforced || !sah_cannot_split(kb, a, kn, bonus, lane_float(leaf_cost, k))) search |= 1ull << k;
+            m = m2;
+            k = k2;
+            a = a2;
+        }
+        const bool listed = (search >> lane) & 1ull;
+        if (valid && !listed) {
+            N.info[par] = LEAF;
+            flag[id] = 0;
+        }
+        if (search) {
+            int at = 0;
+            if (lane == 0) at = atomicAdd(&L->cand_n, __popcll(search));
+            at = __shfl(at, 0, 64);
+            if (listed) cand[at + __popcll(lane ? search << (64 - lane) : 0ull)] = id;
+        }
+    }
+}
+
+// CalcSAH for the listed nodes
 __global__ __launch_bounds__(WPB * 64) void sah_kernel(
-    NodePtrs N, const int *__restrict__ leaf_ids, const Lvl *__restrict__ L,
+    NodePtrs N, const int *__restrict__ leaf_ids, const Lvl *__restrict__ L, const int *__restrict__ cand,
     const float4 *__restrict__ pmin, const float4 *__restrict__ pmax,
     const float *__restrict__ areas, float Ct, int leaf_max, float bonus,
     int *__restrict__ flag, long long *__restrict__ sets, float *__restrict__ careas) {
     __shared__ SahLds lds;
-    const int An = L->An, list_base = L->list_base;
+    const int nc = L->cand_n, list_base = L->list_base;
     const int *list = L->use_list ? leaf_ids : nullptr;
-    for (int id = wave_index(); id < An; id += wave_count()) {
+    for (int c = wave_index(); c < nc; c += wave_count()) {
+        const int id = cand[c];
         const int par = list ? list[id] : list_base + id;
         sah_node(N, id, par, lds, pmin, pmax, areas, Ct, leaf_max, bonus, flag, sets, careas);
     }
@@ -987,6 +1085,7 @@ __global__ void plan_lower_advance_kernel(Lvl *L) {
     L->An = 2 * L->splits;
     L->use_list = 0;
     L->list_base = L->lower_base;
+    L->cand_n = 0;
 }
 
 // CreateLowerSAHChildren (LowerTriangleMap.h:557-600) + PropagateAabbToChildren
@@ -1342,7 +1441,7 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
         for (int ll = 0; graph ? ll < caps->lower_levels : An > 0; ++ll) {
             const long long nAn = graph ? caps->lower : An;
             if (!graph) {
-                sc.lower_flag.reserve(An + 1, s); sc.lower_off.reserve(An + 1, s);
+                sc.lower_flag.reserve(An + 1, s); sc.lower_off.reserve(An + 1, s); sc.lower_list.reserve(An + 1, s);
                 sc.lower_sets.reserve(2 * (size_t)An, s); sc.lower_careas.reserve(2 * (size_t)An, s);
                 seen.lower = std::max(seen.lower, An);
             }
@@ -1351,11 +1450,17 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
                 balanced_kernel<<<grid_for(nAn, WPB, graph ? 2048 : INT_MAX), WPB * 64, 0, s>>>(
                     NP, ctx->leaf_ids.ptr, L, sc.prim_min.ptr, sc.prim_max.ptr, prm.traversal_cost, LS,
                     sc.lower_flag.ptr, sc.lower_sets.ptr, sc.lower_careas.ptr);
-            else
-                sah_kernel<<<grid_for(nAn, WPB, graph ? 2048 : INT_MAX), WPB * 64, 0, s>>>(
-                    NP, ctx->leaf_ids.ptr, L, sc.prim_min.ptr, sc.prim_max.ptr, sc.areas.ptr, prm.traversal_cost,
-                    prm.sah_leaf_max, prm.sah_empty_bonus, sc.lower_flag.ptr, sc.lower_sets.ptr,
-                    sc.lower_careas.ptr);
+            else {
+                sah_screen_kernel<<<grid_for((nAn + 63) / 64, WPB, graph ? 1024 : INT_MAX), WPB * 64, 0, s>>>(
+                    NP, ctx->leaf_ids.ptr, L, sc.areas.ptr, prm.traversal_cost, prm.sah_leaf_max,
+                    prm.sah_empty_bonus, sc.lower_flag.ptr, sc.lower_list.ptr);
+                PM_LAUNCH_CHECK();
+                // the listed count is known on the device only: a grid-stride grid
+                sah_kernel<<<grid_for(nAn, WPB, 2048), WPB * 64, 0, s>>>(
+                    NP, ctx->leaf_ids.ptr, L, sc.lower_list.ptr, sc.prim_min.ptr, sc.prim_max.ptr, sc.areas.ptr,
+                    prm.traversal_cost, prm.sah_leaf_max, prm.sah_empty_bonus, sc.lower_flag.ptr,
+                    sc.lower_sets.ptr, sc.lower_careas.ptr);
+            }
             PM_LAUNCH_CHECK();
             scan_exclusive(sc.lower_flag.ptr, sc.lower_off.ptr, (int)nAn, sc, s, graph ? &L->An : nullptr);
             plan_lower_kernel<<<1, 64, 0, s>>>(L, sc.lower_off.ptr);
@@ -1450,7 +1555,7 @@ std::vector<void **> graph_buffers(pm_ctx *ctx) {
         (void **)&sc.segLLoff.ptr, (void **)&sc.segLRoff.ptr, (void **)&sc.slot_leaf.ptr,
         (void **)&sc.slot_lna.ptr, (void **)&sc.areas.ptr, (void **)&sc.lower_flag.ptr,
         (void **)&sc.lower_off.ptr, (void **)&sc.lower_sets.ptr, (void **)&sc.lower_careas.ptr,
-        (void **)&ctx->graph_off.ptr, (void **)&sc.lvl.ptr};
+        (void **)&sc.lower_list.ptr, (void **)&ctx->graph_off.ptr, (void **)&sc.lvl.ptr};
     for (int i = 0; i < 2; ++i) {
         v.push_back((void **)&sc.ref_min[i].ptr);
         v.push_back((void **)&sc.ref_max[i].ptr);
@@ -1529,6 +1634,7 @@ void capture_batch(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4
     ctx->leaf_ids.reserve((size_t)caps.leaf_ids + 1, s);
     sc.lower_flag.reserve((size_t)caps.lower + 1, s);
     sc.lower_off.reserve((size_t)caps.lower + 1, s);
+    sc.lower_list.reserve((size_t)caps.lower + 1, s);
     sc.lower_sets.reserve(2 * (size_t)caps.lower, s);
     sc.lower_careas.reserve(2 * (size_t)caps.lower, s);
     ctx->packed.reserve((size_t)caps.nodes + 1, s);

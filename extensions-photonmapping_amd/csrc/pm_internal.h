@@ -115,6 +115,7 @@ struct Lvl {
     int leafTri, nextR, created, child_start, next_active;
     int final_refs, nleaf, level;
     int An, list_base, use_list, splits, lower_base, lower_levels;   // lower levels
+    int cand_n;                                        // lower level: nodes listed for the full SAH search
     int status;                                        // 0, or the PM_ERR_* that stopped the build
     int max_levels;
     long long max_refs;
@@ -142,7 +143,7 @@ struct BuildScratch {
     DevBuf<int> scan_tmp[3];                    // block sums for scans, per level
     DevBuf<int> totals;                         // small device-side counters
     DevBuf<float> areas;                        // lower stage: triangle areas per final ref
-    DevBuf<int> lower_list;                     // lower stage: node list
+    DevBuf<int> lower_list;                     // lower stage: nodes listed for the full SAH search
     DevBuf<int> lower_flag, lower_off;
     DevBuf<long long> lower_sets;               // 2 per node
     DevBuf<float> lower_careas;                 // 2 per node

@@ -131,6 +131,32 @@ def test_random_soups_bit_exact(pm, orc, scenes, torch, ctx, n):
             assert_same_hits(h, orc.trace(ref_tree, tris, org, dirs, algo), f"n={n} {name} {algo}")
 
 
+SAH_SEARCH = {
+    "ct1_lower64": dict(preset="reference", traversal_cost=1.0, lower_size=64),
+    "ct2_forced16_bonus": dict(preset="config", traversal_cost=2.0, lower_size=64, sah_leaf_max=16,
+                               sah_empty_bonus=0.5),
+    "ct3_bonus_above_one": dict(preset="reference", traversal_cost=3.0, sah_empty_bonus=1.5),
+}
+
+
+@pytest.mark.parametrize("n,size", [(40, 0.3), (300, 0.3), (3000, 0.1)])
+@pytest.mark.parametrize("name", list(SAH_SEARCH))
+def test_sah_full_search_bit_exact(pm, orc, scenes, torch, ctx, n, size, name):
+    """low traversal costs send most lower nodes past the early rejection
+    into the full plane search, over several lower levels with sparse
+    bitmaps (up to 64 prims per node)"""
+    tris = scenes.random_soup(n, seed=3000 + n, size=size)
+    params = make_params(pm, SAH_SEARCH[name])
+    P, got = gpu_build(pm, ctx, torch, tris, params)
+    ref_tree = orc.build(tris, orc_params(orc, params))
+    assert_same_tree(got, ref_tree.arrays(), f"n={n} {name}")
+    assert ctx.stats()["lower_levels"] >= 2, "no lower split: the full search was not exercised"
+    org, dirs = scenes.random_rays(2048, seed=4000 + n)
+    O, D = dev(torch, org), dev(torch, dirs)
+    h = pm.hits_to_numpy(ctx.trace(pm.SHORT_STACK, O, D))
+    assert_same_hits(h, orc.trace(ref_tree, tris, org, dirs, pm.SHORT_STACK), f"n={n} {name}")
+
+
 def test_counters_match_oracle(pm, orc, scenes, torch, ctx):
     tris, (org, dirs) = scenes.config1(width=64, height=64)
     params = pm.build_params("reference")
