@@ -345,7 +345,7 @@ __global__ __launch_bounds__(TB) void trace_kernel(TreeDev T, const float4 *__re
 constexpr int COOP_WIN = 512;
 
 struct CoopLds {
-    unsigned short pair[COOP_WIN];       // bit << 6 | owner lane
+    alignas(8) unsigned short pair[COOP_WIN];   // bit << 6 | owner lane
     unsigned long long key[64];
     float4 org[64];                      // owner's origin, w = its leaf's first slot (int bits)
     float4 dir[64];                      // owner's direction, w = its leaf exit th
@@ -375,12 +375,40 @@ __device__ __forceinline__ int coop_leaf(const float4 *__restrict__ L, CoopLds &
         if (lane >= dl) incl += y;
     }
     const int W = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
-    int pos = incl - k;                   // this lane's first pair
-    unsigned lo = (unsigned)bm, hi = (unsigned)(bm >> 32);
+    const int excl = incl - k;            // this lane's first pair
+    int pos = excl;
+    // a run of bits b0 .. b0+k-1 (every leaf the upper stage makes) is listed
+    // four pairs to a 64-bit store; other bitmaps bit by bit
+    const int b0 = bm ? __builtin_ctzll(bm) : 0;
+    const unsigned long long run = bm >> b0;
+    const bool contig = (run & (run + 1)) == 0ull;
+    unsigned lo = contig ? 0u : (unsigned)bm, hi = contig ? 0u : (unsigned)(bm >> 32);
+    unsigned long long *pair4 = reinterpret_cast<unsigned long long *>(C.pair);
     for (int wbase = 0; wbase < W; wbase += COOP_WIN) {
         const int wn = W - wbase < COOP_WIN ? W - wbase : COOP_WIN;
         wave_sync();
         // list this lane's pairs of the window, bits in ascending order
+        if (contig) {
+            int i = wbase - excl > 0 ? wbase - excl : 0;
+            const int iend = wbase + wn - excl < k ? wbase + wn - excl : k;
+            int p = excl + i - wbase;
+            while (i < iend && (p & 3)) {
+                C.pair[p++] = (unsigned short)((b0 + i) << 6 | lane);
+                ++i;
+            }
+            unsigned long long v = (unsigned long long)((b0 + i) << 6 | lane) * 0x0001000100010001ull +
+                                   0x00c0008000400000ull;
+            while (i + 4 <= iend) {
+                pair4[p >> 2] = v;
+                v += 256ull * 0x0001000100010001ull;
+                p += 4;
+                i += 4;
+            }
+            while (i < iend) {
+                C.pair[p++] = (unsigned short)((b0 + i) << 6 | lane);
+                ++i;
+            }
+        }
         while (lo && pos < wbase + wn) {
             const int b = __builtin_ctz(lo);
             lo &= lo - 1;
