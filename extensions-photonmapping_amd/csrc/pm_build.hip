@@ -1308,28 +1308,40 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
     const int nA = graph ? (int)caps->active : 0;
     const int nS = graph ? (int)caps->segs : 0;
 
+    // The eager build reads the level state back once per level, after the
+    // node pass (it needs the level's segment count).  What the level writes
+    // after that is reserved from bounds: a ref goes to at most two children
+    // (next refs and leaf refs <= 2R) and a node has two children (<= 2A
+    // created nodes, leaves and box keys).  The next level's node pass runs
+    // on the bound 2A with its exact count read on the device; the loop ends
+    // at the read that finds no active node.
     int cur = 0;
-    for (int lvl = 0; graph ? lvl < caps->upper_levels : h.A > 0; ++lvl) {
+    long long A_bound = h.A;
+    for (int lvl = 0; graph ? lvl < caps->upper_levels : true; ++lvl) {
         if (!graph) {
-            ctx->level_refs.push_back(h.R);
-            sc.node_req.reserve(h.A + 1, s); sc.node_eaddr.reserve(h.A + 1, s);
-            sc.node_nseg.reserve(h.A + 1, s); sc.node_segbase.reserve(h.A + 1, s);
-            sc.slot_leaf.reserve(2 * (size_t)h.A + 1, s); sc.slot_lna.reserve(2 * (size_t)h.A + 1, s);
+            sc.node_req.reserve(A_bound + 1, s); sc.node_eaddr.reserve(A_bound + 1, s);
+            sc.node_nseg.reserve(A_bound + 1, s); sc.node_segbase.reserve(A_bound + 1, s);
+            sc.slot_leaf.reserve(2 * (size_t)A_bound + 1, s); sc.slot_lna.reserve(2 * (size_t)A_bound + 1, s);
         }
         NodePtrs NP = node_ptrs(ctx);
         // planes are kept in slot_lna's storage until the chain kernel has run
         int *planes = sc.slot_lna.ptr;
-        node_prepare_kernel<<<g_nodes(h.A), BS, 0, s>>>(NP, L, seg, sc.keys[cur].ptr, prm.empty_space_splitting,
-                                                          prm.empty_space_threshold, planes, sc.node_req.ptr,
-                                                          sc.node_nseg.ptr);
+        node_prepare_kernel<<<g_nodes(A_bound), BS, 0, s>>>(NP, L, seg, sc.keys[cur].ptr, prm.empty_space_splitting,
+                                                              prm.empty_space_threshold, planes, sc.node_req.ptr,
+                                                              sc.node_nseg.ptr);
         PM_LAUNCH_CHECK();
         scan_exclusive2(sc.node_req.ptr, sc.node_eaddr.ptr, sc.node_nseg.ptr, sc.node_segbase.ptr,
-                        graph ? nA : h.A, sc, s, dA);
+                        graph ? nA : (int)A_bound, sc, s, &L->A);
         plan_nodes_kernel<<<1, 64, 0, s>>>(L, sc.node_eaddr.ptr, sc.node_segbase.ptr);
         PM_LAUNCH_CHECK();
+        const int nxt = cur ^ 1;
         if (!graph) {
             read_state(sc, h, s);
             if (h.status) throw_status(h);
+            if (h.A == 0) break;
+            ctx->level_refs.push_back(h.R);
+            seen.refs = std::max<long long>(seen.refs, h.R);
+            seen.active = std::max<long long>(seen.active, h.A);
             nodes.reserve((size_t)h.nodes_size + 2 * (size_t)h.A, s, h.nodes_size - h.E);
             const size_t S1 = (size_t)h.S + 1;
             sc.seg_node.reserve(S1, s); sc.seg_start.reserve(S1, s); sc.seg_count.reserve(S1, s);
@@ -1337,6 +1349,12 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
             sc.segLoff.reserve(S1, s); sc.segRoff.reserve(S1, s);
             sc.segLLoff.reserve(S1, s); sc.segLRoff.reserve(S1, s);
             seen.segs = std::max<long long>(seen.segs, h.S);
+            const size_t R2 = 2 * (size_t)h.R;
+            sc.prim_min.reserve((size_t)h.final_refs + R2 + 1, s, h.final_refs);
+            sc.prim_max.reserve((size_t)h.final_refs + R2 + 1, s, h.final_refs);
+            ctx->leaf_ids.reserve((size_t)h.nleaf + 2 * (size_t)h.A + 1, s, h.nleaf);
+            sc.ref_min[nxt].reserve(R2 + 1, s); sc.ref_max[nxt].reserve(R2 + 1, s);
+            sc.keys[nxt].reserve(12 * (size_t)h.A + 6, s);
         }
         NP = node_ptrs(ctx);
         empty_chain_kernel<<<g_nodes(h.A), BS, 0, s>>>(NP, L, planes, sc.node_eaddr.ptr);
@@ -1361,18 +1379,6 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
         plan_children_kernel<<<1, 64, 0, s>>>(L, sc.segLoff.ptr, sc.segRoff.ptr, sc.segLLoff.ptr, sc.segLRoff.ptr,
                                               sc.slot_lna.ptr);
         PM_LAUNCH_CHECK();
-        const int nxt = cur ^ 1;
-        if (!graph) {
-            read_state(sc, h, s);
-            if (h.status) throw_status(h);
-            sc.prim_min.reserve((size_t)h.final_refs + h.leafTri + 1, s, h.final_refs);
-            sc.prim_max.reserve((size_t)h.final_refs + h.leafTri + 1, s, h.final_refs);
-            ctx->leaf_ids.reserve((size_t)h.nleaf + h.leaf_nodes + 1, s, h.nleaf);
-            sc.ref_min[nxt].reserve((size_t)h.nextR + 1, s); sc.ref_max[nxt].reserve((size_t)h.nextR + 1, s);
-            sc.keys[nxt].reserve(6 * (size_t)h.created + 6, s);
-            seen.refs = std::max<long long>(seen.refs, h.nextR);
-            seen.active = std::max<long long>(seen.active, h.created);
-        }
         NP = node_ptrs(ctx);
         children_kernel<<<g_nodes(h.A), BS, 0, s>>>(NP, L, prm.empty_space_splitting, sc.node_segbase.ptr,
                                                       sc.segLoff.ptr, sc.segRoff.ptr, sc.segLLoff.ptr,
@@ -1388,16 +1394,7 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
         PM_LAUNCH_CHECK();
         plan_advance_kernel<<<1, 64, 0, s>>>(L);
         PM_LAUNCH_CHECK();
-        if (!graph) {   // the same arithmetic as plan_advance_kernel
-            h.final_refs += h.leafTri;
-            h.nleaf += h.leaf_nodes;
-            h.nodes_size += 2 * h.A;
-            h.active_index = h.nodes_size - h.created;
-            h.A = h.created;
-            h.R = h.nextR;
-            h.level += 1;
-            if (h.A > 0 && h.level >= h.max_levels) { h.status = PM_ERR_DEPTH; throw_status(h); }
-        }
+        A_bound = 2ll * h.A;
         cur = nxt;
     }
     if (graph) {
