@@ -309,55 +309,59 @@ __global__ __launch_bounds__(BS) void node_prepare_kernel(NodePtrs N, const Lvl 
 
 // EmptySpaceSplitting2 (EmptySpaceSplitting.h:74-184): a chain of split
 // nodes with empty leaves between the node and its parent.
-__global__ __launch_bounds__(BS) void empty_chain_kernel(NodePtrs N, const Lvl *__restrict__ L,
-                                                         const int *__restrict__ planes_in,
-                                                         const int *__restrict__ eaddr) {
-    const int A = L->E > 0 ? L->A : 0, active_index = L->active_index;
-    GRID_STRIDE(i, A) {
-        int pl = planes_in[i];
-        if (!pl) continue;
-        const int leafID = active_index + i;
-        const int start = active_index + A + eaddr[i];
-        const int cnt = 2 * __popc(pl);
-        for (int j = 0; j < cnt; ++j) zero_node(N, start + j);
-        const int parentID = N.parent[leafID];
-        if (N.children[parentID].x == leafID) N.children[parentID].x = start;
-        else N.children[parentID].y = start;
-        const f3 minP = mk3(N.bmin[leafID]), maxP = mk3(N.bmax[leafID]);
-        int pid = start;
-        for (int a = 0; a < 3; ++a) {
-            if (!(pl & (1 << a))) continue;
-            pl -= 1 << a;
-            N.info[pid] = (int8_t)(AX_X + a);
-            N.split[pid] = a == 0 ? minP.x : (a == 1 ? minP.y : minP.z);
-            const int left = pid + 1, right = pl ? left + 1 : leafID;
-            N.children[pid] = make_int2(left, right);
-            N.info[left] = LEAF; N.bitmap[left] = 0;
-            pid = right;
-        }
-        for (int a = 0; a < 3; ++a) {
-            if (!(pl & (8 << a))) continue;
-            pl -= 8 << a;
-            // the max-z case is an X split at max.x (EmptySpaceSplitting.h:172-175)
-            N.info[pid] = (int8_t)(a == 1 ? AX_Y : AX_X);
-            N.split[pid] = a == 1 ? maxP.y : maxP.x;
-            const int right = pid + 1, left = pl ? right + 1 : leafID;
-            N.children[pid] = make_int2(left, right);
-            N.info[right] = LEAF; N.bitmap[right] = 0;
-            pid = left;
-        }
+__device__ __forceinline__ void empty_chain_node(const NodePtrs &N, int i, int A, int active_index, int pl,
+                                                 const int *__restrict__ eaddr) {
+    const int leafID = active_index + i;
+    const int start = active_index + A + eaddr[i];
+    const int cnt = 2 * __popc(pl);
+    for (int j = 0; j < cnt; ++j) zero_node(N, start + j);
+    const int parentID = N.parent[leafID];
+    if (N.children[parentID].x == leafID) N.children[parentID].x = start;
+    else N.children[parentID].y = start;
+    const f3 minP = mk3(N.bmin[leafID]), maxP = mk3(N.bmax[leafID]);
+    int pid = start;
+    for (int a = 0; a < 3; ++a) {
+        if (!(pl & (1 << a))) continue;
+        pl -= 1 << a;
+        N.info[pid] = (int8_t)(AX_X + a);
+        N.split[pid] = a == 0 ? minP.x : (a == 1 ? minP.y : minP.z);
+        const int left = pid + 1, right = pl ? left + 1 : leafID;
+        N.children[pid] = make_int2(left, right);
+        N.info[left] = LEAF; N.bitmap[left] = 0;
+        pid = right;
+    }
+    for (int a = 0; a < 3; ++a) {
+        if (!(pl & (8 << a))) continue;
+        pl -= 8 << a;
+        // the max-z case is an X split at max.x (EmptySpaceSplitting.h:172-175)
+        N.info[pid] = (int8_t)(a == 1 ? AX_Y : AX_X);
+        N.split[pid] = a == 1 ? maxP.y : maxP.x;
+        const int right = pid + 1, left = pl ? right + 1 : leafID;
+        N.children[pid] = make_int2(left, right);
+        N.info[right] = LEAF; N.bitmap[right] = 0;
+        pid = left;
     }
 }
 
-// segment table (Segment, TriangleMapUpperCreator.cu:207-239)
-__global__ __launch_bounds__(BS) void seg_fill_kernel(const NodePtrs N, const Lvl *__restrict__ L,
-                                                      int seg, const int *__restrict__ segbase,
-                                                      int *__restrict__ seg_node,
-                                                      int *__restrict__ seg_start,
-                                                      int *__restrict__ seg_count) {
-    // one thread per segment; its node is the last i with segbase[i] <= s
-    // (segbase is the exclusive scan of the nodes' segment counts)
+// Two independent passes in one launch: the empty-space chains of the
+// active nodes, and the segment table (Segment, TriangleMapUpperCreator.cu:
+// 207-239), one thread per segment.  The chains touch only new nodes and the
+// parents' child links; the table reads the active nodes' ref ranges.
+__global__ __launch_bounds__(BS) void chain_segs_kernel(const NodePtrs N, const Lvl *__restrict__ L,
+                                                        const int *__restrict__ planes_in,
+                                                        const int *__restrict__ eaddr, int seg,
+                                                        const int *__restrict__ segbase,
+                                                        int *__restrict__ seg_node,
+                                                        int *__restrict__ seg_start,
+                                                        int *__restrict__ seg_count) {
     const int A = L->A, S = L->S, active_index = L->active_index;
+    const int Ac = L->E > 0 ? A : 0;
+    GRID_STRIDE(i, Ac) {
+        const int pl = planes_in[i];
+        if (pl) empty_chain_node(N, i, A, active_index, pl, eaddr);
+    }
+    // a segment's node is the last i with segbase[i] <= s (segbase is the
+    // exclusive scan of the nodes' segment counts)
     GRID_STRIDE(s, S) {
         int lo = 0, hi = A - 1;
         while (lo < hi) {
@@ -425,34 +429,40 @@ __global__ __launch_bounds__(WPB * 64) void classify_kernel(
     }
 }
 
-// CalcNodeChildSize (TriangleUpperChildren.h:144-162) + MarkNodeLeafs (:190-201)
-__global__ __launch_bounds__(BS) void child_size_kernel(const Lvl *__restrict__ L, int LS,
+// CalcNodeChildSize (TriangleUpperChildren.h:144-162) + MarkNodeLeafs
+// (:190-201) per node, and SetPrimitiveLeafSide (:164-186) counted per
+// segment, in one launch: a segment derives its node's leaf flags from the
+// same scanned counts the node pass reads.
+__device__ __forceinline__ void child_counts(int i, const int *__restrict__ segbase,
+                                             const int *__restrict__ segLoff, const int *__restrict__ segRoff,
+                                             int &nl, int &nr) {
+    const int b = segbase[i], e = segbase[i + 1];
+    nl = segLoff[e] - segLoff[b];
+    nr = segRoff[e] - segRoff[b];
+}
+
+__global__ __launch_bounds__(BS) void leaf_flags_kernel(const Lvl *__restrict__ L, int LS,
                                                         const int *__restrict__ segbase,
                                                         const int *__restrict__ segLoff,
                                                         const int *__restrict__ segRoff,
-                                                        int *__restrict__ slot_leaf) {
-    const int A = L->A;
+                                                        const int *__restrict__ seg_node,
+                                                        const int *__restrict__ segL,
+                                                        const int *__restrict__ segR,
+                                                        int *__restrict__ slot_leaf,
+                                                        int *__restrict__ segLL,
+                                                        int *__restrict__ segLR) {
+    const int A = L->A, S = L->S;
     GRID_STRIDE(i, A) {
-        const int b = segbase[i], e = segbase[i + 1];
-        const int nl = segLoff[e] - segLoff[b];
-        const int nr = segRoff[e] - segRoff[b];
+        int nl, nr;
+        child_counts(i, segbase, segLoff, segRoff, nl, nr);
         slot_leaf[i] = nl < LS;
         slot_leaf[A + i] = nr < LS;
     }
-}
-
-// SetPrimitiveLeafSide (TriangleUpperChildren.h:164-186) counted per segment
-__global__ __launch_bounds__(BS) void seg_leaf_kernel(const Lvl *__restrict__ L, const int *__restrict__ seg_node,
-                                                      const int *__restrict__ slot_leaf,
-                                                      const int *__restrict__ segL,
-                                                      const int *__restrict__ segR,
-                                                      int *__restrict__ segLL,
-                                                      int *__restrict__ segLR) {
-    const int A = L->A, S = L->S;
     GRID_STRIDE(s, S) {
-        const int i = seg_node[s];
-        segLL[s] = slot_leaf[i] ? segL[s] : 0;
-        segLR[s] = slot_leaf[A + i] ? segR[s] : 0;
+        int nl, nr;
+        child_counts(seg_node[s], segbase, segLoff, segRoff, nl, nr);
+        segLL[s] = nl < LS ? segL[s] : 0;
+        segLR[s] = nr < LS ? segR[s] : 0;
     }
 }
 
@@ -1357,10 +1367,9 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
             sc.keys[nxt].reserve(12 * (size_t)h.A + 6, s);
         }
         NP = node_ptrs(ctx);
-        empty_chain_kernel<<<g_nodes(h.A), BS, 0, s>>>(NP, L, planes, sc.node_eaddr.ptr);
-        PM_LAUNCH_CHECK();
-        seg_fill_kernel<<<g_segs(h.S), BS, 0, s>>>(NP, L, seg, sc.node_segbase.ptr, sc.seg_node.ptr,
-                                                     sc.seg_start.ptr, sc.seg_count.ptr);
+        chain_segs_kernel<<<std::max(g_nodes(h.A), g_segs(h.S)), BS, 0, s>>>(
+            NP, L, planes, sc.node_eaddr.ptr, seg, sc.node_segbase.ptr, sc.seg_node.ptr, sc.seg_start.ptr,
+            sc.seg_count.ptr);
         PM_LAUNCH_CHECK();
         launch_classify(seg, g_waves(h.S), s, NP, (const Lvl *)L, (const int *)sc.seg_node.ptr,
                         (const int *)sc.seg_start.ptr, (const int *)sc.seg_count.ptr,
@@ -1368,11 +1377,9 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
                         sc.segR.ptr, dg);
         PM_LAUNCH_CHECK();
         scan_exclusive2(sc.segL.ptr, sc.segLoff.ptr, sc.segR.ptr, sc.segRoff.ptr, graph ? nS : h.S, sc, s, dS);
-        child_size_kernel<<<g_nodes(h.A), BS, 0, s>>>(L, LS, sc.node_segbase.ptr, sc.segLoff.ptr, sc.segRoff.ptr,
-                                                        sc.slot_leaf.ptr);
-        PM_LAUNCH_CHECK();
-        seg_leaf_kernel<<<g_segs(h.S), BS, 0, s>>>(L, sc.seg_node.ptr, sc.slot_leaf.ptr, sc.segL.ptr, sc.segR.ptr,
-                                                     sc.segLL.ptr, sc.segLR.ptr);
+        leaf_flags_kernel<<<std::max(g_nodes(h.A), g_segs(h.S)), BS, 0, s>>>(
+            L, LS, sc.node_segbase.ptr, sc.segLoff.ptr, sc.segRoff.ptr, sc.seg_node.ptr, sc.segL.ptr, sc.segR.ptr,
+            sc.slot_leaf.ptr, sc.segLL.ptr, sc.segLR.ptr);
         PM_LAUNCH_CHECK();
         scan_exclusive2(sc.segLL.ptr, sc.segLLoff.ptr, sc.segLR.ptr, sc.segLRoff.ptr, graph ? nS : h.S, sc, s, dS);
         scan_exclusive(sc.slot_leaf.ptr, sc.slot_lna.ptr, graph ? 2 * nA : 2 * h.A, sc, s, dA, 2);
