@@ -198,7 +198,7 @@ __global__ __launch_bounds__(BS) void tri_box_kernel(const float4 *__restrict__ 
                                                      float4 *__restrict__ rmin,
                                                      float4 *__restrict__ rmax,
                                                      const int *__restrict__ off, int M,
-                                                     int *__restrict__ keys) {
+                                                     int *__restrict__ keys, float4 *__restrict__ rec) {
     __shared__ int red[6][BS / 64];
     const int t0 = blockIdx.x * BOX_TILE;
     const int t1 = min(t0 + BOX_TILE, n) - 1;
@@ -220,6 +220,11 @@ __global__ __launch_bounds__(BS) void tri_box_kernel(const float4 *__restrict__ 
             mxx = omax(mxx, c.x); mxy = omax(mxy, c.y); mxz = omax(mxz, c.z);
             rmin[i] = make_float4(mnx, mny, mnz, __int_as_float(i));
             rmax[i] = make_float4(mxx, mxy, mxz, 0.0f);
+            // the tracers' leaf record of triangle i (v0 | id, e1, e2), copied
+            // into the leaf slots once the tree is built
+            rec[3ll * i] = make_float4(a.x, a.y, a.z, __int_as_float(i));
+            rec[3ll * i + 1] = make_float4(b.x - a.x, b.y - a.y, b.z - a.z, 0.0f);
+            rec[3ll * i + 2] = make_float4(c.x - a.x, c.y - a.y, c.z - a.z, 0.0f);
             e0 = fkey(mnx); e1 = fkey(mny); e2 = fkey(mnz); e3 = fkey(mxx); e4 = fkey(mxy); e5 = fkey(mxz);
         }
         if (tile_uniform) {
@@ -1296,6 +1301,7 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
     if (!graph) {
         nodes.reserve(std::max(1024, 2 * M), s, 0);
         sc.ref_min[0].reserve((size_t)N + 1, s); sc.ref_max[0].reserve((size_t)N + 1, s);
+        sc.tri_rec.reserve(3 * (size_t)N + 3, s);
         sc.keys[0].reserve(6 * (size_t)M, s);
     }
     init_roots_kernel<<<blocks_for(M, BS), BS, 0, s>>>(node_ptrs(ctx), off_dev, M);
@@ -1304,7 +1310,8 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
     PM_LAUNCH_CHECK();
     if (N > 0) {
         tri_box_kernel<<<blocks_for(N, BOX_TILE), BS, 0, s>>>(P0, P1, P2, N, sc.ref_min[0].ptr,
-                                                              sc.ref_max[0].ptr, off_dev, M, sc.keys[0].ptr);
+                                                              sc.ref_max[0].ptr, off_dev, M, sc.keys[0].ptr,
+                                                              sc.tri_rec.ptr);
         PM_LAUNCH_CHECK();
     }
 
@@ -1425,7 +1432,8 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
     const bool sah = prm.lower_algorithm != PM_LOWER_BITMAP;   // SAH or balanced: lower levels
     const bool need_areas = prm.lower_algorithm == PM_LOWER_SAH;
     if (need_areas && !graph) sc.areas.reserve((size_t)h.final_refs + 1, s);
-    build_leaf_view(ctx, need_areas ? sc.areas.ptr : nullptr, false, s, graph ? &L->final_refs : nullptr);
+    build_leaf_view(ctx, need_areas ? sc.areas.ptr : nullptr, false, s, graph ? &L->final_refs : nullptr,
+                    N > 0 ? sc.tri_rec.ptr : nullptr);
 
     // lower stage
     const bool balanced = prm.lower_algorithm == PM_LOWER_BALANCED;
@@ -1559,7 +1567,7 @@ std::vector<void **> graph_buffers(pm_ctx *ctx) {
         (void **)&sc.segLLoff.ptr, (void **)&sc.segLRoff.ptr, (void **)&sc.slot_leaf.ptr,
         (void **)&sc.slot_lna.ptr, (void **)&sc.areas.ptr, (void **)&sc.lower_flag.ptr,
         (void **)&sc.lower_off.ptr, (void **)&sc.lower_sets.ptr, (void **)&sc.lower_careas.ptr,
-        (void **)&sc.lower_list.ptr, (void **)&ctx->graph_off.ptr, (void **)&sc.lvl.ptr};
+        (void **)&sc.lower_list.ptr, (void **)&sc.tri_rec.ptr, (void **)&ctx->graph_off.ptr, (void **)&sc.lvl.ptr};
     for (int i = 0; i < 2; ++i) {
         v.push_back((void **)&sc.ref_min[i].ptr);
         v.push_back((void **)&sc.ref_max[i].ptr);
@@ -1639,6 +1647,7 @@ void capture_batch(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4
     sc.lower_flag.reserve((size_t)caps.lower + 1, s);
     sc.lower_off.reserve((size_t)caps.lower + 1, s);
     sc.lower_list.reserve((size_t)caps.lower + 1, s);
+    sc.tri_rec.reserve(3 * (size_t)N + 3, s);
     sc.lower_sets.reserve(2 * (size_t)caps.lower, s);
     sc.lower_careas.reserve(2 * (size_t)caps.lower, s);
     ctx->packed.reserve((size_t)caps.nodes + 1, s);

@@ -155,6 +155,7 @@ struct BuildScratch {
     DevBuf<int> order_hist, order_offs;         // coherent ray order: radix histograms
     DevBuf<int> trace_work;                     // tracers: rays handed out so far
     DevBuf<int> mesh_off;                       // forest: triangle offsets per mesh (M+1)
+    DevBuf<float4> tri_rec;                     // leaf records (v0 | id, e1, e2) in triangle order
     DevBuf<Lvl> lvl;                            // level state (one record)
     HostPinned host;
 };
@@ -220,9 +221,12 @@ void build_trace_view(pm_ctx *ctx, hipStream_t s);
 // the two halves of the trace view: packed nodes, and the leaf slots'
 // triangles (plus their areas when `areas` is given; `check` verifies the
 // prim indices, needed for uploaded trees only).  With n_dev / m_dev the
-// count is read on the device and ctx's count is its upper bound.
+// count is read on the device and ctx's count is its upper bound.  `rec`:
+// the triangles' leaf records in triangle order (3 float4 each), copied
+// instead of formed from the vertex arrays.
 void build_node_view(pm_ctx *ctx, hipStream_t s, const int *n_dev = nullptr);
-void build_leaf_view(pm_ctx *ctx, float *areas, bool check, hipStream_t s, const int *m_dev = nullptr);
+void build_leaf_view(pm_ctx *ctx, float *areas, bool check, hipStream_t s, const int *m_dev = nullptr,
+                     const float4 *rec = nullptr);
 void trace_canvas(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int nrays, const float4 *n0,
                   const float4 *n1, const float4 *n2, const uchar4 *c0, const pm_light &light, int max_bounces,
                   uchar4 *canvas, const pm_trace_params &prm, hipStream_t s);

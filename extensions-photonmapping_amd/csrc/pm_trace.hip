@@ -647,6 +647,25 @@ __global__ __launch_bounds__(TB) void leaf_tris_kernel(const int *__restrict__ p
         leaf_slot(prim_indices, k, P0, P1, P2, ntris, out, bad, areas);
 }
 
+// the same slots from the triangles' records (a build's prim indices are in
+// range): one contiguous 48-byte gather per slot
+__global__ __launch_bounds__(TB) void leaf_copy_kernel(const int *__restrict__ prim_indices, int m_host,
+                                                       const int *m_dev, const float4 *__restrict__ rec,
+                                                       float4 *__restrict__ out, float *__restrict__ areas) {
+    const int m = m_dev ? *m_dev : m_host;
+    for (int k = blockIdx.x * TB + threadIdx.x; k < m; k += gridDim.x * TB) {
+        const int t = prim_indices[k];
+        const float4 r0 = rec[3ll * t], r1 = rec[3ll * t + 1], r2 = rec[3ll * t + 2];
+        out[3ll * k] = r0;
+        out[3ll * k + 1] = r1;
+        out[3ll * k + 2] = r2;
+        if (areas) {
+            const f3 c = cross3(mk3(r1), mk3(r2));
+            areas[k] = 0.5f * sqrtf(dot3(c, c));
+        }
+    }
+}
+
 // ---- canvas shading: ColorKernels.h:15-85 in the tracers' shading loops
 // (Raytracer.cu:165-262, ShortStack.cu:96-194, BruteTracer.cu:54-96); the
 // specular power is the correctly rounded float of the double power
@@ -865,7 +884,7 @@ void build_node_view(pm_ctx *ctx, hipStream_t s, const int *n_dev) {
     }
 }
 
-void build_leaf_view(pm_ctx *ctx, float *areas, bool check, hipStream_t s, const int *m_dev) {
+void build_leaf_view(pm_ctx *ctx, float *areas, bool check, hipStream_t s, const int *m_dev, const float4 *rec) {
     const int m = ctx->num_prim_indices;
     ctx->leaf_tris.reserve(3 * (size_t)m + 3, s);
     if (m == 0) return;
@@ -879,8 +898,11 @@ void build_leaf_view(pm_ctx *ctx, float *areas, bool check, hipStream_t s, const
         bad = sc.totals.ptr;
     }
     const int blocks = std::min((m + TB - 1) / TB, m_dev ? 8192 : INT_MAX);
-    leaf_tris_kernel<<<blocks, TB, 0, s>>>(ctx->prim_indices.ptr, m, m_dev, ctx->p0, ctx->p1, ctx->p2,
-                                           ctx->ntris, ctx->leaf_tris.ptr, bad, areas);
+    if (rec && !check)
+        leaf_copy_kernel<<<blocks, TB, 0, s>>>(ctx->prim_indices.ptr, m, m_dev, rec, ctx->leaf_tris.ptr, areas);
+    else
+        leaf_tris_kernel<<<blocks, TB, 0, s>>>(ctx->prim_indices.ptr, m, m_dev, ctx->p0, ctx->p1, ctx->p2,
+                                               ctx->ntris, ctx->leaf_tris.ptr, bad, areas);
     PM_LAUNCH_CHECK();
     if (check) {
         PM_HIP(hipMemcpyAsync(sc.host.ptr, sc.totals.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
