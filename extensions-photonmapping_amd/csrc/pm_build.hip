@@ -61,11 +61,34 @@ struct ScanIO {
 // n = n_dev ? n_mult * *n_dev : n_host (a captured build scans lengths it
 // never reads back; the grid then covers the capacity, and blocks past n
 // write zero block sums)
-template <int K>
+//
+// PRE: the block first adds up the sums of the blocks before it
+// (pre_sums[q * sums_stride + b], b < blockIdx.x <= SCAN_T), so that a scan
+// of up to SCAN_T blocks is a reduce pass and this pass.
+template <int K, bool PRE = false>
 __global__ __launch_bounds__(SCAN_T) void scan_block_kernel(ScanIO<K> io, int n_host, const int *n_dev, int n_mult,
-                                                            int *__restrict__ block_sums, int sums_stride) {
+                                                            int *__restrict__ block_sums, int sums_stride,
+                                                            const int *__restrict__ pre_sums = nullptr) {
     __shared__ int wsum[K][SCAN_T / 64];
+    __shared__ int wpre[K][SCAN_T / 64];
     const int n = n_dev ? n_mult * *n_dev : n_host;
+    if (PRE) {
+        const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            int x = (int)threadIdx.x < (int)blockIdx.x ? pre_sums[q * sums_stride + threadIdx.x] : 0;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+            if (lane == 0) wpre[q][w] = x;
+        }
+        __syncthreads();
+        if (threadIdx.x < K) {
+            int t = 0;
+            for (int j = 0; j < SCAN_T / 64; ++j) t += wpre[threadIdx.x][j];
+            wpre[threadIdx.x][0] = t;
+        }
+        // the __syncthreads below (before wsum is read) orders this write
+    }
     const long long base = (long long)blockIdx.x * SCAN_BLK + (long long)threadIdx.x * SCAN_E;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     int v[K][SCAN_E], tsum[K], incl[K];
@@ -94,6 +117,7 @@ __global__ __launch_bounds__(SCAN_T) void scan_block_kernel(ScanIO<K> io, int n_
 #pragma unroll
     for (int q = 0; q < K; ++q) {
         int run = (w ? wsum[q][w - 1] : 0) + incl[q] - tsum[q];
+        if (PRE) run += wpre[q][0];
 #pragma unroll
         for (int j = 0; j < SCAN_E; ++j) {
             const long long k = base + j;
@@ -102,6 +126,34 @@ __global__ __launch_bounds__(SCAN_T) void scan_block_kernel(ScanIO<K> io, int n_
         }
         if (block_sums && threadIdx.x == SCAN_T - 1)
             block_sums[q * sums_stride + blockIdx.x] = wsum[q][SCAN_T / 64 - 1];
+    }
+}
+
+// per-block sums only (the first pass of a PRE scan); blocks past n write 0
+template <int K>
+__global__ __launch_bounds__(SCAN_T) void scan_reduce_kernel(ScanIO<K> io, int n_host, const int *n_dev, int n_mult,
+                                                             int *__restrict__ block_sums, int sums_stride) {
+    __shared__ int wsum[K][SCAN_T / 64];
+    const int n = n_dev ? n_mult * *n_dev : n_host;
+    const long long base = (long long)blockIdx.x * SCAN_BLK + (long long)threadIdx.x * SCAN_E;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        int t = 0;
+#pragma unroll
+        for (int j = 0; j < SCAN_E; ++j) {
+            const long long k = base + j;
+            t += k < n ? io.in[q][k] : 0;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+        if (lane == 0) wsum[q][w] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < K) {
+        int t = 0;
+        for (int j = 0; j < SCAN_T / 64; ++j) t += wsum[threadIdx.x][j];
+        block_sums[threadIdx.x * sums_stride + blockIdx.x] = t;
     }
 }
 
@@ -1201,6 +1253,13 @@ static void scan_level(ScanIO<K> io, int n, const int *n_dev, int n_mult, BuildS
     DevBuf<int> &tmp = sc.scan_tmp[depth];
     const int stride = nb + 1;
     tmp.reserve((size_t)K * stride, s);
+    if (nb <= SCAN_T) {   // block sums, then each block adds up those before it
+        scan_reduce_kernel<K><<<nb, SCAN_T, 0, s>>>(io, n, n_dev, n_mult, tmp.ptr, stride);
+        PM_LAUNCH_CHECK();
+        scan_block_kernel<K, true><<<nb, SCAN_T, 0, s>>>(io, n, n_dev, n_mult, nullptr, stride, tmp.ptr);
+        PM_LAUNCH_CHECK();
+        return;
+    }
     scan_block_kernel<K><<<nb, SCAN_T, 0, s>>>(io, n, n_dev, n_mult, tmp.ptr, stride);
     PM_LAUNCH_CHECK();
     ScanIO<K> sums;
