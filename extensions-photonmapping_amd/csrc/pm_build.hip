@@ -48,32 +48,39 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
     return v;
 }
 
-// K independent exclusive scans of the same length in one pass (the level
-// loop scans its counts in pairs): in[k][0..n) with in[k][n] treated as 0,
-// i.e. out[k][0..n].  in may equal out (each thread reads its elements
-// before it writes them).
+// K independent exclusive scans in one pass (the level loop scans its
+// counts two or three at a time): in[q][0..n_q) with in[q][n_q] treated as
+// 0, i.e. out[q][0..n_q].  n_q = n_mult[q] * *n_dev[q] when n_dev[q] is set
+// (a captured build scans lengths it never reads back; the grid then covers
+// n_host[q], the capacity, and blocks past n_q write zero block sums), else
+// n_host[q].  in may equal out (each thread reads its elements before it
+// writes them).
 template <int K>
 struct ScanIO {
     const int *in[K];
     int *out[K];
+    int n_host[K];
+    const int *n_dev[K];
+    int n_mult[K];
 };
 
-// n = n_dev ? n_mult * *n_dev : n_host (a captured build scans lengths it
-// never reads back; the grid then covers the capacity, and blocks past n
-// write zero block sums)
-//
+template <int K>
+__device__ __forceinline__ int scan_len(const ScanIO<K> &io, int q) {
+    return io.n_dev[q] ? io.n_mult[q] * *io.n_dev[q] : io.n_host[q];
+}
+
 // PRE: the block first adds up the sums of the blocks before it
 // (pre_sums[q * sums_stride + b], b < blockIdx.x <= SCAN_T), so that a scan
 // of up to SCAN_T blocks is a reduce pass and this pass.
 template <int K, bool PRE = false>
-__global__ __launch_bounds__(SCAN_T) void scan_block_kernel(ScanIO<K> io, int n_host, const int *n_dev, int n_mult,
-                                                            int *__restrict__ block_sums, int sums_stride,
+__global__ __launch_bounds__(SCAN_T) void scan_block_kernel(ScanIO<K> io, int *__restrict__ block_sums,
+                                                            int sums_stride,
                                                             const int *__restrict__ pre_sums = nullptr) {
     __shared__ int wsum[K][SCAN_T / 64];
     __shared__ int wpre[K][SCAN_T / 64];
-    const int n = n_dev ? n_mult * *n_dev : n_host;
+    const long long base = (long long)blockIdx.x * SCAN_BLK + (long long)threadIdx.x * SCAN_E;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (PRE) {
-        const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
         for (int q = 0; q < K; ++q) {
             int x = (int)threadIdx.x < (int)blockIdx.x ? pre_sums[q * sums_stride + threadIdx.x] : 0;
@@ -89,16 +96,15 @@ __global__ __launch_bounds__(SCAN_T) void scan_block_kernel(ScanIO<K> io, int n_
         }
         // the __syncthreads below (before wsum is read) orders this write
     }
-    const long long base = (long long)blockIdx.x * SCAN_BLK + (long long)threadIdx.x * SCAN_E;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int v[K][SCAN_E], tsum[K], incl[K];
+    int n[K], v[K][SCAN_E], tsum[K], incl[K];
 #pragma unroll
     for (int q = 0; q < K; ++q) {
+        n[q] = scan_len(io, q);
         tsum[q] = 0;
 #pragma unroll
         for (int j = 0; j < SCAN_E; ++j) {
             const long long k = base + j;
-            v[q][j] = k < n ? io.in[q][k] : 0;
+            v[q][j] = k < n[q] ? io.in[q][k] : 0;
             tsum[q] += v[q][j];
         }
         incl[q] = wave_incl_scan(tsum[q]);
@@ -121,7 +127,7 @@ __global__ __launch_bounds__(SCAN_T) void scan_block_kernel(ScanIO<K> io, int n_
 #pragma unroll
         for (int j = 0; j < SCAN_E; ++j) {
             const long long k = base + j;
-            if (k <= n) io.out[q][k] = run;
+            if (k <= n[q]) io.out[q][k] = run;
             run += v[q][j];
         }
         if (block_sums && threadIdx.x == SCAN_T - 1)
@@ -129,16 +135,16 @@ __global__ __launch_bounds__(SCAN_T) void scan_block_kernel(ScanIO<K> io, int n_
     }
 }
 
-// per-block sums only (the first pass of a PRE scan); blocks past n write 0
+// per-block sums only (the first pass of a PRE scan); blocks past n_q write 0
 template <int K>
-__global__ __launch_bounds__(SCAN_T) void scan_reduce_kernel(ScanIO<K> io, int n_host, const int *n_dev, int n_mult,
-                                                             int *__restrict__ block_sums, int sums_stride) {
+__global__ __launch_bounds__(SCAN_T) void scan_reduce_kernel(ScanIO<K> io, int *__restrict__ block_sums,
+                                                             int sums_stride) {
     __shared__ int wsum[K][SCAN_T / 64];
-    const int n = n_dev ? n_mult * *n_dev : n_host;
     const long long base = (long long)blockIdx.x * SCAN_BLK + (long long)threadIdx.x * SCAN_E;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
     for (int q = 0; q < K; ++q) {
+        const int n = scan_len(io, q);
         int t = 0;
 #pragma unroll
         for (int j = 0; j < SCAN_E; ++j) {
@@ -158,12 +164,12 @@ __global__ __launch_bounds__(SCAN_T) void scan_reduce_kernel(ScanIO<K> io, int n
 }
 
 template <int K>
-__global__ __launch_bounds__(SCAN_T) void scan_add_kernel(ScanIO<K> io, int n_host, const int *n_dev, int n_mult,
-                                                          const int *__restrict__ offs, int offs_stride) {
-    const int n = n_dev ? n_mult * *n_dev : n_host;
+__global__ __launch_bounds__(SCAN_T) void scan_add_kernel(ScanIO<K> io, const int *__restrict__ offs,
+                                                          int offs_stride) {
     const long long base = (long long)blockIdx.x * SCAN_BLK;
 #pragma unroll
     for (int q = 0; q < K; ++q) {
+        const int n = scan_len(io, q);
         const int add = offs[q * offs_stride + blockIdx.x];
         for (int j = threadIdx.x; j < SCAN_BLK; j += SCAN_T) {
             const long long k = base + j;
@@ -1237,15 +1243,17 @@ NodePtrs node_ptrs(pm_ctx *ctx) {
 
 }  // namespace
 
-// exclusive scans, out[n] = total: one pass when n fits a block, else
-// block scans, a scan of the block sums (recursively, in place) and an add.
-// With n_dev the length is read on the device and n is its upper bound.
+// exclusive scans, out[n] = total: one pass when n fits a block, two (block
+// sums, then block scans that add the sums before them) up to SCAN_T
+// blocks, else block scans, a scan of the block sums (recursively, in
+// place) and an add.  The grid covers the largest n_host.
 template <int K>
-static void scan_level(ScanIO<K> io, int n, const int *n_dev, int n_mult, BuildScratch &sc, int depth,
-                       hipStream_t s) {
+static void scan_level(ScanIO<K> io, BuildScratch &sc, int depth, hipStream_t s) {
+    int n = 0;
+    for (int q = 0; q < K; ++q) n = std::max(n, io.n_host[q]);
     const int nb = blocks_for((long long)n + 1, SCAN_BLK);
     if (nb == 1) {
-        scan_block_kernel<K><<<1, SCAN_T, 0, s>>>(io, n, n_dev, n_mult, nullptr, 0);
+        scan_block_kernel<K><<<1, SCAN_T, 0, s>>>(io, nullptr, 0);
         PM_LAUNCH_CHECK();
         return;
     }
@@ -1253,38 +1261,58 @@ static void scan_level(ScanIO<K> io, int n, const int *n_dev, int n_mult, BuildS
     DevBuf<int> &tmp = sc.scan_tmp[depth];
     const int stride = nb + 1;
     tmp.reserve((size_t)K * stride, s);
-    if (nb <= SCAN_T) {   // block sums, then each block adds up those before it
-        scan_reduce_kernel<K><<<nb, SCAN_T, 0, s>>>(io, n, n_dev, n_mult, tmp.ptr, stride);
+    if (nb <= SCAN_T) {
+        scan_reduce_kernel<K><<<nb, SCAN_T, 0, s>>>(io, tmp.ptr, stride);
         PM_LAUNCH_CHECK();
-        scan_block_kernel<K, true><<<nb, SCAN_T, 0, s>>>(io, n, n_dev, n_mult, nullptr, stride, tmp.ptr);
+        scan_block_kernel<K, true><<<nb, SCAN_T, 0, s>>>(io, nullptr, stride, tmp.ptr);
         PM_LAUNCH_CHECK();
         return;
     }
-    scan_block_kernel<K><<<nb, SCAN_T, 0, s>>>(io, n, n_dev, n_mult, tmp.ptr, stride);
+    scan_block_kernel<K><<<nb, SCAN_T, 0, s>>>(io, tmp.ptr, stride);
     PM_LAUNCH_CHECK();
     ScanIO<K> sums;
     for (int q = 0; q < K; ++q) {
         sums.in[q] = tmp.ptr + q * stride;
         sums.out[q] = tmp.ptr + q * stride;
+        sums.n_host[q] = nb;
+        sums.n_dev[q] = nullptr;
+        sums.n_mult[q] = 1;
     }
-    scan_level<K>(sums, nb, nullptr, 1, sc, depth + 1, s);
-    scan_add_kernel<K><<<nb, SCAN_T, 0, s>>>(io, n, n_dev, n_mult, tmp.ptr, stride);
+    scan_level<K>(sums, sc, depth + 1, s);
+    scan_add_kernel<K><<<nb, SCAN_T, 0, s>>>(io, tmp.ptr, stride);
     PM_LAUNCH_CHECK();
+}
+
+template <int K>
+static void scan_set(ScanIO<K> &io, int q, const int *in, int *out, int n, const int *n_dev, int n_mult) {
+    io.in[q] = in; io.out[q] = out;
+    io.n_host[q] = n; io.n_dev[q] = n_dev; io.n_mult[q] = n_mult;
 }
 
 void scan_exclusive(const int *in, int *out, int n, BuildScratch &sc, hipStream_t s, const int *n_dev,
                     int n_mult) {
     ScanIO<1> io;
-    io.in[0] = in; io.out[0] = out;
-    scan_level<1>(io, n, n_dev, n_mult, sc, 0, s);
+    scan_set(io, 0, in, out, n, n_dev, n_mult);
+    scan_level<1>(io, sc, 0, s);
 }
 
 void scan_exclusive2(const int *in0, int *out0, const int *in1, int *out1, int n, BuildScratch &sc,
                      hipStream_t s, const int *n_dev, int n_mult) {
     ScanIO<2> io;
-    io.in[0] = in0; io.out[0] = out0;
-    io.in[1] = in1; io.out[1] = out1;
-    scan_level<2>(io, n, n_dev, n_mult, sc, 0, s);
+    scan_set(io, 0, in0, out0, n, n_dev, n_mult);
+    scan_set(io, 1, in1, out1, n, n_dev, n_mult);
+    scan_level<2>(io, sc, 0, s);
+}
+
+// two scans of length n (n_dev) and a third of length m (m_dev, m_mult)
+static void scan_exclusive3(const int *in0, int *out0, const int *in1, int *out1, int n, const int *n_dev,
+                            const int *in2, int *out2, int m, const int *m_dev, int m_mult, BuildScratch &sc,
+                            hipStream_t s) {
+    ScanIO<3> io;
+    scan_set(io, 0, in0, out0, n, n_dev, 1);
+    scan_set(io, 1, in1, out1, n, n_dev, 1);
+    scan_set(io, 2, in2, out2, m, m_dev, m_mult);
+    scan_level<3>(io, sc, 0, s);
 }
 
 namespace {
@@ -1447,8 +1475,8 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
             L, LS, sc.node_segbase.ptr, sc.segLoff.ptr, sc.segRoff.ptr, sc.seg_node.ptr, sc.segL.ptr, sc.segR.ptr,
             sc.slot_leaf.ptr, sc.segLL.ptr, sc.segLR.ptr);
         PM_LAUNCH_CHECK();
-        scan_exclusive2(sc.segLL.ptr, sc.segLLoff.ptr, sc.segLR.ptr, sc.segLRoff.ptr, graph ? nS : h.S, sc, s, dS);
-        scan_exclusive(sc.slot_leaf.ptr, sc.slot_lna.ptr, graph ? 2 * nA : 2 * h.A, sc, s, dA, 2);
+        scan_exclusive3(sc.segLL.ptr, sc.segLLoff.ptr, sc.segLR.ptr, sc.segLRoff.ptr, graph ? nS : h.S, dS,
+                        sc.slot_leaf.ptr, sc.slot_lna.ptr, graph ? 2 * nA : 2 * h.A, dA, 2, sc, s);
         plan_children_kernel<<<1, 64, 0, s>>>(L, sc.segLoff.ptr, sc.segRoff.ptr, sc.segLLoff.ptr, sc.segLRoff.ptr,
                                               sc.slot_lna.ptr);
         PM_LAUNCH_CHECK();
@@ -1650,7 +1678,7 @@ void reserve_scan_tmp(BuildScratch &sc, long long n, hipStream_t s) {
     for (int depth = 0; depth < 3; ++depth) {
         const long long nb = blocks_for(n + 1, SCAN_BLK);
         if (nb <= 1) return;
-        sc.scan_tmp[depth].reserve(2 * (size_t)(nb + 1), s);
+        sc.scan_tmp[depth].reserve(3 * (size_t)(nb + 1), s);   // up to three scans at a time
         n = nb;
     }
 }
