@@ -55,6 +55,10 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
 // n_host[q], the capacity, and blocks past n_q write zero block sums), else
 // n_host[q].  in may equal out (each thread reads its elements before it
 // writes them).
+//
+// nodes_plan: the node pass's scan (K = 2, empty-space node and segment
+// counts of the level) applies plan_nodes (below) with its two totals, from
+// the thread that writes them.
 template <int K>
 struct ScanIO {
     const int *in[K];
@@ -62,7 +66,11 @@ struct ScanIO {
     int n_host[K];
     const int *n_dev[K];
     int n_mult[K];
+    Lvl *nodes_plan;
 };
+
+__device__ void plan_nodes(Lvl *L, int E, int S);
+__device__ void lvl_fail(Lvl *L, int code);
 
 template <int K>
 __device__ __forceinline__ int scan_len(const ScanIO<K> &io, int q) {
@@ -120,19 +128,25 @@ __global__ __launch_bounds__(SCAN_T) void scan_block_kernel(ScanIO<K> io, int *_
         }
     }
     __syncthreads();
+    int tot[K];
 #pragma unroll
     for (int q = 0; q < K; ++q) {
         int run = (w ? wsum[q][w - 1] : 0) + incl[q] - tsum[q];
         if (PRE) run += wpre[q][0];
+        tot[q] = 0;
 #pragma unroll
         for (int j = 0; j < SCAN_E; ++j) {
             const long long k = base + j;
             if (k <= n[q]) io.out[q][k] = run;
+            if (k == n[q]) tot[q] = run;
             run += v[q][j];
         }
         if (block_sums && threadIdx.x == SCAN_T - 1)
             block_sums[q * sums_stride + blockIdx.x] = wsum[q][SCAN_T / 64 - 1];
     }
+    // final totals are written here unless block sums are still to be added
+    if (K == 2 && io.nodes_plan && !block_sums && base <= n[0] && n[0] < base + SCAN_E)
+        plan_nodes(io.nodes_plan, tot[0], tot[K - 1]);
 }
 
 // per-block sums only (the first pass of a PRE scan); blocks past n_q write 0
@@ -167,15 +181,20 @@ template <int K>
 __global__ __launch_bounds__(SCAN_T) void scan_add_kernel(ScanIO<K> io, const int *__restrict__ offs,
                                                           int offs_stride) {
     const long long base = (long long)blockIdx.x * SCAN_BLK;
+    int tot[K];
+    bool last = false;
 #pragma unroll
     for (int q = 0; q < K; ++q) {
         const int n = scan_len(io, q);
         const int add = offs[q * offs_stride + blockIdx.x];
+        tot[q] = 0;
         for (int j = threadIdx.x; j < SCAN_BLK; j += SCAN_T) {
             const long long k = base + j;
             if (k <= n) io.out[q][k] += add;
+            if (k == n) { tot[q] = io.out[q][k]; last = true; }
         }
     }
+    if (K == 2 && io.nodes_plan && last) plan_nodes(io.nodes_plan, tot[0], tot[K - 1]);
 }
 
 // ------------------------------------------------------------------ helpers
@@ -531,15 +550,62 @@ __global__ __launch_bounds__(BS) void leaf_flags_kernel(const Lvl *__restrict__ 
 
 // CreateUpperChildren (TriangleUpperChildren.h:203-281), MarkLeafNodes
 // (:385-395), PropagateAabbToChildren (EmptySpaceSplitting.h:299-331).
+// The children layout and the next level's sizes from the level's scanned
+// counts (the host arithmetic of TriangleMapUpperCreator.cu:140-180).  Every
+// block of children_kernel works it out; block 0 stores it for the scatter
+// and the advance.  It reads only fields it does not write.
+struct ChildPlan {
+    int status;
+    int L_all, R_all, LL_all, LR_all, leaf_nodes, leafTri, nextR, created, child_start, next_active;
+};
+
+__device__ __forceinline__ ChildPlan plan_children(const Lvl *L, const int *__restrict__ segLoff,
+                                                   const int *__restrict__ segRoff,
+                                                   const int *__restrict__ segLLoff,
+                                                   const int *__restrict__ segLRoff, const int *__restrict__ lna) {
+    ChildPlan P{};
+    const int A = L->A, S = L->S;
+    P.L_all = segLoff[S]; P.R_all = segRoff[S]; P.LL_all = segLLoff[S]; P.LR_all = segLRoff[S];
+    P.leaf_nodes = lna[2 * A];
+    const long long newTri = (long long)P.L_all + P.R_all;
+    if (newTri < L->R) { P.status = PM_ERR_INTERNAL; return P; }
+    if (newTri > INT_MAX) { P.status = PM_ERR_ARG; return P; }
+    P.leafTri = P.LL_all + P.LR_all;
+    P.nextR = (int)newTri - P.leafTri;
+    P.created = 2 * A - P.leaf_nodes;
+    if (P.created > 0 && P.nextR > L->max_refs) { P.status = PM_ERR_DEPTH; return P; }
+    if (P.nextR > L->cap_refs || (long long)L->final_refs + P.leafTri > L->cap_leaf_refs ||
+        (long long)L->nleaf + P.leaf_nodes > L->cap_leaf_ids || P.created > L->cap_active) {
+        P.status = PM_ERR_CAPACITY;
+        return P;
+    }
+    P.child_start = L->nodes_size;
+    P.next_active = L->nodes_size + P.leaf_nodes;
+    return P;
+}
+
 __global__ __launch_bounds__(BS) void children_kernel(
-    NodePtrs N, const Lvl *__restrict__ L, int propagate, const int *__restrict__ segbase,
+    NodePtrs N, Lvl *__restrict__ L, int propagate, const int *__restrict__ segbase,
     const int *__restrict__ segLoff, const int *__restrict__ segRoff,
     const int *__restrict__ segLLoff, const int *__restrict__ segLRoff,
     const int *__restrict__ slot_leaf, const int *__restrict__ lna, int *__restrict__ leaf_ids,
     int *__restrict__ keys_next) {
-    const int A = L->A, active_index = L->active_index, child_start = L->child_start;
-    const int next_active = L->next_active, L_all = L->L_all, LL_all = L->LL_all;
-    const int leaf_nodes = L->leaf_nodes, ulp = L->final_refs, nleaf = L->nleaf;
+    if (L->status) return;
+    const ChildPlan P = plan_children(L, segLoff, segRoff, segLLoff, segLRoff, lna);
+    const int A = L->A, active_index = L->active_index, ulp = L->final_refs, nleaf = L->nleaf;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (P.status) {
+            lvl_fail(L, P.status);
+        } else {
+            L->L_all = P.L_all; L->R_all = P.R_all; L->LL_all = P.LL_all; L->LR_all = P.LR_all;
+            L->leaf_nodes = P.leaf_nodes; L->leafTri = P.leafTri; L->nextR = P.nextR; L->created = P.created;
+            L->child_start = P.child_start;
+            L->next_active = P.next_active;
+        }
+    }
+    if (P.status) return;
+    const int child_start = P.child_start, next_active = P.next_active, L_all = P.L_all, LL_all = P.LL_all;
+    const int leaf_nodes = P.leaf_nodes;
     GRID_STRIDE(i, A) {
         const int par = active_index + i;
         const int b = segbase[i], e = segbase[i + 1];
@@ -695,42 +761,16 @@ __device__ void lvl_fail(Lvl *L, int code) {
     L->A = 0; L->S = 0; L->E = 0; L->An = 0;
 }
 
-// after the node pass: empty-space nodes and segments of the level
-__global__ void plan_nodes_kernel(Lvl *L, const int *__restrict__ eaddr, const int *__restrict__ segbase) {
-    if (threadIdx.x | blockIdx.x) return;
+// after the node pass: empty-space nodes and segments of the level (run by
+// the node scan's thread that holds both totals; the scan's other blocks read
+// only A, which this changes only on failure)
+__device__ void plan_nodes(Lvl *L, int E, int S) {
     const int A = L->A;
-    const int E = eaddr[A], S = segbase[A];
     if ((long long)L->nodes_size + E + 2ll * A > L->cap_nodes) { lvl_fail(L, PM_ERR_CAPACITY); return; }
     if (S > L->cap_segs) { lvl_fail(L, PM_ERR_CAPACITY); return; }
     L->E = E;
     L->S = S;
     L->nodes_size += E;
-}
-
-// after the counts: children layout and the next level's sizes
-__global__ void plan_children_kernel(Lvl *L, const int *__restrict__ segLoff, const int *__restrict__ segRoff,
-                                     const int *__restrict__ segLLoff, const int *__restrict__ segLRoff,
-                                     const int *__restrict__ lna) {
-    if (threadIdx.x | blockIdx.x) return;
-    const int A = L->A, S = L->S;
-    const int L_all = segLoff[S], R_all = segRoff[S], LL_all = segLLoff[S], LR_all = segLRoff[S];
-    const int leaf_nodes = lna[2 * A];
-    const long long newTri = (long long)L_all + R_all;
-    if (newTri < L->R) { lvl_fail(L, PM_ERR_INTERNAL); return; }
-    if (newTri > INT_MAX) { lvl_fail(L, PM_ERR_ARG); return; }
-    const int leafTri = LL_all + LR_all;
-    const int nextR = (int)newTri - leafTri;
-    const int created = 2 * A - leaf_nodes;
-    if (created > 0 && nextR > L->max_refs) { lvl_fail(L, PM_ERR_DEPTH); return; }
-    if (nextR > L->cap_refs || (long long)L->final_refs + leafTri > L->cap_leaf_refs ||
-        (long long)L->nleaf + leaf_nodes > L->cap_leaf_ids || created > L->cap_active) {
-        lvl_fail(L, PM_ERR_CAPACITY);
-        return;
-    }
-    L->L_all = L_all; L->R_all = R_all; L->LL_all = LL_all; L->LR_all = LR_all;
-    L->leaf_nodes = leaf_nodes; L->leafTri = leafTri; L->nextR = nextR; L->created = created;
-    L->child_start = L->nodes_size;
-    L->next_active = L->nodes_size + leaf_nodes;
 }
 
 // after children and scatter: advance to the next level
@@ -1278,6 +1318,7 @@ static void scan_level(ScanIO<K> io, BuildScratch &sc, int depth, hipStream_t s)
         sums.n_dev[q] = nullptr;
         sums.n_mult[q] = 1;
     }
+    sums.nodes_plan = nullptr;
     scan_level<K>(sums, sc, depth + 1, s);
     scan_add_kernel<K><<<nb, SCAN_T, 0, s>>>(io, tmp.ptr, stride);
     PM_LAUNCH_CHECK();
@@ -1287,6 +1328,7 @@ template <int K>
 static void scan_set(ScanIO<K> &io, int q, const int *in, int *out, int n, const int *n_dev, int n_mult) {
     io.in[q] = in; io.out[q] = out;
     io.n_host[q] = n; io.n_dev[q] = n_dev; io.n_mult[q] = n_mult;
+    io.nodes_plan = nullptr;
 }
 
 void scan_exclusive(const int *in, int *out, int n, BuildScratch &sc, hipStream_t s, const int *n_dev,
@@ -1301,6 +1343,16 @@ void scan_exclusive2(const int *in0, int *out0, const int *in1, int *out1, int n
     ScanIO<2> io;
     scan_set(io, 0, in0, out0, n, n_dev, n_mult);
     scan_set(io, 1, in1, out1, n, n_dev, n_mult);
+    scan_level<2>(io, sc, 0, s);
+}
+
+// the node pass's two scans over L->A (n its bound), then plan_nodes
+static void scan_nodes(const int *req, int *eaddr, const int *nseg, int *segbase, int n, Lvl *L,
+                       BuildScratch &sc, hipStream_t s) {
+    ScanIO<2> io;
+    scan_set(io, 0, req, eaddr, n, &L->A, 1);
+    scan_set(io, 1, nseg, segbase, n, &L->A, 1);
+    io.nodes_plan = L;
     scan_level<2>(io, sc, 0, s);
 }
 
@@ -1434,10 +1486,8 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
                                                               prm.empty_space_threshold, planes, sc.node_req.ptr,
                                                               sc.node_nseg.ptr);
         PM_LAUNCH_CHECK();
-        scan_exclusive2(sc.node_req.ptr, sc.node_eaddr.ptr, sc.node_nseg.ptr, sc.node_segbase.ptr,
-                        graph ? nA : (int)A_bound, sc, s, &L->A);
-        plan_nodes_kernel<<<1, 64, 0, s>>>(L, sc.node_eaddr.ptr, sc.node_segbase.ptr);
-        PM_LAUNCH_CHECK();
+        scan_nodes(sc.node_req.ptr, sc.node_eaddr.ptr, sc.node_nseg.ptr, sc.node_segbase.ptr,
+                   graph ? nA : (int)A_bound, L, sc, s);
         const int nxt = cur ^ 1;
         if (!graph) {
             read_state(sc, h, s);
@@ -1477,9 +1527,6 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
         PM_LAUNCH_CHECK();
         scan_exclusive3(sc.segLL.ptr, sc.segLLoff.ptr, sc.segLR.ptr, sc.segLRoff.ptr, graph ? nS : h.S, dS,
                         sc.slot_leaf.ptr, sc.slot_lna.ptr, graph ? 2 * nA : 2 * h.A, dA, 2, sc, s);
-        plan_children_kernel<<<1, 64, 0, s>>>(L, sc.segLoff.ptr, sc.segRoff.ptr, sc.segLLoff.ptr, sc.segLRoff.ptr,
-                                              sc.slot_lna.ptr);
-        PM_LAUNCH_CHECK();
         NP = node_ptrs(ctx);
         children_kernel<<<g_nodes(h.A), BS, 0, s>>>(NP, L, prm.empty_space_splitting, sc.node_segbase.ptr,
                                                       sc.segLoff.ptr, sc.segRoff.ptr, sc.segLLoff.ptr,
