@@ -71,6 +71,7 @@ struct ScanIO {
 
 __device__ void plan_nodes(Lvl *L, int E, int S);
 __device__ void lvl_fail(Lvl *L, int code);
+__device__ void advance_level(Lvl *L);
 
 template <int K>
 __device__ __forceinline__ int scan_len(const ScanIO<K> &io, int q) {
@@ -356,7 +357,13 @@ __global__ __launch_bounds__(BS) void node_prepare_kernel(NodePtrs N, const Lvl 
                                                           int *__restrict__ planes_out,
                                                           int *__restrict__ req,
                                                           int *__restrict__ nseg) {
-    const int A = L->A, active_index = L->active_index;
+    // the previous level's advance is applied by this level's node scan; until
+    // then the level's nodes follow from the stored children plan
+    int A = L->A, active_index = L->active_index;
+    if (L->advance && !L->status && A > 0) {
+        active_index = L->nodes_size + 2 * A - L->created;
+        A = L->created;
+    }
     GRID_STRIDE(i, A) {
         const int node = active_index + i;
         const f3 rMin = {unkey(keys[6 * i + 0]), unkey(keys[6 * i + 1]), unkey(keys[6 * i + 2])};
@@ -601,6 +608,8 @@ __global__ __launch_bounds__(BS) void children_kernel(
             L->leaf_nodes = P.leaf_nodes; L->leafTri = P.leafTri; L->nextR = P.nextR; L->created = P.created;
             L->child_start = P.child_start;
             L->next_active = P.next_active;
+            L->A_scan = P.created;
+            L->advance = 1;
         }
     }
     if (P.status) return;
@@ -758,13 +767,15 @@ __global__ __launch_bounds__(WPB * 64) void scatter_kernel(
 // (TriangleMapUpperCreator.cu:140-180), on the device
 __device__ void lvl_fail(Lvl *L, int code) {
     if (!L->status) L->status = code;
-    L->A = 0; L->S = 0; L->E = 0; L->An = 0;
+    L->A = 0; L->S = 0; L->E = 0; L->An = 0; L->A_scan = 0; L->advance = 0;
 }
 
-// after the node pass: empty-space nodes and segments of the level (run by
-// the node scan's thread that holds both totals; the scan's other blocks read
-// only A, which this changes only on failure)
+// after the node pass: the previous level's advance, then the empty-space
+// nodes and segments of the level (run by the node scan's thread that holds
+// both totals; the scan's other blocks read only A_scan, which this changes
+// only on failure)
 __device__ void plan_nodes(Lvl *L, int E, int S) {
+    advance_level(L);
     const int A = L->A;
     if ((long long)L->nodes_size + E + 2ll * A > L->cap_nodes) { lvl_fail(L, PM_ERR_CAPACITY); return; }
     if (S > L->cap_segs) { lvl_fail(L, PM_ERR_CAPACITY); return; }
@@ -773,9 +784,11 @@ __device__ void plan_nodes(Lvl *L, int E, int S) {
     L->nodes_size += E;
 }
 
-// after children and scatter: advance to the next level
-__global__ void plan_advance_kernel(Lvl *L) {
-    if (threadIdx.x | blockIdx.x) return;
+// after children and scatter: advance to the next level (applied by the next
+// node scan, or by plan_finish_kernel after the last captured level)
+__device__ void advance_level(Lvl *L) {
+    if (!L->advance) return;
+    L->advance = 0;
     if (L->status || L->A == 0) return;
     L->final_refs += L->leafTri;
     L->nleaf += L->leaf_nodes;
@@ -1346,12 +1359,12 @@ void scan_exclusive2(const int *in0, int *out0, const int *in1, int *out1, int n
     scan_level<2>(io, sc, 0, s);
 }
 
-// the node pass's two scans over L->A (n its bound), then plan_nodes
+// the node pass's two scans over L->A_scan (n its bound), then plan_nodes
 static void scan_nodes(const int *req, int *eaddr, const int *nseg, int *segbase, int n, Lvl *L,
                        BuildScratch &sc, hipStream_t s) {
     ScanIO<2> io;
-    scan_set(io, 0, req, eaddr, n, &L->A, 1);
-    scan_set(io, 1, nseg, segbase, n, &L->A, 1);
+    scan_set(io, 0, req, eaddr, n, &L->A_scan, 1);
+    scan_set(io, 1, nseg, segbase, n, &L->A_scan, 1);
     io.nodes_plan = L;
     scan_level<2>(io, sc, 0, s);
 }
@@ -1376,6 +1389,7 @@ __global__ void init_lvl_kernel(Lvl *L, Lvl v) {
 // a captured build that needed more levels than it recorded
 __global__ void plan_finish_kernel(Lvl *L, int lower) {
     if (threadIdx.x | blockIdx.x) return;
+    if (!lower) advance_level(L);
     if (!L->status && (lower ? L->An > 0 : L->A > 0)) L->status = PM_ERR_CAPACITY;
 }
 
@@ -1420,7 +1434,7 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
     const long long BIG = LLONG_MAX / 4;
 
     Lvl h{};
-    h.A = M; h.active_index = 0; h.nodes_size = M; h.R = N;
+    h.A = M; h.active_index = 0; h.nodes_size = M; h.R = N; h.A_scan = M; h.advance = 0;
     h.max_levels = prm.max_upper_levels > 0 ? prm.max_upper_levels : 128;
     h.max_refs = (long long)(prm.max_ref_factor > 0 ? prm.max_ref_factor : 16) * N + 65536;
     h.cap_nodes = graph ? caps->nodes : BIG;
@@ -1539,8 +1553,6 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
                        (const int *)sc.slot_leaf.ptr, (const float4 *)sc.ref_min[cur].ptr,
                        (const float4 *)sc.ref_max[cur].ptr, sc.ref_min[nxt].ptr, sc.ref_max[nxt].ptr,
                        sc.prim_min.ptr, sc.prim_max.ptr, sc.keys[nxt].ptr, dg);
-        PM_LAUNCH_CHECK();
-        plan_advance_kernel<<<1, 64, 0, s>>>(L);
         PM_LAUNCH_CHECK();
         A_bound = 2ll * h.A;
         cur = nxt;

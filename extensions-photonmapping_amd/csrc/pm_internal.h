@@ -116,6 +116,8 @@ struct Lvl {
     int final_refs, nleaf, level;
     int An, list_base, use_list, splits, lower_base, lower_levels;   // lower levels
     int cand_n;                                        // lower level: nodes listed for the full SAH search
+    int advance;                                       // the children plan is stored, the level not yet advanced
+    int A_scan;                                        // active nodes of the coming node pass
     int status;                                        // 0, or the PM_ERR_* that stopped the build
     int max_levels;
     long long max_refs;
