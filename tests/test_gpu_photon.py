@@ -53,3 +53,38 @@ def test_bounce_matches_numpy(pm, scenes):
     assert np.array_equal(src3[:n3].cpu().numpy(), src[:n3].cpu().numpy())
     assert torch.equal(o3[:n3], o2[:n3]) and torch.equal(d3[:n3], d2[:n3])
     ctx.close()
+
+
+def test_bounce_compaction_beyond_1024_scan_blocks(pm, scenes):
+    """6,000,000 rays: the survivor scan spans more than 1,024 blocks of
+    4,096 flags, so it takes the recursive path (block scans, a scan of the
+    block sums, an add).  Survivor count and order are integer work: exact."""
+    import torch
+    tris = scenes.cluster_soup(1000, seed=5)
+    ntris = tris[0].shape[0]
+    absorb = ntris - 2
+    n = 6_000_000
+    rng = np.random.default_rng(11)
+    org = np.zeros((n, 4), np.float32)
+    org[:, :3] = rng.uniform(-1, 1, (n, 3))
+    dirs = np.zeros((n, 4), np.float32)
+    d = rng.normal(size=(n, 3))
+    dirs[:, :3] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    ids = rng.integers(-1, ntris, n).astype(np.int32)
+    hits = np.zeros((n, 4), np.float32)
+    hits[:, 0] = ids.view(np.float32)
+    hits[:, 1] = 0.5
+    hits[:, 2] = 0.25
+    hits[:, 3] = 0.25
+    dev = "cuda:0"
+    P = [torch.from_numpy(t).to(dev) for t in tris]
+    ctx = pm.Context(0)
+    ctx.set_geometry(*P)
+    O, D, H = (torch.from_numpy(a).to(dev) for a in (org, dirs, hits))
+    o2, d2, src, cnt = ctx.photon_bounce(O, D, H, absorb, 77)
+    torch.cuda.synchronize()
+    keep = np.nonzero((ids >= 0) & (ids < absorb))[0]
+    assert cnt == keep.size
+    assert np.array_equal(src[:cnt].cpu().numpy(), keep)
+    assert torch.isfinite(o2[:cnt]).all() and torch.isfinite(d2[:cnt]).all()
+    ctx.close()
