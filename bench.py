@@ -467,6 +467,7 @@ def main() -> int:
             "batched": batched,
             "tree": {k: st[k] for k in ("num_nodes", "leaves", "empty_leaves", "max_depth",
                                         "leaf_refs", "upper_levels", "lower_levels")},
+            "build_roofline": build_roofline(st, ntris, build_ms),
         },
         "roofline": {
             "bound": "hbm",
@@ -484,6 +485,21 @@ def main() -> int:
     print(json.dumps(line), flush=True)
     ctx.close()
     return 0
+
+
+def build_roofline(st: dict, ntris: int, build_ms: float) -> dict:
+    """the build's algorithmic bytes (reference record sizes: 80 B per
+    triangle for the box pass, 136 B per ref read and 32 B per ref written at
+    every upper level, 136 B per final leaf ref for areas and planes, 52 B per
+    prim of every SAH lower node) over the measured build time"""
+    b = (80 * ntris + 136 * st["upper_ref_reads"] + 32 * st["upper_ref_writes"]
+         + 136 * st["num_prim_indices"] + 52 * st["lower_prims"])
+    achieved = b / (build_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "algorithmic_bytes": int(b), "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "counts": {"tris": ntris, "upper_ref_reads": st["upper_ref_reads"],
+                       "upper_ref_writes": st["upper_ref_writes"], "leaf_refs": st["num_prim_indices"],
+                       "lower_prims": st["lower_prims"]}}
 
 
 if __name__ == "__main__":

@@ -92,7 +92,9 @@ class Stats(C.Structure):
                 ("leaves", C.c_int), ("empty_leaves", C.c_int), ("max_depth", C.c_int),
                 ("mean_leaf_depth", C.c_double), ("leaf_refs", C.c_longlong),
                 ("max_leaf_size", C.c_int), ("upper_levels", C.c_int), ("lower_levels", C.c_int),
-                ("num_prim_indices", C.c_int), ("num_upper_leaves", C.c_int)]
+                ("num_prim_indices", C.c_int), ("num_upper_leaves", C.c_int),
+                ("upper_ref_reads", C.c_longlong), ("upper_ref_writes", C.c_longlong),
+                ("lower_prims", C.c_longlong)]
 
 
 _lib = None

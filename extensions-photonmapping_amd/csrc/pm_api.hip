@@ -306,6 +306,9 @@ int pm_tree_stats(pm_ctx *ctx, pm_stats *out, void *stream) {
         st.lower_levels = ctx->lower_levels;
         st.num_prim_indices = ctx->num_prim_indices;
         st.num_upper_leaves = ctx->num_leaf_ids;
+        st.upper_ref_reads = (long long)ctx->ref_reads;
+        st.upper_ref_writes = (long long)ctx->ref_writes;
+        st.lower_prims = (long long)ctx->lower_prims;
         std::vector<std::pair<int, int>> stk;
         for (int r = ctx->num_roots - 1; r >= 0; --r) stk.push_back({r, 0});
         double depth_sum = 0.0;

@@ -790,6 +790,8 @@ __device__ void advance_level(Lvl *L) {
     if (!L->advance) return;
     L->advance = 0;
     if (L->status || L->A == 0) return;
+    L->ref_reads += (unsigned long long)L->R;
+    L->ref_writes += (unsigned long long)L->L_all + (unsigned long long)L->R_all;
     L->final_refs += L->leafTri;
     L->nleaf += L->leaf_nodes;
     L->nodes_size += 2 * L->A;
@@ -1070,6 +1072,12 @@ __global__ __launch_bounds__(WPB * 64) void sah_screen_kernel(
         const bool forced = leaf_max > 0 && n > leaf_max;
         const float leaf_cost = ((float)n - Ct) * nodeA;
         unsigned long long m = __ballot(valid && (forced || leaf_cost > 0.0f));
+        {   // work count: the batch's prims
+            int t = n;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+            if (lane == 0) atomicAdd(&L->lower_prims, (unsigned long long)t);
+        }
         unsigned long long search = 0ull;
         int k = 0;
         float a = 0.0f;
@@ -1647,6 +1655,9 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
     if (!graph) {
         ctx->upper_levels = h.level;
         ctx->lower_levels = h.lower_levels;
+        ctx->ref_reads = h.ref_reads;
+        ctx->ref_writes = h.ref_writes;
+        ctx->lower_prims = h.lower_prims;
         ctx->num_nodes = h.nodes_size;
         ctx->num_prim_indices = h.final_refs;
         ctx->num_leaf_ids = h.nleaf;
@@ -1880,6 +1891,9 @@ int sync_graph_state(pm_ctx *ctx, hipStream_t s) {
     ctx->num_leaf_ids = h.nleaf;
     ctx->upper_levels = h.level;
     ctx->lower_levels = h.lower_levels;
+    ctx->ref_reads = h.ref_reads;
+    ctx->ref_writes = h.ref_writes;
+    ctx->lower_prims = h.lower_prims;
     ctx->has_tree = h.status == 0;
     return h.status;
 }

@@ -118,6 +118,10 @@ struct Lvl {
     int cand_n;                                        // lower level: nodes listed for the full SAH search
     int advance;                                       // the children plan is stored, the level not yet advanced
     int A_scan;                                        // active nodes of the coming node pass
+    // work counts of the build (the algorithmic-bytes formula's inputs)
+    unsigned long long ref_reads;                      // refs of every upper level, summed
+    unsigned long long ref_writes;                     // refs written by every upper level (children + leaves)
+    unsigned long long lower_prims;                    // prims of every node of every SAH lower level
     int status;                                        // 0, or the PM_ERR_* that stopped the build
     int max_levels;
     long long max_refs;
@@ -175,6 +179,7 @@ struct pm_ctx {
     bool has_boxes = false;
     int num_nodes = 0, num_prim_indices = 0, num_leaf_ids = 0;
     int upper_levels = 0, lower_levels = 0;
+    unsigned long long ref_reads = 0, ref_writes = 0, lower_prims = 0;   // work counts of the last build
     pmkd::NodeArrays nodes;
     pmkd::DevBuf<int> prim_indices;
     pmkd::DevBuf<int> leaf_ids;

@@ -177,6 +177,10 @@ typedef struct {
     int max_leaf_size;
     int upper_levels, lower_levels;
     int num_prim_indices, num_upper_leaves;
+    /* work counts of the last build (no reference counterpart): refs of every
+     * upper level summed, refs those levels wrote (children + leaves), and
+     * prims of every node the SAH lower levels examined */
+    long long upper_ref_reads, upper_ref_writes, lower_prims;
 } pm_stats;
 
 /* Defaults of the TriangleMap constructor (TriangleMap.cu:27-49) and the

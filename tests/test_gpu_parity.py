@@ -275,3 +275,31 @@ def test_c2_heightfield_2m_tree_bit_exact(pm, orc, scenes, torch, ctx):
     O, D = dev(torch, org), dev(torch, dirs)
     h = pm.hits_to_numpy(ctx.trace(pm.SHORT_STACK, O, D, pm.trace_params(stack_size=8)))
     assert_same_hits(h, orc.trace(ref_tree, tris, org, dirs, orc.SHORT_STACK, stack_size=8), "C2")
+
+
+@pytest.mark.parametrize("name", ["reference", "ct1_lower64"])
+def test_build_work_counts(pm, orc, scenes, torch, ctx, name):
+    """the build's work counts (inputs of bench.py's build roofline) against
+    the oracle: refs read = the oracle's per-level refs summed; refs written =
+    the later levels' refs + the final leaf refs; SAH lower prims = the bitmap
+    sizes of every node below an upper leaf (each is examined once)"""
+    spec = PRESETS.get(name) or SAH_SEARCH[name]
+    tris = scenes.random_soup(3000, seed=3000 + 3000, size=0.1)
+    params = make_params(pm, spec)
+    P, got = gpu_build(pm, ctx, torch, tris, params)
+    ref_tree = orc.build(tris, orc_params(orc, params))
+    rs = ref_tree.stats()
+    st = ctx.stats()
+    lvl = rs["upper_level_refs"]
+    assert st["upper_ref_reads"] == sum(lvl)
+    assert st["upper_ref_writes"] == sum(lvl[1:]) + st["num_prim_indices"]
+    bm = got["primitive_bitmap"].astype(np.uint64)
+    ch = got["children"].reshape(-1, 2)
+    info = got["info"]
+    total, stack = 0, list(got["leaf_ids"])
+    while stack:
+        nd = stack.pop()
+        total += bin(int(bm[nd])).count("1")
+        if info[nd] & 3:
+            stack.extend(ch[nd])
+    assert st["lower_prims"] == total and total > 0
