@@ -303,3 +303,36 @@ def test_build_work_counts(pm, orc, scenes, torch, ctx, name):
         if info[nd] & 3:
             stack.extend(ch[nd])
     assert st["lower_prims"] == total and total > 0
+
+
+def test_random_configs_bit_exact(pm, orc, scenes, torch, ctx):
+    """40 seeded random cases: soup size 1..120k (log-uniform), triangle size,
+    every preset above and every chunk size; tree arrays and short-stack hits
+    must equal the oracle's"""
+    specs = dict(PRESETS)
+    specs.update(SAH_SEARCH)
+    names = sorted(specs)
+    rng = np.random.default_rng(2024)
+    checked = 0
+    for case in range(40):
+        n = int(np.exp(rng.uniform(0, np.log(120000))))
+        size = min(float(np.exp(rng.uniform(np.log(0.005), np.log(0.3)))), 3.0 / n ** (1 / 3))
+        name = names[rng.integers(len(names))]
+        chunk = int(rng.choice([0, 128, 256, 512, 1024, 2048, 4096]))
+        seed = int(rng.integers(1 << 30))
+        spec = dict(specs[name])
+        params = pm.build_params(spec.pop("preset"), chunk_size=chunk, **spec)
+        tris = scenes.random_soup(n, seed=seed, size=size)
+        label = f"case {case}: n={n} size={size:.4f} {name} chunk={chunk} seed={seed}"
+        try:
+            ref_tree = orc.build(tris, orc_params(orc, params))
+        except RuntimeError:       # the oracle's depth/ref limits; the GPU build is checked elsewhere
+            continue
+        P, got = gpu_build(pm, ctx, torch, tris, params)
+        assert_same_tree(got, ref_tree.arrays(), label)
+        org, dirs = scenes.random_rays(2048, seed=seed + 1)
+        h = pm.hits_to_numpy(ctx.trace(pm.SHORT_STACK, dev(torch, org), dev(torch, dirs)))
+        r = orc.trace(ref_tree, tris, org, dirs, orc.SHORT_STACK, leaf_skipping=bool(params.propagate_aabbs))
+        assert_same_hits(h, r, label)
+        checked += 1
+    assert checked >= 35
