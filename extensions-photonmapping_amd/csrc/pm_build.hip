@@ -484,7 +484,7 @@ __device__ __forceinline__ void divide_sides(const DivideGeom &g, float id_bits,
 }
 
 // SetSplitSide (TriangleUpperChildren.h:10-62) counted per segment
-template <int SEGT>
+template <int SEGT, bool DIV>
 __global__ __launch_bounds__(WPB * 64) void classify_kernel(
     const NodePtrs N, const Lvl *__restrict__ L, const int *__restrict__ seg_node,
     const int *__restrict__ seg_start, const int *__restrict__ seg_count,
@@ -494,25 +494,41 @@ __global__ __launch_bounds__(WPB * 64) void classify_kernel(
     const int lane = threadIdx.x & 63;
     for (int s = wave_index(); s < S; s += wave_count()) {
         const int node = active_index + seg_node[s];
-        const int ax = N.info[node];
-        const float sp = N.split[node];
+        // one chunk per wave: its node's axis and plane are wave-uniform
+        const int ax = __builtin_amdgcn_readfirstlane(N.info[node]);
+        const float sp = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(N.split[node])));
         const int start = seg_start[s], count = seg_count[s];
-        f3 nmn = {0.0f, 0.0f, 0.0f}, nmx = nmn;
-        if (dg.p0) { nmn = mk3(N.bmin[node]); nmx = mk3(N.bmax[node]); }
         int cl = 0, cr = 0;
+        if (!DIV) {
+            // the split axis' coordinates only, loaded without branches (lanes
+            // past the chunk re-read its first ref) so that the chunk's loads
+            // all issue before the first compare
+            const float *lo = reinterpret_cast<const float *>(rmin + start) + (ax - AX_X);
+            const float *hi = reinterpret_cast<const float *>(rmax + start) + (ax - AX_X);
 #pragma unroll 16
-        for (int it = 0; it < SEGT / 64; ++it) {
-            const int j = it * 64 + lane;
-            const bool valid = j < count;
-            bool l = false, r = false;
-            if (valid) {
-                const float4 mn = rmin[start + j];
-                l = comp4(mn, ax) <= sp;
-                r = sp < comp4(rmax[start + j], ax);
-                if (dg.p0 && l && r) divide_sides(dg, mn.w, ax, sp, nmn, nmx, l, r);
+            for (int it = 0; it < SEGT / 64; ++it) {
+                const int j = it * 64 + lane;
+                const int jj = j < count ? j : 0;
+                const float a = lo[4 * jj], b = hi[4 * jj];
+                cl += __popcll(__ballot(j < count && a <= sp));
+                cr += __popcll(__ballot(j < count && sp < b));
             }
-            cl += __popcll(__ballot(l));
-            cr += __popcll(__ballot(r));
+        } else {
+            const f3 nmn = mk3(N.bmin[node]), nmx = mk3(N.bmax[node]);
+#pragma unroll 16
+            for (int it = 0; it < SEGT / 64; ++it) {
+                const int j = it * 64 + lane;
+                const bool valid = j < count;
+                bool l = false, r = false;
+                if (valid) {
+                    const float4 mn = rmin[start + j];
+                    l = comp4(mn, ax) <= sp;
+                    r = sp < comp4(rmax[start + j], ax);
+                    if (l && r) divide_sides(dg, mn.w, ax, sp, nmn, nmx, l, r);
+                }
+                cl += __popcll(__ballot(l));
+                cr += __popcll(__ballot(r));
+            }
         }
         if (lane == 0) { segL[s] = cl; segR[s] = cr; }
     }
@@ -662,7 +678,7 @@ __global__ __launch_bounds__(BS) void children_kernel(
 
 // SplitTriangles (TriangleUpperChildren.h:283-383): stable partition of the
 // refs with clipping, plus the next level's box reduction.
-template <int SEGT>
+template <int SEGT, bool DIV>
 __device__ __forceinline__ void scatter_segment(
     const NodePtrs &N, int s, int A, int active_index, int next_active, int L_all, int LL_all, int ulp,
     const int *__restrict__ seg_node, const int *__restrict__ seg_start,
@@ -675,10 +691,11 @@ __device__ __forceinline__ void scatter_segment(
     const int lane = threadIdx.x & 63;
     const int i = seg_node[s];
     const int node = active_index + i;
-    const int ax = N.info[node];
-    const float sp = N.split[node];
+    // one chunk per wave: its node's axis and plane are wave-uniform
+    const int ax = __builtin_amdgcn_readfirstlane(N.info[node]);
+    const float sp = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(N.split[node])));
     f3 nmn = {0.0f, 0.0f, 0.0f}, nmx = nmn;
-    if (dg.p0) { nmn = mk3(N.bmin[node]); nmx = mk3(N.bmax[node]); }
+    if (DIV) { nmn = mk3(N.bmin[node]); nmx = mk3(N.bmax[node]); }
     const int2 ch = N.children[node];
     const bool leafL = slot_leaf[i] != 0, leafR = slot_leaf[A + i] != 0;
     const int start = seg_start[s], count = seg_count[s];
@@ -692,46 +709,61 @@ __device__ __forceinline__ void scatter_segment(
     int rl = 0, rr = 0;
     int l0 = fkey(F_INF), l1 = l0, l2 = l0, l3 = fkey(-F_INF), l4 = l3, l5 = l3;
     int r0 = l0, r1 = l0, r2 = l0, r3 = l3, r4 = l3, r5 = l3;
-#pragma unroll 16
-    for (int it = 0; it < SEGT / 64; ++it) {
-        const int j = it * 64 + lane;
-        const bool valid = j < count;
-        float4 mn = make_float4(0.0f, 0.0f, 0.0f, 0.0f), mx = mn;
-        bool l = false, r = false;
-        if (valid) {
-            mn = rmin[start + j];
-            mx = rmax[start + j];
-            l = comp4(mn, ax) <= sp;
-            r = sp < comp4(mx, ax);
-            if (dg.p0 && l && r) divide_sides(dg, mn.w, ax, sp, nmn, nmx, l, r);
+    // groups of G iterations: the group's refs are loaded first (lanes past
+    // the chunk re-read its first ref), then classified and written
+#ifndef PM_SCATTER_GROUP
+#define PM_SCATTER_GROUP 4
+#endif
+    constexpr int IT = SEGT / 64, G = IT < PM_SCATTER_GROUP ? IT : PM_SCATTER_GROUP;
+#pragma unroll 4
+    for (int g0 = 0; g0 < IT; g0 += G) {
+        float4 gmn[G], gmx[G];
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            const int jq = (g0 + q) * 64 + lane;
+            const int k = start + (jq < count ? jq : 0);
+            gmn[q] = rmin[k];
+            gmx[q] = rmax[k];
         }
-        const unsigned long long blm = __ballot(l), brm = __ballot(r);
-        if (l) {
-            const float4 cmx = make_float4(ax == AX_X ? omin(sp, mx.x) : mx.x,
-                                           ax == AX_Y ? omin(sp, mx.y) : mx.y,
-                                           ax == AX_Z ? omin(sp, mx.z) : mx.z, mx.w);
-            const long long p = bL + rl + __popcll(blm & lt);
-            dLmin[p] = mn;
-            dLmax[p] = cmx;
-            if (!leafL) {
-                l0 = min(l0, fkey(mn.x)); l1 = min(l1, fkey(mn.y)); l2 = min(l2, fkey(mn.z));
-                l3 = max(l3, fkey(cmx.x)); l4 = max(l4, fkey(cmx.y)); l5 = max(l5, fkey(cmx.z));
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            const int j = (g0 + q) * 64 + lane;
+            const bool valid = j < count;
+            const float4 mn = gmn[q], mx = gmx[q];
+            bool l = false, r = false;
+            if (valid) {
+                l = comp4(mn, ax) <= sp;
+                r = sp < comp4(mx, ax);
+                if (DIV && l && r) divide_sides(dg, mn.w, ax, sp, nmn, nmx, l, r);
             }
-        }
-        if (r) {
-            const float4 cmn = make_float4(ax == AX_X ? omax(sp, mn.x) : mn.x,
-                                           ax == AX_Y ? omax(sp, mn.y) : mn.y,
-                                           ax == AX_Z ? omax(sp, mn.z) : mn.z, mn.w);
-            const long long p = bR + rr + __popcll(brm & lt);
-            dRmin[p] = cmn;
-            dRmax[p] = mx;
-            if (!leafR) {
-                r0 = min(r0, fkey(cmn.x)); r1 = min(r1, fkey(cmn.y)); r2 = min(r2, fkey(cmn.z));
-                r3 = max(r3, fkey(mx.x)); r4 = max(r4, fkey(mx.y)); r5 = max(r5, fkey(mx.z));
+            const unsigned long long blm = __ballot(l), brm = __ballot(r);
+            if (l) {
+                const float4 cmx = make_float4(ax == AX_X ? omin(sp, mx.x) : mx.x,
+                                               ax == AX_Y ? omin(sp, mx.y) : mx.y,
+                                               ax == AX_Z ? omin(sp, mx.z) : mx.z, mx.w);
+                const long long p = bL + rl + __popcll(blm & lt);
+                dLmin[p] = mn;
+                dLmax[p] = cmx;
+                if (!leafL) {
+                    l0 = min(l0, fkey(mn.x)); l1 = min(l1, fkey(mn.y)); l2 = min(l2, fkey(mn.z));
+                    l3 = max(l3, fkey(cmx.x)); l4 = max(l4, fkey(cmx.y)); l5 = max(l5, fkey(cmx.z));
+                }
             }
+            if (r) {
+                const float4 cmn = make_float4(ax == AX_X ? omax(sp, mn.x) : mn.x,
+                                               ax == AX_Y ? omax(sp, mn.y) : mn.y,
+                                               ax == AX_Z ? omax(sp, mn.z) : mn.z, mn.w);
+                const long long p = bR + rr + __popcll(brm & lt);
+                dRmin[p] = cmn;
+                dRmax[p] = mx;
+                if (!leafR) {
+                    r0 = min(r0, fkey(cmn.x)); r1 = min(r1, fkey(cmn.y)); r2 = min(r2, fkey(cmn.z));
+                    r3 = max(r3, fkey(mx.x)); r4 = max(r4, fkey(mx.y)); r5 = max(r5, fkey(mx.z));
+                }
+            }
+            rl += __popcll(blm);
+            rr += __popcll(brm);
         }
-        rl += __popcll(blm);
-        rr += __popcll(brm);
     }
     if (!leafL && rl > 0) {
         l0 = wave_min(l0); l1 = wave_min(l1); l2 = wave_min(l2);
@@ -745,7 +777,7 @@ __device__ __forceinline__ void scatter_segment(
     }
 }
 
-template <int SEGT>
+template <int SEGT, bool DIV>
 __global__ __launch_bounds__(WPB * 64) void scatter_kernel(
     const NodePtrs N, const Lvl *__restrict__ L,
     const int *__restrict__ seg_node, const int *__restrict__ seg_start,
@@ -758,7 +790,7 @@ __global__ __launch_bounds__(WPB * 64) void scatter_kernel(
     const int S = L->S, A = L->A, active_index = L->active_index, next_active = L->next_active;
     const int L_all = L->L_all, LL_all = L->LL_all, ulp = L->final_refs;
     for (int s = wave_index(); s < S; s += wave_count())
-        scatter_segment<SEGT>(N, s, A, active_index, next_active, L_all, LL_all, ulp, seg_node, seg_start,
+        scatter_segment<SEGT, DIV>(N, s, A, active_index, next_active, L_all, LL_all, ulp, seg_node, seg_start,
                               seg_count, segLoff, segRoff, segLLoff, segLRoff, slot_leaf, rmin, rmax, nmin,
                               nmax, fmin, fmax, keys_next, dg);
 }
@@ -1272,28 +1304,43 @@ __global__ void lower_children_kernel(NodePtrs N, const int *__restrict__ leaf_i
 }
 
 // chunk size -> kernel instance (one wave per chunk, chunk/64 refs per lane)
-template <class... Args>
-void launch_classify(int seg, int blocks, hipStream_t s, Args... a) {
+// the chunk size and the split method are template parameters: with BOX
+// splits the chunk loops carry no data-dependent loads, so all of a chunk's
+// ref loads issue before the first is used
+template <bool DIV, class... Args>
+void launch_classify_t(int seg, int blocks, hipStream_t s, Args... a) {
     switch (seg) {
-    case 128: classify_kernel<128><<<blocks, WPB * 64, 0, s>>>(a...); break;
-    case 512: classify_kernel<512><<<blocks, WPB * 64, 0, s>>>(a...); break;
-    case 1024: classify_kernel<1024><<<blocks, WPB * 64, 0, s>>>(a...); break;
-    case 2048: classify_kernel<2048><<<blocks, WPB * 64, 0, s>>>(a...); break;
-    case 4096: classify_kernel<4096><<<blocks, WPB * 64, 0, s>>>(a...); break;
-    default: classify_kernel<256><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    case 128: classify_kernel<128, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    case 512: classify_kernel<512, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    case 1024: classify_kernel<1024, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    case 2048: classify_kernel<2048, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    case 4096: classify_kernel<4096, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    default: classify_kernel<256, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
     }
 }
 
 template <class... Args>
-void launch_scatter(int seg, int blocks, hipStream_t s, Args... a) {
+void launch_classify(int seg, bool div, int blocks, hipStream_t s, Args... a) {
+    if (div) launch_classify_t<true>(seg, blocks, s, a...);
+    else launch_classify_t<false>(seg, blocks, s, a...);
+}
+
+template <bool DIV, class... Args>
+void launch_scatter_t(int seg, int blocks, hipStream_t s, Args... a) {
     switch (seg) {
-    case 128: scatter_kernel<128><<<blocks, WPB * 64, 0, s>>>(a...); break;
-    case 512: scatter_kernel<512><<<blocks, WPB * 64, 0, s>>>(a...); break;
-    case 1024: scatter_kernel<1024><<<blocks, WPB * 64, 0, s>>>(a...); break;
-    case 2048: scatter_kernel<2048><<<blocks, WPB * 64, 0, s>>>(a...); break;
-    case 4096: scatter_kernel<4096><<<blocks, WPB * 64, 0, s>>>(a...); break;
-    default: scatter_kernel<256><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    case 128: scatter_kernel<128, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    case 512: scatter_kernel<512, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    case 1024: scatter_kernel<1024, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    case 2048: scatter_kernel<2048, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    case 4096: scatter_kernel<4096, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    default: scatter_kernel<256, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
     }
+}
+
+template <class... Args>
+void launch_scatter(int seg, bool div, int blocks, hipStream_t s, Args... a) {
+    if (div) launch_scatter_t<true>(seg, blocks, s, a...);
+    else launch_scatter_t<false>(seg, blocks, s, a...);
 }
 
 NodePtrs node_ptrs(pm_ctx *ctx) {
@@ -1537,7 +1584,7 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
             NP, L, planes, sc.node_eaddr.ptr, seg, sc.node_segbase.ptr, sc.seg_node.ptr, sc.seg_start.ptr,
             sc.seg_count.ptr);
         PM_LAUNCH_CHECK();
-        launch_classify(seg, g_waves(h.S), s, NP, (const Lvl *)L, (const int *)sc.seg_node.ptr,
+        launch_classify(seg, dg.p0 != nullptr, g_waves(h.S), s, NP, (const Lvl *)L, (const int *)sc.seg_node.ptr,
                         (const int *)sc.seg_start.ptr, (const int *)sc.seg_count.ptr,
                         (const float4 *)sc.ref_min[cur].ptr, (const float4 *)sc.ref_max[cur].ptr, sc.segL.ptr,
                         sc.segR.ptr, dg);
@@ -1555,7 +1602,7 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
                                                       sc.segLRoff.ptr, sc.slot_leaf.ptr, sc.slot_lna.ptr,
                                                       ctx->leaf_ids.ptr, sc.keys[nxt].ptr);
         PM_LAUNCH_CHECK();
-        launch_scatter(seg, g_waves(h.S), s, NP, (const Lvl *)L, (const int *)sc.seg_node.ptr,
+        launch_scatter(seg, dg.p0 != nullptr, g_waves(h.S), s, NP, (const Lvl *)L, (const int *)sc.seg_node.ptr,
                        (const int *)sc.seg_start.ptr, (const int *)sc.seg_count.ptr, (const int *)sc.segLoff.ptr,
                        (const int *)sc.segRoff.ptr, (const int *)sc.segLLoff.ptr, (const int *)sc.segLRoff.ptr,
                        (const int *)sc.slot_leaf.ptr, (const float4 *)sc.ref_min[cur].ptr,
