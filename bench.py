@@ -169,6 +169,7 @@ def bench_batched(pm, scenes, torch, meshes: int = 256, rays_per_mesh: int = 655
     per_mesh(0)
     b, b_build, b_trace = run_batched(bp, tp)
     hit_frac = float((H.view(torch.int32)[:, 0] >= 0).float().mean())
+    b_roof = build_roofline(ctx.stats(), n, b_build)
     # autotuned chunk size, leaf threshold and stack depth for this workload
     stage(0)
     tune = ctx.autotune(*P, O, D, mesh_offsets=off, ray_mesh=RM, base=bp, reps=2)
@@ -205,6 +206,7 @@ def bench_batched(pm, scenes, torch, meshes: int = 256, rays_per_mesh: int = 655
             "batched_frame_ms": round(b, 3), "per_mesh_frame_ms": round(p, 3),
             "speedup": round(p / b, 2),
             "batched_build_ms": round(b_build, 3),
+            "batched_build_roofline": b_roof,
             "batched_trace_ms": round(b_trace, 3),
             "batched_trace_mrays": round(O.shape[0] / (b_trace * 1e-3) / 1e6, 1),
             "hit_fraction": round(hit_frac, 4),
