@@ -808,6 +808,9 @@ __device__ void lvl_fail(Lvl *L, int code) {
 // only on failure)
 __device__ void plan_nodes(Lvl *L, int E, int S) {
     advance_level(L);
+    // a failed advance (depth) has zeroed the level: E and S, counted over
+    // the nodes the advance would have made, must not be applied
+    if (L->status) return;
     const int A = L->A;
     if ((long long)L->nodes_size + E + 2ll * A > L->cap_nodes) { lvl_fail(L, PM_ERR_CAPACITY); return; }
     if (S > L->cap_segs) { lvl_fail(L, PM_ERR_CAPACITY); return; }

@@ -147,3 +147,34 @@ def test_build_only_graph_then_eager_traces(pm, orc, scenes, torch):
     ctx.batch_graph_launch()
     with pytest.raises(pm.PmError):
         ctx.trace(pm.SHORT_STACK, O, D)
+
+
+def test_depth_error_in_a_replay(pm, orc, scenes, torch):
+    """a replay whose geometry needs more upper levels than max_upper_levels
+    stops with PM_ERR_DEPTH (the status call reports it; the captured levels
+    after it do no work) and leaves the context usable"""
+    n = 4000
+    a = scenes.random_soup(n, seed=5, size=0.01)          # 8 upper levels
+    b = [t.copy() for t in a]                               # 19: half of it shrunk 100x
+    c = np.array([0.3, 0.3, 0.3], np.float32)
+    for t in b:
+        t[: n // 2, :3] = c + (t[: n // 2, :3] - c) * 0.01
+    P = [dev(torch, t) for t in a]
+    off = np.array([0, n], np.int32)
+    ctx = pm.Context(0)
+    bp = pm.build_params("reference", max_upper_levels=8)
+    caps = {"upper_levels": 24, "lower_levels": 6, "nodes": 8192, "active": 4096, "segments": 8192,
+            "level_refs": 65536, "leaf_refs": 65536, "leaf_ids": 8192, "lower_nodes": 16384}
+    ctx.batch_graph_capture(*P, off, None, None, None, None, pm.SHORT_STACK, bp, caps=caps)
+    for t, h in zip(P, b):
+        t.copy_(torch.from_numpy(h))
+    ctx.batch_graph_launch()
+    assert ctx.batch_graph_status() == 3                  # PM_ERR_DEPTH
+    # the same context then builds the deeper geometry eagerly, as the oracle does
+    got = ctx.build(*P, pm.build_params("reference"))
+    torch.cuda.synchronize()
+    ref = orc.build(b, orc.build_params("reference"))
+    arr, ra = ctx.tree_arrays(), ref.arrays()
+    for k in ("info", "children", "primitive_bitmap", "prim_indices"):
+        assert np.array_equal(arr[k], ra[k]), k
+    ctx.close()
