@@ -178,3 +178,44 @@ def test_depth_error_in_a_replay(pm, orc, scenes, torch):
     for k in ("info", "children", "primitive_bitmap", "prim_indices"):
         assert np.array_equal(arr[k], ra[k]), k
     ctx.close()
+
+
+def test_random_configs_graph_equals_eager(pm, scenes, torch):
+    """20 seeded random single-mesh builds (size, triangle size, preset, chunk
+    size): a captured build-only graph's replay gives the eager build's tree
+    bit for bit (the captured build runs fixed grids over its capacities)"""
+    from test_gpu_parity import PRESETS, SAH_SEARCH, TREE_KEYS
+    specs = dict(PRESETS)
+    specs.update(SAH_SEARCH)
+    names = sorted(specs)
+    rng = np.random.default_rng(77)
+    checked = 0
+    for case in range(20):
+        n = int(np.exp(rng.uniform(np.log(2), np.log(60000))))
+        size = min(float(np.exp(rng.uniform(np.log(0.005), np.log(0.3)))), 3.0 / n ** (1 / 3))
+        name = names[rng.integers(len(names))]
+        chunk = int(rng.choice([0, 128, 256, 512, 1024, 2048, 4096]))
+        seed = int(rng.integers(1 << 30))
+        spec = dict(specs[name])
+        bp = pm.build_params(spec.pop("preset"), chunk_size=chunk, **spec)
+        tris = scenes.random_soup(n, seed=seed, size=size)
+        label = f"case {case}: n={n} {name} chunk={chunk} seed={seed}"
+        P = [dev(torch, t) for t in tris]
+        off = np.array([0, n], np.int32)
+        e = pm.Context(0)
+        try:
+            e.build_batched(*P, off, bp)
+        except pm.PmError:          # the reference-count guard; not a capture case
+            e.close()
+            continue
+        torch.cuda.synchronize()
+        ref = {k: np.array(v, copy=True) for k, v in e.tree_arrays().items()}
+        e.close()
+        g = pm.Context(0)
+        g.batch_graph_capture(*P, off, None, None, None, None, pm.SHORT_STACK, bp)
+        g.batch_graph_launch()
+        assert g.batch_graph_status() == 0, label
+        assert_same_tree(g.tree_arrays(), ref, label)
+        g.close()
+        checked += 1
+    assert checked >= 18
