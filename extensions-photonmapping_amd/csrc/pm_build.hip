@@ -885,7 +885,8 @@ __global__ void lower_prep_kernel(NodePtrs N, const int *__restrict__ leaf_ids, 
         const int cnt = (int)N.bitmap[leaf];
         float a = 0.0f;
         unsigned long long bmp = 0;
-        for (int j = 0; j < cnt; ++j) {
+#pragma unroll 8
+        for (int j = 0; j < cnt; ++j) {   // loads of eight issue together; the sum stays in order
             const float aj = areas[pi + j];
             bmp += aj > 0.0f ? (1ull << j) : 0ull;
             a += aj;
