@@ -73,6 +73,14 @@ __device__ void plan_nodes(Lvl *L, int E, int S);
 __device__ void lvl_fail(Lvl *L, int code);
 __device__ void advance_level(Lvl *L);
 
+// the scans move 4 consecutive elements per thread (SCAN_E), as one 16-byte
+// access where the array is 16-byte aligned (device allocations are; block-sum
+// rows inside a scratch buffer may not be)
+static_assert(SCAN_E == 4, "the vector path moves int4");
+__device__ __forceinline__ bool aligned16(const void *p) {
+    return (reinterpret_cast<unsigned long long>(p) & 15ull) == 0ull;
+}
+
 template <int K>
 __device__ __forceinline__ int scan_len(const ScanIO<K> &io, int q) {
     return io.n_dev[q] ? io.n_mult[q] * *io.n_dev[q] : io.n_host[q];
@@ -110,12 +118,18 @@ __global__ __launch_bounds__(SCAN_T) void scan_block_kernel(ScanIO<K> io, int *_
     for (int q = 0; q < K; ++q) {
         n[q] = scan_len(io, q);
         tsum[q] = 0;
+        if (base + SCAN_E <= n[q] && aligned16(io.in[q])) {   // one 16-byte load per thread
+            const int4 t = *reinterpret_cast<const int4 *>(io.in[q] + base);
+            v[q][0] = t.x; v[q][1] = t.y; v[q][2] = t.z; v[q][3] = t.w;
+        } else {
 #pragma unroll
-        for (int j = 0; j < SCAN_E; ++j) {
-            const long long k = base + j;
-            v[q][j] = k < n[q] ? io.in[q][k] : 0;
-            tsum[q] += v[q][j];
+            for (int j = 0; j < SCAN_E; ++j) {
+                const long long k = base + j;
+                v[q][j] = k < n[q] ? io.in[q][k] : 0;
+            }
         }
+#pragma unroll
+        for (int j = 0; j < SCAN_E; ++j) tsum[q] += v[q][j];
         incl[q] = wave_incl_scan(tsum[q]);
         if (lane == 63) wsum[q][w] = incl[q];
     }
@@ -135,12 +149,19 @@ __global__ __launch_bounds__(SCAN_T) void scan_block_kernel(ScanIO<K> io, int *_
         int run = (w ? wsum[q][w - 1] : 0) + incl[q] - tsum[q];
         if (PRE) run += wpre[q][0];
         tot[q] = 0;
+        int o[SCAN_E];
 #pragma unroll
         for (int j = 0; j < SCAN_E; ++j) {
-            const long long k = base + j;
-            if (k <= n[q]) io.out[q][k] = run;
-            if (k == n[q]) tot[q] = run;
+            o[j] = run;
+            if (base + j == n[q]) tot[q] = run;
             run += v[q][j];
+        }
+        if (base + SCAN_E <= n[q] + 1 && aligned16(io.out[q])) {   // one 16-byte store per thread
+            *reinterpret_cast<int4 *>(io.out[q] + base) = make_int4(o[0], o[1], o[2], o[3]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < SCAN_E; ++j)
+                if (base + j <= n[q]) io.out[q][base + j] = o[j];
         }
         if (block_sums && threadIdx.x == SCAN_T - 1)
             block_sums[q * sums_stride + blockIdx.x] = wsum[q][SCAN_T / 64 - 1];
@@ -161,10 +182,15 @@ __global__ __launch_bounds__(SCAN_T) void scan_reduce_kernel(ScanIO<K> io, int *
     for (int q = 0; q < K; ++q) {
         const int n = scan_len(io, q);
         int t = 0;
+        if (base + SCAN_E <= n && aligned16(io.in[q])) {
+            const int4 x = *reinterpret_cast<const int4 *>(io.in[q] + base);
+            t = x.x + x.y + x.z + x.w;
+        } else {
 #pragma unroll
-        for (int j = 0; j < SCAN_E; ++j) {
-            const long long k = base + j;
-            t += k < n ? io.in[q][k] : 0;
+            for (int j = 0; j < SCAN_E; ++j) {
+                const long long k = base + j;
+                t += k < n ? io.in[q][k] : 0;
+            }
         }
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
