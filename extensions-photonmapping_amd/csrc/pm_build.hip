@@ -738,7 +738,7 @@ __device__ __forceinline__ void scatter_segment(
     // groups of G iterations: the group's refs are loaded first (lanes past
     // the chunk re-read its first ref), then classified and written
 #ifndef PM_SCATTER_GROUP
-#define PM_SCATTER_GROUP 4
+#define PM_SCATTER_GROUP 8
 #endif
     constexpr int IT = SEGT / 64, G = IT < PM_SCATTER_GROUP ? IT : PM_SCATTER_GROUP;
 #pragma unroll 4
