@@ -46,6 +46,59 @@ def ray_bytes(cnt) -> int:
     return int(48 * n + 13 * c[0] + 8 * c[1] + 52 * c[2] + 32 * c[3])
 
 
+def bench_heightfield(pm, scenes, torch, reps: int = 5, stack: int = 8) -> dict:
+    """Heightfield terrain, 1024x1024 grid (2,093,058 triangles): SAH build,
+    then 1920x1080 primary rays and one cosine-distributed bounce ray per hit,
+    short stack; build and the two trace passes timed separately (HIP events,
+    median of ``reps`` after one warm-up)."""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    tris, (org, dirs) = scenes.config2_primary()
+    ntris = int(tris[0].shape[0])
+    P = [torch.from_numpy(x).to(dev) for x in tris]
+    O, D = torch.from_numpy(org).to(dev), torch.from_numpy(dirs).to(dev)
+    n = int(O.shape[0])
+    H = torch.empty((n, 4), dtype=torch.float32, device=dev)
+    ctx = pm.Context(0)
+    bp = pm.build_params("reference")
+    t_in = pm.trace_params(stack_size=stack, ray_order=pm.RAY_ORDER_INPUT)
+    t_co = pm.trace_params(stack_size=stack, ray_order=pm.RAY_ORDER_COHERENT)
+
+    def timed(fn):
+        out = []
+        for i in range(reps + 1):
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            ev[0].record(stream)
+            fn()
+            ev[1].record(stream)
+            torch.cuda.synchronize()
+            if i:
+                out.append(ev[0].elapsed_time(ev[1]))
+        return float(np.median(out))
+
+    build_ms = timed(lambda: ctx.build(*P, bp))
+    st = ctx.stats()
+    primary_ms = timed(lambda: ctx.trace(pm.SHORT_STACK, O, D, t_in, out=H))
+    bo, bd, _, nb = ctx.photon_bounce(O, D, H, ntris, seed=2002)
+    bo, bd = bo[:nb].contiguous(), bd[:nb].contiguous()
+    HB = torch.empty((max(nb, 1), 4), dtype=torch.float32, device=dev)
+    bounce_ms = timed(lambda: ctx.trace(pm.SHORT_STACK, bo, bd, t_co, out=HB[:nb]))
+    bounce_in_ms = timed(lambda: ctx.trace(pm.SHORT_STACK, bo, bd, t_in, out=HB[:nb]))
+    ctx.close()
+    best_bounce = min(bounce_ms, bounce_in_ms)
+    return {"workload": f"heightfield 1024x1024 ({ntris} tris), 1920x1080 primary rays + one cosine "
+                        f"bounce ray per hit, short stack depth {stack}",
+            "tris": ntris, "kd_build_ms": round(build_ms, 3),
+            "tree": {k: st[k] for k in ("num_nodes", "leaves", "max_depth", "leaf_refs", "upper_levels",
+                                        "lower_levels")},
+            "primary_rays": n, "primary_ms": round(primary_ms, 3),
+            "primary_mrays": round(n / (primary_ms * 1e-3) / 1e6, 2),
+            "bounce_rays": int(nb), "bounce_ms_coherent": round(bounce_ms, 3),
+            "bounce_ms_input_order": round(bounce_in_ms, 3),
+            "trace_mrays": round((n + nb) / ((primary_ms + best_bounce) * 1e-3) / 1e6, 2)}
+
+
 def bench_fwd_bwd(pm, scenes, torch, reps: int = 5) -> dict:
     """65,536-triangle perturbed sphere, 512x512 rays, loss sum (t - t_target)^2:
     plain forward trace vs forward + hand-written backward (same GPU)."""
@@ -235,6 +288,8 @@ def main() -> int:
                     help="rays the CPU baseline traces (0 disables the baseline)")
     ap.add_argument("--meshes", type=int, default=256,
                     help="meshes of the batched-rebuild comparison (0 skips it)")
+    ap.add_argument("--heightfield", type=int, default=1,
+                    help="1: also time the 2M-triangle heightfield build and its primary + bounce rays")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched-rebuild comparison and print its record (profiling runs)")
     ap.add_argument("--probe", action="store_true",
@@ -410,6 +465,8 @@ def main() -> int:
 
     fwd_bwd = bench_fwd_bwd(pm, scenes, torch)
     log(f"[bench] fwd+bwd: {fwd_bwd}")
+    heightfield = bench_heightfield(pm, scenes, torch, stack=args.stack) if args.heightfield else None
+    log(f"[bench] heightfield: {heightfield}")
     batched = bench_batched(pm, scenes, torch, meshes=args.meshes) if args.meshes > 0 else None
     log(f"[bench] batched: {batched}")
 
@@ -467,6 +524,7 @@ def main() -> int:
             "emission_hit_fraction": round(hit_frac, 4),
             "fwd_bwd": fwd_bwd,
             "batched": batched,
+            "heightfield": heightfield,
             "tree": {k: st[k] for k in ("num_nodes", "leaves", "empty_leaves", "max_depth",
                                         "leaf_refs", "upper_levels", "lower_levels")},
             "build_roofline": build_roofline(st, ntris, build_ms),

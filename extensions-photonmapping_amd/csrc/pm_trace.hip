@@ -367,15 +367,17 @@ __device__ __forceinline__ int coop_leaf(const float4 *__restrict__ L, CoopLds &
     C.key[lane] = ~0ull;
     C.org[lane] = make_float4(o.x, o.y, o.z, __int_as_float(base));
     C.dir[lane] = make_float4(d.x, d.y, d.z, th);
+    // k <= 64 pairs per lane: the wave's exclusive prefix sum of k, bit by
+    // bit -- each bit's ballot counted below this lane (mbcnt) and in all
     const int k = __popcll(bm);
-    int incl = k;
+    int excl = 0, W = 0;                  // this lane's first pair; the wave's pairs
 #pragma unroll
-    for (int dl = 1; dl < 64; dl <<= 1) {
-        const int y = __shfl_up(incl, dl, 64);
-        if (lane >= dl) incl += y;
+    for (int b = 0; b < 7; ++b) {
+        const unsigned long long m = __ballot((k >> b) & 1);
+        excl += (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                               __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u)) << b;
+        W += __popcll(m) << b;
     }
-    const int W = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
-    const int excl = incl - k;            // this lane's first pair
     int pos = excl;
     // a run of bits b0 .. b0+k-1 (every leaf the upper stage makes) is listed
     // four pairs to a 64-bit store; other bitmaps bit by bit
