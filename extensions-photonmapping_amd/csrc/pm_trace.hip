@@ -190,13 +190,16 @@ __device__ __forceinline__ bool descend(const TreeDev &T, RayState &R, int S, in
         const int upper = 0 < dd ? nd.z : nd.y;
         if (R.th < tSplit) {
             node = lower;
-            if (SHORT && tSplit < tNext) {           // Push, ShortStack.h:57-62
+            // here tSplit > th >= 0, and tNext (the sentinel or an earlier
+            // tSplit) is never NaN: the select is exactly fminf(tSplit, tNext)
+            const bool nearer = tSplit < tNext;
+            if (SHORT && nearer) {                   // Push, ShortStack.h:57-62
                 stk_node[R.nxt][lt] = upper;
                 stk_t[R.nxt][lt] = tNext;
                 R.nxt = R.nxt == S - 1 ? 0 : R.nxt + 1;
                 R.count = R.count == S ? S : R.count + 1;
             }
-            tNext = fminf(tSplit, tNext);
+            tNext = nearer ? tSplit : tNext;
         } else {
             node = upper;
         }
