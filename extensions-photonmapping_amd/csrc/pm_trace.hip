@@ -129,7 +129,7 @@ struct RayState {
     f3 o, d, inv;
     float th, tu, tv;
     int root_id;
-    int nxt, count;            // short stack: ring position and fill
+    int nxt, count;            // short stack: ring position (slot * TB) and fill
     int visits;
 };
 
@@ -164,19 +164,22 @@ __device__ __forceinline__ pm_hit ray_record(const RayState &R, int hit) {
 // (a miss) before it reaches a leaf; else node/nd are the leaf and R.th its
 // exit.  trace_step() is the whole pass: it returns true when the ray is
 // done, `hit` being its triangle or -1.
+// The ring lives in one LDS array int[2 * SMAX][TB]: rows 0..SMAX-1 hold the
+// far nodes, rows SMAX.. their tNext bits; `stk` points at this lane's column
+// and R.nxt counts in rows of TB, so a slot's two words are one register
+// offset apart by a constant.
 template <int SMAX, bool SHORT, bool COUNT>
-__device__ __forceinline__ bool descend(const TreeDev &T, RayState &R, int S, int (*stk_node)[TB],
-                                        float (*stk_t)[TB], int4 &cnt, int &node, int4 &nd) {
+__device__ __forceinline__ bool descend(const TreeDev &T, RayState &R, int S, int *stk, int4 &cnt, int &node,
+                                        int4 &nd) {
     if (R.visits++ > T.num_nodes) return false;
-    const int lt = threadIdx.x;
     float tNext;
     if (!SHORT || R.count == 0) {
         node = R.root_id; tNext = F_INF; nd = T.packed[R.root_id];
     } else {                                     // Pop, ShortStack.h:68-72
-        R.nxt = (R.nxt == 0 ? S : R.nxt) - 1;
+        R.nxt = (R.nxt == 0 ? S * TB : R.nxt) - TB;
         R.count--;
-        node = stk_node[R.nxt][lt];
-        tNext = stk_t[R.nxt][lt];
+        node = stk[R.nxt];
+        tNext = __int_as_float(stk[R.nxt + SMAX * TB]);
         nd = T.packed[node];
     }
     int steps = 0;
@@ -194,28 +197,28 @@ __device__ __forceinline__ bool descend(const TreeDev &T, RayState &R, int S, in
             // tSplit) is never NaN: the select is exactly fminf(tSplit, tNext)
             const bool nearer = tSplit < tNext;
             if (SHORT && nearer) {                   // Push, ShortStack.h:57-62
-                stk_node[R.nxt][lt] = upper;
-                stk_t[R.nxt][lt] = tNext;
-                R.nxt = R.nxt == S - 1 ? 0 : R.nxt + 1;
-                R.count = R.count == S ? S : R.count + 1;
+                stk[R.nxt] = upper;
+                stk[R.nxt + SMAX * TB] = __float_as_int(tNext);
+                R.nxt = R.nxt == (S - 1) * TB ? 0 : R.nxt + TB;
+                R.count = min(R.count + 1, S);       // count <= S: count == S ? S : count + 1
             }
             tNext = nearer ? tSplit : tNext;
         } else {
             node = upper;
         }
-        nd = T.packed[node];
+        nd = T.packed[(unsigned)node];   // child indices are in [0, num_nodes)
     }
     R.th = tNext * 1.00001f;
     return true;
 }
 
 template <int SMAX, bool SHORT, bool SKIP, bool COUNT, bool WOOP>
-__device__ __forceinline__ bool trace_step(const TreeDev &T, RayState &R, int S, int (*stk_node)[TB],
-                                           float (*stk_t)[TB], int4 &cnt, int &hit) {
+__device__ __forceinline__ bool trace_step(const TreeDev &T, RayState &R, int S, int *stk, int4 &cnt,
+                                           int &hit) {
     hit = -1;
     int node;
     int4 nd;
-    if (!descend<SMAX, SHORT, COUNT>(T, R, S, stk_node, stk_t, cnt, node, nd)) return true;
+    if (!descend<SMAX, SHORT, COUNT>(T, R, S, stk, cnt, node, nd)) return true;
     hit = test_leaf<SKIP, COUNT, WOOP>(T, nd, node, R.o, R.d, R.inv, R.th, R.tu, R.tv, cnt);
     return hit != -1 || !(R.th < F_INF);
 }
@@ -225,18 +228,17 @@ template <bool SKIP, bool COUNT, bool WOOP>
 __device__ __forceinline__ pm_hit kd_restart_ray(const TreeDev &T, int root_id, f3 o, f3 d, int4 &cnt) {
     RayState R = ray_start(T, root_id, o, d);
     int hit;
-    while (!trace_step<1, false, SKIP, COUNT, WOOP>(T, R, 1, nullptr, nullptr, cnt, hit)) {
+    while (!trace_step<1, false, SKIP, COUNT, WOOP>(T, R, 1, nullptr, cnt, hit)) {
     }
     return ray_record(R, hit);
 }
 
 template <int SMAX, bool SKIP, bool COUNT, bool WOOP>
 __device__ __forceinline__ pm_hit short_stack_ray(const TreeDev &T, int root_id, f3 o, f3 d, int S,
-                                                  int (&stk_node)[SMAX][TB], float (&stk_t)[SMAX][TB],
-                                                  int4 &cnt) {
+                                                  int *stk, int4 &cnt) {
     RayState R = ray_start(T, root_id, o, d);
     int hit;
-    while (!trace_step<SMAX, true, SKIP, COUNT, WOOP>(T, R, S, stk_node, stk_t, cnt, hit)) {
+    while (!trace_step<SMAX, true, SKIP, COUNT, WOOP>(T, R, S, stk, cnt, hit)) {
     }
     return ray_record(R, hit);
 }
@@ -268,8 +270,8 @@ __global__ __launch_bounds__(TB) void trace_kernel(TreeDev T, const float4 *__re
                                                    const unsigned *__restrict__ order, int n, int S,
                                                    pm_hit *__restrict__ out, int4 *__restrict__ counters,
                                                    int *__restrict__ work) {
-    __shared__ int stk_node[SHORT ? SMAX : 1][TB];
-    __shared__ float stk_t[SHORT ? SMAX : 1][TB];
+    __shared__ int stk_rows[2 * (SHORT ? SMAX : 1)][TB];   // short stack: node rows, then tNext rows
+    int *const stk = &stk_rows[0][threadIdx.x];
     const int lane = threadIdx.x & 63;
     const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
     int next = 0, end = 0;          // the wave's chunk (same in every lane)
@@ -322,7 +324,7 @@ __global__ __launch_bounds__(TB) void trace_kernel(TreeDev T, const float4 *__re
         }
         if (busy) {
             int hit;
-            if (trace_step<SMAX, SHORT, SKIP, COUNT, WOOP>(T, R, S, stk_node, stk_t, cnt, hit)) {
+            if (trace_step<SMAX, SHORT, SKIP, COUNT, WOOP>(T, R, S, stk, cnt, hit)) {
                 out[q] = ray_record(R, hit);
                 if (COUNT) counters[q] = cnt;
                 q = -1;
@@ -455,8 +457,8 @@ __global__ __launch_bounds__(TB) void trace_coop_kernel(TreeDev T, const float4 
                                                         const unsigned *__restrict__ order, int n, int S,
                                                         pm_hit *__restrict__ out,
                                                         int4 *__restrict__ counters, int *__restrict__ work) {
-    __shared__ int stk_node[SHORT ? SMAX : 1][TB];
-    __shared__ float stk_t[SHORT ? SMAX : 1][TB];
+    __shared__ int stk_rows[2 * (SHORT ? SMAX : 1)][TB];   // short stack: node rows, then tNext rows
+    int *const stk = &stk_rows[0][threadIdx.x];
     __shared__ CoopLds coop[TB / 64];
     const int lane = threadIdx.x & 63;
     CoopLds &C = coop[threadIdx.x >> 6];
@@ -513,7 +515,7 @@ __global__ __launch_bounds__(TB) void trace_coop_kernel(TreeDev T, const float4 
         if (q >= 0) {
             int node;
             int4 nd;
-            if (!descend<SMAX, SHORT, COUNT>(T, R, S, stk_node, stk_t, cnt, node, nd)) {
+            if (!descend<SMAX, SHORT, COUNT>(T, R, S, stk, cnt, node, nd)) {
                 done = true;
             } else {
                 if (COUNT) cnt.y++;
@@ -756,8 +758,8 @@ __global__ __launch_bounds__(TB) void canvas_kernel(TreeDev T, const float4 *__r
                                                     const float4 *__restrict__ n2,
                                                     const uchar4 *__restrict__ c0, Light L, int max_bounces,
                                                     uchar4 *__restrict__ canvas) {
-    __shared__ int stk_node[ALGO == PM_TRACE_SHORT_STACK ? 16 : 1][TB];
-    __shared__ float stk_t[ALGO == PM_TRACE_SHORT_STACK ? 16 : 1][TB];
+    __shared__ int stk_rows[2 * (ALGO == PM_TRACE_SHORT_STACK ? 16 : 1)][TB];   // node rows, then tNext rows
+    int *const stk = &stk_rows[0][threadIdx.x];
     const int r = blockIdx.x * TB + threadIdx.x;
     if (r >= n) return;
     f3 o = mk3(org[r]), d = mk3(dir[r]);
@@ -771,7 +773,7 @@ __global__ __launch_bounds__(TB) void canvas_kernel(TreeDev T, const float4 *__r
         } else if constexpr (ALGO == PM_TRACE_KD_RESTART) {
             h = kd_restart_ray<SKIP, false, WOOP>(T, 0, o, d, cnt);
         } else {
-            h = short_stack_ray<16, SKIP, false, WOOP>(T, 0, o, d, S, stk_node, stk_t, cnt);
+            h = short_stack_ray<16, SKIP, false, WOOP>(T, 0, o, d, S, stk, cnt);
         }
         if (h.id < 0) break;
         col = blend(col, lighting(h, o, d, n0, n1, n2, c0, L));
