@@ -339,7 +339,7 @@ __global__ __launch_bounds__(TB) void trace_kernel(TreeDev T, const float4 *__re
 // with k_i triangles; tested lane by lane the wave would loop max k_i times
 // with most lanes idle.  Instead the wave lists all (lane, triangle) pairs
 // in LDS (COOP_WIN at a time) and every lane tests the next pair, with the
-// owner's ray and leaf exit th (each owner puts them in LDS first).  An
+// owner's ray and leaf exit th (read from the owner, see CoopLds).  An
 // owner's first hit is the pair with the smallest t, and among equal t the
 // triangle latest in bitmap order -- exactly what the sequential leaf loop
 // (accept 0 <= t <= th, then th = t) ends with -- so each accepted pair
@@ -349,12 +349,23 @@ __global__ __launch_bounds__(TB) void trace_kernel(TreeDev T, const float4 *__re
 // bitmap does; t >= 0 here (-0 keyed as +0).
 constexpr int COOP_WIN = 512;
 
+// The tester reads the owner's ray, leaf exit and first slot either from
+// LDS records the owners write (RECS) or straight from the owner's registers
+// (ds_bpermute, whole wave active).  The short-stack kernel takes the second:
+// its LDS, not its registers, caps its occupancy, and the records' 2 KiB a
+// wave is what brings a block under a sixth of the CU's LDS.
+template <bool RECS>
 struct CoopLds {
     alignas(8) unsigned short pair[COOP_WIN];   // bit << 6 | owner lane
     unsigned long long key[64];
-    float4 org[64];                      // owner's origin, w = its leaf's first slot (int bits)
-    float4 dir[64];                      // owner's direction, w = its leaf exit th
+    float4 org[RECS ? 64 : 1];           // owner's origin, w = its leaf's first slot (int bits)
+    float4 dir[RECS ? 64 : 1];           // owner's direction, w = its leaf exit th
 };
+
+// lane src4 / 4's value of x (every lane of the wave active)
+__device__ __forceinline__ float lane_read(float x, int src4) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src4, __float_as_int(x)));
+}
 
 // orders this lane's LDS accesses with the other lanes' of the wave (a
 // wave's LDS instructions execute in order; this keeps the compiler from
@@ -366,12 +377,15 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // every lane of the wave calls it; returns the lane's winning slot or -1
-__device__ __forceinline__ int coop_leaf(const float4 *__restrict__ L, CoopLds &C, unsigned long long bm,
+template <bool RECS>
+__device__ __forceinline__ int coop_leaf(const float4 *__restrict__ L, CoopLds<RECS> &C, unsigned long long bm,
                                          int base, f3 o, f3 d, float th) {
     const int lane = threadIdx.x & 63;
     C.key[lane] = ~0ull;
-    C.org[lane] = make_float4(o.x, o.y, o.z, __int_as_float(base));
-    C.dir[lane] = make_float4(d.x, d.y, d.z, th);
+    if (RECS) {
+        C.org[lane] = make_float4(o.x, o.y, o.z, __int_as_float(base));
+        C.dir[lane] = make_float4(d.x, d.y, d.z, th);
+    }
     // k <= 64 pairs per lane: the wave's exclusive prefix sum of k, bit by
     // bit -- each bit's ballot counted below this lane (mbcnt) and in all
     const int k = __popcll(bm);
@@ -429,14 +443,26 @@ __device__ __forceinline__ int coop_leaf(const float4 *__restrict__ L, CoopLds &
         wave_sync();
         for (int r = 0; r < wn; r += 64) {
             const int e = r + lane;
+            const int pr = e < wn ? (int)C.pair[e] : 0;
+            const int ow = pr & 63;
+            f3 ro, rd;
+            float rth;
+            int rbase;
+            if (RECS) {
+                const float4 a = C.org[ow], b = C.dir[ow];
+                ro = mk3(a); rd = mk3(b); rth = b.w; rbase = __float_as_int(a.w);
+            } else {
+                const int ow4 = ow << 2;
+                ro = {lane_read(o.x, ow4), lane_read(o.y, ow4), lane_read(o.z, ow4)};
+                rd = {lane_read(d.x, ow4), lane_read(d.y, ow4), lane_read(d.z, ow4)};
+                rth = lane_read(th, ow4);
+                rbase = __builtin_amdgcn_ds_bpermute(ow4, base);
+            }
             if (e < wn) {
-                const int pr = C.pair[e];
-                const int ow = pr & 63;
-                const float4 ro = C.org[ow], rd = C.dir[ow];
-                const int sl = __float_as_int(ro.w) + (pr >> 6);
+                const int sl = rbase + (pr >> 6);
                 const float4 a0 = L[3ll * sl], a1 = L[3ll * sl + 1], a2 = L[3ll * sl + 2];
                 float hx, hy, hz;
-                if (mt_edges_flat(mk3(a0), mk3(a1), mk3(a2), mk3(ro), mk3(rd), rd.w, hx, hy, hz)) {
+                if (mt_edges_flat(mk3(a0), mk3(a1), mk3(a2), ro, rd, rth, hx, hy, hz)) {
                     const unsigned long long key =
                         ((unsigned long long)__float_as_uint(hx + 0.0f) << 32) | (0xffffffffu - (unsigned)sl);
                     atomicMin(&C.key[ow], key);
@@ -459,9 +485,9 @@ __global__ __launch_bounds__(TB) void trace_coop_kernel(TreeDev T, const float4 
                                                         int4 *__restrict__ counters, int *__restrict__ work) {
     __shared__ int stk_rows[2 * (SHORT ? SMAX : 1)][TB];   // short stack: node rows, then tNext rows
     int *const stk = &stk_rows[0][threadIdx.x];
-    __shared__ CoopLds coop[TB / 64];
+    __shared__ CoopLds<!SHORT> coop[TB / 64];
     const int lane = threadIdx.x & 63;
-    CoopLds &C = coop[threadIdx.x >> 6];
+    CoopLds<!SHORT> &C = coop[threadIdx.x >> 6];
     const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
     int next = 0, end = 0;          // the wave's chunk (same in every lane)
     bool exhausted = false;
