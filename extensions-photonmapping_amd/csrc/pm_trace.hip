@@ -348,6 +348,7 @@ __global__ __launch_bounds__(TB) void trace_kernel(TreeDev T, const float4 *__re
 // Slots of one leaf grow with the bit index, so ~slot orders ties like the
 // bitmap does; t >= 0 here (-0 keyed as +0).
 constexpr int COOP_WIN = 512;
+static_assert(COOP_WIN % 64 == 0, "a wave reads pair[r + lane] for r < COOP_WIN, r a multiple of 64");
 
 // The tester reads the owner's ray, leaf exit and first slot either from
 // LDS records the owners write (RECS) or straight from the owner's registers
@@ -443,7 +444,7 @@ __device__ __forceinline__ int coop_leaf(const float4 *__restrict__ L, CoopLds<R
         wave_sync();
         for (int r = 0; r < wn; r += 64) {
             const int e = r + lane;
-            const int pr = e < wn ? (int)C.pair[e] : 0;
+            const int pr = C.pair[e];   // e < COOP_WIN: stale past wn, used only for an in-range lane
             const int ow = pr & 63;
             f3 ro, rd;
             float rth;
@@ -460,7 +461,10 @@ __device__ __forceinline__ int coop_leaf(const float4 *__restrict__ L, CoopLds<R
             }
             if (e < wn) {
                 const int sl = rbase + (pr >> 6);
-                const float4 a0 = L[3ll * sl], a1 = L[3ll * sl + 1], a2 = L[3ll * sl + 2];
+                // 3 * sl < 2^31 (slots are ints): a 32-bit shift-add and a
+                // zero-extended 64-bit shift-add instead of a 64-bit multiply
+                const float4 *R3 = L + (unsigned)(sl + (sl << 1));
+                const float4 a0 = R3[0], a1 = R3[1], a2 = R3[2];
                 float hx, hy, hz;
                 if (mt_edges_flat(mk3(a0), mk3(a1), mk3(a2), ro, rd, rth, hx, hy, hz)) {
                     const unsigned long long key =
