@@ -1467,6 +1467,10 @@ static void scan_exclusive3(const int *in0, int *out0, const int *in1, int *out1
 
 namespace {
 
+__global__ void single_offsets_kernel(int *off, int N) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) { off[0] = 0; off[1] = N; }
+}
+
 __global__ void init_lvl_kernel(Lvl *L, Lvl v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *L = v;
 }
@@ -1763,8 +1767,13 @@ void upload_offsets(pm_ctx *ctx, const int *mesh_offsets, int N, int M, hipStrea
         if (off_h[m] > off_h[m + 1]) throw Error{PM_ERR_ARG, "mesh offsets must be non-decreasing"};
     if (off_h[0] != 0 || off_h[M] != N) throw Error{PM_ERR_ARG, "mesh offsets must span [0, n]"};
     sc.mesh_off.reserve((size_t)M + 1, s);
-    PM_HIP(hipMemcpyAsync(sc.mesh_off.ptr, off_h.data(), (M + 1) * sizeof(int), hipMemcpyHostToDevice, s));
-    PM_HIP(hipStreamSynchronize(s));
+    if (M == 1) {   // {0, N}: written on the device, no staged copy and no stream wait
+        single_offsets_kernel<<<1, 64, 0, s>>>(sc.mesh_off.ptr, N);
+        PM_LAUNCH_CHECK();
+    } else {
+        PM_HIP(hipMemcpyAsync(sc.mesh_off.ptr, off_h.data(), (M + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+        PM_HIP(hipStreamSynchronize(s));
+    }
     ctx->mesh_offsets = off_h;
     ctx->num_roots = M;
 }
