@@ -289,6 +289,22 @@ __device__ __forceinline__ void key_atomics(int *k, int k0, int k1, int k2, int 
     atomicMax(k + 3, k3); atomicMax(k + 4, k4); atomicMax(k + 5, k5);
 }
 
+// One box key word joined by min (is_min) or max, skipping the atomic when
+// the word already holds as good a value.  Box keys only fall (mins) or rise
+// (maxes) during a pass, so a read that is already stale can only cause an
+// atomic that changes nothing: the final keys are those of the plain
+// atomics.  On the top levels every chunk of a level joins the same two
+// children, and the atomics on those few words serialise; with many nodes
+// (JOIN_TEST_NODES or more) they do not, and the read only adds latency.
+constexpr int JOIN_TEST_NODES = 128;
+__device__ __forceinline__ void key_join(int *k, int v, bool is_min) {
+    const int cur = __hip_atomic_load(k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (is_min ? v < cur : v > cur) {
+        if (is_min) atomicMin(k, v);
+        else atomicMax(k, v);
+    }
+}
+
 constexpr int BOX_TILE = BS * 8;   // consecutive triangles per block
 
 // CalcPrimitiveAabb (Kernels/TriangleUpper.h:10-28) + the root boxes.  A
@@ -360,7 +376,13 @@ __global__ __launch_bounds__(BS) void tri_box_kernel(const float4 *__restrict__ 
             k0 = min(k0, red[0][j]); k1 = min(k1, red[1][j]); k2 = min(k2, red[2][j]);
             k3 = max(k3, red[3][j]); k4 = max(k4, red[4][j]); k5 = max(k5, red[5][j]);
         }
-        key_atomics(keys + 6 * m0, k0, k1, k2, k3, k4, k5);
+        int *k = keys + 6 * m0;
+        if (M < JOIN_TEST_NODES) {
+            key_join(k + 0, k0, true); key_join(k + 1, k1, true); key_join(k + 2, k2, true);
+            key_join(k + 3, k3, false); key_join(k + 4, k4, false); key_join(k + 5, k5, false);
+        } else {
+            key_atomics(k, k0, k1, k2, k3, k4, k5);
+        }
     }
 }
 
@@ -713,7 +735,7 @@ __device__ __forceinline__ void scatter_segment(
     const int *__restrict__ segLRoff, const int *__restrict__ slot_leaf,
     const float4 *__restrict__ rmin, const float4 *__restrict__ rmax,
     float4 *__restrict__ nmin, float4 *__restrict__ nmax, float4 *__restrict__ fmin,
-    float4 *__restrict__ fmax, int *__restrict__ keys_next, const DivideGeom &dg) {
+    float4 *__restrict__ fmax, int *__restrict__ keys_next, const DivideGeom &dg, bool join_test) {
     const int lane = threadIdx.x & 63;
     const int i = seg_node[s];
     const int node = active_index + i;
@@ -791,20 +813,34 @@ __device__ __forceinline__ void scatter_segment(
             rr += __popcll(brm);
         }
     }
-    if (!leafL && rl > 0) {
+    const bool doL = !leafL && rl > 0, doR = !leafR && rr > 0;   // wave-uniform
+    if (doL) {
         l0 = wave_min(l0); l1 = wave_min(l1); l2 = wave_min(l2);
         l3 = wave_max(l3); l4 = wave_max(l4); l5 = wave_max(l5);
-        if (lane == 0) key_atomics(keys_next + 6 * (ch.x - next_active), l0, l1, l2, l3, l4, l5);
     }
-    if (!leafR && rr > 0) {
+    if (doR) {
         r0 = wave_min(r0); r1 = wave_min(r1); r2 = wave_min(r2);
         r3 = wave_max(r3); r4 = wave_max(r4); r5 = wave_max(r5);
-        if (lane == 0) key_atomics(keys_next + 6 * (ch.y - next_active), r0, r1, r2, r3, r4, r5);
+    }
+    if (!join_test) {
+        if (lane == 0 && doL) key_atomics(keys_next + 6 * (ch.x - next_active), l0, l1, l2, l3, l4, l5);
+        if (lane == 0 && doR) key_atomics(keys_next + 6 * (ch.y - next_active), r0, r1, r2, r3, r4, r5);
+        return;
+    }
+    // few nodes: lanes 0..5 join the left child's six key words, lanes 6..11
+    // the right's, each skipping a join that cannot change its word
+    const bool right = lane >= 6;
+    const int t = right ? lane - 6 : lane;
+    if (lane < 12 && (right ? doR : doL)) {
+        const int a0 = right ? r0 : l0, a1 = right ? r1 : l1, a2 = right ? r2 : l2;
+        const int a3 = right ? r3 : l3, a4 = right ? r4 : l4, a5 = right ? r5 : l5;
+        const int v = t == 0 ? a0 : t == 1 ? a1 : t == 2 ? a2 : t == 3 ? a3 : t == 4 ? a4 : a5;
+        key_join(keys_next + 6 * ((right ? ch.y : ch.x) - next_active) + t, v, t < 3);
     }
 }
 
 template <int SEGT, bool DIV>
-__global__ __launch_bounds__(WPB * 64) void scatter_kernel(
+__device__ __forceinline__ void scatter_level(
     const NodePtrs N, const Lvl *__restrict__ L,
     const int *__restrict__ seg_node, const int *__restrict__ seg_start,
     const int *__restrict__ seg_count, const int *__restrict__ segLoff,
@@ -818,7 +854,37 @@ __global__ __launch_bounds__(WPB * 64) void scatter_kernel(
     for (int s = wave_index(); s < S; s += wave_count())
         scatter_segment<SEGT, DIV>(N, s, A, active_index, next_active, L_all, LL_all, ulp, seg_node, seg_start,
                               seg_count, segLoff, segRoff, segLLoff, segLRoff, slot_leaf, rmin, rmax, nmin,
-                              nmax, fmin, fmax, keys_next, dg);
+                              nmax, fmin, fmax, keys_next, dg, A < JOIN_TEST_NODES);
+}
+
+// box splits: at most 128 VGPRs (4 waves per SIMD) with a group's eight
+// float4 pairs in flight; DIVIDE (more live state) keeps its own allocation
+template <int SEGT>
+__global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(4))) void scatter_kernel(
+    const NodePtrs N, const Lvl *__restrict__ L,
+    const int *__restrict__ seg_node, const int *__restrict__ seg_start,
+    const int *__restrict__ seg_count, const int *__restrict__ segLoff,
+    const int *__restrict__ segRoff, const int *__restrict__ segLLoff,
+    const int *__restrict__ segLRoff, const int *__restrict__ slot_leaf,
+    const float4 *__restrict__ rmin, const float4 *__restrict__ rmax,
+    float4 *__restrict__ nmin, float4 *__restrict__ nmax, float4 *__restrict__ fmin,
+    float4 *__restrict__ fmax, int *__restrict__ keys_next, DivideGeom dg) {
+    scatter_level<SEGT, false>(N, L, seg_node, seg_start, seg_count, segLoff, segRoff, segLLoff, segLRoff, slot_leaf, rmin, rmax, nmin, nmax,
+        fmin, fmax, keys_next, dg);
+}
+
+template <int SEGT>
+__global__ __launch_bounds__(WPB * 64) void scatter_div_kernel(
+    const NodePtrs N, const Lvl *__restrict__ L,
+    const int *__restrict__ seg_node, const int *__restrict__ seg_start,
+    const int *__restrict__ seg_count, const int *__restrict__ segLoff,
+    const int *__restrict__ segRoff, const int *__restrict__ segLLoff,
+    const int *__restrict__ segLRoff, const int *__restrict__ slot_leaf,
+    const float4 *__restrict__ rmin, const float4 *__restrict__ rmax,
+    float4 *__restrict__ nmin, float4 *__restrict__ nmax, float4 *__restrict__ fmin,
+    float4 *__restrict__ fmax, int *__restrict__ keys_next, DivideGeom dg) {
+    scatter_level<SEGT, true>(N, L, seg_node, seg_start, seg_count, segLoff, segRoff, segLLoff, segLRoff, slot_leaf, rmin, rmax, nmin, nmax,
+        fmin, fmax, keys_next, dg);
 }
 
 // ---- plan kernels: the host arithmetic of the level loop
@@ -1355,15 +1421,21 @@ void launch_classify(int seg, bool div, int blocks, hipStream_t s, Args... a) {
     else launch_classify_t<false>(seg, blocks, s, a...);
 }
 
+template <int SEGT, bool DIV, class... Args>
+void launch_scatter_k(int blocks, hipStream_t s, Args... a) {
+    if constexpr (DIV) scatter_div_kernel<SEGT><<<blocks, WPB * 64, 0, s>>>(a...);
+    else scatter_kernel<SEGT><<<blocks, WPB * 64, 0, s>>>(a...);
+}
+
 template <bool DIV, class... Args>
 void launch_scatter_t(int seg, int blocks, hipStream_t s, Args... a) {
     switch (seg) {
-    case 128: scatter_kernel<128, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
-    case 512: scatter_kernel<512, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
-    case 1024: scatter_kernel<1024, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
-    case 2048: scatter_kernel<2048, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
-    case 4096: scatter_kernel<4096, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
-    default: scatter_kernel<256, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    case 128: launch_scatter_k<128, DIV>(blocks, s, a...); break;
+    case 512: launch_scatter_k<512, DIV>(blocks, s, a...); break;
+    case 1024: launch_scatter_k<1024, DIV>(blocks, s, a...); break;
+    case 2048: launch_scatter_k<2048, DIV>(blocks, s, a...); break;
+    case 4096: launch_scatter_k<4096, DIV>(blocks, s, a...); break;
+    default: launch_scatter_k<256, DIV>(blocks, s, a...); break;
     }
 }
 
