@@ -336,3 +336,17 @@ def test_random_configs_bit_exact(pm, orc, scenes, torch, ctx):
         assert_same_hits(h, r, label)
         checked += 1
     assert checked >= 35
+
+
+def test_pinned_area_and_empty_space_scenes_bit_exact(pm, orc, torch, ctx):
+    """the constructed scenes that pin the oracle's areas (S1, S4) and its
+    empty-space rule (B4) build the same trees on the GPU"""
+    import test_oracle_pins as pins
+    cases = [(t, dict(preset="default", lower_size=4)) for t in pins.b4_scenes()]
+    dy, _ = pins.dyadic_scene(900, seed=41)
+    cases += [(dy, dict(preset="reference", traversal_cost=1e9)), (dy, dict(preset="reference"))]
+    for i, (tris, spec) in enumerate(cases):
+        params = make_params(pm, spec)
+        _, got = gpu_build(pm, ctx, torch, tris, params)
+        ref = orc.build(tris, orc_params(orc, params)).arrays()
+        assert_same_tree(got, ref, f"pinned scene {i}")
