@@ -126,7 +126,7 @@ def load_library(path: str = LIB_PATH):
     L.pm_photon_bounce.argtypes = [P, P, P, P, I, I, C.c_ulonglong, C.c_float, P, P, P, P,
                                    C.POINTER(C.c_int), P]
     L.pm_add_mesh.argtypes = [P, P, I, P, I, P, P, P, P, I, P]
-    L.pm_trace_backward.argtypes = [P, P, P, P, I, P, P, P, P, I, P, P, P, P]
+    L.pm_trace_backward.argtypes = [P, P, P, P, I, P, P, P, I, P, P, P, P, I, P, P, P, P]
     L.pm_build_batched.argtypes = [P, P, P, P, I, P, I, C.POINTER(BuildParams), P]
     L.pm_trace_batched.argtypes = [P, I, P, P, P, I, P, C.POINTER(TraceParams), P, P]
     F = C.c_float
@@ -472,15 +472,19 @@ class Context:
 
     def trace_backward(self, org, dirs, hits, grad_t=None, grad_u=None, grad_v=None, faces=None,
                        num_vertices=None, grad_vertices=None, grad_org=None, grad_dir=None,
-                       stream=None):
+                       tris=None, stream=None):
         """gradients of (t, u, v) w.r.t. vertices (deterministic per-vertex
         sums), ray origins and directions, for the hits of a previous trace
-        over the current geometry; returns (grad_vertices, grad_org, grad_dir)"""
+        over ``tris`` = (p0, p1, p2), the triangles that trace saw (default:
+        the geometry last given to this context); returns (grad_vertices,
+        grad_org, grad_dir)"""
         torch = _torch()
         n = org.shape[0]
         dev = org.device
+        p0, p1, p2 = tris if tris is not None else self._geom
+        nt = p0.shape[0]
         if num_vertices is None:
-            num_vertices = (3 * self._geom[0].shape[0]) if faces is None else int(faces.max().item()) + 1
+            num_vertices = (3 * nt) if faces is None else int(faces.max().item()) + 1
         if grad_vertices is None:
             grad_vertices = torch.empty((max(num_vertices, 1), 3), dtype=torch.float32, device=dev)
         if grad_org is None:
@@ -488,8 +492,9 @@ class Context:
         if grad_dir is None:
             grad_dir = torch.empty((max(n, 1), 3), dtype=torch.float32, device=dev)
         gp = [None if g is None else _dev_ptr(g) for g in (grad_t, grad_u, grad_v)]
+        tp = [_dev_ptr(x) if nt else None for x in (p0, p1, p2)]
         self._check(self._L.pm_trace_backward(
-            self.h, _dev_ptr(org) if n else None, _dev_ptr(dirs) if n else None,
+            self.h, *tp, nt, _dev_ptr(org) if n else None, _dev_ptr(dirs) if n else None,
             _dev_ptr(hits) if n else None, n, *gp,
             _dev_ptr(faces) if faces is not None else None, num_vertices, _dev_ptr(grad_vertices),
             _dev_ptr(grad_org), _dev_ptr(grad_dir), _stream_ptr(stream)))

@@ -49,10 +49,11 @@ class TraceFunction(torch.autograd.Function):
     @staticmethod
     def backward(fctx, gt, gu, gv, gid):
         o4, d4, hits, faces_flat, p0, p1, p2 = fctx.saved_tensors
-        c = fctx.context
-        c.set_geometry(p0, p1, p2)
         gt, gu, gv = (None if g is None else g.contiguous().float() for g in (gt, gu, gv))
-        gvert, go, gd = c.trace_backward(o4, d4, hits, gt, gu, gv, faces_flat, fctx.nv)
+        # the triangles this forward traced, passed explicitly: the context
+        # may hold another mesh's tree and geometry by now, and is not changed
+        gvert, go, gd = fctx.context.trace_backward(o4, d4, hits, gt, gu, gv, faces_flat, fctx.nv,
+                                                    tris=(p0, p1, p2))
         return gvert, None, go, gd, None, None, None
 
 

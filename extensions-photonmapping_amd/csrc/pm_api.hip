@@ -375,18 +375,21 @@ int pm_trace_batched(pm_ctx *ctx, int algo, const float *org, const float *dir, 
     });
 }
 
-int pm_trace_backward(pm_ctx *ctx, const float *org, const float *dir, const pm_hit *hits, int nrays,
+int pm_trace_backward(pm_ctx *ctx, const float *p0, const float *p1, const float *p2, int ntris,
+                      const float *org, const float *dir, const pm_hit *hits, int nrays,
                       const float *grad_t, const float *grad_u, const float *grad_v, const int *faces,
                       int num_vertices, float *grad_vertices, float *grad_org, float *grad_dir,
                       void *stream) {
     return guarded(ctx, [&] {
-        if (nrays < 0 || num_vertices < 0 || (nrays > 0 && (!org || !dir || !hits)) ||
-            (num_vertices > 0 && !grad_vertices))
+        if (nrays < 0 || num_vertices < 0 || ntris < 0 || (nrays > 0 && (!org || !dir || !hits)) ||
+            (num_vertices > 0 && !grad_vertices) || (ntris > 0 && (!p0 || !p1 || !p2)))
             throw Error{PM_ERR_ARG, "invalid arguments"};
-        if (!ctx->p0 && ctx->ntris > 0) throw Error{PM_ERR_NO_TREE, "no geometry"};
-        pmkd::trace_backward(ctx, reinterpret_cast<const float4 *>(org), reinterpret_cast<const float4 *>(dir),
-                             hits, nrays, grad_t, grad_u, grad_v, faces, num_vertices, grad_vertices, grad_org,
-                             grad_dir, (hipStream_t)stream);
+        // the sort's scan scratch is shared with a captured batch: its last replay must be done
+        pmkd::sync_graph_state(ctx, (hipStream_t)stream);
+        pmkd::trace_backward(ctx, reinterpret_cast<const float4 *>(p0), reinterpret_cast<const float4 *>(p1),
+                             reinterpret_cast<const float4 *>(p2), ntris, reinterpret_cast<const float4 *>(org),
+                             reinterpret_cast<const float4 *>(dir), hits, nrays, grad_t, grad_u, grad_v, faces,
+                             num_vertices, grad_vertices, grad_org, grad_dir, (hipStream_t)stream);
     });
 }
 
@@ -442,6 +445,9 @@ int pm_trace_canvas(pm_ctx *ctx, int algo, const float *org, const float *dir, i
             if (ctx->ntris > 0 && !ctx->p0) throw Error{PM_ERR_NO_TREE, "no geometry"};
         } else if (!ctx->has_tree) {
             throw Error{PM_ERR_NO_TREE, "no tree: call pm_build or pm_set_tree first"};
+        } else if (ctx->num_roots > 1) {
+            throw Error{PM_ERR_ARG, "the canvas traces one tree: the context holds a forest of " +
+                                        std::to_string(ctx->num_roots) + " meshes"};
         }
         pmkd::trace_canvas(ctx, algo, reinterpret_cast<const float4 *>(org), reinterpret_cast<const float4 *>(dir),
                            nrays, reinterpret_cast<const float4 *>(n0), reinterpret_cast<const float4 *>(n1),
@@ -493,6 +499,10 @@ int pm_photon_bounce(pm_ctx *ctx, const float *org, const float *dir, const pm_h
         if (n < 0 || (n > 0 && (!org || !dir || !hits || !out_org || !out_dir)))
             throw Error{PM_ERR_ARG, "invalid rays"};
         if (!ctx->p0 && ctx->ntris > 0) throw Error{PM_ERR_NO_TREE, "no geometry"};
+        if (absorb_from < 0 || absorb_from > ctx->ntris)
+            throw Error{PM_ERR_ARG, "absorb_from must be in [0, triangles of the geometry]"};
+        // the compaction's scan scratch is shared with a captured batch
+        pmkd::sync_graph_state(ctx, (hipStream_t)stream);
         const int c = pmkd::photon_bounce(ctx, reinterpret_cast<const float4 *>(org),
                                           reinterpret_cast<const float4 *>(dir), hits, n, absorb_from,
                                           seed, offset, reinterpret_cast<float4 *>(out_org),

@@ -12,7 +12,10 @@
 //   grad o = gT, grad d = gd, grad v1 = ge1, grad v2 = ge2,
 //   grad v0 = -gT - ge1 - ge2
 // (no counterpart in the reference; checked against FP64 central
-// differences and an FP64 adjoint).  Per-vertex gradients are reduced
+// differences and an FP64 adjoint).  A record whose id is not a triangle of
+// the given geometry, or whose triangle's face indices are outside
+// [0, num_vertices), contributes nothing and fails the call (checked in the
+// kernel, read back once at the end).  Per-vertex gradients are reduced
 // deterministically: the 3 corner entries of every ray are keyed by vertex
 // id, put in key order by a stable LSD radix sort (8-bit digits, wave-ballot
 // multisplit ranks), and each vertex sums its entries sequentially in
@@ -32,14 +35,27 @@ __device__ __forceinline__ f3 add3(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a
 // one ray: corner gradients (3 x float3) and origin/direction gradients
 __global__ __launch_bounds__(TB) void mt_backward_kernel(
     const float4 *__restrict__ P0, const float4 *__restrict__ P1, const float4 *__restrict__ P2,
-    const float4 *__restrict__ org, const float4 *__restrict__ dir, const pm_hit *__restrict__ hits,
-    int n, const float *__restrict__ grad_t, const float *__restrict__ grad_u,
-    const float *__restrict__ grad_v, const int *__restrict__ faces, unsigned miss_key,
-    unsigned *__restrict__ keys, float *__restrict__ corner,
-    float *__restrict__ grad_org, float *__restrict__ grad_dir) {
+    int ntris, const float4 *__restrict__ org, const float4 *__restrict__ dir,
+    const pm_hit *__restrict__ hits, int n, const float *__restrict__ grad_t,
+    const float *__restrict__ grad_u, const float *__restrict__ grad_v, const int *__restrict__ faces,
+    unsigned miss_key, unsigned *__restrict__ keys, float *__restrict__ corner,
+    float *__restrict__ grad_org, float *__restrict__ grad_dir, int *__restrict__ bad) {
     const int r = blockIdx.x * TB + threadIdx.x;
     if (r >= n) return;
-    const int id = hits[r].id;
+    int id = hits[r].id;
+    // a record that names no triangle of this geometry, or a hit triangle
+    // whose corners name no vertex, contributes nothing and fails the call
+    int vk[3] = {0, 0, 0};
+    if (id >= ntris || id < -1) {
+        atomicOr(bad, 1);
+        id = -1;
+    } else if (id >= 0) {
+        for (int k = 0; k < 3; ++k) vk[k] = faces ? faces[3 * id + k] : 3 * id + k;
+        if ((unsigned)vk[0] >= miss_key || (unsigned)vk[1] >= miss_key || (unsigned)vk[2] >= miss_key) {
+            atomicOr(bad, 2);
+            id = -1;
+        }
+    }
     const float gt = grad_t ? grad_t[r] : 0.0f;
     const float gu = grad_u ? grad_u[r] : 0.0f;
     const float gv = grad_v ? grad_v[r] : 0.0f;
@@ -67,7 +83,7 @@ __global__ __launch_bounds__(TB) void mt_backward_kernel(
     const f3 gc3[3] = {g0, g1, g2};
     for (int k = 0; k < 3; ++k) {
         const long long e = 3ll * r + k;
-        keys[e] = id >= 0 ? (faces ? (unsigned)faces[3 * id + k] : (unsigned)(3 * id + k)) : miss_key;
+        keys[e] = id >= 0 ? (unsigned)vk[k] : miss_key;
         corner[3 * e] = gc3[k].x; corner[3 * e + 1] = gc3[k].y; corner[3 * e + 2] = gc3[k].z;
     }
     if (grad_org) { grad_org[3 * r] = g_o.x; grad_org[3 * r + 1] = g_o.y; grad_org[3 * r + 2] = g_o.z; }
@@ -97,19 +113,14 @@ __global__ void zero_kernel(float *p, long long n) {
     if (i < n) p[i] = 0.0f;
 }
 
-__global__ __launch_bounds__(TB) void check_faces_kernel(const int *__restrict__ faces, long long n,
-                                                         int nv, int *bad) {
-    const long long i = (long long)blockIdx.x * TB + threadIdx.x;
-    if (i < n && (faces[i] < 0 || faces[i] >= nv)) atomicOr(bad, 1);
-}
-
 __global__ void clear_flag_kernel(int *p) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
 }
 
 }  // namespace
 
-void trace_backward(pm_ctx *ctx, const float4 *org, const float4 *dir, const pm_hit *hits, int n,
+void trace_backward(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2, int ntris,
+                    const float4 *org, const float4 *dir, const pm_hit *hits, int n,
                     const float *grad_t, const float *grad_u, const float *grad_v, const int *faces,
                     int num_vertices, float *grad_vertices, float *grad_org, float *grad_dir,
                     hipStream_t s) {
@@ -122,29 +133,19 @@ void trace_backward(pm_ctx *ctx, const float4 *org, const float4 *dir, const pm_
     if (n == 0) return;
     if (3ll * n > (long long)INT32_MAX / 2) throw Error{PM_ERR_ARG, "too many rays for one backward call"};
     const int m = 3 * n;
-    if (faces) {
-        sc.totals.reserve(8, s);
-        sc.host.ensure(8);
-        clear_flag_kernel<<<1, 64, 0, s>>>(sc.totals.ptr);
-        PM_LAUNCH_CHECK();
-        const long long nf = 3ll * ctx->ntris;
-        if (nf > 0) {
-            check_faces_kernel<<<(int)((nf + TB - 1) / TB), TB, 0, s>>>(faces, nf, num_vertices, sc.totals.ptr);
-            PM_LAUNCH_CHECK();
-        }
-        PM_HIP(hipMemcpyAsync(sc.host.ptr, sc.totals.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
-        PM_HIP(hipStreamSynchronize(s));
-        if (sc.host.ptr[0]) throw Error{PM_ERR_ARG, "face index outside [0, num_vertices)"};
-    } else if (3ll * ctx->ntris > num_vertices) {
+    if (!faces && 3ll * ntris > num_vertices)
         throw Error{PM_ERR_ARG, "without faces, num_vertices must be >= 3 * triangles"};
-    }
+    sc.totals.reserve(8, s);
+    sc.host.ensure(8);
+    clear_flag_kernel<<<1, 64, 0, s>>>(sc.totals.ptr);
+    PM_LAUNCH_CHECK();
     const unsigned miss_key = (unsigned)num_vertices;
     sc.bwd_keys[0].reserve(m, s); sc.bwd_keys[1].reserve(m, s);
     sc.bwd_vals[0].reserve(m, s); sc.bwd_vals[1].reserve(m, s);
     sc.bwd_corner.reserve(3 * (size_t)m, s);
-    mt_backward_kernel<<<(n + TB - 1) / TB, TB, 0, s>>>(ctx->p0, ctx->p1, ctx->p2, org, dir, hits, n, grad_t, grad_u,
+    mt_backward_kernel<<<(n + TB - 1) / TB, TB, 0, s>>>(P0, P1, P2, ntris, org, dir, hits, n, grad_t, grad_u,
                                                        grad_v, faces, miss_key, sc.bwd_keys[0].ptr,
-                                                       sc.bwd_corner.ptr, grad_org, grad_dir);
+                                                       sc.bwd_corner.ptr, grad_org, grad_dir, sc.totals.ptr);
     PM_LAUNCH_CHECK();
     int bits = 1;
     while (bits < 32 && ((unsigned long long)miss_key >> bits)) ++bits;
@@ -158,6 +159,10 @@ void trace_backward(pm_ctx *ctx, const float4 *org, const float4 *dir, const pm_
     segment_sum_kernel<<<(m + TB - 1) / TB, TB, 0, s>>>(K[cur], V[cur], m, miss_key,
                                                         sc.bwd_corner.ptr, grad_vertices);
     PM_LAUNCH_CHECK();
+    PM_HIP(hipMemcpyAsync(sc.host.ptr, sc.totals.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+    PM_HIP(hipStreamSynchronize(s));
+    if (sc.host.ptr[0] & 1) throw Error{PM_ERR_ARG, "hit id outside [-1, triangles)"};
+    if (sc.host.ptr[0] & 2) throw Error{PM_ERR_ARG, "face index outside [0, num_vertices)"};
 }
 
 }  // namespace pmkd

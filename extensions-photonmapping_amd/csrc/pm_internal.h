@@ -245,7 +245,8 @@ int photon_bounce(pm_ctx *ctx, const float4 *org, const float4 *dir, const pm_hi
                   int absorb_from, unsigned long long seed, float offset, float4 *out_org,
                   float4 *out_dir, int *out_src, int *out_count, hipStream_t s);
 // pm_backward.hip
-void trace_backward(pm_ctx *ctx, const float4 *org, const float4 *dir, const pm_hit *hits, int n,
+void trace_backward(pm_ctx *ctx, const float4 *p0, const float4 *p1, const float4 *p2, int ntris,
+                    const float4 *org, const float4 *dir, const pm_hit *hits, int n,
                     const float *grad_t, const float *grad_u, const float *grad_v, const int *faces,
                     int num_vertices, float *grad_vertices, float *grad_org, float *grad_dir,
                     hipStream_t s);

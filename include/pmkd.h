@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PMKD_ABI_VERSION 1
+#define PMKD_ABI_VERSION 2
 
 /* error codes */
 #define PM_OK                  0
@@ -353,17 +353,23 @@ int pm_add_mesh_attributes(pm_ctx *ctx, const float *normals, const unsigned cha
                            void *stream);
 
 /* Backward of the intersection (no reference counterpart; the forward is
- * MoellerTrumbore, IRayTracer.h:111-145).  For every ray with hits[r].id >= 0
- * (the records a pm_trace call produced for these rays) and upstream
- * gradients grad_t/grad_u/grad_v (device float[nrays] each, NULL = zero):
- * grad_org/grad_dir (device float[3*nrays], optional) get the per-ray
- * gradients; grad_vertices (device float[3*num_vertices]) gets the per-vertex
- * sums over corners, where faces (device int[3*triangles]) maps triangle
- * corners to vertex ids (NULL: vertex id = 3*triangle + corner, so
- * num_vertices must be >= 3*triangles).  The sums are deterministic: entries
- * are added in (ray, corner) order after a stable radix sort by vertex id.
- * Uses the geometry of the last build/set_geometry (the forward's). */
-int pm_trace_backward(pm_ctx *ctx, const float *org, const float *dir, const pm_hit *hits, int nrays,
+ * MoellerTrumbore, IRayTracer.h:111-145).  p0/p1/p2 (device float4[ntris])
+ * are the triangles the forward traced (the geometry of the build that made
+ * the hits); the context's own geometry is neither used nor changed.  For
+ * every ray with hits[r].id >= 0 (the records a pm_trace call produced for
+ * these rays) and upstream gradients grad_t/grad_u/grad_v (device
+ * float[nrays] each, NULL = zero): grad_org/grad_dir (device float[3*nrays],
+ * optional) get the per-ray gradients; grad_vertices (device
+ * float[3*num_vertices]) gets the per-vertex sums over corners, where faces
+ * (device int[3*ntris]) maps triangle corners to vertex ids (NULL: vertex
+ * id = 3*triangle + corner, so num_vertices must be >= 3*ntris).  The sums
+ * are deterministic: entries are added in (ray, corner) order after a stable
+ * radix sort by vertex id.  A hit id outside [-1, ntris), or a face index of
+ * a hit triangle outside [0, num_vertices), gives PM_ERR_ARG (the outputs
+ * are then unspecified; nothing is read or written out of range).
+ * Synchronises the stream (one 4-byte read-back at the end). */
+int pm_trace_backward(pm_ctx *ctx, const float *p0, const float *p1, const float *p2, int ntris,
+                      const float *org, const float *dir, const pm_hit *hits, int nrays,
                       const float *grad_t, const float *grad_u, const float *grad_v, const int *faces,
                       int num_vertices, float *grad_vertices, float *grad_org, float *grad_dir,
                       void *stream);

@@ -69,10 +69,38 @@ def test_backward_bit_exact_and_deterministic(pm, orc, scenes, torch):
     gc, _, _ = s["ctx"].trace_backward(s["O"], s["D"], s["H"], G[0])
     r_gc, _, _ = orc.trace_backward(s["tris"], s["org"], s["dirs"], s["h"], g[0])
     assert np.array_equal(gc.cpu().numpy().view(np.int32), r_gc.view(np.int32))
-    with pytest.raises(pm.PmError):
+    # a hit triangle whose face names no vertex: rejected, nothing written out of range
+    hit_tri = int(s["h"]["id"][s["h"]["id"] >= 0][0])
+    for v in (nv + 7, -1):
         bad = F.clone()
-        bad[0] = nv + 7
-        s["ctx"].trace_backward(s["O"], s["D"], s["H"], *G, bad, nv)
+        bad[3 * hit_tri + 1] = v
+        with pytest.raises(pm.PmError, match="face index"):
+            s["ctx"].trace_backward(s["O"], s["D"], s["H"], *G, bad, nv)
+
+
+def test_backward_rejects_foreign_hit_ids_and_takes_its_geometry(pm, orc, scenes, torch):
+    """records naming no triangle of the given geometry fail the call
+    (PM_ERR_ARG) and read nothing out of range; the geometry is the one
+    passed in, whatever the context was built over since"""
+    s = setup_sphere(pm, orc, scenes, torch, 64, 64, 4)
+    dev = "cuda:0"
+    n = s["org"].shape[0]
+    g = np.random.default_rng(4).normal(size=n).astype(np.float32)
+    G = torch.from_numpy(g).to(dev)
+    F = torch.from_numpy(s["faces"].reshape(-1)).to(dev)
+    nv = s["v_pert"].shape[0]
+    nt = s["faces"].shape[0]
+    for bad_id in (nt, nt + 1000, -2):
+        H = s["H"].clone()
+        H.view(torch.int32)[0, 0] = bad_id
+        with pytest.raises(pm.PmError, match="hit id"):
+            s["ctx"].trace_backward(s["O"], s["D"], H, G, None, None, F, nv, tris=s["P"])
+    # the context now holds another (smaller) mesh: the explicit geometry is used
+    other = [p[:16].clone() for p in s["P"]]
+    s["ctx"].build(*other, pm.build_params("reference"))
+    gv, _, _ = s["ctx"].trace_backward(s["O"], s["D"], s["H"], G, None, None, F, nv, tris=s["P"])
+    r_gv, _, _ = orc.trace_backward(s["tris"], s["org"], s["dirs"], s["h"], g, None, None, s["faces"], nv)
+    assert np.array_equal(gv.cpu().numpy().view(np.int32), r_gv.view(np.int32))
 
 
 def test_c5_gradient_matches_fp64_central_differences(pm, orc, scenes, torch):

@@ -107,6 +107,17 @@ def test_forest_argument_errors(pm, scenes, torch, ctx):
     for bad in ([0, 5, 3, n], [1, 5, 10, n], [0, 5, 10, n - 1]):
         with pytest.raises(pm.PmError):
             ctx.build_batched(*P, np.asarray(bad, np.int32))
+    # a forest cannot be shaded into one canvas (every ray would start at root 0)
+    ctx.build_batched(*P, off)
+    O, D = dev(torch, np.zeros((4, 4), np.float32)), dev(torch, np.ones((4, 4), np.float32))
+    N = [torch.zeros_like(P[0]) for _ in range(3)]
+    col = torch.zeros((n, 4), dtype=torch.uint8, device="cuda:0")
+    with pytest.raises(pm.PmError, match="forest"):
+        ctx.trace_canvas(pm.SHORT_STACK, O, D, N, col, [0.0] * 12)
+    # photons absorbed from an id past the geometry: rejected before any kernel runs
+    H = torch.zeros((4, 4), dtype=torch.float32, device="cuda:0")
+    with pytest.raises(pm.PmError, match="absorb_from"):
+        ctx.photon_bounce(O, D, H, n + 1, seed=1)
 
 
 def test_radial_deformer_matches_fp64(pm, scenes, torch, ctx):
