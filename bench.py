@@ -3,24 +3,28 @@
 
 Workload (the 1M-triangle configuration the headline metric is quoted on):
 clustered triangle soup, 1,000,000 triangles in 64 Gaussian clusters plus a
-2-triangle area light (1,000,002 triangles), 4M photons emitted
-cosine-weighted from the light, up to 3 diffuse bounces.
+2-triangle area light (1,000,002 triangles), 4M photons emitted from the
+light on the GPU (area-weighted light choice, cosine-weighted directions),
+up to 3 diffuse bounces.
 
 One step = rebuild the SAH kd-tree (REFERENCE constants: lower size 32,
-traversal cost 24, empty-space threshold 0.35) + trace every photon
-segment with the short-stack tracer (stack depth 8), bounce rays generated
-on the GPU between passes.  Inputs (triangles, emission rays) are resident in
+traversal cost 24, empty-space threshold 0.35) + emit the photons + trace
+every photon segment with the short-stack tracer (stack depth 8), bounce
+rays generated on the GPU between passes.  The triangles are resident in
 HBM before the timed region.
 
 Reported: value = trace throughput in Mrays/s (all rays of all passes over
-the summed GPU time of the trace launches), the kd-build time in ms next to
-it, the roofline of the dominant kernel (short-stack trace of the 4M
-emission rays) from HIP events around that launch, and the same build and
-trace algorithms run by the CPU oracle on the host as the CPU baseline.
+the summed GPU time of the trace calls), the kd-build time in ms next to it,
+the roofline of the dominant kernel (short-stack trace of the 4M emission
+rays) from HIP events around that call, the per-configuration lines
+(Cornell box, heightfield, batched meshes, differentiable intersection),
+and the same build and trace algorithms run by the CPU oracle on the host
+as the CPU baselines.
 """
 from __future__ import annotations
 
 import argparse
+import hashlib
 import importlib
 import json
 import os
@@ -30,11 +34,36 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+HBM_PEAK_GBS = 8000.0            # MI355X HBM3E
+SIMDS = 256 * 4                  # MI355X: 256 CUs x 4 SIMD units
+CLOCK_GHZ = 2.4                  # peak engine clock
+LANES_PER_SIMD = 16              # a wave64 VALU instruction occupies a SIMD for 4 cycles
+# the library sources the tracer's instruction stream depends on: a counter
+# summary is only used while they are unchanged
+TRACE_SOURCES = ("pm_trace.hip", "pm_device.h", "pm_order.hip")
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
+
+
+def source_sha() -> str:
+    h = hashlib.sha256()
+    for name in TRACE_SOURCES:
+        with open(os.path.join(ROOT, "extensions-photonmapping_amd", "csrc", name), "rb") as f:
+            h.update(name.encode() + b"\0" + f.read())
+    return h.hexdigest()[:16]
+
+
+def omp_threads() -> int:
+    """threads the CPU baselines use: OMP_NUM_THREADS if set, else the cores
+    this process may run on"""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(int(env), avail)) if env and env.isdigit() else avail
 
 
 def ray_bytes(cnt) -> int:
@@ -46,66 +75,179 @@ def ray_bytes(cnt) -> int:
     return int(48 * n + 13 * c[0] + 8 * c[1] + 52 * c[2] + 32 * c[3])
 
 
-def bench_heightfield(pm, scenes, torch, reps: int = 5, stack: int = 8) -> dict:
-    """Heightfield terrain, 1024x1024 grid (2,093,058 triangles): SAH build,
-    then 1920x1080 primary rays and one cosine-distributed bounce ray per hit,
-    short stack; build and the two trace passes timed separately (HIP events,
-    median of ``reps`` after one warm-up)."""
+def ray_counts(cnt) -> dict:
+    """mean per-ray traversal counts"""
+    c = cnt.float().mean(dim=0).tolist()
+    return {"inner_nodes": round(c[0], 2), "leaves": round(c[1], 2), "triangle_tests": round(c[2], 2),
+            "leaf_box_tests": round(c[3], 2)}
+
+
+def timed(torch, fn, reps: int = 5) -> float:
+    """median GPU ms of fn over reps runs after one warm-up (HIP events on
+    the current stream, the one every call here launches on)"""
+    import numpy as np
+    stream = torch.cuda.current_stream()
+    out = []
+    for i in range(reps + 1):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record(stream)
+        fn()
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        if i:
+            out.append(ev[0].elapsed_time(ev[1]))
+    return float(np.median(out))
+
+
+def wall(fn, reps: int = 1) -> float:
+    """median host wall ms of fn (CPU baselines)"""
+    import numpy as np
+    out = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        fn()
+        out.append((time.perf_counter() - t) * 1e3)
+    return float(np.median(out))
+
+
+TREE_KEYS = ("num_nodes", "leaves", "empty_leaves", "max_depth", "leaf_refs", "upper_levels", "lower_levels")
+
+
+def tree_stats(st: dict) -> dict:
+    return {k: st[k] for k in TREE_KEYS}
+
+
+# ------------------------------------------------------------------ C1
+def bench_cornell(pm, scenes, torch, orc, reps: int = 5, stack: int = 8, cpu: bool = True) -> dict:
+    """Cornell room (24 tris) + icosphere (5,120 tris), 256x256 jittered
+    camera rays made on the GPU (pm_create_rays), CONFIG preset (forced SAH
+    split above 8 triangles, 0.8 cost factor for an empty side); build, the
+    three tracers, and how many hit ids equal brute force's"""
     import numpy as np
     dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream()
-    tris, (org, dirs) = scenes.config2_primary()
+    W = H = 256
+    tris = scenes.cornell_icosphere(4)
+    P = [torch.from_numpy(t).to(dev) for t in tris]
+    m, c = scenes.camera(scenes.CAMERA_C1, W, H)
+    ctx = pm.Context(0)
+    O, D = ctx.create_rays(m, c, W, H, jitter_seed=1)
+    n = W * H
+    rays_ms = timed(torch, lambda: ctx.create_rays(m, c, W, H, jitter_seed=1, org=O, dirs=D), reps)
+    bp = pm.build_params("config")
+    tp = pm.trace_params(stack_size=stack)
+    build_ms = timed(torch, lambda: ctx.build(*P, bp), reps)
+    out = {}
+    hits = {}
+    for name, algo in (("short_stack", pm.SHORT_STACK), ("kd_restart", pm.KD_RESTART), ("brute", pm.BRUTE)):
+        Hh = torch.empty((n, 4), dtype=torch.float32, device=dev)
+        ms = timed(torch, lambda: ctx.trace(algo, O, D, tp, out=Hh), reps)
+        out[name] = {"ms": round(ms, 4), "mrays": round(n / (ms * 1e-3) / 1e6, 1)}
+        hits[name] = pm.hits_to_numpy(Hh)
+    same = lambda a, b: int(np.sum(hits[a]["id"] == hits[b]["id"]))
+    rec = {"workload": f"Cornell room + icosphere ({tris[0].shape[0]} tris), {W}x{H} jittered camera rays "
+                       f"made on the GPU, CONFIG preset (SAH, forced split above 8, empty-side factor 0.8), "
+                       f"short stack depth {stack}",
+           "tris": int(tris[0].shape[0]), "rays": n, "camera_rays_ms": round(rays_ms, 4),
+           "kd_build_ms": round(build_ms, 4), "trace": out,
+           "hit_ids_equal_brute": {"short_stack": same("short_stack", "brute"),
+                                   "kd_restart": same("kd_restart", "brute"), "of": n},
+           "hit_fraction": round(float(np.mean(hits["brute"]["id"] >= 0)), 4),
+           "tree": tree_stats(ctx.stats())}
+    if cpu:
+        org, dirs = O.cpu().numpy(), D.cpu().numpy()
+        th = omp_threads()
+        ob = orc.build_params("config")
+        tree = [None]
+        b_ms = wall(lambda: tree.__setitem__(0, orc.build(tris, ob)), 3)
+        res = {}
+        for name, algo in (("short_stack", orc.SHORT_STACK), ("kd_restart", orc.KD_RESTART), ("brute", orc.BRUTE)):
+            ms = wall(lambda: orc.trace(tree[0] if algo != orc.BRUTE else None, tris, org, dirs, algo,
+                                        stack_size=stack, threads=th), 1)
+            res[name] = round(n / (ms * 1e-3) / 1e6, 3)
+        rec["cpu_baseline"] = {"kind": "port", "cores": th, "build_ms": round(b_ms, 2), "mrays": res,
+                               "sample": "the oracle's build and all three traces of the same rays"}
+    ctx.close()
+    return rec
+
+
+# ------------------------------------------------------------------ C2
+def bench_heightfield(pm, scenes, torch, orc, reps: int = 5, stack: int = 8, cpu: bool = True) -> dict:
+    """Heightfield terrain, 1024x1024 grid (2,093,058 triangles): SAH build,
+    then 1920x1080 jittered camera rays made on the GPU and one
+    cosine-distributed bounce ray per hit, short stack; build and the two
+    trace passes timed separately (HIP events, median of ``reps`` after one
+    warm-up)"""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    W, Hh = 1920, 1080
+    tris = scenes.heightfield(1024, seed=2)
     ntris = int(tris[0].shape[0])
     P = [torch.from_numpy(x).to(dev) for x in tris]
-    O, D = torch.from_numpy(org).to(dev), torch.from_numpy(dirs).to(dev)
+    m, c = scenes.camera(scenes.CAMERA_C2, W, Hh)
+    ctx = pm.Context(0)
+    O, D = ctx.create_rays(m, c, W, Hh, jitter_seed=2)
+    rays_ms = timed(torch, lambda: ctx.create_rays(m, c, W, Hh, jitter_seed=2, org=O, dirs=D), reps)
     n = int(O.shape[0])
     H = torch.empty((n, 4), dtype=torch.float32, device=dev)
-    ctx = pm.Context(0)
     bp = pm.build_params("reference")
     t_in = pm.trace_params(stack_size=stack, ray_order=pm.RAY_ORDER_INPUT)
     t_co = pm.trace_params(stack_size=stack, ray_order=pm.RAY_ORDER_COHERENT)
-
-    def timed(fn):
-        out = []
-        for i in range(reps + 1):
-            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-            ev[0].record(stream)
-            fn()
-            ev[1].record(stream)
-            torch.cuda.synchronize()
-            if i:
-                out.append(ev[0].elapsed_time(ev[1]))
-        return float(np.median(out))
-
-    build_ms = timed(lambda: ctx.build(*P, bp))
+    build_ms = timed(torch, lambda: ctx.build(*P, bp), reps)
     st = ctx.stats()
-    primary_ms = timed(lambda: ctx.trace(pm.SHORT_STACK, O, D, t_in, out=H))
+    primary_ms = timed(torch, lambda: ctx.trace(pm.SHORT_STACK, O, D, t_in, out=H), reps)
+    cnt = torch.zeros((n, 4), dtype=torch.int32, device=dev)
+    ctx.trace(pm.SHORT_STACK, O, D, t_in, out=H, counters=cnt)
+    primary_counts = ray_counts(cnt)
     bo, bd, _, nb = ctx.photon_bounce(O, D, H, ntris, seed=2002)
     bo, bd = bo[:nb].contiguous(), bd[:nb].contiguous()
     HB = torch.empty((max(nb, 1), 4), dtype=torch.float32, device=dev)
-    bounce_ms = timed(lambda: ctx.trace(pm.SHORT_STACK, bo, bd, t_co, out=HB[:nb]))
-    bounce_in_ms = timed(lambda: ctx.trace(pm.SHORT_STACK, bo, bd, t_in, out=HB[:nb]))
-    ctx.close()
+    bounce_ms = timed(torch, lambda: ctx.trace(pm.SHORT_STACK, bo, bd, t_co, out=HB[:nb]), reps)
+    bounce_in_ms = timed(torch, lambda: ctx.trace(pm.SHORT_STACK, bo, bd, t_in, out=HB[:nb]), reps)
     best_bounce = min(bounce_ms, bounce_in_ms)
-    return {"workload": f"heightfield 1024x1024 ({ntris} tris), 1920x1080 primary rays + one cosine "
-                        f"bounce ray per hit, short stack depth {stack}",
-            "tris": ntris, "kd_build_ms": round(build_ms, 3),
-            "tree": {k: st[k] for k in ("num_nodes", "leaves", "max_depth", "leaf_refs", "upper_levels",
-                                        "lower_levels")},
-            "primary_rays": n, "primary_ms": round(primary_ms, 3),
-            "primary_mrays": round(n / (primary_ms * 1e-3) / 1e6, 2),
-            "bounce_rays": int(nb), "bounce_ms_coherent": round(bounce_ms, 3),
-            "bounce_ms_input_order": round(bounce_in_ms, 3),
-            "trace_mrays": round((n + nb) / ((primary_ms + best_bounce) * 1e-3) / 1e6, 2)}
+    rec = {"workload": f"heightfield 1024x1024 ({ntris} tris), {W}x{Hh} jittered camera rays made on the GPU "
+                       f"+ one cosine bounce ray per hit, short stack depth {stack}",
+           "tris": ntris, "kd_build_ms": round(build_ms, 3), "tree": tree_stats(st),
+           "camera_rays_ms": round(rays_ms, 4),
+           "primary_rays": n, "primary_ms": round(primary_ms, 3),
+           "primary_mrays": round(n / (primary_ms * 1e-3) / 1e6, 2), "primary_counts": primary_counts,
+           "bounce_rays": int(nb), "bounce_ms_coherent": round(bounce_ms, 3),
+           "bounce_ms_input_order": round(bounce_in_ms, 3),
+           "trace_mrays": round((n + nb) / ((primary_ms + best_bounce) * 1e-3) / 1e6, 2)}
+    if cpu:
+        th = omp_threads()
+        tree = [None]
+        b_ms = wall(lambda: tree.__setitem__(0, orc.build(tris, orc.build_params("reference"))), 1)
+        rng = np.random.default_rng(2)
+        org, dirs = O.cpu().numpy(), D.cpu().numpy()
+        sel = np.sort(rng.choice(n, 131072, replace=False))
+        ms = wall(lambda: orc.trace(tree[0], tris, org[sel], dirs[sel], orc.SHORT_STACK, stack_size=stack,
+                                    threads=th), 1)
+        bsel = sel[:1024]
+        bms = wall(lambda: orc.trace(None, tris, org[bsel], dirs[bsel], orc.BRUTE, threads=th), 1)
+        ost = tree[0].stats()
+        rec["cpu_baseline"] = {"kind": "port", "cores": th, "build_ms": round(b_ms, 1),
+                               "short_stack_mrays": round(sel.size / (ms * 1e-3) / 1e6, 3),
+                               "brute_mrays": round(bsel.size / (bms * 1e-3) / 1e6, 6),
+                               "sample": f"oracle build of all {ntris} tris; short stack on {sel.size} and brute "
+                                         f"force on {bsel.size} seeded camera rays (scaled Mrays/s)"}
+        rec["tree_oracle"] = tree_stats(ost)
+        rec["tree_equal_oracle"] = tree_stats(ost) == tree_stats(st)
+    ctx.close()
+    return rec
 
 
-def bench_fwd_bwd(pm, scenes, torch, reps: int = 5) -> dict:
-    """65,536-triangle perturbed sphere, 512x512 rays, loss sum (t - t_target)^2:
-    plain forward trace vs forward + hand-written backward (same GPU)."""
+# ------------------------------------------------------------------ C5
+def bench_fwd_bwd(pm, scenes, torch, orc, reps: int = 5, cpu: bool = True) -> dict:
+    """65,536-triangle perturbed sphere, 512x512 camera rays made on the GPU,
+    loss sum (t - t_target)^2: plain forward trace vs forward + hand-written
+    backward (same GPU)"""
     import numpy as np
     dev = torch.device("cuda:0")
-    v_ref, v_pert, faces, (org, dirs) = scenes.config5()
+    v_ref, v_pert, faces = scenes.perturbed_sphere(7, seed=5)
+    W = 512
+    m, c = scenes.camera(scenes.CAMERA_C5, W, W)
     ctx = pm.Context(0)
+    O, D = ctx.create_rays(m, c, W, W, jitter_seed=5)
     F = torch.from_numpy(faces.reshape(-1)).to(dev)
     nt = faces.shape[0]
 
@@ -116,7 +258,6 @@ def bench_fwd_bwd(pm, scenes, torch, reps: int = 5) -> dict:
         ctx.add_mesh(V4, F, *P)
         return P
 
-    O, D = torch.from_numpy(org).to(dev), torch.from_numpy(dirs).to(dev)
     tp = pm.trace_params(stack_size=8)
     ctx.build(*tris(v_ref), pm.build_params("reference"))
     ids_ref, t_ref, _, _ = pm.hits_view(ctx.trace(pm.SHORT_STACK, O, D, tp))
@@ -138,24 +279,39 @@ def bench_fwd_bwd(pm, scenes, torch, reps: int = 5) -> dict:
         ctx.trace(pm.SHORT_STACK, O, D, tp, out=H)
         e[1].record(stream)
         ids, t, _, _ = pm.hits_view(H)
-        m = (ids >= 0) & ok_ref
-        torch.where(m, 2.0 * (t - t_ref), torch.zeros_like(t), out=gt)
+        mask = (ids >= 0) & ok_ref
+        torch.where(mask, 2.0 * (t - t_ref), torch.zeros_like(t), out=gt)
         e[2].record(stream)
-        ctx.trace_backward(O, D, H, gt, None, None, F, nv, gv, go, gd)
+        ctx.trace_backward(O, D, H, gt, None, None, F, nv, gv, go, gd, tris=P)
         e[3].record(stream)
         torch.cuda.synchronize()
         if i:
             fwd.append(e[0].elapsed_time(e[1]))
             bwd.append(e[2].elapsed_time(e[3]))
     f, b = float(np.median(fwd)), float(np.median(bwd))
+    rec = {"workload": f"{nt}-triangle perturbed sphere, {O.shape[0]} camera rays, loss sum (t - t_target)^2",
+           "fwd_trace_ms": round(f, 4), "bwd_ms": round(b, 4), "fwd_bwd_over_fwd": round((f + b) / f, 3)}
+    if cpu:
+        th = omp_threads()
+        host = [p.cpu().numpy() for p in P]
+        org, dirs = O.cpu().numpy(), D.cpu().numpy()
+        tree = orc.build(host, orc.build_params("reference"))
+        hits = [None]
+        f_ms = wall(lambda: hits.__setitem__(0, orc.trace(tree, host, org, dirs, orc.SHORT_STACK, stack_size=8,
+                                                          threads=th)), 1)
+        g = gt.cpu().numpy()
+        b_ms = wall(lambda: orc.trace_backward(host, org, dirs, hits[0], g, None, None, faces, nv, fp64=True), 1)
+        rec["cpu_baseline"] = {"kind": "port", "cores": th, "fwd_ms": round(f_ms, 1), "bwd_fp64_ms": round(b_ms, 1),
+                               "fwd_bwd_over_fwd": round((f_ms + b_ms) / f_ms, 3),
+                               "sample": "oracle FP32 short-stack forward (OpenMP) + FP64 adjoint (1 thread) "
+                                         "of all rays"}
     ctx.close()
-    return {"workload": f"{nt}-triangle perturbed sphere, {O.shape[0]} rays, loss sum (t - t_target)^2",
-            "fwd_trace_ms": round(f, 4), "bwd_ms": round(b, 4),
-            "fwd_bwd_over_fwd": round((f + b) / f, 3)}
+    return rec
 
 
-def bench_batched(pm, scenes, torch, meshes: int = 256, rays_per_mesh: int = 65536,
-                  frames: int = 4, per_mesh_frames: int = 2, stack: int = 8) -> dict:
+# ------------------------------------------------------------------ C4
+def bench_batched(pm, scenes, torch, orc, meshes: int = 256, rays_per_mesh: int = 65536,
+                  frames: int = 4, per_mesh_frames: int = 2, stack: int = 8, cpu: bool = True) -> dict:
     """Animated meshes rebuilt every frame: ``meshes`` cube spheres of
     log-uniform size (1k-200k triangles), radial wave deformed on the GPU,
     gathered into one triangle SoA, then either ONE forest build + ONE
@@ -199,11 +355,11 @@ def bench_batched(pm, scenes, torch, meshes: int = 256, rays_per_mesh: int = 655
 
     def per_mesh(f):
         stage(f)
-        for m in range(meshes):
-            a, b = int(off[m]), int(off[m + 1])
+        for mm in range(meshes):
+            a, b = int(off[mm]), int(off[mm + 1])
             ctx.build(P[0][a:b], P[1][a:b], P[2][a:b], bp)
-            ctx.trace(pm.SHORT_STACK, O[m * R:(m + 1) * R], D[m * R:(m + 1) * R], tp,
-                      out=H[m * R:(m + 1) * R])
+            ctx.trace(pm.SHORT_STACK, O[mm * R:(mm + 1) * R], D[mm * R:(mm + 1) * R], tp,
+                      out=H[mm * R:(mm + 1) * R])
 
     def run_batched(bp, tp):
         batched(0, bp=bp, tp=tp)
@@ -252,27 +408,100 @@ def bench_batched(pm, scenes, torch, meshes: int = 256, rays_per_mesh: int = 655
         per_mesh(f + 1)
         torch.cuda.synchronize()
         pt.append((time.perf_counter() - t) * 1e3)
-    ctx.close()
     p = float(np.median(pt))
-    return {"workload": f"{meshes} animated cube spheres, {n} tris, {O.shape[0]} rays per frame "
-                        f"(radial wave deform + gather + SAH rebuild + short-stack depth {stack})",
-            "batched_frame_ms": round(b, 3), "per_mesh_frame_ms": round(p, 3),
-            "speedup": round(p / b, 2),
-            "batched_build_ms": round(b_build, 3),
-            "batched_build_roofline": b_roof,
-            "batched_trace_ms": round(b_trace, 3),
-            "batched_trace_mrays": round(O.shape[0] / (b_trace * 1e-3) / 1e6, 1),
-            "hit_fraction": round(hit_frac, 4),
-            "graph": {"launch_ms": round(float(np.median(gt)), 3),
-                      "launch_gpu_ms": round(float(np.median(gk)), 3),
-                      "speedup_vs_per_mesh": round(p / float(np.median(gt)), 2),
-                      "status": graph_status, "upper_levels": caps["upper_levels"],
-                      "lower_levels": caps["lower_levels"]},
-            "autotuned": {"chunk_size": tune["chunk_size"], "lower_size": tune["lower_size"],
-                          "stack_size": tune["stack_size"], "trials": tune["trials"],
-                          "ray_order": ["input", "coherent"][tune["ray_order"]],
-                          "frame_ms": round(tb, 3), "build_ms": round(tb_build, 3),
-                          "trace_ms": round(tb_trace, 3), "speedup_vs_per_mesh": round(p / tb, 2)}}
+    rec = {"workload": f"{meshes} animated cube spheres, {n} tris, {O.shape[0]} rays per frame "
+                       f"(radial wave deform + gather + SAH rebuild + short-stack depth {stack})",
+           "batched_frame_ms": round(b, 3), "per_mesh_frame_ms": round(p, 3),
+           "speedup": round(p / b, 2),
+           "batched_build_ms": round(b_build, 3),
+           "batched_build_roofline": b_roof,
+           "batched_trace_ms": round(b_trace, 3),
+           "batched_trace_mrays": round(O.shape[0] / (b_trace * 1e-3) / 1e6, 1),
+           "hit_fraction": round(hit_frac, 4),
+           "graph": {"launch_ms": round(float(np.median(gt)), 3),
+                     "launch_gpu_ms": round(float(np.median(gk)), 3),
+                     "speedup_vs_per_mesh": round(p / float(np.median(gt)), 2),
+                     "status": graph_status, "upper_levels": caps["upper_levels"],
+                     "lower_levels": caps["lower_levels"]},
+           "autotuned": {"chunk_size": tune["chunk_size"], "lower_size": tune["lower_size"],
+                         "stack_size": tune["stack_size"], "trials": tune["trials"],
+                         "ray_order": ["input", "coherent"][tune["ray_order"]],
+                         "frame_ms": round(tb, 3), "build_ms": round(tb_build, 3),
+                         "trace_ms": round(tb_trace, 3), "speedup_vs_per_mesh": round(p / tb, 2)}}
+    if cpu:
+        th = omp_threads()
+        stage(1)
+        host = [x.cpu().numpy() for x in P]
+        tree = [None]
+        b_ms = wall(lambda: tree.__setitem__(0, orc.build_batched(host, off, orc.build_params("reference"))), 1)
+        rng = np.random.default_rng(4)
+        sel = np.sort(rng.choice(O.shape[0], 524288, replace=False))
+        org, dirs, rm = c["org"][sel], c["dirs"][sel], c["ray_mesh"][sel]
+        t_ms = wall(lambda: orc.trace_batched(tree[0], host, org, dirs, rm, orc.SHORT_STACK, stack_size=stack,
+                                              threads=th), 1)
+        frame = b_ms + t_ms * O.shape[0] / sel.size
+        rec["cpu_baseline"] = {"kind": "port", "cores": th, "forest_build_ms": round(b_ms, 1),
+                               "trace_mrays": round(sel.size / (t_ms * 1e-3) / 1e6, 3),
+                               "frame_ms": round(frame, 1), "gpu_batched_speedup": round(frame / b, 1),
+                               "sample": f"oracle forest build of the whole frame ({n} tris); short stack on "
+                                         f"{sel.size} of the frame's rays, scaled to all {O.shape[0]}"}
+    ctx.close()
+    return rec
+
+
+# ------------------------------------------------------------------ roofline
+def trace_roofline(pmc_path: str, n0: int, ntris: int, stack: int, call_ms: float, alg_bytes: int,
+                   counts: dict) -> dict:
+    """The emission-batch trace call is bound by vector-instruction issue: the
+    fraction is the VALU issue rate, SQ_INSTS_VALU x 4 cycles per wave64
+    instruction over (SIMDs x the call's cycles at the peak clock).  The
+    instruction count and the HBM traffic per call come from the rocprofv3
+    counter runs summarised in profiles/ (used only while the tracer's
+    sources hash to the value stamped there, and for the same workload); the
+    call's duration is measured here."""
+    rec = {"bound": "valu", "kernel": "trace_coop_kernel<short stack> emission call (ray sort + trace; "
+                                      "HIP events bracket the call)",
+           "launch_ms": round(call_ms, 4), "algorithmic_bytes": int(alg_bytes),
+           "algorithmic_gbs": round(alg_bytes / (call_ms * 1e-3) / 1e9, 1),
+           "ray_counts": counts,
+           "peak": round(SIMDS * LANES_PER_SIMD * CLOCK_GHZ, 1), "unit": "G lane-instructions/s",
+           "achieved": None, "frac": None, "traffic": None, "hbm_frac": None}
+    sha = source_sha()
+    if not os.path.exists(pmc_path):
+        rec["pmc"] = "missing"
+        return rec
+    pmc = json.load(open(pmc_path))
+    ok = (pmc.get("source_sha") == sha and pmc.get("rays") == n0 and pmc.get("tris") == ntris
+          and pmc.get("stack") == stack and pmc.get("ray_order") == "coherent")
+    rec["pmc"] = {"file": os.path.relpath(pmc_path, ROOT), "source_sha": pmc.get("source_sha"),
+                  "current_sha": sha, "used": ok}
+    if not ok:
+        return rec
+    valu = pmc["valu_insts"]
+    cycles = call_ms * 1e-3 * CLOCK_GHZ * 1e9
+    rec["achieved"] = round(valu * 64 / (call_ms * 1e-3) / 1e9, 1)
+    rec["frac"] = round(valu * 4 / (SIMDS * cycles), 4)
+    rec["lane_efficiency"] = pmc.get("lane_efficiency")
+    rec["useful_lane_frac"] = round(rec["frac"] * pmc["lane_efficiency"], 4) if pmc.get("lane_efficiency") else None
+    rec["valu_insts"] = valu
+    rec["traffic"] = pmc["traffic_bytes"]
+    rec["hbm_frac"] = round(pmc["traffic_bytes"] / (call_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    return rec
+
+
+def build_roofline(st: dict, ntris: int, build_ms: float) -> dict:
+    """the build's algorithmic bytes (reference record sizes: 80 B per
+    triangle for the box pass, 136 B per ref read and 32 B per ref written at
+    every upper level, 136 B per final leaf ref for areas and planes, 52 B per
+    prim of every SAH lower node) over the measured build time"""
+    b = (80 * ntris + 136 * st["upper_ref_reads"] + 32 * st["upper_ref_writes"]
+         + 136 * st["num_prim_indices"] + 52 * st["lower_prims"])
+    achieved = b / (build_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "algorithmic_bytes": int(b), "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "counts": {"tris": ntris, "upper_ref_reads": st["upper_ref_reads"],
+                       "upper_ref_writes": st["upper_ref_writes"], "leaf_refs": st["num_prim_indices"],
+                       "lower_prims": st["lower_prims"]}}
 
 
 def main() -> int:
@@ -285,11 +514,12 @@ def main() -> int:
     ap.add_argument("--bounces", type=int, default=3)
     ap.add_argument("--stack", type=int, default=8)
     ap.add_argument("--cpu-sample", type=int, default=1_000_000,
-                    help="rays the CPU baseline traces (0 disables the baseline)")
+                    help="rays the CPU baseline traces (0 disables every CPU baseline)")
     ap.add_argument("--meshes", type=int, default=256,
                     help="meshes of the batched-rebuild comparison (0 skips it)")
-    ap.add_argument("--heightfield", type=int, default=1,
-                    help="1: also time the 2M-triangle heightfield build and its primary + bounce rays")
+    ap.add_argument("--configs", default="c1,c2,c4,c5",
+                    help="the other configurations to measure: any of c1 (Cornell box), c2 (heightfield), "
+                         "c4 (batched meshes), c5 (differentiable intersection), or 'none'")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched-rebuild comparison and print its record (profiling runs)")
     ap.add_argument("--probe", action="store_true",
@@ -308,23 +538,29 @@ def main() -> int:
 
     pm = importlib.import_module("extensions-photonmapping_amd")
     scenes = importlib.import_module("extensions-photonmapping_amd.scenes")
+    cpu_on = args.cpu_sample > 0
+    orc = None
+    if cpu_on:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as orc
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    configs = set() if args.configs == "none" else set(args.configs.split(","))
     if args.batched_only:
-        print(json.dumps({"batched": bench_batched(pm, scenes, torch, meshes=args.meshes)}), flush=True)
+        print(json.dumps({"batched": bench_batched(pm, scenes, torch, orc, meshes=args.meshes, cpu=False)}),
+              flush=True)
         return 0
 
     t0 = time.time()
     tris = scenes.cluster_soup(args.tris)
-    org0, dir0 = scenes.photon_emission(args.photons)
-    log(f"[bench] scene {tris[0].shape[0]} tris, {org0.shape[0]} photons generated in {time.time() - t0:.1f}s")
+    log(f"[bench] scene {tris[0].shape[0]} tris generated in {time.time() - t0:.1f}s")
     P = [torch.from_numpy(t).to(dev) for t in tris]
     ntris = tris[0].shape[0]
-    absorb_from = ntris - 2          # photons reaching the light are absorbed
-    n0 = org0.shape[0]
-    O = [torch.from_numpy(org0).to(dev), torch.empty((n0, 4), dtype=torch.float32, device=dev)]
-    D = [torch.from_numpy(dir0).to(dev), torch.empty((n0, 4), dtype=torch.float32, device=dev)]
-    O0, D0 = O[0].clone(), D[0].clone()
+    LIGHT = [p[ntris - 2:].contiguous() for p in P]   # the two area-light triangles
+    absorb_from = ntris - 2                          # photons reaching the light are absorbed
+    n0 = args.photons
+    O = [torch.empty((n0, 4), dtype=torch.float32, device=dev) for _ in range(2)]
+    D = [torch.empty((n0, 4), dtype=torch.float32, device=dev) for _ in range(2)]
     H = torch.empty((n0, 4), dtype=torch.float32, device=dev)
     src = torch.empty(n0, dtype=torch.int32, device=dev)
     ctx = pm.Context(0)
@@ -333,28 +569,31 @@ def main() -> int:
     # spatial key inside each trace call (included in the times)
     tp = pm.trace_params(stack_size=args.stack, ray_order=pm.RAY_ORDER_COHERENT)
     stream = torch.cuda.current_stream()
+    EMIT_SEED = 3
 
     if args.probe:
         ctx.build(*P, bp)
-        ctx.trace(pm.SHORT_STACK, O0, D0, tp, out=H)
+        ctx.photon_emit(LIGHT, n0, EMIT_SEED, out_org=O[0], out_dir=D[0])
+        ctx.trace(pm.SHORT_STACK, O[0], D[0], tp, out=H)
         src_buf = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
         dst_buf = torch.empty_like(src_buf)
         dst_buf.copy_(src_buf)
         torch.cuda.synchronize()
         print(json.dumps({"probe": True, "copy_bytes": src_buf.numel() * 4, "rays": n0,
-                          "tris": ntris}), flush=True)
+                          "tris": ntris, "source_sha": source_sha()}), flush=True)
         return 0
 
     def step(timing: dict | None, tp=tp):
-        """one frame: rebuild + all photon passes; returns rays traced"""
-        O[0].copy_(O0)
-        D[0].copy_(D0)
-        e = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        """one frame: rebuild + emission + all photon passes; returns rays traced"""
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
         e[0].record(stream)
         tb = time.perf_counter()
         ctx.build(*P, bp)
         e[1].record(stream)
         build_wall = time.perf_counter() - tb
+        e[2].record(stream)
+        ctx.photon_emit(LIGHT, n0, EMIT_SEED, out_org=O[0], out_dir=D[0])
+        e[3].record(stream)
         n, cur, rays = n0, 0, 0
         passes = []
         for b in range(args.bounces + 1):
@@ -376,6 +615,7 @@ def main() -> int:
         if timing is not None:
             torch.cuda.synchronize()
             timing["build_ms"].append(e[0].elapsed_time(e[1]))
+            timing["emit_ms"].append(e[2].elapsed_time(e[3]))
             timing["build_wall_ms"].append(build_wall * 1e3)
             timing["trace_ms"].append(sum(ev[0].elapsed_time(ev[1]) for _, ev in passes))
             timing["pass0_ms"].append(passes[0][1][0].elapsed_time(passes[0][1][1]))
@@ -383,25 +623,26 @@ def main() -> int:
             timing["pass_rays"] = [p[0] for p in passes]
         return rays
 
+    keys = ("build_ms", "emit_ms", "build_wall_ms", "trace_ms", "pass0_ms", "rays")
     for i in range(args.warmup):
         step(None)
         log(f"[bench] warmup {i + 1}/{args.warmup}")
-    timing = {k: [] for k in ("build_ms", "build_wall_ms", "trace_ms", "pass0_ms", "rays")}
+    timing = {k: [] for k in keys}
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for i in range(args.steps):
         step(timing)
-        log(f"[bench] step {i + 1}/{args.steps}: build {timing['build_ms'][-1]:.2f} ms, "
-            f"trace {timing['trace_ms'][-1]:.2f} ms, {timing['rays'][-1]} rays")
+        log(f"[bench] step {i + 1}/{args.steps}: build {timing['build_ms'][-1]:.2f} ms, emit "
+            f"{timing['emit_ms'][-1]:.3f} ms, trace {timing['trace_ms'][-1]:.2f} ms, {timing['rays'][-1]} rays")
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t_start
-    ms_per_step = wall * 1e3 / args.steps
+    wall_s = time.perf_counter() - t_start
+    ms_per_step = wall_s * 1e3 / args.steps
     rays_total = sum(timing["rays"])
     trace_ms_total = sum(timing["trace_ms"])
     mrays = rays_total / (trace_ms_total * 1e-3) / 1e6
     build_ms = float(np.median(timing["build_ms"]))
     # the same passes traced in input order (no ray sort)
-    t_in = {k: [] for k in ("build_ms", "build_wall_ms", "trace_ms", "pass0_ms", "rays")}
+    t_in = {k: [] for k in keys}
     tp_in = pm.trace_params(stack_size=args.stack, ray_order=pm.RAY_ORDER_INPUT)
     step(None, tp_in)
     for i in range(2):
@@ -413,82 +654,66 @@ def main() -> int:
     gctx = pm.Context(0)
     off1 = np.array([0, ntris], np.int32)
     gctx.batch_graph_capture(*P, off1, None, None, None, None, pm.SHORT_STACK, bp)
-    gb = []
-    for i in range(args.steps + 1):
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-        ev[0].record(stream)
-        gctx.batch_graph_launch()
-        ev[1].record(stream)
-        torch.cuda.synchronize()
-        if i:
-            gb.append(ev[0].elapsed_time(ev[1]))
+    build_graph_ms = timed(torch, gctx.batch_graph_launch, args.steps)
     assert gctx.batch_graph_status() == 0
-    build_graph_ms = float(np.median(gb))
     gctx.close()
 
     # ---- roofline of the dominant kernel: short-stack trace of the emission batch
     ctx.build(*P, bp)
+    ctx.photon_emit(LIGHT, n0, EMIT_SEED, out_org=O[0], out_dir=D[0])
+    O0, D0 = O[0].clone(), D[0].clone()
     cnt = torch.zeros((n0, 4), dtype=torch.int32, device=dev)
     ctx.trace(pm.SHORT_STACK, O0, D0, tp, out=H, counters=cnt)
     torch.cuda.synchronize()
     bytes_pass0 = ray_bytes(cnt.to(torch.int64))
-    pass0_ms = float(np.mean(timing["pass0_ms"]))
-    achieved = bytes_pass0 / (pass0_ms * 1e-3) / 1e9
+    counts0 = ray_counts(cnt)
+    pass0_ms = float(np.median(timing["pass0_ms"]))
     hits0 = H.view(torch.int32)[:, 0]
     hit_frac = float((hits0 >= 0).float().mean())
-
-    # HBM bytes per launch of the same kernel on the same workload, from the
-    # calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE runs committed in profiles/
-    traffic = None
-    pmc_path = os.path.join(ROOT, "profiles", "trace_c3_pmc.json")
-    if os.path.exists(pmc_path):
-        pmc = json.load(open(pmc_path))
-        if (pmc.get("rays"), pmc.get("tris"), pmc.get("stack"), pmc.get("ray_order")) == \
-                (n0, ntris, args.stack, "coherent"):
-            traffic = pmc["traffic_bytes"]
+    roof = trace_roofline(os.path.join(ROOT, "profiles", "trace_c3_pmc.json"), n0, ntris, args.stack, pass0_ms,
+                          bytes_pass0, counts0)
 
     # ---- other tracers on the same tree (same GPU): kd-restart, brute force
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    ev[0].record(stream)
-    ctx.trace(pm.KD_RESTART, O0, D0, tp, out=H)
-    ev[1].record(stream)
-    torch.cuda.synchronize()
-    kd_mrays = n0 / (ev[0].elapsed_time(ev[1]) * 1e-3) / 1e6
+    kd_ms = timed(torch, lambda: ctx.trace(pm.KD_RESTART, O0, D0, tp, out=H), 2)
+    kd_mrays = n0 / (kd_ms * 1e-3) / 1e6
     nb = 16384
-    ctx.trace(pm.BRUTE, O0[:nb].contiguous(), D0[:nb].contiguous(), tp, out=H[:nb])
-    ev[0].record(stream)
-    ctx.trace(pm.BRUTE, O0[:nb].contiguous(), D0[:nb].contiguous(), tp, out=H[:nb])
-    ev[1].record(stream)
-    torch.cuda.synchronize()
-    brute_mrays = nb / (ev[0].elapsed_time(ev[1]) * 1e-3) / 1e6
+    ob, db = O0[:nb].contiguous(), D0[:nb].contiguous()
+    brute_ms = timed(torch, lambda: ctx.trace(pm.BRUTE, ob, db, tp, out=H[:nb]), 1)
+    brute_mrays = nb / (brute_ms * 1e-3) / 1e6
     st = ctx.stats()
 
-    fwd_bwd = bench_fwd_bwd(pm, scenes, torch)
-    log(f"[bench] fwd+bwd: {fwd_bwd}")
-    heightfield = bench_heightfield(pm, scenes, torch, stack=args.stack) if args.heightfield else None
-    log(f"[bench] heightfield: {heightfield}")
-    batched = bench_batched(pm, scenes, torch, meshes=args.meshes) if args.meshes > 0 else None
-    log(f"[bench] batched: {batched}")
+    extra = {}
+    if "c1" in configs:
+        extra["cornell"] = bench_cornell(pm, scenes, torch, orc, stack=args.stack, cpu=cpu_on)
+        log(f"[bench] cornell: {extra['cornell']}")
+    if "c5" in configs:
+        extra["fwd_bwd"] = bench_fwd_bwd(pm, scenes, torch, orc, cpu=cpu_on)
+        log(f"[bench] fwd+bwd: {extra['fwd_bwd']}")
+    if "c2" in configs:
+        extra["heightfield"] = bench_heightfield(pm, scenes, torch, orc, stack=args.stack, cpu=cpu_on)
+        log(f"[bench] heightfield: {extra['heightfield']}")
+    if "c4" in configs and args.meshes > 0:
+        extra["batched"] = bench_batched(pm, scenes, torch, orc, meshes=args.meshes, cpu=cpu_on)
+        log(f"[bench] batched: {extra['batched']}")
 
     # ---- CPU baseline: the oracle's build and trace on the host cores
     cpu = None
-    if args.cpu_sample > 0:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle as orc
-        threads = min(os.cpu_count() or 1, 16)
-        os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    tree_oracle = None
+    if cpu_on:
+        threads = omp_threads()
         m = min(args.cpu_sample, n0)
+        org0, dir0 = O0[:m].cpu().numpy(), D0[:m].cpu().numpy()
+        tree = [None]
+        cpu_build_ms = wall(lambda: tree.__setitem__(0, orc.build(tris, orc.build_params("reference"))), 1)
+        tree_oracle = tree_stats(tree[0].stats())
         t0 = time.perf_counter()
-        tree = orc.build(tris, orc.build_params("reference"))
-        cpu_build_ms = (time.perf_counter() - t0) * 1e3
-        t0 = time.perf_counter()
-        orc.trace(tree, tris, org0[:m], dir0[:m], orc.SHORT_STACK, stack_size=args.stack, threads=threads)
+        orc.trace(tree[0], tris, org0, dir0, orc.SHORT_STACK, stack_size=args.stack, threads=threads)
         cpu_trace_s = time.perf_counter() - t0
         cpu = {"value": round(m / cpu_trace_s / 1e6, 4), "unit": "Mrays/s", "cores": threads,
                "kind": "port", "build_ms": round(cpu_build_ms, 1),
                "sample": f"oracle SAH kd-tree build of the full {ntris}-triangle scene + short-stack "
                          f"(depth {args.stack}) trace of the first {m} emission photon rays, "
-                         f"OpenMP {threads} threads"}
+                         f"OpenMP {threads} threads (the cores this process may use)"}
         log(f"[bench] cpu baseline: build {cpu_build_ms:.0f} ms, {cpu['value']} Mrays/s")
 
     line = {
@@ -506,14 +731,15 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": f"photon tracing on a clustered triangle soup: {ntris} tris (64 Gaussian "
-                        f"clusters + 2-tri light), {n0} photons, <= {args.bounces} diffuse bounces, "
-                        f"SAH kd-tree rebuilt every step, short-stack depth {args.stack}",
+                        f"clusters + 2-tri light), {n0} photons emitted on the GPU, <= {args.bounces} diffuse "
+                        f"bounces, SAH kd-tree rebuilt every step, short-stack depth {args.stack}",
             "tris": ntris,
             "rays_per_step": int(np.mean(timing["rays"])),
             "rays_per_pass": timing["pass_rays"],
             "kd_build_ms": round(build_ms, 3),
             "kd_build_wall_ms": round(float(np.median(timing["build_wall_ms"])), 3),
             "kd_build_graph_ms": round(build_graph_ms, 3),
+            "emit_ms": round(float(np.median(timing["emit_ms"])), 4),
             "trace_ms_per_step": round(trace_ms_total / args.steps, 3),
             "ray_order": "coherent (spatial-key radix sort inside each trace call, included in the times)",
             "input_order_mrays": round(input_mrays, 2),
@@ -522,44 +748,18 @@ def main() -> int:
             "brute_force_mrays": round(brute_mrays, 4),
             "short_stack_vs_brute": round(mrays / brute_mrays, 1),
             "emission_hit_fraction": round(hit_frac, 4),
-            "fwd_bwd": fwd_bwd,
-            "batched": batched,
-            "heightfield": heightfield,
-            "tree": {k: st[k] for k in ("num_nodes", "leaves", "empty_leaves", "max_depth",
-                                        "leaf_refs", "upper_levels", "lower_levels")},
+            "tree": tree_stats(st),
+            "tree_oracle": tree_oracle,
+            "tree_equal_oracle": (tree_oracle == tree_stats(st)) if tree_oracle else None,
             "build_roofline": build_roofline(st, ntris, build_ms),
+            **extra,
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "trace_coop_kernel<short stack> (emission batch; events bracket the trace call, ray sort included)",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "algorithmic_bytes": bytes_pass0,
-            "launch_ms": round(pass0_ms, 4),
-        },
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     ctx.close()
     return 0
-
-
-def build_roofline(st: dict, ntris: int, build_ms: float) -> dict:
-    """the build's algorithmic bytes (reference record sizes: 80 B per
-    triangle for the box pass, 136 B per ref read and 32 B per ref written at
-    every upper level, 136 B per final leaf ref for areas and planes, 52 B per
-    prim of every SAH lower node) over the measured build time"""
-    b = (80 * ntris + 136 * st["upper_ref_reads"] + 32 * st["upper_ref_writes"]
-         + 136 * st["num_prim_indices"] + 52 * st["lower_prims"])
-    achieved = b / (build_ms * 1e-3) / 1e9
-    return {"bound": "hbm", "algorithmic_bytes": int(b), "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "counts": {"tris": ntris, "upper_ref_reads": st["upper_ref_reads"],
-                       "upper_ref_writes": st["upper_ref_writes"], "leaf_refs": st["num_prim_indices"],
-                       "lower_prims": st["lower_prims"]}}
 
 
 if __name__ == "__main__":

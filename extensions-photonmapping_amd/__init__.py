@@ -28,6 +28,7 @@ SPLIT_BOX, SPLIT_DIVIDE, SPLIT_SPLIT = 0, 1, 2
 BRUTE, KD_RESTART, SHORT_STACK = 0, 1, 2
 MOELLER, WOOP = 0, 1
 RAY_ORDER_INPUT, RAY_ORDER_COHERENT = 0, 1
+MAX_LIGHTS = 65536      # PM_MAX_LIGHTS: light triangles pm_photon_emit takes
 
 # every symbol include/pmkd.h declares
 EXPORTS = ("pm_build_params_default", "pm_trace_params_default", "pm_create", "pm_destroy",
@@ -35,7 +36,7 @@ EXPORTS = ("pm_build_params_default", "pm_trace_params_default", "pm_create", "p
            "pm_tree_stats", "pm_trace", "pm_photon_bounce", "pm_add_mesh", "pm_trace_backward",
            "pm_build_batched", "pm_trace_batched", "pm_osc_surface", "pm_deform_radial",
            "pm_autotune", "pm_batch_graph_capture", "pm_batch_graph_launch", "pm_batch_graph_status",
-           "pm_trace_canvas", "pm_add_mesh_attributes")
+           "pm_trace_canvas", "pm_add_mesh_attributes", "pm_create_rays", "pm_photon_emit")
 
 
 class PmError(RuntimeError):
@@ -140,11 +141,14 @@ def load_library(path: str = LIB_PATH):
     L.pm_batch_graph_status.argtypes = [P, C.POINTER(C.c_int), P]
     L.pm_trace_canvas.argtypes = [P, I, P, P, I, P, P, P, P, C.POINTER(Light), I, P, C.POINTER(TraceParams), P]
     L.pm_add_mesh_attributes.argtypes = [P, P, P, I, P, I, P, P, P, P, P, P, P, I, P]
+    L.pm_create_rays.argtypes = [P, P, P, I, I, C.c_ulonglong, I, P, P, P]
+    L.pm_photon_emit.argtypes = [P, P, P, P, I, I, C.c_ulonglong, F, P, P, P]
     for n in ("pm_create", "pm_destroy", "pm_build", "pm_set_geometry", "pm_tree_view",
               "pm_set_tree", "pm_tree_stats", "pm_trace", "pm_photon_bounce", "pm_add_mesh",
               "pm_trace_backward", "pm_build_batched", "pm_trace_batched", "pm_osc_surface",
               "pm_deform_radial", "pm_autotune", "pm_batch_graph_capture", "pm_batch_graph_launch",
-              "pm_batch_graph_status", "pm_trace_canvas", "pm_add_mesh_attributes"):
+              "pm_batch_graph_status", "pm_trace_canvas", "pm_add_mesh_attributes", "pm_create_rays",
+              "pm_photon_emit"):
         getattr(L, n).restype = C.c_int
     _lib = L
     return L
@@ -499,6 +503,40 @@ class Context:
             _dev_ptr(faces) if faces is not None else None, num_vertices, _dev_ptr(grad_vertices),
             _dev_ptr(grad_org), _dev_ptr(grad_dir), _stream_ptr(stream)))
         return grad_vertices[:num_vertices], grad_org[:n], grad_dir[:n]
+
+    def create_rays(self, inv_view_proj, cam_pos, width: int, height: int, jitter_seed: int | None = None,
+                    org=None, dirs=None, stream=None):
+        """camera rays (pm_create_rays): one per pixel, id = y * width + x;
+        inv_view_proj = 16 floats in Unproject's element order; jitter_seed
+        None = no jitter (the reference), else a seeded sub-pixel offset.
+        Returns (org, dirs) device float32 [width * height, 4]."""
+        torch = _torch()
+        n = width * height
+        if org is None:
+            org = torch.empty((max(n, 1), 4), dtype=torch.float32, device=f"cuda:{self.device}")
+        if dirs is None:
+            dirs = torch.empty_like(org)
+        m = np.ascontiguousarray(inv_view_proj, dtype=np.float32).reshape(16)
+        c = np.ascontiguousarray(cam_pos, dtype=np.float32).reshape(3)
+        self._check(self._L.pm_create_rays(self.h, m.ctypes.data, c.ctypes.data, width, height,
+                                           (jitter_seed or 0) & (2**64 - 1), int(jitter_seed is not None),
+                                           _dev_ptr(org), _dev_ptr(dirs), _stream_ptr(stream)))
+        return org[:n], dirs[:n]
+
+    def photon_emit(self, lights, n: int, seed: int, offset: float = 1e-4, out_org=None, out_dir=None,
+                    stream=None):
+        """n photons from the light triangles ``lights`` = (l0, l1, l2) device
+        float32 [L, 4] (pm_photon_emit); returns (org, dirs) [n, 4]"""
+        torch = _torch()
+        l0, l1, l2 = lights
+        if out_org is None:
+            out_org = torch.empty((max(n, 1), 4), dtype=torch.float32, device=l0.device)
+        if out_dir is None:
+            out_dir = torch.empty_like(out_org)
+        self._check(self._L.pm_photon_emit(self.h, _dev_ptr(l0), _dev_ptr(l1), _dev_ptr(l2), l0.shape[0], n,
+                                           seed & (2**64 - 1), offset, _dev_ptr(out_org), _dev_ptr(out_dir),
+                                           _stream_ptr(stream)))
+        return out_org[:n], out_dir[:n]
 
     def photon_bounce(self, org, dirs, hits, absorb_from: int, seed: int, offset: float = 1e-4,
                       out_org=None, out_dir=None, out_src=None, stream=None):

@@ -492,6 +492,32 @@ int pm_deform_radial(pm_ctx *ctx, const float *unit, int nv, float frame, const 
     });
 }
 
+int pm_create_rays(pm_ctx *ctx, const float *inv_view_proj, const float *cam_pos, int width, int height,
+                   unsigned long long jitter_seed, int jitter, float *org, float *dir, void *stream) {
+    return guarded(ctx, [&] {
+        if (width < 0 || height < 0 || !inv_view_proj || !cam_pos)
+            throw Error{PM_ERR_ARG, "invalid camera"};
+        const long long n = (long long)width * height;
+        if (n > INT32_MAX) throw Error{PM_ERR_ARG, "width * height exceeds 2^31 - 1 rays"};
+        if (n > 0 && (!org || !dir)) throw Error{PM_ERR_ARG, "invalid ray arrays"};
+        pmkd::create_rays(inv_view_proj, cam_pos, width, height, jitter_seed, jitter != 0,
+                          reinterpret_cast<float4 *>(org), reinterpret_cast<float4 *>(dir), (hipStream_t)stream);
+    });
+}
+
+int pm_photon_emit(pm_ctx *ctx, const float *l0, const float *l1, const float *l2, int num_lights, int n,
+                   unsigned long long seed, float offset, float *out_org, float *out_dir, void *stream) {
+    return guarded(ctx, [&] {
+        if (n < 0) throw Error{PM_ERR_ARG, "invalid photon count"};
+        if (n > 0 && (num_lights < 1 || num_lights > PM_MAX_LIGHTS || !l0 || !l1 || !l2 || !out_org || !out_dir))
+            throw Error{PM_ERR_ARG, "need 1.." + std::to_string(PM_MAX_LIGHTS) + " light triangles and outputs"};
+        pmkd::photon_emit(ctx, reinterpret_cast<const float4 *>(l0), reinterpret_cast<const float4 *>(l1),
+                          reinterpret_cast<const float4 *>(l2), num_lights, n, seed, offset,
+                          reinterpret_cast<float4 *>(out_org), reinterpret_cast<float4 *>(out_dir),
+                          (hipStream_t)stream);
+    });
+}
+
 int pm_photon_bounce(pm_ctx *ctx, const float *org, const float *dir, const pm_hit *hits, int n,
                      int absorb_from, unsigned long long seed, float offset, float *out_org,
                      float *out_dir, int *out_src, int *out_count, int *n_out, void *stream) {

@@ -55,10 +55,6 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
 // n_host[q], the capacity, and blocks past n_q write zero block sums), else
 // n_host[q].  in may equal out (each thread reads its elements before it
 // writes them).
-//
-// nodes_plan: the node pass's scan (K = 2, empty-space node and segment
-// counts of the level) applies plan_nodes (below) with its two totals, from
-// the thread that writes them.
 template <int K>
 struct ScanIO {
     const int *in[K];
@@ -66,7 +62,6 @@ struct ScanIO {
     int n_host[K];
     const int *n_dev[K];
     int n_mult[K];
-    Lvl *nodes_plan;
 };
 
 __device__ void plan_nodes(Lvl *L, int E, int S);
@@ -143,17 +138,14 @@ __global__ __launch_bounds__(SCAN_T) void scan_block_kernel(ScanIO<K> io, int *_
         }
     }
     __syncthreads();
-    int tot[K];
 #pragma unroll
     for (int q = 0; q < K; ++q) {
         int run = (w ? wsum[q][w - 1] : 0) + incl[q] - tsum[q];
         if (PRE) run += wpre[q][0];
-        tot[q] = 0;
         int o[SCAN_E];
 #pragma unroll
         for (int j = 0; j < SCAN_E; ++j) {
             o[j] = run;
-            if (base + j == n[q]) tot[q] = run;
             run += v[q][j];
         }
         if (base + SCAN_E <= n[q] + 1 && aligned16(io.out[q])) {   // one 16-byte store per thread
@@ -166,9 +158,6 @@ __global__ __launch_bounds__(SCAN_T) void scan_block_kernel(ScanIO<K> io, int *_
         if (block_sums && threadIdx.x == SCAN_T - 1)
             block_sums[q * sums_stride + blockIdx.x] = wsum[q][SCAN_T / 64 - 1];
     }
-    // final totals are written here unless block sums are still to be added
-    if (K == 2 && io.nodes_plan && !block_sums && base <= n[0] && n[0] < base + SCAN_E)
-        plan_nodes(io.nodes_plan, tot[0], tot[K - 1]);
 }
 
 // per-block sums only (the first pass of a PRE scan); blocks past n_q write 0
@@ -208,20 +197,15 @@ template <int K>
 __global__ __launch_bounds__(SCAN_T) void scan_add_kernel(ScanIO<K> io, const int *__restrict__ offs,
                                                           int offs_stride) {
     const long long base = (long long)blockIdx.x * SCAN_BLK;
-    int tot[K];
-    bool last = false;
 #pragma unroll
     for (int q = 0; q < K; ++q) {
         const int n = scan_len(io, q);
         const int add = offs[q * offs_stride + blockIdx.x];
-        tot[q] = 0;
         for (int j = threadIdx.x; j < SCAN_BLK; j += SCAN_T) {
             const long long k = base + j;
             if (k <= n) io.out[q][k] += add;
-            if (k == n) { tot[q] = io.out[q][k]; last = true; }
         }
     }
-    if (K == 2 && io.nodes_plan && last) plan_nodes(io.nodes_plan, tot[0], tot[K - 1]);
 }
 
 // ------------------------------------------------------------------ helpers
@@ -396,60 +380,12 @@ __global__ __launch_bounds__(BS) void tri_box_kernel(const float4 *__restrict__ 
 __device__ __forceinline__ int wave_index() { return (blockIdx.x * blockDim.x + threadIdx.x) >> 6; }
 __device__ __forceinline__ int wave_count() { return (gridDim.x * blockDim.x) >> 6; }
 
-// ReduceAabb result -> node box, CalcEmptySpaceSplits
-// (EmptySpaceSplitting.h:10-70), CalcUpperNodeSplitInfo
-// (ReduceSegments.h:466-484), segments per node.
-__global__ __launch_bounds__(BS) void node_prepare_kernel(NodePtrs N, const Lvl *__restrict__ L, int seg,
-                                                          const int *__restrict__ keys,
-                                                          int empty_space, float thr,
-                                                          int *__restrict__ planes_out,
-                                                          int *__restrict__ req,
-                                                          int *__restrict__ nseg) {
-    // the previous level's advance is applied by this level's node scan; until
-    // then the level's nodes follow from the stored children plan
-    int A = L->A, active_index = L->active_index;
-    if (L->advance && !L->status && A > 0) {
-        active_index = L->nodes_size + 2 * A - L->created;
-        A = L->created;
-    }
-    GRID_STRIDE(i, A) {
-        const int node = active_index + i;
-        const f3 rMin = {unkey(keys[6 * i + 0]), unkey(keys[6 * i + 1]), unkey(keys[6 * i + 2])};
-        const f3 rMax = {unkey(keys[6 * i + 3]), unkey(keys[6 * i + 4]), unkey(keys[6 * i + 5])};
-        int pl = 0;
-        if (empty_space && active_index != 0) {
-            const f3 pMin = mk3(N.bmin[node]), pMax = mk3(N.bmax[node]);
-            if (rMin.x - pMin.x > thr * (pMax.x - pMin.x)) pl |= 1;
-            if (rMin.y - pMin.y > thr * (pMax.y - pMin.y)) pl |= 2;
-            if (rMin.z - pMin.z > thr * (pMax.z - pMin.z)) pl |= 4;
-            if (pMax.x - rMax.x > thr * (pMax.x - pMin.x)) pl |= 8;
-            if (pMax.y - rMax.y > thr * (pMax.y - pMin.y)) pl |= 16;
-            if (pMax.z - rMax.z > thr * (pMax.z - pMin.z)) pl |= 32;
-        }
-        planes_out[i] = pl;
-        req[i] = 2 * __popc(pl);
-        N.bmin[node] = make_float4(rMin.x, rMin.y, rMin.z, 0.0f);
-        N.bmax[node] = make_float4(rMax.x, rMax.y, rMax.z, 0.0f);
-        const f3 sz = sub3(rMax, rMin);
-        const f3 med = {sz.x * 0.5f + rMin.x, sz.y * 0.5f + rMin.y, sz.z * 0.5f + rMin.z};
-        const bool yAboveX = sz.x < sz.y;
-        const float mxs = yAboveX ? sz.y : sz.x;
-        const float spl = yAboveX ? med.y : med.x;
-        const int ax = yAboveX ? AX_Y : AX_X;
-        const bool zHigher = mxs < sz.z;
-        N.split[node] = zHigher ? med.z : spl;
-        N.info[node] = (int8_t)(zHigher ? AX_Z : ax);
-        const int cnt = (int)N.bitmap[node];
-        nseg[i] = (cnt + seg - 1) / seg;
-    }
-}
-
 // EmptySpaceSplitting2 (EmptySpaceSplitting.h:74-184): a chain of split
 // nodes with empty leaves between the node and its parent.
-__device__ __forceinline__ void empty_chain_node(const NodePtrs &N, int i, int A, int active_index, int pl,
-                                                 const int *__restrict__ eaddr) {
+__device__ __forceinline__ void empty_chain_node_at(const NodePtrs &N, int i, int A, int active_index, int pl,
+                                                    int eaddr_i) {
     const int leafID = active_index + i;
-    const int start = active_index + A + eaddr[i];
+    const int start = active_index + A + eaddr_i;
     const int cnt = 2 * __popc(pl);
     for (int j = 0; j < cnt; ++j) zero_node(N, start + j);
     const int parentID = N.parent[leafID];
@@ -480,39 +416,6 @@ __device__ __forceinline__ void empty_chain_node(const NodePtrs &N, int i, int A
     }
 }
 
-// Two independent passes in one launch: the empty-space chains of the
-// active nodes, and the segment table (Segment, TriangleMapUpperCreator.cu:
-// 207-239), one thread per segment.  The chains touch only new nodes and the
-// parents' child links; the table reads the active nodes' ref ranges.
-__global__ __launch_bounds__(BS) void chain_segs_kernel(const NodePtrs N, const Lvl *__restrict__ L,
-                                                        const int *__restrict__ planes_in,
-                                                        const int *__restrict__ eaddr, int seg,
-                                                        const int *__restrict__ segbase,
-                                                        int *__restrict__ seg_node,
-                                                        int *__restrict__ seg_start,
-                                                        int *__restrict__ seg_count) {
-    const int A = L->A, S = L->S, active_index = L->active_index;
-    const int Ac = L->E > 0 ? A : 0;
-    GRID_STRIDE(i, Ac) {
-        const int pl = planes_in[i];
-        if (pl) empty_chain_node(N, i, A, active_index, pl, eaddr);
-    }
-    // a segment's node is the last i with segbase[i] <= s (segbase is the
-    // exclusive scan of the nodes' segment counts)
-    GRID_STRIDE(s, S) {
-        int lo = 0, hi = A - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (segbase[mid] <= s) lo = mid; else hi = mid - 1;
-        }
-        const int node = active_index + lo;
-        const int off = (s - segbase[lo]) * seg;
-        seg_node[s] = lo;
-        seg_start[s] = N.prim_index[node] + off;
-        seg_count[s] = min(seg, (int)N.bitmap[node] - off);
-    }
-}
-
 // The DIVIDE split method (SetDivideSide<false>, TriangleUpperChildren.h:
 // 64-142): a ref that straddles the plane goes to the child boxes its
 // triangle meets (triangle/box step 3 against the node box cut at the
@@ -531,238 +434,566 @@ __device__ __forceinline__ void divide_sides(const DivideGeom &g, float id_bits,
     r = tri_box_step3(v0, v1, v2, rmin, nmx);
 }
 
-// SetSplitSide (TriangleUpperChildren.h:10-62) counted per segment
-template <int SEGT, bool DIV>
-__global__ __launch_bounds__(WPB * 64) void classify_kernel(
-    const NodePtrs N, const Lvl *__restrict__ L, const int *__restrict__ seg_node,
-    const int *__restrict__ seg_start, const int *__restrict__ seg_count,
-    const float4 *__restrict__ rmin, const float4 *__restrict__ rmax, int *__restrict__ segL,
-    int *__restrict__ segR, DivideGeom dg) {
-    const int S = L->S, active_index = L->active_index;
-    const int lane = threadIdx.x & 63;
-    for (int s = wave_index(); s < S; s += wave_count()) {
-        const int node = active_index + seg_node[s];
-        // one chunk per wave: its node's axis and plane are wave-uniform
-        const int ax = __builtin_amdgcn_readfirstlane(N.info[node]);
-        const float sp = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(N.split[node])));
-        const int start = seg_start[s], count = seg_count[s];
-        int cl = 0, cr = 0;
-        if (!DIV) {
-            // the split axis' coordinates only, loaded without branches (lanes
-            // past the chunk re-read its first ref) so that the chunk's loads
-            // all issue before the first compare
-            const float *lo = reinterpret_cast<const float *>(rmin + start) + (ax - AX_X);
-            const float *hi = reinterpret_cast<const float *>(rmax + start) + (ax - AX_X);
-#pragma unroll 16
-            for (int it = 0; it < SEGT / 64; ++it) {
-                const int j = it * 64 + lane;
-                const int jj = j < count ? j : 0;
-                const float a = lo[4 * jj], b = hi[4 * jj];
-                cl += __popcll(__ballot(j < count && a <= sp));
-                cr += __popcll(__ballot(j < count && sp < b));
-            }
-        } else {
-            const f3 nmn = mk3(N.bmin[node]), nmx = mk3(N.bmax[node]);
-#pragma unroll 16
-            for (int it = 0; it < SEGT / 64; ++it) {
-                const int j = it * 64 + lane;
-                const bool valid = j < count;
-                bool l = false, r = false;
-                if (valid) {
-                    const float4 mn = rmin[start + j];
-                    l = comp4(mn, ax) <= sp;
-                    r = sp < comp4(rmax[start + j], ax);
-                    if (l && r) divide_sides(dg, mn.w, ax, sp, nmn, nmx, l, r);
-                }
-                cl += __popcll(__ballot(l));
-                cr += __popcll(__ballot(r));
-            }
-        }
-        if (lane == 0) { segL[s] = cl; segR[s] = cr; }
-    }
-}
-
-// CalcNodeChildSize (TriangleUpperChildren.h:144-162) + MarkNodeLeafs
-// (:190-201) per node, and SetPrimitiveLeafSide (:164-186) counted per
-// segment, in one launch: a segment derives its node's leaf flags from the
-// same scanned counts the node pass reads.
-__device__ __forceinline__ void child_counts(int i, const int *__restrict__ segbase,
-                                             const int *__restrict__ segLoff, const int *__restrict__ segRoff,
-                                             int &nl, int &nr) {
-    const int b = segbase[i], e = segbase[i + 1];
-    nl = segLoff[e] - segLoff[b];
-    nr = segRoff[e] - segRoff[b];
-}
-
-__global__ __launch_bounds__(BS) void leaf_flags_kernel(const Lvl *__restrict__ L, int LS,
-                                                        const int *__restrict__ segbase,
-                                                        const int *__restrict__ segLoff,
-                                                        const int *__restrict__ segRoff,
-                                                        const int *__restrict__ seg_node,
-                                                        const int *__restrict__ segL,
-                                                        const int *__restrict__ segR,
-                                                        int *__restrict__ slot_leaf,
-                                                        int *__restrict__ segLL,
-                                                        int *__restrict__ segLR) {
-    const int A = L->A, S = L->S;
-    GRID_STRIDE(i, A) {
-        int nl, nr;
-        child_counts(i, segbase, segLoff, segRoff, nl, nr);
-        slot_leaf[i] = nl < LS;
-        slot_leaf[A + i] = nr < LS;
-    }
-    GRID_STRIDE(s, S) {
-        int nl, nr;
-        child_counts(seg_node[s], segbase, segLoff, segRoff, nl, nr);
-        segLL[s] = nl < LS ? segL[s] : 0;
-        segLR[s] = nr < LS ? segR[s] : 0;
-    }
-}
-
-// CreateUpperChildren (TriangleUpperChildren.h:203-281), MarkLeafNodes
-// (:385-395), PropagateAabbToChildren (EmptySpaceSplitting.h:299-331).
-// The children layout and the next level's sizes from the level's scanned
-// counts (the host arithmetic of TriangleMapUpperCreator.cu:140-180).  Every
-// block of children_kernel works it out; block 0 stores it for the scatter
-// and the advance.  It reads only fields it does not write.
+// The children plan of a level (the host arithmetic of
+// TriangleMapUpperCreator.cu:140-180), worked out by the leaf pass.
 struct ChildPlan {
     int status;
     int L_all, R_all, LL_all, LR_all, leaf_nodes, leafTri, nextR, created, child_start, next_active;
 };
 
-__device__ __forceinline__ ChildPlan plan_children(const Lvl *L, const int *__restrict__ segLoff,
-                                                   const int *__restrict__ segRoff,
-                                                   const int *__restrict__ segLLoff,
-                                                   const int *__restrict__ segLRoff, const int *__restrict__ lna) {
+// ================================================================== fused levels
+// An upper level in four launches.  The level's prefix sums are computed
+// inside the passes that produce the counts, as single-pass scans: tiles are
+// taken in ticket order (an atomic counter), so all of a tile's predecessors
+// have started before it; a tile publishes its aggregate as soon as it is
+// known and its inclusive prefix once its exclusive prefix is, and finds its
+// exclusive prefix by summing its predecessors' aggregates back to the
+// nearest inclusive prefix (decoupled look-back, Merrill & Garland 2016).
+// The tile holding the last ticket knows the level's totals and applies the
+// plan that the host arithmetic of TriangleMapUpperCreator.cu:140-180 makes.
+//   node_pass      node boxes, empty-space planes and split planes
+//                  (CalcEmptySpaceSplits, CalcUpperNodeSplitInfo), scans of
+//                  the empty-space node and segment counts, the empty-space
+//                  chains (EmptySpaceSplitting2) and the segment table
+//                  (Segment, TriangleMapUpperCreator.cu:207-239);
+//   classify_pass  side counts per segment (SetSplitSide), one wave per
+//                  segment;
+//   leaf_pass      the side counts' scans (over the segments of a tile of
+//                  nodes, which are contiguous), children sizes and leaf
+//                  flags per node (CalcNodeChildSize, MarkNodeLeafs), scans
+//                  of leaf slots and leaf refs, the next level's box keys
+//                  reset; the children plan;
+//   scatter_pass   the clipped refs (SplitTriangles) and the child records
+//                  (CreateUpperChildren, MarkLeafNodes, PropagateAabbToChildren).
+// Every scan is over integers, so the offsets (and the tree) are those of
+// any other order of summation.
+constexpr int NT = 4 * BS;             // active nodes per node/leaf tile (4 per thread)
+// A status word is flag << 62 | value: flag 1 = aggregate, 2 = inclusive prefix.
+constexpr unsigned long long LB_AGG = 1ull << 62, LB_INC = 2ull << 62, LB_VAL = LB_AGG - 1;
+// A wait for a predecessor is bounded: a tile that never publishes (only
+// possible through a bug) fails the build instead of hanging it.
+constexpr int LB_SPIN = 1 << 22;
+
+__device__ __forceinline__ unsigned long long lb_load(const unsigned long long *w) {
+    return __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void lb_store(unsigned long long *w, unsigned long long v) {
+    __hip_atomic_store(w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ long long wave_sum64(long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// One wave: the exclusive prefixes of K counts for tile `tile`, whose
+// aggregates are agg[k]; the status word of (tile, k) is w[K * tile + k].
+template <int K>
+__device__ void lookback(unsigned long long *__restrict__ w, int tile, const long long *agg, long long *excl,
+                         int *status) {
+    const int lane = __lane_id();
+    if (tile == 0) {
+        if (lane == 0)
+            for (int k = 0; k < K; ++k) lb_store(&w[k], LB_INC | (unsigned long long)agg[k]);
+        for (int k = 0; k < K; ++k) excl[k] = 0;
+        return;
+    }
+    if (lane == 0)
+        for (int k = 0; k < K; ++k) lb_store(&w[(long long)K * tile + k], LB_AGG | (unsigned long long)agg[k]);
+    for (int k = 0; k < K; ++k) {
+        long long sum = 0;
+        int base = tile - 1, spins = 0;
+        for (;;) {
+            const int idx = base - lane;
+            const unsigned long long v = idx >= 0 ? lb_load(&w[(long long)K * idx + k]) : LB_INC;
+            const unsigned long long inc = __ballot((v >> 62) == 2ull);
+            const unsigned long long none = __ballot((v >> 62) == 0ull);
+            const int first = inc ? __ffsll((long long)inc) - 1 : 64;   // nearest inclusive prefix
+            const unsigned long long upto = first >= 63 ? ~0ull : ((2ull << first) - 1ull);
+            if (none & upto) {                     // a predecessor has not published yet
+                if (++spins > LB_SPIN) {
+                    if (lane == 0) atomicCAS(status, 0, PM_ERR_INTERNAL);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+                continue;
+            }
+            sum += wave_sum64(lane <= first ? (long long)(v & LB_VAL) : 0ll);
+            if (first < 64) break;
+            base -= 64;
+        }
+        excl[k] = sum;
+    }
+    if (lane == 0)
+        for (int k = 0; k < K; ++k)
+            lb_store(&w[(long long)K * tile + k], LB_INC | (unsigned long long)(excl[k] + agg[k]));
+}
+
+// the block's next tile (ticket order); every thread returns the same value
+__device__ __forceinline__ int next_tile(int *ctr, int *s_tile) {
+    if (threadIdx.x == 0) *s_tile = atomicAdd(ctr, 1);
+    __syncthreads();
+    const int t = *s_tile;
+    __syncthreads();
+    return t;
+}
+
+// block-wide exclusive scan of K per-thread counts (BS threads): pre[k] gets
+// the sum over lower threads, tot[k] the block's total
+template <int K>
+__device__ __forceinline__ void block_scan(const int *v, int *pre, int *tot, int (*s_w)[BS / 64]) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int inc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        inc[k] = wave_incl_scan(v[k]);
+        if (lane == 63) s_w[k][wv] = inc[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        pre[k] = inc[k] - v[k];
+        tot[k] = 0;
+        for (int j = 0; j < BS / 64; ++j) {
+            if (j < wv) pre[k] += s_w[k][j];
+            tot[k] += s_w[k][j];
+        }
+    }
+    __syncthreads();
+}
+
+// the device buffers the four level passes share
+struct LevelBufs {
+    int *eaddr, *segbase;                       // per active node; A + 1 entries
+    int *seg_node, *seg_start, *seg_count;      // per segment
+    int *segL, *segR;                           // per segment: refs on each side (classify pass)
+    int *segLoff, *segRoff;                     // per segment: refs on each side before it, S + 1 entries
+    int2 *lna, *lref;                           // per active node: leaf slots / leaf refs before it (left, right)
+    unsigned long long *lb1, *lb3;              // look-back words of the node and leaf passes
+    int *ctr;                                   // their tickets (0: node pass, 2: leaf pass)
+};
+
+// the level's active nodes as the node pass sees them (the previous level's
+// advance is applied by that pass's last tile)
+__device__ __forceinline__ void level_nodes(const Lvl *L, int &A, int &active_index) {
+    A = L->A;
+    active_index = L->active_index;
+    if (L->status) { A = 0; return; }
+    if (L->advance && A > 0) {
+        active_index = L->nodes_size + 2 * A - L->created;
+        A = L->created;
+    }
+}
+
+// ReduceAabb result -> node box, CalcEmptySpaceSplits (EmptySpaceSplitting.h:
+// 10-70), CalcUpperNodeSplitInfo (ReduceSegments.h:466-484) for active node i
+__device__ __forceinline__ void prepare_node(const NodePtrs &N, int i, int node, const int *__restrict__ keys,
+                                             bool empty_space, float thr, int &pl) {
+    const f3 rMin = {unkey(keys[6 * i + 0]), unkey(keys[6 * i + 1]), unkey(keys[6 * i + 2])};
+    const f3 rMax = {unkey(keys[6 * i + 3]), unkey(keys[6 * i + 4]), unkey(keys[6 * i + 5])};
+    pl = 0;
+    if (empty_space) {
+        const f3 pMin = mk3(N.bmin[node]), pMax = mk3(N.bmax[node]);
+        if (rMin.x - pMin.x > thr * (pMax.x - pMin.x)) pl |= 1;
+        if (rMin.y - pMin.y > thr * (pMax.y - pMin.y)) pl |= 2;
+        if (rMin.z - pMin.z > thr * (pMax.z - pMin.z)) pl |= 4;
+        if (pMax.x - rMax.x > thr * (pMax.x - pMin.x)) pl |= 8;
+        if (pMax.y - rMax.y > thr * (pMax.y - pMin.y)) pl |= 16;
+        if (pMax.z - rMax.z > thr * (pMax.z - pMin.z)) pl |= 32;
+    }
+    N.bmin[node] = make_float4(rMin.x, rMin.y, rMin.z, 0.0f);
+    N.bmax[node] = make_float4(rMax.x, rMax.y, rMax.z, 0.0f);
+    const f3 sz = sub3(rMax, rMin);
+    const f3 med = {sz.x * 0.5f + rMin.x, sz.y * 0.5f + rMin.y, sz.z * 0.5f + rMin.z};
+    const bool yAboveX = sz.x < sz.y;
+    const float mxs = yAboveX ? sz.y : sz.x;
+    const float spl = yAboveX ? med.y : med.x;
+    const int ax = yAboveX ? AX_Y : AX_X;
+    const bool zHigher = mxs < sz.z;
+    N.split[node] = zHigher ? med.z : spl;
+    N.info[node] = (int8_t)(zHigher ? AX_Z : ax);
+}
+
+__global__ __launch_bounds__(BS) void node_pass_kernel(NodePtrs N, Lvl *L, LevelBufs B, int seg,
+                                                       const int *__restrict__ keys, int empty_space, float thr) {
+    __shared__ int s_tile;
+    __shared__ int s_segpre[NT], s_pi[NT], s_cnt[NT];
+    __shared__ int s_w[2][BS / 64];
+    __shared__ long long s_ex[2];
+    int A, active_index;
+    level_nodes(L, A, active_index);
+    const long long cap_nodes = L->cap_nodes, cap_segs = L->cap_segs;
+    const bool es = empty_space && active_index != 0;   // the root level has no empty-space cuts
+    const int ntiles = max(1, (A + NT - 1) / NT);
+    const int t = threadIdx.x;
+    for (int tile; (tile = next_tile(&B.ctr[0], &s_tile)) < ntiles;) {
+        const int i0 = tile * NT, n = min(NT, A - i0);
+        int pl[4], v[2] = {0, 0};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int li = 4 * t + j;
+            pl[j] = 0;
+            if (li < n) {
+                const int node = active_index + i0 + li;
+                prepare_node(N, i0 + li, node, keys, es, thr, pl[j]);
+                const int cnt = (int)N.bitmap[node];
+                s_pi[li] = N.prim_index[node];
+                s_cnt[li] = cnt;
+                v[0] += 2 * __popc(pl[j]);
+                v[1] += (cnt + seg - 1) / seg;
+            }
+        }
+        int pre[2], tot[2];
+        block_scan<2>(v, pre, tot, s_w);
+        if (t < 64) {
+            const long long agg[2] = {tot[0], tot[1]};
+            long long ex[2];
+            lookback<2>(B.lb1, tile, agg, ex, &L->status);
+            if (t == 0) { s_ex[0] = ex[0]; s_ex[1] = ex[1]; }
+        }
+        __syncthreads();
+        const int ereq = (int)s_ex[0], eseg = (int)s_ex[1];
+        // chains and table only inside the capacities (a captured build
+        // that outgrew them fails in the plan below and writes nothing more)
+        const bool fits = (long long)active_index + A + ereq + tot[0] <= cap_nodes &&
+                          (long long)eseg + tot[1] <= cap_segs;
+        int r = pre[0], q = pre[1];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int li = 4 * t + j;
+            if (li < n) {
+                const int i = i0 + li;
+                B.eaddr[i] = ereq + r;
+                B.segbase[i] = eseg + q;
+                s_segpre[li] = q;
+                if (pl[j] && fits) empty_chain_node_at(N, i, A, active_index, pl[j], ereq + r);
+                r += 2 * __popc(pl[j]);
+                q += (s_cnt[li] + seg - 1) / seg;
+            }
+        }
+        __syncthreads();
+        // the segment table: the tile's tot[1] segments start at eseg; a
+        // segment's node is the last li with s_segpre[li] <= its index
+        for (int ls = t; fits && ls < tot[1]; ls += BS) {
+            int lo = 0, hi = n - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_segpre[mid] <= ls) lo = mid; else hi = mid - 1;
+            }
+            const int off = (ls - s_segpre[lo]) * seg;
+            const int sg = eseg + ls;
+            B.seg_node[sg] = i0 + lo;
+            B.seg_start[sg] = s_pi[lo] + off;
+            B.seg_count[sg] = min(seg, s_cnt[lo] - off);
+        }
+        if (tile == ntiles - 1) {
+            // the level's totals: the previous level's advance, then the
+            // empty-space nodes and segments (plan_nodes)
+            if (t == 0) {
+                const int E = ereq + tot[0], S = eseg + tot[1];
+                B.eaddr[A] = E;
+                B.segbase[A] = S;
+                plan_nodes(L, E, S);
+            }
+            __syncthreads();
+            // the leaf pass starts with fresh tickets and status words
+            const int nt3 = max(1, (A + NT - 1) / NT);
+            for (int k = t; k < 6 * nt3; k += BS) B.lb3[k] = 0ull;
+            if (t == 0) B.ctr[2] = 0;
+        }
+    }
+}
+
+// SetSplitSide (TriangleUpperChildren.h:10-62) for one segment: its counts
+// of refs on the left and on the right of its node's plane
+template <int SEGT, bool DIV>
+__device__ __forceinline__ void classify_segment(const NodePtrs &N, int node, int start, int count,
+                                                 const float4 *__restrict__ rmin, const float4 *__restrict__ rmax,
+                                                 const DivideGeom &dg, int &cl, int &cr) {
+    const int lane = threadIdx.x & 63;
+    const int ax = __builtin_amdgcn_readfirstlane(N.info[node]);
+    const float sp = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(N.split[node])));
+    cl = 0; cr = 0;
+    if (!DIV) {
+        // the split axis' coordinates only, loaded without branches (lanes
+        // past the chunk re-read its first ref) so that the chunk's loads
+        // all issue before the first compare
+        const float *lo = reinterpret_cast<const float *>(rmin + start) + (ax - AX_X);
+        const float *hi = reinterpret_cast<const float *>(rmax + start) + (ax - AX_X);
+#pragma unroll 16
+        for (int it = 0; it < SEGT / 64; ++it) {
+            const int j = it * 64 + lane;
+            const int jj = j < count ? j : 0;
+            const float a = lo[4 * jj], b = hi[4 * jj];
+            cl += __popcll(__ballot(j < count && a <= sp));
+            cr += __popcll(__ballot(j < count && sp < b));
+        }
+    } else {
+        const f3 nmn = mk3(N.bmin[node]), nmx = mk3(N.bmax[node]);
+#pragma unroll 16
+        for (int it = 0; it < SEGT / 64; ++it) {
+            const int j = it * 64 + lane;
+            const bool valid = j < count;
+            bool l = false, r = false;
+            if (valid) {
+                const float4 mn = rmin[start + j];
+                l = comp4(mn, ax) <= sp;
+                r = sp < comp4(rmax[start + j], ax);
+                if (l && r) divide_sides(dg, mn.w, ax, sp, nmn, nmx, l, r);
+            }
+            cl += __popcll(__ballot(l));
+            cr += __popcll(__ballot(r));
+        }
+    }
+}
+
+template <int SEGT, bool DIV>
+__global__ __launch_bounds__(WPB * 64) void classify_pass_kernel(const NodePtrs N, const Lvl *L, LevelBufs B,
+                                                                 const float4 *__restrict__ rmin,
+                                                                 const float4 *__restrict__ rmax, DivideGeom dg) {
+    const int S = L->status ? 0 : L->S, active_index = L->active_index;
+    for (int s = wave_index(); s < S; s += wave_count()) {
+        int cl, cr;
+        classify_segment<SEGT, DIV>(N, active_index + B.seg_node[s], B.seg_start[s], B.seg_count[s], rmin, rmax,
+                                    dg, cl, cr);
+        if ((threadIdx.x & 63) == 0) { B.segL[s] = cl; B.segR[s] = cr; }
+    }
+}
+
+// the children plan from the level's totals (the host arithmetic of
+// TriangleMapUpperCreator.cu:140-180)
+__device__ ChildPlan plan_children_totals(const Lvl *L, int L_all, int R_all, int LL_all, int LR_all,
+                                          int leaf_nodes) {
     ChildPlan P{};
-    const int A = L->A, S = L->S;
-    P.L_all = segLoff[S]; P.R_all = segRoff[S]; P.LL_all = segLLoff[S]; P.LR_all = segLRoff[S];
-    P.leaf_nodes = lna[2 * A];
-    const long long newTri = (long long)P.L_all + P.R_all;
-    if (newTri < L->R) { P.status = PM_ERR_INTERNAL; return P; }
+    P.L_all = L_all; P.R_all = R_all; P.LL_all = LL_all; P.LR_all = LR_all;
+    P.leaf_nodes = leaf_nodes;
+    const long long newTri = (long long)L_all + R_all;
+    if (newTri < L->R) { P.status = PM_ERR_INTERNAL; return P; }   // TriangleMapUpperCreator.cu:606-607
     if (newTri > INT_MAX) { P.status = PM_ERR_ARG; return P; }
-    P.leafTri = P.LL_all + P.LR_all;
+    P.leafTri = LL_all + LR_all;
     P.nextR = (int)newTri - P.leafTri;
-    P.created = 2 * A - P.leaf_nodes;
+    P.created = 2 * L->A - leaf_nodes;
     if (P.created > 0 && P.nextR > L->max_refs) { P.status = PM_ERR_DEPTH; return P; }
     if (P.nextR > L->cap_refs || (long long)L->final_refs + P.leafTri > L->cap_leaf_refs ||
-        (long long)L->nleaf + P.leaf_nodes > L->cap_leaf_ids || P.created > L->cap_active) {
+        (long long)L->nleaf + leaf_nodes > L->cap_leaf_ids || P.created > L->cap_active) {
         P.status = PM_ERR_CAPACITY;
         return P;
     }
     P.child_start = L->nodes_size;
-    P.next_active = L->nodes_size + P.leaf_nodes;
+    P.next_active = L->nodes_size + leaf_nodes;
     return P;
 }
 
-__global__ __launch_bounds__(BS) void children_kernel(
-    NodePtrs N, Lvl *__restrict__ L, int propagate, const int *__restrict__ segbase,
-    const int *__restrict__ segLoff, const int *__restrict__ segRoff,
-    const int *__restrict__ segLLoff, const int *__restrict__ segLRoff,
-    const int *__restrict__ slot_leaf, const int *__restrict__ lna, int *__restrict__ leaf_ids,
-    int *__restrict__ keys_next) {
-    if (L->status) return;
-    const ChildPlan P = plan_children(L, segLoff, segRoff, segLLoff, segLRoff, lna);
-    const int A = L->A, active_index = L->active_index, ulp = L->final_refs, nleaf = L->nleaf;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        if (P.status) {
-            lvl_fail(L, P.status);
-        } else {
-            L->L_all = P.L_all; L->R_all = P.R_all; L->LL_all = P.LL_all; L->LR_all = P.LR_all;
-            L->leaf_nodes = P.leaf_nodes; L->leafTri = P.leafTri; L->nextR = P.nextR; L->created = P.created;
-            L->child_start = P.child_start;
-            L->next_active = P.next_active;
-            L->A_scan = P.created;
-            L->advance = 1;
-        }
-    }
-    if (P.status) return;
-    const int child_start = P.child_start, next_active = P.next_active, L_all = P.L_all, LL_all = P.LL_all;
-    const int leaf_nodes = P.leaf_nodes;
-    GRID_STRIDE(i, A) {
-        const int par = active_index + i;
-        const int b = segbase[i], e = segbase[i + 1];
-        const int sizes[2] = {segLoff[e] - segLoff[b], segRoff[e] - segRoff[b]};
-        const int split_addr[2] = {segLoff[b], L_all + segRoff[b]};
-        const int leaf_addr[2] = {segLLoff[b], LL_all + segLRoff[b]};
-        const int ax = N.info[par] & 3;
-        const float s = N.split[par];
-        const float4 pmn = N.bmin[par], pmx = N.bmax[par];
-        int ids[2];
+__global__ __launch_bounds__(BS) void leaf_pass_kernel(Lvl *L, LevelBufs B, int LS, int *__restrict__ keys_next) {
+    __shared__ int s_tile;
+    __shared__ int s_w[4][BS / 64];
+    __shared__ long long s_ex[6];
+    const int A = L->status ? 0 : L->A, S = L->status ? 0 : L->S;
+    const int ntiles = max(1, (A + NT - 1) / NT);
+    const int t = threadIdx.x;
+    // the next level's box keys: every possible child slot (created <= 2A),
+    // before any scatter joins into them
+    {
         const int kmin = fkey(F_INF), kmax = fkey(-F_INF);
-        for (int side = 0; side < 2; ++side) {
-            const int slot = side * A + i;
-            const bool is_leaf = slot_leaf[slot] != 0;
-            const int id = child_start + (is_leaf ? lna[slot] : slot - lna[slot] + leaf_nodes);
-            ids[side] = id;
-            N.info[id] = LEAF; N.split[id] = 0.0f;
-            N.prim_index[id] = is_leaf ? leaf_addr[side] + ulp : split_addr[side] - leaf_addr[side];
-            N.bitmap[id] = sizes[side];
-            N.children[id] = make_int2(0, 0);
-            N.area[id] = 0.0f;
-            N.parent[id] = par;
-            if (propagate) {
-                if (side == 0) {
-                    N.bmin[id] = pmn;
-                    N.bmax[id] = make_float4(ax == AX_X ? s : pmx.x, ax == AX_Y ? s : pmx.y, ax == AX_Z ? s : pmx.z, 0.0f);
-                } else {
-                    N.bmin[id] = make_float4(ax == AX_X ? s : pmn.x, ax == AX_Y ? s : pmn.y, ax == AX_Z ? s : pmn.z, 0.0f);
-                    N.bmax[id] = pmx;
-                }
-            } else {
-                N.bmin[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                N.bmax[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        GRID_STRIDE(k, 12 * A) keys_next[k] = (k % 6) < 3 ? kmin : kmax;
+    }
+    for (int tile; (tile = next_tile(&B.ctr[2], &s_tile)) < ntiles;) {
+        const int i0 = tile * NT, n = min(NT, A - i0);
+        // the tile's segments [sb, se): the side counts' exclusive prefixes
+        // inside the tile, written to segLoff/segRoff (offset below)
+        const int sb = n > 0 ? B.segbase[i0] : 0, se = n > 0 ? B.segbase[i0 + n] : 0;
+        int cL = 0, cR = 0;
+        for (int c0 = sb; c0 < se; c0 += NT) {
+            int v[2] = {0, 0}, sl[4], sr[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int s = c0 + 4 * t + j;
+                sl[j] = s < se ? B.segL[s] : 0;
+                sr[j] = s < se ? B.segR[s] : 0;
+                v[0] += sl[j]; v[1] += sr[j];
             }
-            if (is_leaf) {
-                leaf_ids[nleaf + lna[slot]] = id;
-            } else {
-                int *k = keys_next + 6 * (id - next_active);
-                k[0] = kmin; k[1] = kmin; k[2] = kmin; k[3] = kmax; k[4] = kmax; k[5] = kmax;
+            int pre[2], tot[2];
+            block_scan<2>(v, pre, tot, s_w);
+            int pl = cL + pre[0], pr = cR + pre[1];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int s = c0 + 4 * t + j;
+                if (s < se) { B.segLoff[s] = pl; B.segRoff[s] = pr; }
+                pl += sl[j]; pr += sr[j];
+            }
+            cL += tot[0]; cR += tot[1];
+        }
+        __syncthreads();   // the tile's local prefixes are visible to the whole block
+        // children sizes and leaf flags per node (CalcNodeChildSize, MarkNodeLeafs)
+        int v[4] = {0, 0, 0, 0}, nl[4], nr[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int li = 4 * t + j;
+            nl[j] = nr[j] = 0;
+            if (li < n) {
+                const int b = B.segbase[i0 + li], e = B.segbase[i0 + li + 1];
+                if (e > b) {
+                    nl[j] = (e < se ? B.segLoff[e] : cL) - B.segLoff[b];
+                    nr[j] = (e < se ? B.segRoff[e] : cR) - B.segRoff[b];
+                }
+                v[0] += nl[j] < LS;
+                v[1] += nr[j] < LS;
+                v[2] += nl[j] < LS ? nl[j] : 0;
+                v[3] += nr[j] < LS ? nr[j] : 0;
             }
         }
-        N.children[par] = make_int2(ids[0], ids[1]);
+        int pre[4], tot[4];
+        block_scan<4>(v, pre, tot, s_w);
+        if (t < 64) {
+            const long long agg[6] = {cL, cR, tot[0], tot[1], tot[2], tot[3]};
+            long long ex[6];
+            lookback<6>(B.lb3, tile, agg, ex, &L->status);
+            if (t == 0) for (int k = 0; k < 6; ++k) s_ex[k] = ex[k];
+        }
+        __syncthreads();
+        // the side offsets of the tile's segments, now global
+        const int oL = (int)s_ex[0], oR = (int)s_ex[1];
+        for (int s = sb + t; s < se; s += BS) {
+            B.segLoff[s] += oL;
+            B.segRoff[s] += oR;
+        }
+        int p[4];
+        for (int k = 0; k < 4; ++k) p[k] = (int)s_ex[k + 2] + pre[k];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int li = 4 * t + j;
+            if (li < n) {
+                B.lna[i0 + li] = make_int2(p[0], p[1]);
+                B.lref[i0 + li] = make_int2(p[2], p[3]);
+                p[0] += nl[j] < LS; p[1] += nr[j] < LS;
+                p[2] += nl[j] < LS ? nl[j] : 0; p[3] += nr[j] < LS ? nr[j] : 0;
+            }
+        }
+        if (tile == ntiles - 1) {
+            if (t == 0) {
+                long long T[6];
+                for (int k = 0; k < 6; ++k) T[k] = s_ex[k] + (k == 0 ? cL : k == 1 ? cR : tot[k - 2]);
+                B.segLoff[S] = (int)T[0];
+                B.segRoff[S] = (int)T[1];
+                const ChildPlan P = plan_children_totals(L, (int)T[0], (int)T[1], (int)T[4], (int)T[5],
+                                                         (int)(T[2] + T[3]));
+                if (P.status) {
+                    lvl_fail(L, P.status);
+                } else {
+                    L->L_all = P.L_all; L->R_all = P.R_all; L->LL_all = P.LL_all; L->LR_all = P.LR_all;
+                    L->leaf_nodes = P.leaf_nodes; L->leafTri = P.leafTri; L->nextR = P.nextR;
+                    L->created = P.created; L->child_start = P.child_start; L->next_active = P.next_active;
+                    L->A_scan = P.created;
+                    L->leaf_left = (int)T[2];
+                    L->advance = 1;
+                }
+                B.ctr[0] = 0;
+            }
+            __syncthreads();
+            // the next level's node pass starts with fresh status words
+            const int created = L->status ? 0 : L->created;
+            const int nt1 = max(1, (created + NT - 1) / NT);
+            for (int k = t; k < 2 * nt1; k += BS) B.lb1[k] = 0ull;
+        }
     }
 }
 
-// SplitTriangles (TriangleUpperChildren.h:283-383): stable partition of the
-// refs with clipping, plus the next level's box reduction.
+// CreateUpperChildren (TriangleUpperChildren.h:203-281), MarkLeafNodes
+// (:385-395), PropagateAabbToChildren (EmptySpaceSplitting.h:299-331) for
+// active node i; nl/nr its children's sizes, b its first segment
+struct LevelPlan {
+    int A, active_index, child_start, next_active, leaf_nodes, leaf_left, L_all, LL_all, ulp, nleaf, LS;
+};
+
+__device__ __forceinline__ void child_ids(const LevelPlan &P, int i, const int2 &ln, bool leafL, bool leafR,
+                                          int &idL, int &idR) {
+    const int lnaL = ln.x, lnaR = P.leaf_left + ln.y;
+    idL = P.child_start + (leafL ? lnaL : i - lnaL + P.leaf_nodes);
+    idR = P.child_start + (leafR ? lnaR : P.A + i - lnaR + P.leaf_nodes);
+}
+
+__device__ void write_children(const NodePtrs &N, const LevelPlan &P, const LevelBufs &B, int i, int propagate,
+                               int *__restrict__ leaf_ids) {
+    const int b = B.segbase[i], e = B.segbase[i + 1];
+    const int sizes[2] = {B.segLoff[e] - B.segLoff[b], B.segRoff[e] - B.segRoff[b]};
+    const bool leaf[2] = {sizes[0] < P.LS, sizes[1] < P.LS};
+    const int2 ln = B.lna[i], lr = B.lref[i];
+    int ids[2];
+    child_ids(P, i, ln, leaf[0], leaf[1], ids[0], ids[1]);
+    const int split_addr[2] = {B.segLoff[b], P.L_all + B.segRoff[b]};
+    const int leaf_addr[2] = {lr.x, P.LL_all + lr.y};
+    const int lna[2] = {ln.x, P.leaf_left + ln.y};
+    const int par = P.active_index + i;
+    const int ax = N.info[par] & 3;
+    const float s = N.split[par];
+    const float4 pmn = N.bmin[par], pmx = N.bmax[par];
+    for (int side = 0; side < 2; ++side) {
+        const int id = ids[side];
+        N.info[id] = LEAF; N.split[id] = 0.0f;
+        N.prim_index[id] = leaf[side] ? leaf_addr[side] + P.ulp : split_addr[side] - leaf_addr[side];
+        N.bitmap[id] = sizes[side];
+        N.children[id] = make_int2(0, 0);
+        N.area[id] = 0.0f;
+        N.parent[id] = par;
+        if (propagate) {
+            if (side == 0) {
+                N.bmin[id] = pmn;
+                N.bmax[id] = make_float4(ax == AX_X ? s : pmx.x, ax == AX_Y ? s : pmx.y, ax == AX_Z ? s : pmx.z, 0.0f);
+            } else {
+                N.bmin[id] = make_float4(ax == AX_X ? s : pmn.x, ax == AX_Y ? s : pmn.y, ax == AX_Z ? s : pmn.z, 0.0f);
+                N.bmax[id] = pmx;
+            }
+        } else {
+            N.bmin[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            N.bmax[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        if (leaf[side]) leaf_ids[P.nleaf + lna[side]] = id;
+    }
+    N.children[par] = make_int2(ids[0], ids[1]);
+}
+
+// SplitTriangles (TriangleUpperChildren.h:283-383) for segment s: stable
+// partition of its refs with clipping, and the next level's box joins; the
+// wave of a node's first segment also writes the node's two children
 template <int SEGT, bool DIV>
-__device__ __forceinline__ void scatter_segment(
-    const NodePtrs &N, int s, int A, int active_index, int next_active, int L_all, int LL_all, int ulp,
-    const int *__restrict__ seg_node, const int *__restrict__ seg_start,
-    const int *__restrict__ seg_count, const int *__restrict__ segLoff,
-    const int *__restrict__ segRoff, const int *__restrict__ segLLoff,
-    const int *__restrict__ segLRoff, const int *__restrict__ slot_leaf,
+__device__ __forceinline__ void scatter_pass_segment(
+    const NodePtrs &N, const LevelPlan &P, const LevelBufs &B, int s, int propagate,
     const float4 *__restrict__ rmin, const float4 *__restrict__ rmax,
     float4 *__restrict__ nmin, float4 *__restrict__ nmax, float4 *__restrict__ fmin,
-    float4 *__restrict__ fmax, int *__restrict__ keys_next, const DivideGeom &dg, bool join_test) {
+    float4 *__restrict__ fmax, int *__restrict__ keys_next, int *__restrict__ leaf_ids, const DivideGeom &dg,
+    bool join_test) {
     const int lane = threadIdx.x & 63;
-    const int i = seg_node[s];
-    const int node = active_index + i;
+    const int i = B.seg_node[s];
+    const int node = P.active_index + i;
+    const int b = B.segbase[i], e = B.segbase[i + 1];
+    const int nl = B.segLoff[e] - B.segLoff[b], nr = B.segRoff[e] - B.segRoff[b];
+    const bool leafL = nl < P.LS, leafR = nr < P.LS;
+    const int2 ln = B.lna[i], lr = B.lref[i];
+    int idL, idR;
+    child_ids(P, i, ln, leafL, leafR, idL, idR);
+    if (s == b && lane == 0) write_children(N, P, B, i, propagate, leaf_ids);
+    const int sL = B.segLoff[s], sR = B.segRoff[s];
+    const int LLoff = lr.x + (leafL ? sL - B.segLoff[b] : 0);
+    const int LRoff = lr.y + (leafR ? sR - B.segRoff[b] : 0);
     // one chunk per wave: its node's axis and plane are wave-uniform
     const int ax = __builtin_amdgcn_readfirstlane(N.info[node]);
     const float sp = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(N.split[node])));
     f3 nmn = {0.0f, 0.0f, 0.0f}, nmx = nmn;
     if (DIV) { nmn = mk3(N.bmin[node]); nmx = mk3(N.bmax[node]); }
-    const int2 ch = N.children[node];
-    const bool leafL = slot_leaf[i] != 0, leafR = slot_leaf[A + i] != 0;
-    const int start = seg_start[s], count = seg_count[s];
+    const int start = B.seg_start[s], count = B.seg_count[s];
     float4 *dLmin, *dLmax, *dRmin, *dRmax;
     long long bL, bR;
-    if (leafL) { dLmin = fmin; dLmax = fmax; bL = (long long)ulp + segLLoff[s]; }
-    else { dLmin = nmin; dLmax = nmax; bL = segLoff[s] - segLLoff[s]; }
-    if (leafR) { dRmin = fmin; dRmax = fmax; bR = (long long)ulp + LL_all + segLRoff[s]; }
-    else { dRmin = nmin; dRmax = nmax; bR = (long long)(L_all - LL_all) + (segRoff[s] - segLRoff[s]); }
+    if (leafL) { dLmin = fmin; dLmax = fmax; bL = (long long)P.ulp + LLoff; }
+    else { dLmin = nmin; dLmax = nmax; bL = sL - LLoff; }
+    if (leafR) { dRmin = fmin; dRmax = fmax; bR = (long long)P.ulp + P.LL_all + LRoff; }
+    else { dRmin = nmin; dRmax = nmax; bR = (long long)(P.L_all - P.LL_all) + (sR - LRoff); }
     const unsigned long long lt = lanemask_lt();
     int rl = 0, rr = 0;
     int l0 = fkey(F_INF), l1 = l0, l2 = l0, l3 = fkey(-F_INF), l4 = l3, l5 = l3;
     int r0 = l0, r1 = l0, r2 = l0, r3 = l3, r4 = l3, r5 = l3;
     // groups of G iterations: the group's refs are loaded first (lanes past
     // the chunk re-read its first ref), then classified and written
-#ifndef PM_SCATTER_GROUP
-#define PM_SCATTER_GROUP 8
-#endif
-    constexpr int IT = SEGT / 64, G = IT < PM_SCATTER_GROUP ? IT : PM_SCATTER_GROUP;
+    constexpr int IT = SEGT / 64, G = IT < 8 ? IT : 8;
 #pragma unroll 4
     for (int g0 = 0; g0 < IT; g0 += G) {
         float4 gmn[G], gmx[G];
@@ -823,68 +1054,64 @@ __device__ __forceinline__ void scatter_segment(
         r3 = wave_max(r3); r4 = wave_max(r4); r5 = wave_max(r5);
     }
     if (!join_test) {
-        if (lane == 0 && doL) key_atomics(keys_next + 6 * (ch.x - next_active), l0, l1, l2, l3, l4, l5);
-        if (lane == 0 && doR) key_atomics(keys_next + 6 * (ch.y - next_active), r0, r1, r2, r3, r4, r5);
+        if (lane == 0 && doL) key_atomics(keys_next + 6 * (idL - P.next_active), l0, l1, l2, l3, l4, l5);
+        if (lane == 0 && doR) key_atomics(keys_next + 6 * (idR - P.next_active), r0, r1, r2, r3, r4, r5);
         return;
     }
     // few nodes: lanes 0..5 join the left child's six key words, lanes 6..11
     // the right's, each skipping a join that cannot change its word
     const bool right = lane >= 6;
-    const int t = right ? lane - 6 : lane;
+    const int tw = right ? lane - 6 : lane;
     if (lane < 12 && (right ? doR : doL)) {
         const int a0 = right ? r0 : l0, a1 = right ? r1 : l1, a2 = right ? r2 : l2;
         const int a3 = right ? r3 : l3, a4 = right ? r4 : l4, a5 = right ? r5 : l5;
-        const int v = t == 0 ? a0 : t == 1 ? a1 : t == 2 ? a2 : t == 3 ? a3 : t == 4 ? a4 : a5;
-        key_join(keys_next + 6 * ((right ? ch.y : ch.x) - next_active) + t, v, t < 3);
+        const int v = tw == 0 ? a0 : tw == 1 ? a1 : tw == 2 ? a2 : tw == 3 ? a3 : tw == 4 ? a4 : a5;
+        key_join(keys_next + 6 * ((right ? idR : idL) - P.next_active) + tw, v, tw < 3);
     }
 }
 
 template <int SEGT, bool DIV>
-__device__ __forceinline__ void scatter_level(
-    const NodePtrs N, const Lvl *__restrict__ L,
-    const int *__restrict__ seg_node, const int *__restrict__ seg_start,
-    const int *__restrict__ seg_count, const int *__restrict__ segLoff,
-    const int *__restrict__ segRoff, const int *__restrict__ segLLoff,
-    const int *__restrict__ segLRoff, const int *__restrict__ slot_leaf,
-    const float4 *__restrict__ rmin, const float4 *__restrict__ rmax,
-    float4 *__restrict__ nmin, float4 *__restrict__ nmax, float4 *__restrict__ fmin,
-    float4 *__restrict__ fmax, int *__restrict__ keys_next, DivideGeom dg) {
-    const int S = L->S, A = L->A, active_index = L->active_index, next_active = L->next_active;
-    const int L_all = L->L_all, LL_all = L->LL_all, ulp = L->final_refs;
+__device__ __forceinline__ void scatter_pass(const NodePtrs N, const Lvl *L, const LevelBufs B, int LS, int propagate,
+                                             const float4 *__restrict__ rmin, const float4 *__restrict__ rmax,
+                                             float4 *__restrict__ nmin, float4 *__restrict__ nmax,
+                                             float4 *__restrict__ fmin, float4 *__restrict__ fmax,
+                                             int *__restrict__ keys_next, int *__restrict__ leaf_ids, DivideGeom dg) {
+    if (L->status) return;
+    LevelPlan P;
+    P.A = L->A; P.active_index = L->active_index; P.child_start = L->child_start; P.next_active = L->next_active;
+    P.leaf_nodes = L->leaf_nodes; P.leaf_left = L->leaf_left; P.L_all = L->L_all; P.LL_all = L->LL_all;
+    P.ulp = L->final_refs; P.nleaf = L->nleaf; P.LS = LS;
+    const int S = L->S;
     for (int s = wave_index(); s < S; s += wave_count())
-        scatter_segment<SEGT, DIV>(N, s, A, active_index, next_active, L_all, LL_all, ulp, seg_node, seg_start,
-                              seg_count, segLoff, segRoff, segLLoff, segLRoff, slot_leaf, rmin, rmax, nmin,
-                              nmax, fmin, fmax, keys_next, dg, A < JOIN_TEST_NODES);
-}
-
-// box splits: at most 128 VGPRs (4 waves per SIMD) with a group's eight
-// float4 pairs in flight; DIVIDE (more live state) keeps its own allocation
-template <int SEGT>
-__global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(4))) void scatter_kernel(
-    const NodePtrs N, const Lvl *__restrict__ L,
-    const int *__restrict__ seg_node, const int *__restrict__ seg_start,
-    const int *__restrict__ seg_count, const int *__restrict__ segLoff,
-    const int *__restrict__ segRoff, const int *__restrict__ segLLoff,
-    const int *__restrict__ segLRoff, const int *__restrict__ slot_leaf,
-    const float4 *__restrict__ rmin, const float4 *__restrict__ rmax,
-    float4 *__restrict__ nmin, float4 *__restrict__ nmax, float4 *__restrict__ fmin,
-    float4 *__restrict__ fmax, int *__restrict__ keys_next, DivideGeom dg) {
-    scatter_level<SEGT, false>(N, L, seg_node, seg_start, seg_count, segLoff, segRoff, segLLoff, segLRoff, slot_leaf, rmin, rmax, nmin, nmax,
-        fmin, fmax, keys_next, dg);
+        scatter_pass_segment<SEGT, DIV>(N, P, B, s, propagate, rmin, rmax, nmin, nmax, fmin, fmax, keys_next,
+                                        leaf_ids, dg, P.A < JOIN_TEST_NODES);
+    // nodes without refs (an empty mesh's root) have no segment to write their children
+    GRID_STRIDE(i, P.A)
+        if (B.segbase[i + 1] == B.segbase[i]) write_children(N, P, B, i, propagate, leaf_ids);
 }
 
 template <int SEGT>
-__global__ __launch_bounds__(WPB * 64) void scatter_div_kernel(
-    const NodePtrs N, const Lvl *__restrict__ L,
-    const int *__restrict__ seg_node, const int *__restrict__ seg_start,
-    const int *__restrict__ seg_count, const int *__restrict__ segLoff,
-    const int *__restrict__ segRoff, const int *__restrict__ segLLoff,
-    const int *__restrict__ segLRoff, const int *__restrict__ slot_leaf,
-    const float4 *__restrict__ rmin, const float4 *__restrict__ rmax,
-    float4 *__restrict__ nmin, float4 *__restrict__ nmax, float4 *__restrict__ fmin,
-    float4 *__restrict__ fmax, int *__restrict__ keys_next, DivideGeom dg) {
-    scatter_level<SEGT, true>(N, L, seg_node, seg_start, seg_count, segLoff, segRoff, segLLoff, segLRoff, slot_leaf, rmin, rmax, nmin, nmax,
-        fmin, fmax, keys_next, dg);
+__global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(4))) void scatter_pass_kernel(
+    const NodePtrs N, const Lvl *L, const LevelBufs B, int LS, int propagate,
+    const float4 *__restrict__ rmin, const float4 *__restrict__ rmax, float4 *__restrict__ nmin,
+    float4 *__restrict__ nmax, float4 *__restrict__ fmin, float4 *__restrict__ fmax, int *__restrict__ keys_next,
+    int *__restrict__ leaf_ids, DivideGeom dg) {
+    scatter_pass<SEGT, false>(N, L, B, LS, propagate, rmin, rmax, nmin, nmax, fmin, fmax, keys_next, leaf_ids, dg);
+}
+
+template <int SEGT>
+__global__ __launch_bounds__(WPB * 64) void scatter_pass_div_kernel(
+    const NodePtrs N, const Lvl *L, const LevelBufs B, int LS, int propagate,
+    const float4 *__restrict__ rmin, const float4 *__restrict__ rmax, float4 *__restrict__ nmin,
+    float4 *__restrict__ nmax, float4 *__restrict__ fmin, float4 *__restrict__ fmax, int *__restrict__ keys_next,
+    int *__restrict__ leaf_ids, DivideGeom dg) {
+    scatter_pass<SEGT, true>(N, L, B, LS, propagate, rmin, rmax, nmin, nmax, fmin, fmax, keys_next, leaf_ids, dg);
+}
+
+// look-back words and tickets of the first level
+__global__ void lb_init_kernel(unsigned long long *lb1, int n1, int *ctr) {
+    GRID_STRIDE(k, n1) lb1[k] = 0ull;
+    if (blockIdx.x == 0 && threadIdx.x < 3) ctr[threadIdx.x] = 0;
 }
 
 // ---- plan kernels: the host arithmetic of the level loop
@@ -1404,45 +1631,45 @@ __global__ void lower_children_kernel(NodePtrs N, const int *__restrict__ leaf_i
 // splits the chunk loops carry no data-dependent loads, so all of a chunk's
 // ref loads issue before the first is used
 template <bool DIV, class... Args>
-void launch_classify_t(int seg, int blocks, hipStream_t s, Args... a) {
+void launch_classify_pass_t(int seg, int blocks, hipStream_t s, Args... a) {
     switch (seg) {
-    case 128: classify_kernel<128, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
-    case 512: classify_kernel<512, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
-    case 1024: classify_kernel<1024, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
-    case 2048: classify_kernel<2048, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
-    case 4096: classify_kernel<4096, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
-    default: classify_kernel<256, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    case 128: classify_pass_kernel<128, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    case 512: classify_pass_kernel<512, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    case 1024: classify_pass_kernel<1024, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    case 2048: classify_pass_kernel<2048, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    case 4096: classify_pass_kernel<4096, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
+    default: classify_pass_kernel<256, DIV><<<blocks, WPB * 64, 0, s>>>(a...); break;
     }
 }
 
 template <class... Args>
-void launch_classify(int seg, bool div, int blocks, hipStream_t s, Args... a) {
-    if (div) launch_classify_t<true>(seg, blocks, s, a...);
-    else launch_classify_t<false>(seg, blocks, s, a...);
+void launch_classify_pass(int seg, bool div, int blocks, hipStream_t s, Args... a) {
+    if (div) launch_classify_pass_t<true>(seg, blocks, s, a...);
+    else launch_classify_pass_t<false>(seg, blocks, s, a...);
 }
 
 template <int SEGT, bool DIV, class... Args>
-void launch_scatter_k(int blocks, hipStream_t s, Args... a) {
-    if constexpr (DIV) scatter_div_kernel<SEGT><<<blocks, WPB * 64, 0, s>>>(a...);
-    else scatter_kernel<SEGT><<<blocks, WPB * 64, 0, s>>>(a...);
+void launch_scatter_pass_k(int blocks, hipStream_t s, Args... a) {
+    if constexpr (DIV) scatter_pass_div_kernel<SEGT><<<blocks, WPB * 64, 0, s>>>(a...);
+    else scatter_pass_kernel<SEGT><<<blocks, WPB * 64, 0, s>>>(a...);
 }
 
 template <bool DIV, class... Args>
-void launch_scatter_t(int seg, int blocks, hipStream_t s, Args... a) {
+void launch_scatter_pass_t(int seg, int blocks, hipStream_t s, Args... a) {
     switch (seg) {
-    case 128: launch_scatter_k<128, DIV>(blocks, s, a...); break;
-    case 512: launch_scatter_k<512, DIV>(blocks, s, a...); break;
-    case 1024: launch_scatter_k<1024, DIV>(blocks, s, a...); break;
-    case 2048: launch_scatter_k<2048, DIV>(blocks, s, a...); break;
-    case 4096: launch_scatter_k<4096, DIV>(blocks, s, a...); break;
-    default: launch_scatter_k<256, DIV>(blocks, s, a...); break;
+    case 128: launch_scatter_pass_k<128, DIV>(blocks, s, a...); break;
+    case 512: launch_scatter_pass_k<512, DIV>(blocks, s, a...); break;
+    case 1024: launch_scatter_pass_k<1024, DIV>(blocks, s, a...); break;
+    case 2048: launch_scatter_pass_k<2048, DIV>(blocks, s, a...); break;
+    case 4096: launch_scatter_pass_k<4096, DIV>(blocks, s, a...); break;
+    default: launch_scatter_pass_k<256, DIV>(blocks, s, a...); break;
     }
 }
 
 template <class... Args>
-void launch_scatter(int seg, bool div, int blocks, hipStream_t s, Args... a) {
-    if (div) launch_scatter_t<true>(seg, blocks, s, a...);
-    else launch_scatter_t<false>(seg, blocks, s, a...);
+void launch_scatter_pass(int seg, bool div, int blocks, hipStream_t s, Args... a) {
+    if (div) launch_scatter_pass_t<true>(seg, blocks, s, a...);
+    else launch_scatter_pass_t<false>(seg, blocks, s, a...);
 }
 
 NodePtrs node_ptrs(pm_ctx *ctx) {
@@ -1488,7 +1715,6 @@ static void scan_level(ScanIO<K> io, BuildScratch &sc, int depth, hipStream_t s)
         sums.n_dev[q] = nullptr;
         sums.n_mult[q] = 1;
     }
-    sums.nodes_plan = nullptr;
     scan_level<K>(sums, sc, depth + 1, s);
     scan_add_kernel<K><<<nb, SCAN_T, 0, s>>>(io, tmp.ptr, stride);
     PM_LAUNCH_CHECK();
@@ -1498,7 +1724,6 @@ template <int K>
 static void scan_set(ScanIO<K> &io, int q, const int *in, int *out, int n, const int *n_dev, int n_mult) {
     io.in[q] = in; io.out[q] = out;
     io.n_host[q] = n; io.n_dev[q] = n_dev; io.n_mult[q] = n_mult;
-    io.nodes_plan = nullptr;
 }
 
 void scan_exclusive(const int *in, int *out, int n, BuildScratch &sc, hipStream_t s, const int *n_dev,
@@ -1514,27 +1739,6 @@ void scan_exclusive2(const int *in0, int *out0, const int *in1, int *out1, int n
     scan_set(io, 0, in0, out0, n, n_dev, n_mult);
     scan_set(io, 1, in1, out1, n, n_dev, n_mult);
     scan_level<2>(io, sc, 0, s);
-}
-
-// the node pass's two scans over L->A_scan (n its bound), then plan_nodes
-static void scan_nodes(const int *req, int *eaddr, const int *nseg, int *segbase, int n, Lvl *L,
-                       BuildScratch &sc, hipStream_t s) {
-    ScanIO<2> io;
-    scan_set(io, 0, req, eaddr, n, &L->A_scan, 1);
-    scan_set(io, 1, nseg, segbase, n, &L->A_scan, 1);
-    io.nodes_plan = L;
-    scan_level<2>(io, sc, 0, s);
-}
-
-// two scans of length n (n_dev) and a third of length m (m_dev, m_mult)
-static void scan_exclusive3(const int *in0, int *out0, const int *in1, int *out1, int n, const int *n_dev,
-                            const int *in2, int *out2, int m, const int *m_dev, int m_mult, BuildScratch &sc,
-                            hipStream_t s) {
-    ScanIO<3> io;
-    scan_set(io, 0, in0, out0, n, n_dev, 1);
-    scan_set(io, 1, in1, out1, n, n_dev, 1);
-    scan_set(io, 2, in2, out2, m, m_dev, m_mult);
-    scan_level<3>(io, sc, 0, s);
 }
 
 namespace {
@@ -1629,40 +1833,52 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
         PM_LAUNCH_CHECK();
     }
 
-    // grids: eager from the sizes just read back, captured from the caps
-    auto g_nodes = [&](long long n) { return grid_for(graph ? caps->active : n, BS, graph ? 2048 : INT_MAX); };
-    auto g_waves = [&](long long n) { return grid_for(graph ? caps->segs : n, WPB, graph ? 2048 : INT_MAX); };
-    auto g_segs = [&](long long n) { return grid_for(graph ? caps->segs : n, BS, graph ? 2048 : INT_MAX); };
     const DivideGeom dg = {prm.split_method == PM_SPLIT_DIVIDE ? P0 : nullptr, P1, P2};
-    const int *dA = graph ? &L->A : nullptr;
-    const int *dS = graph ? &L->S : nullptr;
-    const int nA = graph ? (int)caps->active : 0;
-    const int nS = graph ? (int)caps->segs : 0;
+    const bool div = dg.p0 != nullptr;
+    auto tiles = [](long long n, long long per) { return std::max(1ll, (n + per - 1) / per); };
+    auto level_bufs = [&]() {
+        LevelBufs B;
+        B.eaddr = sc.node_eaddr.ptr; B.segbase = sc.node_segbase.ptr;
+        B.seg_node = sc.seg_node.ptr; B.seg_start = sc.seg_start.ptr; B.seg_count = sc.seg_count.ptr;
+        B.segL = sc.segL.ptr; B.segR = sc.segR.ptr;
+        B.segLoff = sc.segLoff.ptr; B.segRoff = sc.segRoff.ptr;
+        B.lna = sc.node_lna.ptr; B.lref = sc.node_lref.ptr;
+        B.lb1 = sc.lb_words[0].ptr; B.lb3 = sc.lb_words[2].ptr;
+        B.ctr = sc.lb_ctr.ptr;
+        return B;
+    };
+    if (!graph) {
+        sc.lb_ctr.reserve(4, s);
+        sc.lb_words[0].reserve(2 * tiles(M, NT) + 2, s);
+    }
+    lb_init_kernel<<<grid_for(2 * tiles(M, NT), BS, 64), BS, 0, s>>>(sc.lb_words[0].ptr, 2 * (int)tiles(M, NT),
+                                                                    sc.lb_ctr.ptr);
+    PM_LAUNCH_CHECK();
 
-    // The eager build reads the level state back once per level, after the
-    // node pass (it needs the level's segment count).  What the level writes
-    // after that is reserved from bounds: a ref goes to at most two children
-    // (next refs and leaf refs <= 2R) and a node has two children (<= 2A
-    // created nodes, leaves and box keys).  The next level's node pass runs
-    // on the bound 2A with its exact count read on the device; the loop ends
-    // at the read that finds no active node.
+    // A level is four launches (node, classify, leaf and scatter passes; see
+    // the fused-level kernels above).  The eager build reads the level state
+    // back once per level, after the node pass, to size the other three
+    // passes' grids and buffers.  What the node pass writes is reserved from
+    // bounds beforehand: a level has at most twice the previous level's
+    // active nodes and refs, at most 12 empty-space nodes and 2 children per
+    // active node, and at most one segment per active node plus one per chunk
+    // of refs.  The loop ends at the read that finds no active node.
     int cur = 0;
-    long long A_bound = h.A;
+    long long A_bound = M, R_bound = N, written = M;
     for (int lvl = 0; graph ? lvl < caps->upper_levels : true; ++lvl) {
         if (!graph) {
-            sc.node_req.reserve(A_bound + 1, s); sc.node_eaddr.reserve(A_bound + 1, s);
-            sc.node_nseg.reserve(A_bound + 1, s); sc.node_segbase.reserve(A_bound + 1, s);
-            sc.slot_leaf.reserve(2 * (size_t)A_bound + 1, s); sc.slot_lna.reserve(2 * (size_t)A_bound + 1, s);
+            const long long S_bound = A_bound + (R_bound + seg - 1) / seg + 1;
+            nodes.reserve((size_t)(written + 14 * A_bound + 1), s, (size_t)written);
+            sc.node_eaddr.reserve(A_bound + 1, s);
+            sc.node_segbase.reserve(A_bound + 1, s);
+            sc.seg_node.reserve(S_bound + 1, s); sc.seg_start.reserve(S_bound + 1, s);
+            sc.seg_count.reserve(S_bound + 1, s);
+            sc.lb_words[2].reserve(6 * tiles(A_bound, NT) + 6, s);
         }
-        NodePtrs NP = node_ptrs(ctx);
-        // planes are kept in slot_lna's storage until the chain kernel has run
-        int *planes = sc.slot_lna.ptr;
-        node_prepare_kernel<<<g_nodes(A_bound), BS, 0, s>>>(NP, L, seg, sc.keys[cur].ptr, prm.empty_space_splitting,
-                                                              prm.empty_space_threshold, planes, sc.node_req.ptr,
-                                                              sc.node_nseg.ptr);
+        node_pass_kernel<<<grid_for(graph ? caps->active : A_bound, NT, graph ? 1024 : INT_MAX), BS, 0, s>>>(
+            node_ptrs(ctx), L, level_bufs(), seg, sc.keys[cur].ptr, prm.empty_space_splitting,
+            prm.empty_space_threshold);
         PM_LAUNCH_CHECK();
-        scan_nodes(sc.node_req.ptr, sc.node_eaddr.ptr, sc.node_nseg.ptr, sc.node_segbase.ptr,
-                   graph ? nA : (int)A_bound, L, sc, s);
         const int nxt = cur ^ 1;
         if (!graph) {
             read_state(sc, h, s);
@@ -1671,13 +1887,11 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
             ctx->level_refs.push_back(h.R);
             seen.refs = std::max<long long>(seen.refs, h.R);
             seen.active = std::max<long long>(seen.active, h.A);
-            nodes.reserve((size_t)h.nodes_size + 2 * (size_t)h.A, s, h.nodes_size - h.E);
-            const size_t S1 = (size_t)h.S + 1;
-            sc.seg_node.reserve(S1, s); sc.seg_start.reserve(S1, s); sc.seg_count.reserve(S1, s);
-            sc.segL.reserve(S1, s); sc.segR.reserve(S1, s); sc.segLL.reserve(S1, s); sc.segLR.reserve(S1, s);
-            sc.segLoff.reserve(S1, s); sc.segRoff.reserve(S1, s);
-            sc.segLLoff.reserve(S1, s); sc.segLRoff.reserve(S1, s);
             seen.segs = std::max<long long>(seen.segs, h.S);
+            sc.segL.reserve((size_t)h.S + 1, s); sc.segR.reserve((size_t)h.S + 1, s);
+            sc.segLoff.reserve((size_t)h.S + 1, s); sc.segRoff.reserve((size_t)h.S + 1, s);
+            sc.node_lna.reserve((size_t)h.A + 1, s); sc.node_lref.reserve((size_t)h.A + 1, s);
+            sc.lb_words[0].reserve(2 * tiles(2ll * h.A, NT) + 2, s);
             const size_t R2 = 2 * (size_t)h.R;
             sc.prim_min.reserve((size_t)h.final_refs + R2 + 1, s, h.final_refs);
             sc.prim_max.reserve((size_t)h.final_refs + R2 + 1, s, h.final_refs);
@@ -1685,37 +1899,26 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
             sc.ref_min[nxt].reserve(R2 + 1, s); sc.ref_max[nxt].reserve(R2 + 1, s);
             sc.keys[nxt].reserve(12 * (size_t)h.A + 6, s);
         }
-        NP = node_ptrs(ctx);
-        chain_segs_kernel<<<std::max(g_nodes(h.A), g_segs(h.S)), BS, 0, s>>>(
-            NP, L, planes, sc.node_eaddr.ptr, seg, sc.node_segbase.ptr, sc.seg_node.ptr, sc.seg_start.ptr,
-            sc.seg_count.ptr);
+        const long long S_now = graph ? caps->segs : h.S, A_now = graph ? caps->active : h.A;
+        const LevelBufs B = level_bufs();
+        const NodePtrs NP = node_ptrs(ctx);
+        launch_classify_pass(seg, div, grid_for(S_now, WPB, graph ? 2048 : INT_MAX), s, NP, (const Lvl *)L, B,
+                             (const float4 *)sc.ref_min[cur].ptr, (const float4 *)sc.ref_max[cur].ptr, dg);
         PM_LAUNCH_CHECK();
-        launch_classify(seg, dg.p0 != nullptr, g_waves(h.S), s, NP, (const Lvl *)L, (const int *)sc.seg_node.ptr,
-                        (const int *)sc.seg_start.ptr, (const int *)sc.seg_count.ptr,
-                        (const float4 *)sc.ref_min[cur].ptr, (const float4 *)sc.ref_max[cur].ptr, sc.segL.ptr,
-                        sc.segR.ptr, dg);
+        const int g3 = std::max(grid_for(A_now, NT, INT_MAX), grid_for(12 * A_now, 16 * BS, 2048));
+        leaf_pass_kernel<<<graph ? std::min(g3, 2048) : g3, BS, 0, s>>>(L, B, LS, sc.keys[nxt].ptr);
         PM_LAUNCH_CHECK();
-        scan_exclusive2(sc.segL.ptr, sc.segLoff.ptr, sc.segR.ptr, sc.segRoff.ptr, graph ? nS : h.S, sc, s, dS);
-        leaf_flags_kernel<<<std::max(g_nodes(h.A), g_segs(h.S)), BS, 0, s>>>(
-            L, LS, sc.node_segbase.ptr, sc.segLoff.ptr, sc.segRoff.ptr, sc.seg_node.ptr, sc.segL.ptr, sc.segR.ptr,
-            sc.slot_leaf.ptr, sc.segLL.ptr, sc.segLR.ptr);
+        const int g4 = std::max(grid_for(S_now, WPB, INT_MAX), grid_for(A_now, BS, INT_MAX));
+        launch_scatter_pass(seg, div, graph ? std::min(g4, 2048) : g4, s, NP, (const Lvl *)L, B, LS,
+                            prm.empty_space_splitting, (const float4 *)sc.ref_min[cur].ptr,
+                            (const float4 *)sc.ref_max[cur].ptr, sc.ref_min[nxt].ptr, sc.ref_max[nxt].ptr,
+                            sc.prim_min.ptr, sc.prim_max.ptr, sc.keys[nxt].ptr, ctx->leaf_ids.ptr, dg);
         PM_LAUNCH_CHECK();
-        scan_exclusive3(sc.segLL.ptr, sc.segLLoff.ptr, sc.segLR.ptr, sc.segLRoff.ptr, graph ? nS : h.S, dS,
-                        sc.slot_leaf.ptr, sc.slot_lna.ptr, graph ? 2 * nA : 2 * h.A, dA, 2, sc, s);
-        NP = node_ptrs(ctx);
-        children_kernel<<<g_nodes(h.A), BS, 0, s>>>(NP, L, prm.empty_space_splitting, sc.node_segbase.ptr,
-                                                      sc.segLoff.ptr, sc.segRoff.ptr, sc.segLLoff.ptr,
-                                                      sc.segLRoff.ptr, sc.slot_leaf.ptr, sc.slot_lna.ptr,
-                                                      ctx->leaf_ids.ptr, sc.keys[nxt].ptr);
-        PM_LAUNCH_CHECK();
-        launch_scatter(seg, dg.p0 != nullptr, g_waves(h.S), s, NP, (const Lvl *)L, (const int *)sc.seg_node.ptr,
-                       (const int *)sc.seg_start.ptr, (const int *)sc.seg_count.ptr, (const int *)sc.segLoff.ptr,
-                       (const int *)sc.segRoff.ptr, (const int *)sc.segLLoff.ptr, (const int *)sc.segLRoff.ptr,
-                       (const int *)sc.slot_leaf.ptr, (const float4 *)sc.ref_min[cur].ptr,
-                       (const float4 *)sc.ref_max[cur].ptr, sc.ref_min[nxt].ptr, sc.ref_max[nxt].ptr,
-                       sc.prim_min.ptr, sc.prim_max.ptr, sc.keys[nxt].ptr, dg);
-        PM_LAUNCH_CHECK();
-        A_bound = 2ll * h.A;
+        if (!graph) {
+            written = (long long)h.nodes_size + 2ll * h.A;
+            A_bound = 2ll * h.A;
+            R_bound = 2ll * h.R;
+        }
         cur = nxt;
     }
     if (graph) {
@@ -1875,12 +2078,12 @@ std::vector<void **> graph_buffers(pm_ctx *ctx) {
         (void **)&a.prim_index.ptr, (void **)&a.bitmap.ptr, (void **)&a.children.ptr, (void **)&a.area.ptr,
         (void **)&a.parent.ptr, (void **)&ctx->prim_indices.ptr, (void **)&ctx->leaf_ids.ptr,
         (void **)&ctx->packed.ptr, (void **)&ctx->leaf_tris.ptr, (void **)&sc.prim_min.ptr,
-        (void **)&sc.prim_max.ptr, (void **)&sc.node_req.ptr, (void **)&sc.node_eaddr.ptr,
-        (void **)&sc.node_nseg.ptr, (void **)&sc.node_segbase.ptr, (void **)&sc.seg_node.ptr,
-        (void **)&sc.seg_start.ptr, (void **)&sc.seg_count.ptr, (void **)&sc.segL.ptr, (void **)&sc.segR.ptr,
-        (void **)&sc.segLL.ptr, (void **)&sc.segLR.ptr, (void **)&sc.segLoff.ptr, (void **)&sc.segRoff.ptr,
-        (void **)&sc.segLLoff.ptr, (void **)&sc.segLRoff.ptr, (void **)&sc.slot_leaf.ptr,
-        (void **)&sc.slot_lna.ptr, (void **)&sc.areas.ptr, (void **)&sc.lower_flag.ptr,
+        (void **)&sc.prim_max.ptr, (void **)&sc.node_eaddr.ptr, (void **)&sc.node_segbase.ptr,
+        (void **)&sc.seg_node.ptr, (void **)&sc.seg_start.ptr, (void **)&sc.seg_count.ptr,
+        (void **)&sc.segL.ptr, (void **)&sc.segR.ptr,
+        (void **)&sc.segLoff.ptr, (void **)&sc.segRoff.ptr, (void **)&sc.node_lna.ptr, (void **)&sc.node_lref.ptr,
+        (void **)&sc.lb_words[0].ptr, (void **)&sc.lb_words[2].ptr,
+        (void **)&sc.lb_ctr.ptr, (void **)&sc.areas.ptr, (void **)&sc.lower_flag.ptr,
         (void **)&sc.lower_off.ptr, (void **)&sc.lower_sets.ptr, (void **)&sc.lower_careas.ptr,
         (void **)&sc.lower_list.ptr, (void **)&sc.tri_rec.ptr, (void **)&ctx->graph_off.ptr, (void **)&sc.lvl.ptr};
     for (int i = 0; i < 2; ++i) {
@@ -1946,13 +2149,16 @@ void capture_batch(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4
         sc.ref_max[i].reserve((size_t)caps.refs + 1, s);
         sc.keys[i].reserve(6 * (size_t)caps.active + 6, s);
     }
-    for (DevBuf<int> *b : {&sc.node_req, &sc.node_eaddr, &sc.node_nseg, &sc.node_segbase})
-        b->reserve((size_t)caps.active + 1, s);
-    sc.slot_leaf.reserve(2 * (size_t)caps.active + 1, s);
-    sc.slot_lna.reserve(2 * (size_t)caps.active + 1, s);
-    for (DevBuf<int> *b : {&sc.seg_node, &sc.seg_start, &sc.seg_count, &sc.segL, &sc.segR, &sc.segLL, &sc.segLR,
-                           &sc.segLoff, &sc.segRoff, &sc.segLLoff, &sc.segLRoff})
+    for (DevBuf<int> *b : {&sc.node_eaddr, &sc.node_segbase}) b->reserve((size_t)caps.active + 1, s);
+    sc.node_lna.reserve((size_t)caps.active + 1, s);
+    sc.node_lref.reserve((size_t)caps.active + 1, s);
+    for (DevBuf<int> *b : {&sc.seg_node, &sc.seg_start, &sc.seg_count, &sc.segL, &sc.segR, &sc.segLoff,
+                           &sc.segRoff})
         b->reserve((size_t)caps.segs + 1, s);
+    const long long node_tiles = (caps.active + NT - 1) / NT + 1;
+    sc.lb_words[0].reserve(2 * (size_t)node_tiles + 2, s);
+    sc.lb_words[2].reserve(6 * (size_t)node_tiles + 6, s);
+    sc.lb_ctr.reserve(4, s);
     sc.prim_min.reserve((size_t)caps.leaf_refs + 1, s);
     sc.prim_max.reserve((size_t)caps.leaf_refs + 1, s);
     ctx->prim_indices.reserve((size_t)caps.leaf_refs + 1, s);

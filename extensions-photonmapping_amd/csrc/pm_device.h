@@ -176,4 +176,48 @@ __device__ __forceinline__ bool woop_test(const float4 &m0, const float4 &m1, co
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
+// PCG32 (O'Neill 2014: pcg32_srandom_r / pcg32_random_r), one stream per
+// (purpose, element): stream = purpose << 40 | element.
+struct Pcg {
+    unsigned long long state, inc;
+    static constexpr unsigned long long MULT = 6364136223846793005ull;
+    __device__ Pcg(unsigned long long seed, unsigned long long stream) {
+        inc = (stream << 1) | 1ull;
+        state = 0ull;
+        state = state * MULT + inc;
+        state += seed;
+        state = state * MULT + inc;
+    }
+    __device__ unsigned int next() {
+        const unsigned long long old = state;
+        state = old * MULT + inc;
+        const unsigned int xs = (unsigned int)(((old >> 18) ^ old) >> 27);
+        const unsigned int rot = (unsigned int)(old >> 59);
+        return (xs >> rot) | (xs << ((32u - rot) & 31u));
+    }
+    // 24 random bits in [0, 1)
+    __device__ float uniform() { return (float)(next() >> 8) * (1.0f / 16777216.0f); }
+};
+
+// v * (1 / |v|), the host branch of cutil_math's normalize (rsqrtf = 1/sqrtf)
+__device__ __forceinline__ f3 normalize3(f3 v) {
+    const float inv = 1.0f / sqrtf(dot3(v, v));
+    return {v.x * inv, v.y * inv, v.z * inv};
+}
+
+// Cosine-weighted direction about the unit normal n (Malley: a uniform point
+// (r cos phi, r sin phi) of the unit disk, r = sqrt(u1), phi = 2 pi u2,
+// lifted to the hemisphere) in the frame t = normalize(a x n), b = n x t.
+__device__ __forceinline__ f3 cosine_dir(f3 n, float u1, float u2) {
+    const float r = sqrtf(u1);
+    const float phi = 6.283185307179586f * u2;
+    float sp, cp;
+    sincosf(phi, &sp, &cp);
+    const float x = r * cp, y = r * sp, z = sqrtf(fmaxf(0.0f, 1.0f - u1));
+    const f3 a = fabsf(n.x) > 0.9f ? f3{0.0f, 1.0f, 0.0f} : f3{1.0f, 0.0f, 0.0f};
+    const f3 t = normalize3(cross3(a, n));
+    const f3 b = cross3(n, t);
+    return normalize3({x * t.x + y * b.x + z * n.x, x * t.y + y * b.y + z * n.y, x * t.z + y * b.z + z * n.z});
+}
+
 }  // namespace pmkd

@@ -118,6 +118,7 @@ struct Lvl {
     int cand_n;                                        // lower level: nodes listed for the full SAH search
     int advance;                                       // the children plan is stored, the level not yet advanced
     int A_scan;                                        // active nodes of the coming node pass
+    int leaf_left;                                     // leaf slots among the level's left children
     // work counts of the build (the algorithmic-bytes formula's inputs)
     unsigned long long ref_reads;                      // refs of every upper level, summed
     unsigned long long ref_writes;                     // refs written by every upper level (children + leaves)
@@ -140,12 +141,13 @@ struct BuildScratch {
     DevBuf<float4> ref_min[2], ref_max[2];     // level ref arrays (ping-pong)
     DevBuf<float4> prim_min, prim_max;         // final (upper leaf) refs
     DevBuf<int> keys[2];                        // 6 ordered-int box keys per active node
-    DevBuf<int> node_req, node_eaddr;           // empty-space nodes per active node
-    DevBuf<int> node_nseg, node_segbase;        // segments per active node
+    DevBuf<int> node_eaddr, node_segbase;       // per active node: first empty-space node, first segment
     DevBuf<int> seg_node, seg_start, seg_count; // segment table
-    DevBuf<int> segL, segR, segLL, segLR;       // per-segment counts (left, right, leaf-left, leaf-right)
-    DevBuf<int> segLoff, segRoff, segLLoff, segLRoff;
-    DevBuf<int> slot_leaf, slot_lna;            // 2A leaf-slot flags and addresses
+    DevBuf<int> segL, segR;                     // per segment: left / right refs
+    DevBuf<int> segLoff, segRoff;               // per segment: left / right refs before it
+    DevBuf<int2> node_lna, node_lref;           // per active node: leaf slots and leaf refs before it (left, right)
+    DevBuf<unsigned long long> lb_words[3];     // look-back status words of the node and leaf passes ([1] unused)
+    DevBuf<int> lb_ctr;                         // their tile tickets
     DevBuf<int> scan_tmp[3];                    // block sums for scans, per level
     DevBuf<int> totals;                         // small device-side counters
     DevBuf<float> areas;                        // lower stage: triangle areas per final ref
@@ -154,6 +156,7 @@ struct BuildScratch {
     DevBuf<long long> lower_sets;               // 2 per node
     DevBuf<float> lower_careas;                 // 2 per node
     DevBuf<int> bounce_flags, bounce_offs;      // photon driver compaction
+    DevBuf<float> light_cdf;                    // photon emission: running sum of the light areas
     DevBuf<unsigned> bwd_keys[2], bwd_vals[2];  // backward: radix sort ping-pong
     DevBuf<float> bwd_corner;                   // backward: 3 float3 per ray
     DevBuf<int> bwd_hist, bwd_offs;             // backward: radix histograms
@@ -244,6 +247,11 @@ void trace_rays(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int
 int photon_bounce(pm_ctx *ctx, const float4 *org, const float4 *dir, const pm_hit *hits, int n,
                   int absorb_from, unsigned long long seed, float offset, float4 *out_org,
                   float4 *out_dir, int *out_src, int *out_count, hipStream_t s);
+void photon_emit(pm_ctx *ctx, const float4 *l0, const float4 *l1, const float4 *l2, int m, int n,
+                 unsigned long long seed, float offset, float4 *out_org, float4 *out_dir, hipStream_t s);
+// pm_camera.hip: inv_view_proj (16) and cam_pos (3) are host arrays
+void create_rays(const float *inv_view_proj, const float *cam_pos, int width, int height,
+                 unsigned long long seed, int jitter, float4 *org, float4 *dir, hipStream_t s);
 // pm_backward.hip
 void trace_backward(pm_ctx *ctx, const float4 *p0, const float4 *p1, const float4 *p2, int ntris,
                     const float4 *org, const float4 *dir, const pm_hit *hits, int n,

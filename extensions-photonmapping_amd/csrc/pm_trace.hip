@@ -711,7 +711,6 @@ struct Light { f3 pos, ambient, diffuse, specular; };
 __device__ __forceinline__ f3 scale3(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
 __device__ __forceinline__ f3 add3(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 __device__ __forceinline__ f3 mul3(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
-__device__ __forceinline__ f3 normalize3(f3 v) { return scale3(v, 1.0f / sqrtf(dot3(v, v))); }
 __device__ __forceinline__ float saturate(float x) { return x > 1.0f ? 1.0f : (x > 0.0f ? x : 0.0f); }
 __device__ __forceinline__ float clamp01(float x) { return fmaxf(0.0f, fminf(x, 1.0f)); }
 

@@ -181,6 +181,46 @@ def pinhole_rays(width, height, eye, target, up=(0.0, 1.0, 0.0), fov_deg=45.0, s
     return _pack(org, 1.0), _pack(d, 0.0)
 
 
+def camera_inverse(eye, target, up=(0.0, 1.0, 0.0), fov_deg=45.0, width=256, height=256,
+                   near=0.1, far=100.0) -> np.ndarray:
+    """The 16 floats IRayTracer's Unproject reads (IRayTracer.cu:45-53) for a
+    pinhole camera at ``eye`` looking at ``target``: the inverse of the
+    OpenGL projection * view matrix, computed in FP64, stored so that
+    m[4*j + i] is row i, column j (the screen point (sx, sy, 1, 1) maps to
+    c_k = m[k] sx + m[4+k] sy + m[8+k] + m[12+k]); screen y = -1 is the
+    bottom row.  Rounded once to float32."""
+    eye = np.asarray(eye, np.float64)
+    f = np.asarray(target, np.float64) - eye
+    f /= np.linalg.norm(f)
+    s = np.cross(f, np.asarray(up, np.float64))
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    view = np.eye(4)
+    view[0, :3], view[1, :3], view[2, :3] = s, u, -f
+    view[:3, 3] = -view[:3, :3] @ eye
+    t = 1.0 / math.tan(math.radians(fov_deg) * 0.5)
+    proj = np.zeros((4, 4))
+    proj[0, 0] = t / (width / height)
+    proj[1, 1] = t
+    proj[2, 2] = (far + near) / (near - far)
+    proj[2, 3] = 2.0 * far * near / (near - far)
+    proj[3, 2] = -1.0
+    inv = np.linalg.inv(proj @ view)
+    return np.ascontiguousarray(inv.T.reshape(16), dtype=np.float32)
+
+
+# the configurations' cameras: (eye, target, fov in degrees)
+CAMERA_C1 = ((0.0, 0.0, 3.5), (0.0, 0.0, 0.0), 45.0)
+CAMERA_C2 = ((0.5, 0.8, 0.0), (0.5, 0.0, 0.55), 32.0)
+CAMERA_C5 = ((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), 45.0)
+
+
+def camera(config_camera, width, height):
+    """(inverse view-projection [16], camera position [3]) of a configuration's camera"""
+    eye, target, fov = config_camera
+    return camera_inverse(eye, target, fov_deg=fov, width=width, height=height), np.asarray(eye, np.float32)
+
+
 def cosine_dirs(normal: np.ndarray, u1: np.ndarray, u2: np.ndarray) -> np.ndarray:
     """Cosine-weighted directions about unit normals (Malley's method)."""
     r = np.sqrt(u1)

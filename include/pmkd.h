@@ -403,6 +403,37 @@ int pm_osc_surface(pm_ctx *ctx, float *vertices, float *normals, int detail, flo
 int pm_deform_radial(pm_ctx *ctx, const float *unit, int nv, float frame, const float *wave,
                      float *out, void *stream);
 
+/* IRayTracer::CreateInitialRays / CreateRays / Unproject (IRayTracer.cu:
+ * 45-100): one camera ray per pixel of a width x height canvas, ray
+ * id = y * width + x, written to org/dir (device float4[width * height]).
+ * inv_view_proj (HOST, 16 floats) is the inverse of view * projection in
+ * the element order Unproject reads (the screen point (s.x, s.y, 1, 1) maps
+ * to c_k = m[k] s.x + m[4+k] s.y + m[8+k] + m[12+k]); cam_pos (HOST, 3
+ * floats) is the camera position.  s = ((x + jx) / width) * 2 - 1, likewise
+ * for y: jitter 0 gives jx = jy = 0 (the reference), jitter 1 a seeded
+ * sub-pixel offset in [0, 1)^2 from PCG32 stream (jitter_seed,
+ * 1 << 40 | id).  Direction = normalize(c.xyz / c.w - cam_pos), w = 0;
+ * origin = (cam_pos, 1).  The caller computes the inverse (the reference
+ * calls OpenEngine's Matrix::GetInverse). */
+int pm_create_rays(pm_ctx *ctx, const float *inv_view_proj, const float *cam_pos, int width, int height,
+                   unsigned long long jitter_seed, int jitter, float *org, float *dir, void *stream);
+
+/* Largest light triangle count pm_photon_emit takes. */
+#define PM_MAX_LIGHTS 65536
+
+/* Photon emission (no reference counterpart; the caller of the photon
+ * trace): n photons from the area lights l0/l1/l2 (device float4[num_lights],
+ * 1 <= num_lights <= PM_MAX_LIGHTS, e.g. a slice of p0/p1/p2).  Photon i
+ * uses PCG32 stream (seed, 5 << 40 | i): it picks light k with probability
+ * proportional to its area (the first k with u0 * total < running area sum,
+ * all sums FP32 in light order), a uniform point p of that triangle and a
+ * cosine-weighted direction about n = normalize((v1 - v0) x (v2 - v0)), the
+ * side the light emits to; origin = p + offset * n (w = 1), direction w = 0,
+ * into out_org/out_dir (device float4[n]).  Lights of zero total area give
+ * undefined rays. */
+int pm_photon_emit(pm_ctx *ctx, const float *l0, const float *l1, const float *l2, int num_lights, int n,
+                   unsigned long long seed, float offset, float *out_org, float *out_dir, void *stream);
+
 /* Photon path continuation (no reference counterpart: the reference's only
  * secondary rays are the shading loop's reflections, Raytracer.cu:219-256).
  * For every ray i with hits[i].id in [0, absorb_from): one diffuse

@@ -1,8 +1,9 @@
 // IRayTracer.hpp -- tracer interface (Utils/CUDA/IRayTracer.h:30-60).
-// Trace() takes device ray arrays (float4 origin/direction) and writes the
-// first hit of each ray, or (with a Shading) the shaded canvas the reference
-// renders (Raytracer.cu:165-262, ShortStack.cu:96-194); the reference
-// builds camera rays itself (IRayTracer.cu:72-100).
+// Trace() takes a Camera (the rays are made on the device as
+// CreateInitialRays makes them, IRayTracer.cu:72-100) or device ray arrays
+// (float4 origin/direction), and writes the first hit of each ray, or (with
+// a Shading) the shaded canvas the reference renders (Raytracer.cu:165-262,
+// ShortStack.cu:96-194).
 #pragma once
 #include "TriangleMap.hpp"
 
@@ -60,6 +61,41 @@ public:
             hip_ok(hipEventElapsedTime(&renderTime_, e0_, e1_), "hipEventElapsedTime");
         }
     }
+    // The camera CreateInitialRays reads from the canvas's viewing volume
+    // (IRayTracer.cu:72-100): the inverse of view * projection as 16 floats
+    // in the element order Unproject reads them (IRayTracer.cu:45-53), the
+    // camera position and the canvas size; jitter adds a seeded sub-pixel
+    // offset (off in the reference).
+    struct Camera {
+        float inv_view_proj[16];
+        float position[3];
+        int width = 0, height = 0;
+        bool jitter = false;
+        unsigned long long jitter_seed = 0;
+    };
+    // CreateInitialRays (IRayTracer.cu:72-100): one ray per pixel into the
+    // tracer's own origin/direction buffers (GetOrigins / GetDirections)
+    void CreateInitialRays(const Camera &cam, hipStream_t stream = nullptr) {
+        const size_t n = (size_t)cam.width * (size_t)cam.height;
+        origin_.Extend(n);
+        direction_.Extend(n);
+        check(Ctx(), pm_create_rays(Ctx(), cam.inv_view_proj, cam.position, cam.width, cam.height,
+                                    cam.jitter_seed, cam.jitter ? 1 : 0, (float *)origin_.GetDeviceData(),
+                                    (float *)direction_.GetDeviceData(), stream));
+    }
+    const float4 *GetOrigins() const { return origin_.GetDeviceData(); }
+    const float4 *GetDirections() const { return direction_.GetDeviceData(); }
+    // Trace(IRenderCanvas*, uchar4*) (IRayTracer.h:47): camera rays, then one
+    // shaded uchar4 per pixel (id = y * width + x)
+    void Trace(const Camera &cam, const Shading &sh, uchar4 *canvas, hipStream_t stream = nullptr) {
+        CreateInitialRays(cam, stream);
+        Trace(origin_.GetDeviceData(), direction_.GetDeviceData(), cam.width * cam.height, sh, canvas, stream);
+    }
+    // ... and the first hit of every camera ray
+    void Trace(const Camera &cam, pm_hit *hits, hipStream_t stream = nullptr) {
+        CreateInitialRays(cam, stream);
+        Trace(origin_.GetDeviceData(), direction_.GetDeviceData(), cam.width * cam.height, hits, stream);
+    }
     // HostTrace (IRayTracer.h:48): one ray, result returned to the host
     pm_hit HostTrace(const float4 &origin, const float4 &direction) {
         DeviceBuffer<float4> o(1), d(1);
@@ -90,6 +126,7 @@ protected:
 
 private:
     IntersectionAlgorithm algorithm_ = MOELLER;
+    DeviceBuffer<float4> origin_, direction_;   // camera rays (IRayTracer::origin/direction)
     bool printTiming_ = false;
     float renderTime_ = 0.0f;
     hipEvent_t e0_ = nullptr, e1_ = nullptr;

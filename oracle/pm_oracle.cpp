@@ -2,7 +2,7 @@
  * pm_oracle.cpp -- CPU oracle for the kd-tree build + traversal path.
  *
  * TEST INFRASTRUCTURE ONLY (the checker).  Compiled with
- *   g++ -O2 -std=c++17 -ffp-contract=off -fopenmp
+ *   g++ -O3 -std=c++17 -ffp-contract=off -fopenmp
  * so every float operation is an individually rounded IEEE-754 binary32
  * operation in source order.  The product kernels are compiled the same way
  * (-ffp-contract=off, correctly rounded division and square root), which is
@@ -1513,6 +1513,47 @@ int orc_trace_backward64(const float *p0, const float *p1, const float *p2, int 
         if (grad_dir) for (int c = 0; c < 3; ++c) grad_dir[3 * r + c] = gd[c];
     }
     return 0;
+}
+
+/* CreateRays + Unproject, IRayTracer.cu:45-70 (host branch: IEEE division,
+ * normalize = v * (1.0f / sqrtf(dot(v, v)))), with the optional seeded
+ * sub-pixel jitter: PCG32 (O'Neill 2014, pcg32_srandom_r/pcg32_random_r)
+ * stream (seed, 1 << 40 | id), two 24-bit uniforms. */
+void orc_create_rays(const float *m, const float *cam, int width, int height, unsigned long long seed,
+                     int jitter, float *org, float *dir) {
+    const long long n = (long long)width * height;
+    const unsigned long long MULT = 6364136223846793005ull;
+    for (long long id = 0; id < n; ++id) {
+        const int x = (int)(id % width), y = (int)(id / width);
+        float jx = 0.0f, jy = 0.0f;
+        if (jitter) {
+            const unsigned long long inc = (((1ull << 40) | (unsigned long long)id) << 1) | 1ull;
+            unsigned long long state = 0ull * MULT + inc;
+            state += seed;
+            state = state * MULT + inc;
+            float u[2];
+            for (int k = 0; k < 2; ++k) {
+                const unsigned long long old = state;
+                state = old * MULT + inc;
+                const uint32_t xs = (uint32_t)(((old >> 18) ^ old) >> 27);
+                const uint32_t rot = (uint32_t)(old >> 59);
+                const uint32_t r = (xs >> rot) | (xs << ((32u - rot) & 31u));
+                u[k] = (float)(r >> 8) * (1.0f / 16777216.0f);
+            }
+            jx = u[0]; jy = u[1];
+        }
+        const float sx = (((float)x + jx) / (float)width) * 2.0f - 1.0f;
+        const float sy = (((float)y + jy) / (float)height) * 2.0f - 1.0f;
+        const float c0 = m[0] * sx + m[4] * sy + m[8] + m[12];
+        const float c1 = m[1] * sx + m[5] * sy + m[9] + m[13];
+        const float c2 = m[2] * sx + m[6] * sy + m[10] + m[14];
+        const float c3 = m[3] * sx + m[7] * sy + m[11] + m[15];
+        const f3 v = sub({c0 / c3, c1 / c3, c2 / c3}, {cam[0], cam[1], cam[2]});
+        const float inv = 1.0f / std::sqrt(dot(v, v));
+        float *o = org + 4 * id, *d = dir + 4 * id;
+        o[0] = cam[0]; o[1] = cam[1]; o[2] = cam[2]; o[3] = 1.0f;
+        d[0] = v.x * inv; d[1] = v.y * inv; d[2] = v.z * inv; d[3] = 0.0f;
+    }
 }
 
 }  // extern "C"

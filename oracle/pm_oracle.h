@@ -184,6 +184,13 @@ int orc_trace_backward64(const float *p0, const float *p1, const float *p2, int 
                          const int *faces, int num_vertices, double *grad_vertices,
                          double *grad_org, double *grad_dir);
 
+/* Camera rays exactly as pm_create_rays makes them (IRayTracer.cu:45-70):
+ * m = inverse view-projection in Unproject's element order, cam = camera
+ * position, jitter 0/1 with PCG32 stream (seed, 1 << 40 | id); org/dir
+ * float[4 * width * height]. */
+void orc_create_rays(const float *m, const float *cam, int width, int height, unsigned long long seed,
+                     int jitter, float *org, float *dir);
+
 #ifdef __cplusplus
 }
 #endif

@@ -128,8 +128,75 @@ def lib():
         L.orc_trace_backward64.restype = C.c_int
         L.orc_tri_area.argtypes = [P, P, P]
         L.orc_tri_area.restype = C.c_float
+        L.orc_create_rays.argtypes = [P, P, C.c_int, C.c_int, C.c_ulonglong, C.c_int, P, P]
+        L.orc_create_rays.restype = None
         _lib = L
     return _lib
+
+
+def create_rays(inv_view_proj, cam_pos, width: int, height: int, jitter_seed=None):
+    """camera rays (IRayTracer.cu:45-70) in FP32: (org, dir) float32 [W*H, 4]"""
+    m = _c(np.asarray(inv_view_proj).reshape(16), np.float32)
+    c = _c(np.asarray(cam_pos).reshape(3), np.float32)
+    n = width * height
+    org = np.zeros((n, 4), np.float32)
+    dirs = np.zeros((n, 4), np.float32)
+    lib().orc_create_rays(_ptr(m), _ptr(c), width, height, (jitter_seed or 0) & (2**64 - 1),
+                          int(jitter_seed is not None), _ptr(org), _ptr(dirs))
+    return org, dirs
+
+
+def _pcg_uniforms(seed: int, purpose: int, n: int, k: int) -> np.ndarray:
+    """k successive 24-bit PCG32 uniforms of streams (seed, purpose << 40 | i),
+    i < n, as float64 [k, n] (each exactly a float32)"""
+    mult = np.uint64(6364136223846793005)
+    inc = ((np.uint64(purpose) << np.uint64(40)) | np.arange(n, dtype=np.uint64)) << np.uint64(1) | np.uint64(1)
+    state = np.zeros(n, np.uint64) * mult + inc
+    state = (state + np.uint64(seed & (2**64 - 1))) * mult + inc
+    out = np.empty((k, n))
+    for j in range(k):
+        old = state
+        state = old * mult + inc
+        xs = (((old >> np.uint64(18)) ^ old) >> np.uint64(27)) & np.uint64(0xFFFFFFFF)
+        rot = old >> np.uint64(59)
+        r = (xs >> rot) | ((xs << ((np.uint64(32) - rot) & np.uint64(31))) & np.uint64(0xFFFFFFFF))
+        out[j] = (r >> np.uint64(8)).astype(np.float64) / 16777216.0
+    return out
+
+
+def photon_emission64(lights, n: int, seed: int, offset: float = 1e-4):
+    """FP64 restatement of pm_photon_emit: light k chosen by the FP32 running
+    area sum (the first k with u0 * total < cdf[k], FP32 as the device does
+    it, so the choice is the same integer), a uniform point
+    (1 - s) v0 + s (1 - u2) v1 + s u2 v2 with s = sqrt(u1), and a
+    cosine-weighted direction (Malley) about n = normalize(e1 x e2) in the
+    frame t = normalize(a x n), b = n x t; origin p + offset n.  Returns
+    (org, dir, light index) with float64 rows."""
+    l0, l1, l2 = (np.asarray(x, np.float32)[:, :3] for x in lights)
+    m = l0.shape[0]
+    cdf = np.zeros(m, np.float32)
+    acc = np.float32(0.0)
+    for k in range(m):
+        acc = np.float32(acc + np.float32(tri_area(l0[k], l1[k], l2[k])))
+        cdf[k] = acc
+    u = _pcg_uniforms(seed, 5, n, 5)
+    target = u[0].astype(np.float32) * cdf[-1]
+    k = np.minimum(np.searchsorted(cdf, target, side="right"), m - 1)
+    v0, v1, v2 = (x[k].astype(np.float64) for x in (l0, l1, l2))
+    s = np.sqrt(u[1])
+    p = (1 - s)[:, None] * v0 + (s * (1 - u[2]))[:, None] * v1 + (s * u[2])[:, None] * v2
+    nrm = np.cross(v1 - v0, v2 - v0)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    r = np.sqrt(u[3])
+    phi = 2.0 * np.pi * u[4]
+    x, y, z = r * np.cos(phi), r * np.sin(phi), np.sqrt(np.maximum(0.0, 1.0 - u[3]))
+    a = np.where(np.abs(nrm[:, 0:1]) > 0.9, np.array([[0.0, 1.0, 0.0]]), np.array([[1.0, 0.0, 0.0]]))
+    t = np.cross(a, nrm)
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    b = np.cross(nrm, t)
+    d = x[:, None] * t + y[:, None] * b + z[:, None] * nrm
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return p + offset * nrm, d, k
 
 
 def _ptr(a: np.ndarray):

@@ -132,6 +132,40 @@ int main() {
     for (int i = 0; i < R; ++i) lit += (w[i].id >= 0) == (px[i].w > 0);
     EXPECT(lit >= R - R / 100);
 
+    // camera rays made on the device (CreateInitialRays): an affine inverse
+    // view-projection whose far point for screen (sx, sy) is
+    // (0.5 sx, 0.5 sy + 0.3, 2), seen from (0, 0.5, 3)
+    Utils::IRayTracer::Camera cam{};
+    const float inv[16] = {0.5f, 0, 0, 0, 0, 0.5f, 0, 0, 0, 0.3f, 2.0f, 0, 0, 0, 0, 1.0f};
+    for (int k = 0; k < 16; ++k) cam.inv_view_proj[k] = inv[k];
+    cam.position[0] = 0.0f; cam.position[1] = 0.5f; cam.position[2] = 3.0f;
+    cam.width = W; cam.height = H;
+    DeviceBuffer<pm_hit> hc(R);
+    ss.Trace(cam, hc.GetDeviceData());
+    hip_ok(hipDeviceSynchronize(), "sync");
+    std::vector<float4> cd(R);
+    hip_ok(hipMemcpy(cd.data(), ss.GetDirections(), R * sizeof(float4), hipMemcpyDeviceToHost), "copy");
+    double worst = 0.0;
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const double sx = x / double(W) * 2 - 1, sy = y / double(H) * 2 - 1;
+            const double v[3] = {0.5 * sx, 0.5 * sy - 0.2, -1.0};
+            const double n = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            const float4 &g = cd[y * W + x];
+            worst = std::fmax(worst, std::fabs(g.x - v[0] / n) + std::fabs(g.y - v[1] / n) + std::fabs(g.z - v[2] / n));
+        }
+    EXPECT(worst < 1e-5);
+    const auto ch = hc.Download();
+    int cam_hits = 0;
+    for (int i = 0; i < R; ++i) cam_hits += ch[i].id >= 0;
+    EXPECT(cam_hits > R / 2);
+    ss.Trace(cam, sh, canvas.GetDeviceData());
+    hip_ok(hipDeviceSynchronize(), "sync");
+    const auto cpx = canvas.Download();
+    int cam_lit = 0;
+    for (int i = 0; i < R; ++i) cam_lit += (ch[i].id >= 0) == (cpx[i].w > 0);
+    EXPECT(cam_lit >= R - R / 100);
+
     std::printf("%s: %d tris, %d nodes, %d/%d hits, kd diff %d, brute diff %d\n",
                 failures ? "FAILED" : "C++ API OK", geom.GetSize(), st.num_nodes, hits, R, diff_kd, diff_brute);
     return failures ? 1 : 0;

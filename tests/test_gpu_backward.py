@@ -4,7 +4,8 @@ Bars: bit-exact against the oracle's FP32 restatement (same operation order,
 per-vertex sums in (ray, corner) order) and bit-identical from run to run;
 at the 65,536-triangle sphere with 512x512 rays, within 1e-3 relative of FP64
 central differences (h = 1e-4, hit triangles held fixed) on 256 sampled
-vertices x 3 coordinates."""
+vertices x 3 coordinates, in aggregate and per coordinate (2e-3 relative
+plus an absolute floor of 1e-4 of the largest component)."""
 import importlib
 
 import numpy as np
@@ -124,6 +125,12 @@ def test_c5_gradient_matches_fp64_central_differences(pm, orc, scenes, torch):
                          s["h_ref"]["t"][m].astype(np.float64), sample)
     err = np.linalg.norm(gv[sample] - fd) / np.linalg.norm(fd)
     assert err <= 1e-3, err
+    # per vertex coordinate: 2e-3 relative with an absolute floor of 1e-4 of
+    # the largest component (the median component is ~1.4e-2 of it, so the
+    # floor binds only near zero); the worst coordinate uses 0.47 of this
+    per = np.abs(gv[sample] - fd) / (2e-3 * np.abs(fd) + 1e-4 * np.abs(fd).max())
+    worst = np.unravel_index(np.argmax(per), per.shape)
+    assert per.max() <= 1.0, (per.max(), sample[worst[0]], worst[1], gv[sample][worst], fd[worst])
     # and against the FP64 adjoint on every vertex
     h64 = s["h"].copy()
     h64["id"][~m] = -1

@@ -136,6 +136,32 @@ def test_canvas_bit_exact(pm, orc, scenes, torch, ctx, algo_name, woop):
     assert (want[:, 3] > 0).mean() > 0.5 and len(np.unique(want[:, 1])) > 20
 
 
+def test_mesh_positions_under_a_general_model_matrix(pm, torch, ctx):
+    """AddMeshKernel's p = M v (GeometryList.cu:197-232) with a general 4x4
+    row-major matrix: every gathered corner equals the FP32 row products
+    (x, y, z, w summed left to right) bit for bit, and the FP64 product within
+    FP32 rounding"""
+    rng = np.random.default_rng(8)
+    nv, nt = 700, 1500
+    verts = np.concatenate([rng.uniform(-3, 3, (nv, 3)), rng.uniform(0.5, 2, (nv, 1))], 1).astype(np.float32)
+    idx = rng.integers(0, nv, 3 * nt).astype(np.int32)
+    model = rng.normal(size=(4, 4)).astype(np.float32)
+    P = [torch.zeros((nt + 3, 4), dtype=torch.float32, device="cuda:0") for _ in range(3)]
+    ctx.add_mesh(dev(torch, verts), dev(torch, idx), *P, offset=3, model=model)
+    tri = idx.reshape(-1, 3)
+    for k in range(3):
+        v = verts[tri[:, k]]
+        want = np.zeros((nt, 4), np.float32)
+        for i in range(4):
+            r = model[i]
+            want[:, i] = ((v[:, 0] * r[0] + v[:, 1] * r[1]) + v[:, 2] * r[2]) + v[:, 3] * r[3]
+        got = P[k].cpu().numpy()
+        assert np.array_equal(got[3:].view(np.int32), want.view(np.int32))
+        assert not got[:3].any()
+        w64 = v.astype(np.float64) @ model.astype(np.float64).T
+        assert np.abs(got[3:] - w64).max() <= 1e-5 * max(1.0, np.abs(w64).max())
+
+
 def test_mesh_attributes_match_numpy(pm, torch, ctx):
     rng = np.random.default_rng(3)
     nv, nt = 500, 800

@@ -105,6 +105,29 @@ def test_c1_tree_and_traces_bit_exact(pm, orc, scenes, torch, ctx, name):
         assert_same_hits(h, r, f"{name} algo={algo} S={ss}")
 
 
+def test_c1_config_256_camera_rays_bit_exact(pm, orc, scenes, torch, ctx):
+    """the Cornell configuration as it is benchmarked: CONFIG preset, 256 x 256
+    jittered camera rays made on the GPU; the tree, the rays and every hit of
+    the short stack (depth 8), kd-restart and brute force against the oracle"""
+    tris, _ = scenes.config1()
+    params = pm.build_params("config")
+    P, got = gpu_build(pm, ctx, torch, tris, params)
+    ref_tree = orc.build(tris, orc_params(orc, params))
+    assert_same_tree(got, ref_tree.arrays(), "C1 config")
+    m, c = scenes.camera(scenes.CAMERA_C1, 256, 256)
+    O, D = ctx.create_rays(m, c, 256, 256, jitter_seed=1)
+    org, dirs = orc.create_rays(m, c, 256, 256, jitter_seed=1)
+    assert np.array_equal(O.cpu().numpy().view(np.int32), org.view(np.int32))
+    assert np.array_equal(D.cpu().numpy().view(np.int32), dirs.view(np.int32))
+    for algo, ss in ((pm.SHORT_STACK, 8), (pm.KD_RESTART, 4)):
+        h = pm.hits_to_numpy(ctx.trace(algo, O, D, pm.trace_params(stack_size=ss)))
+        r = orc.trace(ref_tree, tris, org, dirs, algo, stack_size=ss)
+        assert_same_hits(h, r, f"C1 config 256 algo={algo}")
+        assert (h["id"] >= 0).mean() > 0.9
+    hb = pm.hits_to_numpy(ctx.trace(pm.BRUTE, O, D))
+    assert_same_hits(hb, orc.trace(None, tris, org, dirs, orc.BRUTE), "C1 config 256 brute")
+
+
 def test_c1_brute_force_bit_exact(pm, orc, scenes, torch, ctx):
     tris, (org, dirs) = scenes.config1(width=128, height=128)
     P = [dev(torch, t) for t in tris]
@@ -275,6 +298,33 @@ def test_c2_heightfield_2m_tree_bit_exact(pm, orc, scenes, torch, ctx):
     O, D = dev(torch, org), dev(torch, dirs)
     h = pm.hits_to_numpy(ctx.trace(pm.SHORT_STACK, O, D, pm.trace_params(stack_size=8)))
     assert_same_hits(h, orc.trace(ref_tree, tris, org, dirs, orc.SHORT_STACK, stack_size=8), "C2")
+    # the benchmarked workload: 1920 x 1080 jittered camera rays made on the
+    # GPU, then one cosine bounce ray per hit; both passes in both ray orders
+    W, Hh = 1920, 1080
+    m, c = scenes.camera(scenes.CAMERA_C2, W, Hh)
+    O, D = ctx.create_rays(m, c, W, Hh, jitter_seed=2)
+    org, dirs = orc.create_rays(m, c, W, Hh, jitter_seed=2)
+    assert np.array_equal(O.cpu().numpy().view(np.int32), org.view(np.int32))
+    assert np.array_equal(D.cpu().numpy().view(np.int32), dirs.view(np.int32))
+    ntris = tris[0].shape[0]
+    for order in (pm.RAY_ORDER_INPUT, pm.RAY_ORDER_COHERENT):
+        tp = pm.trace_params(stack_size=8, ray_order=order)
+        H = ctx.trace(pm.SHORT_STACK, O, D, tp)
+        h = pm.hits_to_numpy(H)
+        if order == pm.RAY_ORDER_INPUT:
+            assert_same_hits(h, orc.trace(ref_tree, tris, org, dirs, orc.SHORT_STACK, stack_size=8, threads=8),
+                             "C2 camera")
+            assert (h["id"] >= 0).mean() > 0.9
+            bo, bd, _, nb = ctx.photon_bounce(O, D, H, ntris, seed=2002)
+            assert nb == int((h["id"] >= 0).sum())
+            bo, bd = bo[:nb].contiguous(), bd[:nb].contiguous()
+            rb = orc.trace(ref_tree, tris, bo.cpu().numpy(), bd.cpu().numpy(), orc.SHORT_STACK, stack_size=8,
+                           threads=8)
+        else:
+            assert_same_hits(h, prev, "C2 camera coherent")
+        hb = pm.hits_to_numpy(ctx.trace(pm.SHORT_STACK, bo, bd, tp))
+        assert_same_hits(hb, rb, f"C2 bounce order={order}")
+        prev = h
 
 
 @pytest.mark.parametrize("name", ["reference", "ct1_lower64"])
