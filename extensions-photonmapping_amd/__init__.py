@@ -36,7 +36,8 @@ EXPORTS = ("pm_build_params_default", "pm_trace_params_default", "pm_create", "p
            "pm_tree_stats", "pm_trace", "pm_photon_bounce", "pm_add_mesh", "pm_trace_backward",
            "pm_build_batched", "pm_trace_batched", "pm_osc_surface", "pm_deform_radial",
            "pm_autotune", "pm_batch_graph_capture", "pm_batch_graph_launch", "pm_batch_graph_status",
-           "pm_trace_canvas", "pm_add_mesh_attributes", "pm_create_rays", "pm_photon_emit")
+           "pm_trace_canvas", "pm_add_mesh_attributes", "pm_create_rays", "pm_photon_emit",
+           "pm_trace_noalloc")
 
 
 class PmError(RuntimeError):
@@ -143,6 +144,8 @@ def load_library(path: str = LIB_PATH):
     L.pm_add_mesh_attributes.argtypes = [P, P, P, I, P, I, P, P, P, P, P, P, P, I, P]
     L.pm_create_rays.argtypes = [P, P, P, I, I, C.c_ulonglong, I, P, P, P]
     L.pm_photon_emit.argtypes = [P, P, P, P, I, I, C.c_ulonglong, F, P, P, P]
+    L.pm_trace_noalloc.argtypes = [P, I, P, P, I, P, C.POINTER(TraceParams), P, P]
+    L.pm_trace_noalloc.restype = C.c_int
     for n in ("pm_create", "pm_destroy", "pm_build", "pm_set_geometry", "pm_tree_view",
               "pm_set_tree", "pm_tree_stats", "pm_trace", "pm_photon_bounce", "pm_add_mesh",
               "pm_trace_backward", "pm_build_batched", "pm_trace_batched", "pm_osc_surface",

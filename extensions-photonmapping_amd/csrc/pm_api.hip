@@ -358,6 +358,34 @@ int pm_trace(pm_ctx *ctx, int algo, const float *org, const float *dir, int nray
     });
 }
 
+int pm_trace_noalloc(pm_ctx *ctx, int algo, const float *org, const float *dir, int nrays, pm_hit *out,
+                     const pm_trace_params *params, int *counters, void *stream) {
+    return guarded(ctx, [&] {
+        if (nrays < 0 || (nrays > 0 && (!org || !dir || !out))) throw Error{PM_ERR_ARG, "invalid rays"};
+        if (algo != PM_TRACE_BRUTE && algo != PM_TRACE_KD_RESTART && algo != PM_TRACE_SHORT_STACK)
+            throw Error{PM_ERR_ARG, "unknown trace algorithm"};
+        pm_trace_params prm;
+        if (params) prm = *params; else pm_trace_params_default(&prm);
+        check_trace_params(prm);
+        if (prm.intersection != PM_INTERSECT_MOELLER)
+            throw Error{PM_ERR_UNSUPPORTED, "pm_trace_noalloc traces with MOELLER only (Woop matrices need memory)"};
+        if (pmkd::ray_order_applies(prm, nrays) && nrays > pmkd::NOALLOC_MAX_ORDERED_RAYS)
+            throw Error{PM_ERR_ARG, "pm_trace_noalloc orders at most " +
+                                        std::to_string(pmkd::NOALLOC_MAX_ORDERED_RAYS) +
+                                        " rays coherently (got " + std::to_string(nrays) +
+                                        "); use PM_RAY_ORDER_INPUT or pm_trace"};
+        pmkd::sync_graph_state(ctx, (hipStream_t)stream);   // after a replay: its sizes and status
+        if (algo == PM_TRACE_BRUTE) {
+            if (ctx->ntris > 0 && !ctx->p0) throw Error{PM_ERR_NO_TREE, "no geometry"};
+        } else if (!ctx->has_tree) {
+            throw Error{PM_ERR_NO_TREE, "no tree: call pm_build or pm_set_tree first"};
+        }
+        const pmkd::StaticScratch &ss = pmkd::static_scratch(ctx->device);
+        pmkd::trace_rays(ctx, algo, reinterpret_cast<const float4 *>(org), reinterpret_cast<const float4 *>(dir),
+                         nrays, nullptr, out, prm, reinterpret_cast<int4 *>(counters), (hipStream_t)stream, &ss);
+    });
+}
+
 int pm_trace_batched(pm_ctx *ctx, int algo, const float *org, const float *dir, const int *ray_mesh,
                      int nrays, pm_hit *out, const pm_trace_params *params, int *counters,
                      void *stream) {

@@ -240,9 +240,12 @@ void build_leaf_view(pm_ctx *ctx, float *areas, bool check, hipStream_t s, const
 void trace_canvas(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int nrays, const float4 *n0,
                   const float4 *n1, const float4 *n2, const uchar4 *c0, const pm_light &light, int max_bounces,
                   uchar4 *canvas, const pm_trace_params &prm, hipStream_t s);
+// ext (optional): the work counter and sort buffers to use instead of the
+// context's (nothing is allocated then)
+struct StaticScratch;
 void trace_rays(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int nrays,
                 const int *ray_mesh, pm_hit *out, const pm_trace_params &prm, int4 *counters,
-                hipStream_t s);
+                hipStream_t s, const StaticScratch *ext = nullptr);
 // pm_photon.hip: returns the number of continuation rays
 int photon_bounce(pm_ctx *ctx, const float4 *org, const float4 *dir, const pm_hit *hits, int n,
                   int absorb_from, unsigned long long seed, float offset, float4 *out_org,
@@ -294,6 +297,22 @@ size_t radix_hist_size(long long n);
 // ctx)
 bool ray_order_applies(const pm_trace_params &prm, int nrays);
 void reserve_ray_order(pm_ctx *ctx, int nrays, hipStream_t s);
+// the sort's buffers when they come from the caller instead of the context:
+// keys/vals (nrays + 1 each), hist/offs (radix_hist_size(nrays) each)
+struct OrderScratch {
+    unsigned *keys[2], *vals[2];
+    int *hist, *offs;
+};
 const unsigned *ray_order(pm_ctx *ctx, const float4 *org, const float4 *dir, const int *ray_root, int nrays,
-                          hipStream_t s);
+                          hipStream_t s, const OrderScratch *ext = nullptr);
+// pm_static.hip: the allocation-free trace's scratch (a file-scope device
+// array), looked up once per device; rays the coherent order can take there
+constexpr int NOALLOC_MAX_ORDERED_RAYS = PM_NOALLOC_MAX_ORDERED_RAYS;
+struct StaticScratch {
+    int *work;
+    OrderScratch order;
+};
+// the calling thread's current device's scratch (PM_ERR_HIP if the symbol
+// lookup fails); call with that device current
+const StaticScratch &static_scratch(int device);
 }  // namespace pmkd

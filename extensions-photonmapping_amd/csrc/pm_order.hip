@@ -312,18 +312,25 @@ void reserve_ray_order(pm_ctx *ctx, int nrays, hipStream_t s) {
 }
 
 const unsigned *ray_order(pm_ctx *ctx, const float4 *org, const float4 *dir, const int *ray_root, int nrays,
-                          hipStream_t s) {
+                          hipStream_t s, const OrderScratch *ext) {
     BuildScratch &sc = ctx->scratch;
-    reserve_ray_order(ctx, nrays, s);
+    OrderScratch o;
+    if (ext) {
+        o = *ext;
+    } else {
+        reserve_ray_order(ctx, nrays, s);
+        o = {{sc.order_keys[0].ptr, sc.order_keys[1].ptr}, {sc.order_vals[0].ptr, sc.order_vals[1].ptr},
+             sc.order_hist.ptr, sc.order_offs.ptr};
+    }
     const int nb = sort_tiles(nrays);
     int root_bits = 0;
     while (root_bits < 31 && ((unsigned)(ctx->num_roots - 1) >> root_bits)) ++root_bits;
     ray_key_kernel<<<nb, TB, 0, s>>>(ctx->nodes.bmin.ptr, ctx->nodes.bmax.ptr, ctx->num_roots, root_bits,
-                                     ray_root, org, dir, nrays, nb, sc.order_keys[0].ptr, sc.order_hist.ptr);
+                                     ray_root, org, dir, nrays, nb, o.keys[0], o.hist);
     PM_LAUNCH_CHECK();
-    unsigned *const K[2] = {sc.order_keys[0].ptr, sc.order_keys[1].ptr};
-    unsigned *const V[2] = {sc.order_vals[0].ptr, sc.order_vals[1].ptr};
-    return V[radix_sort_pairs(K, V, nrays, ORDER_BITS, sc.order_hist.ptr, sc.order_offs.ptr, true, true, sc, s)];
+    unsigned *const K[2] = {o.keys[0], o.keys[1]};
+    unsigned *const V[2] = {o.vals[0], o.vals[1]};
+    return V[radix_sort_pairs(K, V, nrays, ORDER_BITS, o.hist, o.offs, true, true, sc, s)];
 }
 
 }  // namespace pmkd

@@ -1016,7 +1016,7 @@ void launch_canvas(bool skip, bool woop, int blocks, hipStream_t s, const TreeDe
 
 void trace_rays(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int nrays,
                 const int *ray_mesh, pm_hit *out, const pm_trace_params &prm, int4 *counters,
-                hipStream_t s) {
+                hipStream_t s, const StaticScratch *ext) {
     if (nrays == 0) return;
     const bool woop = prm.intersection == PM_INTERSECT_WOOP;
     if (algo == PM_TRACE_BRUTE) {
@@ -1040,17 +1040,23 @@ void trace_rays(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int
     const bool skip = prm.leaf_skipping && ctx->has_boxes;
     const int S = prm.stack_size;
     RayBatch R{org, dir, ray_mesh, nullptr, nrays};
-    if (ray_order_applies(prm, nrays)) R.order = ray_order(ctx, org, dir, ray_mesh, nrays, s);
+    if (ray_order_applies(prm, nrays))
+        R.order = ray_order(ctx, org, dir, ray_mesh, nrays, s, ext ? &ext->order : nullptr);
     if (!ctx->num_cus) {
         int v = 0;
         PM_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, ctx->device));
         ctx->num_cus = v > 0 ? v : 1;
     }
-    BuildScratch &sc = ctx->scratch;
-    sc.trace_work.reserve(1, s);
-    clear_int_kernel<<<1, 64, 0, s>>>(sc.trace_work.ptr);
+    int *work = nullptr;
+    if (ext) {
+        work = ext->work;
+    } else {
+        ctx->scratch.trace_work.reserve(1, s);
+        work = ctx->scratch.trace_work.ptr;
+    }
+    clear_int_kernel<<<1, 64, 0, s>>>(work);
     PM_LAUNCH_CHECK();
-    const Launch L{ctx->num_cus, sc.trace_work.ptr};
+    const Launch L{ctx->num_cus, work};
     if (woop) dispatch_flags<true>(T, skip, counters != nullptr, algo, R, S, out, counters, L, s);
     else dispatch_flags<false>(T, skip, counters != nullptr, algo, R, S, out, counters, L, s);
     PM_LAUNCH_CHECK();

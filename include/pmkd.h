@@ -234,6 +234,21 @@ int pm_tree_stats(pm_ctx *ctx, pm_stats *out, void *stream);
 int pm_trace(pm_ctx *ctx, int algo, const float *org, const float *dir, int nrays,
              pm_hit *out, const pm_trace_params *params, int *counters, void *stream);
 
+/* No counterpart in the reference: pm_trace for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments, records and counters as pm_trace;
+ * allocates nothing: the work counter and the coherent-order sort buffers are
+ * a file-scope device array of the library (about 17 MB, held on every GPU the
+ * library runs on until the process ends).  Limits: at most
+ * PM_NOALLOC_MAX_ORDERED_RAYS rays when the coherent order applies (more:
+ * PM_ERR_ARG; use PM_RAY_ORDER_INPUT, which has no limit), MOELLER
+ * intersection only (WOOP: PM_ERR_UNSUPPORTED).  The tree (pm_build /
+ * pm_set_tree) must exist.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (two streams, two threads) race. */
+#define PM_NOALLOC_MAX_ORDERED_RAYS (1 << 20)
+int pm_trace_noalloc(pm_ctx *ctx, int algo, const float *org, const float *dir, int nrays,
+                     pm_hit *out, const pm_trace_params *params, int *counters, void *stream);
+
 /* Batched build: one kd-tree per mesh for M meshes whose triangles are
  * concatenated in p0/p1/p2 (mesh m owns triangles
  * [mesh_offsets[m], mesh_offsets[m+1]); mesh_offsets is HOST int[M+1],
