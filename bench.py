@@ -522,6 +522,9 @@ def main() -> int:
                          "c4 (batched meshes), c5 (differentiable intersection), or 'none'")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched-rebuild comparison and print its record (profiling runs)")
+    ap.add_argument("--builds-only", action="store_true",
+                    help="profiling runs: rebuild the 1M-triangle scene --warmup + --steps times and print the "
+                         "build times, then exit")
     ap.add_argument("--probe", action="store_true",
                     help="counter runs: one build, one short-stack trace of the emission batch and "
                          "a 256 MiB device copy (the HBM-counter calibration), then exit")
@@ -570,6 +573,21 @@ def main() -> int:
     tp = pm.trace_params(stack_size=args.stack, ray_order=pm.RAY_ORDER_COHERENT)
     stream = torch.cuda.current_stream()
     EMIT_SEED = 3
+
+    if args.builds_only:
+        ms = []
+        for i in range(args.warmup + args.steps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            ctx.build(*P, bp)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            if i >= args.warmup:
+                ms.append(e0.elapsed_time(e1))
+            log(f"[bench] build {i + 1}/{args.warmup + args.steps}: {e0.elapsed_time(e1):.3f} ms")
+        print(json.dumps({"builds_only": True, "tris": ntris, "kd_build_ms": [round(x, 4) for x in ms],
+                          "tree": tree_stats(ctx.stats())}), flush=True)
+        return 0
 
     if args.probe:
         ctx.build(*P, bp)
