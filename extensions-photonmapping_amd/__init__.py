@@ -96,7 +96,7 @@ class Stats(C.Structure):
                 ("max_leaf_size", C.c_int), ("upper_levels", C.c_int), ("lower_levels", C.c_int),
                 ("num_prim_indices", C.c_int), ("num_upper_leaves", C.c_int),
                 ("upper_ref_reads", C.c_longlong), ("upper_ref_writes", C.c_longlong),
-                ("lower_prims", C.c_longlong)]
+                ("lower_prims", C.c_longlong), ("planned_build", C.c_int)]
 
 
 _lib = None

@@ -309,6 +309,7 @@ int pm_tree_stats(pm_ctx *ctx, pm_stats *out, void *stream) {
         st.upper_ref_reads = (long long)ctx->ref_reads;
         st.upper_ref_writes = (long long)ctx->ref_writes;
         st.lower_prims = (long long)ctx->lower_prims;
+        st.planned_build = ctx->planned_build ? 1 : 0;
         std::vector<std::pair<int, int>> stk;
         for (int r = ctx->num_roots - 1; r >= 0; --r) stk.push_back({r, 0});
         double depth_sum = 0.0;

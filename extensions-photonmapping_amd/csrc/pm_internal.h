@@ -117,12 +117,14 @@ struct Lvl {
     int An, list_base, use_list, splits, lower_base, lower_levels;   // lower levels
     int cand_n;                                        // lower level: nodes listed for the full SAH search
     int advance;                                       // the children plan is stored, the level not yet advanced
-    int A_scan;                                        // active nodes of the coming node pass
+    int A_scan, ai_scan;                               // active nodes of the coming node pass and the first's index
     int leaf_left;                                     // leaf slots among the level's left children
     // work counts of the build (the algorithmic-bytes formula's inputs)
     unsigned long long ref_reads;                      // refs of every upper level, summed
     unsigned long long ref_writes;                     // refs written by every upper level (children + leaves)
     unsigned long long lower_prims;                    // prims of every node of every SAH lower level
+    // the largest sizes the build reached (the capacities of a later build derive from them)
+    long long seen_active, seen_segs, seen_refs, seen_lower;
     int status;                                        // 0, or the PM_ERR_* that stopped the build
     int max_levels;
     long long max_refs;
@@ -194,6 +196,8 @@ struct pm_ctx {
     int num_roots = 1;                 // meshes in the last build (roots 0..num_roots-1)
     std::vector<int> mesh_offsets;     // triangle offsets per mesh (host copy)
     pmkd::BuildSizes last_sizes;       // what the last eager build reached
+    long long last_ntris = -1;         // its triangle count (-1: no build yet)
+    bool planned_build = false;        // the last build ran with capacities from the one before it
     // captured batch (build + trace replayed as one graph launch)
     hipGraphExec_t graph_exec = nullptr;
     hipStream_t capture_stream = nullptr;

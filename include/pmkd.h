@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define PMKD_ABI_VERSION 2
+#define PMKD_ABI_VERSION 3
 
 /* error codes */
 #define PM_OK                  0
@@ -181,6 +181,10 @@ typedef struct {
      * upper level summed, refs those levels wrote (children + leaves), and
      * prims of every node the SAH lower levels examined */
     long long upper_ref_reads, upper_ref_writes, lower_prims;
+    /* 1 when the last build ran with capacities derived from the build before
+     * it in this context (one read-back at the end), 0 when it was sized level
+     * by level (a read-back per level); the tree is the same either way */
+    int planned_build;
 } pm_stats;
 
 /* Defaults of the TriangleMap constructor (TriangleMap.cu:27-49) and the

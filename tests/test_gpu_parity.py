@@ -213,6 +213,40 @@ def test_rebuild_in_same_context(pm, orc, scenes, torch, ctx):
         assert_same_tree(got, orc.build(tris, orc_params(orc, params)).arrays(), f"rebuild n={n}")
 
 
+@pytest.mark.skip(reason="capacity-planned builds are switched off in the library (PLANNED_BUILDS in "
+                         "pm_build.hip): the 1M-triangle benchmark faulted the GPU with them on")
+def test_planned_builds_match_level_by_level(pm, orc, scenes, torch):
+    """a context that has built before builds with capacities from that build
+    (no read-back per level); one that outgrows them falls back to sizing
+    level by level; the tree is the oracle's either way"""
+    c = pm.Context(0)
+    params = pm.build_params("reference")
+    steps = [("first", scenes.random_soup(3000, seed=41, size=0.1), 0),
+             ("again", scenes.random_soup(3000, seed=41, size=0.1), 1),
+             ("other", scenes.random_soup(2500, seed=43, size=0.12), 1),
+             ("more refs", scenes.random_soup(3000, seed=42, size=0.3), 0),      # outgrows the capacities
+             ("after growth", scenes.random_soup(3000, seed=42, size=0.3), 1),
+             ("far smaller", scenes.random_soup(500, seed=44, size=0.1), 0),     # outside [n/2, 1.25n]
+             ("cornell config", scenes.config1()[0], 0)]
+    for name, tris, planned in steps:
+        p = pm.build_params("config") if name == "cornell config" else params
+        _, got = gpu_build(pm, c, torch, tris, p)
+        assert c.stats()["planned_build"] == planned, name
+        assert_same_tree(got, orc.build(tris, orc_params(orc, p)).arrays(), name)
+    # the same scene under the config preset again: planned
+    tris = scenes.config1()[0]
+    p = pm.build_params("config")
+    _, got = gpu_build(pm, c, torch, tris, p)
+    assert c.stats()["planned_build"] == 1
+    assert_same_tree(got, orc.build(tris, orc_params(orc, p)).arrays(), "cornell again")
+    # a planned attempt that ends in a depth error raises it (5000 coincident triangles)
+    q = [np.repeat(np.array([v], np.float32), 5000, axis=0) for v in ([0, 0, 0, 1], [1, 0, 0, 1], [0, 1, 0, 1])]
+    with pytest.raises(pm.PmError) as e:
+        c.build(*[dev(torch, x) for x in q], params)
+    assert e.value.code == 3                                   # PM_ERR_DEPTH
+    c.close()
+
+
 def test_error_codes(pm, torch, scenes):
     c = pm.Context(0)
     org, dirs = scenes.random_rays(16, seed=1)
