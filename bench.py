@@ -522,12 +522,15 @@ def main() -> int:
                          "c4 (batched meshes), c5 (differentiable intersection), or 'none'")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched-rebuild comparison and print its record (profiling runs)")
-    ap.add_argument("--builds-only", action="store_true",
-                    help="profiling runs: rebuild the 1M-triangle scene --warmup + --steps times and print the "
-                         "build times, then exit")
-    ap.add_argument("--probe", action="store_true",
-                    help="counter runs: one build, one short-stack trace of the emission batch and "
-                         "a 256 MiB device copy (the HBM-counter calibration), then exit")
+    ap.add_argument("--builds-only", nargs="?", const="c3", choices=("c3", "c4"), default=None,
+                    help="profiling runs: rebuild the 1M-triangle scene (c3, the default) or the batched frame of "
+                         "--meshes meshes (c4) --warmup + --steps times, print the build times, then copy 256 MiB "
+                         "on the device (the HBM-counter calibration) and exit")
+    ap.add_argument("--probe", nargs="?", const="c3", choices=("c3", "c2"), default=None,
+                    help="counter runs: c3 (the default) = one build and one short-stack trace of the emission "
+                         "batch; c2 = the heightfield build, its camera rays traced in input order, one cosine "
+                         "bounce ray per hit traced in coherent order; then a 256 MiB device copy (the "
+                         "HBM-counter calibration), then exit")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -549,6 +552,59 @@ def main() -> int:
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     configs = set() if args.configs == "none" else set(args.configs.split(","))
+
+    def calibration_copy() -> int:
+        src_buf = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst_buf = torch.empty_like(src_buf)
+        dst_buf.copy_(src_buf)
+        torch.cuda.synchronize()
+        return src_buf.numel() * 4
+
+    if args.builds_only == "c4":
+        c = scenes.config4(num_meshes=args.meshes, rays_per_mesh=1)
+        ctx = pm.Context(0)
+        U = torch.from_numpy(c["unit"]).to(dev)
+        V = torch.empty_like(U)
+        F = torch.from_numpy(c["faces"].reshape(-1)).to(dev)
+        off = c["tri_off"]
+        n = int(off[-1])
+        P = [torch.empty((n, 4), dtype=torch.float32, device=dev) for _ in range(3)]
+        ctx.deform_radial(U, 1.0, out=V)
+        ctx.add_mesh(V, F, *P)
+        bp = pm.build_params("reference")
+        ms = []
+        for i in range(args.warmup + args.steps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            ctx.build_batched(*P, off, bp)
+            e1.record()
+            torch.cuda.synchronize()
+            if i >= args.warmup:
+                ms.append(e0.elapsed_time(e1))
+            log(f"[bench] batched build {i + 1}/{args.warmup + args.steps}: {e0.elapsed_time(e1):.3f} ms")
+        print(json.dumps({"builds_only": "c4", "meshes": args.meshes, "tris": n,
+                          "kd_build_ms": [round(x, 4) for x in ms], "tree": tree_stats(ctx.stats()),
+                          "copy_bytes": calibration_copy()}), flush=True)
+        return 0
+
+    if args.probe == "c2":
+        W, Hh = 1920, 1080
+        tris = scenes.heightfield(1024, seed=2)
+        ntris = int(tris[0].shape[0])
+        P = [torch.from_numpy(x).to(dev) for x in tris]
+        m, c = scenes.camera(scenes.CAMERA_C2, W, Hh)
+        ctx = pm.Context(0)
+        ctx.build(*P, pm.build_params("reference"))
+        O, D = ctx.create_rays(m, c, W, Hh, jitter_seed=2)
+        H = ctx.trace(pm.SHORT_STACK, O, D, pm.trace_params(stack_size=args.stack, ray_order=pm.RAY_ORDER_INPUT))
+        bo, bd, _, nb = ctx.photon_bounce(O, D, H, ntris, seed=2002)
+        bo, bd = bo[:nb].contiguous(), bd[:nb].contiguous()
+        ctx.trace(pm.SHORT_STACK, bo, bd, pm.trace_params(stack_size=args.stack, ray_order=pm.RAY_ORDER_COHERENT))
+        torch.cuda.synchronize()
+        print(json.dumps({"probe": "c2", "tris": ntris, "primary_rays": int(O.shape[0]), "bounce_rays": int(nb),
+                          "copy_bytes": calibration_copy(), "source_sha": source_sha()}), flush=True)
+        return 0
+
     if args.batched_only:
         print(json.dumps({"batched": bench_batched(pm, scenes, torch, orc, meshes=args.meshes, cpu=False)}),
               flush=True)
@@ -585,19 +641,16 @@ def main() -> int:
             if i >= args.warmup:
                 ms.append(e0.elapsed_time(e1))
             log(f"[bench] build {i + 1}/{args.warmup + args.steps}: {e0.elapsed_time(e1):.3f} ms")
-        print(json.dumps({"builds_only": True, "tris": ntris, "kd_build_ms": [round(x, 4) for x in ms],
-                          "tree": tree_stats(ctx.stats())}), flush=True)
+        print(json.dumps({"builds_only": "c3", "tris": ntris, "kd_build_ms": [round(x, 4) for x in ms],
+                          "tree": tree_stats(ctx.stats()), "copy_bytes": calibration_copy()}), flush=True)
         return 0
 
     if args.probe:
         ctx.build(*P, bp)
         ctx.photon_emit(LIGHT, n0, EMIT_SEED, out_org=O[0], out_dir=D[0])
         ctx.trace(pm.SHORT_STACK, O[0], D[0], tp, out=H)
-        src_buf = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
-        dst_buf = torch.empty_like(src_buf)
-        dst_buf.copy_(src_buf)
         torch.cuda.synchronize()
-        print(json.dumps({"probe": True, "copy_bytes": src_buf.numel() * 4, "rays": n0,
+        print(json.dumps({"probe": True, "copy_bytes": calibration_copy(), "rays": n0,
                           "tris": ntris, "source_sha": source_sha()}), flush=True)
         return 0
 
