@@ -102,6 +102,7 @@ struct TreeDev {
     const float4 *leaf_woop;   // Woop intersection: 3 float4 per leaf slot (m0, m1, m2)
     int num_nodes;
     int num_roots;             // roots are nodes 0..num_roots-1 (one per mesh)
+    long long num_slots;       // leaf slots (3 float4 each in leaf_tris)
     const int *fail;           // device flag: nonzero = the tree is not valid, every ray misses
 };
 

@@ -56,21 +56,6 @@ __device__ __forceinline__ bool mt_edges(f3 v0, f3 e1, f3 e2, f3 o, f3 d, float 
     return true;
 }
 
-// The same test without early exits, for the cooperative leaf test (its
-// lanes test different triangles, so a wave never skips a part): same
-// values, same acceptance.
-__device__ __forceinline__ bool mt_edges_flat(f3 v0, f3 e1, f3 e2, f3 o, f3 d, float th, float &hx, float &hy,
-                                              float &hz) {
-    const f3 p = cross3(d, e2);
-    const f3 t = sub3(o, v0);
-    const f3 q = cross3(t, e1);
-    const float inv = rcp_rn(dot3(p, e1));
-    hx = inv * dot3(q, e2);
-    hy = inv * dot3(p, t);
-    hz = inv * dot3(q, d);
-    return 0.0f <= hx && hx <= th && hy >= 0.0f && hz >= 0.0f && hy + hz <= 1.0f;
-}
-
 // Leaf test (Raytracer.cu:194-216): optional box test, then Moller-Trumbore
 // over the leaf bitmap in ascending bit order; the next triangle's data is
 // loaded before the current one is tested.
@@ -345,8 +330,10 @@ __global__ __launch_bounds__(TB) void trace_kernel(TreeDev T, const float4 *__re
 // (accept 0 <= t <= th, then th = t) ends with -- so each accepted pair
 // offers the key (t bits, ~slot) to its owner's 64-bit LDS minimum, and the
 // owner recomputes its winning triangle itself for t, u, v and the id.
-// Slots of one leaf grow with the bit index, so ~slot orders ties like the
-// bitmap does; t >= 0 here (-0 keyed as +0).
+// The key's low word is ~(byte offset of the slot's record): slots of one
+// leaf grow with the bit index, so it orders ties like the bitmap does; t >=
+// 0 here (-0 keyed as +0).  Record offsets are 32-bit (48 bytes per slot):
+// trees with 2^32 / 48 slots or more take the lane-per-ray kernel.
 constexpr int COOP_WIN = 512;
 static_assert(COOP_WIN % 64 == 0, "a wave reads pair[r + lane] for r < COOP_WIN, r a multiple of 64");
 
@@ -357,9 +344,9 @@ static_assert(COOP_WIN % 64 == 0, "a wave reads pair[r + lane] for r < COOP_WIN,
 // wave is what brings a block under a sixth of the CU's LDS.
 template <bool RECS>
 struct CoopLds {
-    alignas(8) unsigned short pair[COOP_WIN];   // bit << 6 | owner lane
+    alignas(8) unsigned short pair[COOP_WIN];   // bit << 8 | 4 * owner lane
     unsigned long long key[64];
-    float4 org[RECS ? 64 : 1];           // owner's origin, w = its leaf's first slot (int bits)
+    float4 org[RECS ? 64 : 1];           // owner's origin, w = its leaf's first record offset (bits)
     float4 dir[RECS ? 64 : 1];           // owner's direction, w = its leaf exit th
 };
 
@@ -377,14 +364,39 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// every lane of the wave calls it; returns the lane's winning slot or -1
+// One (lane, triangle) pair: the owner's ray against the slot whose record
+// starts `off` bytes into the leaf triangles; an accepted hit offers its key
+// to the owner.  mt_edges' values without its early exits (the lanes test
+// different triangles, so a wave never skips a part) and the same
+// acceptance, written as min(hx, hy, hz) >= 0, hx <= th, hy + hz <= 1 (fminf
+// returns the other operand of a NaN; a NaN hx fails hx <= th, a NaN hy or
+// hz fails hy + hz <= 1).
+__device__ __forceinline__ void coop_pair(const float4 *__restrict__ L, unsigned long long *key, unsigned off,
+                                          f3 ro, f3 rd, float rth) {
+    const float4 *R3 = reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(L) + off);
+    const float4 a0 = R3[0], a1 = R3[1], a2 = R3[2];
+    const f3 v0 = mk3(a0), e1 = mk3(a1), e2 = mk3(a2);
+    const f3 p = cross3(rd, e2);
+    const f3 t = sub3(ro, v0);
+    const f3 q = cross3(t, e1);
+    const float inv = rcp_rn(dot3(p, e1));
+    const float hx = inv * dot3(q, e2);
+    const float hy = inv * dot3(p, t);
+    const float hz = inv * dot3(q, rd);
+    if (fminf(fminf(hx, hy), hz) >= 0.0f && hx <= rth && hy + hz <= 1.0f)
+        atomicMin(key, ((unsigned long long)__float_as_uint(hx + 0.0f) << 32) | (unsigned)~off);
+}
+
+// every lane of the wave calls it with the byte offset of its leaf's first
+// record (3 float4 per slot; every offset fits 32 bits); returns the byte
+// offset of the lane's winning record, or -1
 template <bool RECS>
-__device__ __forceinline__ int coop_leaf(const float4 *__restrict__ L, CoopLds<RECS> &C, unsigned long long bm,
-                                         int base, f3 o, f3 d, float th) {
+__device__ __forceinline__ long long coop_leaf(const float4 *__restrict__ L, CoopLds<RECS> &C,
+                                               unsigned long long bm, unsigned base48, f3 o, f3 d, float th) {
     const int lane = threadIdx.x & 63;
     C.key[lane] = ~0ull;
     if (RECS) {
-        C.org[lane] = make_float4(o.x, o.y, o.z, __int_as_float(base));
+        C.org[lane] = make_float4(o.x, o.y, o.z, __uint_as_float(base48));
         C.dir[lane] = make_float4(d.x, d.y, d.z, th);
     }
     // k <= 64 pairs per lane: the wave's exclusive prefix sum of k, bit by
@@ -399,8 +411,10 @@ __device__ __forceinline__ int coop_leaf(const float4 *__restrict__ L, CoopLds<R
         W += __popcll(m) << b;
     }
     int pos = excl;
+    // a pair is bit << 8 | 4 * owner lane (the owner's ds_bpermute address);
     // a run of bits b0 .. b0+k-1 (every leaf the upper stage makes) is listed
     // four pairs to a 64-bit store; other bitmaps bit by bit
+    const int lane4 = lane << 2;
     const int b0 = bm ? __builtin_ctzll(bm) : 0;
     const unsigned long long run = bm >> b0;
     const bool contig = (run & (run + 1)) == 0ull;
@@ -415,68 +429,72 @@ __device__ __forceinline__ int coop_leaf(const float4 *__restrict__ L, CoopLds<R
             const int iend = wbase + wn - excl < k ? wbase + wn - excl : k;
             int p = excl + i - wbase;
             while (i < iend && (p & 3)) {
-                C.pair[p++] = (unsigned short)((b0 + i) << 6 | lane);
+                C.pair[p++] = (unsigned short)((b0 + i) << 8 | lane4);
                 ++i;
             }
-            unsigned long long v = (unsigned long long)((b0 + i) << 6 | lane) * 0x0001000100010001ull +
-                                   0x00c0008000400000ull;
+            unsigned long long v = (unsigned long long)((b0 + i) << 8 | lane4) * 0x0001000100010001ull +
+                                   0x0300020001000000ull;
             while (i + 4 <= iend) {
                 pair4[p >> 2] = v;
-                v += 256ull * 0x0001000100010001ull;
+                v += 1024ull * 0x0001000100010001ull;
                 p += 4;
                 i += 4;
             }
             while (i < iend) {
-                C.pair[p++] = (unsigned short)((b0 + i) << 6 | lane);
+                C.pair[p++] = (unsigned short)((b0 + i) << 8 | lane4);
                 ++i;
             }
         }
         while (lo && pos < wbase + wn) {
             const int b = __builtin_ctz(lo);
             lo &= lo - 1;
-            C.pair[pos++ - wbase] = (unsigned short)(b << 6 | lane);
+            C.pair[pos++ - wbase] = (unsigned short)(b << 8 | lane4);
         }
         while (hi && pos < wbase + wn) {
             const int b = __builtin_ctz(hi) + 32;
             hi &= hi - 1;
-            C.pair[pos++ - wbase] = (unsigned short)(b << 6 | lane);
+            C.pair[pos++ - wbase] = (unsigned short)(b << 8 | lane4);
         }
         wave_sync();
-        for (int r = 0; r < wn; r += 64) {
-            const int e = r + lane;
+        // the owner's ray, leaf exit and first record offset, for pair pr
+        auto pair_at = [&](int e, f3 &ro, f3 &rd, float &rth, unsigned &off, int &ow4) {
             const int pr = C.pair[e];   // e < COOP_WIN: stale past wn, used only for an in-range lane
-            const int ow = pr & 63;
-            f3 ro, rd;
-            float rth;
-            int rbase;
+            ow4 = pr & 255;
+            unsigned rbase;
             if (RECS) {
-                const float4 a = C.org[ow], b = C.dir[ow];
-                ro = mk3(a); rd = mk3(b); rth = b.w; rbase = __float_as_int(a.w);
+                const float4 a = C.org[ow4 >> 2], b = C.dir[ow4 >> 2];
+                ro = mk3(a); rd = mk3(b); rth = b.w; rbase = __float_as_uint(a.w);
             } else {
-                const int ow4 = ow << 2;
                 ro = {lane_read(o.x, ow4), lane_read(o.y, ow4), lane_read(o.z, ow4)};
                 rd = {lane_read(d.x, ow4), lane_read(d.y, ow4), lane_read(d.z, ow4)};
                 rth = lane_read(th, ow4);
-                rbase = __builtin_amdgcn_ds_bpermute(ow4, base);
+                rbase = (unsigned)__builtin_amdgcn_ds_bpermute(ow4, (int)base48);
             }
-            if (e < wn) {
-                const int sl = rbase + (pr >> 6);
-                // 3 * sl < 2^31 (slots are ints): a 32-bit shift-add and a
-                // zero-extended 64-bit shift-add instead of a 64-bit multiply
-                const float4 *R3 = L + (unsigned)(sl + (sl << 1));
-                const float4 a0 = R3[0], a1 = R3[1], a2 = R3[2];
-                float hx, hy, hz;
-                if (mt_edges_flat(mk3(a0), mk3(a1), mk3(a2), ro, rd, rth, hx, hy, hz)) {
-                    const unsigned long long key =
-                        ((unsigned long long)__float_as_uint(hx + 0.0f) << 32) | (0xffffffffu - (unsigned)sl);
-                    atomicMin(&C.key[ow], key);
-                }
-            }
+            off = (unsigned)(pr >> 8) * 48u + rbase;
+        };
+        // full rounds of 64 pairs, then the window's tail
+        const int full = wn & ~63;
+        int r = 0;
+        for (; r < full; r += 64) {
+            f3 ro, rd;
+            float rth;
+            unsigned off;
+            int ow4;
+            pair_at(r + lane, ro, rd, rth, off, ow4);
+            coop_pair(L, &C.key[ow4 >> 2], off, ro, rd, rth);
+        }
+        if (r < wn) {
+            f3 ro, rd;
+            float rth;
+            unsigned off;
+            int ow4;
+            pair_at(r + lane, ro, rd, rth, off, ow4);
+            if (r + lane < wn) coop_pair(L, &C.key[ow4 >> 2], off, ro, rd, rth);
         }
     }
     wave_sync();
     const unsigned long long best = C.key[lane];
-    return best == ~0ull ? -1 : (int)(0xffffffffu - (unsigned)(best & 0xffffffffu));
+    return best == ~0ull ? -1ll : (long long)(unsigned)~(unsigned)(best & 0xffffffffu);
 }
 
 // the persistent kernel of trace_kernel with the leaf test done by the wave
@@ -562,11 +580,12 @@ __global__ __launch_bounds__(TB) void trace_coop_kernel(TreeDev T, const float4 
             }
         }
         // the leaves, by the whole wave
-        const int win = coop_leaf(T.leaf_tris, C, bm, base, R.o, R.d, R.th);
+        const long long win = coop_leaf(T.leaf_tris, C, bm, (unsigned)base * 48u, R.o, R.d, R.th);
         if (q >= 0 && !done) {
             int hit = -1;
             if (win >= 0) {             // the owner's own test of its winner
-                const float4 *W = T.leaf_tris + 3ll * win;
+                const float4 *W = reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(T.leaf_tris) +
+                                                                   (unsigned)win);
                 const float4 a0 = W[0];
                 if (mt_edges(mk3(a0), mk3(W[1]), mk3(W[2]), R.o, R.d, R.th, R.tu, R.tv))
                     hit = __float_as_int(a0.w);
@@ -873,7 +892,7 @@ int resident_blocks(K kernel, const RayBatch &R, const Launch &L) {
 template <int SMAX, bool SHORT, bool SKIP, bool COUNT, bool WOOP>
 void launch_trace(const TreeDev &T, const RayBatch &R, int S, pm_hit *out, int4 *cnt, const Launch &L,
                   hipStream_t s) {
-    if (!WOOP && PM_COOP) {
+    if (!WOOP && PM_COOP && 48ll * T.num_slots < (1ll << 32)) {
         auto k = trace_coop_kernel<SMAX, SHORT, SKIP, COUNT>;
         k<<<resident_blocks(k, R, L), TB, 0, s>>>(T, R.org, R.dir, R.root, R.order, R.n, S, out, cnt, L.work);
     } else {
@@ -969,6 +988,7 @@ TreeDev tree_dev(pm_ctx *ctx) {
     T.leaf_woop = nullptr;
     T.num_nodes = ctx->num_nodes;
     T.num_roots = ctx->num_roots;
+    T.num_slots = ctx->num_prim_indices;
     T.fail = ctx->trace_fail;
     return T;
 }
