@@ -544,6 +544,7 @@ def main() -> int:
 
     pm = importlib.import_module("extensions-photonmapping_amd")
     scenes = importlib.import_module("extensions-photonmapping_amd.scenes")
+    photons = importlib.import_module("extensions-photonmapping_amd.photons")
     cpu_on = args.cpu_sample > 0
     orc = None
     if cpu_on:
@@ -654,47 +655,35 @@ def main() -> int:
                           "tris": ntris, "source_sha": source_sha()}), flush=True)
         return 0
 
-    def step(timing: dict | None, tp=tp):
+    # the photon driver: emission + passes, the ray order chosen per pass
+    driver = photons.PhotonTracer(ctx, LIGHT, absorb_from, stack_size=args.stack, capacity=n0)
+
+    def step(timing: dict | None, orders=None):
         """one frame: rebuild + emission + all photon passes; returns rays traced"""
-        e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
         e[0].record(stream)
         tb = time.perf_counter()
         ctx.build(*P, bp)
         e[1].record(stream)
         build_wall = time.perf_counter() - tb
-        e[2].record(stream)
-        ctx.photon_emit(LIGHT, n0, EMIT_SEED, out_org=O[0], out_dir=D[0])
-        e[3].record(stream)
-        n, cur, rays = n0, 0, 0
-        passes = []
-        for b in range(args.bounces + 1):
-            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-            ev[0].record(stream)
-            ctx.trace(pm.SHORT_STACK, O[cur][:n], D[cur][:n], tp, out=H[:n])
-            ev[1].record(stream)
-            passes.append((n, ev))
-            rays += n
-            if b == args.bounces:
-                break
-            nxt = cur ^ 1
-            _, _, _, n = ctx.photon_bounce(O[cur][:n], D[cur][:n], H[:n], absorb_from,
-                                           seed=3 + 1000 * (b + 1), out_org=O[nxt], out_dir=D[nxt],
-                                           out_src=src)
-            cur = nxt
-            if n == 0:
-                break
+        passes = driver.trace(n0, args.bounces, EMIT_SEED, orders=orders, timed=timing is not None)
+        rays = sum(p.count for p in passes)
         if timing is not None:
-            torch.cuda.synchronize()
             timing["build_ms"].append(e[0].elapsed_time(e[1]))
-            timing["emit_ms"].append(e[2].elapsed_time(e[3]))
+            timing["emit_ms"].append(driver.emit_ms)
             timing["build_wall_ms"].append(build_wall * 1e3)
-            timing["trace_ms"].append(sum(ev[0].elapsed_time(ev[1]) for _, ev in passes))
-            timing["pass0_ms"].append(passes[0][1][0].elapsed_time(passes[0][1][1]))
+            timing["trace_ms"].append(sum(p.ms for p in passes))
+            timing["pass0_ms"].append(passes[0].ms)
+            timing["pass_ms"].append([p.ms for p in passes])
             timing["rays"].append(rays)
-            timing["pass_rays"] = [p[0] for p in passes]
+            timing["pass_rays"] = [p.count for p in passes]
         return rays
 
-    keys = ("build_ms", "emit_ms", "build_wall_ms", "trace_ms", "pass0_ms", "rays")
+    keys = ("build_ms", "emit_ms", "build_wall_ms", "trace_ms", "pass0_ms", "pass_ms", "rays")
+    ctx.build(*P, bp)
+    pass_orders = driver.tune_ray_orders(n0, args.bounces, EMIT_SEED, reps=1)
+    order_names = ["coherent" if o == pm.RAY_ORDER_COHERENT else "input" for o in pass_orders]
+    log(f"[bench] ray order per pass (tuned): {order_names}")
     for i in range(args.warmup):
         step(None)
         log(f"[bench] warmup {i + 1}/{args.warmup}")
@@ -714,10 +703,9 @@ def main() -> int:
     build_ms = float(np.median(timing["build_ms"]))
     # the same passes traced in input order (no ray sort)
     t_in = {k: [] for k in keys}
-    tp_in = pm.trace_params(stack_size=args.stack, ray_order=pm.RAY_ORDER_INPUT)
-    step(None, tp_in)
+    step(None, [pm.RAY_ORDER_INPUT])
     for i in range(2):
-        step(t_in, tp_in)
+        step(t_in, [pm.RAY_ORDER_INPUT])
     input_mrays = sum(t_in["rays"]) / (sum(t_in["trace_ms"]) * 1e-3) / 1e6
     log(f"[bench] input ray order: {input_mrays:.1f} Mrays/s")
 
@@ -812,7 +800,12 @@ def main() -> int:
             "kd_build_graph_ms": round(build_graph_ms, 3),
             "emit_ms": round(float(np.median(timing["emit_ms"])), 4),
             "trace_ms_per_step": round(trace_ms_total / args.steps, 3),
-            "ray_order": "coherent (spatial-key radix sort inside each trace call, included in the times)",
+            "ray_order": {"per_pass": order_names,
+                          "chosen_by": "the photon driver's tuning run (both orders timed per pass, before the "
+                                       "timed steps); coherent = spatial-key radix sort inside the trace call, "
+                                       "included in the times"},
+            "pass_ms": [round(float(np.median([t[b] for t in timing["pass_ms"]])), 3)
+                        for b in range(len(timing["pass_rays"]))],
             "input_order_mrays": round(input_mrays, 2),
             "input_order_trace_ms_per_step": round(sum(t_in["trace_ms"]) / len(t_in["trace_ms"]), 3),
             "kd_restart_mrays": round(kd_mrays, 2),

@@ -118,6 +118,8 @@ public:
     // trace time of the last Trace() in ms (with PrintTiming(true))
     float GetRenderTime() const { return renderTime_; }
     pm_trace_params &Params() { return params_; }
+    // the context the tracer runs in (its tree and triangles)
+    pm_ctx *Context() const { return Ctx(); }
 
 protected:
     virtual pm_ctx *Ctx() const = 0;

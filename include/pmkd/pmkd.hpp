@@ -6,6 +6,7 @@
 #include "GeometryList.hpp"
 #include "IRayTracer.hpp"
 #include "KDNode.hpp"
+#include "PhotonTracer.hpp"
 #include "RayTracer.hpp"
 #include "ShortStack.hpp"
 #include "TriangleMap.hpp"

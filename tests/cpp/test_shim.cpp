@@ -166,6 +166,37 @@ int main() {
     for (int i = 0; i < R; ++i) cam_lit += (ch[i].id >= 0) == (cpx[i].w > 0);
     EXPECT(cam_lit >= R - R / 100);
 
+    // photon driver: the box's y = +1 face (triangles 6, 7, normal -y) emits
+    // into the closed box; nothing absorbs, so every pass hits a wall or the
+    // grid and the next pass continues all of its rays
+    {
+        const float4 *P0 = geom.GetP0Data(), *P1 = geom.GetP1Data(), *P2 = geom.GetP2Data();
+        Utils::PhotonTracer pt(ss, P0 + 6, P1 + 6, P2 + 6, 2, geom.GetSize());
+        const int NP = 20000;
+        const std::vector<int> rays = pt.Trace(NP, 2, 77);
+        hip_ok(hipDeviceSynchronize(), "sync");
+        EXPECT(rays.size() == 3 && rays[0] == NP);
+        // each pass's hits are a plain trace of that pass's rays, and the next
+        // pass continues every hit (a closed box: all but crack-slipping rays)
+        for (int b = 0; b < 3 && b < (int)rays.size(); ++b) {
+            DeviceBuffer<pm_hit> hp(rays[b]);
+            ss.Trace(pt.GetOrigins(b), pt.GetDirections(b), rays[b], hp.GetDeviceData());
+            std::vector<pm_hit> a(rays[b]), w = hp.Download();
+            hip_ok(hipMemcpy(a.data(), pt.GetHits(b), rays[b] * sizeof(pm_hit), hipMemcpyDeviceToHost), "copy");
+            int same = 0, hit = 0;
+            for (int i = 0; i < rays[b]; ++i) {
+                same += a[i].id == w[i].id && a[i].t == w[i].t;
+                hit += a[i].id >= 0;
+            }
+            EXPECT(same == rays[b] && hit >= rays[b] - rays[b] / 1000);
+            if (b + 1 < (int)rays.size()) EXPECT(rays[b + 1] == hit);
+        }
+        const std::vector<int> orders = pt.TuneRayOrders(NP, 2, 77);
+        EXPECT(orders.size() == 3);
+        pt.Trace(NP, 2, 77, nullptr, true);
+        EXPECT(pt.GetPassTimes().size() == 3);
+    }
+
     std::printf("%s: %d tris, %d nodes, %d/%d hits, kd diff %d, brute diff %d\n",
                 failures ? "FAILED" : "C++ API OK", geom.GetSize(), st.num_nodes, hits, R, diff_kd, diff_brute);
     return failures ? 1 : 0;

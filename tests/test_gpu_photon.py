@@ -88,3 +88,51 @@ def test_bounce_compaction_beyond_1024_scan_blocks(pm, scenes):
     assert np.array_equal(src[:cnt].cpu().numpy(), keep)
     assert torch.isfinite(o2[:cnt]).all() and torch.isfinite(d2[:cnt]).all()
     ctx.close()
+
+
+def test_photon_driver_passes(pm, orc, scenes):
+    """the photon driver (photons.PhotonTracer) runs emission + bounce passes:
+    each pass's hits equal a direct pm_trace of that pass's rays, in either
+    ray order (and the oracle's for the emission pass); each pass continues
+    exactly the previous pass's hits below absorb_from; tuning picks one
+    valid order per pass"""
+    import importlib
+    import torch
+    photons = importlib.import_module("extensions-photonmapping_amd.photons")
+    tris = scenes.cluster_soup(100_000, seed=3)
+    dev = "cuda:0"
+    P = [torch.from_numpy(t).to(dev) for t in tris]
+    ntris = tris[0].shape[0]
+    ctx = pm.Context(0)
+    ctx.build(*P, pm.build_params("reference"))
+    lights = [p[ntris - 2:].contiguous() for p in P]
+    drv = photons.PhotonTracer(ctx, lights, ntris - 2, stack_size=8)
+    n = 200_000          # above the coherent order's threshold
+    ref = None
+    for orders in ([pm.RAY_ORDER_COHERENT], [pm.RAY_ORDER_INPUT], [pm.RAY_ORDER_INPUT, pm.RAY_ORDER_COHERENT]):
+        passes = drv.trace(n, 3, seed=3, orders=orders)
+        assert passes[0].count == n and len(passes) == 4
+        got = [pm.hits_to_numpy(p.hits).copy() for p in passes]
+        if ref is None:
+            ref = got
+            tree = orc.build(tris, orc.build_params("reference"))
+            o0, d0 = passes[0].org.cpu().numpy(), passes[0].dirs.cpu().numpy()
+            want = orc.trace(tree, tris, o0, d0, orc.SHORT_STACK, stack_size=8, threads=8)
+            assert np.array_equal(got[0].view(np.int32), want.view(np.int32))
+        for a, b in zip(got, ref):
+            assert np.array_equal(a.view(np.int32), b.view(np.int32)), orders
+        for b in range(1, len(passes)):
+            prev = got[b - 1]
+            keep = np.nonzero((prev["id"] >= 0) & (prev["id"] < ntris - 2))[0]
+            assert passes[b].count == keep.shape[0]
+            assert np.array_equal(passes[b].src.cpu().numpy(), keep)
+            direct = ctx.trace(pm.SHORT_STACK, passes[b].org, passes[b].dirs, pm.trace_params(stack_size=8))
+            assert torch.equal(direct.view(torch.int32), passes[b].hits.view(torch.int32))
+    best = drv.tune_ray_orders(n, 3, seed=3, reps=1)
+    assert len(best) == 4 and set(best) <= {pm.RAY_ORDER_INPUT, pm.RAY_ORDER_COHERENT}
+    timed = drv.trace(n, 3, seed=3, timed=True)
+    assert all(p.ms > 0 for p in timed) and drv.emit_ms > 0
+    assert [p.order for p in timed] == best
+    with pytest.raises(ValueError):
+        drv.set_ray_orders([7])
+    ctx.close()
