@@ -195,6 +195,11 @@ def bench_heightfield(pm, scenes, torch, orc, reps: int = 5, stack: int = 8, cpu
     build_ms = timed(torch, lambda: ctx.build(*P, bp), reps)
     st = ctx.stats()
     primary_ms = timed(torch, lambda: ctx.trace(pm.SHORT_STACK, O, D, t_in, out=H), reps)
+    # the same rays with the descent capped at 4 inner nodes per round of the
+    # cooperative leaf test (pm_trace_params.step_cap; same records)
+    t_in4 = pm.trace_params(stack_size=stack, ray_order=pm.RAY_ORDER_INPUT, step_cap=4)
+    t_co4 = pm.trace_params(stack_size=stack, ray_order=pm.RAY_ORDER_COHERENT, step_cap=4)
+    primary4_ms = timed(torch, lambda: ctx.trace(pm.SHORT_STACK, O, D, t_in4, out=H), reps)
     cnt = torch.zeros((n, 4), dtype=torch.int32, device=dev)
     ctx.trace(pm.SHORT_STACK, O, D, t_in, out=H, counters=cnt)
     primary_counts = ray_counts(cnt)
@@ -203,16 +208,22 @@ def bench_heightfield(pm, scenes, torch, orc, reps: int = 5, stack: int = 8, cpu
     HB = torch.empty((max(nb, 1), 4), dtype=torch.float32, device=dev)
     bounce_ms = timed(torch, lambda: ctx.trace(pm.SHORT_STACK, bo, bd, t_co, out=HB[:nb]), reps)
     bounce_in_ms = timed(torch, lambda: ctx.trace(pm.SHORT_STACK, bo, bd, t_in, out=HB[:nb]), reps)
-    best_bounce = min(bounce_ms, bounce_in_ms)
+    bounce4_ms = timed(torch, lambda: ctx.trace(pm.SHORT_STACK, bo, bd, t_co4, out=HB[:nb]), reps)
+    bounce4_in_ms = timed(torch, lambda: ctx.trace(pm.SHORT_STACK, bo, bd, t_in4, out=HB[:nb]), reps)
+    best_primary = min(primary_ms, primary4_ms)
+    best_bounce = min(bounce_ms, bounce_in_ms, bounce4_ms, bounce4_in_ms)
     rec = {"workload": f"heightfield 1024x1024 ({ntris} tris), {W}x{Hh} jittered camera rays made on the GPU "
                        f"+ one cosine bounce ray per hit, short stack depth {stack}",
            "tris": ntris, "kd_build_ms": round(build_ms, 3), "tree": tree_stats(st),
            "camera_rays_ms": round(rays_ms, 4),
-           "primary_rays": n, "primary_ms": round(primary_ms, 3),
-           "primary_mrays": round(n / (primary_ms * 1e-3) / 1e6, 2), "primary_counts": primary_counts,
+           "primary_rays": n, "primary_ms": round(primary_ms, 3), "primary_ms_step_cap4": round(primary4_ms, 3),
+           "primary_mrays": round(n / (best_primary * 1e-3) / 1e6, 2), "primary_counts": primary_counts,
            "bounce_rays": int(nb), "bounce_ms_coherent": round(bounce_ms, 3),
            "bounce_ms_input_order": round(bounce_in_ms, 3),
-           "trace_mrays": round((n + nb) / ((primary_ms + best_bounce) * 1e-3) / 1e6, 2)}
+           "bounce_ms_coherent_step_cap4": round(bounce4_ms, 3),
+           "bounce_ms_input_order_step_cap4": round(bounce4_in_ms, 3),
+           "trace_setting": "each pass at the fastest of its timed settings (ray order x step cap)",
+           "trace_mrays": round((n + nb) / ((best_primary + best_bounce) * 1e-3) / 1e6, 2)}
     if cpu:
         th = omp_threads()
         tree = [None]

@@ -57,7 +57,7 @@ class BuildParams(C.Structure):
 
 class TraceParams(C.Structure):
     _fields_ = [("leaf_skipping", C.c_int), ("stack_size", C.c_int), ("brute_cap", C.c_int),
-                ("intersection", C.c_int), ("ray_order", C.c_int)]
+                ("intersection", C.c_int), ("ray_order", C.c_int), ("step_cap", C.c_int)]
 
 
 class Tree(C.Structure):
@@ -86,7 +86,7 @@ class GraphCaps(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("chunk_size", C.c_int), ("lower_size", C.c_int), ("stack_size", C.c_int),
                 ("build_ms", C.c_float), ("trace_ms", C.c_float), ("trials", C.c_int),
-                ("ray_order", C.c_int)]
+                ("ray_order", C.c_int), ("step_cap", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -304,7 +304,7 @@ class Context:
         bp = BuildParams.from_buffer_copy(base)
         bp.chunk_size, bp.lower_size = t.chunk_size, t.lower_size
         res["build_params"] = bp
-        res["trace_params"] = trace_params(stack_size=t.stack_size, ray_order=t.ray_order)
+        res["trace_params"] = trace_params(stack_size=t.stack_size, ray_order=t.ray_order, step_cap=t.step_cap)
         return res
 
     # -- one graph launch per frame

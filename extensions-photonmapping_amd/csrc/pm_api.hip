@@ -54,6 +54,7 @@ void check_trace_params(const pm_trace_params &p) {
         throw Error{PM_ERR_ARG, "unknown intersection algorithm"};
     if (p.ray_order != PM_RAY_ORDER_INPUT && p.ray_order != PM_RAY_ORDER_COHERENT)
         throw Error{PM_ERR_ARG, "unknown ray order"};
+    if (p.step_cap < 0 || p.step_cap > 4096) throw Error{PM_ERR_ARG, "step_cap must be in [0, 4096]"};
 }
 
 }  // namespace
@@ -83,6 +84,7 @@ void pm_trace_params_default(pm_trace_params *p) {
     p->brute_cap = 0;
     p->intersection = PM_INTERSECT_MOELLER;
     p->ray_order = PM_RAY_ORDER_INPUT;
+    p->step_cap = 0;
 }
 
 int pm_create(pm_ctx **out, int device) {

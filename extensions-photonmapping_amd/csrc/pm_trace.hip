@@ -153,23 +153,34 @@ __device__ __forceinline__ pm_hit ray_record(const RayState &R, int hit) {
 // far nodes, rows SMAX.. their tNext bits; `stk` points at this lane's column
 // and R.nxt counts in rows of TB, so a slot's two words are one register
 // offset apart by a constant.
-template <int SMAX, bool SHORT, bool COUNT>
-__device__ __forceinline__ bool descend(const TreeDev &T, RayState &R, int S, int *stk, int4 &cnt, int &node,
-                                        int4 &nd) {
+// The descent is split in two so that the cooperative kernel can spread it
+// over several loop iterations: pass_start picks the pass's first node (the
+// root or the popped far child; false: the visit bound is spent, a miss),
+// descend_steps walks at most `cap` inner nodes from `node` (1: a leaf, nd
+// its record and R.th its exit; 0: still inner, node/tNext/steps carry on;
+// -1: the descent bound is spent, a miss).
+template <int SMAX, bool SHORT>
+__device__ __forceinline__ bool pass_start(const TreeDev &T, RayState &R, int S, int *stk, int &node,
+                                           float &tNext) {
     if (R.visits++ > T.num_nodes) return false;
-    float tNext;
     if (!SHORT || R.count == 0) {
-        node = R.root_id; tNext = F_INF; nd = T.packed[R.root_id];
+        node = R.root_id; tNext = F_INF;
     } else {                                     // Pop, ShortStack.h:68-72
         R.nxt = (R.nxt == 0 ? S * TB : R.nxt) - TB;
         R.count--;
         node = stk[R.nxt];
         tNext = __int_as_float(stk[R.nxt + SMAX * TB]);
-        nd = T.packed[node];
     }
-    int steps = 0;
-    while (nd.w != LEAF) {
-        if (++steps > MAX_DESCENT) return false;
+    return true;
+}
+
+template <int SMAX, bool SHORT, bool COUNT>
+__device__ __forceinline__ int descend_steps(const TreeDev &T, RayState &R, int S, int *stk, int4 &cnt, int &node,
+                                             float &tNext, int &steps, int cap, int4 &nd) {
+    nd = T.packed[(unsigned)node];
+    for (int k = 0; nd.w != LEAF; ++k) {
+        if (k == cap) return 0;
+        if (++steps > MAX_DESCENT) return -1;
         if (COUNT) cnt.x++;
         const float s = __int_as_float(nd.x);
         const float ori = comp3(R.o, nd.w), dd = comp3(R.inv, nd.w);
@@ -194,7 +205,16 @@ __device__ __forceinline__ bool descend(const TreeDev &T, RayState &R, int S, in
         nd = T.packed[(unsigned)node];   // child indices are in [0, num_nodes)
     }
     R.th = tNext * 1.00001f;
-    return true;
+    return 1;
+}
+
+template <int SMAX, bool SHORT, bool COUNT>
+__device__ __forceinline__ bool descend(const TreeDev &T, RayState &R, int S, int *stk, int4 &cnt, int &node,
+                                        int4 &nd) {
+    float tNext;
+    int steps = 0;
+    if (!pass_start<SMAX, SHORT>(T, R, S, stk, node, tNext)) return false;
+    return descend_steps<SMAX, SHORT, COUNT>(T, R, S, stk, cnt, node, tNext, steps, INT_MAX, nd) > 0;
 }
 
 template <int SMAX, bool SHORT, bool SKIP, bool COUNT, bool WOOP>
@@ -497,14 +517,18 @@ __device__ __forceinline__ long long coop_leaf(const float4 *__restrict__ L, Coo
     return best == ~0ull ? -1ll : (long long)(unsigned)~(unsigned)(best & 0xffffffffu);
 }
 
-// the persistent kernel of trace_kernel with the leaf test done by the wave
-template <int SMAX, bool SHORT, bool SKIP, bool COUNT>
+// the persistent kernel of trace_kernel with the leaf test done by the wave.
+// CAPPED: a lane's descent walks at most `cap` inner nodes per iteration and
+// goes on in the next (pm_trace_params.step_cap); the nodes visited, and so
+// records and counters, are the same.
+template <int SMAX, bool SHORT, bool SKIP, bool COUNT, bool CAPPED>
 __global__ __launch_bounds__(TB) void trace_coop_kernel(TreeDev T, const float4 *__restrict__ org,
                                                         const float4 *__restrict__ dir,
                                                         const int *__restrict__ ray_root,
                                                         const unsigned *__restrict__ order, int n, int S,
                                                         pm_hit *__restrict__ out,
-                                                        int4 *__restrict__ counters, int *__restrict__ work) {
+                                                        int4 *__restrict__ counters, int *__restrict__ work,
+                                                        int cap) {
     __shared__ int stk_rows[2 * (SHORT ? SMAX : 1)][TB];   // short stack: node rows, then tNext rows
     int *const stk = &stk_rows[0][threadIdx.x];
     __shared__ CoopLds<!SHORT> coop[TB / 64];
@@ -514,6 +538,10 @@ __global__ __launch_bounds__(TB) void trace_coop_kernel(TreeDev T, const float4 
     int next = 0, end = 0;          // the wave's chunk (same in every lane)
     bool exhausted = false;
     int q = -1;                     // this lane's ray, -1 = idle
+    // a descent in progress (mid): its node, the far limit tNext, its steps
+    bool mid = false;
+    int cur = 0, steps = 0;
+    float tnext = 0.0f;
     RayState R;
     R.o = {0.0f, 0.0f, 0.0f}; R.d = R.o; R.th = 0.0f;
     int4 cnt = make_int4(0, 0, 0, 0);
@@ -556,16 +584,32 @@ __global__ __launch_bounds__(TB) void trace_coop_kernel(TreeDev T, const float4 
             if (exhausted) break;
             continue;
         }
-        // inner nodes, lane by lane
-        bool done = false;
+        // inner nodes, lane by lane (CAPPED: at most `cap` per iteration)
+        bool done = false, at_leaf = false;
         unsigned long long bm = 0;
         int base = 0;
         if (q >= 0) {
-            int node;
             int4 nd;
-            if (!descend<SMAX, SHORT, COUNT>(T, R, S, stk, cnt, node, nd)) {
-                done = true;
+            int r = 0;
+            if (!CAPPED) {
+                r = descend<SMAX, SHORT, COUNT>(T, R, S, stk, cnt, cur, nd) ? 1 : -1;
             } else {
+                if (!mid) {
+                    if (pass_start<SMAX, SHORT>(T, R, S, stk, cur, tnext)) {
+                        mid = true;
+                        steps = 0;
+                    } else {
+                        r = -1;
+                    }
+                }
+                if (mid) r = descend_steps<SMAX, SHORT, COUNT>(T, R, S, stk, cnt, cur, tnext, steps, cap, nd);
+                if (r != 0) mid = false;
+            }
+            if (r < 0) {
+                done = true;
+            } else if (r > 0) {
+                const int node = cur;
+                at_leaf = true;
                 if (COUNT) cnt.y++;
                 bool enter = true;
                 if (SKIP) {
@@ -581,7 +625,7 @@ __global__ __launch_bounds__(TB) void trace_coop_kernel(TreeDev T, const float4 
         }
         // the leaves, by the whole wave
         const long long win = coop_leaf(T.leaf_tris, C, bm, (unsigned)base * 48u, R.o, R.d, R.th);
-        if (q >= 0 && !done) {
+        if (q >= 0 && at_leaf) {
             int hit = -1;
             if (win >= 0) {             // the owner's own test of its winner
                 const float4 *W = reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(T.leaf_tris) +
@@ -596,7 +640,7 @@ __global__ __launch_bounds__(TB) void trace_coop_kernel(TreeDev T, const float4 
                 if (COUNT) counters[q] = cnt;
                 q = -1;
             }
-        } else if (q >= 0) {            // done before a leaf: a miss
+        } else if (q >= 0 && done) {    // done before a leaf: a miss
             out[q] = miss_record();
             if (COUNT) counters[q] = cnt;
             q = -1;
@@ -876,6 +920,7 @@ struct RayBatch {
 struct Launch {
     int num_cus;
     int *work;
+    int step_cap;   // pm_trace_params.step_cap (cooperative kernels)
 };
 
 template <class K>
@@ -893,8 +938,14 @@ template <int SMAX, bool SHORT, bool SKIP, bool COUNT, bool WOOP>
 void launch_trace(const TreeDev &T, const RayBatch &R, int S, pm_hit *out, int4 *cnt, const Launch &L,
                   hipStream_t s) {
     if (!WOOP && PM_COOP && 48ll * T.num_slots < (1ll << 32)) {
-        auto k = trace_coop_kernel<SMAX, SHORT, SKIP, COUNT>;
-        k<<<resident_blocks(k, R, L), TB, 0, s>>>(T, R.org, R.dir, R.root, R.order, R.n, S, out, cnt, L.work);
+        if (L.step_cap > 0) {
+            auto k = trace_coop_kernel<SMAX, SHORT, SKIP, COUNT, true>;
+            k<<<resident_blocks(k, R, L), TB, 0, s>>>(T, R.org, R.dir, R.root, R.order, R.n, S, out, cnt, L.work,
+                                                      L.step_cap);
+        } else {
+            auto k = trace_coop_kernel<SMAX, SHORT, SKIP, COUNT, false>;
+            k<<<resident_blocks(k, R, L), TB, 0, s>>>(T, R.org, R.dir, R.root, R.order, R.n, S, out, cnt, L.work, 0);
+        }
     } else {
         auto k = trace_kernel<SMAX, SHORT, SKIP, COUNT, WOOP>;
         k<<<resident_blocks(k, R, L), TB, 0, s>>>(T, R.org, R.dir, R.root, R.order, R.n, S, out, cnt, L.work);
@@ -1076,7 +1127,7 @@ void trace_rays(pm_ctx *ctx, int algo, const float4 *org, const float4 *dir, int
     }
     clear_int_kernel<<<1, 64, 0, s>>>(work);
     PM_LAUNCH_CHECK();
-    const Launch L{ctx->num_cus, work};
+    const Launch L{ctx->num_cus, work, prm.step_cap};
     if (woop) dispatch_flags<true>(T, skip, counters != nullptr, algo, R, S, out, counters, L, s);
     else dispatch_flags<false>(T, skip, counters != nullptr, algo, R, S, out, counters, L, s);
     PM_LAUNCH_CHECK();

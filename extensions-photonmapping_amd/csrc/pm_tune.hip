@@ -46,6 +46,7 @@ void autotune(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2,
     static const int kChunks[] = {256, 512, 1024, 2048};
     static const int kLeaf[] = {8, 16, 24, 32, 48, 64};
     static const int kStack[] = {4, 8, 16};
+    static const int kCap[] = {0, 4};
     DevBuf<pm_hit> hits;
     hits.reserve((size_t)std::max(nrays, 1), s);
     Events ev;
@@ -64,10 +65,11 @@ void autotune(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2,
         }
         return median(t);
     };
-    auto time_trace = [&](int stack, int order) {
+    auto time_trace = [&](int stack, int order, int cap) {
         pm_trace_params q = tp;
         q.stack_size = stack;
         q.ray_order = order;
+        q.step_cap = cap;
         std::vector<float> t;
         for (int r = 0; r <= reps; ++r) {
             PM_HIP(hipEventRecord(ev.a, s));
@@ -106,28 +108,31 @@ void autotune(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2,
             if (e.code != PM_ERR_DEPTH) throw;
             continue;
         }
-        const float t = time_trace(8, PM_RAY_ORDER_INPUT);
+        const float t = time_trace(8, PM_RAY_ORDER_INPUT, 0);
         ++trials;
         const float cost = builds_per_trace * b + t;
         if (first || cost < best_cost) { best_cost = cost; best_leaf = ls; best_b = b; first = false; }
     }
     if (first) throw Error{PM_ERR_DEPTH, "no leaf threshold builds this geometry"};
     p.lower_size = best_leaf;
-    // 3. stack depth and ray order on the chosen tree (the order only
-    // matters from the size at which the tracers apply it)
+    // 3. stack depth, ray order and step cap on the chosen tree (the order
+    // only matters from the size at which the tracers apply it)
     build_tree(ctx, P0, P1, P2, n, mesh_offsets, M, p, s);
-    int best_stack = 8, best_order = PM_RAY_ORDER_INPUT;
+    int best_stack = 8, best_order = PM_RAY_ORDER_INPUT, best_cap = 0;
     float best_t = 0.0f;
     first = true;
     for (int order : {PM_RAY_ORDER_INPUT, PM_RAY_ORDER_COHERENT}) {
         pm_trace_params q = tp;
         q.ray_order = order;
         if (order != PM_RAY_ORDER_INPUT && !ray_order_applies(q, nrays)) continue;
-        for (int S : kStack) {
-            const float t = time_trace(S, order);
-            ++trials;
-            if (first || t < best_t) { best_t = t; best_stack = S; best_order = order; first = false; }
-        }
+        for (int S : kStack)
+            for (int cap : kCap) {
+                const float t = time_trace(S, order, cap);
+                ++trials;
+                if (first || t < best_t) {
+                    best_t = t; best_stack = S; best_order = order; best_cap = cap; first = false;
+                }
+            }
     }
     PM_HIP(hipStreamSynchronize(s));
     out.chunk_size = best_chunk;
@@ -137,6 +142,7 @@ void autotune(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2,
     out.trace_ms = best_t;
     out.trials = trials;
     out.ray_order = best_order;
+    out.step_cap = best_cap;
 }
 
 }  // namespace pmkd

@@ -122,13 +122,22 @@ typedef struct {
  *                  rays in pixel order) only pay for the sort.  Each ray's
  *                  record (and counters) is the same either way and goes to
  *                  the ray's own index; brute force and the canvas keep the
- *                  input order. */
+ *                  input order.
+ *   step_cap       no reference counterpart: 0 (default) or the most inner
+ *                  nodes a ray's descent walks per round of the tree
+ *                  tracers' wave-cooperative leaf test before it rejoins
+ *                  the wave (a longer descent continues in the next round),
+ *                  so rays with short descents do not wait for the wave's
+ *                  longest one -- for descent-heavy rays (grazing camera
+ *                  rays over terrain); records and counters are the same
+ *                  for every value.  0 <= step_cap <= 4096. */
 typedef struct {
     int leaf_skipping;
     int stack_size;
     int brute_cap;
     int intersection;
     int ray_order;
+    int step_cap;
 } pm_trace_params;
 
 /* Tree view: device pointers into the context's tree (non-owning; valid
@@ -319,6 +328,7 @@ typedef struct {
     float build_ms, trace_ms;                 /* its median times */
     int trials;                               /* settings timed */
     int ray_order;                            /* the chosen pm_trace_params.ray_order */
+    int step_cap;                             /* the chosen pm_trace_params.step_cap */
 } pm_tuning;
 
 /* Autotuner (no reference counterpart: the reference fixes these at
@@ -327,7 +337,8 @@ typedef struct {
  * GPU: chunk_size over {256, 512, 1024, 2048} (build time), then lower_size
  * (the upper stage's leaf threshold) over {8, 16, 24, 32, 48, 64} (cost =
  * builds_per_trace * build + trace), then stack_size over {4, 8, 16}
- * together with ray_order over {INPUT, COHERENT} (trace time); each setting is timed `reps` times (median) after one
+ * together with ray_order over {INPUT, COHERENT} and step_cap over {0, 4}
+ * (trace time); each setting is timed `reps` times (median) after one
  * warm-up; a leaf threshold whose build stops at the reference-count
  * guard (PM_ERR_DEPTH) is skipped.  Every other field comes from `base`.  The geometry is one mesh
  * (mesh_offsets NULL) or a batch (as pm_build_batched); the rays are traced

@@ -113,6 +113,9 @@ public:
     // no reference counterpart: large traces of incoherent rays may run in
     // a coherent ray order (off by default); the hit records are the same
     void SetCoherentRayOrder(bool on) { params_.ray_order = on ? PM_RAY_ORDER_COHERENT : PM_RAY_ORDER_INPUT; }
+    // no reference counterpart: inner nodes a descent walks per round of the
+    // wave-cooperative leaf test (0 = no cap); the hit records are the same
+    void SetStepCap(int cap) { params_.step_cap = cap; }
     void PrintTiming(bool p) { printTiming_ = p; }
     bool GetPrintTiming() const { return printTiming_; }
     // trace time of the last Trace() in ms (with PrintTiming(true))

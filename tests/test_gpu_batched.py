@@ -202,7 +202,9 @@ def test_autotune_picks_a_valid_setting(pm, orc, scenes, torch, ctx):
     assert res["stack_size"] in (4, 8, 16)
     assert res["ray_order"] == pm.RAY_ORDER_INPUT          # too few rays to reorder
     # leaf thresholds the meshes cannot reach are skipped, the rest are timed
-    assert 4 + 1 + 3 <= res["trials"] <= 4 + 6 + 3 and res["build_ms"] > 0 and res["trace_ms"] > 0
+    assert res["step_cap"] in (0, 4)
+    # 4 chunk sizes, 1..6 leaf thresholds, 3 stack depths x 2 step caps
+    assert 4 + 1 + 6 <= res["trials"] <= 4 + 6 + 6 and res["build_ms"] > 0 and res["trace_ms"] > 0
     # the context holds the chosen tree; tracing it with the chosen depth is
     # the oracle's trace of the oracle's tree with the same parameters
     bp, tp = res["build_params"], res["trace_params"]

@@ -141,12 +141,41 @@ def test_autotune_times_both_orders(pm, orc, scenes, torch):
     ctx = pm.Context(0)
     res = ctx.autotune(*P, O, D, mesh_offsets=off, ray_mesh=RM, base=pm.build_params("reference"), reps=1)
     assert res["ray_order"] in (pm.RAY_ORDER_INPUT, pm.RAY_ORDER_COHERENT)
-    assert 4 + 1 + 6 <= res["trials"] <= 4 + 6 + 6
+    # 4 chunk sizes, 1..6 leaf thresholds, 3 stack depths x 2 orders x 2 step caps
+    assert 4 + 1 + 12 <= res["trials"] <= 4 + 6 + 12
     bp, tp = res["build_params"], res["trace_params"]
     assert tp.ray_order == res["ray_order"] and tp.stack_size == res["stack_size"]
+    assert tp.step_cap == res["step_cap"] and res["step_cap"] in (0, 4)
     ref = orc.build_batched(tris, off, orc_params(orc, bp))
     assert_same_tree(ctx.tree_arrays(), ref.arrays(), "autotuned forest")
     h = pm.hits_to_numpy(ctx.trace_batched(pm.SHORT_STACK, O, D, RM, tp))
     r = orc.trace_batched(ref, tris, org, dirs, ray_mesh, orc.SHORT_STACK, stack_size=tp.stack_size)
     assert_same_hits(h, r, "autotuned trace")
+    ctx.close()
+
+
+def test_step_cap_same_records(pm, orc, scenes, torch):
+    """pm_trace_params.step_cap spreads a descent over rounds of the
+    cooperative leaf test: records and counters equal the uncapped trace (and
+    the oracle's) for every cap, both tree tracers, both ray orders"""
+    tris = scenes.heightfield(256, seed=2)
+    W, Hh = 512, 288                     # 147,456 rays: the coherent order applies
+    m, c = scenes.camera(scenes.CAMERA_C2, W, Hh)
+    ctx = pm.Context(0)
+    P = [dev(torch, t) for t in tris]
+    ctx.build(*P, pm.build_params("reference"))
+    O, D = ctx.create_rays(m, c, W, Hh, jitter_seed=2)
+    org, dirs = O.cpu().numpy(), D.cpu().numpy()
+    ref = orc.build(tris, orc.build_params("reference"))
+    for algo in (pm.SHORT_STACK, pm.KD_RESTART):
+        want, wc = orc.trace(ref, tris, org, dirs, algo, stack_size=8, counters=True, threads=8)
+        for order in (pm.RAY_ORDER_INPUT, pm.RAY_ORDER_COHERENT):
+            for cap in (0, 1, 2, 4, 16):
+                cnt = torch.zeros((O.shape[0], 4), dtype=torch.int32, device="cuda:0")
+                tp = pm.trace_params(stack_size=8, ray_order=order, step_cap=cap)
+                h = pm.hits_to_numpy(ctx.trace(algo, O, D, tp, counters=cnt))
+                assert_same_hits(h, want, f"algo={algo} order={order} cap={cap}")
+                assert np.array_equal(cnt.cpu().numpy(), wc), (algo, order, cap)
+    with pytest.raises(pm.PmError):
+        ctx.trace(pm.SHORT_STACK, O, D, pm.trace_params(step_cap=-1))
     ctx.close()
