@@ -480,53 +480,63 @@ __device__ __forceinline__ unsigned long long lb_load(const unsigned long long *
 __device__ __forceinline__ void lb_store(unsigned long long *w, unsigned long long v) {
     __hip_atomic_store(w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ long long wave_sum64(long long v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
 
-// One wave: the exclusive prefixes of K counts for tile `tile`, whose
+// One wave: the exclusive prefixes of K <= 8 counts for tile `tile`, whose
 // aggregates are agg[k]; the status word of (tile, k) is w[K * tile + k].
+// The K look-backs run side by side: lanes 8k .. 8k+7 walk count k's words
+// over windows of 8 predecessors (lane 8k + j reads tile base - j), each
+// group stopping at its nearest inclusive prefix.
 template <int K>
 __device__ void lookback(unsigned long long *__restrict__ w, int tile, const long long *agg, long long *excl,
                          int *status) {
+    static_assert(K >= 1 && K <= 8, "one group of 8 lanes per count");
     const int lane = __lane_id();
+    const int k = lane >> 3, j = lane & 7;
+    long long a = 0;                       // this group's aggregate
+#pragma unroll
+    for (int q = 0; q < K; ++q) a = q == k ? agg[q] : a;
     if (tile == 0) {
-        if (lane == 0)
-            for (int k = 0; k < K; ++k) lb_store(&w[k], LB_INC | (unsigned long long)agg[k]);
-        for (int k = 0; k < K; ++k) excl[k] = 0;
+        if (j == 0 && k < K) lb_store(&w[k], LB_INC | (unsigned long long)a);
+#pragma unroll
+        for (int q = 0; q < K; ++q) excl[q] = 0;
         return;
     }
-    if (lane == 0)
-        for (int k = 0; k < K; ++k) lb_store(&w[(long long)K * tile + k], LB_AGG | (unsigned long long)agg[k]);
-    for (int k = 0; k < K; ++k) {
-        long long sum = 0;
-        int base = tile - 1, spins = 0;
-        for (;;) {
-            const int idx = base - lane;
-            const unsigned long long v = idx >= 0 ? lb_load(&w[(long long)K * idx + k]) : LB_INC;
-            const unsigned long long inc = __ballot((v >> 62) == 2ull);
-            const unsigned long long none = __ballot((v >> 62) == 0ull);
-            const int first = inc ? __ffsll((long long)inc) - 1 : 64;   // nearest inclusive prefix
-            const unsigned long long upto = first >= 63 ? ~0ull : ((2ull << first) - 1ull);
-            if (none & upto) {                     // a predecessor has not published yet
-                if (++spins > LB_SPIN) {
-                    if (lane == 0) atomicCAS(status, 0, PM_ERR_INTERNAL);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(2);
-                continue;
-            }
-            sum += wave_sum64(lane <= first ? (long long)(v & LB_VAL) : 0ll);
-            if (first < 64) break;
-            base -= 64;
+    if (j == 0 && k < K) lb_store(&w[(long long)K * tile + k], LB_AGG | (unsigned long long)a);
+    long long sum = 0;                     // the group's predecessors summed so far (same in its 8 lanes)
+    bool done = k >= K;
+    int base = tile - 1, spins = 0;
+    while (__ballot(!done)) {
+        unsigned long long v = LB_INC;
+        if (!done) {
+            const int idx = base - j;
+            v = idx >= 0 ? lb_load(&w[(long long)K * idx + k]) : LB_INC;
         }
-        excl[k] = sum;
+        const unsigned long long inc = __ballot(!done && (v >> 62) == 2ull);
+        const unsigned long long none = __ballot(!done && (v >> 62) == 0ull);
+        const unsigned gi = (unsigned)(inc >> (8 * k)) & 0xffu, gn = (unsigned)(none >> (8 * k)) & 0xffu;
+        const int first = gi ? __builtin_ctz(gi) : 8;     // the group's nearest inclusive prefix
+        const unsigned upto = first >= 7 ? 0xffu : ((2u << first) - 1u);
+        const bool ready = !done && !(gn & upto);          // every word up to it is published
+        long long c = ready && j <= first ? (long long)(v & LB_VAL) : 0ll;
+        c += __shfl_xor(c, 1, 64);
+        c += __shfl_xor(c, 2, 64);
+        c += __shfl_xor(c, 4, 64);
+        if (ready) {
+            sum += c;
+            if (first < 8) done = true;
+            else base -= 8;
+        }
+        if (__ballot(!done && !ready)) {                   // a predecessor has not published yet
+            if (++spins > LB_SPIN) {
+                if (lane == 0) atomicCAS(status, 0, PM_ERR_INTERNAL);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
     }
-    if (lane == 0)
-        for (int k = 0; k < K; ++k)
-            lb_store(&w[(long long)K * tile + k], LB_INC | (unsigned long long)(excl[k] + agg[k]));
+#pragma unroll
+    for (int q = 0; q < K; ++q) excl[q] = __shfl(sum, 8 * q, 64);
+    if (j == 0 && k < K) lb_store(&w[(long long)K * tile + k], LB_INC | (unsigned long long)(sum + a));
 }
 
 // the block's next tile (ticket order); every thread returns the same value
@@ -573,31 +583,52 @@ struct LevelBufs {
     int *ctr;                                   // their tickets (0: node pass, 2: leaf pass)
 };
 
-// the level's active nodes as the node pass sees them, read once per block:
-// the leaf pass of the level before stored them (A_scan, ai_scan) and
-// nothing changes them while this pass runs.  (Its last tile applies the
-// level advance to A, active_index and nodes_size, which a block that starts
-// late must not read.)  A failed build has no active nodes.
-__device__ __forceinline__ void level_nodes(const Lvl *L, int &A, int &active_index) {
-    __shared__ int s_lvl[2];
+// the level's active nodes as the node pass sees them, read once per block,
+// together with the block's first tile ticket: the leaf pass of the level
+// before stored them (A_scan, ai_scan) and nothing changes them while this
+// pass runs.  (Its last tile applies the level advance to A, active_index and
+// nodes_size, which a block that starts late must not read.)  A failed build
+// has no active nodes.
+__device__ __forceinline__ int level_nodes(const Lvl *L, int *ctr, int &A, int &active_index) {
+    __shared__ int s_lvl[3];
     if (threadIdx.x == 0) {
         s_lvl[0] = L->status ? 0 : L->A_scan;
         s_lvl[1] = L->ai_scan;
+        s_lvl[2] = atomicAdd(ctr, 1);
     }
     __syncthreads();
     A = s_lvl[0];
     active_index = s_lvl[1];
+    return s_lvl[2];
+}
+
+// what prepare_node reads of active node i: its six box keys and (empty-space
+// test) the box it was given by its parent
+struct NodeIn {
+    int2 k01, k23, k45;
+    float4 pmn, pmx;
+};
+
+__device__ __forceinline__ void load_node(const NodePtrs &N, int i, int node, const int *__restrict__ keys,
+                                          bool empty_space, NodeIn &v) {
+    const int2 *k2 = reinterpret_cast<const int2 *>(keys + 6 * (long long)i);   // 24 i bytes: 8-byte aligned
+    v.k01 = k2[0]; v.k23 = k2[1]; v.k45 = k2[2];
+    if (empty_space) {
+        v.pmn = N.bmin[node];
+        v.pmx = N.bmax[node];
+    }
 }
 
 // ReduceAabb result -> node box, CalcEmptySpaceSplits (EmptySpaceSplitting.h:
-// 10-70), CalcUpperNodeSplitInfo (ReduceSegments.h:466-484) for active node i
-__device__ __forceinline__ void prepare_node(const NodePtrs &N, int i, int node, const int *__restrict__ keys,
-                                             bool empty_space, float thr, int &pl) {
-    const f3 rMin = {unkey(keys[6 * i + 0]), unkey(keys[6 * i + 1]), unkey(keys[6 * i + 2])};
-    const f3 rMax = {unkey(keys[6 * i + 3]), unkey(keys[6 * i + 4]), unkey(keys[6 * i + 5])};
+// 10-70), CalcUpperNodeSplitInfo (ReduceSegments.h:466-484) for an active
+// node, from what load_node read
+__device__ __forceinline__ void prepare_node(const NodePtrs &N, int node, const NodeIn &v, bool empty_space,
+                                             float thr, int &pl) {
+    const f3 rMin = {unkey(v.k01.x), unkey(v.k01.y), unkey(v.k23.x)};
+    const f3 rMax = {unkey(v.k23.y), unkey(v.k45.x), unkey(v.k45.y)};
     pl = 0;
     if (empty_space) {
-        const f3 pMin = mk3(N.bmin[node]), pMax = mk3(N.bmax[node]);
+        const f3 pMin = mk3(v.pmn), pMax = mk3(v.pmx);
         if (rMin.x - pMin.x > thr * (pMax.x - pMin.x)) pl |= 1;
         if (rMin.y - pMin.y > thr * (pMax.y - pMin.y)) pl |= 2;
         if (rMin.z - pMin.z > thr * (pMax.z - pMin.z)) pl |= 4;
@@ -618,37 +649,63 @@ __device__ __forceinline__ void prepare_node(const NodePtrs &N, int i, int node,
     N.info[node] = (int8_t)(zHigher ? AX_Z : ax);
 }
 
+// the segment table entry of a node's k-th segment (Segment,
+// TriangleMapUpperCreator.cu:207-239)
+__device__ __forceinline__ void write_segment(const LevelBufs &B, int sg, int i, int first_ref, int count, int seg,
+                                              int k) {
+    B.seg_node[sg] = i;
+    B.seg_start[sg] = first_ref + k * seg;
+    B.seg_count[sg] = min(seg, count - k * seg);
+}
+
+// segments a node's own thread writes; the rest of a long node's segments
+// are shared out over the block
+constexpr int SEG_DIRECT = 4;
+
 __global__ __launch_bounds__(BS) void node_pass_kernel(NodePtrs N, Lvl *L, LevelBufs B, int seg,
                                                        const int *__restrict__ keys, int empty_space, float thr) {
     __shared__ int s_tile;
-    __shared__ int s_segpre[NT], s_pi[NT], s_cnt[NT];
-    __shared__ int s_w[2][BS / 64];
+    __shared__ int s_xpre[NT], s_segpre[NT], s_pi[NT], s_cnt[NT];
+    __shared__ int s_w[3][BS / 64];
     __shared__ long long s_ex[2];
     int A, active_index;
-    level_nodes(L, A, active_index);
+    int tile = level_nodes(L, &B.ctr[0], A, active_index);
     const long long cap_nodes = L->cap_nodes, cap_segs = L->cap_segs;
     const bool es = empty_space && active_index != 0;   // the root level has no empty-space cuts
     const int ntiles = max(1, (A + NT - 1) / NT);
     const int t = threadIdx.x;
-    for (int tile; (tile = next_tile(&B.ctr[0], &s_tile)) < ntiles;) {
+    for (; tile < ntiles; tile = next_tile(&B.ctr[0], &s_tile)) {
         const int i0 = tile * NT, n = min(NT, A - i0);
-        int pl[4], v[2] = {0, 0};
+        // every input first, then the node records (the stores would
+        // otherwise hold back the next node's loads)
+        NodeIn in[4];
+        int cnt[4], pi[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int li = 4 * t + j;
+            cnt[j] = 0; pi[j] = 0;
+            if (li < n) {
+                const int node = active_index + i0 + li;
+                load_node(N, i0 + li, node, keys, es, in[j]);
+                cnt[j] = (int)N.bitmap[node];
+                pi[j] = N.prim_index[node];
+            }
+        }
+        int pl[4], v[3] = {0, 0, 0};
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int li = 4 * t + j;
             pl[j] = 0;
             if (li < n) {
-                const int node = active_index + i0 + li;
-                prepare_node(N, i0 + li, node, keys, es, thr, pl[j]);
-                const int cnt = (int)N.bitmap[node];
-                s_pi[li] = N.prim_index[node];
-                s_cnt[li] = cnt;
+                prepare_node(N, active_index + i0 + li, in[j], es, thr, pl[j]);
+                const int ns = (cnt[j] + seg - 1) / seg;
                 v[0] += 2 * __popc(pl[j]);
-                v[1] += (cnt + seg - 1) / seg;
+                v[1] += ns;
+                v[2] += max(ns - SEG_DIRECT, 0);
             }
         }
-        int pre[2], tot[2];
-        block_scan<2>(v, pre, tot, s_w);
+        int pre[3], tot[3];
+        block_scan<3>(v, pre, tot, s_w);
         if (t < 64) {
             const long long agg[2] = {tot[0], tot[1]};
             long long ex[2];
@@ -661,34 +718,38 @@ __global__ __launch_bounds__(BS) void node_pass_kernel(NodePtrs N, Lvl *L, Level
         // that outgrew them fails in the plan below and writes nothing more)
         const bool fits = (long long)active_index + A + ereq + tot[0] <= cap_nodes &&
                           (long long)eseg + tot[1] <= cap_segs;
-        int r = pre[0], q = pre[1];
+        int r = pre[0], q = pre[1], x = pre[2];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int li = 4 * t + j;
             if (li < n) {
                 const int i = i0 + li;
+                const int ns = (cnt[j] + seg - 1) / seg;
                 B.eaddr[i] = ereq + r;
                 B.segbase[i] = eseg + q;
-                s_segpre[li] = q;
                 if (pl[j] && fits) empty_chain_node_at(N, i, A, active_index, pl[j], ereq + r);
+                if (fits)
+                    for (int k = 0; k < min(ns, SEG_DIRECT); ++k) write_segment(B, eseg + q + k, i, pi[j], cnt[j], seg, k);
+                s_xpre[li] = x;
+                s_segpre[li] = q;
+                s_pi[li] = pi[j];
+                s_cnt[li] = cnt[j];
                 r += 2 * __popc(pl[j]);
-                q += (s_cnt[li] + seg - 1) / seg;
+                q += ns;
+                x += max(ns - SEG_DIRECT, 0);
             }
         }
         __syncthreads();
-        // the segment table: the tile's tot[1] segments start at eseg; a
-        // segment's node is the last li with s_segpre[li] <= its index
-        for (int ls = t; fits && ls < tot[1]; ls += BS) {
+        // the long nodes' other segments: extra segment ls belongs to the
+        // last node li with s_xpre[li] <= ls
+        for (int ls = t; fits && ls < tot[2]; ls += BS) {
             int lo = 0, hi = n - 1;
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
-                if (s_segpre[mid] <= ls) lo = mid; else hi = mid - 1;
+                if (s_xpre[mid] <= ls) lo = mid; else hi = mid - 1;
             }
-            const int off = (ls - s_segpre[lo]) * seg;
-            const int sg = eseg + ls;
-            B.seg_node[sg] = i0 + lo;
-            B.seg_start[sg] = s_pi[lo] + off;
-            B.seg_count[sg] = min(seg, s_cnt[lo] - off);
+            const int k = SEG_DIRECT + ls - s_xpre[lo];
+            write_segment(B, eseg + s_segpre[lo] + k, i0 + lo, s_pi[lo], s_cnt[lo], seg, k);
         }
         if (tile == ntiles - 1) {
             // the level's totals: the previous level's advance, then the
@@ -788,20 +849,28 @@ __device__ ChildPlan plan_children_totals(const Lvl *L, int L_all, int R_all, in
     return P;
 }
 
+// the leaf pass keeps the side counts' tile-local prefixes of a tile's first
+// SEG_LDS segments in LDS (the rest are read back from segLoff / segRoff)
+constexpr int SEG_LDS = 2048;
+
 __global__ __launch_bounds__(BS) void leaf_pass_kernel(Lvl *L, LevelBufs B, int LS, int *__restrict__ keys_next) {
     __shared__ int s_tile;
     __shared__ int s_w[4][BS / 64];
     __shared__ long long s_ex[6];
-    __shared__ int s_lvl[2];
+    __shared__ int s_lvl[3];
+    __shared__ int s_sb[NT + 1];                 // the tile's nodes' first segments
+    __shared__ int s_lp[SEG_LDS], s_rp[SEG_LDS];  // local side prefixes of its first SEG_LDS segments
     const int t = threadIdx.x;
-    // read once per block: the last tile may fail the level (A = S = 0)
-    // while a late block starts
+    // read once per block, with the first ticket: the last tile may fail the
+    // level (A = S = 0) while a late block starts
     if (t == 0) {
         s_lvl[0] = L->status ? 0 : L->A;
         s_lvl[1] = L->status ? 0 : L->S;
+        s_lvl[2] = atomicAdd(&B.ctr[2], 1);
     }
     __syncthreads();
     const int A = s_lvl[0], S = s_lvl[1];
+    int tile = s_lvl[2];
     const int ntiles = max(1, (A + NT - 1) / NT);
     // the next level's box keys: every possible child slot (created <= 2A),
     // before any scatter joins into them
@@ -809,11 +878,13 @@ __global__ __launch_bounds__(BS) void leaf_pass_kernel(Lvl *L, LevelBufs B, int 
         const int kmin = fkey(F_INF), kmax = fkey(-F_INF);
         GRID_STRIDE(k, 12 * A) keys_next[k] = (k % 6) < 3 ? kmin : kmax;
     }
-    for (int tile; (tile = next_tile(&B.ctr[2], &s_tile)) < ntiles;) {
+    for (; tile < ntiles; tile = next_tile(&B.ctr[2], &s_tile)) {
         const int i0 = tile * NT, n = min(NT, A - i0);
+        for (int li = t; li <= n; li += BS) s_sb[li] = n > 0 ? B.segbase[i0 + li] : 0;
+        __syncthreads();
         // the tile's segments [sb, se): the side counts' exclusive prefixes
         // inside the tile, written to segLoff/segRoff (offset below)
-        const int sb = n > 0 ? B.segbase[i0] : 0, se = n > 0 ? B.segbase[i0 + n] : 0;
+        const int sb = s_sb[0], se = s_sb[n];
         int cL = 0, cR = 0;
         for (int c0 = sb; c0 < se; c0 += NT) {
             int v[2] = {0, 0}, sl[4], sr[4];
@@ -830,12 +901,21 @@ __global__ __launch_bounds__(BS) void leaf_pass_kernel(Lvl *L, LevelBufs B, int 
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int s = c0 + 4 * t + j;
-                if (s < se) { B.segLoff[s] = pl; B.segRoff[s] = pr; }
+                if (s < se) {
+                    if (s - sb < SEG_LDS) { s_lp[s - sb] = pl; s_rp[s - sb] = pr; }
+                    else { B.segLoff[s] = pl; B.segRoff[s] = pr; }
+                }
                 pl += sl[j]; pr += sr[j];
             }
             cL += tot[0]; cR += tot[1];
         }
         __syncthreads();   // the tile's local prefixes are visible to the whole block
+        // a segment's local prefixes (se: the tile's totals)
+        auto lpre = [&](int s, int &l, int &r) {
+            if (s >= se) { l = cL; r = cR; }
+            else if (s - sb < SEG_LDS) { l = s_lp[s - sb]; r = s_rp[s - sb]; }
+            else { l = B.segLoff[s]; r = B.segRoff[s]; }
+        };
         // children sizes and leaf flags per node (CalcNodeChildSize, MarkNodeLeafs)
         int v[4] = {0, 0, 0, 0}, nl[4], nr[4];
 #pragma unroll
@@ -843,10 +923,13 @@ __global__ __launch_bounds__(BS) void leaf_pass_kernel(Lvl *L, LevelBufs B, int 
             const int li = 4 * t + j;
             nl[j] = nr[j] = 0;
             if (li < n) {
-                const int b = B.segbase[i0 + li], e = B.segbase[i0 + li + 1];
+                const int b = s_sb[li], e = s_sb[li + 1];
                 if (e > b) {
-                    nl[j] = (e < se ? B.segLoff[e] : cL) - B.segLoff[b];
-                    nr[j] = (e < se ? B.segRoff[e] : cR) - B.segRoff[b];
+                    int lb, rb, le, re;
+                    lpre(b, lb, rb);
+                    lpre(e, le, re);
+                    nl[j] = le - lb;
+                    nr[j] = re - rb;
                 }
                 v[0] += nl[j] < LS;
                 v[1] += nr[j] < LS;
@@ -866,8 +949,10 @@ __global__ __launch_bounds__(BS) void leaf_pass_kernel(Lvl *L, LevelBufs B, int 
         // the side offsets of the tile's segments, now global
         const int oL = (int)s_ex[0], oR = (int)s_ex[1];
         for (int s = sb + t; s < se; s += BS) {
-            B.segLoff[s] += oL;
-            B.segRoff[s] += oR;
+            int l, r;
+            lpre(s, l, r);
+            B.segLoff[s] = l + oL;
+            B.segRoff[s] = r + oR;
         }
         int p[4];
         for (int k = 0; k < 4; ++k) p[k] = (int)s_ex[k + 2] + pre[k];
