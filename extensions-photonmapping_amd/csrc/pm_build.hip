@@ -23,6 +23,7 @@
 // (TriangleMapBitmapCreator.cu:32-65) is one thread per upper leaf.
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 
 #include "pm_device.h"
 #include "pm_internal.h"
@@ -779,24 +780,38 @@ __device__ __forceinline__ void classify_segment(const NodePtrs &N, int node, in
     const int ax = __builtin_amdgcn_readfirstlane(N.info[node]);
     const float sp = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(N.split[node])));
     cl = 0; cr = 0;
+    // the chunk's iterations (wave-uniform): groups of up to 16 while the
+    // chunk has that many refs left, then groups of 2
+    const int its = (count + 63) >> 6;
     if (!DIV) {
         // the split axis' coordinates only, loaded without branches (lanes
-        // past the chunk re-read its first ref) so that the chunk's loads
-        // all issue before the first compare
+        // past the chunk re-read its first ref) so that a group's loads all
+        // issue before its first compare
         const float *lo = reinterpret_cast<const float *>(rmin + start) + (ax - AX_X);
         const float *hi = reinterpret_cast<const float *>(rmax + start) + (ax - AX_X);
-#pragma unroll 16
-        for (int it = 0; it < SEGT / 64; ++it) {
-            const int j = it * 64 + lane;
-            const int jj = j < count ? j : 0;
-            const float a = lo[4 * jj], b = hi[4 * jj];
-            cl += __popcll(__ballot(j < count && a <= sp));
-            cr += __popcll(__ballot(j < count && sp < b));
-        }
+        auto group = [&](int it0, auto G) {
+            float a[G.value], b[G.value];
+#pragma unroll
+            for (int q = 0; q < G.value; ++q) {
+                const int j = (it0 + q) * 64 + lane;
+                const int jj = j < count ? j : 0;
+                a[q] = lo[4 * jj];
+                b[q] = hi[4 * jj];
+            }
+#pragma unroll
+            for (int q = 0; q < G.value; ++q) {
+                const int j = (it0 + q) * 64 + lane;
+                cl += __popcll(__ballot(j < count && a[q] <= sp));
+                cr += __popcll(__ballot(j < count && sp < b[q]));
+            }
+        };
+        constexpr int GB = SEGT / 64 < 16 ? SEGT / 64 : 16;
+        int it = 0;
+        for (; it + GB <= its; it += GB) group(it, std::integral_constant<int, GB>{});
+        for (; it < its; it += 2) group(it, std::integral_constant<int, 2>{});
     } else {
         const f3 nmn = mk3(N.bmin[node]), nmx = mk3(N.bmax[node]);
-#pragma unroll 16
-        for (int it = 0; it < SEGT / 64; ++it) {
+        for (int it = 0; it < its; ++it) {
             const int j = it * 64 + lane;
             const bool valid = j < count;
             bool l = false, r = false;
@@ -1089,11 +1104,11 @@ __device__ __forceinline__ void scatter_pass_segment(
     int rl = 0, rr = 0;
     int l0 = fkey(F_INF), l1 = l0, l2 = l0, l3 = fkey(-F_INF), l4 = l3, l5 = l3;
     int r0 = l0, r1 = l0, r2 = l0, r3 = l3, r4 = l3, r5 = l3;
-    // groups of G iterations: the group's refs are loaded first (lanes past
-    // the chunk re-read its first ref), then classified and written
-    constexpr int IT = SEGT / 64, G = IT < 8 ? IT : 8;
-#pragma unroll 4
-    for (int g0 = 0; g0 < IT; g0 += G) {
+    // groups of iterations: a group's refs are loaded first (lanes past the
+    // chunk re-read its first ref), then classified and written; groups of 8
+    // while the chunk has that many iterations left, then groups of 2
+    auto group = [&](int g0, auto GC) {
+        constexpr int G = decltype(GC)::value;
         float4 gmn[G], gmx[G];
 #pragma unroll
         for (int q = 0; q < G; ++q) {
@@ -1141,7 +1156,12 @@ __device__ __forceinline__ void scatter_pass_segment(
             rl += __popcll(blm);
             rr += __popcll(brm);
         }
-    }
+    };
+    constexpr int GB = SEGT / 64 < 8 ? SEGT / 64 : 8;
+    const int its = (count + 63) >> 6;   // wave-uniform
+    int g0 = 0;
+    for (; g0 + GB <= its; g0 += GB) group(g0, std::integral_constant<int, GB>{});
+    for (; g0 < its; g0 += 2) group(g0, std::integral_constant<int, 2>{});
     const bool doL = !leafL && rl > 0, doR = !leafR && rr > 0;   // wave-uniform
     if (doL) {
         l0 = wave_min(l0); l1 = wave_min(l1); l2 = wave_min(l2);
