@@ -66,8 +66,8 @@ struct ScanIO {
 };
 
 __device__ void plan_nodes(Lvl *L, int E, int S);
-__device__ void lvl_fail(Lvl *L, int code);
-__device__ void advance_level(Lvl *L);
+__host__ __device__ void lvl_fail(Lvl *L, int code);
+__host__ __device__ void advance_level(Lvl *L);
 
 // the scans move 4 consecutive elements per thread (SCAN_E), as one 16-byte
 // access where the array is 16-byte aligned (device allocations are; block-sum
@@ -443,7 +443,7 @@ struct ChildPlan {
 };
 
 // ================================================================== fused levels
-// An upper level in four launches.  The level's prefix sums are computed
+// An upper level in three launches.  The level's prefix sums are computed
 // inside the passes that produce the counts, as single-pass scans: tiles are
 // taken in ticket order (an atomic counter), so all of a tile's predecessors
 // have started before it; a tile publishes its aggregate as soon as it is
@@ -452,20 +452,26 @@ struct ChildPlan {
 // nearest inclusive prefix (decoupled look-back, Merrill & Garland 2016).
 // The tile holding the last ticket knows the level's totals and applies the
 // plan that the host arithmetic of TriangleMapUpperCreator.cu:140-180 makes.
-//   node_pass      node boxes, empty-space planes and split planes
-//                  (CalcEmptySpaceSplits, CalcUpperNodeSplitInfo), scans of
-//                  the empty-space node and segment counts, the empty-space
-//                  chains (EmptySpaceSplitting2) and the segment table
-//                  (Segment, TriangleMapUpperCreator.cu:207-239);
 //   classify_pass  side counts per segment (SetSplitSide), one wave per
-//                  segment;
-//   leaf_pass      the side counts' scans (over the segments of a tile of
+//                  segment; the wave works out its node's split plane
+//                  (CalcUpperNodeSplitInfo) from the node's box keys;
+//   leaf_pass      node boxes, empty-space planes and split planes
+//                  (ReduceAabb, CalcEmptySpaceSplits, CalcUpperNodeSplitInfo),
+//                  the side counts' scans (over the segments of a tile of
 //                  nodes, which are contiguous), children sizes and leaf
 //                  flags per node (CalcNodeChildSize, MarkNodeLeafs), scans
-//                  of leaf slots and leaf refs, the next level's box keys
-//                  reset; the children plan;
-//   scatter_pass   the clipped refs (SplitTriangles) and the child records
-//                  (CreateUpperChildren, MarkLeafNodes, PropagateAabbToChildren).
+//                  of the empty-space nodes, leaf slots, leaf refs and the
+//                  next level's segments, the empty-space chains
+//                  (EmptySpaceSplitting2), the next level's box keys reset;
+//                  the children plan;
+//   scatter_pass   the clipped refs (SplitTriangles), the child records
+//                  (CreateUpperChildren, MarkLeafNodes, PropagateAabbToChildren)
+//                  and the next level's segment table (Segment,
+//                  TriangleMapUpperCreator.cu:207-239).
+// The roots' segment table comes from root_segments_kernel.  Segment tables,
+// per-node segment bases, box keys, look-back words and tickets alternate
+// between two buffers by level parity: a level's passes read one and write
+// the next level's into the other.
 // Every scan is over integers, so the offsets (and the tree) are those of
 // any other order of summation.
 constexpr int NT = 4 * BS;             // active nodes per node/leaf tile (4 per thread)
@@ -573,35 +579,18 @@ __device__ __forceinline__ void block_scan(const int *v, int *pre, int *tot, int
     __syncthreads();
 }
 
-// the device buffers the four level passes share
+// the device buffers the three level passes share
 struct LevelBufs {
-    int *eaddr, *segbase;                       // per active node; A + 1 entries
-    int *seg_node, *seg_start, *seg_count;      // per segment
-    int *segL, *segR;                           // per segment: refs on each side (classify pass)
-    int *segLoff, *segRoff;                     // per segment: refs on each side before it, S + 1 entries
-    int2 *lna, *lref;                           // per active node: leaf slots / leaf refs before it (left, right)
-    unsigned long long *lb1, *lb3;              // look-back words of the node and leaf passes
-    int *ctr;                                   // their tickets (0: node pass, 2: leaf pass)
+    const int *segbase;                           // per active node: first segment; A + 1 entries
+    const int *seg_node, *seg_start, *seg_count;  // the level's segment table
+    int *segL, *segR;                             // per segment: refs on each side (classify pass)
+    int *segLoff, *segRoff;                       // per segment: refs on each side before it, S + 1 entries
+    int2 *lna, *lref, *lseg;                      // per active node: leaf slots / leaf refs / next-level
+                                                  // segments of non-leaf children before it (left, right)
+    int *segbase_n, *seg_node_n, *seg_start_n, *seg_count_n;   // the same for the next level
+    unsigned long long *lb, *lb_n;                // look-back words of this level's leaf pass and the next's
+    int *ctr;                                     // leaf pass tickets by level parity
 };
-
-// the level's active nodes as the node pass sees them, read once per block,
-// together with the block's first tile ticket: the leaf pass of the level
-// before stored them (A_scan, ai_scan) and nothing changes them while this
-// pass runs.  (Its last tile applies the level advance to A, active_index and
-// nodes_size, which a block that starts late must not read.)  A failed build
-// has no active nodes.
-__device__ __forceinline__ int level_nodes(const Lvl *L, int *ctr, int &A, int &active_index) {
-    __shared__ int s_lvl[3];
-    if (threadIdx.x == 0) {
-        s_lvl[0] = L->status ? 0 : L->A_scan;
-        s_lvl[1] = L->ai_scan;
-        s_lvl[2] = atomicAdd(ctr, 1);
-    }
-    __syncthreads();
-    A = s_lvl[0];
-    active_index = s_lvl[1];
-    return s_lvl[2];
-}
 
 // what prepare_node reads of active node i: its six box keys and (empty-space
 // test) the box it was given by its parent
@@ -620,9 +609,21 @@ __device__ __forceinline__ void load_node(const NodePtrs &N, int i, int node, co
     }
 }
 
+// CalcUpperNodeSplitInfo (ReduceSegments.h:466-484): the longest axis of the
+// reduced box (ties to the lower axis) at its midpoint
+__device__ __forceinline__ void node_split(f3 rMin, f3 rMax, int &ax, float &sp) {
+    const f3 sz = sub3(rMax, rMin);
+    const f3 med = {sz.x * 0.5f + rMin.x, sz.y * 0.5f + rMin.y, sz.z * 0.5f + rMin.z};
+    const bool yAboveX = sz.x < sz.y;
+    const float mxs = yAboveX ? sz.y : sz.x;
+    const float spl = yAboveX ? med.y : med.x;
+    const bool zHigher = mxs < sz.z;
+    sp = zHigher ? med.z : spl;
+    ax = zHigher ? AX_Z : (yAboveX ? AX_Y : AX_X);
+}
+
 // ReduceAabb result -> node box, CalcEmptySpaceSplits (EmptySpaceSplitting.h:
-// 10-70), CalcUpperNodeSplitInfo (ReduceSegments.h:466-484) for an active
-// node, from what load_node read
+// 10-70) and the split plane for an active node, from what load_node read
 __device__ __forceinline__ void prepare_node(const NodePtrs &N, int node, const NodeIn &v, bool empty_space,
                                              float thr, int &pl) {
     const f3 rMin = {unkey(v.k01.x), unkey(v.k01.y), unkey(v.k23.x)};
@@ -639,133 +640,104 @@ __device__ __forceinline__ void prepare_node(const NodePtrs &N, int node, const 
     }
     N.bmin[node] = make_float4(rMin.x, rMin.y, rMin.z, 0.0f);
     N.bmax[node] = make_float4(rMax.x, rMax.y, rMax.z, 0.0f);
-    const f3 sz = sub3(rMax, rMin);
-    const f3 med = {sz.x * 0.5f + rMin.x, sz.y * 0.5f + rMin.y, sz.z * 0.5f + rMin.z};
-    const bool yAboveX = sz.x < sz.y;
-    const float mxs = yAboveX ? sz.y : sz.x;
-    const float spl = yAboveX ? med.y : med.x;
-    const int ax = yAboveX ? AX_Y : AX_X;
-    const bool zHigher = mxs < sz.z;
-    N.split[node] = zHigher ? med.z : spl;
-    N.info[node] = (int8_t)(zHigher ? AX_Z : ax);
+    int ax;
+    float sp;
+    node_split(rMin, rMax, ax, sp);
+    N.split[node] = sp;
+    N.info[node] = (int8_t)ax;
 }
 
 // the segment table entry of a node's k-th segment (Segment,
 // TriangleMapUpperCreator.cu:207-239)
-__device__ __forceinline__ void write_segment(const LevelBufs &B, int sg, int i, int first_ref, int count, int seg,
-                                              int k) {
-    B.seg_node[sg] = i;
-    B.seg_start[sg] = first_ref + k * seg;
-    B.seg_count[sg] = min(seg, count - k * seg);
+__device__ __forceinline__ void write_segment(int *__restrict__ sn, int *__restrict__ ss, int *__restrict__ sc,
+                                              int sg, int i, int first_ref, int count, int seg, int k) {
+    sn[sg] = i;
+    ss[sg] = first_ref + k * seg;
+    sc[sg] = min(seg, count - k * seg);
 }
 
-// segments a node's own thread writes; the rest of a long node's segments
-// are shared out over the block
+// segments a root's own thread writes; the rest of a long root's segments are
+// shared out over the block
 constexpr int SEG_DIRECT = 4;
 
-__global__ __launch_bounds__(BS) void node_pass_kernel(NodePtrs N, Lvl *L, LevelBufs B, int seg,
-                                                       const int *__restrict__ keys, int empty_space, float thr) {
-    __shared__ int s_tile;
+// The roots' segment table (level 0): tiles of NT roots in ticket order, a
+// look-back scan of the roots' segment counts.  Uses the look-back words and
+// ticket of level parity 1 (level 0's leaf pass clears them for level 1).
+__global__ __launch_bounds__(BS) void root_segments_kernel(NodePtrs N, Lvl *L, int *__restrict__ segbase,
+                                                           int *__restrict__ sn, int *__restrict__ ss,
+                                                           int *__restrict__ sc, unsigned long long *lbw, int *ctr,
+                                                           int M, int seg) {
+    __shared__ int s_tile, s_first;
     __shared__ int s_xpre[NT], s_segpre[NT], s_pi[NT], s_cnt[NT];
-    __shared__ int s_w[3][BS / 64];
-    __shared__ long long s_ex[2];
-    int A, active_index;
-    int tile = level_nodes(L, &B.ctr[0], A, active_index);
-    const long long cap_nodes = L->cap_nodes, cap_segs = L->cap_segs;
-    const bool es = empty_space && active_index != 0;   // the root level has no empty-space cuts
-    const int ntiles = max(1, (A + NT - 1) / NT);
+    __shared__ int s_w[2][BS / 64];
+    __shared__ long long s_ex;
     const int t = threadIdx.x;
-    for (; tile < ntiles; tile = next_tile(&B.ctr[0], &s_tile)) {
-        const int i0 = tile * NT, n = min(NT, A - i0);
-        // every input first, then the node records (the stores would
-        // otherwise hold back the next node's loads)
-        NodeIn in[4];
-        int cnt[4], pi[4];
+    if (t == 0) s_first = atomicAdd(ctr, 1);
+    __syncthreads();
+    const long long cap_segs = L->cap_segs;
+    const int ntiles = max(1, (M + NT - 1) / NT);
+    for (int tile = s_first; tile < ntiles; tile = next_tile(ctr, &s_tile)) {
+        const int i0 = tile * NT, n = min(NT, M - i0);
+        int cnt[4], pi[4], v[2] = {0, 0};
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int li = 4 * t + j;
             cnt[j] = 0; pi[j] = 0;
             if (li < n) {
-                const int node = active_index + i0 + li;
-                load_node(N, i0 + li, node, keys, es, in[j]);
-                cnt[j] = (int)N.bitmap[node];
-                pi[j] = N.prim_index[node];
+                cnt[j] = (int)N.bitmap[i0 + li];
+                pi[j] = N.prim_index[i0 + li];
             }
+            const int ns = (cnt[j] + seg - 1) / seg;
+            v[0] += ns;
+            v[1] += max(ns - SEG_DIRECT, 0);
         }
-        int pl[4], v[3] = {0, 0, 0};
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int li = 4 * t + j;
-            pl[j] = 0;
-            if (li < n) {
-                prepare_node(N, active_index + i0 + li, in[j], es, thr, pl[j]);
-                const int ns = (cnt[j] + seg - 1) / seg;
-                v[0] += 2 * __popc(pl[j]);
-                v[1] += ns;
-                v[2] += max(ns - SEG_DIRECT, 0);
-            }
-        }
-        int pre[3], tot[3];
-        block_scan<3>(v, pre, tot, s_w);
+        int pre[2], tot[2];
+        block_scan<2>(v, pre, tot, s_w);
         if (t < 64) {
-            const long long agg[2] = {tot[0], tot[1]};
-            long long ex[2];
-            lookback<2>(B.lb1, tile, agg, ex, &L->status);
-            if (t == 0) { s_ex[0] = ex[0]; s_ex[1] = ex[1]; }
+            const long long agg[1] = {tot[0]};
+            long long ex[1];
+            lookback<1>(lbw, tile, agg, ex, &L->status);
+            if (t == 0) s_ex = ex[0];
         }
         __syncthreads();
-        const int ereq = (int)s_ex[0], eseg = (int)s_ex[1];
-        // chains and table only inside the capacities (a captured build
-        // that outgrew them fails in the plan below and writes nothing more)
-        const bool fits = (long long)active_index + A + ereq + tot[0] <= cap_nodes &&
-                          (long long)eseg + tot[1] <= cap_segs;
-        int r = pre[0], q = pre[1], x = pre[2];
+        const int eseg = (int)s_ex;
+        const bool fits = (long long)eseg + tot[0] <= cap_segs;
+        int q = pre[0], x = pre[1];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int li = 4 * t + j;
             if (li < n) {
                 const int i = i0 + li;
                 const int ns = (cnt[j] + seg - 1) / seg;
-                B.eaddr[i] = ereq + r;
-                B.segbase[i] = eseg + q;
-                if (pl[j] && fits) empty_chain_node_at(N, i, A, active_index, pl[j], ereq + r);
+                segbase[i] = eseg + q;
                 if (fits)
-                    for (int k = 0; k < min(ns, SEG_DIRECT); ++k) write_segment(B, eseg + q + k, i, pi[j], cnt[j], seg, k);
+                    for (int k = 0; k < min(ns, SEG_DIRECT); ++k)
+                        write_segment(sn, ss, sc, eseg + q + k, i, pi[j], cnt[j], seg, k);
                 s_xpre[li] = x;
                 s_segpre[li] = q;
                 s_pi[li] = pi[j];
                 s_cnt[li] = cnt[j];
-                r += 2 * __popc(pl[j]);
                 q += ns;
                 x += max(ns - SEG_DIRECT, 0);
             }
         }
         __syncthreads();
-        // the long nodes' other segments: extra segment ls belongs to the
-        // last node li with s_xpre[li] <= ls
-        for (int ls = t; fits && ls < tot[2]; ls += BS) {
+        // the long roots' other segments: extra segment ls belongs to the
+        // last root li with s_xpre[li] <= ls
+        for (int ls = t; fits && ls < tot[1]; ls += BS) {
             int lo = 0, hi = n - 1;
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
                 if (s_xpre[mid] <= ls) lo = mid; else hi = mid - 1;
             }
             const int k = SEG_DIRECT + ls - s_xpre[lo];
-            write_segment(B, eseg + s_segpre[lo] + k, i0 + lo, s_pi[lo], s_cnt[lo], seg, k);
+            write_segment(sn, ss, sc, eseg + s_segpre[lo] + k, i0 + lo, s_pi[lo], s_cnt[lo], seg, k);
         }
-        if (tile == ntiles - 1) {
-            // the level's totals: the previous level's advance, then the
-            // empty-space nodes and segments (plan_nodes)
-            if (t == 0) {
-                const int E = ereq + tot[0], S = eseg + tot[1];
-                B.eaddr[A] = E;
-                B.segbase[A] = S;
-                plan_nodes(L, E, S);
-            }
-            __syncthreads();
-            // the leaf pass starts with fresh tickets and status words
-            const int nt3 = max(1, (A + NT - 1) / NT);
-            for (int k = t; k < 6 * nt3; k += BS) B.lb3[k] = 0ull;
-            if (t == 0) B.ctr[2] = 0;
+        if (tile == ntiles - 1 && t == 0) {
+            const long long S = (long long)eseg + tot[0];
+            segbase[M] = (int)S;
+            if (S > cap_segs) lvl_fail(L, PM_ERR_CAPACITY);
+            else L->scanS[0] = (int)S;
         }
     }
 }
@@ -773,12 +745,10 @@ __global__ __launch_bounds__(BS) void node_pass_kernel(NodePtrs N, Lvl *L, Level
 // SetSplitSide (TriangleUpperChildren.h:10-62) for one segment: its counts
 // of refs on the left and on the right of its node's plane
 template <int SEGT, bool DIV>
-__device__ __forceinline__ void classify_segment(const NodePtrs &N, int node, int start, int count,
+__device__ __forceinline__ void classify_segment(int ax, float sp, f3 nmn, f3 nmx, int start, int count,
                                                  const float4 *__restrict__ rmin, const float4 *__restrict__ rmax,
                                                  const DivideGeom &dg, int &cl, int &cr) {
     const int lane = threadIdx.x & 63;
-    const int ax = __builtin_amdgcn_readfirstlane(N.info[node]);
-    const float sp = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(N.split[node])));
     cl = 0; cr = 0;
     // the chunk's iterations (wave-uniform): groups of up to 16 while the
     // chunk has that many refs left, then groups of 2
@@ -810,7 +780,6 @@ __device__ __forceinline__ void classify_segment(const NodePtrs &N, int node, in
         for (; it + GB <= its; it += GB) group(it, std::integral_constant<int, GB>{});
         for (; it < its; it += 2) group(it, std::integral_constant<int, 2>{});
     } else {
-        const f3 nmn = mk3(N.bmin[node]), nmx = mk3(N.bmax[node]);
         for (int it = 0; it < its; ++it) {
             const int j = it * 64 + lane;
             const bool valid = j < count;
@@ -827,15 +796,24 @@ __device__ __forceinline__ void classify_segment(const NodePtrs &N, int node, in
     }
 }
 
+// the level's segments, one wave each; a segment's node's plane comes from
+// the node's box keys (the values the leaf pass gives the node record)
 template <int SEGT, bool DIV>
-__global__ __launch_bounds__(WPB * 64) void classify_pass_kernel(const NodePtrs N, const Lvl *L, LevelBufs B,
+__global__ __launch_bounds__(WPB * 64) void classify_pass_kernel(const Lvl *L, LevelBufs B, int par,
+                                                                 const int *__restrict__ keys,
                                                                  const float4 *__restrict__ rmin,
                                                                  const float4 *__restrict__ rmax, DivideGeom dg) {
-    const int S = L->status ? 0 : L->S, active_index = L->active_index;
+    const int S = L->status ? 0 : L->scanS[par];
     for (int s = wave_index(); s < S; s += wave_count()) {
+        const int i = __builtin_amdgcn_readfirstlane(B.seg_node[s]);
+        const int *k = keys + 6 * (long long)i;
+        const f3 rMin = {unkey(k[0]), unkey(k[1]), unkey(k[2])};
+        const f3 rMax = {unkey(k[3]), unkey(k[4]), unkey(k[5])};
+        int ax;
+        float sp;
+        node_split(rMin, rMax, ax, sp);
         int cl, cr;
-        classify_segment<SEGT, DIV>(N, active_index + B.seg_node[s], B.seg_start[s], B.seg_count[s], rmin, rmax,
-                                    dg, cl, cr);
+        classify_segment<SEGT, DIV>(ax, sp, rMin, rMax, B.seg_start[s], B.seg_count[s], rmin, rmax, dg, cl, cr);
         if ((threadIdx.x & 63) == 0) { B.segL[s] = cl; B.segR[s] = cr; }
     }
 }
@@ -868,24 +846,30 @@ __device__ ChildPlan plan_children_totals(const Lvl *L, int L_all, int R_all, in
 // SEG_LDS segments in LDS (the rest are read back from segLoff / segRoff)
 constexpr int SEG_LDS = 2048;
 
-__global__ __launch_bounds__(BS) void leaf_pass_kernel(Lvl *L, LevelBufs B, int LS, int *__restrict__ keys_next) {
+// The level's node work and children plan.  par: the level's parity.
+// The blocks read the level's sizes once (scanA/scanAI/scanS[par]), with
+// their first ticket; the last tile writes the next level's into [par ^ 1].
+__global__ __launch_bounds__(BS) void leaf_pass_kernel(NodePtrs N, Lvl *L, LevelBufs B, int par, int LS, int seg,
+                                                       const int *__restrict__ keys, int *__restrict__ keys_next,
+                                                       int empty_space, float thr) {
     __shared__ int s_tile;
-    __shared__ int s_w[4][BS / 64];
-    __shared__ long long s_ex[6];
-    __shared__ int s_lvl[3];
-    __shared__ int s_sb[NT + 1];                 // the tile's nodes' first segments
+    __shared__ int s_w[7][BS / 64];
+    __shared__ long long s_ex[8];
+    __shared__ int s_lvl[4];
+    __shared__ int s_sb[NT + 1];                  // the tile's nodes' first segments
     __shared__ int s_lp[SEG_LDS], s_rp[SEG_LDS];  // local side prefixes of its first SEG_LDS segments
     const int t = threadIdx.x;
-    // read once per block, with the first ticket: the last tile may fail the
-    // level (A = S = 0) while a late block starts
     if (t == 0) {
-        s_lvl[0] = L->status ? 0 : L->A;
-        s_lvl[1] = L->status ? 0 : L->S;
-        s_lvl[2] = atomicAdd(&B.ctr[2], 1);
+        const bool ok = L->status == 0;   // a failed build has no active nodes
+        s_lvl[0] = ok ? L->scanA[par] : 0;
+        s_lvl[1] = L->scanAI[par];
+        s_lvl[2] = ok ? L->scanS[par] : 0;
+        s_lvl[3] = atomicAdd(&B.ctr[par], 1);
     }
     __syncthreads();
-    const int A = s_lvl[0], S = s_lvl[1];
-    int tile = s_lvl[2];
+    const int A = s_lvl[0], active_index = s_lvl[1], S = s_lvl[2];
+    const long long cap_nodes = L->cap_nodes;
+    const bool es = empty_space && active_index != 0;   // the root level has no empty-space cuts
     const int ntiles = max(1, (A + NT - 1) / NT);
     // the next level's box keys: every possible child slot (created <= 2A),
     // before any scatter joins into them
@@ -893,12 +877,27 @@ __global__ __launch_bounds__(BS) void leaf_pass_kernel(Lvl *L, LevelBufs B, int 
         const int kmin = fkey(F_INF), kmax = fkey(-F_INF);
         GRID_STRIDE(k, 12 * A) keys_next[k] = (k % 6) < 3 ? kmin : kmax;
     }
-    for (; tile < ntiles; tile = next_tile(&B.ctr[2], &s_tile)) {
+    for (int tile = s_lvl[3]; tile < ntiles; tile = next_tile(&B.ctr[par], &s_tile)) {
         const int i0 = tile * NT, n = min(NT, A - i0);
+        // every node input first, then the node records (the stores would
+        // otherwise hold back the next node's loads)
+        NodeIn in[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int li = 4 * t + j;
+            if (li < n) load_node(N, i0 + li, active_index + i0 + li, keys, es, in[j]);
+        }
         for (int li = t; li <= n; li += BS) s_sb[li] = n > 0 ? B.segbase[i0 + li] : 0;
+        int pl[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int li = 4 * t + j;
+            pl[j] = 0;
+            if (li < n) prepare_node(N, active_index + i0 + li, in[j], es, thr, pl[j]);
+        }
         __syncthreads();
         // the tile's segments [sb, se): the side counts' exclusive prefixes
-        // inside the tile, written to segLoff/segRoff (offset below)
+        // inside the tile
         const int sb = s_sb[0], se = s_sb[n];
         int cL = 0, cR = 0;
         for (int c0 = sb; c0 < se; c0 += NT) {
@@ -912,15 +911,15 @@ __global__ __launch_bounds__(BS) void leaf_pass_kernel(Lvl *L, LevelBufs B, int 
             }
             int pre[2], tot[2];
             block_scan<2>(v, pre, tot, s_w);
-            int pl = cL + pre[0], pr = cR + pre[1];
+            int pl2 = cL + pre[0], pr2 = cR + pre[1];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int s = c0 + 4 * t + j;
                 if (s < se) {
-                    if (s - sb < SEG_LDS) { s_lp[s - sb] = pl; s_rp[s - sb] = pr; }
-                    else { B.segLoff[s] = pl; B.segRoff[s] = pr; }
+                    if (s - sb < SEG_LDS) { s_lp[s - sb] = pl2; s_rp[s - sb] = pr2; }
+                    else { B.segLoff[s] = pl2; B.segRoff[s] = pr2; }
                 }
-                pl += sl[j]; pr += sr[j];
+                pl2 += sl[j]; pr2 += sr[j];
             }
             cL += tot[0]; cR += tot[1];
         }
@@ -931,8 +930,10 @@ __global__ __launch_bounds__(BS) void leaf_pass_kernel(Lvl *L, LevelBufs B, int 
             else if (s - sb < SEG_LDS) { l = s_lp[s - sb]; r = s_rp[s - sb]; }
             else { l = B.segLoff[s]; r = B.segRoff[s]; }
         };
-        // children sizes and leaf flags per node (CalcNodeChildSize, MarkNodeLeafs)
-        int v[4] = {0, 0, 0, 0}, nl[4], nr[4];
+        // per node: empty-space nodes, children sizes and leaf flags
+        // (CalcNodeChildSize, MarkNodeLeafs), leaf slots and refs, the
+        // non-leaf children's segments
+        int v[7] = {0, 0, 0, 0, 0, 0, 0}, nl[4], nr[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int li = 4 * t + j;
@@ -946,67 +947,100 @@ __global__ __launch_bounds__(BS) void leaf_pass_kernel(Lvl *L, LevelBufs B, int 
                     nl[j] = le - lb;
                     nr[j] = re - rb;
                 }
-                v[0] += nl[j] < LS;
-                v[1] += nr[j] < LS;
-                v[2] += nl[j] < LS ? nl[j] : 0;
-                v[3] += nr[j] < LS ? nr[j] : 0;
+                v[0] += 2 * __popc(pl[j]);
+                v[1] += nl[j] < LS;
+                v[2] += nr[j] < LS;
+                v[3] += nl[j] < LS ? nl[j] : 0;
+                v[4] += nr[j] < LS ? nr[j] : 0;
+                v[5] += nl[j] < LS ? 0 : (nl[j] + seg - 1) / seg;
+                v[6] += nr[j] < LS ? 0 : (nr[j] + seg - 1) / seg;
             }
         }
-        int pre[4], tot[4];
-        block_scan<4>(v, pre, tot, s_w);
+        int pre[7], tot[7];
+        block_scan<7>(v, pre, tot, s_w);
         if (t < 64) {
-            const long long agg[6] = {cL, cR, tot[0], tot[1], tot[2], tot[3]};
-            long long ex[6];
-            lookback<6>(B.lb3, tile, agg, ex, &L->status);
-            if (t == 0) for (int k = 0; k < 6; ++k) s_ex[k] = ex[k];
+            // the two segment counts share a word (each total < 2^31)
+            const long long agg[8] = {cL, cR, tot[0], tot[1], tot[2], tot[3], tot[4],
+                                      (long long)tot[5] | ((long long)tot[6] << 31)};
+            long long ex[8];
+            lookback<8>(B.lb, tile, agg, ex, &L->status);
+            if (t == 0) for (int k = 0; k < 8; ++k) s_ex[k] = ex[k];
         }
         __syncthreads();
-        // the side offsets of the tile's segments, now global
-        const int oL = (int)s_ex[0], oR = (int)s_ex[1];
-        for (int s = sb + t; s < se; s += BS) {
-            int l, r;
-            lpre(s, l, r);
-            B.segLoff[s] = l + oL;
-            B.segRoff[s] = r + oR;
+        const int oL = (int)s_ex[0], oR = (int)s_ex[1], ereq = (int)s_ex[2];
+        const int oSL = (int)(s_ex[7] & 0x7fffffff), oSR = (int)(s_ex[7] >> 31);
+        // the empty-space chains, only inside the capacity (a captured build
+        // that outgrew it fails in the plan below and writes nothing more)
+        const bool fits = (long long)active_index + A + ereq + tot[0] <= cap_nodes;
+        int r = pre[0];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int li = 4 * t + j;
+            if (li < n) {
+                if (pl[j] && fits) empty_chain_node_at(N, i0 + li, A, active_index, pl[j], ereq + r);
+                r += 2 * __popc(pl[j]);
+            }
         }
-        int p[4];
-        for (int k = 0; k < 4; ++k) p[k] = (int)s_ex[k + 2] + pre[k];
+        // the side offsets of the tile's segments, now global
+        for (int s = sb + t; s < se; s += BS) {
+            int l, rr;
+            lpre(s, l, rr);
+            B.segLoff[s] = l + oL;
+            B.segRoff[s] = rr + oR;
+        }
+        int p[6] = {(int)s_ex[3] + pre[1], (int)s_ex[4] + pre[2], (int)s_ex[5] + pre[3], (int)s_ex[6] + pre[4],
+                    oSL + pre[5], oSR + pre[6]};
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int li = 4 * t + j;
             if (li < n) {
                 B.lna[i0 + li] = make_int2(p[0], p[1]);
                 B.lref[i0 + li] = make_int2(p[2], p[3]);
+                B.lseg[i0 + li] = make_int2(p[4], p[5]);
                 p[0] += nl[j] < LS; p[1] += nr[j] < LS;
                 p[2] += nl[j] < LS ? nl[j] : 0; p[3] += nr[j] < LS ? nr[j] : 0;
+                p[4] += nl[j] < LS ? 0 : (nl[j] + seg - 1) / seg;
+                p[5] += nr[j] < LS ? 0 : (nr[j] + seg - 1) / seg;
             }
         }
         if (tile == ntiles - 1) {
             if (t == 0) {
-                long long T[6];
-                for (int k = 0; k < 6; ++k) T[k] = s_ex[k] + (k == 0 ? cL : k == 1 ? cR : tot[k - 2]);
-                B.segLoff[S] = (int)T[0];
-                B.segRoff[S] = (int)T[1];
-                const ChildPlan P = plan_children_totals(L, (int)T[0], (int)T[1], (int)T[4], (int)T[5],
-                                                         (int)(T[2] + T[3]));
-                if (P.status) {
-                    lvl_fail(L, P.status);
-                } else {
-                    L->L_all = P.L_all; L->R_all = P.R_all; L->LL_all = P.LL_all; L->LR_all = P.LR_all;
-                    L->leaf_nodes = P.leaf_nodes; L->leafTri = P.leafTri; L->nextR = P.nextR;
-                    L->created = P.created; L->child_start = P.child_start; L->next_active = P.next_active;
-                    L->A_scan = P.created;
-                    L->ai_scan = L->nodes_size + 2 * L->A - P.created;   // advance_level's active_index
-                    L->leaf_left = (int)T[2];
-                    L->advance = 1;
+                const int Lall = oL + cL, Rall = oR + cR, E = ereq + tot[0];
+                const int leafL = (int)s_ex[3] + tot[1], leafR = (int)s_ex[4] + tot[2];
+                const int LL = (int)s_ex[5] + tot[3], LR = (int)s_ex[6] + tot[4];
+                const long long SL = (long long)oSL + tot[5], SR = (long long)oSR + tot[6];
+                B.segLoff[S] = Lall;
+                B.segRoff[S] = Rall;
+                // the previous level's advance, this level's empty-space
+                // nodes and segments, then its children plan
+                plan_nodes(L, E, S);
+                if (!L->status) {
+                    const ChildPlan P = plan_children_totals(L, Lall, Rall, LL, LR, leafL + leafR);
+                    if (P.status) {
+                        lvl_fail(L, P.status);
+                    } else if (SL + SR > L->cap_segs) {
+                        lvl_fail(L, PM_ERR_CAPACITY);
+                    } else {
+                        L->L_all = P.L_all; L->R_all = P.R_all; L->LL_all = P.LL_all; L->LR_all = P.LR_all;
+                        L->leaf_nodes = P.leaf_nodes; L->leafTri = P.leafTri; L->nextR = P.nextR;
+                        L->created = P.created; L->child_start = P.child_start; L->next_active = P.next_active;
+                        L->leaf_left = leafL;
+                        L->SL_all = (int)SL;
+                        L->advance = 1;
+                        // the next level as its passes will read it (advance_level's values)
+                        L->scanA[par ^ 1] = P.created;
+                        L->scanAI[par ^ 1] = L->nodes_size + 2 * L->A - P.created;
+                        L->scanS[par ^ 1] = (int)(SL + SR);
+                        B.segbase_n[P.created] = (int)(SL + SR);
+                    }
                 }
-                B.ctr[0] = 0;
+                B.ctr[par ^ 1] = 0;
             }
             __syncthreads();
-            // the next level's node pass starts with fresh status words
+            // the next level's leaf pass starts with fresh status words
             const int created = L->status ? 0 : L->created;
-            const int nt1 = max(1, (created + NT - 1) / NT);
-            for (int k = t; k < 2 * nt1; k += BS) B.lb1[k] = 0ull;
+            const int nt = max(1, (created + NT - 1) / NT);
+            for (int k = t; k < 8 * nt; k += BS) B.lb_n[k] = 0ull;
         }
     }
 }
@@ -1016,6 +1050,7 @@ __global__ __launch_bounds__(BS) void leaf_pass_kernel(Lvl *L, LevelBufs B, int 
 // active node i; nl/nr its children's sizes, b its first segment
 struct LevelPlan {
     int A, active_index, child_start, next_active, leaf_nodes, leaf_left, L_all, LL_all, ulp, nleaf, LS;
+    int SL_all, seg;
 };
 
 __device__ __forceinline__ void child_ids(const LevelPlan &P, int i, const int2 &ln, bool leafL, bool leafR,
@@ -1061,13 +1096,16 @@ __device__ void write_children(const NodePtrs &N, const LevelPlan &P, const Leve
             N.bmax[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
         if (leaf[side]) leaf_ids[P.nleaf + lna[side]] = id;
+        // a non-leaf child's first segment in the next level's table
+        else B.segbase_n[id - P.next_active] = side == 0 ? B.lseg[i].x : P.SL_all + B.lseg[i].y;
     }
     N.children[par] = make_int2(ids[0], ids[1]);
 }
 
 // SplitTriangles (TriangleUpperChildren.h:283-383) for segment s: stable
 // partition of its refs with clipping, and the next level's box joins; the
-// wave of a node's first segment also writes the node's two children
+// wave of a node's first segment also writes the node's two children and
+// its non-leaf children's segments of the next level's table
 template <int SEGT, bool DIV>
 __device__ __forceinline__ void scatter_pass_segment(
     const NodePtrs &N, const LevelPlan &P, const LevelBufs &B, int s, int propagate,
@@ -1086,6 +1124,22 @@ __device__ __forceinline__ void scatter_pass_segment(
     child_ids(P, i, ln, leafL, leafR, idL, idR);
     if (s == b && lane == 0) write_children(N, P, B, i, propagate, leaf_ids);
     const int sL = B.segLoff[s], sR = B.segRoff[s];
+    if (s == b) {
+        // a non-leaf child's refs start where the scatter below puts them
+        const int2 ls = B.lseg[i];
+        const bool leaf[2] = {leafL, leafR};
+        const int sz[2] = {nl, nr}, ci[2] = {idL - P.next_active, idR - P.next_active};
+        const int first[2] = {sL - lr.x, (P.L_all - P.LL_all) + sR - lr.y};
+        const int sbase[2] = {ls.x, P.SL_all + ls.y};
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            if (leaf[side]) continue;
+            const int ns = (sz[side] + P.seg - 1) / P.seg;
+            for (int k = lane; k < ns; k += 64)
+                write_segment(B.seg_node_n, B.seg_start_n, B.seg_count_n, sbase[side] + k, ci[side], first[side],
+                              sz[side], P.seg, k);
+        }
+    }
     const int LLoff = lr.x + (leafL ? sL - B.segLoff[b] : 0);
     const int LRoff = lr.y + (leafR ? sR - B.segRoff[b] : 0);
     // one chunk per wave: its node's axis and plane are wave-uniform
@@ -1105,8 +1159,9 @@ __device__ __forceinline__ void scatter_pass_segment(
     int l0 = fkey(F_INF), l1 = l0, l2 = l0, l3 = fkey(-F_INF), l4 = l3, l5 = l3;
     int r0 = l0, r1 = l0, r2 = l0, r3 = l3, r4 = l3, r5 = l3;
     // groups of iterations: a group's refs are loaded first (lanes past the
-    // chunk re-read its first ref), then classified and written; groups of 8
-    // while the chunk has that many iterations left, then groups of 2
+    // chunk re-read its first ref), then classified and written; groups of 4
+    // while the chunk has that many iterations left, then groups of 2 (more
+    // would cost the kernel its fifth wave per SIMD)
     auto group = [&](int g0, auto GC) {
         constexpr int G = decltype(GC)::value;
         float4 gmn[G], gmx[G];
@@ -1157,7 +1212,7 @@ __device__ __forceinline__ void scatter_pass_segment(
             rr += __popcll(brm);
         }
     };
-    constexpr int GB = SEGT / 64 < 8 ? SEGT / 64 : 8;
+    constexpr int GB = SEGT / 64 < 4 ? SEGT / 64 : 4;
     const int its = (count + 63) >> 6;   // wave-uniform
     int g0 = 0;
     for (; g0 + GB <= its; g0 += GB) group(g0, std::integral_constant<int, GB>{});
@@ -1189,7 +1244,8 @@ __device__ __forceinline__ void scatter_pass_segment(
 }
 
 template <int SEGT, bool DIV>
-__device__ __forceinline__ void scatter_pass(const NodePtrs N, const Lvl *L, const LevelBufs B, int LS, int propagate,
+__device__ __forceinline__ void scatter_pass(const NodePtrs N, const Lvl *L, const LevelBufs B, int LS, int seg,
+                                             int propagate,
                                              const float4 *__restrict__ rmin, const float4 *__restrict__ rmax,
                                              float4 *__restrict__ nmin, float4 *__restrict__ nmax,
                                              float4 *__restrict__ fmin, float4 *__restrict__ fmax,
@@ -1199,6 +1255,7 @@ __device__ __forceinline__ void scatter_pass(const NodePtrs N, const Lvl *L, con
     P.A = L->A; P.active_index = L->active_index; P.child_start = L->child_start; P.next_active = L->next_active;
     P.leaf_nodes = L->leaf_nodes; P.leaf_left = L->leaf_left; P.L_all = L->L_all; P.LL_all = L->LL_all;
     P.ulp = L->final_refs; P.nleaf = L->nleaf; P.LS = LS;
+    P.SL_all = L->SL_all; P.seg = seg;
     const int S = L->S;
     for (int s = wave_index(); s < S; s += wave_count())
         scatter_pass_segment<SEGT, DIV>(N, P, B, s, propagate, rmin, rmax, nmin, nmax, fmin, fmax, keys_next,
@@ -1210,39 +1267,45 @@ __device__ __forceinline__ void scatter_pass(const NodePtrs N, const Lvl *L, con
 
 template <int SEGT>
 __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(4))) void scatter_pass_kernel(
-    const NodePtrs N, const Lvl *L, const LevelBufs B, int LS, int propagate,
+    const NodePtrs N, const Lvl *L, const LevelBufs B, int LS, int seg, int propagate,
     const float4 *__restrict__ rmin, const float4 *__restrict__ rmax, float4 *__restrict__ nmin,
     float4 *__restrict__ nmax, float4 *__restrict__ fmin, float4 *__restrict__ fmax, int *__restrict__ keys_next,
     int *__restrict__ leaf_ids, DivideGeom dg) {
-    scatter_pass<SEGT, false>(N, L, B, LS, propagate, rmin, rmax, nmin, nmax, fmin, fmax, keys_next, leaf_ids, dg);
+    scatter_pass<SEGT, false>(N, L, B, LS, seg, propagate, rmin, rmax, nmin, nmax, fmin, fmax, keys_next, leaf_ids,
+                              dg);
 }
 
 template <int SEGT>
 __global__ __launch_bounds__(WPB * 64) void scatter_pass_div_kernel(
-    const NodePtrs N, const Lvl *L, const LevelBufs B, int LS, int propagate,
+    const NodePtrs N, const Lvl *L, const LevelBufs B, int LS, int seg, int propagate,
     const float4 *__restrict__ rmin, const float4 *__restrict__ rmax, float4 *__restrict__ nmin,
     float4 *__restrict__ nmax, float4 *__restrict__ fmin, float4 *__restrict__ fmax, int *__restrict__ keys_next,
     int *__restrict__ leaf_ids, DivideGeom dg) {
-    scatter_pass<SEGT, true>(N, L, B, LS, propagate, rmin, rmax, nmin, nmax, fmin, fmax, keys_next, leaf_ids, dg);
+    scatter_pass<SEGT, true>(N, L, B, LS, seg, propagate, rmin, rmax, nmin, nmax, fmin, fmax, keys_next, leaf_ids,
+                             dg);
 }
 
-// look-back words and tickets of the first level
-__global__ void lb_init_kernel(unsigned long long *lb1, int n1, int *ctr) {
-    GRID_STRIDE(k, n1) lb1[k] = 0ull;
-    if (blockIdx.x == 0 && threadIdx.x < 3) ctr[threadIdx.x] = 0;
+// look-back words and tickets of the roots' segment scan (parity 1) and of
+// the first level's leaf pass (parity 0)
+__global__ void lb_init_kernel(unsigned long long *lb0, unsigned long long *lb1, int n, int *ctr) {
+    GRID_STRIDE(k, n) {
+        lb0[k] = 0ull;
+        lb1[k] = 0ull;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < 2) ctr[threadIdx.x] = 0;
 }
 
 // ---- plan kernels: the host arithmetic of the level loop
 // (TriangleMapUpperCreator.cu:140-180), on the device
-__device__ void lvl_fail(Lvl *L, int code) {
+__host__ __device__ void lvl_fail(Lvl *L, int code) {
     if (!L->status) L->status = code;
-    L->A = 0; L->S = 0; L->E = 0; L->An = 0; L->A_scan = 0; L->advance = 0;
+    L->A = 0; L->S = 0; L->E = 0; L->An = 0; L->advance = 0;
+    L->scanA[0] = L->scanA[1] = 0;
+    L->scanS[0] = L->scanS[1] = 0;
 }
 
-// after the node pass: the previous level's advance, then the empty-space
-// nodes and segments of the level (run by the node scan's thread that holds
-// both totals; the scan's other blocks read only A_scan, which this changes
-// only on failure)
+// in the leaf pass's last tile: the previous level's advance, then the
+// empty-space nodes and segments of the level
 __device__ void plan_nodes(Lvl *L, int E, int S) {
     advance_level(L);
     // a failed advance (depth) has zeroed the level: E and S, counted over
@@ -1260,8 +1323,9 @@ __device__ void plan_nodes(Lvl *L, int E, int S) {
 }
 
 // after children and scatter: advance to the next level (applied by the next
-// node scan, or by plan_finish_kernel after the last captured level)
-__device__ void advance_level(Lvl *L) {
+// level's leaf pass, or by plan_finish_kernel after the last level; the eager
+// build applies it to its host copy of the state too)
+__host__ __device__ void advance_level(Lvl *L) {
     if (!L->advance) return;
     L->advance = 0;
     if (L->status || L->A == 0) return;
@@ -1921,7 +1985,8 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
     const long long BIG = LLONG_MAX / 4;
 
     Lvl h{};
-    h.A = M; h.active_index = 0; h.nodes_size = M; h.R = N; h.A_scan = M; h.ai_scan = 0; h.advance = 0;
+    h.A = M; h.active_index = 0; h.nodes_size = M; h.R = N; h.advance = 0;
+    h.scanA[0] = M; h.scanAI[0] = 0;   // scanS[0]: root_segments_kernel
     h.max_levels = prm.max_upper_levels > 0 ? prm.max_upper_levels : 128;
     h.max_refs = (long long)(prm.max_ref_factor > 0 ? prm.max_ref_factor : 16) * N + 65536;
     h.cap_nodes = graph ? caps->nodes : BIG;
@@ -1958,95 +2023,103 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
     const DivideGeom dg = {prm.split_method == PM_SPLIT_DIVIDE ? P0 : nullptr, P1, P2};
     const bool div = dg.p0 != nullptr;
     auto tiles = [](long long n, long long per) { return std::max(1ll, (n + per - 1) / per); };
-    auto level_bufs = [&]() {
+    auto level_bufs = [&](int par) {
+        const int q = par ^ 1;
         LevelBufs B;
-        B.eaddr = sc.node_eaddr.ptr; B.segbase = sc.node_segbase.ptr;
-        B.seg_node = sc.seg_node.ptr; B.seg_start = sc.seg_start.ptr; B.seg_count = sc.seg_count.ptr;
+        B.segbase = sc.node_segbase[par].ptr;
+        B.seg_node = sc.seg_node[par].ptr; B.seg_start = sc.seg_start[par].ptr; B.seg_count = sc.seg_count[par].ptr;
         B.segL = sc.segL.ptr; B.segR = sc.segR.ptr;
         B.segLoff = sc.segLoff.ptr; B.segRoff = sc.segRoff.ptr;
-        B.lna = sc.node_lna.ptr; B.lref = sc.node_lref.ptr;
-        B.lb1 = sc.lb_words[0].ptr; B.lb3 = sc.lb_words[2].ptr;
+        B.lna = sc.node_lna.ptr; B.lref = sc.node_lref.ptr; B.lseg = sc.node_lseg.ptr;
+        B.segbase_n = sc.node_segbase[q].ptr;
+        B.seg_node_n = sc.seg_node[q].ptr; B.seg_start_n = sc.seg_start[q].ptr; B.seg_count_n = sc.seg_count[q].ptr;
+        B.lb = sc.lb_words[par].ptr; B.lb_n = sc.lb_words[q].ptr;
         B.ctr = sc.lb_ctr.ptr;
         return B;
     };
+
+    // the roots' segment table (parity 0); eager: sized by the bound M +
+    // ceil(N / seg) (at most one partial segment per root)
+    long long S_now = graph ? caps->segs : M + (N + seg - 1) / seg;
     if (!graph) {
         sc.lb_ctr.reserve(4, s);
-        sc.lb_words[0].reserve(2 * tiles(M, NT) + 2, s);
+        for (int q = 0; q < 2; ++q) sc.lb_words[q].reserve(8 * tiles(M, NT) + 8, s);
+        sc.node_segbase[0].reserve((size_t)M + 1, s);
+        for (DevBuf<int> *b : {&sc.seg_node[0], &sc.seg_start[0], &sc.seg_count[0]}) b->reserve(S_now + 1, s);
     }
-    lb_init_kernel<<<grid_for(2 * tiles(M, NT), BS, 64), BS, 0, s>>>(sc.lb_words[0].ptr, 2 * (int)tiles(M, NT),
-                                                                    sc.lb_ctr.ptr);
+    lb_init_kernel<<<grid_for(8 * tiles(M, NT) + 8, BS, 64), BS, 0, s>>>(
+        sc.lb_words[0].ptr, sc.lb_words[1].ptr, (int)(8 * tiles(M, NT) + 8), sc.lb_ctr.ptr);
+    PM_LAUNCH_CHECK();
+    root_segments_kernel<<<(int)tiles(M, NT), BS, 0, s>>>(node_ptrs(ctx), L, sc.node_segbase[0].ptr,
+                                                           sc.seg_node[0].ptr, sc.seg_start[0].ptr,
+                                                           sc.seg_count[0].ptr, sc.lb_words[1].ptr,
+                                                           sc.lb_ctr.ptr + 1, M, seg);
     PM_LAUNCH_CHECK();
 
-    // A level is four launches (node, classify, leaf and scatter passes; see
-    // the fused-level kernels above).  The eager build reads the level state
-    // back once per level, after the node pass, to size the other three
-    // passes' grids and buffers.  What the node pass writes is reserved from
-    // bounds beforehand: a level has at most twice the previous level's
-    // active nodes and refs, at most 12 empty-space nodes and 2 children per
-    // active node, and at most one segment per active node plus one per chunk
-    // of refs.  The loop ends at the read that finds no active node.
-    int cur = 0;
-    long long A_bound = M, R_bound = N, written = M;
+    // A level is three launches (classify, leaf and scatter passes; see the
+    // level kernels above).  The eager build reads the level state back once
+    // per level, after the leaf pass, to size the scatter pass's outputs and
+    // the next level; what the classify and leaf passes write is reserved
+    // beforehand (a level has at most 12 empty-space nodes and 2 children per
+    // active node, so at most 2A active nodes next).  It ends after the
+    // scatter of the level that creates no active node.
+    long long A_now = M, nodes_now = M;
     for (int lvl = 0; graph ? lvl < caps->upper_levels : true; ++lvl) {
+        const int par = lvl & 1, nxt = par ^ 1;
+        const long long A_g = graph ? caps->active : A_now, S_g = graph ? caps->segs : S_now;
         if (!graph) {
-            const long long S_bound = A_bound + (R_bound + seg - 1) / seg + 1;
-            nodes.reserve((size_t)(written + 14 * A_bound + 1), s, (size_t)written);
-            sc.node_eaddr.reserve(A_bound + 1, s);
-            sc.node_segbase.reserve(A_bound + 1, s);
-            sc.seg_node.reserve(S_bound + 1, s); sc.seg_start.reserve(S_bound + 1, s);
-            sc.seg_count.reserve(S_bound + 1, s);
-            sc.lb_words[2].reserve(6 * tiles(A_bound, NT) + 6, s);
+            for (DevBuf<int> *b : {&sc.segL, &sc.segR, &sc.segLoff, &sc.segRoff}) b->reserve(S_now + 1, s);
+            sc.node_lna.reserve(A_now + 1, s); sc.node_lref.reserve(A_now + 1, s); sc.node_lseg.reserve(A_now + 1, s);
+            nodes.reserve((size_t)(nodes_now + 14 * A_now + 1), s, (size_t)nodes_now);
+            sc.keys[nxt].reserve(12 * (size_t)A_now + 6, s);
+            sc.node_segbase[nxt].reserve(2 * (size_t)A_now + 2, s);
+            sc.lb_words[nxt].reserve(8 * tiles(2 * A_now, NT) + 8, s);
         }
-        node_pass_kernel<<<grid_for(graph ? caps->active : A_bound, NT, graph ? 1024 : INT_MAX), BS, 0, s>>>(
-            node_ptrs(ctx), L, level_bufs(), seg, sc.keys[cur].ptr, prm.empty_space_splitting,
+        const NodePtrs NP = node_ptrs(ctx);
+        LevelBufs B = level_bufs(par);
+        launch_classify_pass(seg, div, grid_for(S_g, WPB, graph ? 2048 : INT_MAX), s, (const Lvl *)L, B, par,
+                             (const int *)sc.keys[par].ptr, (const float4 *)sc.ref_min[par].ptr,
+                             (const float4 *)sc.ref_max[par].ptr, dg);
+        PM_LAUNCH_CHECK();
+        const int g3 = std::max(grid_for(A_g, NT, INT_MAX), grid_for(12 * A_g, 16 * BS, 2048));
+        leaf_pass_kernel<<<graph ? std::min(g3, 2048) : g3, BS, 0, s>>>(
+            NP, L, B, par, LS, seg, sc.keys[par].ptr, sc.keys[nxt].ptr, prm.empty_space_splitting,
             prm.empty_space_threshold);
         PM_LAUNCH_CHECK();
-        const int nxt = cur ^ 1;
         if (!graph) {
             read_state(sc, h, s);
             if (h.status) throw_status(h);
-            if (h.A == 0) break;
             ctx->level_refs.push_back(h.R);
             seen.refs = std::max<long long>(seen.refs, h.R);
             seen.active = std::max<long long>(seen.active, h.A);
             seen.segs = std::max<long long>(seen.segs, h.S);
-            sc.segL.reserve((size_t)h.S + 1, s); sc.segR.reserve((size_t)h.S + 1, s);
-            sc.segLoff.reserve((size_t)h.S + 1, s); sc.segRoff.reserve((size_t)h.S + 1, s);
-            sc.node_lna.reserve((size_t)h.A + 1, s); sc.node_lref.reserve((size_t)h.A + 1, s);
-            sc.lb_words[0].reserve(2 * tiles(2ll * h.A, NT) + 2, s);
-            const size_t R2 = 2 * (size_t)h.R;
-            sc.prim_min.reserve((size_t)h.final_refs + R2 + 1, s, h.final_refs);
-            sc.prim_max.reserve((size_t)h.final_refs + R2 + 1, s, h.final_refs);
-            ctx->leaf_ids.reserve((size_t)h.nleaf + 2 * (size_t)h.A + 1, s, h.nleaf);
-            sc.ref_min[nxt].reserve(R2 + 1, s); sc.ref_max[nxt].reserve(R2 + 1, s);
-            sc.keys[nxt].reserve(12 * (size_t)h.A + 6, s);
+            seen.segs = std::max<long long>(seen.segs, h.scanS[nxt]);
+            sc.prim_min.reserve((size_t)h.final_refs + h.leafTri + 1, s, h.final_refs);
+            sc.prim_max.reserve((size_t)h.final_refs + h.leafTri + 1, s, h.final_refs);
+            ctx->leaf_ids.reserve((size_t)h.nleaf + h.leaf_nodes + 1, s, h.nleaf);
+            sc.ref_min[nxt].reserve((size_t)h.nextR + 1, s); sc.ref_max[nxt].reserve((size_t)h.nextR + 1, s);
+            for (DevBuf<int> *b : {&sc.seg_node[nxt], &sc.seg_start[nxt], &sc.seg_count[nxt]})
+                b->reserve((size_t)h.scanS[nxt] + 1, s);
+            B = level_bufs(par);
         }
-        const long long S_now = graph ? caps->segs : h.S, A_now = graph ? caps->active : h.A;
-        const LevelBufs B = level_bufs();
-        const NodePtrs NP = node_ptrs(ctx);
-        launch_classify_pass(seg, div, grid_for(S_now, WPB, graph ? 2048 : INT_MAX), s, NP, (const Lvl *)L, B,
-                             (const float4 *)sc.ref_min[cur].ptr, (const float4 *)sc.ref_max[cur].ptr, dg);
-        PM_LAUNCH_CHECK();
-        const int g3 = std::max(grid_for(A_now, NT, INT_MAX), grid_for(12 * A_now, 16 * BS, 2048));
-        leaf_pass_kernel<<<graph ? std::min(g3, 2048) : g3, BS, 0, s>>>(L, B, LS, sc.keys[nxt].ptr);
-        PM_LAUNCH_CHECK();
-        const int g4 = std::max(grid_for(S_now, WPB, INT_MAX), grid_for(A_now, BS, INT_MAX));
-        launch_scatter_pass(seg, div, graph ? std::min(g4, 2048) : g4, s, NP, (const Lvl *)L, B, LS,
-                            prm.empty_space_splitting, (const float4 *)sc.ref_min[cur].ptr,
-                            (const float4 *)sc.ref_max[cur].ptr, sc.ref_min[nxt].ptr, sc.ref_max[nxt].ptr,
+        const long long S_sc = graph ? caps->segs : h.S, A_sc = graph ? caps->active : h.A;
+        const int g4 = std::max(grid_for(S_sc, WPB, INT_MAX), grid_for(A_sc, BS, INT_MAX));
+        launch_scatter_pass(seg, div, graph ? std::min(g4, 2048) : g4, s, node_ptrs(ctx), (const Lvl *)L, B, LS, seg,
+                            prm.empty_space_splitting, (const float4 *)sc.ref_min[par].ptr,
+                            (const float4 *)sc.ref_max[par].ptr, sc.ref_min[nxt].ptr, sc.ref_max[nxt].ptr,
                             sc.prim_min.ptr, sc.prim_max.ptr, sc.keys[nxt].ptr, ctx->leaf_ids.ptr, dg);
         PM_LAUNCH_CHECK();
         if (!graph) {
-            written = (long long)h.nodes_size + 2ll * h.A;
-            A_bound = 2ll * h.A;
-            R_bound = 2ll * h.R;
+            if (h.created == 0) break;
+            nodes_now = (long long)h.nodes_size + 2ll * h.A;
+            A_now = h.created;
+            S_now = h.scanS[nxt];
         }
-        cur = nxt;
     }
-    if (graph) {
-        plan_finish_kernel<<<1, 64, 0, s>>>(L, 0);
-        PM_LAUNCH_CHECK();
-    }
+    // the last level's advance
+    plan_finish_kernel<<<1, 64, 0, s>>>(L, 0);
+    PM_LAUNCH_CHECK();
+    if (!graph) advance_level(&h);
 
     // leaf slots: prim indices, the tracers' leaf triangles and (SAH) areas
     NodePtrs NP = node_ptrs(ctx);
@@ -2210,15 +2283,18 @@ void reserve_build_caps(pm_ctx *ctx, const BuildSizes &caps, int N, hipStream_t 
         // child slot of its active nodes (12 per node)
         sc.keys[i].reserve(12 * (size_t)caps.active + 12, s);
     }
-    for (DevBuf<int> *b : {&sc.node_eaddr, &sc.node_segbase}) b->reserve((size_t)caps.active + 1, s);
+    for (int q = 0; q < 2; ++q) {
+        sc.node_segbase[q].reserve((size_t)caps.active + 1, s);
+        for (DevBuf<int> *b : {&sc.seg_node[q], &sc.seg_start[q], &sc.seg_count[q]}) b->reserve((size_t)caps.segs + 1, s);
+    }
     sc.node_lna.reserve((size_t)caps.active + 1, s);
     sc.node_lref.reserve((size_t)caps.active + 1, s);
-    for (DevBuf<int> *b : {&sc.seg_node, &sc.seg_start, &sc.seg_count, &sc.segL, &sc.segR, &sc.segLoff,
-                           &sc.segRoff})
-        b->reserve((size_t)caps.segs + 1, s);
+    sc.node_lseg.reserve((size_t)caps.active + 1, s);
+    for (DevBuf<int> *b : {&sc.segL, &sc.segR, &sc.segLoff, &sc.segRoff}) b->reserve((size_t)caps.segs + 1, s);
+    // the leaf passes' status words (8 per tile of NT nodes; the roots' scan
+    // uses parity 1's, M <= caps.active)
     const long long node_tiles = (caps.active + NT - 1) / NT + 1;
-    sc.lb_words[0].reserve(2 * (size_t)node_tiles + 2, s);
-    sc.lb_words[2].reserve(6 * (size_t)node_tiles + 6, s);
+    for (int q = 0; q < 2; ++q) sc.lb_words[q].reserve(8 * (size_t)node_tiles + 8, s);
     sc.lb_ctr.reserve(4, s);
     sc.prim_min.reserve((size_t)caps.leaf_refs + 1, s);
     sc.prim_max.reserve((size_t)caps.leaf_refs + 1, s);
@@ -2317,18 +2393,20 @@ std::vector<void **> graph_buffers(pm_ctx *ctx) {
         (void **)&a.prim_index.ptr, (void **)&a.bitmap.ptr, (void **)&a.children.ptr, (void **)&a.area.ptr,
         (void **)&a.parent.ptr, (void **)&ctx->prim_indices.ptr, (void **)&ctx->leaf_ids.ptr,
         (void **)&ctx->packed.ptr, (void **)&ctx->leaf_tris.ptr, (void **)&sc.prim_min.ptr,
-        (void **)&sc.prim_max.ptr, (void **)&sc.node_eaddr.ptr, (void **)&sc.node_segbase.ptr,
-        (void **)&sc.seg_node.ptr, (void **)&sc.seg_start.ptr, (void **)&sc.seg_count.ptr,
-        (void **)&sc.segL.ptr, (void **)&sc.segR.ptr,
+        (void **)&sc.prim_max.ptr, (void **)&sc.segL.ptr, (void **)&sc.segR.ptr,
         (void **)&sc.segLoff.ptr, (void **)&sc.segRoff.ptr, (void **)&sc.node_lna.ptr, (void **)&sc.node_lref.ptr,
-        (void **)&sc.lb_words[0].ptr, (void **)&sc.lb_words[2].ptr,
-        (void **)&sc.lb_ctr.ptr, (void **)&sc.areas.ptr, (void **)&sc.lower_flag.ptr,
+        (void **)&sc.node_lseg.ptr, (void **)&sc.lb_ctr.ptr, (void **)&sc.areas.ptr, (void **)&sc.lower_flag.ptr,
         (void **)&sc.lower_off.ptr, (void **)&sc.lower_sets.ptr, (void **)&sc.lower_careas.ptr,
         (void **)&sc.lower_list.ptr, (void **)&sc.tri_rec.ptr, (void **)&ctx->graph_off.ptr, (void **)&sc.lvl.ptr};
     for (int i = 0; i < 2; ++i) {
         v.push_back((void **)&sc.ref_min[i].ptr);
         v.push_back((void **)&sc.ref_max[i].ptr);
         v.push_back((void **)&sc.keys[i].ptr);
+        v.push_back((void **)&sc.node_segbase[i].ptr);
+        v.push_back((void **)&sc.seg_node[i].ptr);
+        v.push_back((void **)&sc.seg_start[i].ptr);
+        v.push_back((void **)&sc.seg_count[i].ptr);
+        v.push_back((void **)&sc.lb_words[i].ptr);
     }
     for (int i = 0; i < 3; ++i) v.push_back((void **)&sc.scan_tmp[i].ptr);
     for (int i = 0; i < 2; ++i) {

@@ -118,7 +118,12 @@ struct Lvl {
     int An, list_base, use_list, splits, lower_base, lower_levels;   // lower levels
     int cand_n;                                        // lower level: nodes listed for the full SAH search
     int advance;                                       // the children plan is stored, the level not yet advanced
-    int A_scan, ai_scan;                               // active nodes of the coming node pass and the first's index
+    // a level's active nodes, their first index and segments as the level's
+    // classify and leaf passes read them, by level parity: written before
+    // the level starts (the level before's leaf pass, or the build's start)
+    // and not changed while those passes run
+    int scanA[2], scanAI[2], scanS[2];
+    int SL_all;                                        // next-level segments of this level's left children
     int leaf_left;                                     // leaf slots among the level's left children
     // work counts of the build (the algorithmic-bytes formula's inputs)
     unsigned long long ref_reads;                      // refs of every upper level, summed
@@ -144,13 +149,14 @@ struct BuildScratch {
     DevBuf<float4> ref_min[2], ref_max[2];     // level ref arrays (ping-pong)
     DevBuf<float4> prim_min, prim_max;         // final (upper leaf) refs
     DevBuf<int> keys[2];                        // 6 ordered-int box keys per active node
-    DevBuf<int> node_eaddr, node_segbase;       // per active node: first empty-space node, first segment
-    DevBuf<int> seg_node, seg_start, seg_count; // segment table
+    DevBuf<int> node_segbase[2];                // per active node: first segment (by level parity)
+    DevBuf<int> seg_node[2], seg_start[2], seg_count[2];   // segment tables (by level parity)
     DevBuf<int> segL, segR;                     // per segment: left / right refs
     DevBuf<int> segLoff, segRoff;               // per segment: left / right refs before it
     DevBuf<int2> node_lna, node_lref;           // per active node: leaf slots and leaf refs before it (left, right)
-    DevBuf<unsigned long long> lb_words[3];     // look-back status words of the node and leaf passes ([1] unused)
-    DevBuf<int> lb_ctr;                         // their tile tickets
+    DevBuf<int2> node_lseg;                     // per active node: its non-leaf children's segments before it
+    DevBuf<unsigned long long> lb_words[2];     // look-back status words of the leaf passes (by level parity)
+    DevBuf<int> lb_ctr;                         // their tile tickets (by level parity)
     DevBuf<int> scan_tmp[3];                    // block sums for scans, per level
     DevBuf<int> totals;                         // small device-side counters
     DevBuf<float> areas;                        // lower stage: triangle areas per final ref
