@@ -851,7 +851,8 @@ constexpr int SEG_LDS = 2048;
 // their first ticket; the last tile writes the next level's into [par ^ 1].
 __global__ __launch_bounds__(BS) void leaf_pass_kernel(NodePtrs N, Lvl *L, LevelBufs B, int par, int LS, int seg,
                                                        const int *__restrict__ keys, int *__restrict__ keys_next,
-                                                       int empty_space, float thr) {
+                                                       int empty_space, float thr, LvlMirror::Rec *mirror,
+                                                       unsigned seq) {
     __shared__ int s_tile;
     __shared__ int s_w[7][BS / 64];
     __shared__ long long s_ex[8];
@@ -1041,6 +1042,18 @@ __global__ __launch_bounds__(BS) void leaf_pass_kernel(NodePtrs N, Lvl *L, Level
             const int created = L->status ? 0 : L->created;
             const int nt = max(1, (created + NT - 1) / NT);
             for (int k = t; k < 8 * nt; k += BS) B.lb_n[k] = 0ull;
+            // an eager build: the level state to the host, then its number
+            if (mirror) {
+                constexpr int NW = (int)(sizeof(Lvl) / sizeof(int));
+                static_assert(NW <= BS, "one word per thread");
+                if (t < NW) {
+                    __hip_atomic_store(reinterpret_cast<int *>(&mirror->lvl) + t, reinterpret_cast<const int *>(L)[t],
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    __threadfence_system();
+                }
+                __syncthreads();
+                if (t == 0) __hip_atomic_store(&mirror->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
         }
     }
 }
@@ -1951,6 +1964,22 @@ void read_state(BuildScratch &sc, Lvl &h, hipStream_t s) {
     std::memcpy(&h, sc.host.ptr, sizeof(Lvl));
 }
 
+// an eager level's state as its leaf pass published it (sequence number
+// seq); the stream is queried now and then, so a pass that failed to run
+// reports the stream's error instead of being waited for
+void wait_level(BuildScratch &sc, Lvl &h, unsigned seq, hipStream_t s) {
+    const LvlMirror::Rec *m = sc.mirror.host;
+    for (unsigned it = 1; __atomic_load_n(&m->seq, __ATOMIC_ACQUIRE) != seq; ++it) {
+        if (it % 4096) continue;
+        const hipError_t q = hipStreamQuery(s);
+        if (q == hipErrorNotReady) continue;
+        PM_HIP(q);
+        if (__atomic_load_n(&m->seq, __ATOMIC_ACQUIRE) == seq) break;
+        throw Error{PM_ERR_INTERNAL, "a level's state was not published"};
+    }
+    std::memcpy(&h, (const void *)&m->lvl, sizeof(Lvl));
+}
+
 [[noreturn]] void throw_status(const Lvl &h) {
     switch (h.status) {
     case PM_ERR_DEPTH:
@@ -2000,6 +2029,7 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
     seen.active = M; seen.refs = N;
 
     sc.lvl.reserve(1, s);
+    if (!graph) sc.mirror.ensure();
     Lvl *L = sc.lvl.ptr;
     init_lvl_kernel<<<1, 64, 0, s>>>(L, h);
     PM_LAUNCH_CHECK();
@@ -2057,13 +2087,14 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
     PM_LAUNCH_CHECK();
 
     // A level is three launches (classify, leaf and scatter passes; see the
-    // level kernels above).  The eager build reads the level state back once
-    // per level, after the leaf pass, to size the scatter pass's outputs and
-    // the next level; what the classify and leaf passes write is reserved
-    // beforehand (a level has at most 12 empty-space nodes and 2 children per
-    // active node, so at most 2A active nodes next).  It ends after the
-    // scatter of the level that creates no active node.
-    long long A_now = M, nodes_now = M;
+    // level kernels above), launched back to back.  The eager build reserves
+    // every buffer a level writes from bounds known before it starts (at most
+    // 12 empty-space nodes and 2 children per active node, at most 2R refs
+    // written, at most one segment per child plus one per chunk of its
+    // refs), then waits for the state the level's leaf pass publishes (while
+    // the scatter pass runs) to size the next level.  It ends after the
+    // level that creates no active node.
+    long long A_now = M, R_now = N, nodes_now = M, fin_now = 0, nleaf_now = 0;
     for (int lvl = 0; graph ? lvl < caps->upper_levels : true; ++lvl) {
         const int par = lvl & 1, nxt = par ^ 1;
         const long long A_g = graph ? caps->active : A_now, S_g = graph ? caps->segs : S_now;
@@ -2074,46 +2105,46 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
             sc.keys[nxt].reserve(12 * (size_t)A_now + 6, s);
             sc.node_segbase[nxt].reserve(2 * (size_t)A_now + 2, s);
             sc.lb_words[nxt].reserve(8 * tiles(2 * A_now, NT) + 8, s);
+            const size_t R2 = 2 * (size_t)R_now;
+            sc.prim_min.reserve((size_t)fin_now + R2 + 1, s, (size_t)fin_now);
+            sc.prim_max.reserve((size_t)fin_now + R2 + 1, s, (size_t)fin_now);
+            ctx->leaf_ids.reserve((size_t)nleaf_now + 2 * (size_t)A_now + 1, s, (size_t)nleaf_now);
+            sc.ref_min[nxt].reserve(R2 + 1, s); sc.ref_max[nxt].reserve(R2 + 1, s);
+            const size_t S2 = 2 * (size_t)A_now + (R2 + seg - 1) / seg + 1;
+            for (DevBuf<int> *b : {&sc.seg_node[nxt], &sc.seg_start[nxt], &sc.seg_count[nxt]}) b->reserve(S2, s);
         }
+        const LevelBufs B = level_bufs(par);
         const NodePtrs NP = node_ptrs(ctx);
-        LevelBufs B = level_bufs(par);
         launch_classify_pass(seg, div, grid_for(S_g, WPB, graph ? 2048 : INT_MAX), s, (const Lvl *)L, B, par,
                              (const int *)sc.keys[par].ptr, (const float4 *)sc.ref_min[par].ptr,
                              (const float4 *)sc.ref_max[par].ptr, dg);
         PM_LAUNCH_CHECK();
         const int g3 = std::max(grid_for(A_g, NT, INT_MAX), grid_for(12 * A_g, 16 * BS, 2048));
+        const unsigned seq = graph ? 0u : ++sc.mirror.next_seq;
         leaf_pass_kernel<<<graph ? std::min(g3, 2048) : g3, BS, 0, s>>>(
             NP, L, B, par, LS, seg, sc.keys[par].ptr, sc.keys[nxt].ptr, prm.empty_space_splitting,
-            prm.empty_space_threshold);
+            prm.empty_space_threshold, graph ? nullptr : sc.mirror.dev, seq);
+        PM_LAUNCH_CHECK();
+        const int g4 = std::max(grid_for(S_g, WPB, INT_MAX), grid_for(A_g, BS, INT_MAX));
+        launch_scatter_pass(seg, div, graph ? std::min(g4, 2048) : g4, s, NP, (const Lvl *)L, B, LS, seg,
+                            prm.empty_space_splitting, (const float4 *)sc.ref_min[par].ptr,
+                            (const float4 *)sc.ref_max[par].ptr, sc.ref_min[nxt].ptr, sc.ref_max[nxt].ptr,
+                            sc.prim_min.ptr, sc.prim_max.ptr, sc.keys[nxt].ptr, ctx->leaf_ids.ptr, dg);
         PM_LAUNCH_CHECK();
         if (!graph) {
-            read_state(sc, h, s);
+            wait_level(sc, h, seq, s);
             if (h.status) throw_status(h);
             ctx->level_refs.push_back(h.R);
             seen.refs = std::max<long long>(seen.refs, h.R);
             seen.active = std::max<long long>(seen.active, h.A);
             seen.segs = std::max<long long>(seen.segs, h.S);
             seen.segs = std::max<long long>(seen.segs, h.scanS[nxt]);
-            sc.prim_min.reserve((size_t)h.final_refs + h.leafTri + 1, s, h.final_refs);
-            sc.prim_max.reserve((size_t)h.final_refs + h.leafTri + 1, s, h.final_refs);
-            ctx->leaf_ids.reserve((size_t)h.nleaf + h.leaf_nodes + 1, s, h.nleaf);
-            sc.ref_min[nxt].reserve((size_t)h.nextR + 1, s); sc.ref_max[nxt].reserve((size_t)h.nextR + 1, s);
-            for (DevBuf<int> *b : {&sc.seg_node[nxt], &sc.seg_start[nxt], &sc.seg_count[nxt]})
-                b->reserve((size_t)h.scanS[nxt] + 1, s);
-            B = level_bufs(par);
-        }
-        const long long S_sc = graph ? caps->segs : h.S, A_sc = graph ? caps->active : h.A;
-        const int g4 = std::max(grid_for(S_sc, WPB, INT_MAX), grid_for(A_sc, BS, INT_MAX));
-        launch_scatter_pass(seg, div, graph ? std::min(g4, 2048) : g4, s, node_ptrs(ctx), (const Lvl *)L, B, LS, seg,
-                            prm.empty_space_splitting, (const float4 *)sc.ref_min[par].ptr,
-                            (const float4 *)sc.ref_max[par].ptr, sc.ref_min[nxt].ptr, sc.ref_max[nxt].ptr,
-                            sc.prim_min.ptr, sc.prim_max.ptr, sc.keys[nxt].ptr, ctx->leaf_ids.ptr, dg);
-        PM_LAUNCH_CHECK();
-        if (!graph) {
             if (h.created == 0) break;
-            nodes_now = (long long)h.nodes_size + 2ll * h.A;
-            A_now = h.created;
-            S_now = h.scanS[nxt];
+            // the next level as the advance will make it
+            Lvl n = h;
+            advance_level(&n);
+            A_now = n.A; R_now = n.R; S_now = h.scanS[nxt];
+            nodes_now = n.nodes_size; fin_now = n.final_refs; nleaf_now = n.nleaf;
         }
     }
     // the last level's advance

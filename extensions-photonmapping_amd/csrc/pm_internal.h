@@ -137,6 +137,27 @@ struct Lvl {
     long long cap_nodes, cap_active, cap_segs, cap_refs, cap_leaf_refs, cap_leaf_ids, cap_lower;
 };
 
+// A copy of the level state in coherent pinned host memory: the last tile of
+// an eager level's leaf pass writes the state there, then the sequence number
+// (system-scope release); the host waits for that number instead of copying
+// the state back and synchronising the stream.
+struct LvlMirror {
+    struct Rec {
+        Lvl lvl;
+        unsigned seq;
+    };
+    Rec *host = nullptr;   // coherent, mapped
+    Rec *dev = nullptr;    // the device's address of it
+    unsigned next_seq = 0;
+    ~LvlMirror() { if (host) (void)hipHostFree(host); }
+    void ensure() {
+        if (host) return;
+        PM_HIP(hipHostMalloc((void **)&host, sizeof(Rec), hipHostMallocCoherent | hipHostMallocMapped));
+        __atomic_store_n(&host->seq, 0u, __ATOMIC_RELEASE);
+        PM_HIP(hipHostGetDevicePointer((void **)&dev, host, 0));
+    }
+};
+
 // Sizes an eager build reached (the capacities a captured build derives
 // from) and the fixed capacities / level counts of a captured build.
 struct BuildSizes {
@@ -176,6 +197,7 @@ struct BuildScratch {
     DevBuf<float4> tri_rec;                     // leaf records (v0 | id, e1, e2) in triangle order
     DevBuf<Lvl> lvl;                            // level state (one record)
     HostPinned host;
+    LvlMirror mirror;                           // the level state as each eager level's leaf pass publishes it
 };
 
 }  // namespace pmkd
