@@ -587,6 +587,7 @@ struct LevelBufs {
     int *segLoff, *segRoff;                       // per segment: refs on each side before it, S + 1 entries
     int2 *lna, *lref, *lseg;                      // per active node: leaf slots / leaf refs / next-level
                                                   // segments of non-leaf children before it (left, right)
+    int4 *nsz;                                    // per active node: children sizes, side offsets of its first segment
     int *segbase_n, *seg_node_n, *seg_start_n, *seg_count_n;   // the same for the next level
     unsigned long long *lb, *lb_n;                // look-back words of this level's leaf pass and the next's
     int *ctr;                                     // leaf pass tickets by level parity
@@ -934,19 +935,20 @@ __global__ __launch_bounds__(BS) void leaf_pass_kernel(NodePtrs N, Lvl *L, Level
         // per node: empty-space nodes, children sizes and leaf flags
         // (CalcNodeChildSize, MarkNodeLeafs), leaf slots and refs, the
         // non-leaf children's segments
-        int v[7] = {0, 0, 0, 0, 0, 0, 0}, nl[4], nr[4];
+        int v[7] = {0, 0, 0, 0, 0, 0, 0}, nl[4], nr[4], lb[4], rb[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int li = 4 * t + j;
             nl[j] = nr[j] = 0;
+            lb[j] = rb[j] = 0;
             if (li < n) {
                 const int b = s_sb[li], e = s_sb[li + 1];
+                lpre(b, lb[j], rb[j]);
                 if (e > b) {
-                    int lb, rb, le, re;
-                    lpre(b, lb, rb);
+                    int le, re;
                     lpre(e, le, re);
-                    nl[j] = le - lb;
-                    nr[j] = re - rb;
+                    nl[j] = le - lb[j];
+                    nr[j] = re - rb[j];
                 }
                 v[0] += 2 * __popc(pl[j]);
                 v[1] += nl[j] < LS;
@@ -998,6 +1000,7 @@ __global__ __launch_bounds__(BS) void leaf_pass_kernel(NodePtrs N, Lvl *L, Level
                 B.lna[i0 + li] = make_int2(p[0], p[1]);
                 B.lref[i0 + li] = make_int2(p[2], p[3]);
                 B.lseg[i0 + li] = make_int2(p[4], p[5]);
+                B.nsz[i0 + li] = make_int4(nl[j], nr[j], lb[j] + oL, rb[j] + oR);
                 p[0] += nl[j] < LS; p[1] += nr[j] < LS;
                 p[2] += nl[j] < LS ? nl[j] : 0; p[3] += nr[j] < LS ? nr[j] : 0;
                 p[4] += nl[j] < LS ? 0 : (nl[j] + seg - 1) / seg;
@@ -1075,13 +1078,13 @@ __device__ __forceinline__ void child_ids(const LevelPlan &P, int i, const int2 
 
 __device__ void write_children(const NodePtrs &N, const LevelPlan &P, const LevelBufs &B, int i, int propagate,
                                int *__restrict__ leaf_ids) {
-    const int b = B.segbase[i], e = B.segbase[i + 1];
-    const int sizes[2] = {B.segLoff[e] - B.segLoff[b], B.segRoff[e] - B.segRoff[b]};
+    const int4 sz = B.nsz[i];
+    const int sizes[2] = {sz.x, sz.y};
     const bool leaf[2] = {sizes[0] < P.LS, sizes[1] < P.LS};
     const int2 ln = B.lna[i], lr = B.lref[i];
     int ids[2];
     child_ids(P, i, ln, leaf[0], leaf[1], ids[0], ids[1]);
-    const int split_addr[2] = {B.segLoff[b], P.L_all + B.segRoff[b]};
+    const int split_addr[2] = {sz.z, P.L_all + sz.w};
     const int leaf_addr[2] = {lr.x, P.LL_all + lr.y};
     const int lna[2] = {ln.x, P.leaf_left + ln.y};
     const int par = P.active_index + i;
@@ -1128,17 +1131,20 @@ __device__ __forceinline__ void scatter_pass_segment(
     bool join_test) {
     const int lane = threadIdx.x & 63;
     const int i = B.seg_node[s];
+    const int start = B.seg_start[s], count = B.seg_count[s];
+    const int sL = B.segLoff[s], sR = B.segRoff[s];
     const int node = P.active_index + i;
-    const int b = B.segbase[i], e = B.segbase[i + 1];
-    const int nl = B.segLoff[e] - B.segLoff[b], nr = B.segRoff[e] - B.segRoff[b];
+    const int b = B.segbase[i];
+    const int4 nsz = B.nsz[i];
+    const int nl = nsz.x, nr = nsz.y;
     const bool leafL = nl < P.LS, leafR = nr < P.LS;
     const int2 ln = B.lna[i], lr = B.lref[i];
     int idL, idR;
     child_ids(P, i, ln, leafL, leafR, idL, idR);
-    if (s == b && lane == 0) write_children(N, P, B, i, propagate, leaf_ids);
-    const int sL = B.segLoff[s], sR = B.segRoff[s];
     if (s == b) {
-        // a non-leaf child's refs start where the scatter below puts them
+        // the node's wave: its two child records, and its non-leaf children's
+        // segments (their refs start where the scatter below puts them)
+        if (lane == 0) write_children(N, P, B, i, propagate, leaf_ids);
         const int2 ls = B.lseg[i];
         const bool leaf[2] = {leafL, leafR};
         const int sz[2] = {nl, nr}, ci[2] = {idL - P.next_active, idR - P.next_active};
@@ -1153,14 +1159,13 @@ __device__ __forceinline__ void scatter_pass_segment(
                               sz[side], P.seg, k);
         }
     }
-    const int LLoff = lr.x + (leafL ? sL - B.segLoff[b] : 0);
-    const int LRoff = lr.y + (leafR ? sR - B.segRoff[b] : 0);
+    const int LLoff = lr.x + (leafL ? sL - nsz.z : 0);
+    const int LRoff = lr.y + (leafR ? sR - nsz.w : 0);
     // one chunk per wave: its node's axis and plane are wave-uniform
     const int ax = __builtin_amdgcn_readfirstlane(N.info[node]);
     const float sp = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(N.split[node])));
     f3 nmn = {0.0f, 0.0f, 0.0f}, nmx = nmn;
     if (DIV) { nmn = mk3(N.bmin[node]); nmx = mk3(N.bmax[node]); }
-    const int start = B.seg_start[s], count = B.seg_count[s];
     float4 *dLmin, *dLmax, *dRmin, *dRmax;
     long long bL, bR;
     if (leafL) { dLmin = fmin; dLmax = fmax; bL = (long long)P.ulp + LLoff; }
@@ -2060,7 +2065,7 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
         B.seg_node = sc.seg_node[par].ptr; B.seg_start = sc.seg_start[par].ptr; B.seg_count = sc.seg_count[par].ptr;
         B.segL = sc.segL.ptr; B.segR = sc.segR.ptr;
         B.segLoff = sc.segLoff.ptr; B.segRoff = sc.segRoff.ptr;
-        B.lna = sc.node_lna.ptr; B.lref = sc.node_lref.ptr; B.lseg = sc.node_lseg.ptr;
+        B.lna = sc.node_lna.ptr; B.lref = sc.node_lref.ptr; B.lseg = sc.node_lseg.ptr; B.nsz = sc.node_sz.ptr;
         B.segbase_n = sc.node_segbase[q].ptr;
         B.seg_node_n = sc.seg_node[q].ptr; B.seg_start_n = sc.seg_start[q].ptr; B.seg_count_n = sc.seg_count[q].ptr;
         B.lb = sc.lb_words[par].ptr; B.lb_n = sc.lb_words[q].ptr;
@@ -2101,6 +2106,7 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
         if (!graph) {
             for (DevBuf<int> *b : {&sc.segL, &sc.segR, &sc.segLoff, &sc.segRoff}) b->reserve(S_now + 1, s);
             sc.node_lna.reserve(A_now + 1, s); sc.node_lref.reserve(A_now + 1, s); sc.node_lseg.reserve(A_now + 1, s);
+            sc.node_sz.reserve(A_now + 1, s);
             nodes.reserve((size_t)(nodes_now + 14 * A_now + 1), s, (size_t)nodes_now);
             sc.keys[nxt].reserve(12 * (size_t)A_now + 6, s);
             sc.node_segbase[nxt].reserve(2 * (size_t)A_now + 2, s);
@@ -2321,6 +2327,7 @@ void reserve_build_caps(pm_ctx *ctx, const BuildSizes &caps, int N, hipStream_t 
     sc.node_lna.reserve((size_t)caps.active + 1, s);
     sc.node_lref.reserve((size_t)caps.active + 1, s);
     sc.node_lseg.reserve((size_t)caps.active + 1, s);
+    sc.node_sz.reserve((size_t)caps.active + 1, s);
     for (DevBuf<int> *b : {&sc.segL, &sc.segR, &sc.segLoff, &sc.segRoff}) b->reserve((size_t)caps.segs + 1, s);
     // the leaf passes' status words (8 per tile of NT nodes; the roots' scan
     // uses parity 1's, M <= caps.active)
@@ -2426,7 +2433,7 @@ std::vector<void **> graph_buffers(pm_ctx *ctx) {
         (void **)&ctx->packed.ptr, (void **)&ctx->leaf_tris.ptr, (void **)&sc.prim_min.ptr,
         (void **)&sc.prim_max.ptr, (void **)&sc.segL.ptr, (void **)&sc.segR.ptr,
         (void **)&sc.segLoff.ptr, (void **)&sc.segRoff.ptr, (void **)&sc.node_lna.ptr, (void **)&sc.node_lref.ptr,
-        (void **)&sc.node_lseg.ptr, (void **)&sc.lb_ctr.ptr, (void **)&sc.areas.ptr, (void **)&sc.lower_flag.ptr,
+        (void **)&sc.node_lseg.ptr, (void **)&sc.node_sz.ptr, (void **)&sc.lb_ctr.ptr, (void **)&sc.areas.ptr, (void **)&sc.lower_flag.ptr,
         (void **)&sc.lower_off.ptr, (void **)&sc.lower_sets.ptr, (void **)&sc.lower_careas.ptr,
         (void **)&sc.lower_list.ptr, (void **)&sc.tri_rec.ptr, (void **)&ctx->graph_off.ptr, (void **)&sc.lvl.ptr};
     for (int i = 0; i < 2; ++i) {

@@ -176,6 +176,7 @@ struct BuildScratch {
     DevBuf<int> segLoff, segRoff;               // per segment: left / right refs before it
     DevBuf<int2> node_lna, node_lref;           // per active node: leaf slots and leaf refs before it (left, right)
     DevBuf<int2> node_lseg;                     // per active node: its non-leaf children's segments before it
+    DevBuf<int4> node_sz;                       // per active node: children sizes, side offsets of its first segment
     DevBuf<unsigned long long> lb_words[2];     // look-back status words of the leaf passes (by level parity)
     DevBuf<int> lb_ctr;                         // their tile tickets (by level parity)
     DevBuf<int> scan_tmp[3];                    // block sums for scans, per level
