@@ -240,14 +240,6 @@ __device__ __forceinline__ void zero_node(const NodePtrs &N, int id) {
 }
 
 // ------------------------------------------------------------------ root
-__global__ void init_keys_kernel(int *keys, int count) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) {
-        const int kmin = fkey(F_INF), kmax = fkey(-F_INF);   // AabbMemset, ReduceSegments.h:328-336
-        keys[6 * i + 0] = kmin; keys[6 * i + 1] = kmin; keys[6 * i + 2] = kmin;
-        keys[6 * i + 3] = kmax; keys[6 * i + 4] = kmax; keys[6 * i + 5] = kmax;
-    }
-}
 
 // TriangleMapUpperCreator.cu:128-137, one root per mesh: root m covers
 // triangles [off[m], off[m+1]) and is its own parent
@@ -290,6 +282,43 @@ __device__ __forceinline__ void key_join(int *k, int v, bool is_min) {
     }
 }
 
+// Even tested, the joins of ~1,000 chunks into the same two nodes queue on
+// one cache line.  So the keys of a level whose parent level has fewer than
+// JOIN_TEST_NODES nodes (and of fewer than that many roots) are kept as
+// KEY_REPL replicas per node, each on its own 128-byte line; block b joins
+// replica b % KEY_REPL (consecutive blocks go to different XCDs), and the
+// readers fold the replicas (min / max are exact, so the keys are those of
+// one copy).
+constexpr int KEY_REPL = 8, KEY_LINE = 32;
+__device__ __forceinline__ int *key_slot(int *keys, int node, bool repl, int r) {
+    return repl ? keys + ((long long)node * KEY_REPL + r) * KEY_LINE : keys + 6 * (long long)node;
+}
+__device__ __forceinline__ void read_keys(const int *__restrict__ keys, int node, bool repl, int k[6]) {
+    const int *k0 = repl ? keys + (long long)node * KEY_REPL * KEY_LINE : keys + 6 * (long long)node;
+#pragma unroll
+    for (int w = 0; w < 6; ++w) k[w] = k0[w];
+    if (!repl) return;
+#pragma unroll
+    for (int r = 1; r < KEY_REPL; ++r)
+#pragma unroll
+        for (int w = 0; w < 6; ++w) {
+            const int v = k0[r * KEY_LINE + w];
+            k[w] = w < 3 ? min(k[w], v) : max(k[w], v);
+        }
+}
+// the reset keys of n nodes (every word a replica has)
+__device__ __forceinline__ long long key_words(long long n, bool repl) { return repl ? n * KEY_REPL * KEY_LINE : 6 * n; }
+__device__ __forceinline__ int key_reset_value(long long k, bool repl) {
+    return (repl ? k % KEY_LINE : k % 6) < 3 ? fkey(F_INF) : fkey(-F_INF);
+}
+
+// AabbMemset (ReduceSegments.h:328-336) of the roots' box keys
+__global__ void init_keys_kernel(int *keys, int count, int repl) {
+    const long long n = key_words(count, repl);
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
+        keys[k] = key_reset_value(k, repl);
+}
+
 constexpr int BOX_TILE = BS * 8;   // consecutive triangles per block
 
 // CalcPrimitiveAabb (Kernels/TriangleUpper.h:10-28) + the root boxes.  A
@@ -310,6 +339,7 @@ __global__ __launch_bounds__(BS) void tri_box_kernel(const float4 *__restrict__ 
     const int m0 = M > 1 ? mesh_of(off, t0, 0, M - 1) : 0;
     const int m1 = M > 1 ? mesh_of(off, t1, m0, M - 1) : 0;
     const bool tile_uniform = m0 == m1;
+    const bool repl = M < JOIN_TEST_NODES;   // the roots' keys are replicated
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int k0 = fkey(F_INF), k1 = k0, k2 = k0, k3 = fkey(-F_INF), k4 = k3, k5 = k3;
     for (int j = 0; j < BOX_TILE / BS; ++j) {
@@ -346,9 +376,9 @@ __global__ __launch_bounds__(BS) void tri_box_kernel(const float4 *__restrict__ 
             if (wm < 0) continue;              // no valid lane (lane 0 invalid => all invalid)
             e0 = wave_min(e0); e1 = wave_min(e1); e2 = wave_min(e2);
             e3 = wave_max(e3); e4 = wave_max(e4); e5 = wave_max(e5);
-            if (lane == 0) key_atomics(keys + 6 * wm, e0, e1, e2, e3, e4, e5);
+            if (lane == 0) key_atomics(key_slot(keys, wm, repl, blockIdx.x % KEY_REPL), e0, e1, e2, e3, e4, e5);
         } else if (valid) {
-            key_atomics(keys + 6 * m, e0, e1, e2, e3, e4, e5);
+            key_atomics(key_slot(keys, m, repl, blockIdx.x % KEY_REPL), e0, e1, e2, e3, e4, e5);
         }
     }
     if (!tile_uniform) return;                 // block-uniform: no thread passes the barrier
@@ -361,8 +391,8 @@ __global__ __launch_bounds__(BS) void tri_box_kernel(const float4 *__restrict__ 
             k0 = min(k0, red[0][j]); k1 = min(k1, red[1][j]); k2 = min(k2, red[2][j]);
             k3 = max(k3, red[3][j]); k4 = max(k4, red[4][j]); k5 = max(k5, red[5][j]);
         }
-        int *k = keys + 6 * m0;
-        if (M < JOIN_TEST_NODES) {
+        int *k = key_slot(keys, m0, repl, blockIdx.x % KEY_REPL);
+        if (repl) {
             key_join(k + 0, k0, true); key_join(k + 1, k1, true); key_join(k + 2, k2, true);
             key_join(k + 3, k3, false); key_join(k + 4, k4, false); key_join(k + 5, k5, false);
         } else {
@@ -601,9 +631,15 @@ struct NodeIn {
 };
 
 __device__ __forceinline__ void load_node(const NodePtrs &N, int i, int node, const int *__restrict__ keys,
-                                          bool empty_space, NodeIn &v) {
-    const int2 *k2 = reinterpret_cast<const int2 *>(keys + 6 * (long long)i);   // 24 i bytes: 8-byte aligned
-    v.k01 = k2[0]; v.k23 = k2[1]; v.k45 = k2[2];
+                                          bool repl, bool empty_space, NodeIn &v) {
+    if (repl) {
+        int k[6];
+        read_keys(keys, i, true, k);
+        v.k01 = make_int2(k[0], k[1]); v.k23 = make_int2(k[2], k[3]); v.k45 = make_int2(k[4], k[5]);
+    } else {
+        const int2 *k2 = reinterpret_cast<const int2 *>(keys + 6 * (long long)i);   // 24 i bytes: 8-byte aligned
+        v.k01 = k2[0]; v.k23 = k2[1]; v.k45 = k2[2];
+    }
     if (empty_space) {
         v.pmn = N.bmin[node];
         v.pmx = N.bmax[node];
@@ -805,9 +841,11 @@ __global__ __launch_bounds__(WPB * 64) void classify_pass_kernel(const Lvl *L, L
                                                                  const float4 *__restrict__ rmin,
                                                                  const float4 *__restrict__ rmax, DivideGeom dg) {
     const int S = L->status ? 0 : L->scanS[par];
+    const bool repl = L->scanR[par];
     for (int s = wave_index(); s < S; s += wave_count()) {
         const int i = __builtin_amdgcn_readfirstlane(B.seg_node[s]);
-        const int *k = keys + 6 * (long long)i;
+        int k[6];
+        read_keys(keys, i, repl, k);
         const f3 rMin = {unkey(k[0]), unkey(k[1]), unkey(k[2])};
         const f3 rMax = {unkey(k[3]), unkey(k[4]), unkey(k[5])};
         int ax;
@@ -857,7 +895,7 @@ __global__ __launch_bounds__(BS) void leaf_pass_kernel(NodePtrs N, Lvl *L, Level
     __shared__ int s_tile;
     __shared__ int s_w[7][BS / 64];
     __shared__ long long s_ex[8];
-    __shared__ int s_lvl[4];
+    __shared__ int s_lvl[5];
     __shared__ int s_sb[NT + 1];                  // the tile's nodes' first segments
     __shared__ int s_lp[SEG_LDS], s_rp[SEG_LDS];  // local side prefixes of its first SEG_LDS segments
     const int t = threadIdx.x;
@@ -867,17 +905,21 @@ __global__ __launch_bounds__(BS) void leaf_pass_kernel(NodePtrs N, Lvl *L, Level
         s_lvl[1] = L->scanAI[par];
         s_lvl[2] = ok ? L->scanS[par] : 0;
         s_lvl[3] = atomicAdd(&B.ctr[par], 1);
+        s_lvl[4] = L->scanR[par];
     }
     __syncthreads();
     const int A = s_lvl[0], active_index = s_lvl[1], S = s_lvl[2];
+    const bool repl_in = s_lvl[4], repl_out = A < JOIN_TEST_NODES;   // this level's keys, the next level's
     const long long cap_nodes = L->cap_nodes;
     const bool es = empty_space && active_index != 0;   // the root level has no empty-space cuts
     const int ntiles = max(1, (A + NT - 1) / NT);
     // the next level's box keys: every possible child slot (created <= 2A),
     // before any scatter joins into them
     {
-        const int kmin = fkey(F_INF), kmax = fkey(-F_INF);
-        GRID_STRIDE(k, 12 * A) keys_next[k] = (k % 6) < 3 ? kmin : kmax;
+        const long long nk = key_words(2ll * A, repl_out);
+        for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < nk;
+             k += (long long)gridDim.x * blockDim.x)
+            keys_next[k] = key_reset_value(k, repl_out);
     }
     for (int tile = s_lvl[3]; tile < ntiles; tile = next_tile(&B.ctr[par], &s_tile)) {
         const int i0 = tile * NT, n = min(NT, A - i0);
@@ -887,7 +929,7 @@ __global__ __launch_bounds__(BS) void leaf_pass_kernel(NodePtrs N, Lvl *L, Level
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int li = 4 * t + j;
-            if (li < n) load_node(N, i0 + li, active_index + i0 + li, keys, es, in[j]);
+            if (li < n) load_node(N, i0 + li, active_index + i0 + li, keys, repl_in, es, in[j]);
         }
         for (int li = t; li <= n; li += BS) s_sb[li] = n > 0 ? B.segbase[i0 + li] : 0;
         int pl[4];
@@ -1035,6 +1077,7 @@ __global__ __launch_bounds__(BS) void leaf_pass_kernel(NodePtrs N, Lvl *L, Level
                         L->scanA[par ^ 1] = P.created;
                         L->scanAI[par ^ 1] = L->nodes_size + 2 * L->A - P.created;
                         L->scanS[par ^ 1] = (int)(SL + SR);
+                        L->scanR[par ^ 1] = repl_out;
                         B.segbase_n[P.created] = (int)(SL + SR);
                     }
                 }
@@ -1250,14 +1293,16 @@ __device__ __forceinline__ void scatter_pass_segment(
         return;
     }
     // few nodes: lanes 0..5 join the left child's six key words, lanes 6..11
-    // the right's, each skipping a join that cannot change its word
+    // the right's, each skipping a join that cannot change its word; the
+    // next level's keys are replicated
     const bool right = lane >= 6;
     const int tw = right ? lane - 6 : lane;
     if (lane < 12 && (right ? doR : doL)) {
         const int a0 = right ? r0 : l0, a1 = right ? r1 : l1, a2 = right ? r2 : l2;
         const int a3 = right ? r3 : l3, a4 = right ? r4 : l4, a5 = right ? r5 : l5;
         const int v = tw == 0 ? a0 : tw == 1 ? a1 : tw == 2 ? a2 : tw == 3 ? a3 : tw == 4 ? a4 : a5;
-        key_join(keys_next + 6 * ((right ? idR : idL) - P.next_active) + tw, v, tw < 3);
+        key_join(key_slot(keys_next, (right ? idR : idL) - P.next_active, true, blockIdx.x % KEY_REPL) + tw, v,
+                 tw < 3);
     }
 }
 
@@ -2000,6 +2045,8 @@ void wait_level(BuildScratch &sc, Lvl &h, unsigned seq, hipStream_t s) {
     }
 }
 
+inline long long key_words_host(long long n, bool repl) { return repl ? n * KEY_REPL * KEY_LINE : 6 * n; }
+
 inline int grid_for(long long n, int per, int cap) {
     return std::max(1, (int)std::min<long long>(blocks_for(std::max(n, 1ll), per), cap));
 }
@@ -2021,6 +2068,8 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
     Lvl h{};
     h.A = M; h.active_index = 0; h.nodes_size = M; h.R = N; h.advance = 0;
     h.scanA[0] = M; h.scanAI[0] = 0;   // scanS[0]: root_segments_kernel
+    const bool repl0 = M < JOIN_TEST_NODES;
+    h.scanR[0] = repl0;
     h.max_levels = prm.max_upper_levels > 0 ? prm.max_upper_levels : 128;
     h.max_refs = (long long)(prm.max_ref_factor > 0 ? prm.max_ref_factor : 16) * N + 65536;
     h.cap_nodes = graph ? caps->nodes : BIG;
@@ -2042,11 +2091,11 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
         nodes.reserve(std::max(1024, 2 * M), s, 0);
         sc.ref_min[0].reserve((size_t)N + 1, s); sc.ref_max[0].reserve((size_t)N + 1, s);
         sc.tri_rec.reserve(3 * (size_t)N + 3, s);
-        sc.keys[0].reserve(6 * (size_t)M, s);
+        sc.keys[0].reserve((size_t)key_words_host(M, repl0), s);
     }
     init_roots_kernel<<<blocks_for(M, BS), BS, 0, s>>>(node_ptrs(ctx), off_dev, M);
     PM_LAUNCH_CHECK();
-    init_keys_kernel<<<blocks_for(M, BS), BS, 0, s>>>(sc.keys[0].ptr, M);
+    init_keys_kernel<<<grid_for(key_words_host(M, repl0), BS, 1024), BS, 0, s>>>(sc.keys[0].ptr, M, repl0);
     PM_LAUNCH_CHECK();
     if (N > 0) {
         tri_box_kernel<<<blocks_for(N, BOX_TILE), BS, 0, s>>>(P0, P1, P2, N, sc.ref_min[0].ptr,
@@ -2108,7 +2157,7 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
             sc.node_lna.reserve(A_now + 1, s); sc.node_lref.reserve(A_now + 1, s); sc.node_lseg.reserve(A_now + 1, s);
             sc.node_sz.reserve(A_now + 1, s);
             nodes.reserve((size_t)(nodes_now + 14 * A_now + 1), s, (size_t)nodes_now);
-            sc.keys[nxt].reserve(12 * (size_t)A_now + 6, s);
+            sc.keys[nxt].reserve((size_t)key_words_host(2 * A_now, A_now < JOIN_TEST_NODES) + 6, s);
             sc.node_segbase[nxt].reserve(2 * (size_t)A_now + 2, s);
             sc.lb_words[nxt].reserve(8 * tiles(2 * A_now, NT) + 8, s);
             const size_t R2 = 2 * (size_t)R_now;
@@ -2318,7 +2367,8 @@ void reserve_build_caps(pm_ctx *ctx, const BuildSizes &caps, int N, hipStream_t 
         sc.ref_max[i].reserve((size_t)caps.refs + 1, s);
         // the leaf pass resets the next level's keys for every possible
         // child slot of its active nodes (12 per node)
-        sc.keys[i].reserve(12 * (size_t)caps.active + 12, s);
+        sc.keys[i].reserve(std::max<size_t>(12 * (size_t)caps.active, key_words_host(2 * JOIN_TEST_NODES, true)) + 12,
+                           s);
     }
     for (int q = 0; q < 2; ++q) {
         sc.node_segbase[q].reserve((size_t)caps.active + 1, s);

@@ -123,6 +123,7 @@ struct Lvl {
     // the level starts (the level before's leaf pass, or the build's start)
     // and not changed while those passes run
     int scanA[2], scanAI[2], scanS[2];
+    int scanR[2];                                      // the level's box keys are replicated (few nodes before)
     int SL_all;                                        // next-level segments of this level's left children
     int leaf_left;                                     // leaf slots among the level's left children
     // work counts of the build (the algorithmic-bytes formula's inputs)
