@@ -370,6 +370,19 @@ struct CoopLds {
     float4 dir[RECS ? 64 : 1];           // owner's direction, w = its leaf exit th
 };
 
+// inclusive prefix sum over the wave's 64 lanes (every lane active): DPP
+// row shifts within each row of 16 lanes, then row broadcasts of lanes 15
+// and 31 into the rows after them (disabled lanes add the old value, 0)
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);   // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);   // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);   // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);   // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);   // row_bcast:15, rows 1 and 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);   // row_bcast:31, rows 2 and 3
+    return v;
+}
+
 // lane src4 / 4's value of x (every lane of the wave active)
 __device__ __forceinline__ float lane_read(float x, int src4) {
     return __int_as_float(__builtin_amdgcn_ds_bpermute(src4, __float_as_int(x)));
@@ -419,17 +432,12 @@ __device__ __forceinline__ long long coop_leaf(const float4 *__restrict__ L, Coo
         C.org[lane] = make_float4(o.x, o.y, o.z, __uint_as_float(base48));
         C.dir[lane] = make_float4(d.x, d.y, d.z, th);
     }
-    // k <= 64 pairs per lane: the wave's exclusive prefix sum of k, bit by
-    // bit -- each bit's ballot counted below this lane (mbcnt) and in all
+    // k <= 64 pairs per lane: the wave's prefix sum of k (this lane's first
+    // pair) and its total (the wave's pairs)
     const int k = __popcll(bm);
-    int excl = 0, W = 0;                  // this lane's first pair; the wave's pairs
-#pragma unroll
-    for (int b = 0; b < 7; ++b) {
-        const unsigned long long m = __ballot((k >> b) & 1);
-        excl += (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
-                                               __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u)) << b;
-        W += __popcll(m) << b;
-    }
+    const int incl = wave_incl_scan(k);
+    const int excl = incl - k;
+    const int W = __builtin_amdgcn_readlane(incl, 63);
     int pos = excl;
     // a pair is bit << 8 | 4 * owner lane (the owner's ds_bpermute address);
     // a run of bits b0 .. b0+k-1 (every leaf the upper stage makes) is listed
