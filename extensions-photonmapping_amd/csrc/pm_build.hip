@@ -504,7 +504,13 @@ struct ChildPlan {
 // the next level's into the other.
 // Every scan is over integers, so the offsets (and the tree) are those of
 // any other order of summation.
-constexpr int NT = 4 * BS;             // active nodes per node/leaf tile (4 per thread)
+// A tile is 256 active nodes, one per thread: 1,024-node tiles (4 per
+// thread) measured 1.85 ms for the 1M build, 512-node tiles 1.77 ms, 256-node
+// tiles 1.73 ms (smaller blocks: slower); a tile's work is a chain of
+// dependent round trips, so more, shorter tiles finish sooner.
+constexpr int LBS = 256;               // threads of a leaf-pass / root-segment block
+constexpr int NPT = 1;                 // active nodes per thread in a tile
+constexpr int NT = NPT * LBS;          // active nodes per tile
 // A status word is flag << 62 | value: flag 1 = aggregate, 2 = inclusive prefix.
 constexpr unsigned long long LB_AGG = 1ull << 62, LB_INC = 2ull << 62, LB_VAL = LB_AGG - 1;
 // A wait for a predecessor is bounded: a tile that never publishes (only
@@ -585,10 +591,10 @@ __device__ __forceinline__ int next_tile(int *ctr, int *s_tile) {
     return t;
 }
 
-// block-wide exclusive scan of K per-thread counts (BS threads): pre[k] gets
+// block-wide exclusive scan of K per-thread counts (T threads): pre[k] gets
 // the sum over lower threads, tot[k] the block's total
-template <int K>
-__device__ __forceinline__ void block_scan(const int *v, int *pre, int *tot, int (*s_w)[BS / 64]) {
+template <int K, int T = BS>
+__device__ __forceinline__ void block_scan(const int *v, int *pre, int *tot, int (*s_w)[T / 64]) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int inc[K];
 #pragma unroll
@@ -601,7 +607,7 @@ __device__ __forceinline__ void block_scan(const int *v, int *pre, int *tot, int
     for (int k = 0; k < K; ++k) {
         pre[k] = inc[k] - v[k];
         tot[k] = 0;
-        for (int j = 0; j < BS / 64; ++j) {
+        for (int j = 0; j < T / 64; ++j) {
             if (j < wv) pre[k] += s_w[k][j];
             tot[k] += s_w[k][j];
         }
@@ -700,13 +706,13 @@ constexpr int SEG_DIRECT = 4;
 // The roots' segment table (level 0): tiles of NT roots in ticket order, a
 // look-back scan of the roots' segment counts.  Uses the look-back words and
 // ticket of level parity 1 (level 0's leaf pass clears them for level 1).
-__global__ __launch_bounds__(BS) void root_segments_kernel(NodePtrs N, Lvl *L, int *__restrict__ segbase,
+__global__ __launch_bounds__(LBS) void root_segments_kernel(NodePtrs N, Lvl *L, int *__restrict__ segbase,
                                                            int *__restrict__ sn, int *__restrict__ ss,
                                                            int *__restrict__ sc, unsigned long long *lbw, int *ctr,
                                                            int M, int seg) {
     __shared__ int s_tile, s_first;
     __shared__ int s_xpre[NT], s_segpre[NT], s_pi[NT], s_cnt[NT];
-    __shared__ int s_w[2][BS / 64];
+    __shared__ int s_w[2][LBS / 64];
     __shared__ long long s_ex;
     const int t = threadIdx.x;
     if (t == 0) s_first = atomicAdd(ctr, 1);
@@ -715,10 +721,10 @@ __global__ __launch_bounds__(BS) void root_segments_kernel(NodePtrs N, Lvl *L, i
     const int ntiles = max(1, (M + NT - 1) / NT);
     for (int tile = s_first; tile < ntiles; tile = next_tile(ctr, &s_tile)) {
         const int i0 = tile * NT, n = min(NT, M - i0);
-        int cnt[4], pi[4], v[2] = {0, 0};
+        int cnt[NPT], pi[NPT], v[2] = {0, 0};
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int li = 4 * t + j;
+        for (int j = 0; j < NPT; ++j) {
+            const int li = NPT * t + j;
             cnt[j] = 0; pi[j] = 0;
             if (li < n) {
                 cnt[j] = (int)N.bitmap[i0 + li];
@@ -729,7 +735,7 @@ __global__ __launch_bounds__(BS) void root_segments_kernel(NodePtrs N, Lvl *L, i
             v[1] += max(ns - SEG_DIRECT, 0);
         }
         int pre[2], tot[2];
-        block_scan<2>(v, pre, tot, s_w);
+        block_scan<2, LBS>(v, pre, tot, s_w);
         if (t < 64) {
             const long long agg[1] = {tot[0]};
             long long ex[1];
@@ -741,8 +747,8 @@ __global__ __launch_bounds__(BS) void root_segments_kernel(NodePtrs N, Lvl *L, i
         const bool fits = (long long)eseg + tot[0] <= cap_segs;
         int q = pre[0], x = pre[1];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int li = 4 * t + j;
+        for (int j = 0; j < NPT; ++j) {
+            const int li = NPT * t + j;
             if (li < n) {
                 const int i = i0 + li;
                 const int ns = (cnt[j] + seg - 1) / seg;
@@ -761,7 +767,7 @@ __global__ __launch_bounds__(BS) void root_segments_kernel(NodePtrs N, Lvl *L, i
         __syncthreads();
         // the long roots' other segments: extra segment ls belongs to the
         // last root li with s_xpre[li] <= ls
-        for (int ls = t; fits && ls < tot[1]; ls += BS) {
+        for (int ls = t; fits && ls < tot[1]; ls += LBS) {
             int lo = 0, hi = n - 1;
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
@@ -888,12 +894,12 @@ constexpr int SEG_LDS = 2048;
 // The level's node work and children plan.  par: the level's parity.
 // The blocks read the level's sizes once (scanA/scanAI/scanS[par]), with
 // their first ticket; the last tile writes the next level's into [par ^ 1].
-__global__ __launch_bounds__(BS) void leaf_pass_kernel(NodePtrs N, Lvl *L, LevelBufs B, int par, int LS, int seg,
+__global__ __launch_bounds__(LBS) void leaf_pass_kernel(NodePtrs N, Lvl *L, LevelBufs B, int par, int LS, int seg,
                                                        const int *__restrict__ keys, int *__restrict__ keys_next,
                                                        int empty_space, float thr, LvlMirror::Rec *mirror,
                                                        unsigned seq) {
     __shared__ int s_tile;
-    __shared__ int s_w[7][BS / 64];
+    __shared__ int s_w[7][LBS / 64];
     __shared__ long long s_ex[8];
     __shared__ int s_lvl[5];
     __shared__ int s_sb[NT + 1];                  // the tile's nodes' first segments
@@ -925,17 +931,17 @@ __global__ __launch_bounds__(BS) void leaf_pass_kernel(NodePtrs N, Lvl *L, Level
         const int i0 = tile * NT, n = min(NT, A - i0);
         // every node input first, then the node records (the stores would
         // otherwise hold back the next node's loads)
-        NodeIn in[4];
+        NodeIn in[NPT];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int li = 4 * t + j;
+        for (int j = 0; j < NPT; ++j) {
+            const int li = NPT * t + j;
             if (li < n) load_node(N, i0 + li, active_index + i0 + li, keys, repl_in, es, in[j]);
         }
-        for (int li = t; li <= n; li += BS) s_sb[li] = n > 0 ? B.segbase[i0 + li] : 0;
-        int pl[4];
+        for (int li = t; li <= n; li += LBS) s_sb[li] = n > 0 ? B.segbase[i0 + li] : 0;
+        int pl[NPT];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int li = 4 * t + j;
+        for (int j = 0; j < NPT; ++j) {
+            const int li = NPT * t + j;
             pl[j] = 0;
             if (li < n) prepare_node(N, active_index + i0 + li, in[j], es, thr, pl[j]);
         }
@@ -945,20 +951,20 @@ __global__ __launch_bounds__(BS) void leaf_pass_kernel(NodePtrs N, Lvl *L, Level
         const int sb = s_sb[0], se = s_sb[n];
         int cL = 0, cR = 0;
         for (int c0 = sb; c0 < se; c0 += NT) {
-            int v[2] = {0, 0}, sl[4], sr[4];
+            int v[2] = {0, 0}, sl[NPT], sr[NPT];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int s = c0 + 4 * t + j;
+            for (int j = 0; j < NPT; ++j) {
+                const int s = c0 + NPT * t + j;
                 sl[j] = s < se ? B.segL[s] : 0;
                 sr[j] = s < se ? B.segR[s] : 0;
                 v[0] += sl[j]; v[1] += sr[j];
             }
             int pre[2], tot[2];
-            block_scan<2>(v, pre, tot, s_w);
+            block_scan<2, LBS>(v, pre, tot, s_w);
             int pl2 = cL + pre[0], pr2 = cR + pre[1];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int s = c0 + 4 * t + j;
+            for (int j = 0; j < NPT; ++j) {
+                const int s = c0 + NPT * t + j;
                 if (s < se) {
                     if (s - sb < SEG_LDS) { s_lp[s - sb] = pl2; s_rp[s - sb] = pr2; }
                     else { B.segLoff[s] = pl2; B.segRoff[s] = pr2; }
@@ -977,10 +983,10 @@ __global__ __launch_bounds__(BS) void leaf_pass_kernel(NodePtrs N, Lvl *L, Level
         // per node: empty-space nodes, children sizes and leaf flags
         // (CalcNodeChildSize, MarkNodeLeafs), leaf slots and refs, the
         // non-leaf children's segments
-        int v[7] = {0, 0, 0, 0, 0, 0, 0}, nl[4], nr[4], lb[4], rb[4];
+        int v[7] = {0, 0, 0, 0, 0, 0, 0}, nl[NPT], nr[NPT], lb[NPT], rb[NPT];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int li = 4 * t + j;
+        for (int j = 0; j < NPT; ++j) {
+            const int li = NPT * t + j;
             nl[j] = nr[j] = 0;
             lb[j] = rb[j] = 0;
             if (li < n) {
@@ -1002,7 +1008,7 @@ __global__ __launch_bounds__(BS) void leaf_pass_kernel(NodePtrs N, Lvl *L, Level
             }
         }
         int pre[7], tot[7];
-        block_scan<7>(v, pre, tot, s_w);
+        block_scan<7, LBS>(v, pre, tot, s_w);
         if (t < 64) {
             // the two segment counts share a word (each total < 2^31)
             const long long agg[8] = {cL, cR, tot[0], tot[1], tot[2], tot[3], tot[4],
@@ -1019,15 +1025,15 @@ __global__ __launch_bounds__(BS) void leaf_pass_kernel(NodePtrs N, Lvl *L, Level
         const bool fits = (long long)active_index + A + ereq + tot[0] <= cap_nodes;
         int r = pre[0];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int li = 4 * t + j;
+        for (int j = 0; j < NPT; ++j) {
+            const int li = NPT * t + j;
             if (li < n) {
                 if (pl[j] && fits) empty_chain_node_at(N, i0 + li, A, active_index, pl[j], ereq + r);
                 r += 2 * __popc(pl[j]);
             }
         }
         // the side offsets of the tile's segments, now global
-        for (int s = sb + t; s < se; s += BS) {
+        for (int s = sb + t; s < se; s += LBS) {
             int l, rr;
             lpre(s, l, rr);
             B.segLoff[s] = l + oL;
@@ -1036,8 +1042,8 @@ __global__ __launch_bounds__(BS) void leaf_pass_kernel(NodePtrs N, Lvl *L, Level
         int p[6] = {(int)s_ex[3] + pre[1], (int)s_ex[4] + pre[2], (int)s_ex[5] + pre[3], (int)s_ex[6] + pre[4],
                     oSL + pre[5], oSR + pre[6]};
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int li = 4 * t + j;
+        for (int j = 0; j < NPT; ++j) {
+            const int li = NPT * t + j;
             if (li < n) {
                 B.lna[i0 + li] = make_int2(p[0], p[1]);
                 B.lref[i0 + li] = make_int2(p[2], p[3]);
@@ -1087,11 +1093,11 @@ __global__ __launch_bounds__(BS) void leaf_pass_kernel(NodePtrs N, Lvl *L, Level
             // the next level's leaf pass starts with fresh status words
             const int created = L->status ? 0 : L->created;
             const int nt = max(1, (created + NT - 1) / NT);
-            for (int k = t; k < 8 * nt; k += BS) B.lb_n[k] = 0ull;
+            for (int k = t; k < 8 * nt; k += LBS) B.lb_n[k] = 0ull;
             // an eager build: the level state to the host, then its number
             if (mirror) {
                 constexpr int NW = (int)(sizeof(Lvl) / sizeof(int));
-                static_assert(NW <= BS, "one word per thread");
+                static_assert(NW <= LBS, "one word per thread");
                 if (t < NW) {
                     __hip_atomic_store(reinterpret_cast<int *>(&mirror->lvl) + t, reinterpret_cast<const int *>(L)[t],
                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -2083,7 +2089,7 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
     seen.active = M; seen.refs = N;
 
     sc.lvl.reserve(1, s);
-    if (!graph) sc.mirror.ensure();
+    const bool mirror = !graph && sc.mirror.ensure();
     Lvl *L = sc.lvl.ptr;
     init_lvl_kernel<<<1, 64, 0, s>>>(L, h);
     PM_LAUNCH_CHECK();
@@ -2134,7 +2140,7 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
     lb_init_kernel<<<grid_for(8 * tiles(M, NT) + 8, BS, 64), BS, 0, s>>>(
         sc.lb_words[0].ptr, sc.lb_words[1].ptr, (int)(8 * tiles(M, NT) + 8), sc.lb_ctr.ptr);
     PM_LAUNCH_CHECK();
-    root_segments_kernel<<<(int)tiles(M, NT), BS, 0, s>>>(node_ptrs(ctx), L, sc.node_segbase[0].ptr,
+    root_segments_kernel<<<(int)tiles(M, NT), LBS, 0, s>>>(node_ptrs(ctx), L, sc.node_segbase[0].ptr,
                                                            sc.seg_node[0].ptr, sc.seg_start[0].ptr,
                                                            sc.seg_count[0].ptr, sc.lb_words[1].ptr,
                                                            sc.lb_ctr.ptr + 1, M, seg);
@@ -2175,10 +2181,10 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
                              (const float4 *)sc.ref_max[par].ptr, dg);
         PM_LAUNCH_CHECK();
         const int g3 = std::max(grid_for(A_g, NT, INT_MAX), grid_for(12 * A_g, 16 * BS, 2048));
-        const unsigned seq = graph ? 0u : ++sc.mirror.next_seq;
-        leaf_pass_kernel<<<graph ? std::min(g3, 2048) : g3, BS, 0, s>>>(
+        const unsigned seq = mirror ? ++sc.mirror.next_seq : 0u;
+        leaf_pass_kernel<<<graph ? std::min(g3, 2048) : g3, LBS, 0, s>>>(
             NP, L, B, par, LS, seg, sc.keys[par].ptr, sc.keys[nxt].ptr, prm.empty_space_splitting,
-            prm.empty_space_threshold, graph ? nullptr : sc.mirror.dev, seq);
+            prm.empty_space_threshold, mirror ? sc.mirror.dev : nullptr, seq);
         PM_LAUNCH_CHECK();
         const int g4 = std::max(grid_for(S_g, WPB, INT_MAX), grid_for(A_g, BS, INT_MAX));
         launch_scatter_pass(seg, div, graph ? std::min(g4, 2048) : g4, s, NP, (const Lvl *)L, B, LS, seg,
@@ -2187,7 +2193,8 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
                             sc.prim_min.ptr, sc.prim_max.ptr, sc.keys[nxt].ptr, ctx->leaf_ids.ptr, dg);
         PM_LAUNCH_CHECK();
         if (!graph) {
-            wait_level(sc, h, seq, s);
+            if (mirror) wait_level(sc, h, seq, s);
+            else read_state(sc, h, s);
             if (h.status) throw_status(h);
             ctx->level_refs.push_back(h.R);
             seen.refs = std::max<long long>(seen.refs, h.R);

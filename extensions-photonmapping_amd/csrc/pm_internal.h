@@ -151,11 +151,21 @@ struct LvlMirror {
     Rec *dev = nullptr;    // the device's address of it
     unsigned next_seq = 0;
     ~LvlMirror() { if (host) (void)hipHostFree(host); }
-    void ensure() {
-        if (host) return;
-        PM_HIP(hipHostMalloc((void **)&host, sizeof(Rec), hipHostMallocCoherent | hipHostMallocMapped));
+    bool failed = false;   // no such memory here: the build copies the state back instead
+    // true when the mirror is usable
+    bool ensure() {
+        if (host || failed) return !failed;
+        if (hipHostMalloc((void **)&host, sizeof(Rec), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+            hipHostGetDevicePointer((void **)&dev, host, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            if (host) (void)hipHostFree(host);
+            host = nullptr;
+            dev = nullptr;
+            failed = true;
+            return false;
+        }
         __atomic_store_n(&host->seq, 0u, __ATOMIC_RELEASE);
-        PM_HIP(hipHostGetDevicePointer((void **)&dev, host, 0));
+        return true;
     }
 };
 
