@@ -39,13 +39,16 @@ constexpr int SCAN_T = 1024, SCAN_E = 4, SCAN_BLK = SCAN_T * SCAN_E;
 inline int blocks_for(long long n, int per) { return (int)((n + per - 1) / per); }
 
 // ------------------------------------------------------------------ scan
+// inclusive prefix sum over the wave's 64 lanes (every lane active): DPP
+// row shifts within each row of 16 lanes, then row broadcasts of lanes 15
+// and 31 into the rows after them (disabled lanes add the old value, 0)
 __device__ __forceinline__ int wave_incl_scan(int v) {
-    const int lane = __lane_id();
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int o = __shfl_up(v, off, 64);
-        if (lane >= off) v += o;
-    }
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);   // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);   // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);   // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);   // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);   // row_bcast:15, rows 1 and 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);   // row_bcast:31, rows 2 and 3
     return v;
 }
 
