@@ -320,18 +320,19 @@ __device__ __forceinline__ void read_keys(const int *__restrict__ keys, int node
             k[w] = w < 3 ? min(k[w], v) : max(k[w], v);
         }
 }
-// the reset keys of n nodes (every word a replica has)
-__device__ __forceinline__ long long key_words(long long n, bool repl) { return repl ? n * KEY_REPL * KEY_LINE : 6 * n; }
-__device__ __forceinline__ int key_reset_value(long long k, bool repl) {
-    return (repl ? k % KEY_LINE : k % 6) < 3 ? fkey(F_INF) : fkey(-F_INF);
+// The box keys of n nodes are 6 * n words, or (replicated) 6 * KEY_REPL * n
+// words spread over n * KEY_REPL lines; key word k of that count (its
+// address: key_addr) starts as +inf for mins, -inf for maxes
+__device__ __forceinline__ long long key_words(long long n, bool repl) { return 6 * n * (repl ? KEY_REPL : 1); }
+__device__ __forceinline__ long long key_addr(long long k, bool repl) { return repl ? (k / 6) * KEY_LINE + k % 6 : k; }
+__device__ __forceinline__ void key_reset(int *keys, long long n, bool repl) {
+    const long long nk = key_words(n, repl);
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < nk; k += (long long)gridDim.x * blockDim.x)
+        keys[key_addr(k, repl)] = k % 6 < 3 ? fkey(F_INF) : fkey(-F_INF);
 }
 
 // AabbMemset (ReduceSegments.h:328-336) of the roots' box keys
-__global__ void init_keys_kernel(int *keys, int count, int repl) {
-    const long long n = key_words(count, repl);
-    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
-        keys[k] = key_reset_value(k, repl);
-}
+__global__ void init_keys_kernel(int *keys, int count, int repl) { key_reset(keys, count, repl); }
 
 constexpr int BOX_TILE = BS * 8;   // consecutive triangles per block
 
@@ -936,10 +937,7 @@ __global__ __launch_bounds__(LBS) void leaf_pass_kernel(NodePtrs N, Lvl *L, Leve
     // the next level's box keys: every possible child slot (created <= 2A),
     // before any scatter joins into them
     {
-        const long long nk = key_words(2ll * A, repl_out);
-        for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < nk;
-             k += (long long)gridDim.x * blockDim.x)
-            keys_next[k] = key_reset_value(k, repl_out);
+        key_reset(keys_next, 2ll * A, repl_out);
     }
     for (int tile = s_lvl[3]; tile < ntiles; tile = next_tile(&B.ctr[par], &s_tile)) {
         const int i0 = tile * NT, n = min(NT, A - i0);
@@ -2065,6 +2063,7 @@ void wait_level(BuildScratch &sc, Lvl &h, unsigned seq, hipStream_t s) {
     }
 }
 
+// ints a box-key buffer of n nodes spans
 inline long long key_words_host(long long n, bool repl) { return repl ? n * KEY_REPL * KEY_LINE : 6 * n; }
 
 inline int grid_for(long long n, int per, int cap) {
@@ -2115,7 +2114,7 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
     }
     init_roots_kernel<<<blocks_for(M, BS), BS, 0, s>>>(node_ptrs(ctx), off_dev, M);
     PM_LAUNCH_CHECK();
-    init_keys_kernel<<<grid_for(key_words_host(M, repl0), BS, 1024), BS, 0, s>>>(sc.keys[0].ptr, M, repl0);
+    init_keys_kernel<<<grid_for(6ll * M * (repl0 ? KEY_REPL : 1), BS, 1024), BS, 0, s>>>(sc.keys[0].ptr, M, repl0);
     PM_LAUNCH_CHECK();
     if (N > 0) {
         tri_box_kernel<<<blocks_for(N, BOX_TILE), BS, 0, s>>>(P0, P1, P2, N, sc.ref_min[0].ptr,
@@ -2194,7 +2193,11 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
                              (const int *)sc.keys[par].ptr, (const float4 *)sc.ref_min[par].ptr,
                              (const float4 *)sc.ref_max[par].ptr, dg);
         PM_LAUNCH_CHECK();
-        const int g3 = std::max(grid_for(A_g, NT, INT_MAX), grid_for(12 * A_g, 16 * BS, 2048));
+        // the tiles, and enough blocks to reset the next level's keys quickly
+        // (replicated after few-node levels: 8 times the words)
+        const int g3 = std::max(grid_for(A_g, NT, INT_MAX), A_g < JOIN_TEST_NODES
+                                                                ? grid_for(12 * KEY_REPL * A_g, 4 * LBS, 2048)
+                                                                : grid_for(12 * A_g, 16 * BS, 2048));
         const unsigned seq = mirror ? ++sc.mirror.next_seq : 0u;
         leaf_pass_kernel<<<graph ? std::min(g3, 2048) : g3, LBS, 0, s>>>(
             NP, L, B, par, LS, seg, sc.keys[par].ptr, sc.keys[nxt].ptr, prm.empty_space_splitting,
