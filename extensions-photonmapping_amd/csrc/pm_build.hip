@@ -2070,11 +2070,12 @@ inline int grid_for(long long n, int per, int cap) {
     return std::max(1, (int)std::min<long long>(blocks_for(std::max(n, 1ll), per), cap));
 }
 
-// The build's kernel sequence.  caps == nullptr: eager (the level state is
-// read back twice per upper level and once per lower level to size buffers
-// and grids; ends when no node is active).  caps != nullptr: captured (every
-// buffer is already at its capacity, grids are fixed, caps->upper_levels /
-// lower_levels levels are recorded, nothing is read back or synchronised).
+// The build's kernel sequence.  caps == nullptr: eager (each upper level's
+// leaf pass publishes the level state to the host, each lower level's state is
+// read back; they size the next level's buffers and grids; ends when no node
+// is active).  caps != nullptr: captured (every buffer is already at its
+// capacity, grids are fixed, caps->upper_levels / lower_levels levels are
+// recorded, nothing is read back or synchronised).
 void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2, int N, int M,
                const int *off_dev, const pm_build_params &prm, hipStream_t s, const BuildSizes *caps) {
     BuildScratch &sc = ctx->scratch;
@@ -2434,11 +2435,12 @@ void reserve_build_caps(pm_ctx *ctx, const BuildSizes &caps, int N, hipStream_t 
 // 1024] of that build's n.
 //
 // Switched off (PLANNED_BUILDS): a run of the 1M-triangle benchmark with it
-// on ended in a GPU memory fault whose cause is not established (a torn read
-// of the level state by late node-pass blocks, since fixed, is the suspected
-// one: planned builds launch the node pass with its capacity grid, so many
-// blocks start after the pass's last tile has advanced the level).  Until a
-// run shows them clean, every build is sized level by level.
+// on ended in a GPU memory fault whose cause is not established (the
+// suspected one, a torn read of the level state by blocks that start after a
+// pass's last tile has advanced the level, cannot happen with the level
+// passes as they are: they read the level's sizes from the slot of their own
+// parity).  Until a run shows planned builds clean, every build is sized
+// level by level.
 constexpr bool PLANNED_BUILDS = false;
 
 bool planned_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2, int N, int M,
