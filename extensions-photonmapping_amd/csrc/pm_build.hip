@@ -1481,7 +1481,13 @@ __global__ void lower_prep_kernel(NodePtrs N, const int *__restrict__ leaf_ids, 
     }
 }
 
-__global__ void plan_lower_start_kernel(Lvl *L) {
+// PRIM_SLOTS counters of the lower levels' prims: each wave of the screen
+// pass adds to slot (wave index % PRIM_SLOTS), so the adds do not queue on one
+// word; plan_lower_kernel folds them into the level state
+constexpr int PRIM_SLOTS = 64;
+
+__global__ void plan_lower_start_kernel(Lvl *L, unsigned long long *parts) {
+    if (blockIdx.x == 0 && threadIdx.x < PRIM_SLOTS) parts[threadIdx.x] = 0ull;
     if (threadIdx.x | blockIdx.x) return;
     L->An = L->status ? 0 : L->nleaf;
     L->use_list = 1;
@@ -1673,7 +1679,8 @@ __device__ __forceinline__ unsigned long long lane_u64(unsigned long long v, int
 // decision is its own, so the list order does not matter.
 __global__ __launch_bounds__(WPB * 64) void sah_screen_kernel(
     NodePtrs N, const int *__restrict__ leaf_ids, Lvl *__restrict__ L, const float *__restrict__ areas,
-    float Ct, int leaf_max, float bonus, int *__restrict__ flag, int *__restrict__ cand) {
+    float Ct, int leaf_max, float bonus, int *__restrict__ flag, int *__restrict__ cand,
+    unsigned long long *__restrict__ parts) {
     const int An = L->An, list_base = L->list_base;
     const int *list = L->use_list ? leaf_ids : nullptr;
     const int lane = threadIdx.x & 63;
@@ -1697,7 +1704,7 @@ __global__ __launch_bounds__(WPB * 64) void sah_screen_kernel(
             int t = n;
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
-            if (lane == 0) atomicAdd(&L->lower_prims, (unsigned long long)t);
+            if (lane == 0) atomicAdd(&parts[wave_index() % PRIM_SLOTS], (unsigned long long)t);
         }
         unsigned long long search = 0ull;
         int k = 0;
@@ -1818,8 +1825,15 @@ __global__ __launch_bounds__(WPB * 64) void balanced_kernel(
 }
 
 // after the SAH pass: place the split nodes' children
-__global__ void plan_lower_kernel(Lvl *L, const int *__restrict__ off) {
-    if (threadIdx.x | blockIdx.x) return;
+__global__ void plan_lower_kernel(Lvl *L, const int *__restrict__ off, unsigned long long *parts) {
+    if (blockIdx.x) return;
+    // the screen pass's prim counts (one wave of PRIM_SLOTS threads)
+    unsigned long long c = threadIdx.x < PRIM_SLOTS ? parts[threadIdx.x] : 0ull;
+    if (threadIdx.x < PRIM_SLOTS) parts[threadIdx.x] = 0ull;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if (threadIdx.x) return;
+    L->lower_prims += c;
     const int An = L->An;
     L->splits = 0;
     if (An == 0) return;
@@ -2263,7 +2277,8 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
                                                                                        sc.areas.ptr);
             PM_LAUNCH_CHECK();
         }
-        plan_lower_start_kernel<<<1, 64, 0, s>>>(L);
+        if (!graph) sc.prims_part.reserve(PRIM_SLOTS, s);
+        plan_lower_start_kernel<<<1, 64, 0, s>>>(L, sc.prims_part.ptr);
         PM_LAUNCH_CHECK();
         long long An = h.nleaf;
         for (int ll = 0; graph ? ll < caps->lower_levels : An > 0; ++ll) {
@@ -2281,7 +2296,7 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
             else {
                 sah_screen_kernel<<<grid_for((nAn + 63) / 64, WPB, graph ? 1024 : INT_MAX), WPB * 64, 0, s>>>(
                     NP, ctx->leaf_ids.ptr, L, sc.areas.ptr, prm.traversal_cost, prm.sah_leaf_max,
-                    prm.sah_empty_bonus, sc.lower_flag.ptr, sc.lower_list.ptr);
+                    prm.sah_empty_bonus, sc.lower_flag.ptr, sc.lower_list.ptr, sc.prims_part.ptr);
                 PM_LAUNCH_CHECK();
                 // the listed count is known on the device only: a grid-stride grid
                 sah_kernel<<<grid_for(nAn, WPB, 2048), WPB * 64, 0, s>>>(
@@ -2291,7 +2306,7 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
             }
             PM_LAUNCH_CHECK();
             scan_exclusive(sc.lower_flag.ptr, sc.lower_off.ptr, (int)nAn, sc, s, graph ? &L->An : nullptr);
-            plan_lower_kernel<<<1, 64, 0, s>>>(L, sc.lower_off.ptr);
+            plan_lower_kernel<<<1, 64, 0, s>>>(L, sc.lower_off.ptr, sc.prims_part.ptr);
             PM_LAUNCH_CHECK();
             if (!graph) {
                 read_state(sc, h, s);
@@ -2421,6 +2436,7 @@ void reserve_build_caps(pm_ctx *ctx, const BuildSizes &caps, int N, hipStream_t 
     sc.tri_rec.reserve(3 * (size_t)N + 3, s);
     sc.lower_sets.reserve(2 * (size_t)caps.lower, s);
     sc.lower_careas.reserve(2 * (size_t)caps.lower, s);
+    sc.prims_part.reserve(PRIM_SLOTS, s);
     ctx->packed.reserve((size_t)caps.nodes + 1, s);
     reserve_scan_tmp(sc, std::max({2 * caps.active, caps.segs, caps.lower}), s);
     sc.lvl.reserve(1, s);
@@ -2509,7 +2525,7 @@ std::vector<void **> graph_buffers(pm_ctx *ctx) {
         (void **)&ctx->packed.ptr, (void **)&ctx->leaf_tris.ptr, (void **)&sc.prim_min.ptr,
         (void **)&sc.prim_max.ptr, (void **)&sc.segL.ptr, (void **)&sc.segR.ptr,
         (void **)&sc.segLoff.ptr, (void **)&sc.segRoff.ptr, (void **)&sc.node_lna.ptr, (void **)&sc.node_lref.ptr,
-        (void **)&sc.node_lseg.ptr, (void **)&sc.node_sz.ptr, (void **)&sc.lb_ctr.ptr, (void **)&sc.areas.ptr, (void **)&sc.lower_flag.ptr,
+        (void **)&sc.node_lseg.ptr, (void **)&sc.node_sz.ptr, (void **)&sc.lb_ctr.ptr, (void **)&sc.prims_part.ptr, (void **)&sc.areas.ptr, (void **)&sc.lower_flag.ptr,
         (void **)&sc.lower_off.ptr, (void **)&sc.lower_sets.ptr, (void **)&sc.lower_careas.ptr,
         (void **)&sc.lower_list.ptr, (void **)&sc.tri_rec.ptr, (void **)&ctx->graph_off.ptr, (void **)&sc.lvl.ptr};
     for (int i = 0; i < 2; ++i) {

@@ -192,6 +192,7 @@ struct BuildScratch {
     DevBuf<int> lb_ctr;                         // their tile tickets (by level parity)
     DevBuf<int> scan_tmp[3];                    // block sums for scans, per level
     DevBuf<int> totals;                         // small device-side counters
+    DevBuf<unsigned long long> prims_part;      // lower stage: prims counted per wave slot (64)
     DevBuf<float> areas;                        // lower stage: triangle areas per final ref
     DevBuf<int> lower_list;                     // lower stage: nodes listed for the full SAH search
     DevBuf<int> lower_flag, lower_off;
