@@ -219,3 +219,29 @@ def test_random_configs_graph_equals_eager(pm, scenes, torch):
         g.close()
         checked += 1
     assert checked >= 18
+
+
+@pytest.mark.parametrize("meshes", [127, 130])
+def test_many_roots_graph_equals_eager_and_oracle(pm, orc, scenes, torch, meshes):
+    """forests on either side of 128 roots (fewer: the roots' and the first
+    levels' box keys are kept as line-separated replicas that the readers
+    fold; more: one copy): the eager build, a captured build's replay and the
+    oracle give the same tree bit for bit"""
+    c = scenes.config4(num_meshes=meshes, rays_per_mesh=1, hi=3000.0)
+    off = c["tri_off"]
+    tris = frame_tris(scenes, c, 2.0)
+    P = [dev(torch, t) for t in tris]
+    bp = pm.build_params("reference")
+    e = pm.Context(0)
+    e.build_batched(*P, off, bp)
+    torch.cuda.synchronize()
+    eager = {k: np.array(v, copy=True) for k, v in e.tree_arrays().items()}
+    e.close()
+    g = pm.Context(0)
+    g.batch_graph_capture(*P, off, None, None, None, None, pm.SHORT_STACK, bp)
+    g.batch_graph_launch()
+    assert g.batch_graph_status() == 0
+    assert_same_tree(g.tree_arrays(), eager, f"{meshes} meshes: replay vs eager")
+    g.close()
+    ref = orc.build_batched(tris, off, orc_params(orc, bp))
+    assert_same_tree(eager, ref.arrays(), f"{meshes} meshes: eager vs oracle")
