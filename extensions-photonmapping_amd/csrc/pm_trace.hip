@@ -938,14 +938,10 @@ int resident_blocks(K kernel, const RayBatch &R, const Launch &L) {
     return (int)std::min<long long>(((long long)R.n + TB - 1) / TB, (long long)(b > 0 ? b : 1) * L.num_cus);
 }
 
-#ifndef PM_COOP
-#define PM_COOP 1
-#endif
-
 template <int SMAX, bool SHORT, bool SKIP, bool COUNT, bool WOOP>
 void launch_trace(const TreeDev &T, const RayBatch &R, int S, pm_hit *out, int4 *cnt, const Launch &L,
                   hipStream_t s) {
-    if (!WOOP && PM_COOP && 48ll * T.num_slots < (1ll << 32)) {
+    if (!WOOP && 48ll * T.num_slots < (1ll << 32)) {
         if (L.step_cap > 0) {
             auto k = trace_coop_kernel<SMAX, SHORT, SKIP, COUNT, true>;
             k<<<resident_blocks(k, R, L), TB, 0, s>>>(T, R.org, R.dir, R.root, R.order, R.n, S, out, cnt, L.work,
