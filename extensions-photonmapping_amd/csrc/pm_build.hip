@@ -1575,6 +1575,104 @@ struct SahLds {
     float4 hi[WPB][64];   // prim max corner
 };
 
+// sah_node for a node whose prims all sit in bits 0..31 (every node of a
+// lower stage with lower_size <= 32): lanes 2i and 2i+1 both take bit i,
+// the even lane its three low planes, the odd lane its three high planes,
+// so a pass over the prims costs each lane half the comparisons.  Candidate
+// numbers, in-order side sums and costs are those of the six-plane lane.
+__device__ __forceinline__ void sah_node_pairs(
+    const NodePtrs &N, int id, int par, SahLds &lds, const float4 *__restrict__ pmin,
+    const float4 *__restrict__ pmax, const float *__restrict__ areas, float Ct, int leaf_max, float bonus,
+    int *__restrict__ flag, long long *__restrict__ sets, float *__restrict__ careas, int pi,
+    unsigned long long bmp) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int bit = lane >> 1, high = lane & 1;
+    const int n = __popcll(bmp);
+    const float nodeA = N.area[par];
+    const bool forced = leaf_max > 0 && n > leaf_max;
+    const float leaf_cost = ((float)n - Ct) * nodeA;
+    const bool has = (bmp >> bit) & 1ull;
+    const float a = has ? areas[pi + bit] : 0.0f;
+    float4 mn = make_float4(0.0f, 0.0f, 0.0f, 0.0f), mx = mn;
+    if (has) { mn = pmin[pi + bit]; mx = pmax[pi + bit]; }
+    float4 *lo = lds.lo[w], *hi = lds.hi[w];
+    if (!high) {
+        lo[bit] = make_float4(mn.x, mn.y, mn.z, a);
+        hi[bit] = mx;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    // plane k (axis k): candidate 2k + high of the prim
+    const float p[3] = {high ? mx.x : mn.x, high ? mx.y : mn.y, high ? mx.z : mn.z};
+    float sL[3], sR[3];
+    int nL[3], nR[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { sL[k] = 0.0f; sR[k] = 0.0f; nL[k] = 0; nR[k] = 0; }
+    for (unsigned long long b = bmp; b; b &= b - 1) {
+        const int j = __builtin_ctzll(b);
+        const float4 q = lo[j], r = hi[j];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float qm = k == 0 ? q.x : k == 1 ? q.y : q.z;
+            const float rm = k == 0 ? r.x : k == 1 ? r.y : r.z;
+            if (qm <= p[k]) { sL[k] += q.w; ++nL[k]; }
+            if (p[k] < rm) { sR[k] += q.w; ++nR[k]; }
+        }
+    }
+    float best = F_INF;
+    int bestc = INT_MAX;
+    const int rank = __popcll(bmp & ((1ull << bit) - 1ull));
+    if (has) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float cost = (float)nL[k] * sL[k] + (float)nR[k] * sR[k];
+            if (bonus != 1.0f && (nL[k] == 0 || nR[k] == 0)) cost = cost * bonus;
+            const bool progress = nL[k] != n && nR[k] != n;
+            if (forced && !progress) continue;
+            const int c = 6 * rank + 2 * k + high;
+            if (cost < best || (cost == best && c < bestc)) { best = cost; bestc = c; }
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const float ob = __shfl_xor(best, off, 64);
+        const int oc = __shfl_xor(bestc, off, 64);
+        if (ob < best || (ob == best && oc < bestc)) { best = ob; bestc = oc; }
+    }
+    bool split = false;
+    if (bestc != INT_MAX) {   // wave-uniform: every lane holds the same winner
+        const int kk = bestc % 6;
+        const int k3 = kk >> 1;
+        const int axis = k3 + 1;
+        const int owner = __builtin_ctzll(__ballot(has && rank == bestc / 6 && high == (kk & 1)));
+        float mp = p[0], ml = sL[0], mr = sR[0];
+#pragma unroll
+        for (int k = 1; k < 3; ++k)
+            if (k3 == k) { mp = p[k]; ml = sL[k]; mr = sR[k]; }
+        const float pos = __shfl(mp, owner, 64);
+        const float aL = __shfl(ml, owner, 64), aR = __shfl(mr, owner, 64);
+        // the children's bitmaps: lane j < 32 tests bit j's prim
+        const bool hasj = lane < 32 && ((bmp >> lane) & 1ull);
+        const float4 qj = lo[lane & 31], rj = hi[lane & 31];
+        const float lo_ax = axis == AX_X ? qj.x : axis == AX_Y ? qj.y : qj.z;
+        const float hi_ax = axis == AX_X ? rj.x : axis == AX_Y ? rj.y : rj.z;
+        const unsigned long long L = __ballot(hasj && lo_ax <= pos);
+        const unsigned long long R = __ballot(hasj && pos < hi_ax);
+        const bool progress = L != bmp && R != bmp;
+        split = progress && (forced || best < leaf_cost);
+        if (split && lane == 0) {
+            N.info[par] = (int8_t)axis;
+            N.split[par] = pos;
+            sets[2 * id] = (long long)L; sets[2 * id + 1] = (long long)R;
+            careas[2 * id] = aL; careas[2 * id + 1] = aR;
+        }
+    }
+    if (lane == 0) {
+        if (!split) N.info[par] = LEAF;
+        flag[id] = split;
+    }
+}
+
 __device__ __forceinline__ void sah_node(
     const NodePtrs &N, int id, int par, SahLds &lds, const float4 *__restrict__ pmin,
     const float4 *__restrict__ pmax, const float *__restrict__ areas, float Ct, int leaf_max, float bonus,
@@ -1582,6 +1680,10 @@ __device__ __forceinline__ void sah_node(
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int pi = N.prim_index[par];
     const unsigned long long bmp = (unsigned long long)N.bitmap[par];
+    if ((bmp >> 32) == 0ull) {
+        sah_node_pairs(N, id, par, lds, pmin, pmax, areas, Ct, leaf_max, bonus, flag, sets, careas, pi, bmp);
+        return;
+    }
     const int n = __popcll(bmp);
     const float nodeA = N.area[par];
     const bool forced = leaf_max > 0 && n > leaf_max;
