@@ -1575,6 +1575,67 @@ struct SahLds {
     float4 hi[WPB][64];   // prim max corner
 };
 
+// sah_cannot_split for two nodes at once, one per half-wave: lanes 0..31
+// hold bits 0..31 of the first node (bitmap b1, n1 prims, leaf cost c1),
+// lanes 32..63 those of the second.  Returns, per lane, its half's answer.
+// The sort network, prefix sums and minimum are the single-node ones over
+// the groups' P <= 32 lanes (shuffles with offsets below P stay in a half).
+template <int P>
+__device__ __forceinline__ bool sah_bound_exceeds_half(float v, int n, float bonus, float leaf_cost) {
+    const int lane = __lane_id(), hl = lane & 31, base = lane & 32;
+#pragma unroll
+    for (int k = 2; k <= P; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const float o = __shfl_xor(v, j, 64);
+            const bool up = (hl & k) == 0 || k == P;
+            const bool low = (hl & j) == 0;
+            v = (low == up) ? fminf(v, o) : fmaxf(v, o);
+        }
+    }
+    float pre = hl < n ? v : 0.0f;
+#pragma unroll
+    for (int off = 1; off < P; off <<= 1) {
+        const float o = __shfl_up(pre, off, 64);
+        if (hl >= off) pre += o;
+    }
+    const int k = hl, m = n - hl;
+    const float sk = __shfl(pre, base + (k > 0 ? k - 1 : 0), 64);
+    const float sm = __shfl(pre, base + (m > 0 ? m - 1 : 0), 64);
+    float f = __int_as_float(0x7f800000);
+    if (k <= n && k < P) {
+        f = (float)k * (k > 0 ? sk : 0.0f) + (float)m * (m > 0 ? sm : 0.0f);
+        if (k == 0 || m == 0) f *= fminf(bonus, 1.0f);
+    }
+#pragma unroll
+    for (int off = P >> 1; off > 0; off >>= 1) f = fminf(f, __shfl_xor(f, off, 64));
+    return __shfl(f, base, 64) * (1.0f - 1.0f / 1024.0f) >= leaf_cost;
+}
+
+__device__ bool sah_cannot_split_pair(unsigned long long b1, unsigned long long b2, float a, int n1, int n2,
+                                      float bonus, float c1, float c2) {
+    const int lane = __lane_id(), hl = lane & 31, base = lane & 32;
+    const unsigned bits = (unsigned)(lane < 32 ? b1 : b2);
+    const int n = lane < 32 ? n1 : n2;
+    const float leaf_cost = lane < 32 ? c1 : c2;
+    const bool has = (bits >> hl) & 1u;
+    const unsigned long long bad = __ballot(has && !(a <= 3.0e38f));
+    const bool half_bad = ((lane < 32 ? bad : bad >> 32) & 0xffffffffull) != 0ull;
+    // present lane -> base + its rank among the present lanes of its half,
+    // absent lane -> base + n + its rank among the absent ones
+    const int rank = __popc(bits & ((1u << hl) - 1u));
+    const int dst = base + (has ? rank : n + (hl - rank));
+    float v = __int_as_float(__builtin_amdgcn_ds_permute(dst << 2, __float_as_int(a)));
+    if (hl >= n) v = __int_as_float(0x7f800000);
+    const int nm = n1 > n2 ? n1 : n2;
+    bool r;
+    if (nm <= 4) r = sah_bound_exceeds_half<4>(v, n, bonus, leaf_cost);
+    else if (nm <= 8) r = sah_bound_exceeds_half<8>(v, n, bonus, leaf_cost);
+    else if (nm <= 16) r = sah_bound_exceeds_half<16>(v, n, bonus, leaf_cost);
+    else r = sah_bound_exceeds_half<32>(v, n, bonus, leaf_cost);
+    return r && !half_bad;
+}
+
 // sah_node for a node whose prims all sit in bits 0..31 (every node of a
 // lower stage with lower_size <= 32): lanes 2i and 2i+1 both take bit i,
 // the even lane its three low planes, the odd lane its three high planes,
@@ -1809,29 +1870,63 @@ __global__ __launch_bounds__(WPB * 64) void sah_screen_kernel(
             if (lane == 0) atomicAdd(&parts[wave_index() % PRIM_SLOTS], (unsigned long long)t);
         }
         unsigned long long search = 0ull;
-        int k = 0;
+        // the nodes of m in groups: two nodes, one per half-wave, when both
+        // bitmaps fit in bits 0..31 (k2 >= 0), else one node on the whole wave
+        auto group = [&](unsigned long long mm, int &k1, int &k2) {
+            k1 = __builtin_ctzll(mm);
+            const unsigned long long rest = mm & (mm - 1);
+            k2 = -1;
+            if (rest) {
+                const int c = __builtin_ctzll(rest);
+                if (((lane_u64(bmp, k1) | lane_u64(bmp, c)) >> 32) == 0ull) k2 = c;
+            }
+        };
+        // this lane's area of the group: bit `lane` of node k1, or bit
+        // lane % 32 of node k1 (lanes 0..31) / k2 (lanes 32..63)
+        auto load = [&](int k1, int k2) {
+            float v = 0.0f;
+            if (k2 < 0) {
+                if ((lane_u64(bmp, k1) >> lane) & 1ull) v = areas[lane_int(pi, k1) + lane];
+            } else {
+                const unsigned long long kb = lane < 32 ? lane_u64(bmp, k1) : lane_u64(bmp, k2);
+                const int kp = lane < 32 ? lane_int(pi, k1) : lane_int(pi, k2);
+                if ((kb >> (lane & 31)) & 1ull) v = areas[kp + (lane & 31)];
+            }
+            return v;
+        };
+        int k1 = 0, k2 = -1;
         float a = 0.0f;
         if (m) {
-            k = __builtin_ctzll(m);
-            const unsigned long long kb = lane_u64(bmp, k);
-            if ((kb >> lane) & 1ull) a = areas[lane_int(pi, k) + lane];
+            group(m, k1, k2);
+            a = load(k1, k2);
         }
         while (m) {
-            const unsigned long long m2 = m & (m - 1);
-            int k2 = 0;
-            float a2 = 0.0f;
-            if (m2) {   // the next node's areas, loaded while this node is decided
-                k2 = __builtin_ctzll(m2);
-                const unsigned long long kb2 = lane_u64(bmp, k2);
-                if ((kb2 >> lane) & 1ull) a2 = areas[lane_int(pi, k2) + lane];
+            m &= ~(1ull << k1);
+            if (k2 >= 0) m &= ~(1ull << k2);
+            int nk1 = 0, nk2 = -1;
+            float na = 0.0f;
+            if (m) {   // the next group's areas, loaded while this group is decided
+                group(m, nk1, nk2);
+                na = load(nk1, nk2);
             }
-            const unsigned long long kb = lane_u64(bmp, k);
+            const unsigned long long kb = lane_u64(bmp, k1);
             const int kn = __popcll(kb);
             const bool kforced = leaf_max > 0 && kn > leaf_max;
-            if (kforced || !sah_cannot_split(kb, a, kn, bonus, lane_float(leaf_cost, k))) search |= 1ull << k;
-            m = m2;
-            k = k2;
-            a = a2;
+            if (k2 < 0) {
+                if (kforced || !sah_cannot_split(kb, a, kn, bonus, lane_float(leaf_cost, k1))) search |= 1ull << k1;
+            } else {
+                const unsigned long long kb2 = lane_u64(bmp, k2);
+                const int kn2 = __popcll(kb2);
+                const bool kforced2 = leaf_max > 0 && kn2 > leaf_max;
+                const bool r = sah_cannot_split_pair(kb, kb2, a, kn, kn2, bonus, lane_float(leaf_cost, k1),
+                                                     lane_float(leaf_cost, k2));
+                const unsigned long long rb = __ballot(r);
+                if (kforced || !(rb & 1ull)) search |= 1ull << k1;
+                if (kforced2 || !((rb >> 32) & 1ull)) search |= 1ull << k2;
+            }
+            k1 = nk1;
+            k2 = nk2;
+            a = na;
         }
         const bool listed = (search >> lane) & 1ull;
         if (valid && !listed) {
