@@ -1106,17 +1106,16 @@ __global__ __launch_bounds__(LBS) void leaf_pass_kernel(NodePtrs N, Lvl *L, Leve
             const int created = L->status ? 0 : L->created;
             const int nt = max(1, (created + NT - 1) / NT);
             for (int k = t; k < 8 * nt; k += LBS) B.lb_n[k] = 0ull;
-            // an eager build: the level state to the host, then its number
+            // an eager build: the level state to the host, each word tagged
+            // with the level's number (one 64-bit store per word)
             if (mirror) {
-                constexpr int NW = (int)(sizeof(Lvl) / sizeof(int));
+                constexpr int NW = LvlMirror::NW;
                 static_assert(NW <= LBS, "one word per thread");
-                if (t < NW) {
-                    __hip_atomic_store(reinterpret_cast<int *>(&mirror->lvl) + t, reinterpret_cast<const int *>(L)[t],
+                if (t < NW)
+                    __hip_atomic_store(&mirror->w[t],
+                                       (unsigned long long)seq << 32 |
+                                           (unsigned)reinterpret_cast<const int *>(L)[t],
                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-                    __threadfence_system();
-                }
-                __syncthreads();
-                if (t == 0) __hip_atomic_store(&mirror->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
             }
         }
     }
@@ -2248,15 +2247,26 @@ void read_state(BuildScratch &sc, Lvl &h, hipStream_t s) {
 // reports the stream's error instead of being waited for
 void wait_level(BuildScratch &sc, Lvl &h, unsigned seq, hipStream_t s) {
     const LvlMirror::Rec *m = sc.mirror.host;
-    for (unsigned it = 1; __atomic_load_n(&m->seq, __ATOMIC_ACQUIRE) != seq; ++it) {
+    int words[LvlMirror::NW];
+    // every word of the record tagged with seq (word 0 is the cheap spin)
+    auto take = [&]() {
+        if ((unsigned)(__atomic_load_n(&m->w[0], __ATOMIC_ACQUIRE) >> 32) != seq) return false;
+        for (int t = 0; t < LvlMirror::NW; ++t) {
+            const unsigned long long v = __atomic_load_n(&m->w[t], __ATOMIC_ACQUIRE);
+            if ((unsigned)(v >> 32) != seq) return false;
+            words[t] = (int)(unsigned)v;
+        }
+        return true;
+    };
+    for (unsigned it = 1; !take(); ++it) {
         if (it % 4096) continue;
         const hipError_t q = hipStreamQuery(s);
         if (q == hipErrorNotReady) continue;
         PM_HIP(q);
-        if (__atomic_load_n(&m->seq, __ATOMIC_ACQUIRE) == seq) break;
+        if (take()) break;
         throw Error{PM_ERR_INTERNAL, "a level's state was not published"};
     }
-    std::memcpy(&h, (const void *)&m->lvl, sizeof(Lvl));
+    std::memcpy(&h, words, sizeof(Lvl));
 }
 
 [[noreturn]] void throw_status(const Lvl &h) {
@@ -2410,7 +2420,11 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
         const int g3 = std::max(grid_for(A_g, NT, INT_MAX), A_g < JOIN_TEST_NODES
                                                                 ? grid_for(12 * KEY_REPL * A_g, 4 * LBS, 2048)
                                                                 : grid_for(12 * A_g, 16 * BS, 2048));
-        const unsigned seq = mirror ? ++sc.mirror.next_seq : 0u;
+        unsigned seq = 0u;
+        if (mirror) {   // never 0, the tag of a record no level wrote
+            seq = ++sc.mirror.next_seq;
+            if (seq == 0u) seq = sc.mirror.next_seq = 1u;
+        }
         leaf_pass_kernel<<<graph ? std::min(g3, 2048) : g3, LBS, 0, s>>>(
             NP, L, B, par, LS, seg, sc.keys[par].ptr, sc.keys[nxt].ptr, prm.empty_space_splitting,
             prm.empty_space_threshold, mirror ? sc.mirror.dev : nullptr, seq);

@@ -143,9 +143,13 @@ struct Lvl {
 // (system-scope release); the host waits for that number instead of copying
 // the state back and synchronising the stream.
 struct LvlMirror {
+    static constexpr int NW = (int)(sizeof(Lvl) / sizeof(int));
+    // word t of the level state in the low half of w[t], the number of the
+    // level that wrote it in the high half: the host takes the record once
+    // every word carries the number it waits for, so the device needs no
+    // ordering between the words' stores
     struct Rec {
-        Lvl lvl;
-        unsigned seq;
+        unsigned long long w[NW];
     };
     Rec *host = nullptr;   // coherent, mapped
     Rec *dev = nullptr;    // the device's address of it
@@ -164,7 +168,7 @@ struct LvlMirror {
             failed = true;
             return false;
         }
-        __atomic_store_n(&host->seq, 0u, __ATOMIC_RELEASE);
+        for (int t = 0; t < NW; ++t) __atomic_store_n(&host->w[t], 0ull, __ATOMIC_RELEASE);
         return true;
     }
 };
