@@ -255,17 +255,6 @@ __device__ __forceinline__ void zero_node(const NodePtrs &N, int id) {
 
 // ------------------------------------------------------------------ root
 
-// TriangleMapUpperCreator.cu:128-137, one root per mesh: root m covers
-// triangles [off[m], off[m+1]) and is its own parent
-__global__ void init_roots_kernel(NodePtrs N, const int *__restrict__ off, int M) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= M) return;
-    zero_node(N, m);
-    N.prim_index[m] = off[m];
-    N.bitmap[m] = off[m + 1] - off[m];
-    N.parent[m] = m;
-}
-
 // mesh of triangle i: the last m with off[m] <= i, searched in [lo, hi]
 __device__ __forceinline__ int mesh_of(const int *__restrict__ off, int i, int lo, int hi) {
     while (lo < hi) {
@@ -331,8 +320,6 @@ __device__ __forceinline__ void key_reset(int *keys, long long n, bool repl) {
         keys[key_addr(k, repl)] = k % 6 < 3 ? fkey(F_INF) : fkey(-F_INF);
 }
 
-// AabbMemset (ReduceSegments.h:328-336) of the roots' box keys
-__global__ void init_keys_kernel(int *keys, int count, int repl) { key_reset(keys, count, repl); }
 
 constexpr int BOX_TILE = BS * 8;   // consecutive triangles per block
 
@@ -1365,16 +1352,6 @@ __global__ __launch_bounds__(WPB * 64) void scatter_pass_div_kernel(
                              dg);
 }
 
-// look-back words and tickets of the roots' segment scan (parity 1) and of
-// the first level's leaf pass (parity 0)
-__global__ void lb_init_kernel(unsigned long long *lb0, unsigned long long *lb1, int n, int *ctr) {
-    GRID_STRIDE(k, n) {
-        lb0[k] = 0ull;
-        lb1[k] = 0ull;
-    }
-    if (blockIdx.x == 0 && threadIdx.x < 2) ctr[threadIdx.x] = 0;
-}
-
 // ---- plan kernels: the host arithmetic of the level loop
 // (TriangleMapUpperCreator.cu:140-180), on the device
 __host__ __device__ void lvl_fail(Lvl *L, int code) {
@@ -2224,8 +2201,27 @@ __global__ void single_offsets_kernel(int *off, int N) {
     if (threadIdx.x == 0 && blockIdx.x == 0) { off[0] = 0; off[1] = N; }
 }
 
-__global__ void init_lvl_kernel(Lvl *L, Lvl v) {
+// a build's first launch, each part over the whole grid: the level state;
+// one root per mesh (TriangleMapUpperCreator.cu:128-137): root m covers
+// triangles [off[m], off[m+1]) and is its own parent; the roots' box keys
+// (AabbMemset, ReduceSegments.h:328-336); the look-back words and tickets of
+// the roots' segment scan (parity 1) and of the first level's leaf pass
+// (parity 0)
+__global__ void build_init_kernel(Lvl *L, Lvl v, NodePtrs N, const int *__restrict__ off, int M, int *keys,
+                                  int repl, unsigned long long *lb0, unsigned long long *lb1, int nlb, int *ctr) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *L = v;
+    GRID_STRIDE(m, M) {
+        zero_node(N, m);
+        N.prim_index[m] = off[m];
+        N.bitmap[m] = off[m + 1] - off[m];
+        N.parent[m] = m;
+    }
+    key_reset(keys, M, repl);
+    GRID_STRIDE(k, nlb) {
+        lb0[k] = 0ull;
+        lb1[k] = 0ull;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < 2) ctr[threadIdx.x] = 0;
 }
 
 // a captured build that needed more levels than it recorded
@@ -2326,18 +2322,24 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
     sc.lvl.reserve(1, s);
     const bool mirror = !graph && sc.mirror.ensure();
     Lvl *L = sc.lvl.ptr;
-    init_lvl_kernel<<<1, 64, 0, s>>>(L, h);
-    PM_LAUNCH_CHECK();
+    auto tiles = [](long long n, long long per) { return std::max(1ll, (n + per - 1) / per); };
+    const int nlb = (int)(8 * tiles(M, NT) + 8);
     if (!graph) {
         nodes.reserve(std::max(1024, 2 * M), s, 0);
         sc.ref_min[0].reserve((size_t)N + 1, s); sc.ref_max[0].reserve((size_t)N + 1, s);
         sc.tri_rec.reserve(3 * (size_t)N + 3, s);
         sc.keys[0].reserve((size_t)key_words_host(M, repl0), s);
+        sc.lb_ctr.reserve(4, s);
+        for (int q = 0; q < 2; ++q) sc.lb_words[q].reserve(nlb, s);
     }
-    init_roots_kernel<<<blocks_for(M, BS), BS, 0, s>>>(node_ptrs(ctx), off_dev, M);
-    PM_LAUNCH_CHECK();
-    init_keys_kernel<<<grid_for(6ll * M * (repl0 ? KEY_REPL : 1), BS, 1024), BS, 0, s>>>(sc.keys[0].ptr, M, repl0);
-    PM_LAUNCH_CHECK();
+    {
+        const long long g = std::max({(long long)blocks_for(M, BS),
+                                      (long long)grid_for(6ll * M * (repl0 ? KEY_REPL : 1), BS, 1024),
+                                      (long long)grid_for(nlb, BS, 64)});
+        build_init_kernel<<<(int)g, BS, 0, s>>>(L, h, node_ptrs(ctx), off_dev, M, sc.keys[0].ptr, repl0,
+                                                 sc.lb_words[0].ptr, sc.lb_words[1].ptr, nlb, sc.lb_ctr.ptr);
+        PM_LAUNCH_CHECK();
+    }
     if (N > 0) {
         tri_box_kernel<<<blocks_for(N, BOX_TILE), BS, 0, s>>>(P0, P1, P2, N, sc.ref_min[0].ptr,
                                                               sc.ref_max[0].ptr, off_dev, M, sc.keys[0].ptr,
@@ -2347,7 +2349,6 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
 
     const DivideGeom dg = {prm.split_method == PM_SPLIT_DIVIDE ? P0 : nullptr, P1, P2};
     const bool div = dg.p0 != nullptr;
-    auto tiles = [](long long n, long long per) { return std::max(1ll, (n + per - 1) / per); };
     auto level_bufs = [&](int par) {
         const int q = par ^ 1;
         LevelBufs B;
@@ -2367,14 +2368,9 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
     // ceil(N / seg) (at most one partial segment per root)
     long long S_now = graph ? caps->segs : M + (N + seg - 1) / seg;
     if (!graph) {
-        sc.lb_ctr.reserve(4, s);
-        for (int q = 0; q < 2; ++q) sc.lb_words[q].reserve(8 * tiles(M, NT) + 8, s);
         sc.node_segbase[0].reserve((size_t)M + 1, s);
         for (DevBuf<int> *b : {&sc.seg_node[0], &sc.seg_start[0], &sc.seg_count[0]}) b->reserve(S_now + 1, s);
     }
-    lb_init_kernel<<<grid_for(8 * tiles(M, NT) + 8, BS, 64), BS, 0, s>>>(
-        sc.lb_words[0].ptr, sc.lb_words[1].ptr, (int)(8 * tiles(M, NT) + 8), sc.lb_ctr.ptr);
-    PM_LAUNCH_CHECK();
     root_segments_kernel<<<(int)tiles(M, NT), LBS, 0, s>>>(node_ptrs(ctx), L, sc.node_segbase[0].ptr,
                                                            sc.seg_node[0].ptr, sc.seg_start[0].ptr,
                                                            sc.seg_count[0].ptr, sc.lb_words[1].ptr,
