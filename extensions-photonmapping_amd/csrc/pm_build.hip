@@ -584,8 +584,11 @@ __device__ void lookback(unsigned long long *__restrict__ w, int tile, const lon
     if (j == 0 && k < K) lb_store(&w[(long long)K * tile + k], LB_INC | (unsigned long long)(sum + a));
 }
 
-// the block's next tile (ticket order); every thread returns the same value
-__device__ __forceinline__ int next_tile(int *ctr, int *s_tile) {
+// the block's next tile (ticket order); every thread returns the same value.
+// With at least as many blocks as tiles, the blocks' first tickets cover
+// every tile and a block stops after its first, with no second ticket
+__device__ __forceinline__ int next_tile(int *ctr, int *s_tile, int ntiles) {
+    if (ntiles <= (int)gridDim.x) return ntiles;
     if (threadIdx.x == 0) *s_tile = atomicAdd(ctr, 1);
     __syncthreads();
     const int t = *s_tile;
@@ -721,7 +724,7 @@ __global__ __launch_bounds__(LBS) void root_segments_kernel(NodePtrs N, Lvl *L, 
     __syncthreads();
     const long long cap_segs = L->cap_segs;
     const int ntiles = max(1, (M + NT - 1) / NT);
-    for (int tile = s_first; tile < ntiles; tile = next_tile(ctr, &s_tile)) {
+    for (int tile = s_first; tile < ntiles; tile = next_tile(ctr, &s_tile, ntiles)) {
         const int i0 = tile * NT, n = min(NT, M - i0);
         int cnt[NPT], pi[NPT], v[2] = {0, 0};
 #pragma unroll
@@ -926,7 +929,7 @@ __global__ __launch_bounds__(LBS) void leaf_pass_kernel(NodePtrs N, Lvl *L, Leve
     {
         key_reset(keys_next, 2ll * A, repl_out);
     }
-    for (int tile = s_lvl[3]; tile < ntiles; tile = next_tile(&B.ctr[par], &s_tile)) {
+    for (int tile = s_lvl[3]; tile < ntiles; tile = next_tile(&B.ctr[par], &s_tile, ntiles)) {
         const int i0 = tile * NT, n = min(NT, A - i0);
         // every node input first, then the node records (the stores would
         // otherwise hold back the next node's loads)
