@@ -742,10 +742,12 @@ def main() -> int:
     torch.cuda.synchronize()
     bytes_pass0 = ray_bytes(cnt.to(torch.int64))
     counts0 = ray_counts(cnt)
-    pass0_ms = float(np.median(timing["pass0_ms"]))
     hits0 = H.view(torch.int32)[:, 0]
     hit_frac = float((hits0 >= 0).float().mean())
-    roof = trace_roofline(os.path.join(ROOT, "profiles", "trace_c3_pmc.json"), n0, ntris, args.stack, pass0_ms,
+    # the call the counter summary describes: the emission batch in coherent
+    # order (the photon driver may have timed its first pass in input order)
+    roof_ms = timed(torch, lambda: ctx.trace(pm.SHORT_STACK, O0, D0, tp, out=H), max(args.steps, 3))
+    roof = trace_roofline(os.path.join(ROOT, "profiles", "trace_c3_pmc.json"), n0, ntris, args.stack, roof_ms,
                           bytes_pass0, counts0)
 
     # ---- other tracers on the same tree (same GPU): kd-restart, brute force
