@@ -892,6 +892,17 @@ __device__ ChildPlan plan_children_totals(const Lvl *L, int L_all, int R_all, in
     return P;
 }
 
+// an eager build's level state to the host: word w of *L, tagged with the
+// level's number seq, in one 64-bit system-scope store per word (the
+// block's threads share the words; the host takes the record once every
+// word carries seq, so the stores need no order)
+__device__ __forceinline__ void publish_level(const Lvl *L, LvlMirror::Rec *mirror, unsigned seq) {
+    for (int w = threadIdx.x; w < LvlMirror::NW; w += blockDim.x)
+        __hip_atomic_store(&mirror->w[w],
+                           (unsigned long long)seq << 32 | (unsigned)reinterpret_cast<const int *>(L)[w],
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // the leaf pass keeps the side counts' tile-local prefixes of a tile's first
 // SEG_LDS segments in LDS (the rest are read back from segLoff / segRoff)
 constexpr int SEG_LDS = 2048;
@@ -1096,17 +1107,8 @@ __global__ __launch_bounds__(LBS) void leaf_pass_kernel(NodePtrs N, Lvl *L, Leve
             const int created = L->status ? 0 : L->created;
             const int nt = max(1, (created + NT - 1) / NT);
             for (int k = t; k < 8 * nt; k += LBS) B.lb_n[k] = 0ull;
-            // an eager build: the level state to the host, each word tagged
-            // with the level's number (one 64-bit store per word)
-            if (mirror) {
-                constexpr int NW = LvlMirror::NW;
-                static_assert(NW <= LBS, "one word per thread");
-                if (t < NW)
-                    __hip_atomic_store(&mirror->w[t],
-                                       (unsigned long long)seq << 32 |
-                                           (unsigned)reinterpret_cast<const int *>(L)[t],
-                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-            }
+            // an eager build: the level state to the host
+            if (mirror) publish_level(L, mirror, seq);
         }
     }
 }
@@ -2000,15 +2002,8 @@ __global__ __launch_bounds__(WPB * 64) void balanced_kernel(
     }
 }
 
-// after the SAH pass: place the split nodes' children
-__global__ void plan_lower_kernel(Lvl *L, const int *__restrict__ off, unsigned long long *parts) {
-    if (blockIdx.x) return;
-    // the screen pass's prim counts (one wave of PRIM_SLOTS threads)
-    unsigned long long c = threadIdx.x < PRIM_SLOTS ? parts[threadIdx.x] : 0ull;
-    if (threadIdx.x < PRIM_SLOTS) parts[threadIdx.x] = 0ull;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-    if (threadIdx.x) return;
+// after the SAH pass: place the split nodes' children (c: the level's prims)
+__device__ void plan_lower_apply(Lvl *L, const int *__restrict__ off, unsigned long long c) {
     L->lower_prims += c;
     const int An = L->An;
     L->splits = 0;
@@ -2023,6 +2018,21 @@ __global__ void plan_lower_kernel(Lvl *L, const int *__restrict__ off, unsigned 
     L->splits = splits;
     L->lower_base = L->nodes_size;
     L->nodes_size += 2 * splits;
+}
+
+// one block of 64 threads; an eager build (mirror) also publishes the state
+__global__ void plan_lower_kernel(Lvl *L, const int *__restrict__ off, unsigned long long *parts,
+                                  LvlMirror::Rec *mirror, unsigned seq) {
+    if (blockIdx.x) return;
+    // the screen pass's prim counts (one wave of PRIM_SLOTS threads)
+    unsigned long long c = threadIdx.x < PRIM_SLOTS ? parts[threadIdx.x] : 0ull;
+    if (threadIdx.x < PRIM_SLOTS) parts[threadIdx.x] = 0ull;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if (threadIdx.x == 0) plan_lower_apply(L, off, c);
+    if (!mirror) return;
+    __syncthreads();
+    publish_level(L, mirror, seq);
 }
 
 // after the children: the next lower level is the new children
@@ -2419,11 +2429,7 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
         const int g3 = std::max(grid_for(A_g, NT, INT_MAX), A_g < JOIN_TEST_NODES
                                                                 ? grid_for(12 * KEY_REPL * A_g, 4 * LBS, 2048)
                                                                 : grid_for(12 * A_g, 16 * BS, 2048));
-        unsigned seq = 0u;
-        if (mirror) {   // never 0, the tag of a record no level wrote
-            seq = ++sc.mirror.next_seq;
-            if (seq == 0u) seq = sc.mirror.next_seq = 1u;
-        }
+        const unsigned seq = mirror ? sc.mirror.take_seq() : 0u;
         leaf_pass_kernel<<<graph ? std::min(g3, 2048) : g3, LBS, 0, s>>>(
             NP, L, B, par, LS, seg, sc.keys[par].ptr, sc.keys[nxt].ptr, prm.empty_space_splitting,
             prm.empty_space_threshold, mirror ? sc.mirror.dev : nullptr, seq);
@@ -2516,10 +2522,13 @@ void run_build(pm_ctx *ctx, const float4 *P0, const float4 *P1, const float4 *P2
             }
             PM_LAUNCH_CHECK();
             scan_exclusive(sc.lower_flag.ptr, sc.lower_off.ptr, (int)nAn, sc, s, graph ? &L->An : nullptr);
-            plan_lower_kernel<<<1, 64, 0, s>>>(L, sc.lower_off.ptr, sc.prims_part.ptr);
+            const unsigned lseq = mirror ? sc.mirror.take_seq() : 0u;
+            plan_lower_kernel<<<1, 64, 0, s>>>(L, sc.lower_off.ptr, sc.prims_part.ptr,
+                                               mirror ? sc.mirror.dev : nullptr, lseq);
             PM_LAUNCH_CHECK();
             if (!graph) {
-                read_state(sc, h, s);
+                if (mirror) wait_level(sc, h, lseq, s);
+                else read_state(sc, h, s);
                 if (h.status) throw_status(h);
                 if (h.splits == 0) break;
                 nodes.reserve((size_t)h.nodes_size, s, h.nodes_size - 2 * h.splits);

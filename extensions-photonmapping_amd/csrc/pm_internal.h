@@ -138,10 +138,11 @@ struct Lvl {
     long long cap_nodes, cap_active, cap_segs, cap_refs, cap_leaf_refs, cap_leaf_ids, cap_lower;
 };
 
-// A copy of the level state in coherent pinned host memory: the last tile of
-// an eager level's leaf pass writes the state there, then the sequence number
-// (system-scope release); the host waits for that number instead of copying
-// the state back and synchronising the stream.
+// A copy of the level state in coherent pinned host memory: an eager build's
+// upper levels (the last tile of the leaf pass) and lower levels (the plan
+// kernel) write it there, each word tagged with the level's sequence number;
+// the host waits for a record whose words all carry the number instead of
+// copying the state back and synchronising the stream.
 struct LvlMirror {
     static constexpr int NW = (int)(sizeof(Lvl) / sizeof(int));
     // word t of the level state in the low half of w[t], the number of the
@@ -154,6 +155,11 @@ struct LvlMirror {
     Rec *host = nullptr;   // coherent, mapped
     Rec *dev = nullptr;    // the device's address of it
     unsigned next_seq = 0;
+    // the next level's number: never 0, the tag of a record no level wrote
+    unsigned take_seq() {
+        if (++next_seq == 0u) next_seq = 1u;
+        return next_seq;
+    }
     ~LvlMirror() { if (host) (void)hipHostFree(host); }
     bool failed = false;   // no such memory here: the build copies the state back instead
     // true when the mirror is usable
