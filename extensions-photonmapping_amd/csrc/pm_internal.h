@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <utility>
@@ -162,9 +163,16 @@ struct LvlMirror {
     }
     ~LvlMirror() { if (host) (void)hipHostFree(host); }
     bool failed = false;   // no such memory here: the build copies the state back instead
-    // true when the mirror is usable
+    // true when the mirror is usable; PMKD_HOST_MIRROR=0 in the environment
+    // (read when a context's first eager build starts) makes that context's
+    // builds copy the state back instead
     bool ensure() {
         if (host || failed) return !failed;
+        const char *env = std::getenv("PMKD_HOST_MIRROR");
+        if (env && env[0] == '0') {
+            failed = true;
+            return false;
+        }
         if (hipHostMalloc((void **)&host, sizeof(Rec), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
             hipHostGetDevicePointer((void **)&dev, host, 0) != hipSuccess) {
             (void)hipGetLastError();

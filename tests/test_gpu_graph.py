@@ -245,3 +245,25 @@ def test_many_roots_graph_equals_eager_and_oracle(pm, orc, scenes, torch, meshes
     g.close()
     ref = orc.build_batched(tris, off, orc_params(orc, bp))
     assert_same_tree(eager, ref.arrays(), f"{meshes} meshes: eager vs oracle")
+
+
+def test_copy_back_levels_equal_published_levels(pm, orc, scenes, torch, monkeypatch):
+    """PMKD_HOST_MIRROR=0: an eager build that copies each level's state back
+    (no coherent pinned memory) gives the tree and counts of one that reads the
+    levels' published states, and the oracle's"""
+    tris = scenes.cluster_soup(60_000, seed=11)
+    P = [dev(torch, t) for t in tris]
+    bp = pm.build_params("reference")
+    pub = pm.Context(0)
+    pub.build(*P, bp)
+    monkeypatch.setenv("PMKD_HOST_MIRROR", "0")
+    cpy = pm.Context(0)
+    cpy.build(*P, bp)
+    cpy.build(*P, bp)   # and again on the same context (grown buffers kept)
+    monkeypatch.delenv("PMKD_HOST_MIRROR")
+    assert_same_tree(cpy.tree_arrays(), pub.tree_arrays(), "copy-back vs published")
+    sp, sc_ = pub.stats(), cpy.stats()
+    for k in ("upper_levels", "lower_levels", "upper_ref_reads", "upper_ref_writes", "lower_prims"):
+        assert sc_[k] == sp[k], k
+    ref = orc.build(tris, orc_params(orc, bp)).arrays()
+    assert_same_tree(cpy.tree_arrays(), ref, "copy-back vs oracle")
